@@ -1,0 +1,183 @@
+"""Synthetic MD systems for the force path: molten NaCl and Lennard-Jones argon.
+
+Everything is in DL_POLY internal units: length in Angstrom, time in ps, mass
+in Dalton, energy in 10 J/mol (``constants.F90:100-116``), charge in e.
+
+A ``System`` carries exactly what DL_POLY's CONFIG/FIELD/CONTROL trio would
+carry for the two-body + SPME path: cell (row vectors a, b, c as in
+``configuration_type%cell``), positions, velocities, per-atom type index,
+per-type mass/charge, the vdW pair table (``vdw.F90`` type keys) and the
+cutoffs / Ewald / SPME settings.
+"""
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+
+# constants.F90:100,104 (internal units)
+R4PIE0 = 138935.4835
+BOLTZ = 8.31451115e-1
+
+# vdw.F90:64-113 potential keys used by the configs
+VDW_LENNARD_JONES = 2
+VDW_BORN_HUGGINS_MEYER = 5
+VDW_NAMES = {VDW_LENNARD_JONES: "lj", VDW_BORN_HUGGINS_MEYER: "bhm"}
+
+# Tosi-Fumi NaCl Born-Huggins-Meyer parameters, DL_POLY form
+# A*exp(B*(sigma-r)) - C/r^6 - D/r^8, converted from kJ/mol to internal
+# units (x100).  Literature values (Tosi & Fumi 1964, b = 0.338e-19 J,
+# rho = 0.317 A, Pauling factors 1.25/1.00/0.75).
+TOSI_FUMI = {
+    (0, 0): (2544.4, 3.15457, 2.340, 10117.0, 4818.0),
+    (1, 0): (2035.5, 3.15457, 2.755, 67448.0, 83708.0),
+    (1, 1): (1526.6, 3.15457, 3.170, 698570.0, 1403160.0),
+}
+NA_MASS, CL_MASS, AR_MASS = 22.98977, 35.453, 39.948
+
+
+@dataclass
+class System:
+    name: str
+    cell: np.ndarray                 # (9,) a, b, c row vectors
+    pos: np.ndarray                  # (N, 3) float64
+    vel: np.ndarray                  # (N, 3) float64
+    types: np.ndarray                # (N,) int32, 0-based type index
+    type_names: List[str]
+    masses: np.ndarray               # (ntypes,)
+    charges: np.ndarray              # (ntypes,)
+    vdw: List[Tuple[int, int, int, Tuple[float, ...]]]  # (ti, tj, key, params)
+    cutoff: float = 12.0             # neigh%cutoff (Coulomb / list)
+    padding: float = 0.0             # neigh%padding
+    vdw_cutoff: float = 12.0         # vdws%cutoff
+    electrostatics: bool = True      # SPME on/off
+    alpha: float = 0.28              # ewld%alpha
+    kvec: Tuple[int, int, int] = (32, 32, 32)
+    nsplines: int = 8
+    epsilon: float = 1.0             # electro%eps
+    timestep: float = 0.002          # ps
+    temperature: float = 0.0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def natms(self):
+        return int(self.pos.shape[0])
+
+    @property
+    def ntypes(self):
+        return len(self.type_names)
+
+    @property
+    def q(self):
+        return self.charges[self.types].astype(np.float64)
+
+    @property
+    def mass(self):
+        return self.masses[self.types].astype(np.float64)
+
+    @property
+    def volume(self):
+        c = self.cell.reshape(3, 3)
+        return float(abs(np.linalg.det(c)))
+
+
+def _streams(seed, n):
+    ss = np.random.SeedSequence(seed)
+    return [np.random.Generator(np.random.PCG64(s)) for s in ss.spawn(n)]
+
+
+def _maxwell(rng, mass, temperature):
+    """Maxwell-Boltzmann velocities at T with the centre-of-mass momentum removed."""
+    sig = np.sqrt(BOLTZ * temperature / mass)[:, None]
+    v = rng.standard_normal((mass.shape[0], 3)) * sig
+    p = (mass[:, None] * v).sum(0)
+    v -= p / mass.sum()
+    return v
+
+
+def _wrap(pos, L):
+    """Fold into [-L/2, L/2) per axis (DL_POLY's origin is the cell centre);
+    L is a scalar or a per-axis vector."""
+    return pos - L * np.floor(pos / L + 0.5)
+
+
+FCC = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
+
+
+def nacl(ncells=20, a0=5.64, expand=1.08, temperature=1200.0, jitter=0.1, seed=20261202,
+         kvec=None, nsplines=8, alpha=0.28, cutoff=12.0, padding=0.5, timestep=0.002,
+         jitter_on=True, charges=(1.0, -1.0)):
+    """Molten NaCl: ncells^3 rocksalt cells (8 ions each), lattice a0*expand
+    (the 8% expansion read as linear), positions jittered by U(-jitter, jitter)
+    and Maxwell-Boltzmann velocities at ``temperature``.  Na ions are listed
+    before Cl ions, matching a FIELD with molecule types Na then Cl."""
+    a = a0 * expand
+    nc = np.array([ncells] * 3 if np.isscalar(ncells) else list(ncells))
+    Lv = nc * a
+    r_jit, r_vel = _streams(seed, 2)
+    g = np.stack(np.meshgrid(np.arange(nc[0]), np.arange(nc[1]), np.arange(nc[2]),
+                             indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    na = (g[:, None, :] + FCC[None, :, :]).reshape(-1, 3) * a
+    cl = na + np.array([0.5 * a, 0.0, 0.0])
+    pos = np.concatenate([na, cl]) - 0.5 * Lv
+    n = pos.shape[0]
+    if jitter_on and jitter > 0:
+        pos = pos + r_jit.uniform(-jitter, jitter, size=(n, 3))
+    pos = _wrap(pos, Lv)
+    types = np.concatenate([np.zeros(n // 2, np.int32), np.ones(n // 2, np.int32)])
+    masses = np.array([NA_MASS, CL_MASS])
+    vel = _maxwell(r_vel, masses[types], temperature) if temperature > 0 else np.zeros((n, 3))
+    if kvec is None:
+        kvec = tuple(smooth235_at_least(int(np.ceil(x))) for x in Lv)
+    vdw = [(i, j, VDW_BORN_HUGGINS_MEYER, TOSI_FUMI[(i, j)]) for (i, j) in [(0, 0), (1, 0), (1, 1)]]
+    return System(name="nacl_%d" % n, cell=np.diag(Lv).reshape(9).astype(np.float64),
+                  pos=pos, vel=vel, types=types, type_names=["Na", "Cl"], masses=masses,
+                  charges=np.array(charges, dtype=np.float64), vdw=vdw, cutoff=cutoff, padding=padding,
+                  vdw_cutoff=cutoff, electrostatics=True, alpha=alpha, kvec=tuple(kvec),
+                  nsplines=nsplines, timestep=timestep, temperature=temperature,
+                  meta=dict(seed=seed, a=a, ncells=tuple(int(x) for x in nc), jitter=jitter))
+
+
+def argon(ncells=10, density=1.374, temperature=120.0, seed=20261202, epsilon=99.6,
+          sigma=3.405, cutoff=8.5, padding=0.5, timestep=0.005):
+    """Lennard-Jones argon on an FCC lattice (4 atoms per cell), cubic box at
+    mass density ``density`` g/cm^3, Maxwell-Boltzmann velocities at T.
+    epsilon = 0.996 kJ/mol = 99.6 internal units."""
+    n = 4 * ncells ** 3
+    # number density in A^-3: rho / (m * 1.66053906660e-24 g) * 1e-24 cm^3/A^3
+    nd = density / (AR_MASS * 1.66053906660)
+    L = (n / nd) ** (1.0 / 3.0)
+    a = L / ncells
+    (r_vel,) = _streams(seed, 1)
+    g = np.stack(np.meshgrid(np.arange(ncells), np.arange(ncells), np.arange(ncells),
+                             indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    pos = (g[:, None, :] + FCC[None, :, :] + 0.25).reshape(-1, 3) * a - 0.5 * L
+    pos = _wrap(pos, L)
+    types = np.zeros(n, np.int32)
+    masses = np.array([AR_MASS])
+    vel = _maxwell(r_vel, masses[types], temperature) if temperature > 0 else np.zeros((n, 3))
+    return System(name="argon_%d" % n, cell=np.diag([L, L, L]).reshape(9).astype(np.float64),
+                  pos=pos, vel=vel, types=types, type_names=["Ar"], masses=masses,
+                  charges=np.array([0.0]), vdw=[(0, 0, VDW_LENNARD_JONES, (epsilon, sigma))],
+                  cutoff=cutoff, padding=padding, vdw_cutoff=cutoff, electrostatics=False,
+                  alpha=0.0, kvec=(0, 0, 0), nsplines=8, timestep=timestep,
+                  temperature=temperature, meta=dict(seed=seed, a=a, ncells=ncells))
+
+
+def smooth235_at_least(n):
+    """Smallest 2^a 3^b 5^c >= n (the FFT sizes parallel_fft.F90:2031-2054 accepts)."""
+    m = max(1, n)
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def pair_key(ti, tj):
+    """vdw.F90:1875-1881 key for 1-based types: max*(max-1)/2 + min (returned 0-based)."""
+    a, b = ti + 1, tj + 1
+    hi, lo = max(a, b), min(a, b)
+    return hi * (hi - 1) // 2 + lo - 1

@@ -1,0 +1,756 @@
+/*
+ * dlpx_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, scalar, double-precision restatement of DL_POLY 5.1's per-step
+ * force path (link-cell pair set, tabulated vdW, erfc-damped real-space
+ * Coulomb, SPME reciprocal space, NVE velocity Verlet).  It is the checker
+ * the GPU path is compared against; it is never linked into, loaded by or
+ * called from the product library.  Each function names the reference lines
+ * it restates.
+ *
+ * Pinning (what fixes each function independently of this file):
+ *   orc_vdw_table      closed forms of LJ/BHM; test_vdw.F90:40-63 known answers
+ *   orc_erfc_tables    A&S 7.1.26 coefficients; exact Gaussian derivative
+ *   orc_interp3        exact for quadratic tables
+ *   orc_pairs_brute    O(N^2) minimum-image search (the link-cell set is
+ *                      compared with it)
+ *   orc_bspline        partition of unity, derivative sums, symmetry, finite
+ *                      differences, closed-form cubic B-spline
+ *   orc_fft3d          naive DFT on small 2.3.5 sizes
+ *   orc_two_body       REVCON forces and REVIVE energies/virials/stress of the
+ *                      compiled reference on generated NaCl/argon systems;
+ *                      Madelung constant of rocksalt; direct Ewald sum
+ *   orc_nve_step       reference trajectories (REVCON/REVIVE after n steps)
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+static const double R4PIE0 = 138935.4835;       /* constants.F90:100 */
+static const double ZERO_PLUS = 2.2250738585072014e-308; /* Tiny(1.0_wp) */
+
+/* ------------------------------------------------------------------ */
+/* cell helpers: numerics.F90:1450-1509 (invert), 1344-1448 (dcell)    */
+/* ------------------------------------------------------------------ */
+void orc_invert(const double *a, double *b, double *det)
+{
+    b[0] = a[4] * a[8] - a[5] * a[7];
+    b[1] = a[2] * a[7] - a[1] * a[8];
+    b[2] = a[1] * a[5] - a[2] * a[4];
+    b[3] = a[5] * a[6] - a[3] * a[8];
+    b[4] = a[0] * a[8] - a[2] * a[6];
+    b[5] = a[2] * a[3] - a[0] * a[5];
+    b[6] = a[3] * a[7] - a[4] * a[6];
+    b[7] = a[1] * a[6] - a[0] * a[7];
+    b[8] = a[0] * a[4] - a[1] * a[3];
+    double d = a[0] * b[0] + a[3] * b[1] + a[6] * b[2];
+    double r = 0.0;
+    if (fabs(d) > 0.0) r = 1.0 / d;
+    for (int i = 0; i < 9; i++) b[i] *= r;
+    *det = d;
+}
+
+void orc_dcell(const double *a, double *b)
+{
+    b[0] = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    b[1] = sqrt(a[3] * a[3] + a[4] * a[4] + a[5] * a[5]);
+    b[2] = sqrt(a[6] * a[6] + a[7] * a[7] + a[8] * a[8]);
+    b[3] = (a[0] * a[3] + a[1] * a[4] + a[2] * a[5]) / (b[0] * b[1]);
+    b[4] = (a[0] * a[6] + a[1] * a[7] + a[2] * a[8]) / (b[0] * b[2]);
+    b[5] = (a[3] * a[6] + a[4] * a[7] + a[5] * a[8]) / (b[1] * b[2]);
+    double axb1 = a[1] * a[5] - a[2] * a[4], axb2 = a[2] * a[3] - a[0] * a[5], axb3 = a[0] * a[4] - a[1] * a[3];
+    double bxc1 = a[4] * a[8] - a[5] * a[7], bxc2 = a[5] * a[6] - a[3] * a[8], bxc3 = a[3] * a[7] - a[4] * a[6];
+    double cxa1 = a[7] * a[2] - a[8] * a[1], cxa2 = a[8] * a[0] - a[6] * a[2], cxa3 = a[6] * a[1] - a[7] * a[0];
+    b[9] = fabs(a[0] * bxc1 + a[1] * bxc2 + a[2] * bxc3);
+    double d[3], x[3], y[3];
+    d[0] = b[9] / sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+    d[1] = b[9] / sqrt(cxa1 * cxa1 + cxa2 * cxa2 + cxa3 * cxa3);
+    d[2] = b[9] / sqrt(axb1 * axb1 + axb2 * axb2 + axb3 * axb3);
+    x[0] = fabs(a[0]) / b[0]; y[0] = fabs(a[1]) / b[0];
+    x[1] = fabs(a[3]) / b[1]; y[1] = fabs(a[4]) / b[1];
+    x[2] = fabs(a[6]) / b[2]; y[2] = fabs(a[7]) / b[2];
+    if (x[0] >= x[1] && x[0] >= x[2]) {
+        b[6] = d[0];
+        if (y[1] >= y[2]) { b[7] = d[1]; b[8] = d[2]; } else { b[7] = d[2]; b[8] = d[1]; }
+    } else if (x[1] >= x[0] && x[1] >= x[2]) {
+        b[6] = d[1];
+        if (y[0] >= y[2]) { b[7] = d[0]; b[8] = d[2]; } else { b[7] = d[2]; b[8] = d[0]; }
+    } else {
+        b[6] = d[2];
+        if (y[0] >= y[1]) { b[7] = d[0]; b[8] = d[1]; } else { b[7] = d[1]; b[8] = d[0]; }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* potentials and tables                                               */
+/* ------------------------------------------------------------------ */
+/* two_body_potentials.F90:260-270 (LJ), 499-514 (BHM).  kind: 2 LJ, 5 BHM */
+int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
+{
+    if (kind == 2) {
+        double sor6 = pow(p[1] / r, 6);
+        *e = 4.0 * p[0] * sor6 * (sor6 - 1.0);
+        *g = 24.0 * p[0] * sor6 * (2.0 * sor6 - 1.0);
+        return 0;
+    }
+    if (kind == 5) {
+        double ri2 = 1.0 / (r * r);
+        double t1 = p[0] * exp(p[1] * (p[2] - r));
+        double t2 = -p[3] * (ri2 * ri2 * ri2);
+        double t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
+        *e = t1 + t2 + t3;
+        *g = t1 * r * p[1] + 6.0 * t2 + 8.0 * t3;
+        return 0;
+    }
+    return -1;
+}
+
+/* vdw.F90:1397-1576: tables at r_i = i*dlrpot, i = 1..max_grid,
+ * dlrpot = rvdw/(max_grid-4); index 0 holds Huge. Arrays have max_grid+1 slots. */
+int orc_vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc)
+{
+    double dlrpot = rvdw / (double)(max_grid - 4);
+    for (int i = 1; i <= max_grid; i++) {
+        double r = (double)i * dlrpot, e, g;
+        if (orc_vdw_eval(kind, p, r, &e, &g)) return -1;
+        pot[i] = e;
+        frc[i] = g;
+    }
+    pot[0] = 1.79769313486231570815e308;
+    frc[0] = 1.79769313486231570815e308;
+    return 0;
+}
+
+/* numerics.F90:3647-3667 (A&S 7.1.26 erfc) and 3669-3683 (2/sqrt(pi) exp(-x^2)) */
+double orc_erfc_as(double x)
+{
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741,
+                 a4 = -1.453152027, a5 = 1.061405429, pp = 0.3275911;
+    double tt = 1.0 / (1.0 + pp * x);
+    return tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * exp(-(x * x));
+}
+
+/* electrostatic.F90:88-127 + numerics.F90:215-248: tab[i-1] = f(i*dr), dr = rc/(n-4) */
+void orc_erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d)
+{
+    const double rsqrpi = 1.0 / sqrt(ORC_PI);
+    double dr = rc / (double)(n - 4);
+    for (int i = 1; i <= n; i++) {
+        double r = (double)i * dr;
+        double g = orc_erfc_as(alpha * r) / r;
+        erfc_t[i - 1] = g;
+        double rsq = r * r;
+        erfc_d[i - 1] = (g + alpha * (2.0 * exp(-((alpha * r) * (alpha * r))) * rsqrpi)) / rsq;
+    }
+}
+
+/* numerics.F90:250-290 three_p_interp on a table where tab(l) = f(l*dr).
+ * tab0 points at the value for l = 0 (which may be virtual). */
+double orc_interp3(const double *tab0, double rdr, double r)
+{
+    int l = (int)(r * rdr);
+    if (l < 1) return NAN; /* below the first sample: the reference reads outside its table */
+    double ppp = r * rdr - (double)l;
+    double g0 = tab0[l], g1 = tab0[l + 1], g2 = tab0[l + 2];
+    if (l == 0) g0 = g0 * r;
+    double t1 = g0 + (g1 - g0) * ppp;
+    double t2 = g1 + (g2 - g1) * (ppp - 1.0);
+    return t1 + (t2 - t1) * ppp * 0.5;
+}
+
+/* vdw.F90:617-967 for LJ (2) and BHM (5): energy and virial long-range corrections.
+ * kinds/params are indexed by pair key (0-based key = hi*(hi-1)/2+lo-1, 1-based types). */
+void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*5 per key*/,
+                 double rvdw, double volm, double *elrc, double *vlrc)
+{
+    const double twopi = 2.0 * ORC_PI;
+    double plrc = 0.0, e = 0.0;
+    int ivdw = 0;
+    for (int i = 1; i <= ntypes; i++) {
+        for (int j = 1; j <= i; j++) {
+            double eadd = 0.0, padd = 0.0, r = rvdw;
+            const double *p = params + 5 * ivdw;
+            if (kinds[ivdw] == 2) {
+                double eps = p[0], sig = p[1];
+                eadd = 4.0 * eps * (pow(sig, 12) / (9.0 * pow(r, 9)) - pow(sig, 6) / (3.0 * pow(r, 3)));
+                padd = 8.0 * eps * (6.0 * pow(sig, 12) / (9.0 * pow(r, 9)) - pow(sig, 6) / pow(r, 3));
+            } else if (kinds[ivdw] == 5) {
+                double c = p[3], d = p[4];
+                eadd = -c / (3.0 * pow(r, 3)) - d / (5.0 * pow(r, 5));
+                padd = -2.0 * c / pow(r, 3) - 8.0 * d / (5.0 * pow(r, 5));
+            }
+            if (i != j) { eadd *= 2.0; padd *= 2.0; }
+            double denprd = twopi * (num_type[i - 1] * num_type[j - 1]) / (volm * volm);
+            e += volm * denprd * eadd;
+            plrc += denprd * padd / 3.0;
+            ivdw++;
+        }
+    }
+    *elrc = e;
+    *vlrc = plrc * (-3.0 * volm);
+}
+
+/* ------------------------------------------------------------------ */
+/* pair sets                                                           */
+/* ------------------------------------------------------------------ */
+/* Minimum-image separation r_ij = x_i - (x_j + s L) for an orthorhombic
+ * cell; s is returned in sh (box lengths). */
+static inline void min_image(const double *L, const double *xi, const double *xj, double dx[3], int sh[3])
+{
+    for (int d = 0; d < 3; d++) {
+        double s = nearbyint((xi[d] - xj[d]) / L[d]);
+        sh[d] = (int)s;
+        dx[d] = xi[d] - (xj[d] + s * L[d]);
+    }
+}
+
+/* Brute force O(N^2): every unordered pair i<j with minimum-image r^2 <= rcut^2.
+ * Writes (i, j, s_x, s_y, s_z) where r_ij = x_i - (x_j + s*L).
+ * Returns the number of pairs (entries past cap are counted, not written). */
+long orc_pairs_brute(int n, const double *cell, const double *xyz, double rcut,
+                     int32_t *out, long cap)
+{
+    double L[3] = {cell[0], cell[4], cell[8]};
+    double rcsq = rcut * rcut;
+    long np = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) {
+            double dx[3];
+            int sh[3];
+            min_image(L, xyz + 3 * i, xyz + 3 * j, dx, sh);
+            double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (rsq <= rcsq) {
+                if (np < cap) {
+                    int32_t *o = out + 5 * np;
+                    o[0] = i; o[1] = j; o[2] = sh[0]; o[3] = sh[1]; o[4] = sh[2];
+                }
+                np++;
+            }
+        }
+    return np;
+}
+
+/* Link-cell restatement (neighbours.F90:356-1027 criterion r^2 <= rcut^2,
+ * cells of edge >= rcut, periodic images instead of halo copies).  Same
+ * output convention as orc_pairs_brute.  Falls back to brute force when a
+ * dimension has fewer than 3 cells. */
+long orc_pairs_cells(int n, const double *cell, const double *xyz, double rcut, int32_t *out, long cap)
+{
+    double L[3] = {cell[0], cell[4], cell[8]};
+    int nc[3];
+    for (int d = 0; d < 3; d++) nc[d] = (int)(L[d] / rcut);
+    if (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) return orc_pairs_brute(n, cell, xyz, rcut, out, cap);
+    int ncell = nc[0] * nc[1] * nc[2];
+    int *head = malloc(sizeof(int) * ncell), *next = malloc(sizeof(int) * n), *cid = malloc(sizeof(int) * n);
+    for (int c = 0; c < ncell; c++) head[c] = -1;
+    for (int i = n - 1; i >= 0; i--) {
+        int c3[3];
+        for (int d = 0; d < 3; d++) {
+            double s = xyz[3 * i + d] / L[d] + 0.5;
+            s -= floor(s);
+            int k = (int)(s * nc[d]);
+            if (k >= nc[d]) k = nc[d] - 1;
+            if (k < 0) k = 0;
+            c3[d] = k;
+        }
+        cid[i] = (c3[2] * nc[1] + c3[1]) * nc[0] + c3[0];
+        next[i] = head[cid[i]];
+        head[cid[i]] = i;
+    }
+    double rcsq = rcut * rcut;
+    long np = 0;
+    for (int i = 0; i < n; i++) {
+        int c = cid[i];
+        int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+        for (int oz = -1; oz <= 1; oz++)
+            for (int oy = -1; oy <= 1; oy++)
+                for (int ox = -1; ox <= 1; ox++) {
+                    int jx = (cx + ox + nc[0]) % nc[0], jy = (cy + oy + nc[1]) % nc[1], jz = (cz + oz + nc[2]) % nc[2];
+                    for (int j = head[(jz * nc[1] + jy) * nc[0] + jx]; j >= 0; j = next[j]) {
+                        if (j <= i) continue;
+                        double dx[3];
+                        int sh[3];
+                        min_image(L, xyz + 3 * i, xyz + 3 * j, dx, sh);
+                        double rsq = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+                        if (rsq <= rcsq) {
+                            if (np < cap) {
+                                int32_t *o = out + 5 * np;
+                                o[0] = i; o[1] = j; o[2] = sh[0]; o[3] = sh[1]; o[4] = sh[2];
+                            }
+                            np++;
+                        }
+                    }
+                }
+    }
+    free(head); free(next); free(cid);
+    return np;
+}
+
+/* ------------------------------------------------------------------ */
+/* B-splines: bspline.F90:192-306 (values, 1st and 2nd derivatives)    */
+/* ------------------------------------------------------------------ */
+/* derivs[(l*S + t)] for l = 0,1,2 and t = 0..S-1 (t = reference index j-1);
+ * point t sits at 0-based grid index floor(u) - S + 1 + t. */
+void orc_bspline(double u, int S, double *derivs)
+{
+    double ncombk[22][22];
+    memset(ncombk, 0, sizeof(ncombk));
+    for (int nn = 1; nn <= S; nn++)
+        for (int k = 0; k <= nn; k++) {
+            double pr = 1.0;
+            for (int i = nn - k + 1; i <= nn; i++) pr *= (double)i;
+            int kk = k > 1 ? k : 1;
+            double pi_ = 1.0;
+            for (int i = 1; i <= kk; i++) pi_ *= 1.0 / (double)i;
+            ncombk[k][nn] = pr * pi_;
+        }
+    const int nd = 2;
+    double zd[24], fd[24];
+    for (int i = 0; i < 24; i++) zd[i] = 0.0;
+    zd[S] = u - trunc(u);
+    zd[S - 1] = 1.0 - zd[S];
+    double centre = zd[S];
+    for (int k = S - 2; k >= nd + 1; k--) {
+        double k_r = (double)(S - k + 1);
+        double km1_rr = 1.0 / (double)(S - k);
+        for (int j = k; j <= S - 1; j++) {
+            double pt = centre + (double)(S - j);
+            zd[j] = (pt * zd[j] + (k_r - pt) * zd[j + 1]) * km1_rr;
+        }
+        zd[S] = zd[S] * centre * km1_rr;
+    }
+    for (int l = nd; l >= 1; l--) {
+        double k_r = (double)(S - l + 1);
+        double km1_rr = 1.0 / (double)(S - l);
+        for (int i = 0; i < 24; i++) fd[i] = 0.0;
+        for (int j = 1; j <= S - 1; j++) {
+            int lim = l < S - j ? l : S - j;
+            for (int n = 0; n <= lim; n += 2) {
+                fd[j] = fd[j] + ncombk[n][l] * zd[j + n];
+                fd[j] = fd[j] - ncombk[n + 1][l] * zd[j + n + 1];
+            }
+            double pt = centre + (double)(S - j);
+            zd[j] = (pt * zd[j] + (k_r - pt) * zd[j + 1]) * km1_rr;
+        }
+        for (int j = 1; j <= S - 1; j++) derivs[l * S + (j - 1)] = fd[j];
+        derivs[l * S + (S - 1)] = zd[S];
+        zd[S] = zd[S] * centre * km1_rr;
+    }
+    for (int j = 1; j <= S; j++) derivs[j - 1] = zd[j];
+}
+
+/* bspline.F90:73-190 (+ spl_cexp 308-353): norm2[m] = |b(m)|^2, m = 0..K-1 */
+void orc_bspline_norm2(int K, int S, double *norm2)
+{
+    const double twopi = 2.0 * ORC_PI;
+    double *wr = malloc(sizeof(double) * K), *wi = malloc(sizeof(double) * K);
+    wr[0] = 1.0; wi[0] = 0.0;
+    for (int i = 1; i <= K / 2; i++) {
+        double arg = (twopi / (double)K) * (double)i;
+        wr[i] = cos(arg); wi[i] = sin(arg);
+        wr[K - i] = wr[i]; wi[K - i] = -wi[i];
+    }
+    double cs[24];
+    for (int i = 0; i < 24; i++) cs[i] = 0.0;
+    cs[2] = 1.0;
+    for (int k = 3; k <= S; k++)
+        for (int j = k; j >= 2; j--)
+            cs[j] = ((double)(j - 1) * cs[j] + (double)(k - j + 1) * cs[j - 1]) * (1.0 / (double)(k - 1));
+    for (int i = 0; i < K; i++) {
+        double tr = 0.0, ti = 0.0;
+        for (int k = 0; k <= S - 2; k++) {
+            int m = (int)(((long)i * k) % K);
+            tr += cs[k + 2] * wr[m];
+            ti += cs[k + 2] * wi[m];
+        }
+        int m = (int)(((long)i * (S - 1)) % K);
+        /* b = w / t */
+        double den = tr * tr + ti * ti;
+        double br = (wr[m] * tr + wi[m] * ti) / den;
+        double bi = (wi[m] * tr - wr[m] * ti) / den;
+        norm2[i] = br * br + bi * bi;
+    }
+    free(wr); free(wi);
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: complex, unnormalised, sign +1 => sum_k c_k exp(+2 pi i j k/n)  */
+/* (gpfa235.F90:72 convention).  Recursive mixed radix for 2^a 3^b 5^c; */
+/* any other length falls back to a direct DFT.                         */
+/* ------------------------------------------------------------------ */
+static void fft_rec(const double *in, int stride, double *out, int n, int sign, double *tmp)
+{
+    if (n == 1) { out[0] = in[0]; out[1] = in[1]; return; }
+    int p = (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : (n % 5 == 0) ? 5 : n;
+    int m = n / p;
+    if (p == n) { /* direct DFT */
+        for (int k = 0; k < n; k++) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < n; j++) {
+                double a = sign * 2.0 * ORC_PI * (double)(((long)j * k) % n) / (double)n;
+                double c = cos(a), s = sin(a);
+                double xr = in[2 * j * stride], xi = in[2 * j * stride + 1];
+                sr += xr * c - xi * s; si += xr * s + xi * c;
+            }
+            out[2 * k] = sr; out[2 * k + 1] = si;
+        }
+        return;
+    }
+    /* decimation in time: sub-transform r takes elements r, r+p, r+2p, ... */
+    for (int r = 0; r < p; r++) fft_rec(in + 2 * r * stride, stride * p, tmp + 2 * r * m, m, sign, out);
+    for (int k = 0; k < m; k++) {
+        for (int s = 0; s < p; s++) {
+            double sr = 0.0, si = 0.0;
+            int kk = k + s * m;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)r * kk) % n;
+                double a = sign * 2.0 * ORC_PI * (double)e / (double)n;
+                double c = cos(a), sn = sin(a);
+                double xr = tmp[2 * (r * m + k)], xi = tmp[2 * (r * m + k) + 1];
+                sr += xr * c - xi * sn; si += xr * sn + xi * c;
+            }
+            out[2 * kk] = sr; out[2 * kk + 1] = si;
+        }
+    }
+}
+
+void orc_fft1d(double *data, int n, int sign)
+{
+    double *a = malloc(sizeof(double) * 2 * n), *b = malloc(sizeof(double) * 2 * n), *t = malloc(sizeof(double) * 2 * n);
+    memcpy(a, data, sizeof(double) * 2 * n);
+    fft_rec(a, 1, b, n, sign, t);
+    memcpy(data, b, sizeof(double) * 2 * n);
+    free(a); free(b); free(t);
+}
+
+/* grid[(z*K2 + y)*K1 + x] interleaved complex; transforms x, y then z */
+void orc_fft3d(double *grid, const int *K, int sign)
+{
+    int K1 = K[0], K2 = K[1], K3 = K[2];
+    int mx = K1 > K2 ? K1 : K2; if (K3 > mx) mx = K3;
+    double *line = malloc(sizeof(double) * 2 * mx);
+    for (int z = 0; z < K3; z++)
+        for (int y = 0; y < K2; y++) {
+            double *p = grid + 2 * ((long)(z * K2 + y) * K1);
+            orc_fft1d(p, K1, sign);
+        }
+    for (int z = 0; z < K3; z++)
+        for (int x = 0; x < K1; x++) {
+            for (int y = 0; y < K2; y++) { long o = 2 * ((long)(z * K2 + y) * K1 + x); line[2 * y] = grid[o]; line[2 * y + 1] = grid[o + 1]; }
+            orc_fft1d(line, K2, sign);
+            for (int y = 0; y < K2; y++) { long o = 2 * ((long)(z * K2 + y) * K1 + x); grid[o] = line[2 * y]; grid[o + 1] = line[2 * y + 1]; }
+        }
+    for (int y = 0; y < K2; y++)
+        for (int x = 0; x < K1; x++) {
+            for (int z = 0; z < K3; z++) { long o = 2 * ((long)(z * K2 + y) * K1 + x); line[2 * z] = grid[o]; line[2 * z + 1] = grid[o + 1]; }
+            orc_fft1d(line, K3, sign);
+            for (int z = 0; z < K3; z++) { long o = 2 * ((long)(z * K2 + y) * K1 + x); grid[o] = line[2 * z]; grid[o + 1] = line[2 * z + 1]; }
+        }
+    free(line);
+}
+
+/* ------------------------------------------------------------------ */
+/* the two-body force evaluation                                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int natms, ntypes;
+    const double *cell;      /* 9, row vectors */
+    const double *xyz;       /* N*3 */
+    const double *chge;      /* N */
+    const int32_t *ltype;    /* N, 0-based */
+    double rcut, padding, rvdw, alpha, epsq;
+    int kvec[3], nspl, electro;
+    int max_grid;            /* vdW tables: nkey rows of max_grid+1 */
+    const double *vdw_pot, *vdw_frc;
+    int nsamples;            /* erfc tables: nsamples values, tab[i-1] = f(i*dr) */
+    const double *erfc_t, *erfc_d;
+    double elrc, vlrc;
+} orc_input;
+
+typedef struct {
+    double *frc;             /* N*3 out (overwritten) */
+    double buffer[20];       /* two_body.F90:708-727 slots */
+    double stress[9];        /* two_body_forces contribution to stats%stress */
+    double engcpe, vircpe, engsrp, virsrp; /* two_body.F90:766-782 increments */
+    double eng_self, eng_recip, eng_nz;
+    double recip_net[3];     /* mean reciprocal force removed (ewald_general.F90:857-864) */
+    long npairs;             /* pairs within rcut+padding */
+    /* optional intermediates (NULL to skip): K1*K2*K3 reals each */
+    double *charge_grid, *potential_grid;
+} orc_output;
+
+static int key_of(int ti, int tj)
+{
+    int a = ti + 1, b = tj + 1, hi = a > b ? a : b, lo = a > b ? b : a;
+    return hi * (hi - 1) / 2 + lo - 1;
+}
+
+int orc_two_body(const orc_input *in, orc_output *out)
+{
+    const int n = in->natms;
+    const double *c = in->cell;
+    if (!(c[1] == 0.0 && c[2] == 0.0 && c[3] == 0.0 && c[5] == 0.0 && c[6] == 0.0 && c[7] == 0.0)) return 2;
+    double L[3] = {c[0], c[4], c[8]};
+    double *f = out->frc;
+    memset(f, 0, sizeof(double) * 3 * n);
+    memset(out->buffer, 0, sizeof(out->buffer));
+    memset(out->stress, 0, sizeof(out->stress));
+    double scaling = R4PIE0 / in->epsq;     /* two_body.F90:188 */
+    double volm = fabs(L[0] * L[1] * L[2]);
+
+    /* ---- reciprocal space: ewald_spole.F90:244-477 ---- */
+    double engcpe_rc = 0.0, vircpe_rc = 0.0, sumchg = 0.0, self = 0.0;
+    for (int i = 0; i < n; i++) sumchg += in->chge[i];
+    int any_q = 0;
+    for (int i = 0; i < n; i++) if (fabs(in->chge[i]) > ZERO_PLUS) any_q = 1;
+    if (in->electro && any_q) {
+        const int S = in->nspl, K1 = in->kvec[0], K2 = in->kvec[1], K3 = in->kvec[2];
+        const long KK = (long)K1 * K2 * K3;
+        /* spme.F90:159-231 self interaction (pot_order 1: alpha * 1/sqrt(pi)) */
+        double sq = 0.0;
+        for (int i = 0; i < n; i++) sq += in->chge[i] * in->chge[i];
+        self = -sq * (scaling * in->alpha * (1.0 / sqrt(ORC_PI)) / 1.0);
+        double rcell[9], det;
+        orc_invert(c, rcell, &det);
+        if (fabs(det) < 1.0e-6) return 120;
+        double kr[3] = {(double)K1, (double)K2, (double)K3};
+        double scale = ORC_PI * sqrt(ORC_PI) * pow(in->alpha, 1 - 3) * (0.5 * (1.0 / volm)) * scaling;
+        int *idx = malloc(sizeof(int) * 3 * n);
+        double *spl = malloc(sizeof(double) * 3 * 3 * S * n);
+        for (int i = 0; i < n; i++) {
+            const double *r = in->xyz + 3 * i;
+            for (int d = 0; d < 3; d++) {
+                double u = kr[d] * (rcell[d] * r[0] + rcell[d + 3] * r[1] + rcell[d + 6] * r[2] + 0.5);
+                /* An atom that drifted out of the cell between list rebuilds is
+                 * represented by its periodic image inside [0, K): the reference
+                 * spreads that image from its halo (ewald_spole.F90:381-384). */
+                if (u < 0.0 || u >= kr[d]) u -= kr[d] * floor(u / kr[d]);
+                idx[3 * i + d] = (int)u;
+                orc_bspline(u, S, spl + (3 * i + d) * 3 * S);
+            }
+        }
+        double *Q = calloc(KK, sizeof(double));
+        /* ewald_general.F90:517-576, periodic images folded with (g mod K) */
+        for (int i = 0; i < n; i++) {
+            double qi = in->chge[i];
+            if (fabs(qi) <= ZERO_PLUS) continue;
+            const double *Mx = spl + (3 * i + 0) * 3 * S, *My = spl + (3 * i + 1) * 3 * S, *Mz = spl + (3 * i + 2) * 3 * S;
+            for (int tz = 0; tz < S; tz++) {
+                int gz = ((idx[3 * i + 2] - S + 1 + tz) % K3 + K3) % K3;
+                double f1 = qi * Mz[tz];
+                for (int ty = 0; ty < S; ty++) {
+                    int gy = ((idx[3 * i + 1] - S + 1 + ty) % K2 + K2) % K2;
+                    double f2 = f1 * My[ty];
+                    for (int tx = 0; tx < S; tx++) {
+                        int gx = ((idx[3 * i] - S + 1 + tx) % K1 + K1) % K1;
+                        Q[((long)gz * K2 + gy) * K1 + gx] += f2 * Mx[tx];
+                    }
+                }
+            }
+        }
+        if (out->charge_grid) memcpy(out->charge_grid, Q, sizeof(double) * KK);
+        /* ewald_spole.F90:1257-1386 */
+        double *G = malloc(sizeof(double) * 2 * KK);
+        for (long g = 0; g < KK; g++) { G[2 * g] = Q[g]; G[2 * g + 1] = 0.0; }
+        orc_fft3d(G, in->kvec, +1);
+        double *n2x = malloc(sizeof(double) * K1), *n2y = malloc(sizeof(double) * K2), *n2z = malloc(sizeof(double) * K3);
+        orc_bspline_norm2(K1, S, n2x); orc_bspline_norm2(K2, S, n2y); orc_bspline_norm2(K3, S, n2z);
+        double rcp[10];
+        orc_dcell(rcell, rcp);
+        double recip_conv_fac = ORC_PI / in->alpha;
+        double test_fac = pow(1.0e-6 / recip_conv_fac, 2);
+        double m1 = kr[0] * rcp[6], m2 = kr[1] * rcp[7], m3 = kr[2] * rcp[8];
+        double mn = m1 < m2 ? m1 : m2; if (m3 < mn) mn = m3;
+        double cut_off = 0.5 * 1.05 * mn, cut_off_2 = cut_off * cut_off;
+        const double sqrpi = sqrt(ORC_PI);
+        double st[3][3] = {{0}};
+        for (int l = 0; l < K3; l++) {
+            int ll = l; if (2 * ll > K3) ll -= K3;
+            double rp3[3] = {ll * rcell[2], ll * rcell[5], ll * rcell[8]};
+            double bb3 = n2z[l];
+            for (int k = 0; k < K2; k++) {
+                int kk = k; if (2 * kk > K2) kk -= K2;
+                double rp2[3] = {rp3[0] + kk * rcell[1], rp3[1] + kk * rcell[4], rp3[2] + kk * rcell[7]};
+                double bb2 = bb3 * n2y[k];
+                for (int j = 0; j < K1; j++) {
+                    int jj = j; if (2 * jj > K1) jj -= K1;
+                    double rp[3] = {rp2[0] + jj * rcell[0], rp2[1] + jj * rcell[3], rp2[2] + jj * rcell[6]};
+                    double bb1 = bb2 * n2x[j];
+                    double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+                    long o = 2 * (((long)l * K2 + k) * K1 + j);
+                    if (k2 <= cut_off_2 && k2 > test_fac) {
+                        double fp = sqrt(k2) * recip_conv_fac;
+                        double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
+                        double pr = fac * G[o], pi_ = fac * G[o + 1];
+                        double w = -2.0 * ((1.0 + fp * fp) / k2);
+                        /* Real(pc * w * conj(Q)) */
+                        double pv = (pr * G[o] + pi_ * G[o + 1]) * w;
+                        for (int a = 0; a < 3; a++)
+                            for (int b = 0; b < 3; b++) st[a][b] += rp[a] * rp[b] * pv;
+                        G[o] = pr; G[o + 1] = pi_;
+                    } else {
+                        G[o] = 0.0; G[o + 1] = 0.0;
+                    }
+                }
+            }
+        }
+        orc_fft3d(G, in->kvec, -1);
+        if (out->potential_grid) for (long g = 0; g < KK; g++) out->potential_grid[g] = G[2 * g];
+        /* ewald_general.F90:717-869 */
+        double rk[3];
+        /* Matmul(reshape(rcell,[3,3]), K): element d = sum_e rcell(d + 3(e-1)) K_e */
+        for (int d = 0; d < 3; d++) rk[d] = rcell[d] * kr[0] + rcell[d + 3] * kr[1] + rcell[d + 6] * kr[2];
+        double eng = 0.0, ftot[3] = {0, 0, 0};
+        double *fr = calloc(3 * n, sizeof(double));
+        for (int i = 0; i < n; i++) {
+            double qi = in->chge[i];
+            if (fabs(qi) <= ZERO_PLUS) continue;
+            const double *Sx = spl + (3 * i + 0) * 3 * S, *Sy = spl + (3 * i + 1) * 3 * S, *Sz = spl + (3 * i + 2) * 3 * S;
+            double et = 0.0, cf[3] = {0, 0, 0};
+            for (int tz = 0; tz < S; tz++) {
+                int gz = ((idx[3 * i + 2] - S + 1 + tz) % K3 + K3) % K3;
+                double e2 = qi * Sz[tz];
+                double f13 = qi * Sz[tz], f23 = qi * Sz[tz], f33 = qi * Sz[S + tz];
+                for (int ty = 0; ty < S; ty++) {
+                    int gy = ((idx[3 * i + 1] - S + 1 + ty) % K2 + K2) % K2;
+                    double e1 = e2 * Sy[ty];
+                    double f12 = f13 * Sy[ty], f22 = f23 * Sy[S + ty], f32 = f33 * Sy[ty];
+                    for (int tx = 0; tx < S; tx++) {
+                        int gx = ((idx[3 * i] - S + 1 + tx) % K1 + K1) % K1;
+                        double phi = G[2 * (((long)gz * K2 + gy) * K1 + gx)];
+                        double fx = f12 * Sx[S + tx] * phi * rk[0];
+                        double fy = f22 * Sx[tx] * phi * rk[1];
+                        double fz = f32 * Sx[tx] * phi * rk[2];
+                        ftot[0] -= fx; ftot[1] -= fy; ftot[2] -= fz;
+                        cf[0] += fx; cf[1] += fy; cf[2] += fz;
+                        et += e1 * Sx[tx] * phi;
+                    }
+                }
+            }
+            eng += et;
+            fr[3 * i] -= cf[0]; fr[3 * i + 1] -= cf[1]; fr[3 * i + 2] -= cf[2];
+        }
+        for (int d = 0; d < 3; d++) { ftot[d] /= (double)n; out->recip_net[d] = ftot[d]; }
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) fr[3 * i + d] -= ftot[d];
+        eng *= scale;
+        for (int i = 0; i < 3 * n; i++) f[i] += fr[i] * scale * 2.0;
+        double s9[9];
+        /* stress_temp(beta,alpha) column-major reshape => s9[(alpha-1)*3 + beta-1] */
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) s9[a * 3 + b] = st[a][b] * scale;
+        s9[0] += eng; s9[4] += eng; s9[8] += eng;
+        for (int k = 0; k < 9; k++) out->stress[k] += s9[k];
+        vircpe_rc = -(s9[0] + s9[4] + s9[8]);
+        engcpe_rc = eng + self;
+        out->eng_recip = eng;
+        free(fr); free(G); free(Q); free(idx); free(spl); free(n2x); free(n2y); free(n2z);
+    }
+    out->eng_self = self;
+
+    /* ---- real space: two_body.F90:339-525, vdw.F90:1790-2024, ewald_spole.F90:58-242 ---- */
+    double rcut_ext = in->rcut + in->padding;
+    long cap = 0;
+    long np = orc_pairs_cells(n, c, in->xyz, rcut_ext, NULL, 0);
+    cap = np;
+    int32_t *pairs = malloc(sizeof(int32_t) * 5 * (cap > 0 ? cap : 1));
+    orc_pairs_cells(n, c, in->xyz, rcut_ext, pairs, cap);
+    out->npairs = np;
+    double engvdw = 0.0, virvdw = 0.0, engrl = 0.0, virrl = 0.0;
+    double st_v[6] = {0}, st_c[6] = {0};
+    double dlrpot = in->rvdw / (double)(in->max_grid - 4), rdr = 1.0 / dlrpot;
+    double dr_e = in->rcut / (double)(in->nsamples - 4), rdr_e = 1.0 / dr_e;
+    int nrow = in->max_grid + 1;
+    for (long p = 0; p < np; p++) {
+        int i = pairs[5 * p], j = pairs[5 * p + 1];
+        double xx = in->xyz[3 * i] - (in->xyz[3 * j] + pairs[5 * p + 2] * L[0]);
+        double yy = in->xyz[3 * i + 1] - (in->xyz[3 * j + 1] + pairs[5 * p + 3] * L[1]);
+        double zz = in->xyz[3 * i + 2] - (in->xyz[3 * j + 2] + pairs[5 * p + 4] * L[2]);
+        double rrr = sqrt(xx * xx + yy * yy + zz * zz);
+        /* vdW */
+        if (in->max_grid > 0 && rrr < in->rvdw) {
+            int k = key_of(in->ltype[i], in->ltype[j]);
+            const double *tf = in->vdw_frc + (long)k * nrow, *tp = in->vdw_pot + (long)k * nrow;
+            double r_rrr = 1.0 / rrr, rsq = rrr * rrr, r_rsq = r_rrr * r_rrr;
+            int l = (int)(rrr * rdr);
+            double ppp = rrr * rdr - (double)l;
+            double gk = tf[l]; if (l == 0) gk = gk * rrr;
+            double gk1 = tf[l + 1], gk2 = tf[l + 2];
+            double t1 = gk + (gk1 - gk) * ppp, t2 = gk1 + (gk2 - gk1) * (ppp - 1.0);
+            double gamma = (t1 + (t2 - t1) * ppp * 0.5) * r_rsq;
+            double fx = gamma * xx, fy = gamma * yy, fz = gamma * zz;
+            f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
+            f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
+            double vk = tp[l], vk1 = tp[l + 1], vk2 = tp[l + 2];
+            t1 = vk + (vk1 - vk) * ppp; t2 = vk1 + (vk2 - vk1) * (ppp - 1.0);
+            engvdw += t1 + (t2 - t1) * ppp * 0.5;
+            virvdw -= gamma * rsq;
+            st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
+            st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
+        }
+        /* erfc-damped Coulomb */
+        if (in->electro) {
+            double qi = in->chge[i] * scaling, qj = in->chge[j];
+            if (fabs(qi) >= ZERO_PLUS && fabs(qj) > ZERO_PLUS && rrr < in->rcut) {
+                double prefac = qi * qj;
+                double eg = prefac * orc_interp3(in->erfc_d - 1, rdr_e, rrr);
+                double fx = eg * xx, fy = eg * yy, fz = eg * zz;
+                f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
+                f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
+                engrl += prefac * orc_interp3(in->erfc_t - 1, rdr_e, rrr);
+                virrl -= eg * (rrr * rrr);
+                st_c[0] += xx * fx; st_c[1] += xx * fy; st_c[2] += xx * fz;
+                st_c[3] += yy * fy; st_c[4] += yy * fz; st_c[5] += zz * fz;
+            }
+        }
+    }
+    free(pairs);
+    const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int k = 0; k < 9; k++) out->stress[k] += st_v[map[k]] + st_c[map[k]];
+
+    /* ---- bookkeeping: two_body.F90:689-790 ---- */
+    double engcpe_nz = 0.0, vircpe_nz = 0.0;
+    if (in->electro && fabs(sumchg) > 1.0e-6) {
+        double factor_nz = -0.5 * (ORC_PI * R4PIE0 / in->epsq) * pow(sumchg / in->alpha, 2);
+        engcpe_nz = factor_nz / volm;
+        vircpe_nz = -3.0 * engcpe_nz;
+    }
+    out->eng_nz = engcpe_nz;
+    out->buffer[7] = engvdw; out->buffer[8] = virvdw;
+    out->buffer[9] = engcpe_rc; out->buffer[10] = vircpe_rc;
+    out->buffer[11] = engrl; out->buffer[12] = virrl;
+    out->engcpe = engcpe_rc + engrl + engcpe_nz;
+    out->vircpe = vircpe_rc + virrl + vircpe_nz;
+    double tmp = -vircpe_nz / 3.0;
+    out->stress[0] += tmp; out->stress[4] += tmp; out->stress[8] += tmp;
+    out->engsrp = engvdw + in->elrc;
+    out->virsrp = virvdw + in->vlrc;
+    tmp = -in->vlrc / 3.0;
+    out->stress[0] += tmp; out->stress[4] += tmp; out->stress[8] += tmp;
+    return 0;
+}
+
+/* nve.F90:127-193 (stage 1) and 194-215 (stage 2) for one species-free
+ * velocity-Verlet step; mass <= 1e-6 atoms are left untouched. */
+void orc_nve_first(int n, double dt, const double *mass, double *x, double *v, const double *f)
+{
+    for (int i = 0; i < n; i++) {
+        if (mass[i] <= 1.0e-6) continue;
+        double tmp = (0.5 * dt) / mass[i];
+        for (int d = 0; d < 3; d++) {
+            v[3 * i + d] += tmp * f[3 * i + d];
+            x[3 * i + d] += dt * v[3 * i + d];
+        }
+    }
+}
+
+void orc_nve_second(int n, double dt, const double *mass, double *v, const double *f)
+{
+    for (int i = 0; i < n; i++) {
+        if (mass[i] <= 1.0e-6) continue;
+        double tmp = (0.5 * dt) / mass[i];
+        for (int d = 0; d < 3; d++) v[3 * i + d] += tmp * f[3 * i + d];
+    }
+}

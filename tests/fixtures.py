@@ -1,0 +1,42 @@
+"""Load the golden fixtures written by tests/golden/make_fixtures.py."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+for p in (os.path.join(ROOT, "dl-poly_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+from dlpx import systems  # noqa: E402
+from dlpx.fileio import STP_ENGCPE, STP_ENGSRP, STP_VIRCPE, STP_VIRSRP  # noqa: E402
+
+NAMES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    st = z["settings"]
+    vdw = [(int(t[0]), int(t[1]), int(k), tuple(p[:5] if k == 5 else p[:2]))
+           for t, k, p in zip(z["vdw_types"], z["vdw_kind"], z["vdw_params"])]
+    s = systems.System(name=name, cell=z["cell"], pos=z["pos"], vel=z["vel"], types=z["types"],
+                       type_names=[str(x) for x in z["type_names"]], masses=z["masses"],
+                       charges=z["charges"], vdw=vdw, cutoff=float(st[0]), padding=float(st[1]),
+                       vdw_cutoff=float(st[2]), electrostatics=bool(z["electrostatics"]),
+                       alpha=float(st[3]), kvec=tuple(int(k) for k in z["kvec"]),
+                       nsplines=int(z["nsplines"]), epsilon=float(st[4]), timestep=float(st[5]),
+                       temperature=float(st[6]))
+    ref = dict(frc=z["ref_frc"], pos=z["ref_pos"], vel=z["ref_vel"], stress=z["ref_stress"],
+               stpval=z["ref_stpval"], steps=int(z["steps"]),
+               engsrp=z["ref_stpval"][STP_ENGSRP], engcpe=z["ref_stpval"][STP_ENGCPE],
+               virsrp=z["ref_stpval"][STP_VIRSRP], vircpe=z["ref_stpval"][STP_VIRCPE])
+    return s, ref
+
+
+def printed_digits_bound(ref, sig=10):
+    """Half a unit in the last of `sig` printed significant figures (g20.10)."""
+    a = np.abs(ref)
+    e = np.floor(np.log10(np.where(a > 0, a, 1.0)))
+    return 0.5 * 10.0 ** (e - (sig - 1))

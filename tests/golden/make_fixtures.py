@@ -1,0 +1,92 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of the force path from the compiled reference.
+
+Each case is generated from a seed by ``dlpx.systems``, written as DL_POLY
+CONFIG/FIELD/CONTROL, and run through ``oracle/_ref/DLPOLY.Z`` (the unmodified
+DL_POLY 5.1 sources built by ``oracle/build_ref.py``).  The fixture keeps the
+inputs and the reference's outputs:
+
+* ``frc``: forces from REVCON (``3g20.10``: 10 significant figures);
+* ``stpval`` / ``stress``: the binary REVIVE record of the last step's
+  energies and virials (``statistics.F90:1118-1145``) and the configurational
+  stress, at full double precision;
+* for trajectory cases, REVCON positions and velocities after ``steps``
+  velocity-Verlet NVE steps.
+
+Run in the build container (it needs the reference binary):
+    python tests/golden/make_fixtures.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "dl-poly_amd"))
+from dlpx import fileio, systems  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    return {
+        # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
+        "nacl_1000_k32": (systems.nacl(ncells=5, kvec=(32, 32, 32)), 0),
+        # odd 2.3.5 mesh (30 = 2.3.5, 36 = 2^2.3^2, 40 = 2^3.5) and spline order 6
+        "nacl_1000_k30_36_40_s6": (systems.nacl(ncells=5, kvec=(30, 36, 40), nsplines=6, seed=7), 0),
+        # the 4,096-ion NaCl with a 48^3 mesh
+        "nacl_4096_k48": (systems.nacl(ncells=8, kvec=(48, 48, 48), seed=11), 0),
+        # orthorhombic box, non-neutral charges (net-charge correction, two_body.F90:689-696)
+        "nacl_1440_orth_q": (systems.nacl(ncells=(5, 6, 6), kvec=(32, 36, 36), seed=13,
+                                          charges=(1.05, -1.0), cutoff=10.0, alpha=0.336), 0),
+        # argon, the reference's CPU-runnable configuration at full size
+        "argon_4000": (systems.argon(), 0),
+        # short NVE trajectories: integrator + neighbour-list refresh
+        "argon_4000_nve20": (systems.argon(), 20),
+        "nacl_1000_nve10": (systems.nacl(ncells=5, kvec=(32, 32, 32)), 10),
+    }
+
+
+def run_case(name, sysm, steps):
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, sysm, steps=steps)
+        r = subprocess.run([REF], cwd=d, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0 or not os.path.exists(os.path.join(d, "REVCON")):
+            out = open(os.path.join(d, "OUTPUT")).read()[-3000:] if os.path.exists(os.path.join(d, "OUTPUT")) else ""
+            raise RuntimeError("reference failed for %s:\n%s\n%s" % (name, r.stdout[-2000:], out))
+        rc = fileio.read_revcon(os.path.join(d, "REVCON"))
+        rv = fileio.read_revive(os.path.join(d, "REVIVE"))
+    vdw_kind = np.array([v[2] for v in sysm.vdw], np.int32)
+    vdw_types = np.array([[v[0], v[1]] for v in sysm.vdw], np.int32)
+    vdw_params = np.zeros((len(sysm.vdw), 5))
+    for i, v in enumerate(sysm.vdw):
+        vdw_params[i, :len(v[3])] = v[3]
+    np.savez_compressed(
+        os.path.join(OUT, name + ".npz"),
+        cell=sysm.cell, pos=sysm.pos, vel=sysm.vel, types=sysm.types, masses=sysm.masses,
+        charges=sysm.charges, type_names=np.array(sysm.type_names), vdw_kind=vdw_kind,
+        vdw_types=vdw_types, vdw_params=vdw_params,
+        settings=np.array([sysm.cutoff, sysm.padding, sysm.vdw_cutoff, sysm.alpha, sysm.epsilon,
+                           sysm.timestep, sysm.temperature]),
+        kvec=np.array(sysm.kvec, np.int32), nsplines=np.int32(sysm.nsplines),
+        electrostatics=np.int32(sysm.electrostatics), steps=np.int32(steps),
+        ref_frc=rc["frc"], ref_pos=rc["pos"], ref_vel=rc["vel"], ref_stpval=rv["stpval"],
+        ref_stress=rv["stress"], ref_nstep=np.int32(rv["nstep"]))
+    print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))
+
+
+def main():
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: python oracle/build_ref.py")
+    only = set(sys.argv[1:])
+    for name, (sysm, steps) in cases().items():
+        if only and name not in only:
+            continue
+        run_case(name, sysm, steps)
+
+
+if __name__ == "__main__":
+    main()
