@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Throughput of the per-step force path on one MI355X.
+
+Workload (default): molten NaCl, 64,000 ions (20^3 rocksalt cells, lattice
+5.64 A expanded 8%, 1200 K, jitter U(-0.1, 0.1) A), Born-Huggins-Meyer
+(Tosi-Fumi) + SPME (rc 12 A, alpha 0.28 A^-1, mesh 128^3, B-spline order 8),
+Verlet padding 0.5 A, velocity-Verlet NVE with dt = 2 fs.  A step is one
+velocity-Verlet step: neighbour-list check (and rebuild when triggered), the
+full two-body + SPME force evaluation, and both integrator half-steps, all
+resident in HBM.
+
+Prints one JSON line: value = mixed-precision atom-steps/s; the FP64 path's
+rate, the mixed/FP64 ratio and the mixed path's force error against FP64 are
+reported beside it, with a per-stage roofline and the compiled reference
+(DL_POLY 5.1, serial) timed on the host as the CPU baseline.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dl-poly_amd"))
+
+import numpy as np  # noqa: E402
+
+from dlpx import fileio, lib, systems  # noqa: E402
+
+METRIC = "atom-steps/s (pair+SPME), mixed vs FP64 on 1 MI355X at RMS force err≤1e-5"
+HBM_PEAK = 8.0e12            # B/s
+VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    if name == "nacl64k":
+        s = systems.nacl(ncells=20, kvec=(128, 128, 128), padding=0.5)
+        desc = ("molten NaCl 64,000 ions, BHM (Tosi-Fumi) + SPME, rc 12 A, alpha 0.28/A, mesh 128^3, "
+                "order 8, padding 0.5 A, NVE dt 2 fs")
+    elif name == "nacl1728k":
+        s = systems.nacl(ncells=60, kvec=(360, 360, 360), padding=0.5)
+        desc = ("molten NaCl 1,728,000 ions, BHM (Tosi-Fumi) + SPME, rc 12 A, alpha 0.28/A, mesh 360^3, "
+                "order 8, padding 0.5 A, NVE dt 2 fs")
+    elif name == "argon4k":
+        s = systems.argon()
+        desc = "LJ argon 4,000 atoms, rc 8.5 A, padding 0.5 A, NVE dt 5 fs"
+    else:
+        raise SystemExit("unknown workload %s" % name)
+    return s, desc
+
+
+def pairs_within(ctx, s, r):
+    """Unordered pairs with separation < r (from the GPU list, measured on the host)."""
+    p = ctx.get_pairs()
+    L = np.diag(s.cell.reshape(3, 3))
+    x = s.pos - L * np.floor(s.pos / L + 0.5)
+    d = x[p[:, 0]] - (x[p[:, 1]] + p[:, 2:5] * L)
+    return int(((d * d).sum(1) < r * r).sum())
+
+
+def stage_work(s, npairs_rc, npairs_vdw_only, precision):
+    """Algorithmic flops and bytes per step of each stage (per-unit figures:
+    pair 60 flop per in-cutoff Coulomb+vdW pair, 40 per vdW-only pair, 60 B per
+    atom; spread 600 + 2 S^3 flop and 36 B per atom plus one grid write;
+    FFT+influence 2 x 2.5 K^3 log2 K^3 + 10 K^3 flop and 4 grid passes;
+    gather 8 S^3 flop and 24 B per atom plus one grid read)."""
+    n, S = s.natms, s.nsplines
+    K3 = float(np.prod(s.kvec)) if s.electrostatics else 0.0
+    gb = 8.0 if precision == lib.FP64 else 4.0   # grid element bytes
+    out = {"pair": (60.0 * npairs_rc + 40.0 * npairs_vdw_only, 60.0 * n)}
+    if s.electrostatics:
+        out["spread"] = ((600.0 + 2 * S ** 3) * n, 36.0 * n + gb * K3)
+        out["fft"] = (2 * 2.5 * K3 * math.log2(K3) + 10 * K3, 4 * gb * K3)
+        out["gather"] = (8.0 * S ** 3 * n, 24.0 * n + gb * K3)
+    return out
+
+
+def run(s, precision, steps, warmup):
+    ctx = lib.Context(s, precision=precision)
+    ctx.md_load(s.pos, s.vel)
+    if warmup:
+        ctx.md_run(warmup, s.timestep, energies=False)
+    ctx.set_timing(False)
+    t0 = time.perf_counter()
+    _, nreb = ctx.md_run(steps, s.timestep, energies=False)
+    t1 = time.perf_counter()
+    # a second, separately timed pass with per-stage HIP events on the library stream
+    ctx.set_timing(True)
+    ctx.md_run(steps, s.timestep, energies=False)
+    ms = ctx.get_timing()
+    ctx.set_timing(False)
+    ctx.close()
+    return dict(seconds=t1 - t0, rebuilds=nreb, stage_ms=ms)
+
+
+def force_error(s):
+    """Mixed-precision forces/energies against the FP64 path on the starting configuration."""
+    res = {}
+    for prec in (lib.FP64, lib.MIXED):
+        ctx = lib.Context(s, precision=prec)
+        ctx.link_cell_pairs(s.pos)
+        res[prec] = ctx.two_body_forces(s.pos)
+        if prec == lib.FP64:
+            rc = max(s.cutoff, s.vdw_cutoff)
+            np_rc = pairs_within(ctx, s, rc)
+        ctx.close()
+    f64, fmx = res[lib.FP64]["frc"], res[lib.MIXED]["frc"]
+    err = float(np.sqrt(((fmx - f64) ** 2).sum() / (f64 ** 2).sum()))
+    e64 = res[lib.FP64]["engcpe"] + res[lib.FP64]["engsrp"]
+    emx = res[lib.MIXED]["engcpe"] + res[lib.MIXED]["engsrp"]
+    v64 = res[lib.FP64]["vircpe"] + res[lib.FP64]["virsrp"]
+    vmx = res[lib.MIXED]["vircpe"] + res[lib.MIXED]["virsrp"]
+    return dict(force_rms_rel=err, energy_rel=abs(emx - e64) / abs(e64), virial_rel=abs(vmx - v64) / abs(v64),
+                npairs_rc=np_rc)
+
+
+def cpu_baseline(s, steps=4):
+    """The compiled reference (oracle/_ref/DLPOLY.Z, serial, 1 core) on the same
+    configuration: cumulative 'cpu [s]' of OUTPUT between step 1 and the last
+    step (initialisation and the step-0 force call excluded)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, s, steps=steps, print_every=1)
+        t0 = time.time()
+        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=900)
+        wall = time.time() - t0
+        out = open(os.path.join(d, "OUTPUT")).read() if os.path.exists(os.path.join(d, "OUTPUT")) else ""
+    if r.returncode != 0:
+        log("reference run failed:", r.stdout[-500:])
+        return None
+    lines = out.splitlines()
+    cpu = {}
+    for i, ln in enumerate(lines):
+        m = re.match(r"^\s+(\d+)\s+-?\d\.\d+E[+-]\d+", ln)
+        if m and i + 2 < len(lines):
+            try:
+                cpu[int(m.group(1))] = float(lines[i + 2].split()[0])
+            except (ValueError, IndexError):
+                pass
+    if 1 not in cpu or steps not in cpu:
+        return None
+    dt = cpu[steps] - cpu[1]
+    return dict(value=s.natms * (steps - 1) / dt, unit="atom-steps/s", cores=1, kind="reference",
+                sample="DL_POLY 5.1 serial (oracle/_ref/DLPOLY.Z), same %d-ion configuration, steps 2..%d of a "
+                       "%d-step NVE run (cpu[s] column of OUTPUT, %.1f s wall incl. set-up)"
+                       % (s.natms, steps, steps, wall))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="nacl64k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the force path on a single MI355X "
+              "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    s, desc = workload(args.workload)
+    log("workload:", desc)
+
+    err = force_error(s)
+    log("mixed vs fp64:", err)
+    runs = {}
+    for prec in (lib.FP64, lib.MIXED):
+        runs[prec] = run(s, prec, args.steps, args.warmup)
+        log("precision", prec, runs[prec])
+    rate = {p: s.natms * args.steps / runs[p]["seconds"] for p in runs}
+
+    # roofline per stage (algorithmic work / measured stage time) for the mixed path
+    npairs_rc = err["npairs_rc"]
+    work = stage_work(s, npairs_rc if s.electrostatics else 0, 0 if s.electrostatics else npairs_rc, lib.MIXED)
+    ms = runs[lib.MIXED]["stage_ms"]
+    nev = max(ms[7], 1.0)
+    names = ["pair", "spread", "fft", "gather"]
+    stages = {}
+    tmin_sum = tmeas_sum = 0.0
+    for k, nm in enumerate(names):
+        if nm not in work:
+            continue
+        t = ms[k] / nev * 1e-3  # s per launch
+        flops, byts = work[nm]
+        t_flop, t_byte = flops / VALU_PEAK[lib.MIXED], byts / HBM_PEAK
+        tmin = max(t_flop, t_byte)
+        stages[nm] = dict(ms=t * 1e3, bound="hbm" if t_byte >= t_flop else "valu",
+                          achieved_tflops=flops / t / 1e12, achieved_gbs=byts / t / 1e9, tmin_frac=tmin / t)
+        tmin_sum += tmin
+        tmeas_sum += t
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    d = stages[dom]
+    flops, byts = work[dom]
+    t = d["ms"] * 1e-3
+    if d["bound"] == "hbm":
+        roof = dict(bound="hbm", achieved=byts / t / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s")
+    else:
+        roof = dict(bound="valu", achieved=flops / t / 1e12, peak=VALU_PEAK[lib.MIXED] / 1e12, unit="TFLOP/s")
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = {"pair": "k_pair<float,4>", "spread": "k_spread<float,8>", "fft": "k_fft_*<float>",
+                      "gather": "k_gather<float,8>"}[dom]
+    roof["stage_fraction_sum_tmin_over_tmeasured"] = tmin_sum / tmeas_sum if tmeas_sum else None
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(s)
+    out = {
+        "metric": METRIC,
+        "value": rate[lib.MIXED],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": runs[lib.MIXED]["seconds"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulation)",
+        "data": "synthetic (seeded generator: lattice + jitter + Maxwell-Boltzmann velocities)",
+        "config": {"workload": desc, "natms": s.natms, "kvec": list(s.kvec), "nsplines": s.nsplines,
+                   "cutoff": s.cutoff, "padding": s.padding, "alpha": s.alpha, "timestep_ps": s.timestep},
+        "fp64": {"value": rate[lib.FP64], "ms_per_step": runs[lib.FP64]["seconds"] / args.steps * 1e3,
+                 "list_rebuilds": runs[lib.FP64]["rebuilds"]},
+        "mixed_over_fp64": rate[lib.MIXED] / rate[lib.FP64],
+        "mixed_error_vs_fp64": {k: err[k] for k in ("force_rms_rel", "energy_rel", "virial_rel")},
+        "list_rebuilds": runs[lib.MIXED]["rebuilds"],
+        "stages_mixed": stages,
+        "stages_fp64_ms": {nm: runs[lib.FP64]["stage_ms"][k] / max(runs[lib.FP64]["stage_ms"][7], 1.0)
+                           for k, nm in enumerate(names)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

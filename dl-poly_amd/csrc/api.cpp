@@ -1,0 +1,612 @@
+// The C ABI (include/dlpx.h): context life cycle, tables, and the
+// link_cell_pairs / vnl_check / two_body_forces / resident-MD entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+namespace dlpx {
+
+int load_atoms(Ctx *c, const double *st_dev, const int32_t *st_t_dev, bool identity, bool with_vel);
+int store_atoms(Ctx *c, double *st_dev, bool with_pos, bool with_vel);
+int spme_finish(Ctx *c, double two_scale);
+
+int fail(Ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename T>
+static int dalloc(Ctx *c, T **p, size_t count)
+{
+    DLPX_HIP(c, hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1)));
+    return 0;
+}
+
+static int set_cell(Ctx *c, const double *cell)
+{
+    if (!cell) return fail(c, DLPX_ERR_BAD_ARG, "cell is NULL");
+    if (cell[1] != 0.0 || cell[2] != 0.0 || cell[3] != 0.0 || cell[5] != 0.0 || cell[6] != 0.0 || cell[7] != 0.0)
+        return fail(c, DLPX_ERR_UNSUPPORTED, "only orthorhombic cells (imcon 1, 2) are supported");
+    double det;
+    invert_cell(cell, c->rcell, &det);
+    if (std::fabs(det) < 1.0e-6) return fail(c, DLPX_ERR_SINGULAR_CELL, "singular cell");
+    std::memcpy(c->cell, cell, sizeof(double) * 9);
+    for (int d = 0; d < 3; d++) {
+        c->L[d] = cell[4 * d];
+        if (c->L[d] <= 0.0) return fail(c, DLPX_ERR_BAD_ARG, "cell lengths must be positive");
+        c->invL[d] = 1.0 / c->L[d];
+    }
+    double p[10];
+    dcell(cell, p);
+    c->volm = p[9];
+    return 0;
+}
+
+int upload_tables(Ctx *c)
+{
+    if (!c->tables_dirty) return 0;
+    const int nt = c->prm.ntypes;
+    if (c->nkeys != nt * (nt + 1) / 2) return fail(c, DLPX_ERR_STATE, "vdW tables not set for every type pair");
+    if (c->prm.electrostatics && c->nsamples <= 0) return fail(c, DLPX_ERR_STATE, "erfc tables not set");
+    const bool mixed = c->prm.precision == DLPX_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    const int mg1 = c->max_grid + 1, ns1 = c->nsamples + 1;
+    std::vector<double> v((size_t)2 * c->nkeys * mg1), e((size_t)2 * std::max(ns1, 1), 0.0);
+    for (int k = 0; k < c->nkeys; k++)
+        for (int l = 0; l < mg1; l++) {
+            v[2 * ((size_t)k * mg1 + l)] = c->vdw_frc[(size_t)k * mg1 + l];
+            v[2 * ((size_t)k * mg1 + l) + 1] = c->vdw_pot[(size_t)k * mg1 + l];
+        }
+    for (int l = 1; l < ns1; l++) {  // sample l <-> r = l * spacing
+        e[2 * l] = c->erfc_d[l - 1];
+        e[2 * l + 1] = c->erfc_t[l - 1];
+    }
+    if (c->d_vdw) (void)hipFree(c->d_vdw);
+    if (c->d_erfc) (void)hipFree(c->d_erfc);
+    c->d_vdw = c->d_erfc = nullptr;
+    DLPX_HIP(c, hipMalloc(&c->d_vdw, rs * v.size()));
+    DLPX_HIP(c, hipMalloc(&c->d_erfc, rs * e.size()));
+    if (mixed) {
+        std::vector<float> vf(v.begin(), v.end()), ef(e.begin(), e.end());
+        for (auto &x : vf)
+            if (!std::isfinite(x)) x = 3.0e38f;  // slot 0 holds Huge
+        DLPX_HIP(c, hipMemcpy(c->d_vdw, vf.data(), rs * vf.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, hipMemcpy(c->d_erfc, ef.data(), rs * ef.size(), hipMemcpyHostToDevice));
+    } else {
+        DLPX_HIP(c, hipMemcpy(c->d_vdw, v.data(), rs * v.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, hipMemcpy(c->d_erfc, e.data(), rs * e.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<int32_t> kt((size_t)nt * nt);
+    for (int a = 1; a <= nt; a++)
+        for (int b = 1; b <= nt; b++) {
+            const int hi = std::max(a, b), lo = std::min(a, b);
+            const int key = hi * (hi - 1) / 2 + lo - 1;
+            // vdw.F90:1883: an undefined pair (tab_potential(0,k) ~ 0) is skipped
+            kt[(size_t)(a - 1) * nt + (b - 1)] = (c->ltp[key] < 0) ? -1 : key;
+        }
+    if (c->d_keytab) (void)hipFree(c->d_keytab);
+    c->d_keytab = nullptr;
+    DLPX_HIP(c, hipMalloc((void **)&c->d_keytab, sizeof(int32_t) * kt.size()));
+    DLPX_HIP(c, hipMemcpy(c->d_keytab, kt.data(), sizeof(int32_t) * kt.size(), hipMemcpyHostToDevice));
+    c->tables_dirty = false;
+    return 0;
+}
+
+// host arrays (strided) -> pinned staging -> device staging -> cell order
+static int upload(Ctx *c, const dlpx_atoms *a, bool identity, const double *vx, const double *vy, const double *vz,
+                  const double *mass)
+{
+    if (!a || !a->x || !a->y || !a->z || !a->chge || !a->ltype) return fail(c, DLPX_ERR_BAD_ARG, "atom arrays");
+    if (a->natms <= 0 || a->natms > c->cap) return fail(c, DLPX_ERR_BAD_ARG, "natms outside 1..natms_max");
+    if (!identity && a->natms != c->natms) return fail(c, DLPX_ERR_STATE, "natms changed since the list was built");
+    const int n = a->natms, st = a->stride;
+    auto at = [st](const double *p, int i) { return *(const double *)((const char *)p + (size_t)st * i); };
+    for (int i = 0; i < n; i++) {
+        double *r = c->h_st + (size_t)8 * i;
+        r[0] = at(a->x, i);
+        r[1] = at(a->y, i);
+        r[2] = at(a->z, i);
+        r[3] = at(a->chge, i);
+        if (vx) {
+            r[4] = at(vx, i);
+            r[5] = at(vy, i);
+            r[6] = at(vz, i);
+            r[7] = at(mass, i);
+        }
+        const int t = a->ltype[i];
+        if (t < 1 || t > c->prm.ntypes) return fail(c, DLPX_ERR_BAD_ARG, "ltype outside 1..ntypes");
+        c->h_t[i] = t;
+    }
+    c->natms = n;
+    DLPX_HIP(c, hipMemcpyAsync(c->stage, c->h_st, sizeof(double) * 8 * n, hipMemcpyHostToDevice, c->stream));
+    DLPX_HIP(c, hipMemcpyAsync(c->stage_i, c->h_t, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    return load_atoms(c, c->stage, c->stage_i, identity, vx != nullptr);
+}
+
+static void charges_summary(Ctx *c, const dlpx_atoms *a)
+{
+    // spme.F90:159-231 self interaction, two_body.F90:689-696 net charge
+    double sq = 0.0, sum = 0.0;
+    for (int i = 0; i < a->natms; i++) {
+        const double q = *(const double *)((const char *)a->chge + (size_t)a->stride * i);
+        sq += q * q;
+        sum += q;
+    }
+    const double scaling = kR4pie0 / c->prm.epsilon;
+    c->self_energy = -sq * (scaling * c->prm.alpha * (1.0 / std::sqrt(kPi)) / 1.0);
+    c->sumchg = sum;
+    c->self_done = true;
+}
+
+// pair + SPME forces into c->frc (cell order); partial sums reduced to red_out
+static int evaluate(Ctx *c)
+{
+    int r;
+    if ((r = upload_tables(c))) return r;
+    // one capacity for every stage's partial rows, so no stage reallocates the
+    // buffer after an earlier stage of this evaluation has written to it
+    if ((r = ensure_red_rows(c, std::max({pair_blocks(c), ceil_div(c->natms, 64), ceil_div(c->natms, 256), 2048}))))
+        return r;
+    for (int k = 0; k < kReduceSlots; k++) c->red_rows[k] = 0;
+    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if ((r = pair_forces(c))) return r;
+    for (int k = R_ENGVDW; k <= R_S33; k++) c->red_rows[k] = pair_blocks(c);
+    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    if (c->prm.electrostatics) {
+        if ((r = spme_forces(c))) return r;  // records ev[2..4] when timing
+    } else if (c->timing) {
+        for (int k = 2; k <= 4; k++) DLPX_HIP(c, hipEventRecord(c->ev[k], c->stream));
+    }
+    if ((r = reduce_partials(c))) return r;
+    if (c->prm.electrostatics) {
+        const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
+                             (kR4pie0 / c->prm.epsilon);
+        if ((r = spme_finish(c, 2.0 * scale))) return r;
+    }
+    if (c->timing) {
+        DLPX_HIP(c, hipEventRecord(c->ev[5], c->stream));
+        c->timed_evals++;
+        c->timing_pending = true;
+    }
+    return 0;
+}
+
+// accumulate the per-stage event times of the last evaluation (ms):
+// [0] pair, [1] spread, [2] FFT + influence, [3] gather, [4] reductions/finish
+static int collect_timing(Ctx *c)
+{
+    if (!c->timing || !c->timing_pending) return 0;
+    DLPX_HIP(c, hipEventSynchronize(c->ev[5]));
+    float ms;
+    for (int k = 0; k < 5; k++) {
+        DLPX_HIP(c, hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+        c->stage_ms[k] += ms;
+    }
+    c->timing_pending = false;
+    return 0;
+}
+
+// two_body.F90:708-790 bookkeeping from the reduced sums
+static void assemble(Ctx *c, const double *red, dlpx_two_body_out *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    const double engvdw = red[R_ENGVDW], virvdw = red[R_VIRVDW];
+    const double engrl = red[R_ENGRL], virrl = red[R_VIRRL];
+    double st[9] = {red[R_S11], red[R_S12], red[R_S13], red[R_S12], red[R_S22],
+                    red[R_S23], red[R_S13], red[R_S23], red[R_S33]};
+    double engrc = 0.0, virrc = 0.0, engnz = 0.0, virnz = 0.0;
+    if (c->prm.electrostatics) {
+        const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
+                             (kR4pie0 / c->prm.epsilon);
+        const double eng = red[R_ERECIP] * scale;
+        double rs[9] = {red[R_RS11], red[R_RS12], red[R_RS13], red[R_RS12], red[R_RS22],
+                        red[R_RS23], red[R_RS13], red[R_RS23], red[R_RS33]};
+        for (int k = 0; k < 9; k++) rs[k] *= scale;
+        rs[0] += eng;
+        rs[4] += eng;
+        rs[8] += eng;
+        for (int k = 0; k < 9; k++) st[k] += rs[k];
+        virrc = -(rs[0] + rs[4] + rs[8]);
+        engrc = eng + c->self_energy;
+        if (std::fabs(c->sumchg) > 1.0e-6) {
+            const double factor_nz = -0.5 * (kPi * kR4pie0 / c->prm.epsilon) * std::pow(c->sumchg / c->prm.alpha, 2);
+            engnz = factor_nz / c->volm;
+            virnz = -3.0 * engnz;
+        }
+    }
+    o->buffer[7] = engvdw;
+    o->buffer[8] = virvdw;
+    o->buffer[9] = engrc;
+    o->buffer[10] = virrc;
+    o->buffer[11] = engrl;
+    o->buffer[12] = virrl;
+    o->engcpe = engrc + engrl + engnz;
+    o->vircpe = virrc + virrl + virnz;
+    o->engsrp = engvdw + c->elrc;
+    o->virsrp = virvdw + c->vlrc;
+    const double dnz = -virnz / 3.0, dlrc = -c->vlrc / 3.0;
+    st[0] += dnz + dlrc;
+    st[4] += dnz + dlrc;
+    st[8] += dnz + dlrc;
+    std::memcpy(o->stress, st, sizeof(st));
+}
+
+static int fetch_red(Ctx *c)
+{
+    DLPX_HIP(c, hipMemcpyAsync(c->h_red, c->red_out, sizeof(double) * kReduceSlots, hipMemcpyDeviceToHost, c->stream));
+    DLPX_HIP(c, hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+}  // namespace dlpx
+
+using namespace dlpx;
+
+extern "C" {
+
+int dlpx_version(void) { return 1; }
+
+const char *dlpx_last_error(const dlpx_ctx *ctx)
+{
+    return ctx ? reinterpret_cast<const Ctx *>(ctx)->err.c_str() : "no context";
+}
+
+int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
+{
+    if (!p || !out) return DLPX_ERR_BAD_ARG;
+    *out = nullptr;
+    if (p->natms_max <= 0 || p->ntypes <= 0 || p->cutoff <= 0.0 || p->padding < 0.0 || p->vdw_cutoff <= 0.0 ||
+        (p->precision != DLPX_FP64 && p->precision != DLPX_MIXED) || p->epsilon <= 0.0)
+        return DLPX_ERR_BAD_ARG;
+    if (p->electrostatics && p->alpha <= 0.0) return DLPX_ERR_BAD_ARG;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return DLPX_ERR_BAD_ARG;
+    c->prm = *p;
+    c->cap = p->natms_max;
+    int r = 0;
+    auto bail = [&](int code) {
+        dlpx_destroy(reinterpret_cast<dlpx_ctx *>(c));
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DLPX_ERR_HIP);
+    for (int k = 0; k < 8; k++)
+        if (hipEventCreate(&c->ev[k]) != hipSuccess) return bail(DLPX_ERR_HIP);
+    const size_t n = c->cap;
+    if ((r = dalloc(c, &c->pos, n)) || (r = dalloc(c, &c->pos_alt, n)) || (r = dalloc(c, &c->vel, n)) ||
+        (r = dalloc(c, &c->vel_alt, n)) || (r = dalloc(c, &c->frc, n)) || (r = dalloc(c, &c->frc_recip, n)) ||
+        (r = dalloc(c, &c->xref, n)) || (r = dalloc(c, &c->typ, n)) || (r = dalloc(c, &c->typ_alt, n)) ||
+        (r = dalloc(c, &c->gid, n)) || (r = dalloc(c, &c->gid_alt, n)) || (r = dalloc(c, &c->slot, n)) ||
+        (r = dalloc(c, &c->stage, 9 * n)) || (r = dalloc(c, &c->stage_i, n)) || (r = dalloc(c, &c->cell_of, n)) ||
+        (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, 8)) ||
+        (r = dalloc(c, &c->red_out, kReduceSlots)))
+        return bail(r);
+    if (hipMemset(c->vel, 0, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipHostMalloc((void **)&c->h_red, sizeof(double) * kReduceSlots, hipHostMallocDefault) != hipSuccess)
+        return bail(DLPX_ERR_HIP);
+    if (hipHostMalloc((void **)&c->h_st, sizeof(double) * 9 * n, hipHostMallocDefault) != hipSuccess)
+        return bail(DLPX_ERR_HIP);
+    if (hipHostMalloc((void **)&c->h_t, sizeof(int32_t) * n, hipHostMallocDefault) != hipSuccess)
+        return bail(DLPX_ERR_HIP);
+    if (p->electrostatics && (r = spme_setup(c))) return bail(r);
+    *out = reinterpret_cast<dlpx_ctx *>(c);
+    return 0;
+}
+
+int dlpx_destroy(dlpx_ctx *ctx)
+{
+    if (!ctx) return 0;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
+                   c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
+                   c->cell_start, c->cell_fill, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
+                   c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (c->h_st) (void)hipHostFree(c->h_st);
+    if (c->h_t) (void)hipHostFree(c->h_t);
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    for (int k = 0; k < 8; k++)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int dlpx_set_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, int32_t max_grid, const double *tab_potential,
+                        const double *tab_force, const int32_t *ltp)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    const int nt = c->prm.ntypes;
+    if (nkeys != nt * (nt + 1) / 2 || max_grid < 10 || !tab_potential || !tab_force || !ltp)
+        return fail(c, DLPX_ERR_BAD_ARG, "vdW tables: need ntypes*(ntypes+1)/2 rows of max_grid+1 values");
+    c->nkeys = nkeys;
+    c->max_grid = max_grid;
+    const size_t m = (size_t)nkeys * (max_grid + 1);
+    c->vdw_pot.assign(tab_potential, tab_potential + m);
+    c->vdw_frc.assign(tab_force, tab_force + m);
+    c->ltp.assign(ltp, ltp + nkeys);
+    c->gen_params.clear();
+    c->tables_dirty = true;
+    return 0;
+}
+
+int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, const double *erfc_deriv)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    if (nsamples < 10 || !erfc || !erfc_deriv) return fail(c, DLPX_ERR_BAD_ARG, "erfc tables");
+    c->nsamples = nsamples;
+    c->erfc_t.assign(erfc, erfc + nsamples);
+    c->erfc_d.assign(erfc_deriv, erfc_deriv + nsamples);
+    c->tables_dirty = true;
+    return 0;
+}
+
+int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, const double *params)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    const int nt = c->prm.ntypes;
+    if (nkeys != nt * (nt + 1) / 2 || !kind || !params) return fail(c, DLPX_ERR_BAD_ARG, "vdW parameters");
+    const int mg = table_size(c->prm.vdw_cutoff);
+    std::vector<double> pot((size_t)nkeys * (mg + 1), 0.0), frc((size_t)nkeys * (mg + 1), 0.0);
+    for (int k = 0; k < nkeys; k++) {
+        if (kind[k] == 2 || kind[k] == 5) {
+            vdw_table(kind[k], params + 5 * k, c->prm.vdw_cutoff, mg, pot.data() + (size_t)k * (mg + 1),
+                      frc.data() + (size_t)k * (mg + 1));
+        } else if (kind[k] >= 0) {
+            return fail(c, DLPX_ERR_UNSUPPORTED, "only Lennard-Jones (2) and Born-Huggins-Meyer (5) are generated");
+        }
+    }
+    std::vector<int32_t> ltp(kind, kind + nkeys);
+    const int r = dlpx_set_vdw_tables(ctx, nkeys, mg, pot.data(), frc.data(), ltp.data());
+    if (r == 0) c->gen_params.assign(params, params + 5 * (size_t)nkeys);
+    return r;
+}
+
+int dlpx_generate_erfc_tables(dlpx_ctx *ctx)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    const int ns = table_size(c->prm.cutoff);
+    std::vector<double> t(ns), d(ns);
+    erfc_tables(c->prm.cutoff, c->prm.alpha, ns, t.data(), d.data());
+    return dlpx_set_erfc_tables(ctx, ns, t.data(), d.data());
+}
+
+int dlpx_set_lrc(dlpx_ctx *ctx, double elrc, double vlrc)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    c->elrc = elrc;
+    c->vlrc = vlrc;
+    return 0;
+}
+
+int dlpx_compute_lrc(dlpx_ctx *ctx, const double cell[9], const double *num_type)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !num_type) return DLPX_ERR_BAD_ARG;
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    if (c->gen_params.empty())
+        return fail(c, DLPX_ERR_STATE, "tables were not generated here: pass the host's vdws%elrc/vlrc to dlpx_set_lrc");
+    vdw_lrc(c->prm.ntypes, num_type, c->ltp.data(), c->gen_params.data(), c->prm.vdw_cutoff, c->volm, &c->elrc,
+            &c->vlrc);
+    return 0;
+}
+
+int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !elrc || !vlrc) return DLPX_ERR_BAD_ARG;
+    *elrc = c->elrc;
+    *vlrc = c->vlrc;
+    return 0;
+}
+
+int dlpx_link_cell_pairs(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, int64_t *npairs,
+                         int32_t *ibig)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    c->resident = false;
+    if ((r = upload(c, atoms, true, nullptr, nullptr, nullptr, nullptr))) return r;
+    r = build_list(c);
+    if (ibig) *ibig = c->h_flags[1];
+    if (r) return r;
+    if (npairs) *npairs = (int64_t)c->h_flags[3] / 2;
+    return 0;
+}
+
+int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    if (!c->have_list) return fail(c, DLPX_ERR_STATE, "no neighbour list");
+    const int n = c->natms, G = kLanesPerAtom;
+    std::vector<int32_t> nbr((size_t)c->cap_lane * c->cap * G), cnt((size_t)n * G), gid(n);
+    std::vector<double4> pos(n);
+    DLPX_HIP(c, hipMemcpy(nbr.data(), c->nbr, sizeof(int32_t) * nbr.size(), hipMemcpyDeviceToHost));
+    DLPX_HIP(c, hipMemcpy(cnt.data(), c->cnt, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost));
+    DLPX_HIP(c, hipMemcpy(gid.data(), c->gid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    DLPX_HIP(c, hipMemcpy(pos.data(), c->xref, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    int64_t np = 0;
+    for (int i = 0; i < n; i++)
+        for (int g = 0; g < G; g++)
+            for (int k = 0; k < std::min(cnt[(size_t)i * G + g], c->cap_lane); k++) {
+                const int j = nbr[((size_t)k * c->cap + i) * G + g];
+                const int hi = gid[i], hj = gid[j];
+                if (hi >= hj) continue;
+                if (np < cap && pairs) {
+                    int32_t *o = pairs + 5 * np;
+                    o[0] = hi;
+                    o[1] = hj;
+                    const double d[3] = {pos[i].x - pos[j].x, pos[i].y - pos[j].y, pos[i].z - pos[j].z};
+                    for (int a = 0; a < 3; a++) o[2 + a] = (int32_t)std::rint(d[a] * c->invL[a]);
+                }
+                np++;
+            }
+    if (npairs) *npairs = np;
+    return 0;
+}
+
+int dlpx_vnl_check(dlpx_ctx *ctx, const dlpx_atoms *atoms, int32_t *update)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !update) return DLPX_ERR_BAD_ARG;
+    if (!c->have_list) {
+        *update = 1;
+        return 0;
+    }
+    int r;
+    if ((r = upload(c, atoms, false, nullptr, nullptr, nullptr, nullptr))) return r;
+    int u = 0;
+    if ((r = vnl_check_dev(c, &u))) return r;
+    *update = u;
+    return 0;
+}
+
+int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, dlpx_two_body_out *out)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out) return DLPX_ERR_BAD_ARG;
+    if (!c->have_list) return fail(c, DLPX_ERR_STATE, "call dlpx_link_cell_pairs first");
+    if (!atoms->fx || !atoms->fy || !atoms->fz) return fail(c, DLPX_ERR_BAD_ARG, "force arrays");
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    if ((r = upload(c, atoms, false, nullptr, nullptr, nullptr, nullptr))) return r;
+    charges_summary(c, atoms);
+    if ((r = evaluate(c))) return r;
+    if ((r = store_atoms(c, c->stage, false, false))) return r;
+    const int n = c->natms;
+    DLPX_HIP(c, hipMemcpyAsync(c->h_st, c->stage, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
+    if ((r = fetch_red(c))) return r;
+    const int st = atoms->stride;
+    for (int i = 0; i < n; i++) {
+        const double *f = c->h_st + (size_t)9 * i + 6;
+        *(double *)((char *)atoms->fx + (size_t)st * i) += f[0];
+        *(double *)((char *)atoms->fy + (size_t)st * i) += f[1];
+        *(double *)((char *)atoms->fz + (size_t)st * i) += f[2];
+    }
+    assemble(c, c->h_red, out);
+    return 0;
+}
+
+int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, const double *vx, const double *vy,
+                 const double *vz, const double *mass)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !vx || !vy || !vz || !mass) return DLPX_ERR_BAD_ARG;
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    c->resident = true;
+    if ((r = upload(c, atoms, true, vx, vy, vz, mass))) return r;
+    charges_summary(c, atoms);
+    if ((r = build_list(c))) return r;
+    // forces at the starting configuration (md_vv computes them before step 1)
+    if ((r = evaluate(c))) return r;
+    return fetch_red(c);
+}
+
+int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, int32_t *nrebuilds)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || nsteps < 0) return DLPX_ERR_BAD_ARG;
+    if (!c->resident || !c->have_list) return fail(c, DLPX_ERR_STATE, "call dlpx_md_load first");
+    int r, rebuilds = 0;
+    if (energies && nsteps > c->energies_cap) {
+        if (c->energies_dev) (void)hipFree(c->energies_dev);
+        c->energies_dev = nullptr;
+        DLPX_HIP(c, hipMalloc((void **)&c->energies_dev, sizeof(double) * kReduceSlots * (size_t)nsteps));
+        c->energies_cap = nsteps;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        // nve.F90:127-193 stage 1, refresh_mappings/vnl_check, forces, stage 2 (drivers.F90:2105-2199)
+        if ((r = nve_first(c, tstep))) return r;
+        int update = 0;
+        if (c->prm.padding <= 0.0) update = 1;
+        else if ((r = vnl_check_dev(c, &update))) return r;
+        if (update) {
+            if ((r = build_list(c))) return r;
+            rebuilds++;
+        }
+        if ((r = evaluate(c))) return r;
+        if ((r = nve_second(c, tstep, true))) return r;
+        if (c->timing && (r = collect_timing(c))) return r;
+        if (energies) {
+            if ((r = reduce_partials(c))) return r;
+            DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
+                                       sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
+    DLPX_HIP(c, hipStreamSynchronize(c->stream));
+    if (energies && nsteps > 0) {
+        std::vector<double> red((size_t)kReduceSlots * nsteps);
+        DLPX_HIP(c, hipMemcpy(red.data(), c->energies_dev, sizeof(double) * red.size(), hipMemcpyDeviceToHost));
+        for (int s = 0; s < nsteps; s++) {
+            dlpx_two_body_out o;
+            assemble(c, red.data() + (size_t)kReduceSlots * s, &o);
+            double *e = energies + (size_t)5 * s;
+            e[0] = o.engcpe;
+            e[1] = o.vircpe;
+            e[2] = o.engsrp;
+            e[3] = o.virsrp;
+            e[4] = 0.5 * red[(size_t)kReduceSlots * s + R_KIN];
+        }
+    }
+    if (nrebuilds) *nrebuilds = rebuilds;
+    return 0;
+}
+
+int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *fx,
+                  double *fy, double *fz, int32_t stride)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    int r;
+    if ((r = store_atoms(c, c->stage, true, true))) return r;
+    const int n = c->natms;
+    DLPX_HIP(c, hipMemcpyAsync(c->h_st, c->stage, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));
+    DLPX_HIP(c, hipStreamSynchronize(c->stream));
+    double *dst[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
+    for (int k = 0; k < 9; k++) {
+        if (!dst[k]) continue;
+        for (int i = 0; i < n; i++) *(double *)((char *)dst[k] + (size_t)stride * i) = c->h_st[(size_t)9 * i + k];
+    }
+    return 0;
+}
+
+int dlpx_set_timing(dlpx_ctx *ctx, int32_t enable)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    c->timing = enable != 0;
+    c->timing_pending = false;
+    c->timed_evals = 0;
+    for (double &v : c->stage_ms) v = 0.0;
+    return 0;
+}
+
+int dlpx_get_timing(dlpx_ctx *ctx, double *ms, int32_t n)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !ms) return DLPX_ERR_BAD_ARG;
+    for (int k = 0; k < n && k < 8; k++) ms[k] = c->stage_ms[k];
+    if (n > 7) ms[7] = (double)c->timed_evals;
+    return 0;
+}
+
+}  // extern "C"

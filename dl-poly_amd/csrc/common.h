@@ -1,0 +1,166 @@
+// Internal definitions shared by the dlpx HIP sources (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dlpx.h"
+
+namespace dlpx {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kR4pie0 = 138935.4835;                   // constants.F90:100
+constexpr double kZeroPlus = 2.2250738585072014e-308;     // Tiny(1.0_wp)
+constexpr double kHalfMinus = 0.49999999999999994;        // Nearest(0.5_wp, -1.0_wp)
+constexpr int kLanesPerAtom = 4;                          // lanes sharing one atom's list
+constexpr int kMaxSplines = 16;
+constexpr int kReduceSlots = 24;
+
+// Columns of the per-evaluation partial-sum buffer (FP64, one row per block;
+// each column is written by exactly one kernel, which records its row count
+// in Ctx::red_rows, and is summed in fixed order).
+enum Red : int {
+    R_ENGVDW = 0, R_VIRVDW, R_ENGRL, R_VIRRL,
+    R_S11, R_S12, R_S13, R_S22, R_S23, R_S33,   // real-space stress
+    R_ERECIP,                                    // sum_i q_i sum M Phi (before scale)
+    R_FNETX, R_FNETY, R_FNETZ,                   // sum of unscaled reciprocal forces
+    R_RS11, R_RS12, R_RS13, R_RS22, R_RS23, R_RS33,  // reciprocal stress (before scale)
+    R_KIN,                                       // kinetic energy (2 x)
+    R_SPARE0, R_SPARE1, R_SPARE2
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    dlpx_params prm{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // box (orthorhombic)
+    double cell[9]{};
+    double L[3]{}, invL[3]{};
+    double rcell[9]{};
+    double volm = 0.0;
+
+    // atoms
+    int natms = 0;
+    int cap = 0;            // allocated atoms
+    bool have_list = false;
+    bool resident = false;
+    bool have_types = false;
+
+    // host tables
+    int nkeys = 0, max_grid = 0;
+    std::vector<double> vdw_pot, vdw_frc;  // nkeys x (max_grid+1)
+    std::vector<int32_t> ltp;
+    std::vector<double> gen_params;        // 5 per key when the tables were generated here
+    int nsamples = 0;
+    std::vector<double> erfc_t, erfc_d;    // nsamples
+    double elrc = 0.0, vlrc = 0.0;
+    bool tables_dirty = true;
+
+    // per-atom device arrays (sorted order unless noted)
+    double4 *pos = nullptr, *pos_alt = nullptr;     // x, y, z, q
+    double4 *vel = nullptr, *vel_alt = nullptr;     // vx, vy, vz, mass
+    double4 *frc = nullptr;                          // total force
+    double4 *frc_recip = nullptr;                    // unscaled reciprocal force
+    double4 *xref = nullptr;                         // positions at list build
+    int32_t *typ = nullptr, *typ_alt = nullptr;      // 0-based type
+    int32_t *gid = nullptr, *gid_alt = nullptr;      // sorted -> host index
+    int32_t *slot = nullptr;                         // host index -> sorted
+    double *stage = nullptr;                         // host-order staging, 9 doubles per atom
+    int32_t *stage_i = nullptr;                      // host-order int staging
+    double *h_st = nullptr;                          // pinned host staging, 9 doubles per atom
+    int32_t *h_t = nullptr;                          // pinned host staging, 1 int per atom
+    int32_t *cell_of = nullptr, *perm = nullptr;
+
+    // link cells
+    int nc[3]{1, 1, 1};
+    int ncell = 0, ncell_cap = 0;
+    int32_t *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
+
+    // Verlet list: lane-interleaved chunks; element (c, i, g) at (c*cap + i)*G + g
+    int cap_lane = 0;
+    int32_t *nbr = nullptr;
+    int32_t *cnt = nullptr;          // cap*G per-lane counts
+    int32_t *flags = nullptr;        // [0] overflow lane max, [1] ibig, [2] update flag, [3] pairs
+    int32_t *h_flags = nullptr;      // pinned mirror
+
+    // tables on device
+    void *d_vdw = nullptr;           // nkeys x (max_grid+1) x {frc, pot}, FP64 or FP32
+    void *d_erfc = nullptr;          // nsamples+1 x {deriv, erfc} (index 0 virtual)
+    int32_t *d_keytab = nullptr;     // ntypes*ntypes -> key or -1
+
+    // SPME
+    int K[3]{0, 0, 0};
+    int K1h = 0;
+    size_t nreal = 0, nhalf = 0;
+    void *Q = nullptr;               // real grid K3 x K2 x K1 (spread, then potential)
+    void *H = nullptr;               // complex half grid K3 x K2 x K1h
+    void *tw = nullptr;              // twiddles per axis: 3 x max(K) complex
+    double *bsp_n2 = nullptr;        // |b|^2 per axis, K1 + K2 + K3 (FP64)
+    double self_energy = 0.0;
+    double sumchg = 0.0;
+    bool self_done = false;
+
+    // reductions
+    double *red = nullptr;           // red_cap rows x kReduceSlots partials
+    int red_cap = 0;
+    int red_rows[kReduceSlots]{};    // rows written per column by the last evaluation
+    double *red_out = nullptr;       // kReduceSlots totals (device)
+    double *h_red = nullptr;         // pinned mirror of red_out
+    double *energies_dev = nullptr;  // per-step energies (resident runs)
+    int energies_cap = 0;
+
+    // timing
+    bool timing = false;
+    bool timing_pending = false;
+    int timed_evals = 0;
+    hipEvent_t ev[8]{};
+    double stage_ms[8]{};      // pair, spread, fft+influence, gather, finish, list build
+};
+
+// error helpers
+int fail(Ctx *c, int code, const std::string &msg);
+#define DLPX_HIP(c, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::dlpx::fail((c), DLPX_ERR_HIP + (int)e_,                           \
+                                std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// host table generation (tables.cpp)
+int table_size(double cut);
+void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc);
+void erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d);
+void bspline_norm2(int K, int S, double *norm2);
+void invert_cell(const double *a, double *b, double *det);
+void dcell(const double *a, double *b);
+void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
+             double volm, double *elrc, double *vlrc);
+
+// device stages (launch wrappers, one per .hip file)
+int upload_tables(Ctx *c);
+int build_list(Ctx *c);                      // from c->pos (current order), sorts & permutes
+int vnl_check_dev(Ctx *c, int *update);
+int pair_forces(Ctx *c);                     // writes c->frc and reduction slots
+int spme_forces(Ctx *c);                     // adds into c->frc and reduction slots
+int spme_setup(Ctx *c);
+int ensure_red_rows(Ctx *c, int rows);
+int reduce_partials(Ctx *c);                 // red -> red_out (device)
+int pair_blocks(const Ctx *c);
+int nve_first(Ctx *c, double dt);
+int nve_second(Ctx *c, double dt, bool kinetic);
+int fft_forward(Ctx *c);
+int fft_inverse(Ctx *c);
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace dlpx

@@ -1,0 +1,320 @@
+// Link-cell Verlet neighbour list on the GPU (neighbours.F90:356-1414).
+//
+// Atoms are binned into link cells of edge >= cutoff+padding, counting-sorted
+// by cell (deterministic order inside a cell) and every per-atom array is
+// permuted into that order, so the pair and SPME kernels read neighbours that
+// sit close in HBM.  The list is a full list (each pair stored from both ends)
+// split over G lanes per atom: lane g keeps every G-th candidate it accepts,
+// element (c, i, g) lives at nbr[(c*cap + i)*G + g], so the G lanes of the
+// pair kernel read 64 consecutive ints per wavefront.
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "device_util.h"
+
+namespace dlpx {
+
+struct CellGrid {
+    int nc[3];
+    double L[3], invL[3];
+};
+
+__device__ __forceinline__ int cell_coord(double x, double L, double invL, int nc)
+{
+    // x is already folded into [-L/2, L/2)
+    int k = (int)((x * invL + 0.5) * (double)nc);
+    return k < 0 ? 0 : (k >= nc ? nc - 1 : k);
+}
+
+// Fold positions into the primary cell (relocate_particles / pbcshift) and
+// count atoms per link cell.
+__global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, int32_t *count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    p.x -= g.L[0] * floor(p.x * g.invL[0] + 0.5);
+    p.y -= g.L[1] * floor(p.y * g.invL[1] + 0.5);
+    p.z -= g.L[2] * floor(p.z * g.invL[2] + 0.5);
+    // guard the rounding edge x == L/2 after folding
+    if (p.x >= 0.5 * g.L[0]) p.x -= g.L[0];
+    if (p.y >= 0.5 * g.L[1]) p.y -= g.L[1];
+    if (p.z >= 0.5 * g.L[2]) p.z -= g.L[2];
+    pos[i] = p;
+    const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.nc[0]);
+    const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.nc[1]);
+    const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.nc[2]);
+    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    cell_of[i] = c;
+    atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of the cell counts in one workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *start, int32_t *fill, int n)
+{
+    __shared__ int32_t part[1024];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < n ? count[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        const int incl = part[threadIdx.x];
+        if (i < n) {
+            start[i] = carry + incl - v;
+            fill[i] = carry + incl - v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[n] = carry;
+}
+
+__global__ void k_scatter(const int32_t *cell_of, int32_t *fill, int32_t *perm, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = atomicAdd(&fill[cell_of[i]], 1);
+    perm[p] = i;
+}
+
+// Make the order inside each cell deterministic (ascending previous index).
+__global__ void k_sort_cells(int32_t *perm, const int32_t *start, int ncell)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int s = start[c], e = start[c + 1];
+    for (int a = s + 1; a < e; a++) {
+        const int v = perm[a];
+        int b = a - 1;
+        while (b >= s && perm[b] > v) {
+            perm[b + 1] = perm[b];
+            b--;
+        }
+        perm[b + 1] = v;
+    }
+}
+
+__global__ void k_permute(const int32_t *perm, int n, const double4 *pos, double4 *pos_new,
+                          const int32_t *typ, int32_t *typ_new, const int32_t *gid, int32_t *gid_new,
+                          const double4 *vel, double4 *vel_new, int32_t *slot, double4 *xref)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    const double4 p = pos[o];
+    pos_new[i] = p;
+    xref[i] = p;
+    typ_new[i] = typ[o];
+    const int g = gid[o];
+    gid_new[i] = g;
+    slot[g] = i;
+    if (vel) vel_new[i] = vel[o];
+}
+
+// One atom per G lanes; lane g scans every G-th atom of each stencil cell.
+// Criterion r^2 <= (cutoff+padding)^2 as at neighbours.F90:991-992, with the
+// separation formed as x_i - (x_j + s L) and no fused multiply-add so the
+// accepted set is reproducible bit for bit on the host.
+template <int G>
+__global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ pos, int n, int cap, CellGrid g,
+                                                    const int32_t *__restrict__ start, double rc2, int cap_lane,
+                                                    int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
+                                                    int32_t *flags)
+{
+#pragma clang fp contract(off)
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, lane = tid % G;
+    int count = 0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int cx = cell_coord(pi.x, g.L[0], g.invL[0], g.nc[0]);
+        const int cy = cell_coord(pi.y, g.L[1], g.invL[1], g.nc[1]);
+        const int cz = cell_coord(pi.z, g.L[2], g.invL[2], g.nc[2]);
+        const int lo[3] = {g.nc[0] >= 3 ? -1 : 0, g.nc[1] >= 3 ? -1 : 0, g.nc[2] >= 3 ? -1 : 0};
+        const int hi[3] = {g.nc[0] >= 2 ? 1 : 0, g.nc[1] >= 2 ? 1 : 0, g.nc[2] >= 2 ? 1 : 0};
+        for (int oz = lo[2]; oz <= hi[2]; oz++) {
+            const int jz = (cz + oz + g.nc[2]) % g.nc[2];
+            for (int oy = lo[1]; oy <= hi[1]; oy++) {
+                const int jy = (cy + oy + g.nc[1]) % g.nc[1];
+                for (int ox = lo[0]; ox <= hi[0]; ox++) {
+                    const int jx = (cx + ox + g.nc[0]) % g.nc[0];
+                    const int jc = (jz * g.nc[1] + jy) * g.nc[0] + jx;
+                    const int e = start[jc + 1];
+                    for (int j = start[jc] + lane; j < e; j += G) {
+                        if (j == i) continue;
+                        const double4 pj = pos[j];
+                        const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
+                        const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
+                        const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
+                        const double rsq = dx * dx + dy * dy + dz * dz;
+                        if (rsq <= rc2) {
+                            if (count < cap_lane) nbr[((size_t)count * cap + i) * G + lane] = j;
+                            count++;
+                        }
+                    }
+                }
+            }
+        }
+        cnt[(size_t)i * G + lane] = count;
+        if (count > cap_lane) atomicMax(&flags[0], count);
+    }
+    // per-atom total (lanes of one atom are adjacent in the wavefront)
+    int tot = count;
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+    if (i < n && lane == 0) {
+        atomicMax(&flags[1], tot);
+        atomicAdd(&flags[3], tot);
+    }
+}
+
+// vnl_check (neighbours.F90:150-178): largest minimum-image displacement since
+// the list was built, as the bit pattern of a non-negative double.
+__global__ void k_max_disp(const double4 *pos, const double4 *xref, int n, CellGrid g,
+                           unsigned long long *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double r2 = 0.0;
+    if (i < n) {
+        const double4 p = pos[i], q = xref[i];
+        const double dx = min_image_1(p.x, q.x, g.L[0], g.invL[0]);
+        const double dy = min_image_1(p.y, q.y, g.L[1], g.invL[1]);
+        const double dz = min_image_1(p.z, q.z, g.L[2], g.invL[2]);
+        r2 = dx * dx + dy * dy + dz * dz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(r2));
+}
+
+static CellGrid make_grid(const Ctx *c)
+{
+    CellGrid g;
+    for (int d = 0; d < 3; d++) {
+        g.nc[d] = c->nc[d];
+        g.L[d] = c->L[d];
+        g.invL[d] = c->invL[d];
+    }
+    return g;
+}
+
+template <typename T>
+static int grow(Ctx *c, T **p, size_t count)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    DLPX_HIP(c, hipMalloc((void **)p, sizeof(T) * (count > 0 ? count : 1)));
+    return 0;
+}
+
+// Bin, sort, permute and build the list from c->pos (current order).
+int build_list(Ctx *c)
+{
+    const int n = c->natms;
+    const double rc_ext = c->prm.cutoff + c->prm.padding;
+    const double cut = rc_ext + 1.0e-6;  // neighbours.F90:415 (smalldr)
+    for (int d = 0; d < 3; d++) {
+        if (rc_ext >= 0.5 * c->L[d])
+            return fail(c, DLPX_ERR_CUTOFF, "cutoff+padding exceeds half the cell width");
+        c->nc[d] = (int)(c->L[d] / cut);
+        if (c->nc[d] < 1) return fail(c, DLPX_ERR_LINK_CELLS, "too few link cells");
+    }
+    const int ncell = c->nc[0] * c->nc[1] * c->nc[2];
+    if (ncell + 1 > c->ncell_cap) {
+        int r;
+        if ((r = grow(c, &c->cell_count, ncell + 1))) return r;
+        if ((r = grow(c, &c->cell_start, ncell + 1))) return r;
+        if ((r = grow(c, &c->cell_fill, ncell + 1))) return r;
+        c->ncell_cap = ncell + 1;
+    }
+    c->ncell = ncell;
+    const CellGrid g = make_grid(c);
+    hipStream_t s = c->stream;
+    DLPX_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int32_t) * (ncell + 1), s));
+    const int b = 256, nb = ceil_div(n, b);
+    hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, c->pos, n, g, c->cell_of, c->cell_count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->cell_count, c->cell_start, c->cell_fill, ncell);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, c->cell_of, c->cell_fill, c->perm, n);
+    hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div(ncell, 64)), dim3(64), 0, s, c->perm, c->cell_start, ncell);
+    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, c->perm, n, c->pos, c->pos_alt, c->typ, c->typ_alt,
+                       c->gid, c->gid_alt, c->resident ? c->vel : nullptr, c->resident ? c->vel_alt : nullptr,
+                       c->slot, c->xref);
+    std::swap(c->pos, c->pos_alt);
+    std::swap(c->typ, c->typ_alt);
+    std::swap(c->gid, c->gid_alt);
+    if (c->resident) std::swap(c->vel, c->vel_alt);
+
+    constexpr int G = kLanesPerAtom;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (c->cap_lane == 0) {
+            // first guess: density-based estimate of the per-atom count (neighbours.F90 max_list sizing)
+            const double rho = (double)n / (c->L[0] * c->L[1] * c->L[2]);
+            const double est = 4.18879020478639 * rho * rc_ext * rc_ext * rc_ext;  // 4/3 pi r^3 rho
+            const int per_atom = c->prm.max_list > 0 ? c->prm.max_list : (int)(1.5 * est) + 64;
+            c->cap_lane = (per_atom + G - 1) / G;
+            int r;
+            if ((r = grow(c, &c->nbr, (size_t)c->cap_lane * c->cap * G))) return r;
+        }
+        DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
+        const int nthreads = n * G;
+        if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
+        hipLaunchKernelGGL(k_build_list<G>, dim3(ceil_div(nthreads, 256)), dim3(256), 0, s, c->pos, n, c->cap, g,
+                           c->cell_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
+        if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
+        DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+        DLPX_HIP(c, hipStreamSynchronize(s));
+        if (c->timing) {
+            float ms;
+            DLPX_HIP(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
+            c->stage_ms[5] += ms;
+        }
+        if (c->h_flags[0] == 0) break;  // no lane overflowed
+        if (c->prm.max_list > 0) {
+            c->have_list = false;
+            return fail(c, DLPX_ERR_LIST_OVERFLOW,
+                        "neighbour list overflow: an atom has " + std::to_string(c->h_flags[1]) +
+                            " neighbours, max_list is " + std::to_string(c->prm.max_list));
+        }
+        // grow with margin and rebuild
+        c->cap_lane = c->h_flags[0] + c->h_flags[0] / 4 + 8;
+        int r;
+        if ((r = grow(c, &c->nbr, (size_t)c->cap_lane * c->cap * G))) return r;
+    }
+    if (c->h_flags[0] != 0) {
+        c->have_list = false;
+        return fail(c, DLPX_ERR_LIST_OVERFLOW, "neighbour list overflow after regrowth");
+    }
+    c->have_list = true;
+    return 0;
+}
+
+int vnl_check_dev(Ctx *c, int *update)
+{
+    const int n = c->natms;
+    hipStream_t s = c->stream;
+    unsigned long long *d = (unsigned long long *)(c->flags + 4);
+    DLPX_HIP(c, hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_disp, dim3(ceil_div(n, 256)), dim3(256), 0, s, c->pos, c->xref, n, make_grid(c), d);
+    DLPX_HIP(c, hipMemcpyAsync(c->h_flags + 4, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    DLPX_HIP(c, hipStreamSynchronize(s));
+    unsigned long long bits;
+    memcpy(&bits, c->h_flags + 4, sizeof(bits));
+    double r2;
+    memcpy(&r2, &bits, sizeof(r2));
+    const double tol = sqrt(r2);
+    *update = (tol >= kHalfMinus * c->prm.padding) ? 1 : 0;
+    return 0;
+}
+
+}  // namespace dlpx

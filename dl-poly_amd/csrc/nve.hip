@@ -1,0 +1,119 @@
+// Velocity-Verlet NVE integration on the device (nve.F90:127-215) and the
+// host-order <-> cell-order transfers of per-atom data.
+#include "common.h"
+#include "device_util.h"
+
+namespace dlpx {
+
+// Stage 1: v += dt/2 F/m ; x += dt v  (atoms of mass <= 1e-6 stay frozen)
+__global__ void k_vv1(double4 *pos, double4 *vel, const double4 *frc, int n, double dt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 v = vel[i];
+    if (v.w <= 1.0e-6) return;
+    const double4 f = frc[i];
+    const double tmp = (0.5 * dt) / v.w;
+    v.x += tmp * f.x;
+    v.y += tmp * f.y;
+    v.z += tmp * f.z;
+    double4 p = pos[i];
+    p.x += dt * v.x;
+    p.y += dt * v.y;
+    p.z += dt * v.z;
+    vel[i] = v;
+    pos[i] = p;
+}
+
+// Stage 2: v += dt/2 F/m, and sum m v^2 per block
+__global__ void __launch_bounds__(256) k_vv2(double4 *vel, const double4 *frc, int n, double dt, double *red)
+{
+    __shared__ double lred[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (i < n) {
+        double4 v = vel[i];
+        if (v.w > 1.0e-6) {
+            const double4 f = frc[i];
+            const double tmp = (0.5 * dt) / v.w;
+            v.x += tmp * f.x;
+            v.y += tmp * f.y;
+            v.z += tmp * f.z;
+            vel[i] = v;
+            acc[0] = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+        }
+    }
+    block_sum<1>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_KIN);
+}
+
+// host-order staging (8 doubles per atom: x y z q vx vy vz m) -> cell order
+__global__ void k_load(const double *__restrict__ st, const int32_t *__restrict__ st_t, int n, int cap,
+                       const int32_t *__restrict__ slot, bool identity, bool with_vel, double4 *pos, int32_t *typ,
+                       int32_t *gid, double4 *vel)
+{
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= n) return;
+    const int s = identity ? h : slot[h];
+    const double *r = st + (size_t)8 * h;
+    pos[s] = make_double4(r[0], r[1], r[2], r[3]);
+    typ[s] = st_t[h] - 1;
+    if (identity) gid[s] = h;
+    if (with_vel) vel[s] = make_double4(r[4], r[5], r[6], r[7]);
+}
+
+// cell order -> host-order staging: x y z | vx vy vz | fx fy fz
+__global__ void k_store(double *__restrict__ st, int n, const int32_t *__restrict__ gid, const double4 *pos,
+                        const double4 *vel, const double4 *frc)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int h = gid[s];
+    double *r = st + (size_t)9 * h;
+    if (pos) {
+        const double4 p = pos[s];
+        r[0] = p.x; r[1] = p.y; r[2] = p.z;
+    }
+    if (vel) {
+        const double4 v = vel[s];
+        r[3] = v.x; r[4] = v.y; r[5] = v.z;
+    }
+    const double4 f = frc[s];
+    r[6] = f.x; r[7] = f.y; r[8] = f.z;
+}
+
+int nve_first(Ctx *c, double dt)
+{
+    hipLaunchKernelGGL(k_vv1, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, c->pos, c->vel, c->frc,
+                       c->natms, dt);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int nve_second(Ctx *c, double dt, bool /*kinetic*/)
+{
+    const int nb = ceil_div(c->natms, 256);
+    int r;
+    if ((r = ensure_red_rows(c, nb))) return r;
+    hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red);
+    c->red_rows[R_KIN] = nb;
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int load_atoms(Ctx *c, const double *st_dev, const int32_t *st_t_dev, bool identity, bool with_vel)
+{
+    hipLaunchKernelGGL(k_load, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, st_dev, st_t_dev, c->natms,
+                       c->cap, c->slot, identity, with_vel, c->pos, c->typ, c->gid, c->vel);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int store_atoms(Ctx *c, double *st_dev, bool with_pos, bool with_vel)
+{
+    hipLaunchKernelGGL(k_store, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, st_dev, c->natms, c->gid,
+                       with_pos ? c->pos : nullptr, with_vel ? c->vel : nullptr, c->frc);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+}  // namespace dlpx

@@ -1,0 +1,182 @@
+// Real-space two-body forces on the Verlet list: tabulated vdW
+// (vdw.F90:1790-2024) and erfc-damped Coulomb (ewald_spole.F90:58-242),
+// both by DL_POLY's 3-point interpolation of its own tables.
+//
+// G lanes share one atom's (full) neighbour list; the force on the atom is
+// summed in FP64 across the lanes, energies/virial/stress are halved (each
+// pair is met from both ends) and reduced per block.  The tables live in LDS
+// as {force, energy} pairs per sample so one lookup is three 16-byte (FP64)
+// or 8-byte (FP32) LDS reads.
+#include "common.h"
+#include "device_util.h"
+
+namespace dlpx {
+
+template <typename R> struct R2T;
+template <> struct R2T<double> { using type = double2; };
+template <> struct R2T<float> { using type = float2; };
+
+struct PairParams {
+    int ntypes, nkeys, mg1, ns1;       // mg1 = max_grid+1, ns1 = nsamples+1
+    double L[3], invL[3];
+    double rmax2;                      // max(rc, rvdw)^2
+    double rc, rvdw;
+    double rdr_v, rdr_e;               // reciprocal table spacings
+    double scaling;                    // r4pie0/eps
+    int electro;
+    int cap_lane;                      // list capacity per lane
+};
+
+// 3-point Newton interpolation of DL_POLY (numerics.F90:250-290) on the
+// .x (force) and .y (energy) members of consecutive samples.
+template <typename R, typename R2>
+__device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
+{
+    const R s = r * rdr;
+    const int l = (int)s;
+    const R ppp = s - (R)l;
+    const R2 a = tab[l], b = tab[l + 1], c = tab[l + 2];
+    R f0 = a.x;
+    if (l == 0) f0 = f0 * r;
+    R t1 = f0 + (b.x - f0) * ppp;
+    R t2 = b.x + (c.x - b.x) * (ppp - (R)1);
+    f = t1 + (t2 - t1) * ppp * (R)0.5;
+    R e0 = a.y;
+    if (l == 0) e0 = e0 * r;
+    t1 = e0 + (b.y - e0) * ppp;
+    t2 = b.y + (c.y - b.y) * (ppp - (R)1);
+    e = t1 + (t2 - t1) * ppp * (R)0.5;
+}
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
+                                              int n, int cap, const int32_t *__restrict__ nbr,
+                                              const int32_t *__restrict__ cnt, PairParams pp,
+                                              const typename R2T<R>::type *__restrict__ vdw_g,
+                                              const typename R2T<R>::type *__restrict__ erfc_g,
+                                              const int32_t *__restrict__ keytab_g, double4 *__restrict__ frc,
+                                              double *__restrict__ red)
+{
+    using R2 = typename R2T<R>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    R2 *tv = (R2 *)smem;
+    R2 *te = tv + pp.nkeys * pp.mg1;
+    int32_t *kt = (int32_t *)(te + (pp.electro ? pp.ns1 : 0));
+    double *lred = (double *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
+
+    for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tv[k] = vdw_g[k];
+    if (pp.electro)
+        for (int k = threadIdx.x; k < pp.ns1; k += blockDim.x) te[k] = erfc_g[k];
+    for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = keytab_g[k];
+    __syncthreads();
+
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, lane = tid % G;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // engvdw, virvdw, engrl, virrl, s11..s33
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int ti = typ[i];
+        const R qi = (R)(pi.w * pp.scaling);
+        const bool qi_on = fabs(pi.w * pp.scaling) >= kZeroPlus && pp.electro;
+        const int m = min(cnt[(size_t)i * G + lane], pp.cap_lane);
+        const R rvdw = (R)pp.rvdw, rc = (R)pp.rc, rdr_v = (R)pp.rdr_v, rdr_e = (R)pp.rdr_e;
+        const int32_t *ktrow = kt + ti * pp.ntypes;
+        for (int c = 0; c < m; c++) {
+            const int j = nbr[((size_t)c * cap + i) * G + lane];
+            const double4 pj = pos[j];
+            const double dxd = min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]);
+            const double dyd = min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]);
+            const double dzd = min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]);
+            const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
+            const R rsq = dx * dx + dy * dy + dz * dz;
+            if ((double)rsq >= pp.rmax2) continue;
+            const R rrr = sqrt(rsq);
+            R gam = (R)0;
+            const int key = ktrow[typ[j]];
+            if (key >= 0 && rrr < rvdw) {
+                R gf, ge;
+                interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
+                const R rinv = (R)1 / rrr;
+                const R g = gf * (rinv * rinv);
+                gam = g;
+                acc[0] += (double)ge;
+                acc[1] -= (double)(g * (rrr * rrr));
+            }
+            if (qi_on && rrr < rc && fabs(pj.w) > kZeroPlus) {
+                R df, de;
+                interp2<R, R2>(te, rrr, rdr_e, df, de);
+                const R prefac = qi * (R)pj.w;
+                const R eg = prefac * df;
+                gam += eg;
+                acc[2] += (double)(prefac * de);
+                acc[3] -= (double)(eg * (rrr * rrr));
+            }
+            const R fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
+            fx += (double)fxp;
+            fy += (double)fyp;
+            fz += (double)fzp;
+            acc[4] += (double)(dx * fxp);
+            acc[5] += (double)(dx * fyp);
+            acc[6] += (double)(dx * fzp);
+            acc[7] += (double)(dy * fyp);
+            acc[8] += (double)(dy * fzp);
+            acc[9] += (double)(dz * fzp);
+        }
+    }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    if (i < n && lane == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] *= 0.5;  // every pair is met twice
+    __syncthreads();
+    block_sum<10>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ENGVDW);
+}
+
+template <typename R>
+static int launch_pair(Ctx *c)
+{
+    using R2 = typename R2T<R>::type;
+    constexpr int G = kLanesPerAtom;
+    PairParams pp;
+    pp.ntypes = c->prm.ntypes;
+    pp.nkeys = c->nkeys;
+    pp.mg1 = c->max_grid + 1;
+    pp.ns1 = c->nsamples + 1;
+    for (int d = 0; d < 3; d++) {
+        pp.L[d] = c->L[d];
+        pp.invL[d] = c->invL[d];
+    }
+    pp.electro = c->prm.electrostatics ? 1 : 0;
+    pp.rc = c->prm.cutoff;
+    pp.rvdw = c->prm.vdw_cutoff;
+    const double rmax = pp.electro ? std::max(pp.rc, pp.rvdw) : pp.rvdw;
+    pp.rmax2 = rmax * rmax;
+    pp.rdr_v = 1.0 / (c->prm.vdw_cutoff / (double)(c->max_grid - 4));
+    pp.rdr_e = pp.electro ? 1.0 / (c->prm.cutoff / (double)(c->nsamples - 4)) : 0.0;
+    pp.scaling = kR4pie0 / c->prm.epsilon;
+    pp.cap_lane = c->cap_lane;
+    const size_t lds = sizeof(R2) * ((size_t)pp.nkeys * pp.mg1 + (pp.electro ? pp.ns1 : 0)) +
+                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * 4;
+    if (lds > 160 * 1024)
+        return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    const int nthreads = c->natms * G;
+    hipLaunchKernelGGL((k_pair<R, G>), dim3(ceil_div(nthreads, 256)), dim3(256), lds, c->stream, c->pos, c->typ,
+                       c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw, (const R2 *)c->d_erfc,
+                       c->d_keytab, c->frc, c->red);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int pair_forces(Ctx *c)
+{
+    if (c->prm.precision == DLPX_MIXED) return launch_pair<float>(c);
+    return launch_pair<double>(c);
+}
+
+int pair_blocks(const Ctx *c) { return ceil_div((long)c->natms * kLanesPerAtom, 256); }
+
+}  // namespace dlpx

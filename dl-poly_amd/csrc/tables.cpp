@@ -1,0 +1,177 @@
+// Host-side set-up arithmetic of the force path: interpolation tables,
+// B-spline moduli, cell algebra and vdW long-range corrections.  These run
+// once per context (or per cell change) on the host, in FP64.
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace dlpx {
+
+static int nint_pos(double x) { return (int)std::floor(x + 0.5); }
+
+// bounds.F90:811,820 -- max(1004, nint(cut/delr_max) + 4), delr_max = 0.01 A
+int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
+
+// two_body_potentials.F90:260-270 (Lennard-Jones) and 499-514 (Born-Huggins-Meyer):
+// returns U(r) and gamma(r) = -r dU/dr.
+static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
+{
+    if (kind == 2) {
+        double s = p[1] / r, s2 = s * s, s6 = s2 * s2 * s2;
+        *e = 4.0 * p[0] * s6 * (s6 - 1.0);
+        *g = 24.0 * p[0] * s6 * (2.0 * s6 - 1.0);
+    } else {
+        double ri2 = 1.0 / (r * r);
+        double t1 = p[0] * std::exp(p[1] * (p[2] - r));
+        double t2 = -p[3] * (ri2 * ri2 * ri2);
+        double t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
+        *e = t1 + t2 + t3;
+        *g = t1 * r * p[1] + 6.0 * t2 + 8.0 * t3;
+    }
+}
+
+// vdw.F90:1397-1451: samples at r_i = i * rvdw/(max_grid-4), i = 1..max_grid;
+// slot 0 holds Huge (never used inside the cutoff).
+void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc)
+{
+    const double dlrpot = rvdw / (double)(max_grid - 4);
+    for (int i = 1; i <= max_grid; i++) vdw_eval(kind, p, (double)i * dlrpot, &pot[i], &frc[i]);
+    pot[0] = frc[0] = 1.79769313486231570815e308;
+}
+
+// numerics.F90:3647-3667: the Abramowitz & Stegun 7.1.26 form of erfc that the
+// reference tabulates (not the exact erfc).
+static double erfc_as(double x)
+{
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027,
+                 a5 = 1.061405429, pp = 0.3275911;
+    const double tt = 1.0 / (1.0 + pp * x);
+    return tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * std::exp(-(x * x));
+}
+
+// electrostatic.F90:88-127 with numerics.F90:215-248 and 3669-3683:
+// g(r) = erfc(alpha r)/r and g'(r) = [g(r) + alpha 2/sqrt(pi) exp(-alpha^2 r^2)]/r^2,
+// value i-1 at r = i * rc/(n-4).
+void erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d)
+{
+    const double rsqrpi = 1.0 / std::sqrt(kPi);
+    const double dr = rc / (double)(n - 4);
+    for (int i = 1; i <= n; i++) {
+        const double r = (double)i * dr;
+        const double g = erfc_as(alpha * r) / r;
+        erfc_t[i - 1] = g;
+        const double ar = alpha * r;
+        erfc_d[i - 1] = (g + alpha * (2.0 * std::exp(-(ar * ar)) * rsqrpi)) / (r * r);
+    }
+}
+
+// bspline.F90:73-190: |b(m)|^2 of the Euler exponential spline,
+// b(m) = exp(2 pi i (S-1) m/K) / sum_{k=0}^{S-2} M_S(k+1) exp(2 pi i m k/K).
+void bspline_norm2(int K, int S, double *norm2)
+{
+    std::vector<double> wr(K), wi(K);
+    wr[0] = 1.0;
+    wi[0] = 0.0;
+    for (int i = 1; i <= K / 2; i++) {
+        const double arg = (2.0 * kPi / (double)K) * (double)i;
+        wr[i] = std::cos(arg);
+        wi[i] = std::sin(arg);
+        wr[K - i] = wr[i];
+        wi[K - i] = -wi[i];
+    }
+    // cardinal B-spline of order S at the integer knots
+    std::vector<double> m(S + 2, 0.0);
+    m[2] = 1.0;
+    for (int k = 3; k <= S; k++)
+        for (int j = k; j >= 2; j--)
+            m[j] = ((double)(j - 1) * m[j] + (double)(k - j + 1) * m[j - 1]) * (1.0 / (double)(k - 1));
+    for (int i = 0; i < K; i++) {
+        double tr = 0.0, ti = 0.0;
+        for (int k = 0; k <= S - 2; k++) {
+            const int e = (int)(((long)i * k) % K);
+            tr += m[k + 2] * wr[e];
+            ti += m[k + 2] * wi[e];
+        }
+        const int e = (int)(((long)i * (S - 1)) % K);
+        const double den = tr * tr + ti * ti;
+        const double br = (wr[e] * tr + wi[e] * ti) / den;
+        const double bi = (wi[e] * tr - wr[e] * ti) / den;
+        norm2[i] = br * br + bi * bi;
+    }
+}
+
+// numerics.F90:1450-1509 (inverse by cofactors) and 1344-1448 (cell widths)
+void invert_cell(const double *a, double *b, double *det)
+{
+    b[0] = a[4] * a[8] - a[5] * a[7];
+    b[1] = a[2] * a[7] - a[1] * a[8];
+    b[2] = a[1] * a[5] - a[2] * a[4];
+    b[3] = a[5] * a[6] - a[3] * a[8];
+    b[4] = a[0] * a[8] - a[2] * a[6];
+    b[5] = a[2] * a[3] - a[0] * a[5];
+    b[6] = a[3] * a[7] - a[4] * a[6];
+    b[7] = a[1] * a[6] - a[0] * a[7];
+    b[8] = a[0] * a[4] - a[1] * a[3];
+    const double d = a[0] * b[0] + a[3] * b[1] + a[6] * b[2];
+    const double r = std::fabs(d) > 0.0 ? 1.0 / d : 0.0;
+    for (int i = 0; i < 9; i++) b[i] *= r;
+    *det = d;
+}
+
+void dcell(const double *a, double *b)
+{
+    b[0] = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    b[1] = std::sqrt(a[3] * a[3] + a[4] * a[4] + a[5] * a[5]);
+    b[2] = std::sqrt(a[6] * a[6] + a[7] * a[7] + a[8] * a[8]);
+    b[3] = (a[0] * a[3] + a[1] * a[4] + a[2] * a[5]) / (b[0] * b[1]);
+    b[4] = (a[0] * a[6] + a[1] * a[7] + a[2] * a[8]) / (b[0] * b[2]);
+    b[5] = (a[3] * a[6] + a[4] * a[7] + a[5] * a[8]) / (b[1] * b[2]);
+    const double axb[3] = {a[1] * a[5] - a[2] * a[4], a[2] * a[3] - a[0] * a[5], a[0] * a[4] - a[1] * a[3]};
+    const double bxc[3] = {a[4] * a[8] - a[5] * a[7], a[5] * a[6] - a[3] * a[8], a[3] * a[7] - a[4] * a[6]};
+    const double cxa[3] = {a[7] * a[2] - a[8] * a[1], a[8] * a[0] - a[6] * a[2], a[6] * a[1] - a[7] * a[0]};
+    b[9] = std::fabs(a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2]);
+    const double d[3] = {b[9] / std::sqrt(bxc[0] * bxc[0] + bxc[1] * bxc[1] + bxc[2] * bxc[2]),
+                         b[9] / std::sqrt(cxa[0] * cxa[0] + cxa[1] * cxa[1] + cxa[2] * cxa[2]),
+                         b[9] / std::sqrt(axb[0] * axb[0] + axb[1] * axb[1] + axb[2] * axb[2])};
+    const double x[3] = {std::fabs(a[0]) / b[0], std::fabs(a[3]) / b[1], std::fabs(a[6]) / b[2]};
+    const double y[3] = {std::fabs(a[1]) / b[0], std::fabs(a[4]) / b[1], std::fabs(a[7]) / b[2]};
+    // widths ordered to follow the axis each cell vector is most aligned with
+    int ix, i1, i2;
+    if (x[0] >= x[1] && x[0] >= x[2]) { ix = 0; i1 = 1; i2 = 2; }
+    else if (x[1] >= x[0] && x[1] >= x[2]) { ix = 1; i1 = 0; i2 = 2; }
+    else { ix = 2; i1 = 0; i2 = 1; }
+    b[6] = d[ix];
+    if (y[i1] >= y[i2]) { b[7] = d[i1]; b[8] = d[i2]; } else { b[7] = d[i2]; b[8] = d[i1]; }
+}
+
+// vdw.F90:617-967 for Lennard-Jones (2) and Born-Huggins-Meyer (5)
+void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
+             double volm, double *elrc, double *vlrc)
+{
+    double plrc = 0.0, e = 0.0;
+    int key = 0;
+    for (int i = 1; i <= ntypes; i++)
+        for (int j = 1; j <= i; j++, key++) {
+            const double *p = params + 5 * key, r = rvdw;
+            double eadd = 0.0, padd = 0.0;
+            if (kinds[key] == 2) {
+                eadd = 4.0 * p[0] * (std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / (3.0 * std::pow(r, 3)));
+                padd = 8.0 * p[0] * (6.0 * std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / std::pow(r, 3));
+            } else if (kinds[key] == 5) {
+                eadd = -p[3] / (3.0 * std::pow(r, 3)) - p[4] / (5.0 * std::pow(r, 5));
+                padd = -2.0 * p[3] / std::pow(r, 3) - 8.0 * p[4] / (5.0 * std::pow(r, 5));
+            }
+            if (i != j) {
+                eadd *= 2.0;
+                padd *= 2.0;
+            }
+            const double denprd = 2.0 * kPi * (num_type[i - 1] * num_type[j - 1]) / (volm * volm);
+            e += volm * denprd * eadd;
+            plrc += denprd * padd / 3.0;
+        }
+    *elrc = e;
+    *vlrc = plrc * (-3.0 * volm);
+}
+
+}  // namespace dlpx

@@ -138,19 +138,22 @@ def nacl(ncells=20, a0=5.64, expand=1.08, temperature=1200.0, jitter=0.1, seed=2
 
 
 def argon(ncells=10, density=1.374, temperature=120.0, seed=20261202, epsilon=99.6,
-          sigma=3.405, cutoff=8.5, padding=0.5, timestep=0.005):
+          sigma=3.405, cutoff=8.5, padding=0.5, timestep=0.005, jitter=0.0):
     """Lennard-Jones argon on an FCC lattice (4 atoms per cell), cubic box at
     mass density ``density`` g/cm^3, Maxwell-Boltzmann velocities at T.
-    epsilon = 0.996 kJ/mol = 99.6 internal units."""
+    epsilon = 0.996 kJ/mol = 99.6 internal units.  ``jitter`` > 0 displaces
+    every coordinate by U(-jitter, jitter) (the perfect lattice has zero force)."""
     n = 4 * ncells ** 3
     # number density in A^-3: rho / (m * 1.66053906660e-24 g) * 1e-24 cm^3/A^3
     nd = density / (AR_MASS * 1.66053906660)
     L = (n / nd) ** (1.0 / 3.0)
     a = L / ncells
-    (r_vel,) = _streams(seed, 1)
+    r_vel, r_jit = _streams(seed, 2)
     g = np.stack(np.meshgrid(np.arange(ncells), np.arange(ncells), np.arange(ncells),
                              indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
     pos = (g[:, None, :] + FCC[None, :, :] + 0.25).reshape(-1, 3) * a - 0.5 * L
+    if jitter > 0:
+        pos = pos + r_jit.uniform(-jitter, jitter, size=pos.shape)
     pos = _wrap(pos, L)
     types = np.zeros(n, np.int32)
     masses = np.array([AR_MASS])

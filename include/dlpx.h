@@ -1,0 +1,154 @@
+/*
+ * dlpx.h -- C ABI of the MI355X force path for DL_POLY-style molecular dynamics.
+ *
+ * The library replaces DL_POLY 5.1's per-step force evaluation on one GPU:
+ * the link-cell Verlet neighbour list, the real-space two-body loop
+ * (tabulated vdW + erfc-damped Coulomb) and the SPME reciprocal sum.  A
+ * Fortran host reaches it through ISO_C_BINDING (plain pointers, sizes and
+ * byte strides; no C++ types cross the boundary).
+ *
+ * Conventions (all in DL_POLY internal units: Angstrom, ps, Dalton, 10 J/mol, e):
+ *   - cell[9] holds the cell vectors a, b, c as rows (configuration_type%cell).
+ *     Orthorhombic cells only (imcon 1 and 2); others return DLPX_ERR_UNSUPPORTED.
+ *   - Positions/forces are addressed by base pointer + byte stride, so both
+ *     DL_POLY's corePart array (stride 64: xxx,yyy,zzz,fxx,fyy,fzz,chge) and
+ *     plain SoA arrays (stride 8) are accepted without copies on the host.
+ *   - ltype is DL_POLY's 1-based atom type (config%ltype).
+ *   - vdW pair keys are 0-based: key = hi*(hi-1)/2 + lo - 1 for 1-based types
+ *     (vdw.F90:1875-1881).
+ *   - Every entry point returns 0 on success or a DL_POLY error code
+ *     (106 list overflow, 95 cutoff vs cell width, 307 link cells, 120 singular
+ *     cell) or a DLPX_ERR_* code; dlpx_last_error() gives the message.
+ *   - A context is used from one host thread at a time; every call is
+ *     synchronous on return.
+ */
+#ifndef DLPX_H
+#define DLPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DLPX_OK 0
+#define DLPX_ERR_LIST_OVERFLOW 106  /* neighbours.F90:1189-1194 */
+#define DLPX_ERR_CUTOFF 95          /* neighbours.F90:409-412 */
+#define DLPX_ERR_LINK_CELLS 307     /* neighbours.F90:431 */
+#define DLPX_ERR_SINGULAR_CELL 120  /* ewald_spole.F90:346 */
+#define DLPX_ERR_BAD_ARG 1001
+#define DLPX_ERR_UNSUPPORTED 1002
+#define DLPX_ERR_STATE 1003
+#define DLPX_ERR_HIP 2000           /* + hipError_t */
+
+#define DLPX_FP64 0                 /* FP64 everywhere */
+#define DLPX_MIXED 1                /* FP32 pair/erfc, spreading, FFT, gather; FP64 accumulation */
+
+typedef struct dlpx_params {
+    int32_t precision;      /* DLPX_FP64 or DLPX_MIXED */
+    int32_t natms_max;      /* capacity in atoms (config%mxatms) */
+    int32_t ntypes;         /* number of atom types (sites%ntype_atom) */
+    int32_t max_list;       /* neighbours per atom (neigh%max_list); 0 = grow as needed */
+    double cutoff;          /* neigh%cutoff (real-space Coulomb and list) */
+    double padding;         /* neigh%padding (Verlet shell) */
+    double vdw_cutoff;      /* vdws%cutoff */
+    int32_t electrostatics; /* 0 = none, 1 = SPME (electro%key == ELECTROSTATIC_SPME) */
+    int32_t nsplines;       /* ewld%bspline%num_splines */
+    int32_t kvec[3];        /* ewld%kspace%k_vec_dim, each 2^a 3^b 5^c */
+    double alpha;           /* ewld%alpha */
+    double epsilon;         /* electro%eps */
+} dlpx_params;
+
+typedef struct dlpx_atoms {
+    int32_t natms;
+    int32_t stride;         /* bytes between consecutive atoms in every array below */
+    const double *x, *y, *z;
+    const double *chge;
+    double *fx, *fy, *fz;   /* forces are ADDED (callers zero them first) */
+    const int32_t *ltype;   /* 1-based, stride 4 bytes */
+} dlpx_atoms;
+
+/* Results of one two_body_forces call. */
+typedef struct dlpx_two_body_out {
+    double buffer[20];      /* the gsum buffer of two_body.F90:708-727 */
+    double stress[9];       /* added to stats%stress (row-major 3x3) */
+    double engcpe, vircpe;  /* increments of stats%engcpe/vircpe (two_body.F90:766-767) */
+    double engsrp, virsrp;  /* increments of stats%engsrp/virsrp incl. vdW LRC (:780-781) */
+} dlpx_two_body_out;
+
+typedef struct dlpx_ctx dlpx_ctx;
+
+/* Context creation (ewald_spme_init / init_list / set_bounds sizing). */
+int dlpx_create(const dlpx_params *params, dlpx_ctx **ctx);
+int dlpx_destroy(dlpx_ctx *ctx);
+const char *dlpx_last_error(const dlpx_ctx *ctx);
+int dlpx_version(void);
+
+/* Tables.  dlpx_set_vdw_tables takes the host's vdws%tab_potential and
+ * vdws%tab_force (vdw.F90:1397-1576): nkeys rows of max_grid+1 values
+ * (index 0..max_grid), row = pair key.  ltp[key] is the potential kind
+ * (vdw.F90:64-113 keys, -1 = no interaction).  dlpx_set_erfc_tables takes
+ * electro%erfc%table and electro%erfc_deriv%table (nsamples values each,
+ * value i-1 at r = i*rcut/(nsamples-4); electrostatic.F90:88-127). */
+int dlpx_set_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, int32_t max_grid, const double *tab_potential,
+                        const double *tab_force, const int32_t *ltp);
+int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, const double *erfc_deriv);
+/* The same tables generated in the library from potential parameters:
+ * kind[key] (2 = Lennard-Jones eps,sig; 5 = Born-Huggins-Meyer A,B,sigma,C,D;
+ * -1 none), params 5 per key.  Sizes follow bounds.F90:811,820. */
+int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, const double *params);
+int dlpx_generate_erfc_tables(dlpx_ctx *ctx);
+/* vdW long-range corrections vdws%elrc / vdws%vlrc (vdw.F90:617-967).  Pass the
+ * host's values, or (after dlpx_generate_vdw_tables) call dlpx_compute_lrc
+ * with the number of atoms of each type. */
+int dlpx_set_lrc(dlpx_ctx *ctx, double elrc, double vlrc);
+int dlpx_compute_lrc(dlpx_ctx *ctx, const double cell[9], const double *num_type);
+int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
+
+/* link_cell_pairs (neighbours.F90:356-1414): build the Verlet list of pairs
+ * within cutoff+padding from the given positions.  On success *npairs is the
+ * number of unordered pairs and *ibig the largest per-atom count; with a fixed
+ * max_list a larger count returns 106 with *ibig set. */
+int dlpx_link_cell_pairs(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, int64_t *npairs,
+                         int32_t *ibig);
+/* The current list as unordered pairs (i < j, 0-based host indices) with the
+ * image shift s such that r_ij = x_i - (x_j + s.L); 5 int32 per pair.
+ * *npairs returns the count; at most cap pairs are written. */
+int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs);
+
+/* vnl_check (neighbours.F90:123-296): *update = 1 when some atom moved by
+ * at least padding/2 since the list was built. */
+int dlpx_vnl_check(dlpx_ctx *ctx, const dlpx_atoms *atoms, int32_t *update);
+
+/* two_body_forces (two_body.F90:76-796) for vdW + SPME Coulomb on the current
+ * list: forces are added into atoms->fx/fy/fz; energies, virials and stress as
+ * in dlpx_two_body_out. */
+int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms,
+                         dlpx_two_body_out *out);
+
+/* Resident mode: positions, velocities and forces stay in HBM between steps.
+ * dlpx_md_load copies a configuration in (vx/vy/vz and mass with the same
+ * stride as atoms); dlpx_md_run performs nsteps velocity-Verlet NVE steps
+ * (nve.F90:127-215) with the displacement-triggered list rebuild and the
+ * two-body forces of every step; when energies is not NULL, step s writes
+ * energies[5*s + {0..4}] = engcpe, vircpe, engsrp, virsrp (as in
+ * dlpx_two_body_out) and the kinetic energy.  dlpx_md_store copies positions,
+ * velocities and forces back (any pointer may be NULL). */
+int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, const double *vx,
+                 const double *vy, const double *vz, const double *mass);
+int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, int32_t *nrebuilds);
+int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                  double *fx, double *fy, double *fz, int32_t stride);
+
+/* Per-stage device time (ms, HIP events on the context's stream), summed over
+ * the force evaluations since dlpx_set_timing(ctx, 1): ms[0] pair forces,
+ * ms[1] B-spline charge spreading, ms[2] FFT + influence function + inverse
+ * FFT, ms[3] force gather, ms[4] reductions and reciprocal-force finish,
+ * ms[5] neighbour-list kernel, ms[7] number of evaluations timed. */
+int dlpx_set_timing(dlpx_ctx *ctx, int32_t enable);
+int dlpx_get_timing(dlpx_ctx *ctx, double *ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

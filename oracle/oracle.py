@@ -74,9 +74,9 @@ def table_size(cut):
     return max(1004, nint(cut / DELR_MAX) + 4)
 
 
-def vdw_tables(sysm):
+def vdw_tables(sysm, max_grid=None):
     """Per pair key rows of length max_grid+1 (vdw.F90:1397-1576)."""
-    mg = table_size(sysm.vdw_cutoff)
+    mg = max_grid or table_size(sysm.vdw_cutoff)
     nkey = sysm.ntypes * (sysm.ntypes + 1) // 2
     pot = np.zeros((nkey, mg + 1)); frc = np.zeros((nkey, mg + 1))
     from_key = {}
@@ -113,15 +113,30 @@ def vdw_lrc(sysm, from_key):
     return e.value, v.value
 
 
-def two_body(sysm, pos=None, grids=False):
+def erfc_tables_exact(sysm, n=None):
+    """Tables of the true erfc(alpha r)/r and its exact derivative term (for
+    checks that need a force table consistent with the energy table)."""
+    from scipy import special
+    n = n or table_size(sysm.cutoff)
+    r = np.arange(1, n + 1) * (sysm.cutoff / (n - 4))
+    a = sysm.alpha
+    et = special.erfc(a * r) / r
+    ed = (et + a * 2.0 / math.sqrt(math.pi) * np.exp(-(a * r) ** 2)) / r ** 2
+    return n, et, ed
+
+
+def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
     """One call of the restated two_body_forces (with its own pair search).
+    exact_erfc=True swaps the reference's A&S erfc tables for the true erfc;
+    table_points overrides the number of table samples (finer tables make the
+    interpolated force the derivative of the interpolated energy to O(dr^2)).
 
     Returns dict: frc (N,3), buffer (20), stress (9), engcpe, vircpe, engsrp,
     virsrp, eng_self, eng_recip, npairs and, with grids=True, the real charge
     grid Q and potential grid Re(Phi) shaped (K3, K2, K1)."""
     L = lib()
-    mg, pot, frc, from_key = vdw_tables(sysm)
-    ns, et, ed = erfc_tables(sysm)
+    mg, pot, frc, from_key = vdw_tables(sysm, table_points)
+    ns, et, ed = erfc_tables_exact(sysm, table_points) if exact_erfc else erfc_tables(sysm)
     elrc, vlrc = vdw_lrc(sysm, from_key)
     xyz = np.ascontiguousarray(sysm.pos if pos is None else pos, dtype=np.float64)
     cell = np.ascontiguousarray(sysm.cell, dtype=np.float64)

@@ -42,8 +42,9 @@ def cases():
         # orthorhombic box, non-neutral charges (net-charge correction, two_body.F90:689-696)
         "nacl_1440_orth_q": (systems.nacl(ncells=(5, 6, 6), kvec=(32, 36, 36), seed=13,
                                           charges=(1.05, -1.0), cutoff=10.0, alpha=0.336), 0),
-        # argon, the reference's CPU-runnable configuration at full size
-        "argon_4000": (systems.argon(), 0),
+        # argon, the reference's CPU-runnable configuration at full size, with
+        # positions jittered by U(-0.2, 0.2) A so the step-0 forces are not zero
+        "argon_4000_jit": (systems.argon(jitter=0.2), 0),
         # short NVE trajectories: integrator + neighbour-list refresh
         "argon_4000_nve20": (systems.argon(), 20),
         "nacl_1000_nve10": (systems.nacl(ncells=5, kvec=(32, 32, 32)), 10),

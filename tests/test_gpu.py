@@ -1,0 +1,176 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with the
+compiled reference's fixtures.  Needs an MI355X.
+
+Bars: FP64 forces RMS relative error <= 1e-10 against the oracle (and every
+printed REVCON digit of the reference), energies/virials <= 1e-11 relative,
+neighbour pair set bit-exact; mixed precision forces RMS relative error
+<= 1e-5, energies <= 1e-6."""
+import numpy as np
+import pytest
+
+import fixtures
+import oracle
+from dlpx import lib, systems
+
+pytestmark = pytest.mark.gpu
+
+STATIC = [n for n in fixtures.NAMES if "nve" not in n]
+TRAJ = [n for n in fixtures.NAMES if "nve" in n]
+
+
+def rms_rel(f, ref):
+    return float(np.sqrt(((f - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+def gpu_two_body(s, precision=lib.FP64, pos=None, **kw):
+    ctx = lib.Context(s, precision=precision, **kw)
+    p = s.pos if pos is None else pos
+    ctx.link_cell_pairs(p)
+    r = ctx.two_body_forces(p)
+    return ctx, r
+
+
+def check_energies(r, ref, tol):
+    for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
+        assert abs(r[k] - ref[k]) <= tol * max(abs(ref[k]), 1.0), (k, r[k], ref[k])
+
+
+@pytest.mark.parametrize("name", STATIC)
+def test_fp64_matches_oracle_and_reference(name):
+    s, ref = fixtures.load(name)
+    print("case", name, flush=True)
+    o = oracle.two_body(s)
+    ctx, r = gpu_two_body(s)
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-10
+    assert np.abs(r["frc"] - o["frc"]).max() <= 1e-10 * np.abs(o["frc"]).max()
+    check_energies(r, o, 1e-11)
+    assert np.abs(r["stress"] - o["stress"]).max() <= 1e-11 * np.abs(o["stress"]).max()
+    assert np.allclose(r["buffer"], o["buffer"], rtol=1e-11, atol=1e-9)
+    # and the reference itself: all printed force digits, full-precision energies
+    bound = fixtures.printed_digits_bound(ref["frc"]) + 1e-12 * np.abs(ref["frc"]).max()
+    assert (np.abs(r["frc"] - ref["frc"]) / bound).max() <= 1.05
+    check_energies(r, ref, 1e-11)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", STATIC)
+def test_pair_set_bit_exact(name):
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    ctx = lib.Context(s)
+    npairs, ibig = ctx.link_cell_pairs(s.pos)
+    g = ctx.get_pairs()
+    o = oracle.pairs(s.cell, s.pos, s.cutoff + s.padding, "cells")
+    assert npairs == len(o) == len(g)
+    assert {tuple(p) for p in g} == {tuple(p) for p in o}
+    counts = np.bincount(o[:, :2].ravel(), minlength=s.natms)
+    assert ibig == counts.max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", STATIC)
+def test_mixed_precision_error_budget(name):
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    o = oracle.two_body(s)
+    ctx, r = gpu_two_body(s, precision=lib.MIXED)
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-5
+    check_energies(r, o, 1e-6)
+    ctx.close()
+
+
+def test_forces_added_to_existing_and_aos_stride():
+    s, _ = fixtures.load("nacl_1000_k32")
+    ctx = lib.Context(s)
+    ctx.link_cell_pairs(s.pos)
+    base = np.full((s.natms, 3), 7.25)
+    r1 = ctx.two_body_forces(s.pos)
+    r2 = ctx.two_body_forces(s.pos, frc=base.copy())
+    assert np.array_equal(r2["frc"] - 7.25, r1["frc"] + 7.25 - 7.25) or \
+        np.abs(r2["frc"] - 7.25 - r1["frc"]).max() < 1e-12 * np.abs(r1["frc"]).max()
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", TRAJ)
+def test_nve_trajectory_matches_reference(name):
+    s, ref = fixtures.load(name)
+    print("case", name, flush=True)
+    ctx = lib.Context(s)
+    ctx.md_load(s.pos, s.vel)
+    e, nreb = ctx.md_run(ref["steps"], s.timestep)
+    x, v, f = ctx.md_store()
+    L = np.diag(s.cell.reshape(3, 3))
+    dx = x - ref["pos"]
+    dx -= L * np.round(dx / L)
+    assert (np.abs(dx) / (fixtures.printed_digits_bound(ref["pos"]) + 1e-12)).max() <= 1.05
+    assert (np.abs(v - ref["vel"]) / (fixtures.printed_digits_bound(ref["vel"]) + 1e-12)).max() <= 1.05
+    bound = fixtures.printed_digits_bound(ref["frc"]) + 1e-12 * np.abs(ref["frc"]).max()
+    assert (np.abs(f - ref["frc"]) / bound).max() <= 1.05
+    assert abs(e[-1, 2] - ref["engsrp"]) <= 1e-10 * abs(ref["engsrp"])
+    assert abs(e[-1, 0] - ref["engcpe"]) <= 1e-10 * max(abs(ref["engcpe"]), 1.0)
+    ctx.close()
+
+
+def test_list_overflow_returns_error_106():
+    s, _ = fixtures.load("nacl_1000_k32")
+    ctx = lib.Context(s, max_list=64)
+    with pytest.raises(lib.DlpxError) as ei:
+        ctx.link_cell_pairs(s.pos)
+    assert ei.value.code == 106 and ei.value.ibig > 64
+    ctx.close()
+
+
+def test_cutoff_beyond_half_cell_returns_error_95():
+    s = systems.nacl(ncells=3, kvec=(20, 20, 20), cutoff=9.0, padding=0.5)
+    ctx = lib.Context(s)
+    with pytest.raises(lib.DlpxError) as ei:
+        ctx.link_cell_pairs(s.pos)
+    assert ei.value.code == 95
+    ctx.close()
+
+
+def test_vnl_check_trigger():
+    s, _ = fixtures.load("nacl_1000_k32")
+    ctx = lib.Context(s)
+    ctx.link_cell_pairs(s.pos)
+    p = s.pos.copy()
+    p[17, 0] += 0.2  # < padding/2
+    assert ctx.vnl_check(p) is False
+    p[17, 0] += 0.06  # 0.26 >= padding/2
+    assert ctx.vnl_check(p) is True
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def nacl64k():
+    s = systems.nacl(ncells=20, kvec=(128, 128, 128))
+    o = oracle.two_body(s)
+    return s, o
+
+
+def test_full_size_nacl_fp64_and_mixed(nacl64k):
+    s, o = nacl64k
+    print("case nacl 64000 K=128", flush=True)
+    ctx, r = gpu_two_body(s)
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-10
+    check_energies(r, o, 1e-11)
+    ctx.close()
+    ctx, rm = gpu_two_body(s, precision=lib.MIXED)
+    assert rms_rel(rm["frc"], o["frc"]) <= 1e-5
+    check_energies(rm, o, 1e-6)
+    # net force vanishes (pair forces antisymmetric, mean reciprocal force removed)
+    assert np.abs(r["frc"].sum(0)).max() <= 1e-8 * np.abs(r["frc"]).max() * np.sqrt(s.natms)
+    ctx.close()
+
+
+def test_nve_energy_conservation_fp64_and_mixed():
+    s = systems.nacl(ncells=8, kvec=(48, 48, 48), seed=3)
+    for prec, tol in ((lib.FP64, 2e-5), (lib.MIXED, 5e-5)):
+        ctx = lib.Context(s, precision=prec)
+        ctx.md_load(s.pos, s.vel)
+        e, nreb = ctx.md_run(200, s.timestep)
+        etot = e[:, 0] + e[:, 2] + e[:, 4]
+        # total energy fluctuates at the 1e-5 level and does not drift
+        assert np.abs(etot - etot[0]).max() <= tol * abs(etot[0])
+        assert nreb > 0
+        ctx.close()

@@ -1,0 +1,255 @@
+"""The oracle (C restatement in oracle/dlpx_oracle.c) pinned against the
+compiled reference's golden fixtures and against what the mathematics fixes.
+CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+import fixtures
+import oracle
+from dlpx import systems
+
+STATIC = [n for n in fixtures.NAMES if "nve" not in n]
+TRAJ = [n for n in fixtures.NAMES if "nve" in n]
+
+
+def _force_margin(f, ref):
+    """|f - ref| in units of half the last printed digit of REVCON (g20.10),
+    with a floor of 1e-12 max|F| for components that are nearly zero."""
+    bound = fixtures.printed_digits_bound(ref) + 1e-12 * np.abs(ref).max()
+    return float((np.abs(f - ref) / bound).max())
+
+
+@pytest.mark.parametrize("name", STATIC)
+def test_oracle_matches_reference_fixture(name):
+    s, ref = fixtures.load(name)
+    r = oracle.two_body(s)
+    # forces: every printed digit of the reference (10 significant figures)
+    assert _force_margin(r["frc"], ref["frc"]) <= 1.01
+    # energies and virials from the binary REVIVE record (full precision)
+    for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
+        assert abs(r[k] - ref[k]) <= 1e-12 * max(abs(ref[k]), 1.0), k
+    assert np.abs(r["stress"] - ref["stress"]).max() <= 1e-12 * np.abs(ref["stress"]).max()
+
+
+@pytest.mark.parametrize("name", TRAJ)
+def test_oracle_nve_matches_reference_trajectory(name):
+    s, ref = fixtures.load(name)
+    x, v, f, r = oracle.nve(s, ref["steps"])
+    L = np.diag(s.cell.reshape(3, 3))
+    dx = x - ref["pos"]
+    dx -= L * np.round(dx / L)  # the reference folds positions at list rebuilds
+    assert (np.abs(dx) / (fixtures.printed_digits_bound(ref["pos"]) + 1e-12)).max() <= 1.01
+    assert (np.abs(v - ref["vel"]) / (fixtures.printed_digits_bound(ref["vel"]) + 1e-12)).max() <= 1.01
+    assert _force_margin(f, ref["frc"]) <= 1.01
+    for k in ("engsrp", "engcpe"):
+        assert abs(r[k] - ref[k]) <= 1e-11 * abs(ref[k]), k
+
+
+def _eval(kind, p, r):
+    import ctypes as C
+    e, g = C.c_double(), C.c_double()
+    pp = np.ascontiguousarray(np.asarray(p, np.float64))
+    oracle.lib().orc_vdw_eval(C.c_int(kind), pp.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(r),
+                              C.byref(e), C.byref(g))
+    return e.value, g.value
+
+
+def test_vdw_known_answers():
+    # test_vdw.F90:40-63 at r = 1 A with parameters 1..7: virial = -gamma
+    e, g = _eval(2, [1, 2], 1.0)
+    assert e == 16128.0 and -g == -195072.0
+    e, g = _eval(5, [1, 2, 3, 4, 5], 1.0)
+    assert abs(e - 45.598150033144236) < 1e-12 and abs(-g - (-45.196300066288472)) < 1e-12
+
+
+@pytest.mark.parametrize("r", [2.1, 3.3, 7.9, 11.5])
+def test_vdw_closed_forms_and_gamma(r):
+    eps, sig = 99.6, 3.405
+    e, g = _eval(2, [eps, sig], r)
+    assert abs(e - 4 * eps * ((sig / r) ** 12 - (sig / r) ** 6)) <= 1e-12 * abs(e)
+    A, B, s0, Cc, D = systems.TOSI_FUMI[(1, 0)]
+    e, g = _eval(5, [A, B, s0, Cc, D], r)
+    u = lambda x: A * math.exp(B * (s0 - x)) - Cc / x ** 6 - D / x ** 8
+    assert abs(e - u(r)) <= 1e-12 * abs(e)
+    h = 1e-5  # gamma = -r dU/dr
+    assert abs(g - (-r * (u(r + h) - u(r - h)) / (2 * h))) <= 1e-6 * abs(g)
+
+
+def test_erfc_tables_closed_form():
+    rc, alpha, n = 12.0, 0.28, oracle.table_size(12.0)
+    s = systems.nacl(ncells=5, kvec=(32, 32, 32))
+    nn, et, ed = oracle.erfc_tables(s)
+    assert nn == n == 1204
+    r = np.arange(1, n + 1) * (rc / (n - 4))
+    # A&S 7.1.26 stays within 1.5e-7 of the true erfc
+    assert np.abs(et * r - special.erfc(alpha * r)).max() < 1.6e-7
+    g = et
+    dexact = (g + alpha * 2.0 / math.sqrt(math.pi) * np.exp(-(alpha * r) ** 2)) / r ** 2
+    assert np.allclose(ed, dexact, rtol=1e-13, atol=0)
+
+
+def test_interpolation_exact_for_quadratics():
+    import ctypes as C
+    a, b, c = 0.7, -1.3, 0.25
+    dr = 0.01
+    tab = np.array([a + b * (i * dr) + c * (i * dr) ** 2 for i in range(0, 1210)])
+    p = tab.ctypes.data_as(C.POINTER(C.c_double))
+    rng = np.random.default_rng(3)
+    for r in rng.uniform(0.05, 11.9, 200):
+        v = oracle.lib().orc_interp3(p, C.c_double(1.0 / dr), C.c_double(r))
+        assert abs(v - (a + b * r + c * r * r)) < 1e-12
+
+
+def _pairs_numpy(cell, x, rcut):
+    L = np.diag(cell.reshape(3, 3))
+    out = set()
+    for i in range(len(x)):
+        d = x[i] - x[i + 1:]
+        s = np.rint(d / L)
+        d = x[i] - (x[i + 1:] + s * L)
+        r2 = (d * d).sum(1)
+        for k in np.nonzero(r2 <= rcut * rcut)[0]:
+            j = i + 1 + k
+            out.add((i, j) + tuple(int(v) for v in s[k]))
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_pair_sets_cells_equal_bruteforce(seed):
+    rng = np.random.default_rng(seed)
+    L = np.array([25.0, 27.5, 31.0])
+    x = rng.uniform(-0.5, 0.5, (400, 3)) * L
+    cell = np.diag(L).reshape(9)
+    a = {tuple(p) for p in oracle.pairs(cell, x, 8.2, "cells")}
+    b = {tuple(p) for p in oracle.pairs(cell, x, 8.2, "brute")}
+    c = _pairs_numpy(cell, x, 8.2)
+    assert a == b == c and len(a) > 1000
+
+
+def _bspline_closed(x, n):
+    """Cardinal B-spline M_n(x) by the Cox-de Boor recursion (support [0, n])."""
+    if n == 1:
+        return 1.0 if 0.0 <= x < 1.0 else 0.0
+    return (x * _bspline_closed(x, n - 1) + (n - x) * _bspline_closed(x - 1, n - 1)) / (n - 1)
+
+
+@pytest.mark.parametrize("S", [4, 6, 8, 10, 12])
+def test_bspline_values_and_derivatives(S):
+    rng = np.random.default_rng(S)
+    for u in rng.uniform(0, 50, 40):
+        d = oracle.bspline(u, S)
+        w = u - math.floor(u)
+        # point t sits at floor(u) - S + 1 + t: value M_S(u - g) = M_S(w + S - 1 - t)
+        closed = np.array([_bspline_closed(w + S - 1 - t, S) for t in range(S)])
+        assert np.allclose(d[0], closed, atol=1e-14)
+        assert abs(d[0].sum() - 1.0) < 1e-14          # partition of unity
+        assert abs(d[1].sum()) < 1e-13               # derivatives sum to zero
+        dclosed = np.array([_bspline_closed(w + S - 1 - t, S - 1) - _bspline_closed(w + S - 2 - t, S - 1)
+                            for t in range(S)])
+        assert np.allclose(d[1], dclosed, atol=1e-13)  # d/du M_S(u - g)
+    # symmetry M_S(x) = M_S(S - x)
+    a, b = oracle.bspline(10.3, S), oracle.bspline(10.7, S)
+    assert np.allclose(a[0], b[0][::-1], atol=1e-14)
+
+
+@pytest.mark.parametrize("K", [(6, 10, 15), (8, 9, 5), (32, 30, 36)])
+def test_fft3d_convention(K):
+    rng = np.random.default_rng(sum(K))
+    g = rng.standard_normal((K[2], K[1], K[0])) + 1j * rng.standard_normal((K[2], K[1], K[0]))
+    plus = oracle.fft3d(g, +1)
+    minus = oracle.fft3d(g, -1)
+    # gpfa235.F90:72: +1 is sum c_k exp(+2 pi i jk/n), unnormalised
+    assert np.allclose(plus, np.fft.ifftn(g) * g.size, atol=1e-10)
+    assert np.allclose(minus, np.fft.fftn(g), atol=1e-10)
+
+
+@pytest.mark.parametrize("K,S", [(32, 8), (30, 6), (45, 8), (128, 8)])
+def test_bspline_moduli(K, S):
+    n2 = oracle.bspline_norm2(K, S)
+    Mk = np.array([_bspline_closed(k + 1.0, S) for k in range(S - 1)])
+    m = np.arange(K)
+    den = (Mk[None, :] * np.exp(2j * np.pi * np.outer(m, np.arange(S - 1)) / K)).sum(1)
+    b = np.exp(2j * np.pi * (S - 1) * m / K) / den
+    assert np.allclose(n2, np.abs(b) ** 2, rtol=1e-12)
+
+
+def test_madelung_rocksalt():
+    s = systems.nacl(ncells=5, kvec=(64, 64, 64), jitter_on=False, temperature=0.0)
+    r = oracle.two_body(s)
+    r0 = s.meta["a"] / 2.0
+    e_madelung = -s.natms / 2.0 * 1.747564594633182 * systems.R4PIE0 / r0
+    # limited by the A&S erfc (|error| < 1.5e-7) in the real-space tables
+    assert abs(r["engcpe"] - e_madelung) < 2e-6 * abs(e_madelung)
+    assert np.abs(r["frc"]).max() < 1e-3  # perfect lattice: forces vanish by symmetry
+
+
+def _ewald_recip(s, kcut):
+    """Explicit reciprocal-space Ewald sum over m = (j/Lx, k/Ly, l/Lz), |m| <= kcut."""
+    L = np.diag(s.cell.reshape(3, 3))
+    V = L.prod()
+    nmax = np.ceil(kcut * L).astype(int)
+    mx, my, mz = np.meshgrid(*[np.arange(-n, n + 1) for n in nmax], indexing="ij")
+    m = np.stack([mx.ravel() / L[0], my.ravel() / L[1], mz.ravel() / L[2]], 1)
+    m2 = (m * m).sum(1)
+    keep = (m2 > 0) & (m2 <= kcut * kcut)
+    m, m2 = m[keep], m2[keep]
+    q = s.q
+    ph = np.exp(2j * np.pi * (s.pos @ m.T))
+    S = (q[:, None] * ph).sum(0)
+    w = np.exp(-(np.pi ** 2) * m2 / s.alpha ** 2) / m2
+    E = systems.R4PIE0 / (2 * np.pi * V) * (w * np.abs(S) ** 2).sum()
+    return E
+
+
+def test_spme_energy_converges_to_explicit_ewald_sum():
+    s = systems.nacl(ncells=3, kvec=(40, 40, 40), cutoff=8.5, padding=0.0, alpha=0.395, seed=5)
+    r = oracle.two_body(s)
+    L = s.cell[0]
+    kcut = 0.5 * 1.05 * 40 / L  # the reference's spherical cutoff
+    e = _ewald_recip(s, kcut)
+    assert abs(r["eng_recip"] - e) < 1e-7 * abs(e)
+
+
+def test_forces_are_minus_energy_gradient():
+    # With the true erfc the real-space force table is the exact derivative of
+    # the energy table (the reference's A&S erfc energy is not, at ~1e-4), and
+    # with 100x finer tables the derivative of the 3-point interpolant of the
+    # energy matches the interpolated force to ~1e-7 (at the reference's
+    # 0.01 A spacing the O(dr^2) mismatch is ~1e-4 for the Born-Mayer wall).
+    s = systems.nacl(ncells=3, kvec=(40, 40, 40), cutoff=8.5, padding=0.0, alpha=0.395, seed=9)
+    fine = 85004
+    r0 = oracle.two_body(s, exact_erfc=True, table_points=fine)
+    rng = np.random.default_rng(0)
+    h = 1e-5
+    for i in rng.choice(s.natms, 4, replace=False):
+        for d in range(3):
+            xp = s.pos.copy(); xp[i, d] += h
+            xm = s.pos.copy(); xm[i, d] -= h
+            ep = oracle.two_body(s, pos=xp, exact_erfc=True, table_points=fine)
+            em = oracle.two_body(s, pos=xm, exact_erfc=True, table_points=fine)
+            fd = -((ep["engcpe"] + ep["engsrp"]) - (em["engcpe"] + em["engsrp"])) / (2 * h)
+            # tabulated force and energy are separate 3-point interpolants and
+            # the mean reciprocal force is removed
+            assert abs(fd - r0["frc"][i, d]) < 1e-6 * np.abs(r0["frc"][i]).max() + 1e-4
+
+
+def test_vdw_lrc_matches_tail_integrals():
+    s = systems.nacl(ncells=5, kvec=(32, 32, 32))
+    _, _, _, from_key = oracle.vdw_tables(s)
+    elrc, vlrc = oracle.vdw_lrc(s, from_key)
+    V = s.volume
+    n = np.bincount(s.types, minlength=2).astype(float)
+    rc = s.vdw_cutoff
+    e = v = 0.0
+    for (ti, tj, kind, p) in s.vdw:
+        A, B, s0, Cc, D = p
+        ut = lambda r: -Cc / r ** 6 - D / r ** 8   # the reference keeps only the dispersion tail
+        dut = lambda r: 6 * Cc / r ** 7 + 8 * D / r ** 9
+        mult = 1.0 if ti == tj else 2.0
+        e += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r * r * ut(r), rc, np.inf)[0]
+        v += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r ** 3 * dut(r), rc, np.inf)[0]
+    assert abs(elrc - e) < 1e-10 * abs(e)
+    assert abs(vlrc - v) < 1e-10 * abs(v)
