@@ -122,7 +122,7 @@ def force_error(s):
                 npairs_rc=np_rc)
 
 
-def cpu_baseline(s, steps=4):
+def cpu_baseline(s, steps=8):
     """The compiled reference (oracle/_ref/DLPOLY.Z, serial, 1 core) on the same
     configuration: cumulative 'cpu [s]' of OUTPUT between step 1 and the last
     step (initialisation and the step-0 force call excluded)."""

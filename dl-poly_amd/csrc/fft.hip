@@ -162,7 +162,7 @@ struct LineMap {
 
 // Complex-to-complex transforms of strided lines, in place.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_c2c(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
+__global__ void __launch_bounds__(128) k_fft_c2c(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
                                                  const typename C2T<T>::type *__restrict__ tw, int sign)
 {
     using T2 = typename C2T<T>::type;
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(256) k_fft_c2c(typename C2T<T>::type *data, Li
 // x axis, forward: real lines 2P and 2P+1 of Q (K1 contiguous values each)
 // packed as one complex line; output half spectra to H (K1h per line).
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H, int nreal_lines,
+__global__ void __launch_bounds__(128) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H, int nreal_lines,
                                                    int K1h, int lpb, FftPlan pl,
                                                    const typename C2T<T>::type *__restrict__ tw)
 {
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(const T *__restrict__ Q, type
 // x axis, inverse: rebuild the full Hermitian line of two half spectra,
 // transform with sign -1 and write the real parts to Q.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
+__global__ void __launch_bounds__(128) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
                                                    int nreal_lines, int K1h, int lpb, FftPlan pl,
                                                    const typename C2T<T>::type *__restrict__ tw)
 {
@@ -284,11 +284,12 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(const typename C2T<T>::type *
     }
 }
 
+constexpr int kFftThreads = 128;
+
 template <typename T>
 static int lines_per_block(int n)
 {
-    const int elems = sizeof(T) == 8 ? 1024 : 2048;
-    return std::max(1, elems / (n + 1));
+    return std::max(1, 1024 / (n + 1));
 }
 
 template <typename T>
@@ -313,7 +314,7 @@ static int run_c2c(Ctx *c, int axis, int sign)
     const T2 *tw = (const T2 *)c->tw + (axis == 1 ? c->K[0] : c->K[0] + c->K[1]);
     const size_t lds = lds_bytes<T>(lm.lpb, lm.n);
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_c2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_fft_c2c<T>, dim3(ceil_div(lm.nl, lm.lpb)), dim3(256), lds, c->stream, (T2 *)c->H, lm, pl, tw,
+    hipLaunchKernelGGL(k_fft_c2c<T>, dim3(ceil_div(lm.nl, lm.lpb)), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw,
                        sign);
     DLPX_HIP(c, hipGetLastError());
     return 0;
@@ -330,11 +331,11 @@ static int run_x(Ctx *c, bool forward)
     const size_t lds = lds_bytes<T>(lpb, n);
     if (forward) {
         DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(ceil_div(npairs, lpb)), dim3(256), lds, c->stream, (const T *)c->Q,
+        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(ceil_div(npairs, lpb)), dim3(kFftThreads), lds, c->stream, (const T *)c->Q,
                            (T2 *)c->H, nlines, c->K1h, lpb, pl, (const T2 *)c->tw);
     } else {
         DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(ceil_div(npairs, lpb)), dim3(256), lds, c->stream, (const T2 *)c->H,
+        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(ceil_div(npairs, lpb)), dim3(kFftThreads), lds, c->stream, (const T2 *)c->H,
                            (T *)c->Q, nlines, c->K1h, lpb, pl, (const T2 *)c->tw);
     }
     DLPX_HIP(c, hipGetLastError());

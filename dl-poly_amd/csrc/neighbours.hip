@@ -27,6 +27,25 @@ __device__ __forceinline__ int cell_coord(double x, double L, double invL, int n
     return k < 0 ? 0 : (k >= nc ? nc - 1 : k);
 }
 
+// Runs of equal keys among consecutive active lanes of a wavefront: the
+// lane that starts this lane's run and the run's length (all lanes of the
+// run get the same answer).  Lanes past the end of the data must have exited.
+__device__ __forceinline__ void run_of(int key, int &head, int &len)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long active = __ballot(1);
+    const int prev = __shfl_up(key, 1, 64);
+    const bool is_head = lane == 0 || prev != key || !((active >> (lane - 1)) & 1ull);
+    const unsigned long long heads = __ballot(is_head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    head = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int next = after ? (__ffsll((long long)after) - 1) : 64;
+    // active lanes are a prefix of the wavefront (only the tail wave is partial)
+    const int nact = 64 - __clzll(active);
+    len = min(next, nact) - head;
+}
+
 // Fold positions into the primary cell (relocate_particles / pbcshift) and
 // count atoms per link cell.
 __global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, int32_t *count)
@@ -47,7 +66,11 @@ __global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, in
     const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.nc[2]);
     const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
     cell_of[i] = c;
-    atomicAdd(&count[c], 1);
+    // atoms are mostly in cell order already: one atomic per run of equal
+    // cells within the wavefront
+    int head, len;
+    run_of(c, head, len);
+    if ((int)(threadIdx.x & 63) == head) atomicAdd(&count[c], len);
 }
 
 // Exclusive scan of the cell counts in one workgroup of 1024 threads.
@@ -84,24 +107,46 @@ __global__ void k_scatter(const int32_t *cell_of, int32_t *fill, int32_t *perm, 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int p = atomicAdd(&fill[cell_of[i]], 1);
-    perm[p] = i;
+    const int c = cell_of[i];
+    int head, len;
+    run_of(c, head, len);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == head) base = atomicAdd(&fill[c], len);
+    base = __shfl(base, head, 64);
+    perm[base + (lane - head)] = i;
 }
 
-// Make the order inside each cell deterministic (ascending previous index).
-__global__ void k_sort_cells(int32_t *perm, const int32_t *start, int ncell)
+// Make the order inside each cell deterministic (ascending previous index):
+// one workgroup per cell, every member's rank is the number of smaller
+// members (indices are distinct), so the sort is one parallel pass in LDS.
+constexpr int kSortLds = 2048;
+__global__ void __launch_bounds__(256) k_sort_cells(int32_t *perm, const int32_t *start, int ncell)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int s = start[c], e = start[c + 1];
-    for (int a = s + 1; a < e; a++) {
-        const int v = perm[a];
-        int b = a - 1;
-        while (b >= s && perm[b] > v) {
-            perm[b + 1] = perm[b];
-            b--;
-        }
-        perm[b + 1] = v;
+    __shared__ int32_t v[kSortLds];
+    const int c = blockIdx.x;
+    const int s = start[c], n = start[c + 1] - s;
+    if (n <= 1) return;
+    if (n > kSortLds) {  // a very crowded cell: serial insertion sort
+        if (threadIdx.x == 0)
+            for (int a = s + 1; a < s + n; a++) {
+                const int x = perm[a];
+                int b = a - 1;
+                while (b >= s && perm[b] > x) {
+                    perm[b + 1] = perm[b];
+                    b--;
+                }
+                perm[b + 1] = x;
+            }
+        return;
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] = perm[s + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int x = v[k];
+        int rank = 0;
+        for (int j = 0; j < n; j++) rank += v[j] < x;
+        perm[s + rank] = x;
     }
 }
 
@@ -122,27 +167,40 @@ __global__ void k_permute(const int32_t *perm, int n, const double4 *pos, double
     if (vel) vel_new[i] = vel[o];
 }
 
-// One atom per G lanes; lane g scans every G-th atom of each stencil cell.
-// Criterion r^2 <= (cutoff+padding)^2 as at neighbours.F90:991-992, with the
-// separation formed as x_i - (x_j + s L) and no fused multiply-add so the
+// One workgroup per link cell: its atoms are the i atoms (256/G per pass, G
+// lanes each); the positions of every stencil cell are staged through LDS in
+// chunks, and lane g tests the chunk atoms j = g, g+G, ... (chunks are a
+// multiple of G long, so lane g always meets j = start + g mod G).
+//
+// Positions are folded into [-L/2, L/2) at the rebuild, so the image shift of
+// a stencil cell is -L, 0 or +L per axis and is taken once per cell.  The
+// separation is x_i - (x_j + s L) and the criterion r^2 <= (cutoff+padding)^2
+// (neighbours.F90:991-992).  A single-precision distance settles every pair
+// that is clearly inside or outside; pairs within a relative 1e-5 of the
+// cutoff are decided in double precision without fused multiply-add, so the
 // accepted set is reproducible bit for bit on the host.
+constexpr int kListChunk = 256;
 template <int G>
-__global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ pos, int n, int cap, CellGrid g,
+__global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ pos, int cap, CellGrid g,
                                                     const int32_t *__restrict__ start, double rc2, int cap_lane,
                                                     int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
                                                     int32_t *flags)
 {
-#pragma clang fp contract(off)
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = tid / G, lane = tid % G;
-    int count = 0;
-    if (i < n) {
-        const double4 pi = pos[i];
-        const int cx = cell_coord(pi.x, g.L[0], g.invL[0], g.nc[0]);
-        const int cy = cell_coord(pi.y, g.L[1], g.invL[1], g.nc[1]);
-        const int cz = cell_coord(pi.z, g.L[2], g.invL[2], g.nc[2]);
-        const int lo[3] = {g.nc[0] >= 3 ? -1 : 0, g.nc[1] >= 3 ? -1 : 0, g.nc[2] >= 3 ? -1 : 0};
-        const int hi[3] = {g.nc[0] >= 2 ? 1 : 0, g.nc[1] >= 2 ? 1 : 0, g.nc[2] >= 2 ? 1 : 0};
+    __shared__ double sx[kListChunk], sy[kListChunk], sz[kListChunk];
+    __shared__ float fx[kListChunk], fy[kListChunk], fz[kListChunk];
+    const int c = blockIdx.x;
+    const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+    const int hs = start[c], he = start[c + 1];
+    const int lo[3] = {g.nc[0] >= 3 ? -1 : 0, g.nc[1] >= 3 ? -1 : 0, g.nc[2] >= 3 ? -1 : 0};
+    const int hi[3] = {g.nc[0] >= 2 ? 1 : 0, g.nc[1] >= 2 ? 1 : 0, g.nc[2] >= 2 ? 1 : 0};
+    const float rc2_lo = (float)(rc2 * (1.0 - 2.0e-5)), rc2_hi = (float)(rc2 * (1.0 + 2.0e-5));
+    const bool small = g.nc[0] < 3 || g.nc[1] < 3 || g.nc[2] < 3;
+    for (int h0 = hs; h0 < he; h0 += 256 / G) {
+        const int i = h0 + (int)threadIdx.x / G, lane = threadIdx.x % G;
+        const bool active = i < he;
+        const double4 pi = active ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const float pfx = (float)pi.x, pfy = (float)pi.y, pfz = (float)pi.z;
+        int count = 0;
         for (int oz = lo[2]; oz <= hi[2]; oz++) {
             const int jz = (cz + oz + g.nc[2]) % g.nc[2];
             for (int oy = lo[1]; oy <= hi[1]; oy++) {
@@ -150,32 +208,73 @@ __global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ 
                 for (int ox = lo[0]; ox <= hi[0]; ox++) {
                     const int jx = (cx + ox + g.nc[0]) % g.nc[0];
                     const int jc = (jz * g.nc[1] + jy) * g.nc[0] + jx;
-                    const int e = start[jc + 1];
-                    for (int j = start[jc] + lane; j < e; j += G) {
-                        if (j == i) continue;
-                        const double4 pj = pos[j];
-                        const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
-                        const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
-                        const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
-                        const double rsq = dx * dx + dy * dy + dz * dz;
-                        if (rsq <= rc2) {
-                            if (count < cap_lane) nbr[((size_t)count * cap + i) * G + lane] = j;
-                            count++;
+                    // image shift of the stencil cell (folded positions)
+                    const double shx = (double)((cx + ox) - jx) / g.nc[0] * g.L[0];
+                    const double shy = (double)((cy + oy) - jy) / g.nc[1] * g.L[1];
+                    const double shz = (double)((cz + oz) - jz) / g.nc[2] * g.L[2];
+                    const int js = start[jc], je = start[jc + 1];
+                    for (int j0 = js; j0 < je; j0 += kListChunk) {
+                        const int m = min(kListChunk, je - j0);
+                        __syncthreads();
+                        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+                            const double4 p = pos[j0 + k];
+                            sx[k] = p.x;
+                            sy[k] = p.y;
+                            sz[k] = p.z;
+                            fx[k] = (float)(p.x + shx);
+                            fy[k] = (float)(p.y + shy);
+                            fz[k] = (float)(p.z + shz);
+                        }
+                        __syncthreads();
+                        if (!active) continue;
+                        for (int k = lane; k < m; k += G) {
+                            const int j = j0 + k;
+                            if (j == i) continue;
+                            bool in;
+                            if (small) {
+                                // 1-2 cells per axis: the stencil shift need not be the
+                                // minimum image, so every candidate is decided exactly
+                                const double dx = min_image_1(pi.x, sx[k], g.L[0], g.invL[0]);
+                                const double dy = min_image_1(pi.y, sy[k], g.L[1], g.invL[1]);
+                                const double dz = min_image_1(pi.z, sz[k], g.L[2], g.invL[2]);
+                                const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                                             __dmul_rn(dz, dz));
+                                in = rsq <= rc2;
+                            } else {
+                                const float ex = pfx - fx[k], ey = pfy - fy[k], ez = pfz - fz[k];
+                                const float r2f = ex * ex + ey * ey + ez * ez;
+                                if (r2f > rc2_hi) continue;
+                                in = r2f < rc2_lo;
+                                if (!in) {
+                                    const double dx = __dsub_rn(pi.x, __dadd_rn(sx[k], shx));
+                                    const double dy = __dsub_rn(pi.y, __dadd_rn(sy[k], shy));
+                                    const double dz = __dsub_rn(pi.z, __dadd_rn(sz[k], shz));
+                                    const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                                                 __dmul_rn(dz, dz));
+                                    in = rsq <= rc2;
+                                }
+                            }
+                            if (in) {
+                                if (count < cap_lane) nbr[((size_t)count * cap + i) * G + lane] = j;
+                                count++;
+                            }
                         }
                     }
                 }
             }
         }
-        cnt[(size_t)i * G + lane] = count;
-        if (count > cap_lane) atomicMax(&flags[0], count);
-    }
-    // per-atom total (lanes of one atom are adjacent in the wavefront)
-    int tot = count;
+        if (active) {
+            cnt[(size_t)i * G + lane] = count;
+            if (count > cap_lane) atomicMax(&flags[0], count);
+        }
+        // per-atom total (lanes of one atom are adjacent in the wavefront)
+        int tot = count;
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
-    if (i < n && lane == 0) {
-        atomicMax(&flags[1], tot);
-        atomicAdd(&flags[3], tot);
+        for (int o = G / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+        if (active && lane == 0) {
+            atomicMax(&flags[1], tot);
+            atomicAdd(&flags[3], tot);
+        }
     }
 }
 
@@ -246,7 +345,7 @@ int build_list(Ctx *c)
     hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, c->pos, n, g, c->cell_of, c->cell_count);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->cell_count, c->cell_start, c->cell_fill, ncell);
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, c->cell_of, c->cell_fill, c->perm, n);
-    hipLaunchKernelGGL(k_sort_cells, dim3(ceil_div(ncell, 64)), dim3(64), 0, s, c->perm, c->cell_start, ncell);
+    hipLaunchKernelGGL(k_sort_cells, dim3(ncell), dim3(256), 0, s, c->perm, c->cell_start, ncell);
     hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, c->perm, n, c->pos, c->pos_alt, c->typ, c->typ_alt,
                        c->gid, c->gid_alt, c->resident ? c->vel : nullptr, c->resident ? c->vel_alt : nullptr,
                        c->slot, c->xref);
@@ -267,10 +366,9 @@ int build_list(Ctx *c)
             if ((r = grow(c, &c->nbr, (size_t)c->cap_lane * c->cap * G))) return r;
         }
         DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
-        const int nthreads = n * G;
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_build_list<G>, dim3(ceil_div(nthreads, 256)), dim3(256), 0, s, c->pos, n, c->cap, g,
-                           c->cell_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
+        hipLaunchKernelGGL(k_build_list<G>, dim3(ncell), dim3(256), 0, s, c->pos, c->cap, g, c->cell_start,
+                           rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
         DLPX_HIP(c, hipStreamSynchronize(s));

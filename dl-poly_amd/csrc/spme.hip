@@ -144,6 +144,125 @@ __global__ void __launch_bounds__(256) k_spread(const double4 *__restrict__ pos,
     }
 }
 
+// Unwrapped fractional grid coordinate of one axis (no folding): the base
+// grid point floor(u) - S + 1 may be negative or >= K; callers fold indices.
+__device__ __forceinline__ double grid_u(const SpmeParams &sp, int d, double x, double y, double z)
+{
+    return sp.kr[d] * (sp.rcell[d] * x + sp.rcell[d + 3] * y + sp.rcell[d + 6] * z + 0.5);
+}
+
+__device__ __forceinline__ int wrapk(int v, int K)
+{
+    v %= K;
+    return v < 0 ? v + K : v;
+}
+
+constexpr int kSlabBytes = 32768;
+
+// Charge spreading by link cell (one workgroup per cell of the sorted atom
+// order): the cell's atoms touch a compact window of grid points, which is
+// accumulated in LDS in z-slabs with LDS atomics and then added to the global
+// grid with one atomic per window point, instead of S^3 global atomics per
+// atom.  A window too large for LDS (very large cells, or atoms far outside
+// their cell) falls back to per-atom global atomics for that cell.
+template <typename R, int S>
+__global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict__ pos,
+                                                      const int32_t *__restrict__ cell_start, SpmeParams sp,
+                                                      R *__restrict__ Q)
+{
+    constexpr int kCap = kSlabBytes / (int)sizeof(R);
+    __shared__ R slab[kCap];
+    __shared__ R sM[3][kAtomsPerBlock][S];
+    __shared__ int sB[3][kAtomsPerBlock];
+    __shared__ R sQ[kAtomsPerBlock];
+    __shared__ int wlo[3], whi[3];
+    const int c = blockIdx.x, t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    const int s0 = cell_start[c], n = cell_start[c + 1] - s0;
+    if (n <= 0) return;
+    if (t < 3) {
+        wlo[t] = 0x7fffffff;
+        whi[t] = -0x7fffffff;
+    }
+    __syncthreads();
+    for (int k = t; k < n; k += blockDim.x) {
+        const double4 p = pos[s0 + k];
+        for (int d = 0; d < 3; d++) {
+            const int b = (int)floor(grid_u(sp, d, p.x, p.y, p.z)) - S + 1;
+            atomicMin(&wlo[d], b);
+            atomicMax(&whi[d], b);
+        }
+    }
+    __syncthreads();
+    const int lo0 = wlo[0], lo1 = wlo[1], lo2 = wlo[2];
+    // window extents in 64-bit so that non-finite coordinates cannot overflow
+    const long ex = (long)whi[0] - lo0 + S, ey = (long)whi[1] - lo1 + S, ez = (long)whi[2] - lo2 + S;
+    const bool tile = ex <= sp.K[0] && ey <= sp.K[1] && ez <= sp.K[2] && ex * ey <= kCap;
+    const int Wx = tile ? (int)ex : 0, Wy = tile ? (int)ey : 0, Wz = tile ? (int)ez : 1;
+    const int Dz = tile ? min(Wz, kCap / (Wx * Wy)) : Wz;
+    for (int z0 = 0; z0 < Wz; z0 += Dz) {
+        if (tile) {
+            for (int k = t; k < Wx * Wy * Dz; k += blockDim.x) slab[k] = (R)0;
+        }
+        for (int a0 = 0; a0 < n; a0 += kAtomsPerBlock) {
+            __syncthreads();
+            if (t < 3 * kAtomsPerBlock) {
+                const int d = t / kAtomsPerBlock, a = t % kAtomsPerBlock;
+                if (a0 + a < n) {
+                    const double4 p = pos[s0 + a0 + a];
+                    const double u = grid_u(sp, d, p.x, p.y, p.z);
+                    const double fl = floor(u);
+                    R M[S];
+                    bspline<R, false, S>((R)(u - fl), M, nullptr);
+#pragma unroll
+                    for (int k = 0; k < S; k++) sM[d][a][k] = M[k];
+                    sB[d][a] = (int)fl - S + 1;
+                    if (d == 0) sQ[a] = (R)p.w;
+                }
+            }
+            __syncthreads();
+            const int na = min(kAtomsPerBlock, n - a0);
+            for (int a = wid; a < na; a += 4) {
+                const R q = sQ[a];
+                if (fabs((double)q) <= kZeroPlus) continue;
+                const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
+                for (int pp = lane; pp < S * S; pp += 64) {
+                    const int tx = pp % S, ty = pp / S;
+                    const R mxy = sM[0][a][tx] * sM[1][a][ty];
+                    if (tile) {
+                        const int lxy = (by + ty - lo1) * Wx + (bx + tx - lo0);
+#pragma unroll
+                        for (int tz = 0; tz < S; tz++) {
+                            const int lz = bz + tz - lo2 - z0;
+                            const R v = q * sM[2][a][tz] * mxy;
+                            // single precision: contributions below 1e-30 (spline tails)
+                            // are dropped rather than added as denormals
+                            if (lz >= 0 && lz < Dz && (sizeof(R) == 8 || fabs(v) >= (R)1e-30))
+                                atomicAdd(&slab[lz * Wx * Wy + lxy], v);
+                        }
+                    } else {
+                        const int gx = wrapk(bx + tx, sp.K[0]), gy = wrapk(by + ty, sp.K[1]);
+#pragma unroll
+                        for (int tz = 0; tz < S; tz++) {
+                            const int gz = wrapk(bz + tz, sp.K[2]);
+                            unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], q * sM[2][a][tz] * mxy);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (!tile) break;  // the direct path did every z at once
+        const int dz = min(Dz, Wz - z0);
+        for (int k = t; k < Wx * Wy * dz; k += blockDim.x) {
+            const R v = slab[k];
+            if (v == (R)0) continue;
+            const int lx = k % Wx, ly = (k / Wx) % Wy, lz = k / (Wx * Wy);
+            const int gx = wrapk(lo0 + lx, sp.K[0]), gy = wrapk(lo1 + ly, sp.K[1]), gz = wrapk(lo2 + z0 + lz, sp.K[2]);
+            unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], v);
+        }
+    }
+}
+
 // Influence function and reciprocal stress over the half spectrum
 // (x index 0..K1/2), ewald_spole.F90:1296-1380.
 struct InflParams {
@@ -298,21 +417,20 @@ __global__ void k_finish(double4 *__restrict__ frc, const double4 *__restrict__ 
 struct RedRows {
     int rows[kReduceSlots];
 };
+// one workgroup per column; fixed summation order (deterministic)
 __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ red, RedRows rr, double *out)
 {
     __shared__ double part[256];
-    for (int col = 0; col < kReduceSlots; col++) {
-        double s = 0.0;
-        for (int r = threadIdx.x; r < rr.rows[col]; r += blockDim.x) s += red[(size_t)r * kReduceSlots + col];
-        part[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[col] = part[0];
+    const int col = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rr.rows[col]; r += blockDim.x) s += red[(size_t)r * kReduceSlots + col];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[col] = part[0];
 }
 
 int ensure_red_rows(Ctx *c, int rows)
@@ -329,7 +447,7 @@ int reduce_partials(Ctx *c)
 {
     RedRows rr;
     for (int k = 0; k < kReduceSlots; k++) rr.rows[k] = c->red_rows[k];
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, c->red, rr, c->red_out);
+    hipLaunchKernelGGL(k_reduce, dim3(kReduceSlots), dim3(256), 0, c->stream, c->red, rr, c->red_out);
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -407,8 +525,11 @@ static int spme_run(Ctx *c)
     // ewald_general.F90:517-576 charge spreading
     DLPX_HIP(c, hipMemsetAsync(c->Q, 0, sizeof(R) * c->nreal, s));
     switch (sp.S) {
-#define DLPX_SPREAD(SS) \
-    case SS: hipLaunchKernelGGL((k_spread<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, (R *)c->Q); break;
+#define DLPX_SPREAD(SS)                                                                                    \
+    case SS:                                                                                               \
+        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(c->ncell), dim3(256), 0, s, c->pos, c->cell_start, sp, \
+                           (R *)c->Q);                                                                     \
+        break;
         DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
 #undef DLPX_SPREAD
         default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");

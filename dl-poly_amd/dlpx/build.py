@@ -34,6 +34,31 @@ def _compile(src):
     return out
 
 
+FC = os.environ.get("DLPX_FC", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(PKG, "fortran")
+
+
+def build_fortran(verbose=False):
+    """The ISO_C_BINDING module and the Fortran host driver, linked to libdlpx.so."""
+    if not os.path.exists(FC):
+        if verbose:
+            print("no Fortran compiler at %s: Fortran host driver not built" % FC)
+        return None
+    mod = os.path.join(OBJ, "fortran")
+    os.makedirs(mod, exist_ok=True)
+    obj = os.path.join(mod, "dlpx_binding.o")
+    exe = os.path.join(FORTRAN, "dlpx_host_demo")
+    for cmd in ([FC, "-O2", "-c", os.path.join(FORTRAN, "dlpx_binding.F90"), "-module-dir", mod, "-o", obj],
+                [FC, "-O2", "-I", mod, os.path.join(FORTRAN, "dlpx_host_demo.F90"), obj, "-L", HERE, "-ldlpx",
+                 "-Wl,-rpath,$ORIGIN/../dlpx", "-o", exe]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    if verbose:
+        print(exe)
+    return exe
+
+
 def build(jobs=8, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(jobs) as ex:
@@ -45,6 +70,7 @@ def build(jobs=8, verbose=False):
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -174,3 +174,59 @@ def test_nve_energy_conservation_fp64_and_mixed():
         assert np.abs(etot - etot[0]).max() <= tol * abs(etot[0])
         assert nreb > 0
         ctx.close()
+
+
+def _fortran_demo_system():
+    """The configuration dl-poly_amd/fortran/dlpx_host_demo.F90 builds, with the
+    same arithmetic in the same order."""
+    nc = 5
+    a = 5.64 * 1.08
+    L = nc * a
+    basis = [(0.0, 0.0, 0.0), (0.0, 0.5, 0.5), (0.5, 0.0, 0.5), (0.5, 0.5, 0.0)]
+    pos = []
+    for sp in (1, 2):
+        for ix in range(nc):
+            for iy in range(nc):
+                for iz in range(nc):
+                    for b in basis:
+                        x = a * (ix + b[0]) - 0.5 * L
+                        y = a * (iy + b[1]) - 0.5 * L
+                        z = a * (iz + b[2]) - 0.5 * L
+                        if sp == 2:
+                            x = x + 0.5 * a
+                        if x >= 0.5 * L:
+                            x = x - L
+                        pos.append([x, y, z])
+    pos = np.array(pos)
+    for n in range(1, len(pos) + 1):
+        for d in range(1, 4):
+            pos[n - 1, d - 1] += 0.1 * (((37 * n + 11 * d) % 101) / 100.0 - 0.5)
+    s = systems.nacl(ncells=5, kvec=(32, 32, 32))
+    s.pos = pos
+    return s
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fortran_host_through_iso_c_binding(tmp_path, precision):
+    import os
+    import subprocess
+    exe = os.path.join(fixtures.ROOT, "dl-poly_amd", "fortran", "dlpx_host_demo")
+    out = tmp_path / "demo.bin"
+    r = subprocess.run([exe, str(precision), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(out, dtype="<f8")
+    eng, stress, buf, frc = raw[:4], raw[4:13], raw[13:33], raw[33:].reshape(-1, 3)
+    s = _fortran_demo_system()
+    ctx, py = gpu_two_body(s, precision=precision)
+    # same inputs through two hosts; the only difference is the order of the
+    # atomic adds of charge spreading (FP64: ~1e-16, FP32 mixed grid: ~1e-7)
+    tol = 1e-12 if precision == 0 else 2e-6
+    assert np.abs(frc - py["frc"]).max() <= tol * np.abs(py["frc"]).max()
+    assert np.allclose(eng, [py["engcpe"], py["vircpe"], py["engsrp"], py["virsrp"]], rtol=tol)
+    assert np.abs(stress - py["stress"]).max() <= tol * np.abs(py["stress"]).max()
+    assert np.abs(buf - py["buffer"]).max() <= tol * np.abs(py["buffer"]).max()
+    if precision == 0:
+        o = oracle.two_body(s)
+        assert rms_rel(frc, o["frc"]) <= 1e-10
+    ctx.close()

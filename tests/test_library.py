@@ -111,3 +111,10 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(lib, "LIB", "/nonexistent/libdlpx.so")
     with pytest.raises(ImportError):
         lib.load()
+
+
+def test_fortran_binding_builds_and_links():
+    exe = os.path.join(ROOT, "dl-poly_amd", "fortran", "dlpx_host_demo")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libdlpx.so" in out and "not found" not in out
