@@ -150,7 +150,7 @@ static int evaluate(Ctx *c)
     if ((r = upload_tables(c))) return r;
     // one capacity for every stage's partial rows, so no stage reallocates the
     // buffer after an earlier stage of this evaluation has written to it
-    if ((r = ensure_red_rows(c, std::max({pair_blocks(c), ceil_div(c->natms, 64), ceil_div(c->natms, 256), 2048}))))
+    if ((r = ensure_red_rows(c, std::max({pair_blocks(c), gather_blocks(c), conv_blocks(c), ceil_div(c->natms, 256), 2048}))))
         return r;
     for (int k = 0; k < kReduceSlots; k++) c->red_rows[k] = 0;
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -447,7 +447,7 @@ int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs)
     for (int i = 0; i < n; i++)
         for (int g = 0; g < G; g++)
             for (int k = 0; k < std::min(cnt[(size_t)i * G + g], c->cap_lane); k++) {
-                const int j = nbr[((size_t)k * c->cap + i) * G + g];
+                const int j = nbr[list_slot(i, k, g, c->cap_lane)];
                 const int hi = gid[i], hj = gid[j];
                 if (hi >= hj) continue;
                 if (np < cap && pairs) {

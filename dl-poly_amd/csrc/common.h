@@ -18,6 +18,14 @@ constexpr int kLanesPerAtom = 4;                          // lanes sharing one a
 constexpr int kMaxSplines = 16;
 constexpr int kReduceSlots = 24;
 
+// Verlet list layout: atom-major, the G lanes of one atom interleaved, so the
+// entries of one atom (and of one workgroup's atoms) are contiguous in HBM.
+// Lane g of atom i holds its c-th neighbour at this index.
+__host__ __device__ inline size_t list_slot(int i, int c, int lane, int cap_lane)
+{
+    return ((size_t)i * cap_lane + c) * kLanesPerAtom + lane;
+}
+
 // Columns of the per-evaluation partial-sum buffer (FP64, one row per block;
 // each column is written by exactly one kernel, which records its row count
 // in Ctx::red_rows, and is summed in fixed order).
@@ -156,10 +164,21 @@ int spme_setup(Ctx *c);
 int ensure_red_rows(Ctx *c, int rows);
 int reduce_partials(Ctx *c);                 // red -> red_out (device)
 int pair_blocks(const Ctx *c);
+int gather_blocks(const Ctx *c);
 int nve_first(Ctx *c, double dt);
 int nve_second(Ctx *c, double dt, bool kinetic);
-int fft_forward(Ctx *c);
-int fft_inverse(Ctx *c);
+
+// Influence function of the SPME convolution over the half spectrum
+// (x index 0..K1/2), ewald_spole.F90:1296-1380.
+struct InflParams {
+    int K[3], K1h;
+    double rcell[9];
+    double cut2, test_fac, rcf;  // cut_off^2, (1e-6/rcf)^2, pi/alpha
+};
+// Q <- FFT-(theta . FFT+(Q)) in place on the real grid c->Q, with the
+// reciprocal stress partials in reduction slots R_RS11..R_RS33.
+int fft_convolve(Ctx *c, const InflParams &ip);
+int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

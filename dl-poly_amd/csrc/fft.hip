@@ -8,15 +8,19 @@
 // and split into their half spectra (k = 0..K1/2).  The y and z passes then
 // run over the K1/2+1 remaining columns only; the inverse mirrors this and
 // writes the real potential grid back over the charge grid.
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
+#include "device_util.h"
 
 namespace dlpx {
 
 struct FftPlan {
     int n, np;
     int rad[24];
+    int ls[24];        // Ls, the product of the radices of the passes before p
+    float inv_ls[24];  // 1/Ls
 };
 
 static FftPlan make_plan(int n)
@@ -29,7 +33,34 @@ static FftPlan make_plan(int n)
     while (m % 2 == 0) { p.rad[p.np++] = 2; m /= 2; }
     while (m % 3 == 0) { p.rad[p.np++] = 3; m /= 3; }
     while (m % 5 == 0) { p.rad[p.np++] = 5; m /= 5; }
+    int ls = 1;
+    for (int q = 0; q < p.np; q++) {
+        p.ls[q] = ls;
+        p.inv_ls[q] = 1.0f / (float)ls;
+        ls *= p.rad[q];
+    }
     return p;
+}
+
+// x / d and x mod d for 0 <= x < 2^20 from a float reciprocal: the quotient
+// estimate is within one of the truth, and one correction step fixes it.
+struct FastDiv {
+    int d;
+    float inv;
+};
+static FastDiv make_div(int d) { return FastDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int fdivmod(int x, FastDiv f, int &r)
+{
+    int q = (int)((float)x * f.inv);
+    r = x - q * f.d;
+    if (r < 0) {
+        r += f.d;
+        q--;
+    } else if (r >= f.d) {
+        r -= f.d;
+        q++;
+    }
+    return q;
 }
 
 template <typename T> struct C2T;
@@ -101,20 +132,24 @@ __device__ __forceinline__ void dft_small(T2 *x, int sign)
     }
 }
 
-// One Stockham pass of radix R: butterfly j of line ll reads
-// src[j + q m] (m = n/R), twiddles by exp(sign 2 pi i q k/(Ls R)) with
-// k = j mod Ls, and writes dst[(j - k) R + k + s Ls].
+constexpr int kFftThreads = 256;
+
+// One Stockham pass of radix R over one line: butterfly j (0 <= j < n/R)
+// reads src[j + q m] (m = n/R), twiddles input q by exp(sign 2 pi i q k/(Ls R))
+// with k = j mod Ls, and writes dst[(j - k) R + k + s Ls].  The tpl threads
+// of a line take butterflies jt, jt + tpl, ...
 template <typename T, typename T2, int R>
-__device__ __forceinline__ void stockham_pass(const T2 *src, T2 *dst, int nl, int ns, int n, int Ls,
-                                              const T2 *__restrict__ tw, int sign)
+__device__ __forceinline__ void stockham_pass(const T2 *src, T2 *dst, int n, int Ls, float inv_ls,
+                                              const T2 *tw, int sign, int jt, int tpl)
 {
     const int m = n / R, tstride = n / (Ls * R);
-    for (int b = threadIdx.x; b < nl * m; b += blockDim.x) {
-        const int ll = b / m, j = b - ll * m;
-        const int k = j % Ls;
+    const FastDiv fl{Ls, inv_ls};
+    for (int j = jt; j < m; j += tpl) {
+        int k;
+        (void)fdivmod(j, fl, k);
         T2 x[R];
 #pragma unroll
-        for (int q = 0; q < R; q++) x[q] = src[ll * ns + j + q * m];
+        for (int q = 0; q < R; q++) x[q] = src[j + q * m];
 #pragma unroll
         for (int q = 1; q < R; q++) {
             T2 w = tw[q * k * tstride];
@@ -124,198 +159,348 @@ __device__ __forceinline__ void stockham_pass(const T2 *src, T2 *dst, int nl, in
         dft_small<T, T2, R>(x, sign);
         const int base = (j - k) * R + k;
 #pragma unroll
-        for (int s = 0; s < R; s++) dst[ll * ns + base + s * Ls] = x[s];
+        for (int s = 0; s < R; s++) dst[base + s * Ls] = x[s];
     }
 }
 
-// Stockham passes over nl lines of length n stored at buf0[ll*ns + e]; the
-// result ends in the returned buffer.
+// All passes of the plan on the lines of a workgroup (line ll at
+// buf[ll * (n + 1)]); each thread works on line lw only.  The result ends in
+// the returned buffer.  Every thread of the workgroup must call this.
 template <typename T, typename T2>
-__device__ T2 *stockham(T2 *buf0, T2 *buf1, int nl, int ns, const FftPlan &pl, const T2 *__restrict__ tw, int sign)
+__device__ T2 *stockham(T2 *buf0, T2 *buf1, const FftPlan &pl, const T2 *tw, int sign, int lw, bool active, int jt,
+                        int tpl)
 {
-    const int n = pl.n;
+    const int n = pl.n, off = lw * (n + 1);
     T2 *src = buf0, *dst = buf1;
-    int Ls = 1;
     for (int p = 0; p < pl.np; p++) {
-        const int r = pl.rad[p];
-        if (r == 4) stockham_pass<T, T2, 4>(src, dst, nl, ns, n, Ls, tw, sign);
-        else if (r == 2) stockham_pass<T, T2, 2>(src, dst, nl, ns, n, Ls, tw, sign);
-        else if (r == 3) stockham_pass<T, T2, 3>(src, dst, nl, ns, n, Ls, tw, sign);
-        else stockham_pass<T, T2, 5>(src, dst, nl, ns, n, Ls, tw, sign);
+        if (active) {
+            const int r = pl.rad[p];
+            if (r == 4) stockham_pass<T, T2, 4>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            else if (r == 2) stockham_pass<T, T2, 2>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            else if (r == 3) stockham_pass<T, T2, 3>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            else stockham_pass<T, T2, 5>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+        }
         __syncthreads();
         T2 *t = src;
         src = dst;
         dst = t;
-        Ls *= r;
     }
     return src;
 }
 
+// Strided lines of the half-spectrum array: line L starts at
+// (L % na) * sa + (L / na) * sb and its elements are se apart.
 struct LineMap {
-    int n;         // transform length
     int nl;        // number of lines
-    int lpb;       // lines per block
-    long se;       // element stride
-    long sa, sb;   // line L -> (L % na) * sa + (L / na) * sb
     int na;
+    long sa, sb;
+    long se;
 };
 
-// Complex-to-complex transforms of strided lines, in place.
-template <typename T>
-__global__ void __launch_bounds__(128) k_fft_c2c(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
-                                                 const typename C2T<T>::type *__restrict__ tw, int sign)
+template <typename T2>
+__device__ __forceinline__ size_t line_base(const LineMap &lm, int L)
 {
-    using T2 = typename C2T<T>::type;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int n = lm.n, ns = n + 1;
-    T2 *b0 = (T2 *)smem, *b1 = b0 + lm.lpb * ns;
-    const int L0 = blockIdx.x * lm.lpb;
-    const int nl = min(lm.lpb, lm.nl - L0);
-    for (int idx = threadIdx.x; idx < lm.lpb * n; idx += blockDim.x) {
-        const int ll = idx % lm.lpb, e = idx / lm.lpb;
-        if (ll < nl) {
-            const int L = L0 + ll;
-            b0[ll * ns + e] = data[(L % lm.na) * lm.sa + (long)(L / lm.na) * lm.sb + e * lm.se];
-        }
-    }
-    __syncthreads();
-    T2 *res = stockham<T, T2>(b0, b1, nl, ns, pl, tw, sign);
-    for (int idx = threadIdx.x; idx < lm.lpb * n; idx += blockDim.x) {
-        const int ll = idx % lm.lpb, e = idx / lm.lpb;
-        if (ll < nl) {
-            const int L = L0 + ll;
-            data[(L % lm.na) * lm.sa + (long)(L / lm.na) * lm.sb + e * lm.se] = res[ll * ns + e];
-        }
-    }
+    return (size_t)(L % lm.na) * lm.sa + (size_t)(L / lm.na) * lm.sb;
 }
 
-// x axis, forward: real lines 2P and 2P+1 of Q (K1 contiguous values each)
-// packed as one complex line; output half spectra to H (K1h per line).
-template <typename T>
-__global__ void __launch_bounds__(128) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H, int nreal_lines,
-                                                   int K1h, int lpb, FftPlan pl,
-                                                   const typename C2T<T>::type *__restrict__ tw)
+// LPB strided lines per workgroup: thread t moves elements of line t % LPB,
+// so the LPB lines (adjacent x columns) are read and written as contiguous
+// runs of LPB elements.
+template <typename T2, int LPB>
+__device__ __forceinline__ void load_lines(const T2 *data, const LineMap &lm, int n, int L0, int nl, T2 *b0)
+{
+    const int ll = threadIdx.x % LPB;
+    if (ll >= nl) return;
+    const size_t base = line_base<T2>(lm, L0 + ll);
+    for (int e = threadIdx.x / LPB; e < n; e += kFftThreads / LPB) b0[ll * (n + 1) + e] = data[base + (size_t)e * lm.se];
+}
+
+template <typename T2, int LPB>
+__device__ __forceinline__ void store_lines(T2 *data, const LineMap &lm, int n, int L0, int nl, const T2 *res)
+{
+    const int ll = threadIdx.x % LPB;
+    if (ll >= nl) return;
+    const size_t base = line_base<T2>(lm, L0 + ll);
+    for (int e = threadIdx.x / LPB; e < n; e += kFftThreads / LPB) data[base + (size_t)e * lm.se] = res[ll * (n + 1) + e];
+}
+
+template <typename T2>
+__device__ __forceinline__ void load_twiddles(const T2 *tw, int n, T2 *stw)
+{
+    for (int k = threadIdx.x; k < n; k += kFftThreads) stw[k] = tw[k];
+}
+
+// Complex-to-complex transforms of strided lines (the y axis), in place.
+template <typename T, int LPB>
+__global__ void __launch_bounds__(kFftThreads) k_fft_c2c(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
+                                                         const typename C2T<T>::type *__restrict__ tw, int sign)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = pl.n, ns = n + 1;
-    T2 *b0 = (T2 *)smem, *b1 = b0 + lpb * ns;
-    const int npairs = (nreal_lines + 1) / 2;
-    const int P0 = blockIdx.x * lpb;
-    const int np = min(lpb, npairs - P0);
-    for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
-        const int ll = idx / n, e = idx - ll * n;
-        if (ll < np) {
-            const int a = 2 * (P0 + ll), b = a + 1;
-            T2 z;
-            z.x = Q[(size_t)a * n + e];
-            z.y = b < nreal_lines ? Q[(size_t)b * n + e] : (T)0;
-            b0[ll * ns + e] = z;
+    T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
+    const int L0 = blockIdx.x * LPB, nl = min(LPB, lm.nl - L0);
+    load_twiddles(tw, n, stw);
+    load_lines<T2, LPB>(data, lm, n, L0, nl, b0);
+    __syncthreads();
+    constexpr int TPL = kFftThreads / LPB;
+    const int lw = threadIdx.x / TPL;
+    T2 *res = stockham<T, T2>(b0, b1, pl, stw, sign, lw, lw < nl, threadIdx.x % TPL, TPL);
+    store_lines<T2, LPB>(data, lm, n, L0, nl, res);
+}
+
+// The z axis with the influence function in between: forward transform of
+// each z line, multiply by theta(m) (ewald_spole.F90:1296-1380) while summing
+// the reciprocal stress, inverse transform, all in LDS.  Line L is the column
+// (kx, ky) = (L % K1h, L / K1h) of the half spectrum.
+template <typename T, int LPB>
+__global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
+                                                            const typename C2T<T>::type *__restrict__ tw,
+                                                            InflParams ip, const double *__restrict__ n2,
+                                                            double *__restrict__ red)
+{
+    using T2 = typename C2T<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *lred = (double *)smem;  // 6 x 4 wavefronts
+    const int n = pl.n, ns = n + 1;
+    T2 *b0 = (T2 *)(smem + 6 * 4 * sizeof(double)), *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
+    const int L0 = blockIdx.x * LPB, nl = min(LPB, lm.nl - L0);
+    load_twiddles(tw, n, stw);
+    load_lines<T2, LPB>(data, lm, n, L0, nl, b0);
+    __syncthreads();
+    constexpr int TPL = kFftThreads / LPB;
+    const int lw = threadIdx.x / TPL, jt = threadIdx.x % TPL;
+    const bool active = lw < nl;
+    T2 *res = stockham<T, T2>(b0, b1, pl, stw, +1, lw, active, jt, TPL);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (active) {
+        const double sqrpi = 1.7724538509055159;
+        const int L = L0 + lw;
+        const int j = L % ip.K1h, k = L / ip.K1h;
+        const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
+        const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
+        const double bxy = n2y[k] * n2x[j];
+        const double wgt = (j == 0 || 2 * j == ip.K[0]) ? 1.0 : 2.0;  // Hermitian partner
+        for (int l = jt; l < n; l += TPL) {
+            const int ll = (2 * l > ip.K[2]) ? l - ip.K[2] : l;
+            const double r3[3] = {ll * ip.rcell[2], ll * ip.rcell[5], ll * ip.rcell[8]};
+            const double r2[3] = {r3[0] + kk * ip.rcell[1], r3[1] + kk * ip.rcell[4], r3[2] + kk * ip.rcell[7]};
+            const double rp[3] = {r2[0] + j * ip.rcell[0], r2[1] + j * ip.rcell[3], r2[2] + j * ip.rcell[6]};
+            const double bb1 = n2z[l] * bxy;
+            const double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+            T2 h = res[lw * ns + l];
+            if (k2 <= ip.cut2 && k2 > ip.test_fac) {
+                const double fp = sqrt(k2) * ip.rcf;
+                const double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
+                const double hr = (double)h.x, hi = (double)h.y;
+                const double pv = wgt * fac * (hr * hr + hi * hi) * (-2.0 * ((1.0 + fp * fp) / k2));
+                acc[0] += rp[0] * rp[0] * pv;
+                acc[1] += rp[0] * rp[1] * pv;
+                acc[2] += rp[0] * rp[2] * pv;
+                acc[3] += rp[1] * rp[1] * pv;
+                acc[4] += rp[1] * rp[2] * pv;
+                acc[5] += rp[2] * rp[2] * pv;
+                h.x = (T)(fac * hr);
+                h.y = (T)(fac * hi);
+            } else {
+                h.x = (T)0;
+                h.y = (T)0;
+            }
+            res[lw * ns + l] = h;
         }
     }
     __syncthreads();
-    T2 *res = stockham<T, T2>(b0, b1, np, ns, pl, tw, +1);
-    for (int idx = threadIdx.x; idx < lpb * K1h; idx += blockDim.x) {
-        const int ll = idx / K1h, k = idx - ll * K1h;
-        if (ll < np) {
-            const int a = 2 * (P0 + ll), b = a + 1;
-            const T2 z = res[ll * ns + k];
-            const T2 zc = res[ll * ns + (k == 0 ? 0 : n - k)];
-            T2 A, B;
-            A.x = (T)0.5 * (z.x + zc.x);
-            A.y = (T)0.5 * (z.y - zc.y);
-            B.x = (T)0.5 * (z.y + zc.y);
-            B.y = (T)-0.5 * (z.x - zc.x);
-            H[(size_t)a * K1h + k] = A;
-            if (b < nreal_lines) H[(size_t)b * K1h + k] = B;
-        }
+    T2 *res2 = stockham<T, T2>(res, res == b0 ? b1 : b0, pl, stw, -1, lw, active, jt, TPL);
+    store_lines<T2, LPB>(data, lm, n, L0, nl, res2);
+    block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
+}
+
+// x axis, forward: real lines 2P and 2P+1 of Q (n contiguous values each)
+// packed as one complex line; output half spectra to H (K1h per line).
+template <typename T, int LPB>
+__global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H,
+                                                           int nreal_lines, int K1h, FftPlan pl,
+                                                           const typename C2T<T>::type *__restrict__ tw,
+                                                           FastDiv div_n, FastDiv div_h)
+{
+    using T2 = typename C2T<T>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = pl.n, ns = n + 1;
+    T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
+    const int npairs = (nreal_lines + 1) / 2;
+    const int P0 = blockIdx.x * LPB;
+    const int np = min(LPB, npairs - P0);
+    load_twiddles(tw, n, stw);
+    for (int idx = threadIdx.x; idx < np * n; idx += kFftThreads) {
+        int e;
+        const int ll = fdivmod(idx, div_n, e);
+        const int a = 2 * (P0 + ll), b = a + 1;
+        T2 z;
+        z.x = Q[(size_t)a * n + e];
+        z.y = b < nreal_lines ? Q[(size_t)b * n + e] : (T)0;
+        b0[ll * ns + e] = z;
+    }
+    __syncthreads();
+    constexpr int TPL = kFftThreads / LPB;
+    const int lw = threadIdx.x / TPL;
+    T2 *res = stockham<T, T2>(b0, b1, pl, stw, +1, lw, lw < np, threadIdx.x % TPL, TPL);
+    for (int idx = threadIdx.x; idx < np * K1h; idx += kFftThreads) {
+        int k;
+        const int ll = fdivmod(idx, div_h, k);
+        const int a = 2 * (P0 + ll), b = a + 1;
+        const T2 z = res[ll * ns + k];
+        const T2 zc = res[ll * ns + (k == 0 ? 0 : n - k)];
+        T2 A, B;
+        A.x = (T)0.5 * (z.x + zc.x);
+        A.y = (T)0.5 * (z.y - zc.y);
+        B.x = (T)0.5 * (z.y + zc.y);
+        B.y = (T)-0.5 * (z.x - zc.x);
+        H[(size_t)a * K1h + k] = A;
+        if (b < nreal_lines) H[(size_t)b * K1h + k] = B;
     }
 }
 
 // x axis, inverse: rebuild the full Hermitian line of two half spectra,
 // transform with sign -1 and write the real parts to Q.
-template <typename T>
-__global__ void __launch_bounds__(128) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
-                                                   int nreal_lines, int K1h, int lpb, FftPlan pl,
-                                                   const typename C2T<T>::type *__restrict__ tw)
+template <typename T, int LPB>
+__global__ void __launch_bounds__(kFftThreads) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
+                                                           int nreal_lines, int K1h, FftPlan pl,
+                                                           const typename C2T<T>::type *__restrict__ tw,
+                                                           FastDiv div_n)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = pl.n, ns = n + 1;
-    T2 *b0 = (T2 *)smem, *b1 = b0 + lpb * ns;
+    T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
     const int npairs = (nreal_lines + 1) / 2;
-    const int P0 = blockIdx.x * lpb;
-    const int np = min(lpb, npairs - P0);
-    for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
-        const int ll = idx / n, k = idx - ll * n;
-        if (ll < np) {
-            const int a = 2 * (P0 + ll), b = a + 1;
-            const bool hb = b < nreal_lines;
-            T2 z;
-            if (k < K1h) {
-                const T2 A = H[(size_t)a * K1h + k];
-                T2 B;
-                B.x = hb ? H[(size_t)b * K1h + k].x : (T)0;
-                B.y = hb ? H[(size_t)b * K1h + k].y : (T)0;
-                z.x = A.x - B.y;
-                z.y = A.y + B.x;
-            } else {
-                const int km = n - k;
-                const T2 A = H[(size_t)a * K1h + km];
-                T2 B;
-                B.x = hb ? H[(size_t)b * K1h + km].x : (T)0;
-                B.y = hb ? H[(size_t)b * K1h + km].y : (T)0;
-                z.x = A.x + B.y;
-                z.y = -A.y + B.x;
-            }
-            b0[ll * ns + k] = z;
+    const int P0 = blockIdx.x * LPB;
+    const int np = min(LPB, npairs - P0);
+    load_twiddles(tw, n, stw);
+    for (int idx = threadIdx.x; idx < np * n; idx += kFftThreads) {
+        int k;
+        const int ll = fdivmod(idx, div_n, k);
+        const int a = 2 * (P0 + ll), b = a + 1;
+        const bool hb = b < nreal_lines;
+        const int kh = k < K1h ? k : n - k;
+        const T2 A = H[(size_t)a * K1h + kh];
+        T2 B;
+        B.x = hb ? H[(size_t)b * K1h + kh].x : (T)0;
+        B.y = hb ? H[(size_t)b * K1h + kh].y : (T)0;
+        T2 z;
+        if (k < K1h) {
+            z.x = A.x - B.y;
+            z.y = A.y + B.x;
+        } else {
+            z.x = A.x + B.y;
+            z.y = -A.y + B.x;
         }
+        b0[ll * ns + k] = z;
     }
     __syncthreads();
-    T2 *res = stockham<T, T2>(b0, b1, np, ns, pl, tw, -1);
-    for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
-        const int ll = idx / n, e = idx - ll * n;
-        if (ll < np) {
-            const int a = 2 * (P0 + ll), b = a + 1;
-            const T2 z = res[ll * ns + e];
-            Q[(size_t)a * n + e] = z.x;
-            if (b < nreal_lines) Q[(size_t)b * n + e] = z.y;
-        }
+    constexpr int TPL = kFftThreads / LPB;
+    const int lw = threadIdx.x / TPL;
+    T2 *res = stockham<T, T2>(b0, b1, pl, stw, -1, lw, lw < np, threadIdx.x % TPL, TPL);
+    for (int idx = threadIdx.x; idx < np * n; idx += kFftThreads) {
+        int e;
+        const int ll = fdivmod(idx, div_n, e);
+        const int a = 2 * (P0 + ll), b = a + 1;
+        const T2 z = res[ll * ns + e];
+        Q[(size_t)a * n + e] = z.x;
+        if (b < nreal_lines) Q[(size_t)b * n + e] = z.y;
     }
 }
 
-constexpr int kFftThreads = 128;
-
+// Lines per workgroup: the largest of 16, 8, 4, 2, 1 whose two Stockham
+// buffers and twiddles fit 72 KB of LDS (two workgroups per CU); 16 lines give
+// 128 B (FP32) or 256 B (FP64) contiguous runs in the y and z passes.
 template <typename T>
-static int lines_per_block(int n)
+static size_t fft_lds(int lpb, int n, size_t extra)
 {
-    return std::max(1, 1024 / (n + 1));
+    return sizeof(typename C2T<T>::type) * ((size_t)2 * lpb * (n + 1) + n) + extra;
 }
 
 template <typename T>
-static size_t lds_bytes(int lpb, int n)
+static int pick_lpb(int n, size_t extra)
 {
-    return 2 * sizeof(typename C2T<T>::type) * (size_t)lpb * (n + 1);
+    for (int lpb = 16; lpb >= 1; lpb /= 2)
+        if (fft_lds<T>(lpb, n, extra) <= 72 * 1024) return lpb;
+    return fft_lds<T>(1, n, extra) <= 160 * 1024 ? 1 : 0;
 }
 
+#define DLPX_LPB_SWITCH(lpb, CALL) \
+    switch (lpb) {                 \
+        case 16: CALL(16); break;  \
+        case 8: CALL(8); break;    \
+        case 4: CALL(4); break;    \
+        case 2: CALL(2); break;    \
+        default: CALL(1); break;   \
+    }
+
 template <typename T>
-static int run_c2c(Ctx *c, int axis, int sign)
+static int run_y(Ctx *c, int sign)
 {
     using T2 = typename C2T<T>::type;
-    LineMap lm;
     const int K1h = c->K1h, K2 = c->K[1], K3 = c->K[2];
-    if (axis == 1) {
-        lm.n = K2; lm.nl = K1h * K3; lm.se = K1h; lm.na = K1h; lm.sa = 1; lm.sb = (long)K1h * K2;
-    } else {
-        lm.n = K3; lm.nl = K1h * K2; lm.se = (long)K1h * K2; lm.na = K1h * K2; lm.sa = 1; lm.sb = 0;
-    }
-    lm.lpb = lines_per_block<T>(lm.n);
-    const FftPlan pl = make_plan(lm.n);
-    const T2 *tw = (const T2 *)c->tw + (axis == 1 ? c->K[0] : c->K[0] + c->K[1]);
-    const size_t lds = lds_bytes<T>(lm.lpb, lm.n);
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_c2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_fft_c2c<T>, dim3(ceil_div(lm.nl, lm.lpb)), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw,
-                       sign);
+    LineMap lm;
+    lm.nl = K1h * K3;
+    lm.na = K1h;
+    lm.sa = 1;
+    lm.sb = (long)K1h * K2;
+    lm.se = K1h;
+    const int lpb = pick_lpb<T>(K2, 0);
+    if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    const FftPlan pl = make_plan(K2);
+    const T2 *tw = (const T2 *)c->tw + c->K[0];
+    const size_t lds = fft_lds<T>(lpb, K2, 0);
+    const int nb = ceil_div(lm.nl, lpb);
+#define DLPX_Y(L)                                                                                                   \
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_c2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                    (int)lds));                                                                     \
+    hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw, sign)
+    DLPX_LPB_SWITCH(lpb, DLPX_Y)
+#undef DLPX_Y
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+static LineMap z_lines(const Ctx *c)
+{
+    LineMap lm;
+    lm.nl = c->K1h * c->K[1];
+    lm.na = c->K1h * c->K[1];
+    lm.sa = 1;
+    lm.sb = 0;
+    lm.se = (long)c->K1h * c->K[1];
+    return lm;
+}
+
+template <typename T>
+static int z_lpb(const Ctx *c)
+{
+    return pick_lpb<T>(c->K[2], 6 * 4 * sizeof(double));
+}
+
+template <typename T>
+static int run_z_conv(Ctx *c, const InflParams &ip)
+{
+    using T2 = typename C2T<T>::type;
+    const int K3 = c->K[2];
+    const LineMap lm = z_lines(c);
+    const size_t extra = 6 * 4 * sizeof(double);
+    const int lpb = z_lpb<T>(c);
+    if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    const FftPlan pl = make_plan(K3);
+    const T2 *tw = (const T2 *)c->tw + c->K[0] + c->K[1];
+    const size_t lds = fft_lds<T>(lpb, K3, extra);
+    const int nb = ceil_div(lm.nl, lpb);
+#define DLPX_Z(L)                                                                                                   \
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_z_conv<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
+                                    (int)lds));                                                                     \
+    hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw,   \
+                       ip, c->bsp_n2, c->red)
+    DLPX_LPB_SWITCH(lpb, DLPX_Z)
+#undef DLPX_Z
+    for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -325,42 +510,54 @@ static int run_x(Ctx *c, bool forward)
 {
     using T2 = typename C2T<T>::type;
     const int n = c->K[0], nlines = c->K[1] * c->K[2];
-    const int lpb = lines_per_block<T>(n);
+    const int lpb = pick_lpb<T>(n, 0);
+    if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
     const int npairs = (nlines + 1) / 2;
     const FftPlan pl = make_plan(n);
-    const size_t lds = lds_bytes<T>(lpb, n);
+    const size_t lds = fft_lds<T>(lpb, n, 0);
+    const int nb = ceil_div(npairs, lpb);
+    const FastDiv dn = make_div(n), dh = make_div(c->K1h);
     if (forward) {
-        DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fft_x_r2c<T>, dim3(ceil_div(npairs, lpb)), dim3(kFftThreads), lds, c->stream, (const T *)c->Q,
-                           (T2 *)c->H, nlines, c->K1h, lpb, pl, (const T2 *)c->tw);
+#define DLPX_R2C(L)                                                                                                 \
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                    (int)lds));                                                                     \
+    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (const T *)c->Q,           \
+                       (T2 *)c->H, nlines, c->K1h, pl, (const T2 *)c->tw, dn, dh)
+        DLPX_LPB_SWITCH(lpb, DLPX_R2C)
+#undef DLPX_R2C
     } else {
-        DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fft_x_c2r<T>, dim3(ceil_div(npairs, lpb)), dim3(kFftThreads), lds, c->stream, (const T2 *)c->H,
-                           (T *)c->Q, nlines, c->K1h, lpb, pl, (const T2 *)c->tw);
+#define DLPX_C2R(L)                                                                                                 \
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
+                                    (int)lds));                                                                     \
+    hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (const T2 *)c->H,          \
+                       (T *)c->Q, nlines, c->K1h, pl, (const T2 *)c->tw, dn)
+        DLPX_LPB_SWITCH(lpb, DLPX_C2R)
+#undef DLPX_C2R
     }
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
 
 template <typename T>
-static int forward_t(Ctx *c)
+static int convolve_t(Ctx *c, const InflParams &ip)
 {
     int r;
     if ((r = run_x<T>(c, true))) return r;
-    if ((r = run_c2c<T>(c, 1, +1))) return r;
-    return run_c2c<T>(c, 2, +1);
-}
-
-template <typename T>
-static int inverse_t(Ctx *c)
-{
-    int r;
-    if ((r = run_c2c<T>(c, 2, -1))) return r;
-    if ((r = run_c2c<T>(c, 1, -1))) return r;
+    if ((r = run_y<T>(c, +1))) return r;
+    if ((r = run_z_conv<T>(c, ip))) return r;
+    if ((r = run_y<T>(c, -1))) return r;
     return run_x<T>(c, false);
 }
 
-int fft_forward(Ctx *c) { return c->prm.precision == DLPX_MIXED ? forward_t<float>(c) : forward_t<double>(c); }
-int fft_inverse(Ctx *c) { return c->prm.precision == DLPX_MIXED ? inverse_t<float>(c) : inverse_t<double>(c); }
+int fft_convolve(Ctx *c, const InflParams &ip)
+{
+    return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip) : convolve_t<double>(c, ip);
+}
+
+int conv_blocks(const Ctx *c)
+{
+    const int lpb = c->prm.precision == DLPX_MIXED ? z_lpb<float>(c) : z_lpb<double>(c);
+    return lpb ? ceil_div((long)c->K1h * c->K[1], lpb) : 0;
+}
 
 }  // namespace dlpx

@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ 
                                                     int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
                                                     int32_t *flags)
 {
+    static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
     __shared__ double sx[kListChunk], sy[kListChunk], sz[kListChunk];
     __shared__ float fx[kListChunk], fy[kListChunk], fz[kListChunk];
     const int c = blockIdx.x;
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ 
                                 }
                             }
                             if (in) {
-                                if (count < cap_lane) nbr[((size_t)count * cap + i) * G + lane] = j;
+                                if (count < cap_lane) nbr[list_slot(i, count, lane, cap_lane)] = j;
                                 count++;
                             }
                         }

@@ -48,8 +48,14 @@ __device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
     e = t1 + (t2 - t1) * ppp * (R)0.5;
 }
 
-template <typename R, int G>
-__global__ void __launch_bounds__(256) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
+// 512-thread workgroups: the LDS tables (about 77 KB in FP64 for three vdW
+// keys plus erfc) are loaded once per workgroup, so larger groups put more
+// wavefronts per CU behind one copy to hide the latency of the neighbour
+// position gathers.
+constexpr int kPairThreads = 512;
+
+template <typename R, int G, int NT>
+__global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
                                               const typename R2T<R>::type *__restrict__ vdw_g,
@@ -82,9 +88,20 @@ __global__ void __launch_bounds__(256) k_pair(const double4 *__restrict__ pos, c
         const int m = min(cnt[(size_t)i * G + lane], pp.cap_lane);
         const R rvdw = (R)pp.rvdw, rc = (R)pp.rc, rdr_v = (R)pp.rdr_v, rdr_e = (R)pp.rdr_e;
         const int32_t *ktrow = kt + ti * pp.ntypes;
+        const int32_t *lst = nbr + list_slot(i, 0, lane, pp.cap_lane);  // entry c at lst[c * G]
+        // the next neighbour's index, position and type are fetched one
+        // iteration ahead so their latency overlaps the current pair
+        int jn = m > 0 ? lst[0] : i;
+        double4 pn = pos[jn];
+        int tn = typ[jn];
         for (int c = 0; c < m; c++) {
-            const int j = nbr[((size_t)c * cap + i) * G + lane];
-            const double4 pj = pos[j];
+            const double4 pj = pn;
+            const int tj = tn;
+            if (c + 1 < m) {
+                jn = lst[(size_t)(c + 1) * G];
+                pn = pos[jn];
+                tn = typ[jn];
+            }
             const double dxd = min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]);
             const double dyd = min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]);
             const double dzd = min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]);
@@ -93,7 +110,7 @@ __global__ void __launch_bounds__(256) k_pair(const double4 *__restrict__ pos, c
             if ((double)rsq >= pp.rmax2) continue;
             const R rrr = sqrt(rsq);
             R gam = (R)0;
-            const int key = ktrow[typ[j]];
+            const int key = ktrow[tj];
             if (key >= 0 && rrr < rvdw) {
                 R gf, ge;
                 interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
@@ -158,13 +175,14 @@ static int launch_pair(Ctx *c)
     pp.scaling = kR4pie0 / c->prm.epsilon;
     pp.cap_lane = c->cap_lane;
     const size_t lds = sizeof(R2) * ((size_t)pp.nkeys * pp.mg1 + (pp.electro ? pp.ns1 : 0)) +
-                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * 4;
+                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * (kPairThreads / 64);
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, G, kPairThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
     const int nthreads = c->natms * G;
-    hipLaunchKernelGGL((k_pair<R, G>), dim3(ceil_div(nthreads, 256)), dim3(256), lds, c->stream, c->pos, c->typ,
+    hipLaunchKernelGGL((k_pair<R, G, kPairThreads>), dim3(ceil_div(nthreads, kPairThreads)), dim3(kPairThreads), lds,
+                       c->stream, c->pos, c->typ,
                        c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw, (const R2 *)c->d_erfc,
                        c->d_keytab, c->frc, c->red);
     DLPX_HIP(c, hipGetLastError());
@@ -177,6 +195,6 @@ int pair_forces(Ctx *c)
     return launch_pair<double>(c);
 }
 
-int pair_blocks(const Ctx *c) { return ceil_div((long)c->natms * kLanesPerAtom, 256); }
+int pair_blocks(const Ctx *c) { return ceil_div((long)c->natms * kLanesPerAtom, kPairThreads); }
 
 }  // namespace dlpx

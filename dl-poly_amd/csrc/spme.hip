@@ -1,14 +1,16 @@
 // SPME reciprocal space (ewald_spole.F90:244-477, ewald_general.F90:517-869,
 // bspline.F90:192-306, ewald_spole.F90:1257-1386) on the GPU.
 //
-//  k_spread     B-splines of 64 atoms per block go to LDS; each wavefront then
-//               scatters one atom's S^3 charge weights, lanes over the (x, y)
-//               plane so one atomic instruction touches S rows of S contiguous
-//               grid points.
-//  k_influence  Q^ <- theta(m) Q^ on the half spectrum with the spherical
-//               cutoff 0.525 min_d K_d w*_d, and the reciprocal stress.
-//  k_gather     same LDS staging; lanes over (x, y) read the potential grid,
-//               wave reduction gives the atom's energy and force.
+//  k_spread_cells  one workgroup per link cell: B-splines of its atoms go to
+//               LDS and each wavefront adds one atom's S^3 charge weights into
+//               an LDS window of the grid (lanes over the (x, y) plane), which
+//               is then added to the global grid.
+//  (fft.hip)    Q <- FFT-(theta . FFT+(Q)); theta(m) on the half spectrum
+//               with the spherical cutoff 0.525 min_d K_d w*_d, and the
+//               reciprocal stress.
+//  k_gather     B-splines of a block's atoms staged in LDS; a group of lanes
+//               per atom (lane = x column) reads the potential grid, a group
+//               reduction gives the atom's energy and force.
 //  k_finish     subtracts the mean reciprocal force (ewald_general.F90:857-864),
 //               scales by 2*scale and adds into the total force.
 // FP64 or FP32 (mixed) arithmetic; sums across lanes and atoms in FP64.
@@ -99,51 +101,6 @@ __device__ __forceinline__ void grid_coord(const SpmeParams &sp, int d, double x
 
 constexpr int kAtomsPerBlock = 64;
 
-template <typename R, int S>
-__global__ void __launch_bounds__(256) k_spread(const double4 *__restrict__ pos, int n, SpmeParams sp,
-                                                R *__restrict__ Q)
-{
-    __shared__ R sM[3][kAtomsPerBlock][S];
-    __shared__ int sI[3][kAtomsPerBlock];
-    __shared__ R sQ[kAtomsPerBlock];
-    const int a0 = blockIdx.x * kAtomsPerBlock;
-    const int t = threadIdx.x;
-    if (t < 3 * kAtomsPerBlock) {
-        const int d = t / kAtomsPerBlock, a = t % kAtomsPerBlock;
-        const int i = a0 + a;
-        if (i < n) {
-            const double4 p = pos[i];
-            int idx;
-            double w;
-            grid_coord(sp, d, p.x, p.y, p.z, idx, w);
-            R M[S];
-            bspline<R, false, S>((R)w, M, nullptr);
-#pragma unroll
-            for (int k = 0; k < S; k++) sM[d][a][k] = M[k];
-            sI[d][a] = idx;
-            if (d == 0) sQ[a] = (R)p.w;
-        }
-    }
-    __syncthreads();
-    const int wid = t >> 6, lane = t & 63;
-    const int na = min(kAtomsPerBlock, n - a0);
-    for (int a = wid; a < na; a += 4) {
-        const R q = sQ[a];
-        if (fabs((double)q) <= kZeroPlus) continue;
-        const int bx = sI[0][a] - S + 1 + sp.K[0], by = sI[1][a] - S + 1 + sp.K[1], bz = sI[2][a] - S + 1 + sp.K[2];
-        for (int p = lane; p < S * S; p += 64) {
-            const int tx = p % S, ty = p / S;
-            const int gx = (bx + tx) % sp.K[0], gy = (by + ty) % sp.K[1];
-            const R mxy = sM[0][a][tx] * sM[1][a][ty];
-#pragma unroll
-            for (int tz = 0; tz < S; tz++) {
-                const int gz = (bz + tz) % sp.K[2];
-                unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], q * sM[2][a][tz] * mxy);
-            }
-        }
-    }
-}
-
 // Unwrapped fractional grid coordinate of one axis (no folding): the base
 // grid point floor(u) - S + 1 may be negative or >= K; callers fold indices.
 __device__ __forceinline__ double grid_u(const SpmeParams &sp, int d, double x, double y, double z)
@@ -165,13 +122,17 @@ constexpr int kSlabBytes = 32768;
 // grid with one atomic per window point, instead of S^3 global atomics per
 // atom.  A window too large for LDS (very large cells, or atoms far outside
 // their cell) falls back to per-atom global atomics for that cell.
+//
+// The window accumulates in FP64 at either precision: FP64 LDS atomic adds
+// (ds_add_f64) sustain several times the rate of FP32 ones (ds_add_f32) on
+// gfx950, and the FP32 grid then receives one rounding per window point.
 template <typename R, int S>
 __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict__ pos,
                                                       const int32_t *__restrict__ cell_start, SpmeParams sp,
                                                       R *__restrict__ Q)
 {
-    constexpr int kCap = kSlabBytes / (int)sizeof(R);
-    __shared__ R slab[kCap];
+    constexpr int kCap = kSlabBytes / (int)sizeof(double);
+    __shared__ double slab[kCap];
     __shared__ R sM[3][kAtomsPerBlock][S];
     __shared__ int sB[3][kAtomsPerBlock];
     __shared__ R sQ[kAtomsPerBlock];
@@ -201,7 +162,7 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
     const int Dz = tile ? min(Wz, kCap / (Wx * Wy)) : Wz;
     for (int z0 = 0; z0 < Wz; z0 += Dz) {
         if (tile) {
-            for (int k = t; k < Wx * Wy * Dz; k += blockDim.x) slab[k] = (R)0;
+            for (int k = t; k < Wx * Wy * Dz; k += blockDim.x) slab[k] = 0.0;
         }
         for (int a0 = 0; a0 < n; a0 += kAtomsPerBlock) {
             __syncthreads();
@@ -225,20 +186,19 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
                 const R q = sQ[a];
                 if (fabs((double)q) <= kZeroPlus) continue;
                 const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
+                // spline points of this atom inside the current z-slab
+                const int tz0 = tile ? max(0, z0 + lo2 - bz) : 0;
+                const int tz1 = tile ? min(S, z0 + lo2 + Dz - bz) : S;
+                if (tz0 >= tz1) continue;
                 for (int pp = lane; pp < S * S; pp += 64) {
                     const int tx = pp % S, ty = pp / S;
                     const R mxy = sM[0][a][tx] * sM[1][a][ty];
                     if (tile) {
                         const int lxy = (by + ty - lo1) * Wx + (bx + tx - lo0);
-#pragma unroll
-                        for (int tz = 0; tz < S; tz++) {
-                            const int lz = bz + tz - lo2 - z0;
-                            const R v = q * sM[2][a][tz] * mxy;
-                            // single precision: contributions below 1e-30 (spline tails)
-                            // are dropped rather than added as denormals
-                            if (lz >= 0 && lz < Dz && (sizeof(R) == 8 || fabs(v) >= (R)1e-30))
-                                atomicAdd(&slab[lz * Wx * Wy + lxy], v);
-                        }
+                        const int plane = Wx * Wy;
+                        int l = (bz + tz0 - lo2 - z0) * plane + lxy;  // lz in [0, Dz)
+                        for (int tz = tz0; tz < tz1; tz++, l += plane)
+                            atomicAdd(&slab[l], (double)(q * sM[2][a][tz] * mxy));
                     } else {
                         const int gx = wrapk(bx + tx, sp.K[0]), gy = wrapk(by + ty, sp.K[1]);
 #pragma unroll
@@ -254,7 +214,7 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
         if (!tile) break;  // the direct path did every z at once
         const int dz = min(Dz, Wz - z0);
         for (int k = t; k < Wx * Wy * dz; k += blockDim.x) {
-            const R v = slab[k];
+            const R v = (R)slab[k];
             if (v == (R)0) continue;
             const int lx = k % Wx, ly = (k / Wx) % Wy, lz = k / (Wx * Wy);
             const int gx = wrapk(lo0 + lx, sp.K[0]), gy = wrapk(lo1 + ly, sp.K[1]), gz = wrapk(lo2 + z0 + lz, sp.K[2]);
@@ -263,74 +223,34 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
     }
 }
 
-// Influence function and reciprocal stress over the half spectrum
-// (x index 0..K1/2), ewald_spole.F90:1296-1380.
-struct InflParams {
-    int K[3], K1h;
-    double rcell[9];
-    double cut2, test_fac, rcf;  // cut_off^2, (1e-6/rcf)^2, pi/alpha
-    int nblocks;
-};
+// Lanes per atom in the gather: the next power of two >= S, lane tx < S
+// taking the grid column x = base + tx.
+constexpr int gather_lanes(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : 16; }
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_influence(typename std::conditional<sizeof(R) == 8, double2, float2>::type *H,
-                                                   const double *__restrict__ n2, InflParams ip, double *red)
-{
-    __shared__ double lred[6 * 4];
-    const double sqrpi = 1.7724538509055159;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    const size_t total = (size_t)ip.K1h * ip.K[1] * ip.K[2];
-    const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int j = (int)(e % ip.K1h);
-        const int k = (int)((e / ip.K1h) % ip.K[1]);
-        const int l = (int)(e / ((size_t)ip.K1h * ip.K[1]));
-        const int jj = j;
-        const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
-        const int ll = (2 * l > ip.K[2]) ? l - ip.K[2] : l;
-        double r3[3] = {ll * ip.rcell[2], ll * ip.rcell[5], ll * ip.rcell[8]};
-        double r2[3] = {r3[0] + kk * ip.rcell[1], r3[1] + kk * ip.rcell[4], r3[2] + kk * ip.rcell[7]};
-        double rp[3] = {r2[0] + jj * ip.rcell[0], r2[1] + jj * ip.rcell[3], r2[2] + jj * ip.rcell[6]};
-        const double bb1 = (n2z[l] * n2y[k]) * n2x[j];
-        const double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
-        auto h = H[e];
-        if (k2 <= ip.cut2 && k2 > ip.test_fac) {
-            const double fp = sqrt(k2) * ip.rcf;
-            const double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
-            const double hr = (double)h.x, hi = (double)h.y;
-            const double pv = fac * (hr * hr + hi * hi) * (-2.0 * ((1.0 + fp * fp) / k2));
-            const double wgt = (j == 0 || 2 * j == ip.K[0]) ? 1.0 : 2.0;  // Hermitian partner
-            acc[0] += wgt * rp[0] * rp[0] * pv;
-            acc[1] += wgt * rp[0] * rp[1] * pv;
-            acc[2] += wgt * rp[0] * rp[2] * pv;
-            acc[3] += wgt * rp[1] * rp[1] * pv;
-            acc[4] += wgt * rp[1] * rp[2] * pv;
-            acc[5] += wgt * rp[2] * rp[2] * pv;
-            h.x = (R)(fac * hr);
-            h.y = (R)(fac * hi);
-        } else {
-            h.x = (R)0;
-            h.y = (R)0;
-        }
-        H[e] = h;
-    }
-    block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
-}
-
+// Force interpolation (ewald_general.F90:776-855).  A group of gather_lanes(S)
+// lanes serves one atom: lane tx reads the S x S (y, z) potential values of
+// its grid column, so one wavefront load instruction covers 64/G atoms with
+// runs of S contiguous x points each, and the cross-lane reduction spans only
+// the group.  z and y sums are separable:
+//   E  = sum_y M_y sum_z M_z Phi,  Fy = sum_y dM_y sum_z M_z Phi,
+//   Fz = sum_y M_y sum_z dM_z Phi,
+// then e = M_x E, fx = dM_x E, fy = M_x Fy, fz = M_x Fz per lane.  The z sums
+// run in R, everything after them in FP64.
 template <typename R, int S>
 __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos, int n, SpmeParams sp,
                                                 const R *__restrict__ Phi, double4 *__restrict__ frc_recip,
                                                 double *red)
 {
-    __shared__ R sM[3][kAtomsPerBlock][S];
-    __shared__ R sD[3][kAtomsPerBlock][S];
-    __shared__ int sI[3][kAtomsPerBlock];
-    __shared__ double sQ[kAtomsPerBlock];
+    constexpr int G = gather_lanes(S), APB = 256 / G;
+    __shared__ R sM[3][APB][S];
+    __shared__ R sD[3][APB][S];
+    __shared__ int sB[3][APB];
+    __shared__ double sQ[APB];
     __shared__ double lred[4 * 4];
-    const int a0 = blockIdx.x * kAtomsPerBlock;
+    const int a0 = blockIdx.x * APB;
     const int t = threadIdx.x;
-    if (t < 3 * kAtomsPerBlock) {
-        const int d = t / kAtomsPerBlock, a = t % kAtomsPerBlock;
+    if (t < 3 * APB) {
+        const int d = t / APB, a = t % APB;
         const int i = a0 + a;
         if (i < n) {
             const double4 p = pos[i];
@@ -344,57 +264,73 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
                 sM[d][a][k] = M[k];
                 sD[d][a][k] = D[k];
             }
-            sI[d][a] = idx;
+            // first grid point of the stencil, in [0, K)
+            int b = idx - S + 1;
+            if (b < 0) b += sp.K[d];
+            sB[d][a] = b;
             if (d == 0) sQ[a] = p.w;
         }
     }
     __syncthreads();
-    const int wid = t >> 6, lane = t & 63;
-    const int na = min(kAtomsPerBlock, n - a0);
+    const int a = t / G, tx = t % G, i = a0 + a;
     double acc[4] = {0, 0, 0, 0};  // energy, fx, fy, fz (unscaled, summed over atoms)
-    for (int a = wid; a < na; a += 4) {
-        const double q = sQ[a];
-        double e = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-        if (fabs(q) > kZeroPlus) {
-            const int bx = sI[0][a] - S + 1 + sp.K[0], by = sI[1][a] - S + 1 + sp.K[1], bz = sI[2][a] - S + 1 + sp.K[2];
-            for (int p = lane; p < S * S; p += 64) {
-                const int tx = p % S, ty = p / S;
-                const int gx = (bx + tx) % sp.K[0], gy = (by + ty) % sp.K[1];
-                const R mx = sM[0][a][tx], my = sM[1][a][ty], dx = sD[0][a][tx], dy = sD[1][a][ty];
-                R se = (R)0, sx = (R)0, sy = (R)0, sz = (R)0;
+    double e = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    const double q = i < n ? sQ[a] : 0.0;
+    if (i < n && tx < S && fabs(q) > kZeroPlus) {
+        int gx = sB[0][a] + tx;
+        if (gx >= sp.K[0]) gx -= sp.K[0];
+        const int by = sB[1][a], bz = sB[2][a];
+        const size_t plane = (size_t)sp.K[0] * sp.K[1];
+        // z planes of the stencil, wrapped once
+        size_t zoff[S];
 #pragma unroll
-                for (int tz = 0; tz < S; tz++) {
-                    const int gz = (bz + tz) % sp.K[2];
-                    const R phi = Phi[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx];
-                    const R mz = sM[2][a][tz], dz = sD[2][a][tz];
-                    se += mz * phi;
-                    sz += dz * phi;
-                }
-                // separable in z: sum_z M_z phi and sum_z dM_z phi
-                sx = dx * my * se;
-                sy = mx * dy * se;
-                e += (double)(mx * my * se);
-                fx += (double)sx;
-                fy += (double)sy;
-                fz += (double)(mx * my * sz);
+        for (int tz = 0; tz < S; tz++) {
+            int gz = bz + tz;
+            if (gz >= sp.K[2]) gz -= sp.K[2];
+            zoff[tz] = (size_t)gz * plane + gx;
+        }
+        double E = 0.0, Fy = 0.0, Fz = 0.0;
+        for (int ty = 0; ty < S; ty++) {
+            int gy = by + ty;
+            if (gy >= sp.K[1]) gy -= sp.K[1];
+            const size_t yoff = (size_t)gy * sp.K[0];
+            R se = (R)0, sz = (R)0;
+#pragma unroll
+            for (int tz = 0; tz < S; tz++) {
+                const R phi = Phi[zoff[tz] + yoff];
+                se += sM[2][a][tz] * phi;
+                sz += sD[2][a][tz] * phi;
             }
+            const double my = (double)sM[1][a][ty], dy = (double)sD[1][a][ty];
+            E += my * (double)se;
+            Fy += dy * (double)se;
+            Fz += my * (double)sz;
         }
-        e = wave_sum(e);
-        fx = wave_sum(fx);
-        fy = wave_sum(fy);
-        fz = wave_sum(fz);
-        if (lane == 0) {
-            // ewald_general.F90:800-855: F_i = -q sum dM.M.M.Phi * recip_kmax
-            const double Fx = -q * fx * sp.rk[0], Fy = -q * fy * sp.rk[1], Fz = -q * fz * sp.rk[2];
-            frc_recip[a0 + a] = make_double4(Fx, Fy, Fz, 0.0);
-            acc[0] += q * e;
-            acc[1] += Fx;
-            acc[2] += Fy;
-            acc[3] += Fz;
-        }
+        const double mx = (double)sM[0][a][tx], dx = (double)sD[0][a][tx];
+        e = mx * E;
+        fx = dx * E;
+        fy = mx * Fy;
+        fz = mx * Fz;
     }
-    __syncthreads();
+    e = group_sum<G>(e);
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    if (i < n && tx == 0) {
+        // ewald_general.F90:800-855: F_i = -q sum dM.M.M.Phi * recip_kmax
+        const double Fx_ = -q * fx * sp.rk[0], Fy_ = -q * fy * sp.rk[1], Fz_ = -q * fz * sp.rk[2];
+        frc_recip[i] = make_double4(Fx_, Fy_, Fz_, 0.0);
+        acc[0] = q * e;
+        acc[1] = Fx_;
+        acc[2] = Fy_;
+        acc[3] = Fz_;
+    }
     block_sum<4>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ERECIP);
+}
+
+int gather_blocks(const Ctx *c)
+{
+    return ceil_div(c->natms, 256 / gather_lanes(c->prm.nsplines));
 }
 
 // total force += 2 scale (F_recip - mean F_recip) (ewald_general.F90:857-864,
@@ -519,7 +455,7 @@ static int spme_run(Ctx *c)
     const int n = c->natms;
     hipStream_t s = c->stream;
     const SpmeParams sp = spme_params(c);
-    const int nb = ceil_div(n, kAtomsPerBlock);
+    const int nb = gather_blocks(c);
     int r;
     if ((r = ensure_red_rows(c, nb))) return r;
     // ewald_general.F90:517-576 charge spreading
@@ -535,7 +471,6 @@ static int spme_run(Ctx *c)
         default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
     }
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[2], s));
-    if ((r = fft_forward(c))) return r;
     // ewald_spole.F90:1296-1306 cutoff and convergence factors
     InflParams ip;
     for (int d = 0; d < 3; d++) ip.K[d] = c->K[d];
@@ -548,13 +483,7 @@ static int spme_run(Ctx *c)
     const double m = std::min(std::min((double)c->K[0] * rcp[6], (double)c->K[1] * rcp[7]), (double)c->K[2] * rcp[8]);
     const double cut_off = 0.5 * 1.05 * m;
     ip.cut2 = cut_off * cut_off;
-    const int nbi = std::min(ceil_div((long)c->nhalf, 256), 2048);
-    if ((r = ensure_red_rows(c, nbi))) return r;
-    ip.nblocks = nbi;
-    using R2 = typename std::conditional<sizeof(R) == 8, double2, float2>::type;
-    hipLaunchKernelGGL(k_influence<R>, dim3(nbi), dim3(256), 0, s, (R2 *)c->H, c->bsp_n2, ip, c->red);
-    for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nbi;
-    if ((r = fft_inverse(c))) return r;
+    if ((r = fft_convolve(c, ip))) return r;
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[3], s));
     switch (sp.S) {
 #define DLPX_GATHER(SS)                                                                                     \
