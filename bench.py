@@ -36,6 +36,41 @@ HBM_PEAK = 8.0e12            # B/s
 VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP32
 
 
+# kernels of each stage (mixed precision, spline order 8)
+KERNELS = {"pair": "k_pair<float, 4, 512>", "spread": "k_spread_cells<float, 8>",
+           "fft": "k_fft_x_r2c/k_fft_c2c/k_fft_z_conv/k_fft_x_c2r<float>", "gather": "k_gather<float, 8>"}
+
+
+def traffic_from_profile(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for ln in open(path):
+        f = ln.split("|")
+        if len(f) >= 3 and f[0].strip() == kernel:
+            try:
+                return float(f[-1])
+            except ValueError:
+                return None
+    return None
+
+
+def copy_calibration(mib=256, reps=5):
+    """A plain device-to-device copy of known size (for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc runs)."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "reps": reps}),
+          flush=True)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -163,7 +198,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="nacl64k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the force path on a single MI355X "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
@@ -207,9 +247,8 @@ def main():
     else:
         roof = dict(bound="valu", achieved=flops / t / 1e12, peak=VALU_PEAK[lib.MIXED] / 1e12, unit="TFLOP/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = {"pair": "k_pair<float,4>", "spread": "k_spread<float,8>", "fft": "k_fft_*<float>",
-                      "gather": "k_gather<float,8>"}[dom]
+    roof["kernel"] = KERNELS[dom]
+    roof["traffic"] = traffic_from_profile(args.workload, roof["kernel"])
     roof["stage_fraction_sum_tmin_over_tmeasured"] = tmin_sum / tmeas_sum if tmeas_sum else None
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(s)

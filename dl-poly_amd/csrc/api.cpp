@@ -153,15 +153,33 @@ static int evaluate(Ctx *c)
     if ((r = ensure_red_rows(c, std::max({pair_blocks(c), gather_blocks(c), conv_blocks(c), ceil_div(c->natms, 256), 2048}))))
         return r;
     for (int k = 0; k < kReduceSlots; k++) c->red_rows[k] = 0;
-    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    if ((r = pair_forces(c))) return r;
-    for (int k = R_ENGVDW; k <= R_S33; k++) c->red_rows[k] = pair_blocks(c);
-    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    if (c->prm.electrostatics) {
-        if ((r = spme_forces(c))) return r;  // records ev[2..4] when timing
-    } else if (c->timing) {
-        for (int k = 2; k <= 4; k++) DLPX_HIP(c, hipEventRecord(c->ev[k], c->stream));
+    if (c->timing) {
+        // stages one after another on one stream, each between HIP events
+        c->sp = c->stream;
+        DLPX_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        if ((r = pair_forces(c))) return r;
+        DLPX_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        if (c->prm.electrostatics) {
+            if ((r = spme_forces(c))) return r;  // records ev[2..4]
+        } else {
+            for (int k = 2; k <= 4; k++) DLPX_HIP(c, hipEventRecord(c->ev[k], c->stream));
+        }
+    } else if (c->prm.electrostatics) {
+        // the reciprocal-space chain and the real-space pair kernel read the
+        // same positions and write disjoint outputs (frc_recip and its
+        // reduction columns; frc and its columns): run them concurrently
+        c->sp = c->recip_stream;
+        DLPX_HIP(c, hipEventRecord(c->fork_ev, c->stream));
+        DLPX_HIP(c, hipStreamWaitEvent(c->recip_stream, c->fork_ev, 0));
+        if ((r = spme_forces(c))) return r;
+        if ((r = pair_forces(c))) return r;
+        DLPX_HIP(c, hipEventRecord(c->join_ev, c->recip_stream));
+        DLPX_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+        c->sp = c->stream;
+    } else {
+        if ((r = pair_forces(c))) return r;
     }
+    for (int k = R_ENGVDW; k <= R_S33; k++) c->red_rows[k] = pair_blocks(c);
     if ((r = reduce_partials(c))) return r;
     if (c->prm.electrostatics) {
         const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
@@ -275,6 +293,11 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->recip_stream, hipStreamNonBlocking) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
+        return bail(DLPX_ERR_HIP);
+    c->sp = c->stream;
     for (int k = 0; k < 8; k++)
         if (hipEventCreate(&c->ev[k]) != hipSuccess) return bail(DLPX_ERR_HIP);
     const size_t n = c->cap;
@@ -287,7 +310,9 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         (r = dalloc(c, &c->red_out, kReduceSlots)))
         return bail(r);
     if (hipMemset(c->vel, 0, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
-    if (hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocDefault) != hipSuccess) return bail(DLPX_ERR_HIP);
+    // host words the GPU writes directly (the displacement maximum of each step)
+    if (hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipHostGetDevicePointer((void **)&c->h_flags_dev, c->h_flags, 0) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (hipHostMalloc((void **)&c->h_red, sizeof(double) * kReduceSlots, hipHostMallocDefault) != hipSuccess)
         return bail(DLPX_ERR_HIP);
     if (hipHostMalloc((void **)&c->h_st, sizeof(double) * 9 * n, hipHostMallocDefault) != hipSuccess)
@@ -304,6 +329,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     if (!ctx) return 0;
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->recip_stream) (void)hipStreamSynchronize(c->recip_stream);
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
@@ -316,6 +342,9 @@ int dlpx_destroy(dlpx_ctx *ctx)
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (int k = 0; k < 8; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->recip_stream) (void)hipStreamDestroy(c->recip_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -533,12 +562,16 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
         DLPX_HIP(c, hipMalloc((void **)&c->energies_dev, sizeof(double) * kReduceSlots * (size_t)nsteps));
         c->energies_cap = nsteps;
     }
+    // displacement maximum and workgroup counter of the fused stage-1 kernel
+    DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
     for (int s = 0; s < nsteps; s++) {
         // nve.F90:127-193 stage 1, refresh_mappings/vnl_check, forces, stage 2 (drivers.F90:2105-2199)
         if ((r = nve_first(c, tstep))) return r;
-        int update = 0;
-        if (c->prm.padding <= 0.0) update = 1;
-        else if ((r = vnl_check_dev(c, &update))) return r;
+        int update = 1;
+        if (c->prm.padding > 0.0) {
+            DLPX_HIP(c, hipStreamSynchronize(c->stream));
+            update = nve_first_displacement(c) >= kHalfMinus * c->prm.padding ? 1 : 0;
+        }
         if (update) {
             if ((r = build_list(c))) return r;
             rebuilds++;

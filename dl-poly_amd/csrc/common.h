@@ -18,6 +18,20 @@ constexpr int kLanesPerAtom = 4;                          // lanes sharing one a
 constexpr int kMaxSplines = 16;
 constexpr int kReduceSlots = 24;
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of an
+// MI355X (workgroup b on XCD b % 8), each with its own L2.  Kernels whose
+// workgroups read spatially local data (atoms in cell order, grid windows)
+// renumber them so that each XCD takes one contiguous range of the work:
+// launch xcd_grid(nb) workgroups and use xcd_block(blockIdx.x, gridDim.x) as
+// the logical index (values >= nb do no work).  The mapping is a bijection
+// whatever the dispatch order, so only the cache locality depends on it.
+constexpr int kXcds = 8;
+inline int xcd_grid(int nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_block(int b, int nb_launched)
+{
+    return (b % kXcds) * (nb_launched / kXcds) + b / kXcds;
+}
+
 // Verlet list layout: atom-major, the G lanes of one atom interleaved, so the
 // entries of one atom (and of one workgroup's atoms) are contiguous in HBM.
 // Lane g of atom i holds its c-th neighbour at this index.
@@ -48,6 +62,12 @@ struct Ctx {
     dlpx_params prm{};
     std::string err;
     hipStream_t stream = nullptr;
+    // SPME runs on recip_stream, concurrently with the pair kernel on stream,
+    // between a fork and a join event; sp is the stream the SPME kernels of
+    // the current evaluation use (stream itself when stages are timed).
+    hipStream_t recip_stream = nullptr;
+    hipStream_t sp = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int device = 0;
 
     // box (orthorhombic)
@@ -97,8 +117,11 @@ struct Ctx {
     int cap_lane = 0;
     int32_t *nbr = nullptr;
     int32_t *cnt = nullptr;          // cap*G per-lane counts
-    int32_t *flags = nullptr;        // [0] overflow lane max, [1] ibig, [2] update flag, [3] pairs
-    int32_t *h_flags = nullptr;      // pinned mirror
+    int32_t *flags = nullptr;        // [0] overflow lane max, [1] ibig, [3] pairs, [4..5] displacement
+                                     // maximum (double bits), [6] workgroups done
+    int32_t *h_flags = nullptr;      // pinned, mapped host words: [0..3] list flags, [4..5] vnl_check,
+                                     // [8..9] displacement maximum of the last md step
+    int32_t *h_flags_dev = nullptr;  // device address of h_flags
 
     // tables on device
     void *d_vdw = nullptr;           // nkeys x (max_grid+1) x {frc, pot}, FP64 or FP32
@@ -165,7 +188,8 @@ int ensure_red_rows(Ctx *c, int rows);
 int reduce_partials(Ctx *c);                 // red -> red_out (device)
 int pair_blocks(const Ctx *c);
 int gather_blocks(const Ctx *c);
-int nve_first(Ctx *c, double dt);
+int nve_first(Ctx *c, double dt);            // also the vnl_check displacement (flags[4..6])
+double nve_first_displacement(Ctx *c);       // after the stream has reached nve_first
 int nve_second(Ctx *c, double dt, bool kinetic);
 
 // Influence function of the SPME convolution over the half spectrum

@@ -456,7 +456,7 @@ static int run_y(Ctx *c, int sign)
 #define DLPX_Y(L)                                                                                                   \
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_c2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                     (int)lds));                                                                     \
-    hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw, sign)
+    hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw, sign)
     DLPX_LPB_SWITCH(lpb, DLPX_Y)
 #undef DLPX_Y
     DLPX_HIP(c, hipGetLastError());
@@ -496,7 +496,7 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
 #define DLPX_Z(L)                                                                                                   \
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_z_conv<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                                     (int)lds));                                                                     \
-    hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (T2 *)c->H, lm, pl, tw,   \
+    hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw,   \
                        ip, c->bsp_n2, c->red)
     DLPX_LPB_SWITCH(lpb, DLPX_Z)
 #undef DLPX_Z
@@ -521,7 +521,7 @@ static int run_x(Ctx *c, bool forward)
 #define DLPX_R2C(L)                                                                                                 \
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                     (int)lds));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (const T *)c->Q,           \
+    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)c->Q,           \
                        (T2 *)c->H, nlines, c->K1h, pl, (const T2 *)c->tw, dn, dh)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
@@ -529,7 +529,7 @@ static int run_x(Ctx *c, bool forward)
 #define DLPX_C2R(L)                                                                                                 \
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                     (int)lds));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->stream, (const T2 *)c->H,          \
+    hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T2 *)c->H,          \
                        (T *)c->Q, nlines, c->K1h, pl, (const T2 *)c->tw, dn)
         DLPX_LPB_SWITCH(lpb, DLPX_C2R)
 #undef DLPX_C2R

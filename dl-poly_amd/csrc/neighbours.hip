@@ -189,7 +189,8 @@ __global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ 
     static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
     __shared__ double sx[kListChunk], sy[kListChunk], sz[kListChunk];
     __shared__ float fx[kListChunk], fy[kListChunk], fz[kListChunk];
-    const int c = blockIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x);
+    if (c >= g.nc[0] * g.nc[1] * g.nc[2]) return;
     const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
     const int hs = start[c], he = start[c + 1];
     const int lo[3] = {g.nc[0] >= 3 ? -1 : 0, g.nc[1] >= 3 ? -1 : 0, g.nc[2] >= 3 ? -1 : 0};
@@ -368,7 +369,7 @@ int build_list(Ctx *c)
         }
         DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_build_list<G>, dim3(ncell), dim3(256), 0, s, c->pos, c->cap, g, c->cell_start,
+        hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(256), 0, s, c->pos, c->cap, g, c->cell_start,
                            rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));

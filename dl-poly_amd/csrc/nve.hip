@@ -1,28 +1,73 @@
 // Velocity-Verlet NVE integration on the device (nve.F90:127-215) and the
 // host-order <-> cell-order transfers of per-atom data.
+#include <cmath>
+#include <cstring>
+
 #include "common.h"
 #include "device_util.h"
 
 namespace dlpx {
 
-// Stage 1: v += dt/2 F/m ; x += dt v  (atoms of mass <= 1e-6 stay frozen)
-__global__ void k_vv1(double4 *pos, double4 *vel, const double4 *frc, int n, double dt)
+// Stage 1: v += dt/2 F/m ; x += dt v  (atoms of mass <= 1e-6 stay frozen),
+// fused with the displacement test of vnl_check (neighbours.F90:150-178):
+// the largest minimum-image |x - x_ref|^2 goes to *dmax (as the bit pattern
+// of a non-negative double), and the last workgroup to finish copies it to
+// host memory and resets *dmax and *done for the next step.
+struct Box {
+    double L[3], invL[3];
+};
+// 1024-thread workgroups: the maximum is reduced in LDS first, so only one
+// atomic per workgroup (63 at 64k atoms) meets the single device word.
+constexpr int kVv1Threads = 1024;
+__global__ void __launch_bounds__(kVv1Threads) k_vv1(double4 *pos, double4 *vel, const double4 *frc, int n,
+                                                     double dt, const double4 *__restrict__ xref, Box b,
+                                                     unsigned long long *dmax, unsigned int *done,
+                                                     unsigned long long *host_dmax)
 {
+    __shared__ double wmax[kVv1Threads / 64];
+    __shared__ bool last;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 v = vel[i];
-    if (v.w <= 1.0e-6) return;
-    const double4 f = frc[i];
-    const double tmp = (0.5 * dt) / v.w;
-    v.x += tmp * f.x;
-    v.y += tmp * f.y;
-    v.z += tmp * f.z;
-    double4 p = pos[i];
-    p.x += dt * v.x;
-    p.y += dt * v.y;
-    p.z += dt * v.z;
-    vel[i] = v;
-    pos[i] = p;
+    double r2 = 0.0;
+    if (i < n) {
+        double4 v = vel[i];
+        double4 p = pos[i];
+        if (v.w > 1.0e-6) {
+            const double4 f = frc[i];
+            const double tmp = (0.5 * dt) / v.w;
+            v.x += tmp * f.x;
+            v.y += tmp * f.y;
+            v.z += tmp * f.z;
+            p.x += dt * v.x;
+            p.y += dt * v.y;
+            p.z += dt * v.z;
+            vel[i] = v;
+            pos[i] = p;
+        }
+        const double4 q = xref[i];
+        const double dx = min_image_1(p.x, q.x, b.L[0], b.invL[0]);
+        const double dy = min_image_1(p.y, q.y, b.L[1], b.invL[1]);
+        const double dz = min_image_1(p.z, q.z, b.L[2], b.invL[2]);
+        r2 = dx * dx + dy * dy + dz * dz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wmax[w]);
+        atomicMax(dmax, (unsigned long long)__double_as_longlong(m));
+        __threadfence();
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        *host_dmax = atomicAdd(dmax, 0ull);
+        __threadfence_system();
+        *dmax = 0ull;
+        *done = 0u;
+    }
 }
 
 // Stage 2: v += dt/2 F/m, and sum m v^2 per block
@@ -83,10 +128,26 @@ __global__ void k_store(double *__restrict__ st, int n, const int32_t *__restric
 
 int nve_first(Ctx *c, double dt)
 {
-    hipLaunchKernelGGL(k_vv1, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, c->pos, c->vel, c->frc,
-                       c->natms, dt);
+    Box b;
+    for (int d = 0; d < 3; d++) {
+        b.L[d] = c->L[d];
+        b.invL[d] = c->invL[d];
+    }
+    // device words flags[4..5] (maximum) and flags[6] (workgroup count) are
+    // zero between launches; the result lands in host word h_flags[8..9]
+    hipLaunchKernelGGL(k_vv1, dim3(ceil_div(c->natms, kVv1Threads)), dim3(kVv1Threads), 0, c->stream, c->pos,
+                       c->vel, c->frc,
+                       c->natms, dt, c->xref, b, (unsigned long long *)(c->flags + 4), (unsigned int *)(c->flags + 6),
+                       (unsigned long long *)(c->h_flags_dev + 8));
     DLPX_HIP(c, hipGetLastError());
     return 0;
+}
+
+double nve_first_displacement(Ctx *c)
+{
+    double r2;
+    memcpy(&r2, c->h_flags + 8, sizeof(r2));
+    return std::sqrt(r2);
 }
 
 int nve_second(Ctx *c, double dt, bool /*kinetic*/)

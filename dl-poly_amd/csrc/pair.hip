@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = keytab_g[k];
     __syncthreads();
 
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = tid / G, lane = tid % G;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // engvdw, virvdw, engrl, virrl, s11..s33
@@ -180,8 +180,7 @@ static int launch_pair(Ctx *c)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, G, kPairThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-    const int nthreads = c->natms * G;
-    hipLaunchKernelGGL((k_pair<R, G, kPairThreads>), dim3(ceil_div(nthreads, kPairThreads)), dim3(kPairThreads), lds,
+    hipLaunchKernelGGL((k_pair<R, G, kPairThreads>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,
                        c->stream, c->pos, c->typ,
                        c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw, (const R2 *)c->d_erfc,
                        c->d_keytab, c->frc, c->red);
@@ -195,6 +194,6 @@ int pair_forces(Ctx *c)
     return launch_pair<double>(c);
 }
 
-int pair_blocks(const Ctx *c) { return ceil_div((long)c->natms * kLanesPerAtom, kPairThreads); }
+int pair_blocks(const Ctx *c) { return xcd_grid(ceil_div((long)c->natms * kLanesPerAtom, kPairThreads)); }
 
 }  // namespace dlpx

@@ -128,8 +128,8 @@ constexpr int kSlabBytes = 32768;
 // gfx950, and the FP32 grid then receives one rounding per window point.
 template <typename R, int S>
 __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict__ pos,
-                                                      const int32_t *__restrict__ cell_start, SpmeParams sp,
-                                                      R *__restrict__ Q)
+                                                      const int32_t *__restrict__ cell_start, int ncell,
+                                                      SpmeParams sp, R *__restrict__ Q)
 {
     constexpr int kCap = kSlabBytes / (int)sizeof(double);
     __shared__ double slab[kCap];
@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
     __shared__ int sB[3][kAtomsPerBlock];
     __shared__ R sQ[kAtomsPerBlock];
     __shared__ int wlo[3], whi[3];
-    const int c = blockIdx.x, t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    const int c = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    if (c >= ncell) return;
     const int s0 = cell_start[c], n = cell_start[c + 1] - s0;
     if (n <= 0) return;
     if (t < 3) {
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
     __shared__ int sB[3][APB];
     __shared__ double sQ[APB];
     __shared__ double lred[4 * 4];
-    const int a0 = blockIdx.x * APB;
+    const int a0 = xcd_block(blockIdx.x, gridDim.x) * APB;
     const int t = threadIdx.x;
     if (t < 3 * APB) {
         const int d = t / APB, a = t % APB;
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
 
 int gather_blocks(const Ctx *c)
 {
-    return ceil_div(c->natms, 256 / gather_lanes(c->prm.nsplines));
+    return xcd_grid(ceil_div(c->natms, 256 / gather_lanes(c->prm.nsplines)));
 }
 
 // total force += 2 scale (F_recip - mean F_recip) (ewald_general.F90:857-864,
@@ -453,18 +454,17 @@ template <typename R>
 static int spme_run(Ctx *c)
 {
     const int n = c->natms;
-    hipStream_t s = c->stream;
+    hipStream_t s = c->sp;  // the evaluation's reciprocal-space stream
     const SpmeParams sp = spme_params(c);
-    const int nb = gather_blocks(c);
+    const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
     int r;
-    if ((r = ensure_red_rows(c, nb))) return r;
     // ewald_general.F90:517-576 charge spreading
     DLPX_HIP(c, hipMemsetAsync(c->Q, 0, sizeof(R) * c->nreal, s));
     switch (sp.S) {
 #define DLPX_SPREAD(SS)                                                                                    \
     case SS:                                                                                               \
-        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(c->ncell), dim3(256), 0, s, c->pos, c->cell_start, sp, \
-                           (R *)c->Q);                                                                     \
+        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(256), 0, s, c->pos,      \
+                           c->cell_start, c->ncell, sp, (R *)c->Q);                                        \
         break;
         DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
 #undef DLPX_SPREAD
