@@ -136,6 +136,21 @@ def run(s, precision, steps, warmup):
     return dict(seconds=t1 - t0, rebuilds=nreb, stage_ms=ms)
 
 
+def nve_drift(s, precision, steps):
+    """Total-energy drift of a resident NVE run from the bench configuration:
+    least-squares slope of E_tot = engcpe + engsrp + kinetic, relative to
+    |E_tot(0)|, per ns; and the largest relative excursion from E_tot(0)."""
+    ctx = lib.Context(s, precision=precision)
+    ctx.md_load(s.pos, s.vel)
+    e, _ = ctx.md_run(steps, s.timestep)
+    ctx.close()
+    etot = e[:, 0] + e[:, 2] + e[:, 4]
+    t_ps = np.arange(1, steps + 1) * s.timestep
+    slope = np.polyfit(t_ps, etot, 1)[0]  # energy units per ps
+    return dict(steps=steps, ps=steps * s.timestep, drift_rel_per_ns=float(slope * 1000.0 / abs(etot[0])),
+                max_rel_excursion=float(np.abs(etot - etot[0]).max() / abs(etot[0])))
+
+
 def force_error(s):
     """Mixed-precision forces/energies against the FP64 path on the starting configuration."""
     res = {}
@@ -198,6 +213,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="nacl64k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="NVE steps of the energy-drift runs (0 skips them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -251,6 +268,10 @@ def main():
     roof["traffic"] = traffic_from_profile(args.workload, roof["kernel"])
     roof["stage_fraction_sum_tmin_over_tmeasured"] = tmin_sum / tmeas_sum if tmeas_sum else None
 
+    drift = None
+    if args.drift_steps > 0:
+        drift = {"mixed": nve_drift(s, lib.MIXED, args.drift_steps), "fp64": nve_drift(s, lib.FP64, args.drift_steps)}
+        log("nve drift:", drift)
     cpu = None if args.no_cpu_baseline else cpu_baseline(s)
     out = {
         "metric": METRIC,
@@ -275,6 +296,7 @@ def main():
         "stages_mixed": stages,
         "stages_fp64_ms": {nm: runs[lib.FP64]["stage_ms"][k] / max(runs[lib.FP64]["stage_ms"][7], 1.0)
                            for k, nm in enumerate(names)},
+        "nve_drift": drift,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -2,6 +2,7 @@
 // link_cell_pairs / vnl_check / two_body_forces / resident-MD entry points.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -298,6 +299,12 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
         return bail(DLPX_ERR_HIP);
     c->sp = c->stream;
+    // lanes per atom of the pair kernel; DLPX_PAIR_LANES (4, 8, 16) overrides
+    c->pair_lanes = p->precision == DLPX_MIXED ? 8 : 4;
+    if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
+        const int v = std::atoi(e);
+        if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
+    }
     for (int k = 0; k < 8; k++)
         if (hipEventCreate(&c->ev[k]) != hipSuccess) return bail(DLPX_ERR_HIP);
     const size_t n = c->cap;
@@ -576,7 +583,10 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
             if ((r = build_list(c))) return r;
             rebuilds++;
         }
-        if ((r = evaluate(c))) return r;
+        c->resident_step = true;
+        r = evaluate(c);
+        c->resident_step = false;
+        if (r) return r;
         if ((r = nve_second(c, tstep, true))) return r;
         if (c->timing && (r = collect_timing(c))) return r;
         if (energies) {

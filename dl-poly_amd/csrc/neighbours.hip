@@ -5,8 +5,8 @@
 // permuted into that order, so the pair and SPME kernels read neighbours that
 // sit close in HBM.  The list is a full list (each pair stored from both ends)
 // split over G lanes per atom: lane g keeps every G-th candidate it accepts,
-// element (c, i, g) lives at nbr[(c*cap + i)*G + g], so the G lanes of the
-// pair kernel read 64 consecutive ints per wavefront.
+// element (i, c, g) lives at nbr[(i*cap_lane + c)*G + g] (list_slot), so one
+// atom's entries, and one workgroup's atoms, are contiguous.
 #include <cmath>
 #include <cstring>
 
@@ -167,101 +167,137 @@ __global__ void k_permute(const int32_t *perm, int n, const double4 *pos, double
     if (vel) vel_new[i] = vel[o];
 }
 
-// One workgroup per link cell: its atoms are the i atoms (256/G per pass, G
-// lanes each); the positions of every stencil cell are staged through LDS in
-// chunks, and lane g tests the chunk atoms j = g, g+G, ... (chunks are a
-// multiple of G long, so lane g always meets j = start + g mod G).
+// One workgroup per link cell.  The atoms of the (up to) 27 stencil cells
+// are staged once in LDS as float4 {x + sx, y + sy, z + sz, j}, with the
+// cell's periodic image shift s L already added, and then every atom of the
+// cell (G lanes each, lane g taking candidates k = g, g + G, ...) tests them.
 //
-// Positions are folded into [-L/2, L/2) at the rebuild, so the image shift of
-// a stencil cell is -L, 0 or +L per axis and is taken once per cell.  The
-// separation is x_i - (x_j + s L) and the criterion r^2 <= (cutoff+padding)^2
-// (neighbours.F90:991-992).  A single-precision distance settles every pair
-// that is clearly inside or outside; pairs within a relative 1e-5 of the
-// cutoff are decided in double precision without fused multiply-add, so the
-// accepted set is reproducible bit for bit on the host.
-constexpr int kListChunk = 256;
+// Positions are folded into [-L/2, L/2) at the rebuild, so a stencil cell's
+// image shift is -L, 0 or +L per axis.  The separation is x_i - (x_j + s L)
+// and the criterion r^2 <= (cutoff+padding)^2 (neighbours.F90:991-992).  A
+// single-precision distance settles every pair that is clearly inside or
+// outside; pairs within a relative 2e-5 of the cutoff are decided in double
+// precision without fused multiply-add from the FP64 positions, with the
+// arithmetic of the host, so the accepted set is reproducible bit for bit.
+// With fewer than 3 cells on an axis the stencil shift need not be the
+// minimum image, and every candidate is decided exactly by minimum image.
+//
+// Stencils with more than kListStage candidates are processed in rounds.
+constexpr int kListThreads = 512;
+constexpr int kListStage = 3072;  // 48 KB of float4
 template <int G>
-__global__ void __launch_bounds__(256) k_build_list(const double4 *__restrict__ pos, int cap, CellGrid g,
-                                                    const int32_t *__restrict__ start, double rc2, int cap_lane,
-                                                    int32_t *__restrict__ nbr, int32_t *__restrict__ cnt,
-                                                    int32_t *flags)
+__global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__restrict__ pos, int cap, CellGrid g,
+                                                             const int32_t *__restrict__ start, double rc2,
+                                                             int cap_lane, int32_t *__restrict__ nbr,
+                                                             int32_t *__restrict__ cnt, int32_t *flags)
 {
     static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
-    __shared__ double sx[kListChunk], sy[kListChunk], sz[kListChunk];
-    __shared__ float fx[kListChunk], fy[kListChunk], fz[kListChunk];
+    __shared__ float4 cand[kListStage];
+    __shared__ int s_start[27], s_off[28];
+    __shared__ double s_sh[27][3];
     const int c = xcd_block(blockIdx.x, gridDim.x);
     if (c >= g.nc[0] * g.nc[1] * g.nc[2]) return;
+    const int t = threadIdx.x;
     const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
-    const int hs = start[c], he = start[c + 1];
-    const int lo[3] = {g.nc[0] >= 3 ? -1 : 0, g.nc[1] >= 3 ? -1 : 0, g.nc[2] >= 3 ? -1 : 0};
-    const int hi[3] = {g.nc[0] >= 2 ? 1 : 0, g.nc[1] >= 2 ? 1 : 0, g.nc[2] >= 2 ? 1 : 0};
-    const float rc2_lo = (float)(rc2 * (1.0 - 2.0e-5)), rc2_hi = (float)(rc2 * (1.0 + 2.0e-5));
     const bool small = g.nc[0] < 3 || g.nc[1] < 3 || g.nc[2] < 3;
-    for (int h0 = hs; h0 < he; h0 += 256 / G) {
-        const int i = h0 + (int)threadIdx.x / G, lane = threadIdx.x % G;
+    if (t < 27) {
+        // stencil cell t: offsets -1..1 per axis, fewer when an axis has 1-2 cells
+        const int o[3] = {t % 3 - 1, (t / 3) % 3 - 1, t / 9 - 1};
+        const int cc[3] = {cx, cy, cz};
+        bool valid = true;
+        int jj[3];
+        for (int d = 0; d < 3; d++) {
+            const int lo = g.nc[d] >= 3 ? -1 : 0, hi = g.nc[d] >= 2 ? 1 : 0;
+            valid = valid && o[d] >= lo && o[d] <= hi;
+            jj[d] = (cc[d] + o[d] + g.nc[d]) % g.nc[d];
+            s_sh[t][d] = (double)((cc[d] + o[d]) - jj[d]) / g.nc[d] * g.L[d];
+        }
+        const int jc = (jj[2] * g.nc[1] + jj[1]) * g.nc[0] + jj[0];
+        s_start[t] = start[jc];
+        s_off[t + 1] = valid ? start[jc + 1] - start[jc] : 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        s_off[0] = 0;
+        for (int k = 0; k < 27; k++) s_off[k + 1] += s_off[k];
+    }
+    __syncthreads();
+    const int ncand = s_off[27];
+    const int hs = start[c], he = start[c + 1];
+    const float rc2_lo = (float)(rc2 * (1.0 - 2.0e-5)), rc2_hi = (float)(rc2 * (1.0 + 2.0e-5));
+    for (int h0 = hs; h0 < he; h0 += kListThreads / G) {
+        const int i = h0 + t / G, lane = t % G;
         const bool active = i < he;
         const double4 pi = active ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
         const float pfx = (float)pi.x, pfy = (float)pi.y, pfz = (float)pi.z;
         int count = 0;
-        for (int oz = lo[2]; oz <= hi[2]; oz++) {
-            const int jz = (cz + oz + g.nc[2]) % g.nc[2];
-            for (int oy = lo[1]; oy <= hi[1]; oy++) {
-                const int jy = (cy + oy + g.nc[1]) % g.nc[1];
-                for (int ox = lo[0]; ox <= hi[0]; ox++) {
-                    const int jx = (cx + ox + g.nc[0]) % g.nc[0];
-                    const int jc = (jz * g.nc[1] + jy) * g.nc[0] + jx;
-                    // image shift of the stencil cell (folded positions)
-                    const double shx = (double)((cx + ox) - jx) / g.nc[0] * g.L[0];
-                    const double shy = (double)((cy + oy) - jy) / g.nc[1] * g.L[1];
-                    const double shz = (double)((cz + oz) - jz) / g.nc[2] * g.L[2];
-                    const int js = start[jc], je = start[jc + 1];
-                    for (int j0 = js; j0 < je; j0 += kListChunk) {
-                        const int m = min(kListChunk, je - j0);
-                        __syncthreads();
-                        for (int k = threadIdx.x; k < m; k += blockDim.x) {
-                            const double4 p = pos[j0 + k];
-                            sx[k] = p.x;
-                            sy[k] = p.y;
-                            sz[k] = p.z;
-                            fx[k] = (float)(p.x + shx);
-                            fy[k] = (float)(p.y + shy);
-                            fz[k] = (float)(p.z + shz);
+        for (int r0 = 0; r0 < ncand; r0 += kListStage) {
+            const int m = min(kListStage, ncand - r0);
+            // stage candidates r0 .. r0+m-1 (skipped when a single round is
+            // already in LDS from the previous pass of atoms)
+            if (!(r0 == 0 && m == ncand && h0 > hs)) {
+                __syncthreads();
+                constexpr int U = 6;
+                for (int k0 = t; k0 < m; k0 += U * kListThreads) {
+                    double4 pk[U];
+                    int jk[U], tk[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int k = k0 + u * kListThreads;
+                        int q = 0;
+                        if (k < m) {
+                            const int kk = r0 + k;
+                            while (s_off[q + 1] <= kk) q++;
+                            jk[u] = s_start[q] + (kk - s_off[q]);
+                            pk[u] = pos[jk[u]];
                         }
-                        __syncthreads();
-                        if (!active) continue;
-                        for (int k = lane; k < m; k += G) {
-                            const int j = j0 + k;
-                            if (j == i) continue;
-                            bool in;
-                            if (small) {
-                                // 1-2 cells per axis: the stencil shift need not be the
-                                // minimum image, so every candidate is decided exactly
-                                const double dx = min_image_1(pi.x, sx[k], g.L[0], g.invL[0]);
-                                const double dy = min_image_1(pi.y, sy[k], g.L[1], g.invL[1]);
-                                const double dz = min_image_1(pi.z, sz[k], g.L[2], g.invL[2]);
-                                const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
-                                                             __dmul_rn(dz, dz));
-                                in = rsq <= rc2;
-                            } else {
-                                const float ex = pfx - fx[k], ey = pfy - fy[k], ez = pfz - fz[k];
-                                const float r2f = ex * ex + ey * ey + ez * ez;
-                                if (r2f > rc2_hi) continue;
-                                in = r2f < rc2_lo;
-                                if (!in) {
-                                    const double dx = __dsub_rn(pi.x, __dadd_rn(sx[k], shx));
-                                    const double dy = __dsub_rn(pi.y, __dadd_rn(sy[k], shy));
-                                    const double dz = __dsub_rn(pi.z, __dadd_rn(sz[k], shz));
-                                    const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
-                                                                 __dmul_rn(dz, dz));
-                                    in = rsq <= rc2;
-                                }
-                            }
-                            if (in) {
-                                if (count < cap_lane) nbr[list_slot(i, count, lane, cap_lane)] = j;
-                                count++;
-                            }
+                        tk[u] = q;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int k = k0 + u * kListThreads;
+                        if (k < m) {
+                            const int q = tk[u];
+                            cand[k] = make_float4((float)(pk[u].x + s_sh[q][0]), (float)(pk[u].y + s_sh[q][1]),
+                                                  (float)(pk[u].z + s_sh[q][2]), __int_as_float(jk[u]));
                         }
                     }
+                }
+                __syncthreads();
+            }
+            if (!active) continue;
+            for (int k = lane; k < m; k += G) {
+                const float4 e = cand[k];
+                const int j = __float_as_int(e.w);
+                if (j == i) continue;
+                bool in;
+                if (small) {
+                    const double4 pj = pos[j];
+                    const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
+                    const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
+                    const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
+                    const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+                    in = rsq <= rc2;
+                } else {
+                    const float ex = pfx - e.x, ey = pfy - e.y, ez = pfz - e.z;
+                    const float r2f = ex * ex + ey * ey + ez * ez;
+                    if (r2f > rc2_hi) continue;
+                    in = r2f < rc2_lo;
+                    if (!in) {
+                        // x_i - (x_j + s L) with the stencil's s, which for a pair this
+                        // close (well under L/2) is the minimum image
+                        const double4 pj = pos[j];
+                        const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
+                        const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
+                        const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
+                        const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                                     __dmul_rn(dz, dz));
+                        in = rsq <= rc2;
+                    }
+                }
+                if (in) {
+                    if (count < cap_lane) nbr[list_slot(i, count, lane, cap_lane)] = j;
+                    count++;
                 }
             }
         }
@@ -369,7 +405,8 @@ int build_list(Ctx *c)
         }
         DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(256), 0, s, c->pos, c->cap, g, c->cell_start,
+        hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, c->cap, g,
+                           c->cell_start,
                            rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));

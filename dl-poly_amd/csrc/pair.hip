@@ -29,6 +29,22 @@ struct PairParams {
 
 // 3-point Newton interpolation of DL_POLY (numerics.F90:250-290) on the
 // .x (force) and .y (energy) members of consecutive samples.
+// r and 1/r^2 from r^2: FP64 with IEEE square root and division; FP32 with
+// the hardware reciprocal square root (about one ulp), well inside the mixed
+// path's error budget.
+__device__ __forceinline__ void r_rinv2(double rsq, double &r, double &rinv2)
+{
+    r = sqrt(rsq);
+    const double ri = 1.0 / r;
+    rinv2 = ri * ri;
+}
+__device__ __forceinline__ void r_rinv2(float rsq, float &r, float &rinv2)
+{
+    const float ri = __builtin_amdgcn_rsqf(rsq);
+    r = rsq * ri;
+    rinv2 = ri * ri;
+}
+
 template <typename R, typename R2>
 __device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
 {
@@ -54,7 +70,11 @@ __device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
 // position gathers.
 constexpr int kPairThreads = 512;
 
-template <typename R, int G, int NT>
+// GP lanes serve one atom (a multiple of the list's kLanesPerAtom lanes):
+// pair lane p walks list lane p % kLanesPerAtom, entries c = p / kLanesPerAtom,
+// c + GP / kLanesPerAtom, ...  More lanes per atom give more wavefronts for
+// the same atoms (latency hiding on small systems) at the same total work.
+template <typename R, int GP, int NT>
 __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
@@ -76,8 +96,10 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = keytab_g[k];
     __syncthreads();
 
+    constexpr int G = kLanesPerAtom, CS = GP / G;
+    static_assert(GP % G == 0 && GP <= 64, "GP must be a multiple of the list lanes");
     const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int i = tid / G, lane = tid % G;
+    const int i = tid / GP, p = tid % GP, lane = p % G, c0 = p / G;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // engvdw, virvdw, engrl, virrl, s11..s33
     if (i < n) {
@@ -89,33 +111,36 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
         const R rvdw = (R)pp.rvdw, rc = (R)pp.rc, rdr_v = (R)pp.rdr_v, rdr_e = (R)pp.rdr_e;
         const int32_t *ktrow = kt + ti * pp.ntypes;
         const int32_t *lst = nbr + list_slot(i, 0, lane, pp.cap_lane);  // entry c at lst[c * G]
-        // the next neighbour's index, position and type are fetched one
-        // iteration ahead so their latency overlaps the current pair
-        int jn = m > 0 ? lst[0] : i;
+        const R rmax2 = (R)pp.rmax2;
+        // software pipeline: the neighbour index is loaded two iterations
+        // ahead and its position and type one iteration ahead, so neither
+        // load's latency sits in front of the pair arithmetic
+        int jn = c0 < m ? lst[(size_t)c0 * G] : i;
         double4 pn = pos[jn];
         int tn = typ[jn];
-        for (int c = 0; c < m; c++) {
+        jn = c0 + CS < m ? lst[(size_t)(c0 + CS) * G] : i;
+        for (int c = c0; c < m; c += CS) {
             const double4 pj = pn;
             const int tj = tn;
-            if (c + 1 < m) {
-                jn = lst[(size_t)(c + 1) * G];
+            if (c + CS < m) {
                 pn = pos[jn];
                 tn = typ[jn];
+                if (c + 2 * CS < m) jn = lst[(size_t)(c + 2 * CS) * G];
             }
             const double dxd = min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]);
             const double dyd = min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]);
             const double dzd = min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]);
             const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
             const R rsq = dx * dx + dy * dy + dz * dz;
-            if ((double)rsq >= pp.rmax2) continue;
-            const R rrr = sqrt(rsq);
+            if (rsq >= rmax2) continue;
+            R rrr, rinv2;
+            r_rinv2(rsq, rrr, rinv2);
             R gam = (R)0;
             const int key = ktrow[tj];
             if (key >= 0 && rrr < rvdw) {
                 R gf, ge;
                 interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
-                const R rinv = (R)1 / rrr;
-                const R g = gf * (rinv * rinv);
+                const R g = gf * rinv2;
                 gam = g;
                 acc[0] += (double)ge;
                 acc[1] -= (double)(g * (rrr * rrr));
@@ -141,10 +166,10 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
             acc[9] += (double)(dz * fzp);
         }
     }
-    fx = group_sum<G>(fx);
-    fy = group_sum<G>(fy);
-    fz = group_sum<G>(fz);
-    if (i < n && lane == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
+    fx = group_sum<GP>(fx);
+    fy = group_sum<GP>(fy);
+    fz = group_sum<GP>(fz);
+    if (i < n && p == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] *= 0.5;  // every pair is met twice
     __syncthreads();
@@ -155,7 +180,6 @@ template <typename R>
 static int launch_pair(Ctx *c)
 {
     using R2 = typename R2T<R>::type;
-    constexpr int G = kLanesPerAtom;
     PairParams pp;
     pp.ntypes = c->prm.ntypes;
     pp.nkeys = c->nkeys;
@@ -178,12 +202,18 @@ static int launch_pair(Ctx *c)
                        sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * (kPairThreads / 64);
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, G, kPairThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
-    hipLaunchKernelGGL((k_pair<R, G, kPairThreads>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,
-                       c->stream, c->pos, c->typ,
-                       c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw, (const R2 *)c->d_erfc,
-                       c->d_keytab, c->frc, c->red);
+#define DLPX_PAIR(GP)                                                                                        \
+    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, GP, kPairThreads>,                                  \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                     \
+    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads>), dim3(pair_blocks(c)), dim3(kPairThreads), lds, c->stream, \
+                       c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,             \
+                       (const R2 *)c->d_erfc, c->d_keytab, c->frc, c->red)
+    switch (c->pair_lanes) {
+        case 16: DLPX_PAIR(16); break;
+        case 8: DLPX_PAIR(8); break;
+        default: DLPX_PAIR(4); break;
+    }
+#undef DLPX_PAIR
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -194,6 +224,6 @@ int pair_forces(Ctx *c)
     return launch_pair<double>(c);
 }
 
-int pair_blocks(const Ctx *c) { return xcd_grid(ceil_div((long)c->natms * kLanesPerAtom, kPairThreads)); }
+int pair_blocks(const Ctx *c) { return xcd_grid(ceil_div((long)c->natms * c->pair_lanes, kPairThreads)); }
 
 }  // namespace dlpx

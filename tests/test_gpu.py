@@ -163,6 +163,56 @@ def test_full_size_nacl_fp64_and_mixed(nacl64k):
     ctx.close()
 
 
+def _resident_forces_vs_oracle(s, steps, precision):
+    """Forces of the last resident step against the oracle at the positions
+    that step ended with (resident steps use their own spreading kernel)."""
+    ctx = lib.Context(s, precision=precision)
+    ctx.md_load(s.pos, s.vel)
+    ctx.md_run(steps, s.timestep)
+    x, v, f = ctx.md_store()
+    ctx.close()
+    o = oracle.two_body(s, pos=x)
+    return rms_rel(f, o["frc"])
+
+
+@pytest.mark.parametrize("name", ["nacl_4096_k48", "nacl_1440_orth_q"])
+def test_resident_step_forces_match_oracle(name):
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    assert _resident_forces_vs_oracle(s, 3, lib.FP64) <= 1e-10
+    assert _resident_forces_vs_oracle(s, 3, lib.MIXED) <= 1e-5
+
+
+def test_resident_step_forces_full_size(nacl64k):
+    s, _ = nacl64k
+    print("case nacl 64000 K=128 resident", flush=True)
+    assert _resident_forces_vs_oracle(s, 2, lib.FP64) <= 1e-10
+
+
+def test_full_size_1728k_against_strided_oracle_fixture():
+    """1,728,000-ion NaCl, mesh 360^3: FP64 forces of every 1000th atom and
+    the energies against the oracle (tests/golden/large), mixed against FP64."""
+    import os
+    z = np.load(os.path.join(fixtures.GOLDEN, "large", "nacl_1728k_strided.npz"))
+    s = systems.nacl(ncells=60, kvec=(360, 360, 360), padding=0.5)
+    assert np.allclose([s.pos.sum(), (s.pos ** 2).sum()], z["pos_checksum"], rtol=1e-13, atol=0)
+    print("case nacl 1728000 K=360", flush=True)
+    idx = z["idx"]
+    ctx, r = gpu_two_body(s)
+    err = np.sqrt(((r["frc"][idx] - z["frc"]) ** 2).mean()) / float(z["frc_rms"])
+    assert err <= 1e-10
+    e = [r["engcpe"], r["vircpe"], r["engsrp"], r["virsrp"]]
+    assert np.all(np.abs(np.array(e) - z["energies"]) <= 1e-11 * np.abs(z["energies"]))
+    assert np.abs(r["stress"] - z["stress"]).max() <= 1e-11 * np.abs(z["stress"]).max()
+    f64 = r["frc"]
+    assert np.abs(f64.sum(0)).max() <= 1e-8 * np.abs(f64).max() * np.sqrt(s.natms)
+    ctx.close()
+    ctx, rm = gpu_two_body(s, precision=lib.MIXED)
+    assert rms_rel(rm["frc"], f64) <= 1e-5
+    check_energies(rm, r, 1e-6)
+    ctx.close()
+
+
 def test_nve_energy_conservation_fp64_and_mixed():
     s = systems.nacl(ncells=8, kvec=(48, 48, 48), seed=3)
     for prec, tol in ((lib.FP64, 2e-5), (lib.MIXED, 5e-5)):
