@@ -300,7 +300,7 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         return bail(DLPX_ERR_HIP);
     c->sp = c->stream;
     // lanes per atom of the pair kernel; DLPX_PAIR_LANES (4, 8, 16) overrides
-    c->pair_lanes = p->precision == DLPX_MIXED ? 8 : 4;
+    c->pair_lanes = 8;
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
@@ -583,10 +583,7 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
             if ((r = build_list(c))) return r;
             rebuilds++;
         }
-        c->resident_step = true;
-        r = evaluate(c);
-        c->resident_step = false;
-        if (r) return r;
+        if ((r = evaluate(c))) return r;
         if ((r = nve_second(c, tstep, true))) return r;
         if (c->timing && (r = collect_timing(c))) return r;
         if (energies) {

@@ -67,9 +67,6 @@ struct Ctx {
     // the current evaluation use (stream itself when stages are timed).
     hipStream_t recip_stream = nullptr;
     int pair_lanes = kLanesPerAtom;  // lanes per atom in the pair kernel: 4, 8 or 16
-    // true while dlpx_md_run evaluates forces: every atom is then within half
-    // the padding of the position it was binned at (kernels may rely on it)
-    bool resident_step = false;
     hipStream_t sp = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int device = 0;

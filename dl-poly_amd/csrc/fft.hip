@@ -1,5 +1,6 @@
 // 3D FFT of the SPME grid: batched 1D Stockham transforms in LDS for
-// 2^a 3^b 5^c lengths (the sizes parallel_fft.F90:1999-2054 accepts),
+// 2^a 3^b 5^c lengths (the sizes parallel_fft.F90:1999-2054 accepts), radix
+// 16, 8, 4, 2, 3 and 5 passes,
 // replacing DL_POLY's GPFA/DaFT (gpfa235.F90).  Convention of
 // gpfa235.F90:72: sign +1 is sum_k c_k exp(+2 pi i j k/n), no normalisation.
 //
@@ -29,6 +30,8 @@ static FftPlan make_plan(int n)
     p.n = n;
     p.np = 0;
     int m = n;
+    while (m % 16 == 0) { p.rad[p.np++] = 16; m /= 16; }
+    if (m % 8 == 0) { p.rad[p.np++] = 8; m /= 8; }
     while (m % 4 == 0) { p.rad[p.np++] = 4; m /= 4; }
     while (m % 2 == 0) { p.rad[p.np++] = 2; m /= 2; }
     while (m % 3 == 0) { p.rad[p.np++] = 3; m /= 3; }
@@ -80,11 +83,81 @@ template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { T2 r; r.
 // multiply by s*i
 template <typename T2> __device__ __forceinline__ T2 cmuli(T2 a, int s) { T2 r; r.x = -s * a.y; r.y = s * a.x; return r; }
 
-// In-place DFT of length R (2, 3, 4, 5) with kernel exp(sign 2 pi i jk/R).
+// In-place 4-point DFT with kernel exp(sign 2 pi i jk/4).
+template <typename T2>
+__device__ __forceinline__ void dft4(T2 &a, T2 &b, T2 &c, T2 &d, int sign)
+{
+    const T2 s02 = cadd(a, c), d02 = csub(a, c);
+    const T2 s13 = cadd(b, d), d13 = cmuli(csub(b, d), sign);
+    a = cadd(s02, s13);
+    c = csub(s02, s13);
+    b = cadd(d02, d13);
+    d = csub(d02, d13);
+}
+
+// a * exp(sign 2 pi i m/16) for m in 0..9 (cos/sin of multiples of pi/8)
+template <typename T, typename T2>
+__device__ __forceinline__ T2 tw16(T2 a, int m, int sign)
+{
+    const T cs[10] = {(T)1, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173, (T)0,
+                      (T)-0.38268343236508977173, (T)-0.70710678118654752440, (T)-0.92387953251128675613, (T)-1,
+                      (T)-0.92387953251128675613};
+    const T sn[10] = {(T)0, (T)0.38268343236508977173, (T)0.70710678118654752440, (T)0.92387953251128675613, (T)1,
+                      (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173, (T)0,
+                      (T)-0.38268343236508977173};
+    const T c = cs[m], si = sign > 0 ? sn[m] : -sn[m];
+    T2 r;
+    r.x = a.x * c - a.y * si;
+    r.y = a.x * si + a.y * c;
+    return r;
+}
+
+// In-place DFT of length R (2, 3, 4, 5, 8, 16) with kernel exp(sign 2 pi i jk/R).
 template <typename T, typename T2, int R>
 __device__ __forceinline__ void dft_small(T2 *x, int sign)
 {
-    if (R == 2) {
+    if (R == 16) {
+        // 16 = 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle by w16^(n2 k1),
+        // DFT4 over n2; output k = k1 + 4 k2
+        T2 a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            a[n2][0] = x[n2];
+            a[n2][1] = x[4 + n2];
+            a[n2][2] = x[8 + n2];
+            a[n2][3] = x[12 + n2];
+            dft4(a[n2][0], a[n2][1], a[n2][2], a[n2][3], sign);
+        }
+#pragma unroll
+        for (int n2 = 1; n2 < 4; n2++)
+#pragma unroll
+            for (int k1 = 1; k1 < 4; k1++) a[n2][k1] = tw16<T, T2>(a[n2][k1], n2 * k1, sign);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            dft4(a[0][k1], a[1][k1], a[2][k1], a[3][k1], sign);
+            x[k1] = a[0][k1];
+            x[k1 + 4] = a[1][k1];
+            x[k1 + 8] = a[2][k1];
+            x[k1 + 12] = a[3][k1];
+        }
+    } else if (R == 8) {
+        // 8 = 2 x 4: DFT4 of the even and odd inputs, odd k twiddled by w8^k
+        T2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+        T2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+        dft4(e0, e1, e2, e3, sign);
+        dft4(o0, o1, o2, o3, sign);
+        o1 = tw16<T, T2>(o1, 2, sign);
+        o2 = tw16<T, T2>(o2, 4, sign);
+        o3 = tw16<T, T2>(o3, 6, sign);
+        x[0] = cadd(e0, o0);
+        x[4] = csub(e0, o0);
+        x[1] = cadd(e1, o1);
+        x[5] = csub(e1, o1);
+        x[2] = cadd(e2, o2);
+        x[6] = csub(e2, o2);
+        x[3] = cadd(e3, o3);
+        x[7] = csub(e3, o3);
+    } else if (R == 2) {
         T2 a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
@@ -175,7 +248,9 @@ __device__ T2 *stockham(T2 *buf0, T2 *buf1, const FftPlan &pl, const T2 *tw, int
     for (int p = 0; p < pl.np; p++) {
         if (active) {
             const int r = pl.rad[p];
-            if (r == 4) stockham_pass<T, T2, 4>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            if (r == 16) stockham_pass<T, T2, 16>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            else if (r == 8) stockham_pass<T, T2, 8>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            else if (r == 4) stockham_pass<T, T2, 4>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else if (r == 2) stockham_pass<T, T2, 2>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else if (r == 3) stockham_pass<T, T2, 3>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else stockham_pass<T, T2, 5>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);

@@ -224,152 +224,6 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
     }
 }
 
-// Charge spreading by owned bricks (resident steps): the grid is cut into
-// disjoint bricks, brick d-range [ceil(c K/nc), ceil((c+1) K/nc)) for link
-// cell c, so every grid point has exactly one owner.  The workgroup of cell
-// c accumulates in LDS the part of every atom's S^3 stencil that falls in its
-// brick, taking the atoms of the cells that can reach it (offsets omin..omax
-// per axis: an atom at u spreads to floor(u)-S+1..floor(u), and atoms may sit
-// up to half the padding outside the cell they were binned in), and then
-// stores the brick with plain stores: no global atomics and no clearing of
-// the grid.  Requires atoms within delta (grid units) of their cell, which the
-// resident integrator guarantees between list rebuilds, and distinct cells
-// for all offsets (nc >= omax - omin + 1).
-struct BrickParams {
-    int nc[3];
-    int omin[3], omax[3];
-    double L[3];
-};
-constexpr int kBrickCap = 4096;  // doubles (32 KB)
-
-template <typename R, int S>
-__global__ void __launch_bounds__(256) k_spread_bricks(const double4 *__restrict__ pos,
-                                                       const int32_t *__restrict__ cell_start, BrickParams bp,
-                                                       SpmeParams sp, R *__restrict__ Q)
-{
-    constexpr int kMaxCells = 64;  // (omax - omin + 1)^3, checked on the host
-    __shared__ double acc[kBrickCap];
-    __shared__ R sM[3][kAtomsPerBlock][S];
-    __shared__ int sB[3][kAtomsPerBlock];
-    __shared__ R sQ[kAtomsPerBlock];
-    __shared__ int s_start[kMaxCells], s_off[kMaxCells + 1], s_shift[kMaxCells][3];
-    const int ncell = bp.nc[0] * bp.nc[1] * bp.nc[2];
-    const int c = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x, wid = t >> 6, lane = t & 63;
-    if (c >= ncell) return;
-    const int cc[3] = {c % bp.nc[0], (c / bp.nc[0]) % bp.nc[1], c / (bp.nc[0] * bp.nc[1])};
-    int lo[3], W[3], no[3];
-    for (int d = 0; d < 3; d++) {
-        lo[d] = (cc[d] * sp.K[d] + bp.nc[d] - 1) / bp.nc[d];
-        W[d] = ((cc[d] + 1) * sp.K[d] + bp.nc[d] - 1) / bp.nc[d] - lo[d];
-        no[d] = bp.omax[d] - bp.omin[d] + 1;
-    }
-    const int nsrc = no[0] * no[1] * no[2];
-    const int plane = W[0] * W[1], vol = plane * W[2];
-    for (int k = t; k < vol; k += blockDim.x) acc[k] = 0.0;
-    if (t < nsrc) {
-        // source cell t of the (omin..omax)^3 block of offsets, with its
-        // periodic image shift in grid units
-        const int o[3] = {bp.omin[0] + t % no[0], bp.omin[1] + (t / no[0]) % no[1], bp.omin[2] + t / (no[0] * no[1])};
-        int jc = 0;
-        for (int d = 2; d >= 0; d--) {
-            const int v = cc[d] + o[d];
-            const int w = v < 0 ? v + bp.nc[d] : (v >= bp.nc[d] ? v - bp.nc[d] : v);
-            s_shift[t][d] = v < 0 ? -sp.K[d] : (v >= bp.nc[d] ? sp.K[d] : 0);
-            jc = jc * bp.nc[d] + w;
-        }
-        s_start[t] = cell_start[jc];
-        s_off[t + 1] = cell_start[jc + 1] - cell_start[jc];
-    }
-    __syncthreads();
-    if (t == 0) {
-        s_off[0] = 0;
-        for (int k = 0; k < nsrc; k++) s_off[k + 1] += s_off[k];
-    }
-    __syncthreads();
-    const int ncand = s_off[nsrc];
-    // candidates in batches of kAtomsPerBlock; threads t < 3 kAtomsPerBlock
-    // (axis t / kAtomsPerBlock of atom t % kAtomsPerBlock) prefetch the next
-    // batch's position while the current batch is accumulated
-    const int sd = t / kAtomsPerBlock, sa = t % kAtomsPerBlock;
-    const bool stager = t < 3 * kAtomsPerBlock;
-    double4 pnext = make_double4(0.0, 0.0, 0.0, 0.0);
-    int qnext = 0;
-    auto fetch = [&](int k) {
-        if (stager && k < ncand) {
-            int q = 0;
-            while (s_off[q + 1] <= k) q++;
-            qnext = q;
-            pnext = pos[s_start[q] + (k - s_off[q])];
-        }
-    };
-    fetch(sa);
-    for (int b0 = 0; b0 < ncand; b0 += kAtomsPerBlock) {
-        __syncthreads();
-        if (stager && b0 + sa < ncand) {
-            const double4 p = pnext;
-            const double u = grid_u(sp, sd, p.x, p.y, p.z);
-            const double fl = floor(u);
-            R M[S];
-            bspline<R, false, S>((R)(u - fl), M, nullptr);
-#pragma unroll
-            for (int k = 0; k < S; k++) sM[sd][sa][k] = M[k];
-            // first stencil point relative to the brick
-            sB[sd][sa] = (int)fl - S + 1 + s_shift[qnext][sd] - lo[sd];
-            if (sd == 0) sQ[sa] = (R)p.w;
-        }
-        fetch(b0 + kAtomsPerBlock + sa);
-        __syncthreads();
-        const int na = min(kAtomsPerBlock, ncand - b0);
-        for (int a = wid; a < na; a += 4) {
-            const R q = sQ[a];
-            if (fabs((double)q) <= kZeroPlus) continue;
-            const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
-            const int tx0 = max(0, -bx), tx1 = min(S, W[0] - bx);
-            const int ty0 = max(0, -by), ty1 = min(S, W[1] - by);
-            const int tz0 = max(0, -bz), tz1 = min(S, W[2] - bz);
-            if (tx0 >= tx1 || ty0 >= ty1 || tz0 >= tz1) continue;
-            for (int pp = lane; pp < S * S; pp += 64) {
-                const int tx = pp % S, ty = pp / S;
-                if (tx < tx0 || tx >= tx1 || ty < ty0 || ty >= ty1) continue;
-                const R mxy = sM[0][a][tx] * sM[1][a][ty];
-                int l = ((bz + tz0) * W[1] + (by + ty)) * W[0] + (bx + tx);
-                for (int tz = tz0; tz < tz1; tz++, l += plane) atomicAdd(&acc[l], (double)(q * sM[2][a][tz] * mxy));
-            }
-        }
-    }
-    __syncthreads();
-    for (int k = t; k < vol; k += blockDim.x) {
-        const int lx = k % W[0], ly = (k / W[0]) % W[1], lz = k / plane;
-        Q[((size_t)(lo[2] + lz) * sp.K[1] + (lo[1] + ly)) * sp.K[0] + (lo[0] + lx)] = (R)acc[k];
-    }
-}
-
-// Brick set-up for the current cells, or false when the brick kernel does
-// not apply (too few cells per axis for distinct offsets, or a brick larger
-// than the LDS accumulator).
-static bool brick_params(const Ctx *c, BrickParams &bp)
-{
-    const int S = c->prm.nsplines;
-    int maxw[3];
-    for (int d = 0; d < 3; d++) {
-        bp.nc[d] = c->nc[d];
-        bp.L[d] = c->L[d];
-        const double w = (double)c->K[d] / c->nc[d];                        // brick width in grid points
-        const double delta = 0.5 * c->prm.padding * c->K[d] / c->L[d] + 1.0;  // drift since the rebuild, + rounding
-        // atoms of cell c+o lie in [(c+o) w - delta, (c+o+1) w + delta); they reach
-        // brick c = [ceil(c w), ceil((c+1) w)) when floor(u) - S + 1 < hi and floor(u) >= lo
-        bp.omin[d] = -(int)std::ceil(delta / w);
-        bp.omax[d] = (int)std::ceil((S + delta) / w);
-        if (bp.omax[d] - bp.omin[d] + 1 > c->nc[d]) return false;
-        maxw[d] = 0;
-        for (int k = 0; k < c->nc[d]; k++)
-            maxw[d] = std::max(maxw[d], ((k + 1) * c->K[d] + c->nc[d] - 1) / c->nc[d] -
-                                            (k * c->K[d] + c->nc[d] - 1) / c->nc[d]);
-    }
-    const int nsrc = (bp.omax[0] - bp.omin[0] + 1) * (bp.omax[1] - bp.omin[1] + 1) * (bp.omax[2] - bp.omin[2] + 1);
-    return nsrc <= 64 && (long)maxw[0] * maxw[1] * maxw[2] <= kBrickCap;
-}
-
 // Lanes per atom in the gather: the next power of two >= S, lane tx < S
 // taking the grid column x = base + tx.
 constexpr int gather_lanes(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : 16; }
@@ -605,30 +459,16 @@ static int spme_run(Ctx *c)
     const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
     int r;
     // ewald_general.F90:517-576 charge spreading
-    BrickParams bp;
-    if (c->resident_step && brick_params(c, bp)) {
-        switch (sp.S) {
-#define DLPX_SPREAD(SS)                                                                                    \
-    case SS:                                                                                               \
-        hipLaunchKernelGGL((k_spread_bricks<R, SS>), dim3(xcd_grid(c->ncell)), dim3(256), 0, s, c->pos,     \
-                           c->cell_start, bp, sp, (R *)c->Q);                                              \
-        break;
-            DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
-#undef DLPX_SPREAD
-            default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
-        }
-    } else {
-        DLPX_HIP(c, hipMemsetAsync(c->Q, 0, sizeof(R) * c->nreal, s));
-        switch (sp.S) {
+    DLPX_HIP(c, hipMemsetAsync(c->Q, 0, sizeof(R) * c->nreal, s));
+    switch (sp.S) {
 #define DLPX_SPREAD(SS)                                                                                    \
     case SS:                                                                                               \
         hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(256), 0, s, c->pos,      \
                            c->cell_start, c->ncell, sp, (R *)c->Q);                                        \
         break;
-            DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
+        DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
 #undef DLPX_SPREAD
-            default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
-        }
+        default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
     }
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[2], s));
     // ewald_spole.F90:1296-1306 cutoff and convergence factors
