@@ -131,10 +131,10 @@ struct Ctx {
 
     // SPME
     int K[3]{0, 0, 0};
-    int K1h = 0;
+    int K1h = 0, K1hp = 0;           // K1/2+1 and the padded row stride of H
     size_t nreal = 0, nhalf = 0;
     void *Q = nullptr;               // real grid K3 x K2 x K1 (spread, then potential)
-    void *H = nullptr;               // complex half grid K3 x K2 x K1h
+    void *H = nullptr;               // complex half grid K3 x K2 x K1hp (x < K1h used)
     void *tw = nullptr;              // twiddles per axis: 3 x max(K) complex
     double *bsp_n2 = nullptr;        // |b|^2 per axis, K1 + K2 + K3 (FP64)
     double self_energy = 0.0;

@@ -1,6 +1,6 @@
 // 3D FFT of the SPME grid: batched 1D Stockham transforms in LDS for
 // 2^a 3^b 5^c lengths (the sizes parallel_fft.F90:1999-2054 accepts), radix
-// 16, 8, 4, 2, 3 and 5 passes,
+// 4, 2, 3 and 5 passes,
 // replacing DL_POLY's GPFA/DaFT (gpfa235.F90).  Convention of
 // gpfa235.F90:72: sign +1 is sum_k c_k exp(+2 pi i j k/n), no normalisation.
 //
@@ -30,8 +30,6 @@ static FftPlan make_plan(int n)
     p.n = n;
     p.np = 0;
     int m = n;
-    while (m % 16 == 0) { p.rad[p.np++] = 16; m /= 16; }
-    if (m % 8 == 0) { p.rad[p.np++] = 8; m /= 8; }
     while (m % 4 == 0) { p.rad[p.np++] = 4; m /= 4; }
     while (m % 2 == 0) { p.rad[p.np++] = 2; m /= 2; }
     while (m % 3 == 0) { p.rad[p.np++] = 3; m /= 3; }
@@ -95,69 +93,11 @@ __device__ __forceinline__ void dft4(T2 &a, T2 &b, T2 &c, T2 &d, int sign)
     d = csub(d02, d13);
 }
 
-// a * exp(sign 2 pi i m/16) for m in 0..9 (cos/sin of multiples of pi/8)
-template <typename T, typename T2>
-__device__ __forceinline__ T2 tw16(T2 a, int m, int sign)
-{
-    const T cs[10] = {(T)1, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173, (T)0,
-                      (T)-0.38268343236508977173, (T)-0.70710678118654752440, (T)-0.92387953251128675613, (T)-1,
-                      (T)-0.92387953251128675613};
-    const T sn[10] = {(T)0, (T)0.38268343236508977173, (T)0.70710678118654752440, (T)0.92387953251128675613, (T)1,
-                      (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173, (T)0,
-                      (T)-0.38268343236508977173};
-    const T c = cs[m], si = sign > 0 ? sn[m] : -sn[m];
-    T2 r;
-    r.x = a.x * c - a.y * si;
-    r.y = a.x * si + a.y * c;
-    return r;
-}
-
-// In-place DFT of length R (2, 3, 4, 5, 8, 16) with kernel exp(sign 2 pi i jk/R).
+// In-place DFT of length R (2, 3, 4, 5) with kernel exp(sign 2 pi i jk/R).
 template <typename T, typename T2, int R>
 __device__ __forceinline__ void dft_small(T2 *x, int sign)
 {
-    if (R == 16) {
-        // 16 = 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle by w16^(n2 k1),
-        // DFT4 over n2; output k = k1 + 4 k2
-        T2 a[4][4];
-#pragma unroll
-        for (int n2 = 0; n2 < 4; n2++) {
-            a[n2][0] = x[n2];
-            a[n2][1] = x[4 + n2];
-            a[n2][2] = x[8 + n2];
-            a[n2][3] = x[12 + n2];
-            dft4(a[n2][0], a[n2][1], a[n2][2], a[n2][3], sign);
-        }
-#pragma unroll
-        for (int n2 = 1; n2 < 4; n2++)
-#pragma unroll
-            for (int k1 = 1; k1 < 4; k1++) a[n2][k1] = tw16<T, T2>(a[n2][k1], n2 * k1, sign);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; k1++) {
-            dft4(a[0][k1], a[1][k1], a[2][k1], a[3][k1], sign);
-            x[k1] = a[0][k1];
-            x[k1 + 4] = a[1][k1];
-            x[k1 + 8] = a[2][k1];
-            x[k1 + 12] = a[3][k1];
-        }
-    } else if (R == 8) {
-        // 8 = 2 x 4: DFT4 of the even and odd inputs, odd k twiddled by w8^k
-        T2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
-        T2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
-        dft4(e0, e1, e2, e3, sign);
-        dft4(o0, o1, o2, o3, sign);
-        o1 = tw16<T, T2>(o1, 2, sign);
-        o2 = tw16<T, T2>(o2, 4, sign);
-        o3 = tw16<T, T2>(o3, 6, sign);
-        x[0] = cadd(e0, o0);
-        x[4] = csub(e0, o0);
-        x[1] = cadd(e1, o1);
-        x[5] = csub(e1, o1);
-        x[2] = cadd(e2, o2);
-        x[6] = csub(e2, o2);
-        x[3] = cadd(e3, o3);
-        x[7] = csub(e3, o3);
-    } else if (R == 2) {
+    if (R == 2) {
         T2 a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
@@ -205,7 +145,9 @@ __device__ __forceinline__ void dft_small(T2 *x, int sign)
     }
 }
 
-constexpr int kFftThreads = 256;
+// 512 threads per workgroup: the y and z passes have few workgroups (one per
+// 16 lines) and are latency-bound, so each needs many wavefronts in flight.
+constexpr int kFftThreads = 512;
 
 // One Stockham pass of radix R over one line: butterfly j (0 <= j < n/R)
 // reads src[j + q m] (m = n/R), twiddles input q by exp(sign 2 pi i q k/(Ls R))
@@ -248,9 +190,7 @@ __device__ T2 *stockham(T2 *buf0, T2 *buf1, const FftPlan &pl, const T2 *tw, int
     for (int p = 0; p < pl.np; p++) {
         if (active) {
             const int r = pl.rad[p];
-            if (r == 16) stockham_pass<T, T2, 16>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
-            else if (r == 8) stockham_pass<T, T2, 8>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
-            else if (r == 4) stockham_pass<T, T2, 4>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
+            if (r == 4) stockham_pass<T, T2, 4>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else if (r == 2) stockham_pass<T, T2, 2>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else if (r == 3) stockham_pass<T, T2, 3>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
             else stockham_pass<T, T2, 5>(src + off, dst + off, n, pl.ls[p], pl.inv_ls[p], tw, sign, jt, tpl);
@@ -263,38 +203,62 @@ __device__ T2 *stockham(T2 *buf0, T2 *buf1, const FftPlan &pl, const T2 *tw, int
     return src;
 }
 
-// Strided lines of the half-spectrum array: line L starts at
-// (L % na) * sa + (L / na) * sb and its elements are se apart.
+// Strided lines of the half-spectrum array.  Line L is (a, b) = (L % na_pad,
+// L / na_pad); it exists when a < na, starts at a * sa + b * sb and its
+// elements are se apart.  na_pad (na rounded up to the lines per workgroup)
+// keeps every workgroup's lines on one aligned run of x columns, and the
+// array's row stride K1hp (K1/2+1 rounded up to 16) keeps those runs on whole
+// 128-byte lines: with rows of K1/2+1 elements every run straddled two.
 struct LineMap {
-    int nl;        // number of lines
-    int na;
+    int nl;        // number of line slots, na_pad * (number of b)
+    int na, na_pad;
     long sa, sb;
     long se;
 };
 
+__device__ __forceinline__ bool line_ok(const LineMap &lm, int L) { return L < lm.nl && L % lm.na_pad < lm.na; }
+
 template <typename T2>
 __device__ __forceinline__ size_t line_base(const LineMap &lm, int L)
 {
-    return (size_t)(L % lm.na) * lm.sa + (size_t)(L / lm.na) * lm.sb;
+    return (size_t)(L % lm.na_pad) * lm.sa + (size_t)(L / lm.na_pad) * lm.sb;
 }
 
 // LPB strided lines per workgroup: thread t moves elements of line t % LPB,
 // so the LPB lines (adjacent x columns) are read and written as contiguous
 // runs of LPB elements.
+// Loads are issued kLoadBatch at a time into registers before any is stored
+// to LDS, so a workgroup waits for one memory round trip per batch rather
+// than one per element.
+constexpr int kLoadBatch = 8;
+
 template <typename T2, int LPB>
 __device__ __forceinline__ void load_lines(const T2 *data, const LineMap &lm, int n, int L0, int nl, T2 *b0)
 {
     const int ll = threadIdx.x % LPB;
-    if (ll >= nl) return;
+    if (ll >= nl || !line_ok(lm, L0 + ll)) return;
     const size_t base = line_base<T2>(lm, L0 + ll);
-    for (int e = threadIdx.x / LPB; e < n; e += kFftThreads / LPB) b0[ll * (n + 1) + e] = data[base + (size_t)e * lm.se];
+    constexpr int step = kFftThreads / LPB;
+    for (int e0 = threadIdx.x / LPB; e0 < n; e0 += kLoadBatch * step) {
+        T2 v[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int e = e0 + u * step;
+            if (e < n) v[u] = data[base + (size_t)e * lm.se];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int e = e0 + u * step;
+            if (e < n) b0[ll * (n + 1) + e] = v[u];
+        }
+    }
 }
 
 template <typename T2, int LPB>
 __device__ __forceinline__ void store_lines(T2 *data, const LineMap &lm, int n, int L0, int nl, const T2 *res)
 {
     const int ll = threadIdx.x % LPB;
-    if (ll >= nl) return;
+    if (ll >= nl || !line_ok(lm, L0 + ll)) return;
     const size_t base = line_base<T2>(lm, L0 + ll);
     for (int e = threadIdx.x / LPB; e < n; e += kFftThreads / LPB) data[base + (size_t)e * lm.se] = res[ll * (n + 1) + e];
 }
@@ -320,14 +284,16 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_c2c(typename C2T<T>::type *
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
     const int lw = threadIdx.x / TPL;
-    T2 *res = stockham<T, T2>(b0, b1, pl, stw, sign, lw, lw < nl, threadIdx.x % TPL, TPL);
+    T2 *res = stockham<T, T2>(b0, b1, pl, stw, sign, lw, lw < nl && line_ok(lm, L0 + lw), threadIdx.x % TPL, TPL);
     store_lines<T2, LPB>(data, lm, n, L0, nl, res);
 }
+
+constexpr int kConvRedBytes = 6 * (kFftThreads / 64) * sizeof(double);
 
 // The z axis with the influence function in between: forward transform of
 // each z line, multiply by theta(m) (ewald_spole.F90:1296-1380) while summing
 // the reciprocal stress, inverse transform, all in LDS.  Line L is the column
-// (kx, ky) = (L % K1h, L / K1h) of the half spectrum.
+// (kx, ky) = (a, b) of the half spectrum (see LineMap).
 template <typename T, int LPB>
 __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
                                                             const typename C2T<T>::type *__restrict__ tw,
@@ -336,22 +302,22 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    double *lred = (double *)smem;  // 6 x 4 wavefronts
+    double *lred = (double *)smem;  // 6 values x wavefronts
     const int n = pl.n, ns = n + 1;
-    T2 *b0 = (T2 *)(smem + 6 * 4 * sizeof(double)), *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
+    T2 *b0 = (T2 *)(smem + kConvRedBytes), *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
     const int L0 = blockIdx.x * LPB, nl = min(LPB, lm.nl - L0);
     load_twiddles(tw, n, stw);
     load_lines<T2, LPB>(data, lm, n, L0, nl, b0);
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
     const int lw = threadIdx.x / TPL, jt = threadIdx.x % TPL;
-    const bool active = lw < nl;
+    const bool active = lw < nl && line_ok(lm, L0 + lw);
     T2 *res = stockham<T, T2>(b0, b1, pl, stw, +1, lw, active, jt, TPL);
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (active) {
         const double sqrpi = 1.7724538509055159;
         const int L = L0 + lw;
-        const int j = L % ip.K1h, k = L / ip.K1h;
+        const int j = L % lm.na_pad, k = L / lm.na_pad;
         const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
         const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
         const double bxy = n2y[k] * n2x[j];
@@ -391,10 +357,10 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
 }
 
 // x axis, forward: real lines 2P and 2P+1 of Q (n contiguous values each)
-// packed as one complex line; output half spectra to H (K1h per line).
+// packed as one complex line; output half spectra to H (K1h of a row of hs).
 template <typename T, int LPB>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H,
-                                                           int nreal_lines, int K1h, FftPlan pl,
+                                                           int nreal_lines, int K1h, int hs, FftPlan pl,
                                                            const typename C2T<T>::type *__restrict__ tw,
                                                            FastDiv div_n, FastDiv div_h)
 {
@@ -406,14 +372,28 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__
     const int P0 = blockIdx.x * LPB;
     const int np = min(LPB, npairs - P0);
     load_twiddles(tw, n, stw);
-    for (int idx = threadIdx.x; idx < np * n; idx += kFftThreads) {
-        int e;
-        const int ll = fdivmod(idx, div_n, e);
-        const int a = 2 * (P0 + ll), b = a + 1;
-        T2 z;
-        z.x = Q[(size_t)a * n + e];
-        z.y = b < nreal_lines ? Q[(size_t)b * n + e] : (T)0;
-        b0[ll * ns + e] = z;
+    for (int i0 = threadIdx.x; i0 < np * n; i0 += kLoadBatch * kFftThreads) {
+        T2 z[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kFftThreads;
+            if (idx < np * n) {
+                int e;
+                const int ll = fdivmod(idx, div_n, e);
+                const int a = 2 * (P0 + ll), b = a + 1;
+                z[u].x = Q[(size_t)a * n + e];
+                z[u].y = b < nreal_lines ? Q[(size_t)b * n + e] : (T)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kFftThreads;
+            if (idx < np * n) {
+                int e;
+                const int ll = fdivmod(idx, div_n, e);
+                b0[ll * ns + e] = z[u];
+            }
+        }
     }
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
@@ -430,8 +410,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__
         A.y = (T)0.5 * (z.y - zc.y);
         B.x = (T)0.5 * (z.y + zc.y);
         B.y = (T)-0.5 * (z.x - zc.x);
-        H[(size_t)a * K1h + k] = A;
-        if (b < nreal_lines) H[(size_t)b * K1h + k] = B;
+        H[(size_t)a * hs + k] = A;
+        if (b < nreal_lines) H[(size_t)b * hs + k] = B;
     }
 }
 
@@ -439,7 +419,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__
 // transform with sign -1 and write the real parts to Q.
 template <typename T, int LPB>
 __global__ void __launch_bounds__(kFftThreads) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
-                                                           int nreal_lines, int K1h, FftPlan pl,
+                                                           int nreal_lines, int K1h, int hs, FftPlan pl,
                                                            const typename C2T<T>::type *__restrict__ tw,
                                                            FastDiv div_n)
 {
@@ -451,25 +431,42 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_c2r(const typename C2T<T>
     const int P0 = blockIdx.x * LPB;
     const int np = min(LPB, npairs - P0);
     load_twiddles(tw, n, stw);
-    for (int idx = threadIdx.x; idx < np * n; idx += kFftThreads) {
-        int k;
-        const int ll = fdivmod(idx, div_n, k);
-        const int a = 2 * (P0 + ll), b = a + 1;
-        const bool hb = b < nreal_lines;
-        const int kh = k < K1h ? k : n - k;
-        const T2 A = H[(size_t)a * K1h + kh];
-        T2 B;
-        B.x = hb ? H[(size_t)b * K1h + kh].x : (T)0;
-        B.y = hb ? H[(size_t)b * K1h + kh].y : (T)0;
-        T2 z;
-        if (k < K1h) {
-            z.x = A.x - B.y;
-            z.y = A.y + B.x;
-        } else {
-            z.x = A.x + B.y;
-            z.y = -A.y + B.x;
+    for (int i0 = threadIdx.x; i0 < np * n; i0 += kLoadBatch * kFftThreads) {
+        T2 A[kLoadBatch], B[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kFftThreads;
+            if (idx < np * n) {
+                int k;
+                const int ll = fdivmod(idx, div_n, k);
+                const int a = 2 * (P0 + ll), b = a + 1;
+                const int kh = k < K1h ? k : n - k;
+                A[u] = H[(size_t)a * hs + kh];
+                if (b < nreal_lines) {
+                    B[u] = H[(size_t)b * hs + kh];
+                } else {
+                    B[u].x = (T)0;
+                    B[u].y = (T)0;
+                }
+            }
         }
-        b0[ll * ns + k] = z;
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; u++) {
+            const int idx = i0 + u * kFftThreads;
+            if (idx < np * n) {
+                int k;
+                const int ll = fdivmod(idx, div_n, k);
+                T2 z;
+                if (k < K1h) {
+                    z.x = A[u].x - B[u].y;
+                    z.y = A[u].y + B[u].x;
+                } else {
+                    z.x = A[u].x + B[u].y;
+                    z.y = -A[u].y + B[u].x;
+                }
+                b0[ll * ns + k] = z;
+            }
+        }
     }
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
@@ -516,14 +513,15 @@ static int run_y(Ctx *c, int sign)
 {
     using T2 = typename C2T<T>::type;
     const int K1h = c->K1h, K2 = c->K[1], K3 = c->K[2];
-    LineMap lm;
-    lm.nl = K1h * K3;
-    lm.na = K1h;
-    lm.sa = 1;
-    lm.sb = (long)K1h * K2;
-    lm.se = K1h;
     const int lpb = pick_lpb<T>(K2, 0);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    LineMap lm;
+    lm.na = K1h;
+    lm.na_pad = (K1h + lpb - 1) / lpb * lpb;
+    lm.nl = lm.na_pad * K3;
+    lm.sa = 1;
+    lm.sb = (long)c->K1hp * K2;
+    lm.se = c->K1hp;
     const FftPlan pl = make_plan(K2);
     const T2 *tw = (const T2 *)c->tw + c->K[0];
     const size_t lds = fft_lds<T>(lpb, K2, 0);
@@ -538,21 +536,22 @@ static int run_y(Ctx *c, int sign)
     return 0;
 }
 
-static LineMap z_lines(const Ctx *c)
+static LineMap z_lines(const Ctx *c, int lpb)
 {
     LineMap lm;
-    lm.nl = c->K1h * c->K[1];
-    lm.na = c->K1h * c->K[1];
+    lm.na = c->K1h;
+    lm.na_pad = (c->K1h + lpb - 1) / lpb * lpb;
+    lm.nl = lm.na_pad * c->K[1];
     lm.sa = 1;
-    lm.sb = 0;
-    lm.se = (long)c->K1h * c->K[1];
+    lm.sb = c->K1hp;
+    lm.se = (long)c->K1hp * c->K[1];
     return lm;
 }
 
 template <typename T>
 static int z_lpb(const Ctx *c)
 {
-    return pick_lpb<T>(c->K[2], 6 * 4 * sizeof(double));
+    return pick_lpb<T>(c->K[2], kConvRedBytes);
 }
 
 template <typename T>
@@ -560,10 +559,10 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
 {
     using T2 = typename C2T<T>::type;
     const int K3 = c->K[2];
-    const LineMap lm = z_lines(c);
-    const size_t extra = 6 * 4 * sizeof(double);
+    const size_t extra = kConvRedBytes;
     const int lpb = z_lpb<T>(c);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    const LineMap lm = z_lines(c, lpb);
     const FftPlan pl = make_plan(K3);
     const T2 *tw = (const T2 *)c->tw + c->K[0] + c->K[1];
     const size_t lds = fft_lds<T>(lpb, K3, extra);
@@ -597,7 +596,7 @@ static int run_x(Ctx *c, bool forward)
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                     (int)lds));                                                                     \
     hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)c->Q,           \
-                       (T2 *)c->H, nlines, c->K1h, pl, (const T2 *)c->tw, dn, dh)
+                       (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
     } else {
@@ -605,7 +604,7 @@ static int run_x(Ctx *c, bool forward)
     DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                     (int)lds));                                                                     \
     hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T2 *)c->H,          \
-                       (T *)c->Q, nlines, c->K1h, pl, (const T2 *)c->tw, dn)
+                       (T *)c->Q, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn)
         DLPX_LPB_SWITCH(lpb, DLPX_C2R)
 #undef DLPX_C2R
     }
@@ -632,7 +631,7 @@ int fft_convolve(Ctx *c, const InflParams &ip)
 int conv_blocks(const Ctx *c)
 {
     const int lpb = c->prm.precision == DLPX_MIXED ? z_lpb<float>(c) : z_lpb<double>(c);
-    return lpb ? ceil_div((long)c->K1h * c->K[1], lpb) : 0;
+    return lpb ? ceil_div((long)z_lines(c, lpb).nl, lpb) : 0;
 }
 
 }  // namespace dlpx

@@ -195,9 +195,15 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
     __shared__ float4 cand[kListStage];
     __shared__ int s_start[27], s_off[28];
     __shared__ double s_sh[27][3];
+    __shared__ int s_ovf, s_big, s_tot;  // workgroup partials of flags[0], [1], [3]
     const int c = xcd_block(blockIdx.x, gridDim.x);
     if (c >= g.nc[0] * g.nc[1] * g.nc[2]) return;
     const int t = threadIdx.x;
+    if (t == 0) {
+        s_ovf = 0;
+        s_big = 0;
+        s_tot = 0;
+    }
     const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
     const bool small = g.nc[0] < 3 || g.nc[1] < 3 || g.nc[2] < 3;
     if (t < 27) {
@@ -303,16 +309,24 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
         }
         if (active) {
             cnt[(size_t)i * G + lane] = count;
-            if (count > cap_lane) atomicMax(&flags[0], count);
+            if (count > cap_lane) atomicMax(&s_ovf, count);
         }
         // per-atom total (lanes of one atom are adjacent in the wavefront)
         int tot = count;
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
         if (active && lane == 0) {
-            atomicMax(&flags[1], tot);
-            atomicAdd(&flags[3], tot);
+            atomicMax(&s_big, tot);
+            atomicAdd(&s_tot, tot);
         }
+    }
+    // one global atomic per workgroup and flag: a single address hit by every
+    // atom would serialise tens of thousands of atomics
+    __syncthreads();
+    if (t == 0) {
+        if (s_ovf) atomicMax(&flags[0], s_ovf);
+        atomicMax(&flags[1], s_big);
+        atomicAdd(&flags[3], s_tot);
     }
 }
 

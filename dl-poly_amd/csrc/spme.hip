@@ -126,8 +126,11 @@ constexpr int kSlabBytes = 32768;
 // The window accumulates in FP64 at either precision: FP64 LDS atomic adds
 // (ds_add_f64) sustain several times the rate of FP32 ones (ds_add_f32) on
 // gfx950, and the FP32 grid then receives one rounding per window point.
+// 512 threads: one workgroup per link cell gives only ~730 workgroups at 64k
+// ions, so each needs many wavefronts to keep the LDS atomics busy.
+constexpr int kSpreadThreads = 512;
 template <typename R, int S>
-__global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict__ pos,
+__global__ void __launch_bounds__(kSpreadThreads) k_spread_cells(const double4 *__restrict__ pos,
                                                       const int32_t *__restrict__ cell_start, int ncell,
                                                       SpmeParams sp, R *__restrict__ Q)
 {
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(256) k_spread_cells(const double4 *__restrict_
             }
             __syncthreads();
             const int na = min(kAtomsPerBlock, n - a0);
-            for (int a = wid; a < na; a += 4) {
+            for (int a = wid; a < na; a += kSpreadThreads / 64) {
                 const R q = sQ[a];
                 if (fabs((double)q) <= kZeroPlus) continue;
                 const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
@@ -291,21 +294,36 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
             zoff[tz] = (size_t)gz * plane + gx;
         }
         double E = 0.0, Fy = 0.0, Fz = 0.0;
-        for (int ty = 0; ty < S; ty++) {
-            int gy = by + ty;
-            if (gy >= sp.K[1]) gy -= sp.K[1];
-            const size_t yoff = (size_t)gy * sp.K[0];
-            R se = (R)0, sz = (R)0;
+        // two y rows per iteration, their 2S grid values loaded into registers
+        // before any is used, so a lane has 2S loads in flight at once
+        for (int ty = 0; ty < S; ty += 2) {
+            int gy0 = by + ty, gy1 = by + ty + 1;
+            if (gy0 >= sp.K[1]) gy0 -= sp.K[1];
+            if (gy1 >= sp.K[1]) gy1 -= sp.K[1];
+            const size_t yo0 = (size_t)gy0 * sp.K[0], yo1 = (size_t)gy1 * sp.K[0];
+            R ph0[S], ph1[S];
 #pragma unroll
             for (int tz = 0; tz < S; tz++) {
-                const R phi = Phi[zoff[tz] + yoff];
-                se += sM[2][a][tz] * phi;
-                sz += sD[2][a][tz] * phi;
+                ph0[tz] = Phi[zoff[tz] + yo0];
+                ph1[tz] = Phi[zoff[tz] + yo1];
             }
-            const double my = (double)sM[1][a][ty], dy = (double)sD[1][a][ty];
-            E += my * (double)se;
-            Fy += dy * (double)se;
-            Fz += my * (double)sz;
+            R se0 = (R)0, sz0 = (R)0, se1 = (R)0, sz1 = (R)0;
+#pragma unroll
+            for (int tz = 0; tz < S; tz++) {
+                const R mz = sM[2][a][tz], dz = sD[2][a][tz];
+                se0 += mz * ph0[tz];
+                sz0 += dz * ph0[tz];
+                se1 += mz * ph1[tz];
+                sz1 += dz * ph1[tz];
+            }
+            const double my0 = (double)sM[1][a][ty], dy0 = (double)sD[1][a][ty];
+            const double my1 = (double)sM[1][a][ty + 1], dy1 = (double)sD[1][a][ty + 1];
+            E += my0 * (double)se0;
+            Fy += dy0 * (double)se0;
+            Fz += my0 * (double)sz0;
+            E += my1 * (double)se1;
+            Fy += dy1 * (double)se1;
+            Fz += my1 * (double)sz1;
         }
         const double mx = (double)sM[0][a][tx], dx = (double)sD[0][a][tx];
         e = mx * E;
@@ -419,7 +437,8 @@ int spme_setup(Ctx *c)
     }
     c->K1h = c->K[0] / 2 + 1;
     c->nreal = (size_t)c->K[0] * c->K[1] * c->K[2];
-    c->nhalf = (size_t)c->K1h * c->K[1] * c->K[2];
+    c->K1hp = (c->K1h + 15) / 16 * 16;  // row stride of the half spectrum (128-byte aligned runs)
+    c->nhalf = (size_t)c->K1hp * c->K[1] * c->K[2];
     const size_t rs = c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double);
     DLPX_HIP(c, hipMalloc(&c->Q, rs * c->nreal));
     DLPX_HIP(c, hipMalloc(&c->H, 2 * rs * c->nhalf));
@@ -463,7 +482,8 @@ static int spme_run(Ctx *c)
     switch (sp.S) {
 #define DLPX_SPREAD(SS)                                                                                    \
     case SS:                                                                                               \
-        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(256), 0, s, c->pos,      \
+        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(kSpreadThreads), 0, s,   \
+                           c->pos,                                                                         \
                            c->cell_start, c->ncell, sp, (R *)c->Q);                                        \
         break;
         DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
