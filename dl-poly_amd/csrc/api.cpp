@@ -301,6 +301,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     c->sp = c->stream;
     // lanes per atom of the pair kernel; DLPX_PAIR_LANES (4, 8, 16) overrides
     c->pair_lanes = 8;
+    // resident steps replay a captured graph unless DLPX_GRAPHS=0
+    if (const char *e = std::getenv("DLPX_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
@@ -349,6 +351,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (int k = 0; k < 8; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->recip_stream) (void)hipStreamDestroy(c->recip_stream);
@@ -557,6 +560,48 @@ int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, c
     return fetch_red(c);
 }
 
+// What a captured step graph bakes in (buffers, sizes, launch shapes, the
+// time step): a change of any of them forces a new capture.
+static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
+{
+    uint64_t t;
+    memcpy(&t, &tstep, sizeof(t));
+    return {(uint64_t)c->pos,      (uint64_t)c->vel,     (uint64_t)c->frc,     (uint64_t)c->frc_recip,
+            (uint64_t)c->xref,     (uint64_t)c->typ,     (uint64_t)c->nbr,     (uint64_t)c->cnt,
+            (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
+            (uint64_t)c->Q,        (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
+            (uint64_t)c->d_keytab, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
+            (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   t};
+}
+
+// Capture forces (both streams), stage 2 of this step and stage 1 of the
+// next into one executable graph.  Only called after an eager step, so every
+// buffer is sized and every kernel attribute set before the capture.
+static int capture_step_graph(Ctx *c, double tstep)
+{
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+    c->step_graph = nullptr;
+    DLPX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = evaluate(c);
+    if (!r) r = nve_second(c, tstep, true);
+    if (!r) r = nve_first(c, tstep);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, DLPX_ERR_HIP + (int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->step_graph = nullptr;
+        return fail(c, DLPX_ERR_HIP + (int)ei, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    c->graph_key = step_graph_key(c, tstep);
+    return 0;
+}
+
 int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, int32_t *nrebuilds)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -569,11 +614,14 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
         DLPX_HIP(c, hipMalloc((void **)&c->energies_dev, sizeof(double) * kReduceSlots * (size_t)nsteps));
         c->energies_cap = nsteps;
     }
+    // per-step energies and stage timing need host work between the kernels
+    // of a step; otherwise steps after the first replay one captured graph
+    const bool graphs = c->use_graphs && !energies && !c->timing;
     // displacement maximum and workgroup counter of the fused stage-1 kernel
     DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
+    // nve.F90:127-193 stage 1, refresh_mappings/vnl_check, forces, stage 2 (drivers.F90:2105-2199)
+    if (nsteps > 0 && (r = nve_first(c, tstep))) return r;
     for (int s = 0; s < nsteps; s++) {
-        // nve.F90:127-193 stage 1, refresh_mappings/vnl_check, forces, stage 2 (drivers.F90:2105-2199)
-        if ((r = nve_first(c, tstep))) return r;
         int update = 1;
         if (c->prm.padding > 0.0) {
             DLPX_HIP(c, hipStreamSynchronize(c->stream));
@@ -583,6 +631,14 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
             if ((r = build_list(c))) return r;
             rebuilds++;
         }
+        const bool last = s == nsteps - 1;
+        if (graphs && s > 0 && !last) {
+            // forces + stage 2 of this step + stage 1 of the next, as one graph
+            if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
+                if ((r = capture_step_graph(c, tstep))) return r;
+            DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
+            continue;
+        }
         if ((r = evaluate(c))) return r;
         if ((r = nve_second(c, tstep, true))) return r;
         if (c->timing && (r = collect_timing(c))) return r;
@@ -591,6 +647,7 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
             DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
                                        sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
         }
+        if (!last && (r = nve_first(c, tstep))) return r;
     }
     DLPX_HIP(c, hipStreamSynchronize(c->stream));
     if (energies && nsteps > 0) {

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -69,6 +70,11 @@ struct Ctx {
     int pair_lanes = kLanesPerAtom;  // lanes per atom in the pair kernel: 4, 8 or 16
     hipStream_t sp = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // resident steps replay one captured graph: forces, stage 2 and the next
+    // stage 1; graph_key records what the capture baked in (buffers, sizes)
+    bool use_graphs = true;
+    hipGraphExec_t step_graph = nullptr;
+    std::vector<uint64_t> graph_key;
     int device = 0;
 
     // box (orthorhombic)
@@ -166,6 +172,21 @@ int fail(Ctx *c, int code, const std::string &msg);
         if (e_ != hipSuccess)                                                          \
             return ::dlpx::fail((c), DLPX_ERR_HIP + (int)e_,                           \
                                 std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Let a kernel use up to the full 160 KB of dynamic LDS, once per device and
+// process (each launch still gets only what it asks for).  Done at a kernel's
+// first launch on a device, which is never inside a stream capture (captured
+// steps follow an eager one).
+#define DLPX_LDS_MAX(c, fn)                                                                                     \
+    do {                                                                                                        \
+        static std::atomic<uint64_t> done_{0};                                                                 \
+        const uint64_t bit_ = 1ull << ((c)->device & 63);                                                      \
+        if (!(done_.load(std::memory_order_acquire) & bit_)) {                                                 \
+            DLPX_HIP((c), hipFuncSetAttribute((const void *)(fn), hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                              160 * 1024));                                                    \
+            done_.fetch_or(bit_, std::memory_order_acq_rel);                                                   \
+        }                                                                                                       \
     } while (0)
 
 // host table generation (tables.cpp)

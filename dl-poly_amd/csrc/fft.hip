@@ -527,8 +527,7 @@ static int run_y(Ctx *c, int sign)
     const size_t lds = fft_lds<T>(lpb, K2, 0);
     const int nb = ceil_div(lm.nl, lpb);
 #define DLPX_Y(L)                                                                                                   \
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_c2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                                    (int)lds));                                                                     \
+    DLPX_LDS_MAX(c, (k_fft_c2c<T, L>));                                                                     \
     hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw, sign)
     DLPX_LPB_SWITCH(lpb, DLPX_Y)
 #undef DLPX_Y
@@ -568,8 +567,7 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
     const size_t lds = fft_lds<T>(lpb, K3, extra);
     const int nb = ceil_div(lm.nl, lpb);
 #define DLPX_Z(L)                                                                                                   \
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_z_conv<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                                    (int)lds));                                                                     \
+    DLPX_LDS_MAX(c, (k_fft_z_conv<T, L>));                                                                     \
     hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw,   \
                        ip, c->bsp_n2, c->red)
     DLPX_LPB_SWITCH(lpb, DLPX_Z)
@@ -593,16 +591,14 @@ static int run_x(Ctx *c, bool forward)
     const FastDiv dn = make_div(n), dh = make_div(c->K1h);
     if (forward) {
 #define DLPX_R2C(L)                                                                                                 \
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_r2c<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                    (int)lds));                                                                     \
+    DLPX_LDS_MAX(c, (k_fft_x_r2c<T, L>));                                                                     \
     hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)c->Q,           \
                        (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
     } else {
 #define DLPX_C2R(L)                                                                                                 \
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_fft_x_c2r<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,    \
-                                    (int)lds));                                                                     \
+    DLPX_LDS_MAX(c, (k_fft_x_c2r<T, L>));                                                                     \
     hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T2 *)c->H,          \
                        (T *)c->Q, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn)
         DLPX_LPB_SWITCH(lpb, DLPX_C2R)

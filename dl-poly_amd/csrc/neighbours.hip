@@ -167,6 +167,19 @@ __global__ void k_permute(const int32_t *perm, int n, const double4 *pos, double
     if (vel) vel_new[i] = vel[o];
 }
 
+// The permuted arrays back into the primary ones, so device pointers stay
+// the same across rebuilds (captured step graphs keep them as arguments).
+__global__ void k_copy_back(int n, const double4 *pos_alt, double4 *pos, const int32_t *typ_alt, int32_t *typ,
+                            const int32_t *gid_alt, int32_t *gid, const double4 *vel_alt, double4 *vel)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos[i] = pos_alt[i];
+    typ[i] = typ_alt[i];
+    gid[i] = gid_alt[i];
+    if (vel) vel[i] = vel_alt[i];
+}
+
 // One workgroup per link cell.  The atoms of the (up to) 27 stencil cells
 // are staged once in LDS as float4 {x + sx, y + sy, z + sz, j}, with the
 // cell's periodic image shift s L already added, and then every atom of the
@@ -401,10 +414,8 @@ int build_list(Ctx *c)
     hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, c->perm, n, c->pos, c->pos_alt, c->typ, c->typ_alt,
                        c->gid, c->gid_alt, c->resident ? c->vel : nullptr, c->resident ? c->vel_alt : nullptr,
                        c->slot, c->xref);
-    std::swap(c->pos, c->pos_alt);
-    std::swap(c->typ, c->typ_alt);
-    std::swap(c->gid, c->gid_alt);
-    if (c->resident) std::swap(c->vel, c->vel_alt);
+    hipLaunchKernelGGL(k_copy_back, dim3(nb), dim3(b), 0, s, n, c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt,
+                       c->gid, c->resident ? c->vel_alt : nullptr, c->resident ? c->vel : nullptr);
 
     constexpr int G = kLanesPerAtom;
     for (int attempt = 0; attempt < 2; attempt++) {

@@ -203,8 +203,7 @@ static int launch_pair(Ctx *c)
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
 #define DLPX_PAIR(GP)                                                                                        \
-    DLPX_HIP(c, hipFuncSetAttribute((const void *)k_pair<R, GP, kPairThreads>,                                  \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                     \
+    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads>));                                                            \
     hipLaunchKernelGGL((k_pair<R, GP, kPairThreads>), dim3(pair_blocks(c)), dim3(kPairThreads), lds, c->stream, \
                        c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,             \
                        (const R2 *)c->d_erfc, c->d_keytab, c->frc, c->red)

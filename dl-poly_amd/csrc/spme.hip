@@ -126,11 +126,16 @@ constexpr int kSlabBytes = 32768;
 // The window accumulates in FP64 at either precision: FP64 LDS atomic adds
 // (ds_add_f64) sustain several times the rate of FP32 ones (ds_add_f32) on
 // gfx950, and the FP32 grid then receives one rounding per window point.
-// 512 threads: one workgroup per link cell gives only ~730 workgroups at 64k
-// ions, so each needs many wavefronts to keep the LDS atomics busy.
-constexpr int kSpreadThreads = 512;
+// One workgroup per link cell gives only ~730 workgroups at 64k ions, so each
+// needs many wavefronts to keep the LDS atomics busy: 512 threads in FP32;
+// the FP64 build's register use makes 256 the better trade there (measured).
+template <typename R>
+constexpr int spread_threads()
+{
+    return sizeof(R) == 4 ? 512 : 256;
+}
 template <typename R, int S>
-__global__ void __launch_bounds__(kSpreadThreads) k_spread_cells(const double4 *__restrict__ pos,
+__global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const double4 *__restrict__ pos,
                                                       const int32_t *__restrict__ cell_start, int ncell,
                                                       SpmeParams sp, R *__restrict__ Q)
 {
@@ -186,7 +191,7 @@ __global__ void __launch_bounds__(kSpreadThreads) k_spread_cells(const double4 *
             }
             __syncthreads();
             const int na = min(kAtomsPerBlock, n - a0);
-            for (int a = wid; a < na; a += kSpreadThreads / 64) {
+            for (int a = wid; a < na; a += spread_threads<R>() / 64) {
                 const R q = sQ[a];
                 if (fabs((double)q) <= kZeroPlus) continue;
                 const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
@@ -225,6 +230,14 @@ __global__ void __launch_bounds__(kSpreadThreads) k_spread_cells(const double4 *
             unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], v);
         }
     }
+}
+
+// Grid-stride clear of n grid values.
+template <typename R>
+__global__ void k_zero(R *__restrict__ q, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        q[i] = (R)0;
 }
 
 // Lanes per atom in the gather: the next power of two >= S, lane tx < S
@@ -477,13 +490,15 @@ static int spme_run(Ctx *c)
     const SpmeParams sp = spme_params(c);
     const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
     int r;
-    // ewald_general.F90:517-576 charge spreading
-    DLPX_HIP(c, hipMemsetAsync(c->Q, 0, sizeof(R) * c->nreal, s));
+    // ewald_general.F90:517-576 charge spreading; the grid is cleared with a
+    // kernel (this sequence is also captured into step graphs)
+    hipLaunchKernelGGL(k_zero<R>, dim3(std::min<size_t>((c->nreal + 1023) / 1024, 4096)), dim3(256), 0, s, (R *)c->Q,
+                       c->nreal);
     switch (sp.S) {
 #define DLPX_SPREAD(SS)                                                                                    \
     case SS:                                                                                               \
-        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(kSpreadThreads), 0, s,   \
-                           c->pos,                                                                         \
+        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(spread_threads<R>()), 0, \
+                           s, c->pos,                                                                      \
                            c->cell_start, c->ncell, sp, (R *)c->Q);                                        \
         break;
         DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)

@@ -214,6 +214,48 @@ def test_full_size_1728k_against_strided_oracle_fixture():
     ctx.close()
 
 
+def _chunked_md(s, precision, graphs, chunks=3, steps=12):
+    """Resident NVE in chunks; state after each chunk.  DLPX_GRAPHS is read
+    when the context is created."""
+    import os
+    old = os.environ.get("DLPX_GRAPHS")
+    os.environ["DLPX_GRAPHS"] = "1" if graphs else "0"
+    try:
+        ctx = lib.Context(s, precision=precision)
+    finally:
+        if old is None:
+            del os.environ["DLPX_GRAPHS"]
+        else:
+            os.environ["DLPX_GRAPHS"] = old
+    ctx.md_load(s.pos, s.vel)
+    out = []
+    for _ in range(chunks):
+        _, nreb = ctx.md_run(steps, s.timestep, energies=False)
+        out.append(ctx.md_store() + (nreb,))
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [(lib.FP64, 1e-9), (lib.MIXED, 1e-5)])
+def test_graph_replayed_steps_match_eager_steps(precision, tol):
+    """Steps 2..n-1 of each dlpx_md_run replay one captured graph; every chunk
+    (several replays, with list rebuilds and host synchronisation between
+    them) must agree with the eager path.  The only difference is the order of
+    the charge-spreading atomics."""
+    s, _ = fixtures.load("nacl_4096_k48")
+    print("case graphs vs eager, precision", precision, flush=True)
+    eager = _chunked_md(s, precision, graphs=False)
+    graph = _chunked_md(s, precision, graphs=True)
+    L = np.diag(s.cell.reshape(3, 3))
+    assert sum(c[3] for c in eager) > 0  # the run crosses list rebuilds
+    for (xe, ve, fe, ne), (xg, vg, fg, ng) in zip(eager, graph):
+        assert ne == ng
+        dx = xg - xe
+        dx -= L * np.round(dx / L)
+        assert np.abs(dx).max() <= tol * L.max()
+        assert rms_rel(vg, ve) <= tol and rms_rel(fg, fe) <= 10 * tol
+
+
 def test_nve_energy_conservation_fp64_and_mixed():
     s = systems.nacl(ncells=8, kvec=(48, 48, 48), seed=3)
     for prec, tol in ((lib.FP64, 2e-5), (lib.MIXED, 5e-5)):
