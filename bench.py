@@ -94,6 +94,83 @@ def workload(name):
     return s, desc
 
 
+def ensemble_systems(nsys=256, seed=20261202):
+    """The batched-ensemble workload: nsys systems drawn from a seed, half LJ
+    argon and half molten NaCl (shuffled), sizes uniform in 1,000-20,000 atoms
+    rounded to whole lattices (4 m^3 argon, 8 m^3 NaCl), cubic boxes at the
+    densities of the single-system workloads, NaCl meshes the smallest
+    2^a 3^b 5^c size of at least one point per A, clamped to 32..64."""
+    rng = np.random.default_rng(seed)
+    kinds = ["argon"] * (nsys // 2) + ["nacl"] * (nsys - nsys // 2)
+    rng.shuffle(kinds)
+    out = []
+    for i, kind in enumerate(kinds):
+        n = int(rng.integers(1000, 20001))
+        if kind == "argon":
+            m = max(4, int(round((n / 4.0) ** (1.0 / 3.0))))
+            out.append(systems.argon(ncells=m, seed=seed + 1 + i))
+        else:
+            m = max(5, int(round((n / 8.0) ** (1.0 / 3.0))))
+            L = m * 5.64 * 1.08
+            K = min(64, max(32, systems.smooth235_at_least(int(math.ceil(L)))))
+            out.append(systems.nacl(ncells=m, kvec=(K, K, K), seed=seed + 1 + i))
+    return out
+
+
+def run_ensemble(args):
+    """Mixed-precision ensemble (dlpx_md_run_ensemble over all systems) against
+    the per-system FP64 loop (dlpx_md_run one system after another)."""
+    syss = ensemble_systems()
+    natoms = sum(s.natms for s in syss)
+    log("ensemble: %d systems, %d atoms (%d argon, %d NaCl)" % (len(syss), natoms,
+        sum(1 for s in syss if not s.electrostatics), sum(1 for s in syss if s.electrostatics)))
+    # per-system FP64 loop
+    t64 = 0.0
+    for s in syss:
+        ctx = lib.Context(s, precision=lib.FP64)
+        ctx.md_load(s.pos, s.vel)
+        ctx.md_run(args.warmup, s.timestep, energies=False)
+        t0 = time.perf_counter()
+        ctx.md_run(args.steps, s.timestep, energies=False)
+        t64 += time.perf_counter() - t0
+        ctx.close()
+    # mixed ensemble
+    ctxs = []
+    for s in syss:
+        ctx = lib.Context(s, precision=lib.MIXED)
+        ctx.md_load(s.pos, s.vel)
+        ctxs.append(ctx)
+    lib.md_run_ensemble(ctxs, args.warmup)
+    t0 = time.perf_counter()
+    nreb = lib.md_run_ensemble(ctxs, args.steps)
+    tmx = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    rate_mx, rate_64 = natoms * args.steps / tmx, natoms * args.steps / t64
+    return {
+        "metric": METRIC + " (batched ensemble)",
+        "value": rate_mx,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulation)",
+        "data": "synthetic (seeded generator: %d systems, %d atoms)" % (len(syss), natoms),
+        "config": {"workload": "ensemble of 256 systems, half LJ argon and half molten NaCl, 1,000-20,000 "
+                               "atoms each, NaCl meshes 32^3-64^3, order 8; all steps of all systems",
+                   "systems": len(syss), "atoms": natoms},
+        "fp64_per_system_loop": {"value": rate_64, "ms_per_step": t64 / args.steps * 1e3},
+        "mixed_ensemble_over_fp64_loop": rate_mx / rate_64,
+        "list_rebuilds": int(nreb.sum()),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def pairs_within(ctx, s, r, pos):
     """Unordered pairs with separation < r (from the GPU list, measured on the host)."""
     p = ctx.get_pairs()
@@ -253,6 +330,9 @@ def main():
         print("bench.py measures one GPU: this build runs the force path on a single MI355X "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble256":
+        print(json.dumps(run_ensemble(args)), flush=True)
+        return
     s, desc = workload(args.workload)
     log("workload:", desc)
 

@@ -602,6 +602,52 @@ static int capture_step_graph(Ctx *c, double tstep)
     return 0;
 }
 
+// Start of a resident run: clear the counters of the fused stage-1 kernel
+// and do stage 1 of the first step.
+static int md_begin(Ctx *c, int nsteps, double tstep)
+{
+    // displacement maximum and workgroup counter of the fused stage-1 kernel
+    DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
+    return nsteps > 0 ? nve_first(c, tstep) : 0;
+}
+
+// Step s of a resident run (nve.F90:127-215 with refresh_mappings/vnl_check
+// and the force call, drivers.F90:2105-2199): wait for this context's stage 1,
+// rebuild the list if an atom moved by padding/2, then forces, stage 2 and
+// the next step's stage 1.  With graphs, steps 2..n-1 replay one captured
+// graph.  Returns with the step's work queued on c->stream.
+static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool energies, int &rebuilds)
+{
+    int r;
+    int update = 1;
+    if (c->prm.padding > 0.0) {
+        DLPX_HIP(c, hipStreamSynchronize(c->stream));
+        update = nve_first_displacement(c) >= kHalfMinus * c->prm.padding ? 1 : 0;
+    }
+    if (update) {
+        if ((r = build_list(c))) return r;
+        rebuilds++;
+    }
+    const bool last = s == nsteps - 1;
+    if (graphs && s > 0 && !last) {
+        // forces + stage 2 of this step + stage 1 of the next, as one graph
+        if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
+            if ((r = capture_step_graph(c, tstep))) return r;
+        DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
+        return 0;
+    }
+    if ((r = evaluate(c))) return r;
+    if ((r = nve_second(c, tstep, true))) return r;
+    if (c->timing && (r = collect_timing(c))) return r;
+    if (energies) {
+        if ((r = reduce_partials(c))) return r;
+        DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
+                                   sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (!last && (r = nve_first(c, tstep))) return r;
+    return 0;
+}
+
 int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, int32_t *nrebuilds)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -617,38 +663,9 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
     // per-step energies and stage timing need host work between the kernels
     // of a step; otherwise steps after the first replay one captured graph
     const bool graphs = c->use_graphs && !energies && !c->timing;
-    // displacement maximum and workgroup counter of the fused stage-1 kernel
-    DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
-    // nve.F90:127-193 stage 1, refresh_mappings/vnl_check, forces, stage 2 (drivers.F90:2105-2199)
-    if (nsteps > 0 && (r = nve_first(c, tstep))) return r;
-    for (int s = 0; s < nsteps; s++) {
-        int update = 1;
-        if (c->prm.padding > 0.0) {
-            DLPX_HIP(c, hipStreamSynchronize(c->stream));
-            update = nve_first_displacement(c) >= kHalfMinus * c->prm.padding ? 1 : 0;
-        }
-        if (update) {
-            if ((r = build_list(c))) return r;
-            rebuilds++;
-        }
-        const bool last = s == nsteps - 1;
-        if (graphs && s > 0 && !last) {
-            // forces + stage 2 of this step + stage 1 of the next, as one graph
-            if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
-                if ((r = capture_step_graph(c, tstep))) return r;
-            DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
-            continue;
-        }
-        if ((r = evaluate(c))) return r;
-        if ((r = nve_second(c, tstep, true))) return r;
-        if (c->timing && (r = collect_timing(c))) return r;
-        if (energies) {
-            if ((r = reduce_partials(c))) return r;
-            DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
-                                       sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
-        }
-        if (!last && (r = nve_first(c, tstep))) return r;
-    }
+    if ((r = md_begin(c, nsteps, tstep))) return r;
+    for (int s = 0; s < nsteps; s++)
+        if ((r = md_step(c, s, nsteps, tstep, graphs, energies != nullptr, rebuilds))) return r;
     DLPX_HIP(c, hipStreamSynchronize(c->stream));
     if (energies && nsteps > 0) {
         std::vector<double> red((size_t)kReduceSlots * nsteps);
@@ -665,6 +682,39 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
         }
     }
     if (nrebuilds) *nrebuilds = rebuilds;
+    return 0;
+}
+
+int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
+                         int32_t *nrebuilds)
+{
+    if (!ctxs || nctx < 0 || nsteps < 0 || !tsteps) return DLPX_ERR_BAD_ARG;
+    std::vector<Ctx *> cs(nctx);
+    for (int k = 0; k < nctx; k++) {
+        cs[k] = reinterpret_cast<Ctx *>(ctxs[k]);
+        if (!cs[k]) return DLPX_ERR_BAD_ARG;
+        if (!cs[k]->resident || !cs[k]->have_list) return fail(cs[k], DLPX_ERR_STATE, "call dlpx_md_load first");
+        if (cs[k]->device != cs[0]->device) return fail(cs[k], DLPX_ERR_BAD_ARG, "ensemble contexts on different devices");
+        for (int j = 0; j < k; j++)
+            if (cs[j] == cs[k]) return fail(cs[k], DLPX_ERR_BAD_ARG, "a context appears twice in the ensemble");
+    }
+    std::vector<int> rebuilds(nctx, 0);
+    int r;
+    for (int k = 0; k < nctx; k++)
+        if ((r = md_begin(cs[k], nsteps, tsteps[k]))) return r;
+    // lockstep over systems: while the host waits for one system's stage 1,
+    // the steps queued for the others run on the GPU
+    for (int s = 0; s < nsteps; s++)
+        for (int k = 0; k < nctx; k++) {
+            Ctx *c = cs[k];
+            const bool graphs = c->use_graphs && !c->timing;
+            if ((r = md_step(c, s, nsteps, tsteps[k], graphs, false, rebuilds[k]))) return r;
+        }
+    for (int k = 0; k < nctx; k++) {
+        Ctx *c = cs[k];
+        DLPX_HIP(c, hipStreamSynchronize(c->stream));
+        if (nrebuilds) nrebuilds[k] = rebuilds[k];
+    }
     return 0;
 }
 

@@ -22,8 +22,8 @@ FP64, MIXED = 0, 1
 EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dlpx_set_vdw_tables",
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
-           "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_store", "dlpx_set_timing",
-           "dlpx_get_timing"]
+           "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
+           "dlpx_set_timing", "dlpx_get_timing"]
 
 
 class DlpxError(RuntimeError):
@@ -230,3 +230,23 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def md_run_ensemble(contexts, nsteps, tsteps=None):
+    """dlpx_md_run_ensemble: nsteps resident NVE steps of every context (each
+    loaded with md_load), stepped in lockstep so different systems overlap
+    on the GPU.  tsteps defaults to each system's own time step.  Returns the
+    list rebuilds per system."""
+    n = len(contexts)
+    if n == 0:
+        return np.zeros(0, np.int32)
+    ts = np.ascontiguousarray([c.sys.timestep for c in contexts] if tsteps is None else tsteps, np.float64)
+    hs = (C.c_void_p * n)(*[c.h for c in contexts])
+    nreb = np.zeros(n, np.int32)
+    lib = contexts[0].lib
+    rc = lib.dlpx_md_run_ensemble(hs, C.c_int32(n), C.c_int32(nsteps), _ptr(ts), _ptr(nreb))
+    if rc != 0:
+        bad = [c for c in contexts if c.lib.dlpx_last_error(c.h)]
+        msg = lib.dlpx_last_error(bad[0].h).decode() if bad else "dlpx_md_run_ensemble failed"
+        raise DlpxError(rc, msg)
+    return nreb

@@ -140,6 +140,16 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
 int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                   double *fx, double *fy, double *fz, int32_t stride);
 
+/* Batched ensemble: the same as dlpx_md_run(ctxs[k], nsteps, tsteps[k], NULL,
+ * &nrebuilds[k]) for every k, with the systems stepped in lockstep so that
+ * the steps of different systems run on the GPU at the same time (each
+ * context has its own streams).  Every context must have been loaded with
+ * dlpx_md_load, be on the same device and appear once.  No counterpart in
+ * DL_POLY, which runs one system per process: it serves hosts that run many
+ * small independent systems. */
+int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
+                         int32_t *nrebuilds);
+
 /* Per-stage device time (ms, HIP events on the context's stream), summed over
  * the force evaluations since dlpx_set_timing(ctx, 1): ms[0] pair forces,
  * ms[1] B-spline charge spreading, ms[2] FFT + influence function + inverse

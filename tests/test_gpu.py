@@ -256,6 +256,38 @@ def test_graph_replayed_steps_match_eager_steps(precision, tol):
         assert rms_rel(vg, ve) <= tol and rms_rel(fg, fe) <= 10 * tol
 
 
+def test_ensemble_matches_separate_runs():
+    """dlpx_md_run_ensemble over systems of different kinds and sizes gives
+    each system's own dlpx_md_run (up to the order of the spreading atomics)."""
+    syss = [systems.argon(ncells=6, seed=5), systems.nacl(ncells=5, kvec=(32, 32, 32), seed=6),
+            systems.nacl(ncells=6, kvec=(40, 40, 40), seed=7)]
+    print("case ensemble of", [s.natms for s in syss], flush=True)
+    steps = 15
+    ref = []
+    for s in syss:
+        ctx = lib.Context(s)
+        ctx.md_load(s.pos, s.vel)
+        _, nreb = ctx.md_run(steps, s.timestep, energies=False)
+        ref.append(ctx.md_store() + (nreb,))
+        ctx.close()
+    ctxs = []
+    for s in syss:
+        ctx = lib.Context(s)
+        ctx.md_load(s.pos, s.vel)
+        ctxs.append(ctx)
+    nreb = lib.md_run_ensemble(ctxs, steps)
+    for k, (s, ctx) in enumerate(zip(syss, ctxs)):
+        x, v, f = ctx.md_store()
+        xr, vr, fr, nr = ref[k]
+        L = np.diag(s.cell.reshape(3, 3))
+        dx = x - xr
+        dx -= L * np.round(dx / L)
+        assert nreb[k] == nr
+        assert np.abs(dx).max() <= 1e-9 * L.max()
+        assert rms_rel(v, vr) <= 1e-9 and rms_rel(f, fr) <= 1e-8
+        ctx.close()
+
+
 def test_nve_energy_conservation_fp64_and_mixed():
     s = systems.nacl(ncells=8, kvec=(48, 48, 48), seed=3)
     for prec, tol in ((lib.FP64, 2e-5), (lib.MIXED, 5e-5)):
