@@ -100,13 +100,14 @@ int upload_tables(Ctx *c)
 
 // host arrays (strided) -> pinned staging -> device staging -> cell order
 static int upload(Ctx *c, const dlpx_atoms *a, bool identity, const double *vx, const double *vy, const double *vz,
-                  const double *mass)
+                  const double *mass, int vstride = 0)
 {
     if (!a || !a->x || !a->y || !a->z || !a->chge || !a->ltype) return fail(c, DLPX_ERR_BAD_ARG, "atom arrays");
     if (a->natms <= 0 || a->natms > c->cap) return fail(c, DLPX_ERR_BAD_ARG, "natms outside 1..natms_max");
     if (!identity && a->natms != c->natms) return fail(c, DLPX_ERR_STATE, "natms changed since the list was built");
-    const int n = a->natms, st = a->stride;
+    const int n = a->natms, st = a->stride, vst = vstride > 0 ? vstride : a->stride;
     auto at = [st](const double *p, int i) { return *(const double *)((const char *)p + (size_t)st * i); };
+    auto atv = [vst](const double *p, int i) { return *(const double *)((const char *)p + (size_t)vst * i); };
     for (int i = 0; i < n; i++) {
         double *r = c->h_st + (size_t)8 * i;
         r[0] = at(a->x, i);
@@ -114,10 +115,10 @@ static int upload(Ctx *c, const dlpx_atoms *a, bool identity, const double *vx, 
         r[2] = at(a->z, i);
         r[3] = at(a->chge, i);
         if (vx) {
-            r[4] = at(vx, i);
-            r[5] = at(vy, i);
-            r[6] = at(vz, i);
-            r[7] = at(mass, i);
+            r[4] = atv(vx, i);
+            r[5] = atv(vy, i);
+            r[6] = atv(vz, i);
+            r[7] = atv(mass, i);
         }
         const int t = a->ltype[i];
         if (t < 1 || t > c->prm.ntypes) return fail(c, DLPX_ERR_BAD_ARG, "ltype outside 1..ntypes");
@@ -545,14 +546,14 @@ int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
 }
 
 int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, const double *vx, const double *vy,
-                 const double *vz, const double *mass)
+                 const double *vz, const double *mass, int32_t vstride)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c || !vx || !vy || !vz || !mass) return DLPX_ERR_BAD_ARG;
+    if (!c || !vx || !vy || !vz || !mass || vstride < 0) return DLPX_ERR_BAD_ARG;
     int r;
     if ((r = set_cell(c, cell))) return r;
     c->resident = true;
-    if ((r = upload(c, atoms, true, vx, vy, vz, mass))) return r;
+    if ((r = upload(c, atoms, true, vx, vy, vz, mass, vstride))) return r;
     charges_summary(c, atoms);
     if ((r = build_list(c))) return r;
     // forces at the starting configuration (md_vv computes them before step 1)
@@ -719,10 +720,11 @@ int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, co
 }
 
 int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *fx,
-                  double *fy, double *fz, int32_t stride)
+                  double *fy, double *fz, int32_t stride, int32_t vstride)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
-    if (!c) return DLPX_ERR_BAD_ARG;
+    if (!c || stride < 0 || vstride < 0) return DLPX_ERR_BAD_ARG;
+    if (!c->resident) return fail(c, DLPX_ERR_STATE, "call dlpx_md_load first");
     int r;
     if ((r = store_atoms(c, c->stage, true, true))) return r;
     const int n = c->natms;
@@ -731,7 +733,8 @@ int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, do
     double *dst[9] = {x, y, z, vx, vy, vz, fx, fy, fz};
     for (int k = 0; k < 9; k++) {
         if (!dst[k]) continue;
-        for (int i = 0; i < n; i++) *(double *)((char *)dst[k] + (size_t)stride * i) = c->h_st[(size_t)9 * i + k];
+        const size_t st = (k >= 3 && k < 6 && vstride > 0) ? vstride : stride;
+        for (int i = 0; i < n; i++) *(double *)((char *)dst[k] + st * i) = c->h_st[(size_t)9 * i + k];
     }
     return 0;
 }

@@ -196,7 +196,8 @@ class Context:
         m[:, 0] = self.sys.mass
         self._keep += [v, m]
         self._chk(self.lib.dlpx_md_load(self.h, _ptr(self.cell), C.byref(a), _ptr(v),
-                                        C.c_void_p(v.ctypes.data + 8), C.c_void_p(v.ctypes.data + 16), _ptr(m)))
+                                        C.c_void_p(v.ctypes.data + 8), C.c_void_p(v.ctypes.data + 16), _ptr(m),
+                                        C.c_int32(24)))
 
     def md_run(self, nsteps, tstep, energies=True):
         e = np.zeros((nsteps, 5)) if energies else None
@@ -209,7 +210,7 @@ class Context:
         out = np.zeros((self.n, 9))
         b = out.ctypes.data
         ptrs = [C.c_void_p(b + 8 * k) for k in range(9)]
-        self._chk(self.lib.dlpx_md_store(self.h, *ptrs, C.c_int32(72)))
+        self._chk(self.lib.dlpx_md_store(self.h, *ptrs, C.c_int32(72), C.c_int32(0)))
         return out[:, 0:3].copy(), out[:, 3:6].copy(), out[:, 6:9].copy()
 
     def set_timing(self, on=True):

@@ -47,7 +47,8 @@ Module dlpx_binding
 
   Public :: dlpx_create, dlpx_destroy, dlpx_version, dlpx_set_vdw_tables, dlpx_set_erfc_tables, &
             dlpx_generate_vdw_tables, dlpx_generate_erfc_tables, dlpx_set_lrc, dlpx_compute_lrc, &
-            dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_error_message
+            dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_md_load, &
+            dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_error_message
 
   Interface
     Integer(c_int) Function dlpx_create(params, ctx) Bind(C, name='dlpx_create')
@@ -149,6 +150,48 @@ Module dlpx_binding
       Type(dlpx_atoms), Intent(In) :: atoms
       Type(dlpx_two_body_out), Intent(Out) :: out
     End Function dlpx_two_body_forces
+
+    !> Resident mode (nve.F90:127-215 steps on the device).  vx, vy, vz and
+    !> mass are read with byte stride vstride (0 = the atoms' stride), e.g.
+    !> c_loc(config%vxx(1)) ... c_loc(config%weight(1)) with vstride 8.
+    Integer(c_int) Function dlpx_md_load(ctx, cell, atoms, vx, vy, vz, mass, vstride) Bind(C, name='dlpx_md_load')
+      Import :: c_int, c_int32_t, c_ptr, c_double, dlpx_atoms
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(In) :: cell(9)
+      Type(dlpx_atoms), Intent(In) :: atoms
+      Type(c_ptr), Value :: vx, vy, vz, mass
+      Integer(c_int32_t), Value :: vstride
+    End Function dlpx_md_load
+
+    !> energies: c_null_ptr, or c_loc(e(1,1)) of e(5, nsteps) (engcpe, vircpe,
+    !> engsrp, virsrp, kinetic energy per step)
+    Integer(c_int) Function dlpx_md_run(ctx, nsteps, tstep, energies, nrebuilds) Bind(C, name='dlpx_md_run')
+      Import :: c_int, c_int32_t, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Value :: nsteps
+      Real(c_double), Value :: tstep
+      Type(c_ptr), Value :: energies
+      Integer(c_int32_t), Intent(Out) :: nrebuilds
+    End Function dlpx_md_run
+
+    !> positions and forces written with byte stride stride, velocities with
+    !> vstride (0 = stride); any pointer may be c_null_ptr
+    Integer(c_int) Function dlpx_md_store(ctx, x, y, z, vx, vy, vz, fx, fy, fz, stride, vstride) &
+        Bind(C, name='dlpx_md_store')
+      Import :: c_int, c_int32_t, c_ptr
+      Type(c_ptr), Value :: ctx, x, y, z, vx, vy, vz, fx, fy, fz
+      Integer(c_int32_t), Value :: stride, vstride
+    End Function dlpx_md_store
+
+    !> dlpx_md_run over many independent contexts in lockstep
+    Integer(c_int) Function dlpx_md_run_ensemble(ctxs, nctx, nsteps, tsteps, nrebuilds) &
+        Bind(C, name='dlpx_md_run_ensemble')
+      Import :: c_int, c_int32_t, c_ptr, c_double
+      Type(c_ptr), Intent(In) :: ctxs(*)
+      Integer(c_int32_t), Value :: nctx, nsteps
+      Real(c_double), Intent(In) :: tsteps(*)
+      Integer(c_int32_t), Intent(Out) :: nrebuilds(*)
+    End Function dlpx_md_run_ensemble
   End Interface
 
 Contains

@@ -7,9 +7,14 @@
 !> deterministic integer-arithmetic displacement pattern, Tosi-Fumi BHM + SPME
 !> (rc 12 A, padding 0.5 A, alpha 0.28, K = 32^3, order 8).
 !>
+!> Then 10 resident NVE steps (dt 2 fs) from those positions, with velocities
+!> vx_n,d = 0.02*(((53 n + 7 d) mod 97)/96 - 1/2) A/ps and the Na/Cl masses
+!> held in separate arrays (DL_POLY's config%vxx/.../config%weight layout).
+!>
 !> Usage: dlpx_host_demo <precision 0|1> <output file>
 !> Output (stream, little-endian doubles): engcpe, vircpe, engsrp, virsrp,
-!> stress(9), buffer(0:19), then fxx, fyy, fzz of every atom.
+!> stress(9), buffer(0:19), then fxx, fyy, fzz of every atom; then after the
+!> resident steps xxx, yyy, zzz | vx, vy, vz | fxx, fyy, fzz of every atom.
 Program dlpx_host_demo
   Use, Intrinsic :: iso_c_binding
   Use dlpx_binding
@@ -26,6 +31,8 @@ Program dlpx_host_demo
   Integer, Parameter :: ncells = 5, natms = 8 * ncells**3
   Type(corePart), Allocatable, Target :: parts(:)
   Integer(c_int32_t), Allocatable, Target :: ltype(:)
+  Real(c_double), Allocatable, Target :: vx(:), vy(:), vz(:), weight(:)
+  Integer(c_int32_t) :: nreb
   Real(c_double), Target :: cell(9)
   Real(c_double) :: a, L, basis(3, 4), params(5, 3), num_type(2)
   Integer(c_int32_t) :: kinds(3), ibig
@@ -119,6 +126,26 @@ Program dlpx_host_demo
 
   Open (Newunit=u, File=Trim(outfile), Access='stream', Form='unformatted', Status='replace')
   Write (u) out%engcpe, out%vircpe, out%engsrp, out%virsrp, out%stress, out%buffer
+  Write (u) (parts(n)%fxx, parts(n)%fyy, parts(n)%fzz, n=1, natms)
+
+  ! resident steps: velocities and masses in their own arrays (stride 8)
+  Allocate (vx(natms), vy(natms), vz(natms), weight(natms))
+  Do n = 1, natms
+    vx(n) = 0.02_c_double * (Real(Mod(53 * n + 7, 97), c_double) / 96.0_c_double - 0.5_c_double)
+    vy(n) = 0.02_c_double * (Real(Mod(53 * n + 14, 97), c_double) / 96.0_c_double - 0.5_c_double)
+    vz(n) = 0.02_c_double * (Real(Mod(53 * n + 21, 97), c_double) / 96.0_c_double - 0.5_c_double)
+    weight(n) = Merge(22.98977_c_double, 35.453_c_double, ltype(n) == 1)
+  End Do
+  Call check(dlpx_md_load(ctx, cell, atoms, c_loc(vx(1)), c_loc(vy(1)), c_loc(vz(1)), c_loc(weight(1)), &
+                          Int(c_sizeof(vx(1)), c_int32_t)))
+  Call check(dlpx_md_run(ctx, 10_c_int32_t, 0.002_c_double, c_null_ptr, nreb))
+  Call check(dlpx_md_store(ctx, c_loc(parts(1)%xxx), c_loc(parts(1)%yyy), c_loc(parts(1)%zzz), &
+                           c_loc(vx(1)), c_loc(vy(1)), c_loc(vz(1)), &
+                           c_loc(parts(1)%fxx), c_loc(parts(1)%fyy), c_loc(parts(1)%fzz), &
+                           atoms%stride, Int(c_sizeof(vx(1)), c_int32_t)))
+  Write (*, '(a,i0)') 'resident steps done, list rebuilds ', nreb
+  Write (u) (parts(n)%xxx, parts(n)%yyy, parts(n)%zzz, n=1, natms)
+  Write (u) (vx(n), vy(n), vz(n), n=1, natms)
   Write (u) (parts(n)%fxx, parts(n)%fyy, parts(n)%fzz, n=1, natms)
   Close (u)
   ierr = dlpx_destroy(ctx)

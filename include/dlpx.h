@@ -127,18 +127,21 @@ int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
                          dlpx_two_body_out *out);
 
 /* Resident mode: positions, velocities and forces stay in HBM between steps.
- * dlpx_md_load copies a configuration in (vx/vy/vz and mass with the same
- * stride as atoms); dlpx_md_run performs nsteps velocity-Verlet NVE steps
+ * dlpx_md_load copies a configuration in: vx/vy/vz and mass are read with
+ * byte stride vstride (0 = atoms->stride), so DL_POLY's config%vxx/vyy/vzz and
+ * config%weight (stride 8) go in beside corePart positions (stride 64).
+ * dlpx_md_run performs nsteps velocity-Verlet NVE steps
  * (nve.F90:127-215) with the displacement-triggered list rebuild and the
  * two-body forces of every step; when energies is not NULL, step s writes
  * energies[5*s + {0..4}] = engcpe, vircpe, engsrp, virsrp (as in
  * dlpx_two_body_out) and the kinetic energy.  dlpx_md_store copies positions,
- * velocities and forces back (any pointer may be NULL). */
+ * velocities and forces back (any pointer may be NULL): positions and forces
+ * with byte stride stride, velocities with vstride (0 = stride). */
 int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, const double *vx,
-                 const double *vy, const double *vz, const double *mass);
+                 const double *vy, const double *vz, const double *mass, int32_t vstride);
 int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, int32_t *nrebuilds);
 int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
-                  double *fx, double *fy, double *fz, int32_t stride);
+                  double *fx, double *fy, double *fz, int32_t stride, int32_t vstride);
 
 /* Batched ensemble: the same as dlpx_md_run(ctxs[k], nsteps, tsteps[k], NULL,
  * &nrebuilds[k]) for every k, with the systems stepped in lockstep so that

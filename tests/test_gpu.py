@@ -341,8 +341,11 @@ def test_fortran_host_through_iso_c_binding(tmp_path, precision):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     raw = np.fromfile(out, dtype="<f8")
-    eng, stress, buf, frc = raw[:4], raw[4:13], raw[13:33], raw[33:].reshape(-1, 3)
     s = _fortran_demo_system()
+    n = s.natms
+    eng, stress, buf = raw[:4], raw[4:13], raw[13:33]
+    frc = raw[33:33 + 3 * n].reshape(-1, 3)
+    md_x, md_v, md_f = (raw[33 + 3 * n * (k + 1):33 + 3 * n * (k + 2)].reshape(-1, 3) for k in range(3))
     ctx, py = gpu_two_body(s, precision=precision)
     # same inputs through two hosts; the only difference is the order of the
     # atomic adds of charge spreading (FP64: ~1e-16, FP32 mixed grid: ~1e-7)
@@ -355,3 +358,19 @@ def test_fortran_host_through_iso_c_binding(tmp_path, precision):
         o = oracle.two_body(s)
         assert rms_rel(frc, o["frc"]) <= 1e-10
     ctx.close()
+    # the resident steps, with velocities and masses passed in their own arrays
+    vel = np.zeros((n, 3))
+    for i in range(1, n + 1):
+        for d in range(3):
+            vel[i - 1, d] = 0.02 * (((53 * i + 7 * (d + 1)) % 97) / 96.0 - 0.5)
+    ctx = lib.Context(s, precision=precision)
+    ctx.md_load(s.pos, vel)
+    ctx.md_run(10, 0.002, energies=False)
+    x, v, f = ctx.md_store()
+    ctx.close()
+    L = np.diag(s.cell.reshape(3, 3))
+    dx = md_x - x
+    dx -= L * np.round(dx / L)
+    tol_md = 1e-10 if precision == 0 else 1e-5
+    assert np.abs(dx).max() <= tol_md * L.max()
+    assert rms_rel(md_v, v) <= tol_md and rms_rel(md_f, f) <= 10 * tol_md
