@@ -42,6 +42,24 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds, double *
     }
 }
 
+// floor(x / d) and the remainder x - d floor(x / d) in [0, d), for |x| < 2^20,
+// from a float reciprocal inv = 1/d: the estimate is within one of the
+// truth and one correction step fixes it (integer division by a runtime
+// value costs tens of instructions on this hardware).
+__device__ __forceinline__ int fdiv_floor(int x, int d, float inv, int &r)
+{
+    int q = (int)floorf((float)x * inv);
+    r = x - q * d;
+    if (r < 0) {
+        r += d;
+        q--;
+    } else if (r >= d) {
+        r -= d;
+        q++;
+    }
+    return q;
+}
+
 // r_ij = x_i - (x_j + s L) with s = rint((x_i - x_j)/L) per axis.
 __device__ __forceinline__ double min_image_1(double xi, double xj, double L, double invL)
 {

@@ -285,24 +285,37 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
                 __syncthreads();
             }
             if (!active) continue;
-            for (int k = lane; k < m; k += G) {
-                const float4 e = cand[k];
-                const int j = __float_as_int(e.w);
-                if (j == i) continue;
-                bool in;
-                if (small) {
+            if (small) {
+                // 1-2 cells per axis: every candidate decided by exact minimum image
+                for (int k = lane; k < m; k += G) {
+                    const int j = __float_as_int(cand[k].w);
+                    if (j == i) continue;
                     const double4 pj = pos[j];
                     const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
                     const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
                     const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
                     const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-                    in = rsq <= rc2;
-                } else {
-                    const float ex = pfx - e.x, ey = pfy - e.y, ez = pfz - e.z;
-                    const float r2f = ex * ex + ey * ey + ez * ez;
-                    if (r2f > rc2_hi) continue;
-                    in = r2f < rc2_lo;
-                    if (!in) {
+                    if (rsq <= rc2) {
+                        if (count < cap_lane) nbr[list_slot(i, count, lane, cap_lane)] = j;
+                        count++;
+                    }
+                }
+                continue;
+            }
+            // the accept test is predicated rather than branched (divergent
+            // branches here cost more scalar than vector instructions), and
+            // the rare band of pairs within 2e-5 of the cutoff is settled in
+            // FP64 behind a wavefront-uniform test
+#pragma unroll 2
+            for (int k = lane; k < m; k += G) {
+                const float4 e = cand[k];
+                const int j = __float_as_int(e.w);
+                const float ex = pfx - e.x, ey = pfy - e.y, ez = pfz - e.z;
+                const float r2f = ex * ex + ey * ey + ez * ez;
+                bool in = r2f < rc2_lo && j != i;
+                const bool near = r2f >= rc2_lo && r2f <= rc2_hi && j != i;
+                if (__any(near)) {
+                    if (near) {
                         // x_i - (x_j + s L) with the stencil's s, which for a pair this
                         // close (well under L/2) is the minimum image
                         const double4 pj = pos[j];

@@ -25,6 +25,7 @@ namespace dlpx {
 
 struct SpmeParams {
     int K[3];
+    float invK[3];          // 1/K (fast wrapping of grid indices)
     int K1h;
     int S;
     double kr[3];           // K as real
@@ -222,11 +223,24 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
         __syncthreads();
         if (!tile) break;  // the direct path did every z at once
         const int dz = min(Dz, Wz - z0);
+        const float iWx = 1.0f / (float)Wx, iWy = 1.0f / (float)Wy;
+        // the float-reciprocal wrap is exact for window coordinates below 2^20
+        // in magnitude (always, for positions near the cell)
+        const bool near = abs(lo0) < (1 << 19) && abs(lo1) < (1 << 19) && abs(lo2) < (1 << 19);
         for (int k = t; k < Wx * Wy * dz; k += blockDim.x) {
             const R v = (R)slab[k];
             if (v == (R)0) continue;
-            const int lx = k % Wx, ly = (k / Wx) % Wy, lz = k / (Wx * Wy);
-            const int gx = wrapk(lo0 + lx, sp.K[0]), gy = wrapk(lo1 + ly, sp.K[1]), gz = wrapk(lo2 + z0 + lz, sp.K[2]);
+            int lx, ly, gx, gy, gz;
+            const int lz = fdiv_floor(fdiv_floor(k, Wx, iWx, lx), Wy, iWy, ly);
+            if (near) {
+                (void)fdiv_floor(lo0 + lx, sp.K[0], sp.invK[0], gx);
+                (void)fdiv_floor(lo1 + ly, sp.K[1], sp.invK[1], gy);
+                (void)fdiv_floor(lo2 + z0 + lz, sp.K[2], sp.invK[2], gz);
+            } else {
+                gx = wrapk(lo0 + lx, sp.K[0]);
+                gy = wrapk(lo1 + ly, sp.K[1]);
+                gz = wrapk(lo2 + z0 + lz, sp.K[2]);
+            }
             unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], v);
         }
     }
@@ -426,6 +440,7 @@ static SpmeParams spme_params(const Ctx *c)
     for (int d = 0; d < 3; d++) {
         sp.K[d] = c->K[d];
         sp.kr[d] = (double)c->K[d];
+        sp.invK[d] = 1.0f / (float)c->K[d];
     }
     sp.K1h = c->K1h;
     sp.S = c->prm.nsplines;
