@@ -216,7 +216,12 @@ static void assemble(Ctx *c, const double *red, dlpx_two_body_out *o)
 {
     std::memset(o, 0, sizeof(*o));
     const double engvdw = red[R_ENGVDW], virvdw = red[R_VIRVDW];
-    const double engrl = red[R_ENGRL], virrl = red[R_VIRRL];
+    // the real-space stress trace is sum gamma r^2 = -(virvdw + virrl)
+    // (vdw.F90:1985-2022, ewald_spole.F90:189-197), so the pair kernel does not
+    // accumulate the Coulomb virial separately; with no electrostatics it is
+    // exactly zero rather than the rounding difference of two vdW sums
+    const double engrl = red[R_ENGRL];
+    const double virrl = c->prm.electrostatics ? -(red[R_S11] + red[R_S22] + red[R_S33]) - virvdw : 0.0;
     double st[9] = {red[R_S11], red[R_S12], red[R_S13], red[R_S12], red[R_S22],
                     red[R_S23], red[R_S13], red[R_S23], red[R_S33]};
     double engrc = 0.0, virrc = 0.0, engnz = 0.0, virnz = 0.0;

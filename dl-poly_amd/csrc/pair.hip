@@ -101,7 +101,9 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = tid / GP, p = tid % GP, lane = p % G, c0 = p / G;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // engvdw, virvdw, engrl, virrl, s11..s33
+    // engvdw, virvdw, engrl, (unused: the Coulomb virial is -tr(stress) - virvdw,
+    // formed after the reduction), s11..s33
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < n) {
         const double4 pi = pos[i];
         const int ti = typ[i];
@@ -145,14 +147,14 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
                 acc[0] += (double)ge;
                 acc[1] -= (double)(g * (rrr * rrr));
             }
-            if (qi_on && rrr < rc && fabs(pj.w) > kZeroPlus) {
+            // no |q_j| > Tiny test: such a q_j contributes exact (or subnormal) zeros
+            if (qi_on && rrr < rc) {
                 R df, de;
                 interp2<R, R2>(te, rrr, rdr_e, df, de);
                 const R prefac = qi * (R)pj.w;
                 const R eg = prefac * df;
                 gam += eg;
                 acc[2] += (double)(prefac * de);
-                acc[3] -= (double)(eg * (rrr * rrr));
             }
             const R fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
             fx += (double)fxp;
