@@ -208,11 +208,11 @@ def run(s, precision, steps, warmup):
     t1 = time.perf_counter()
     # a second, separately timed pass with per-stage HIP events on the library stream
     ctx.set_timing(True)
-    ctx.md_run(steps, s.timestep, energies=False)
+    _, nreb_t = ctx.md_run(steps, s.timestep, energies=False)
     ms = ctx.get_timing()
     ctx.set_timing(False)
     ctx.close()
-    return dict(seconds=t1 - t0, rebuilds=nreb, stage_ms=ms)
+    return dict(seconds=t1 - t0, rebuilds=nreb, stage_ms=ms, timed_rebuilds=int(np.sum(nreb_t)))
 
 
 def nve_drift(s, precision, steps):
@@ -402,6 +402,9 @@ def main():
         "mixed_error_vs_fp64": dict({k: err[k] for k in ("force_rms_rel", "energy_rel", "virial_rel")},
                                     configuration="after 50 FP64 NVE steps from the start"),
         "list_rebuilds": runs[lib.MIXED]["rebuilds"],
+        # list build (binning, sort, permute excluded): ms per rebuild, from the timed pass
+        "list_build_ms": {("mixed" if p == lib.MIXED else "fp64"): runs[p]["stage_ms"][5] / max(runs[p]["timed_rebuilds"], 1)
+                          for p in runs},
         "stages_mixed": stages,
         "stages_fp64_ms": {nm: runs[lib.FP64]["stage_ms"][k] / max(runs[lib.FP64]["stage_ms"][7], 1.0)
                            for k, nm in enumerate(names) if nm in work},

@@ -309,6 +309,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     c->pair_lanes = 8;
     // resident steps replay a captured graph unless DLPX_GRAPHS=0
     if (const char *e = std::getenv("DLPX_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
+    // the list build bins by half-edge subcells unless DLPX_LIST_SUBCELLS=0
+    if (const char *e = std::getenv("DLPX_LIST_SUBCELLS")) c->list_subcells = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
@@ -347,7 +349,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     if (c->recip_stream) (void)hipStreamSynchronize(c->recip_stream);
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
-                   c->cell_start, c->cell_fill, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
+                   c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
                    c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);

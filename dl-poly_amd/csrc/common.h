@@ -73,6 +73,7 @@ struct Ctx {
     // resident steps replay one captured graph: forces, stage 2 and the next
     // stage 1; graph_key records what the capture baked in (buffers, sizes)
     bool use_graphs = true;
+    bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
     hipGraphExec_t step_graph = nullptr;
     std::vector<uint64_t> graph_key;
     int device = 0;
@@ -119,6 +120,10 @@ struct Ctx {
     int nc[3]{1, 1, 1};
     int ncell = 0, ncell_cap = 0;
     int32_t *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
+    // binning keys: link cells, or subcells (2 x 2 x 2 per link cell, numbered
+    // cell-major so each link cell's atoms stay contiguous)
+    int nkey_cap = 0;
+    int32_t *key_start = nullptr;
 
     // Verlet list: lane-interleaved chunks; element (c, i, g) at (c*cap + i)*G + g
     int cap_lane = 0;

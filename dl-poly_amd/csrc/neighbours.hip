@@ -1,9 +1,11 @@
 // Link-cell Verlet neighbour list on the GPU (neighbours.F90:356-1414).
 //
-// Atoms are binned into link cells of edge >= cutoff+padding, counting-sorted
-// by cell (deterministic order inside a cell) and every per-atom array is
-// permuted into that order, so the pair and SPME kernels read neighbours that
-// sit close in HBM.  The list is a full list (each pair stored from both ends)
+// Atoms are binned into link cells of edge >= cutoff+padding or, with at
+// least 3 link cells per axis, into the 2 x 2 x 2 subcells of each link cell
+// (numbered cell-major, so a link cell's atoms stay contiguous), then
+// counting-sorted by that key (deterministic order inside a key), and every
+// per-atom array is permuted into that order, so the pair and SPME kernels
+// read neighbours that sit close in HBM.  The list is a full list (each pair stored from both ends)
 // split over G lanes per atom: lane g keeps every G-th candidate it accepts,
 // element (i, c, g) lives at nbr[(i*cap_lane + c)*G + g] (list_slot), so one
 // atom's entries, and one workgroup's atoms, are contiguous.
@@ -18,7 +20,17 @@ namespace dlpx {
 struct CellGrid {
     int nc[3];
     double L[3], invL[3];
+    int sub;  // 1: bin by link cell; 2: by subcell (half the link-cell edge)
 };
+
+// Binning key of a subcell (sx, sy, sz) of edge L/(2 nc): link-cell-major,
+// then the 2 x 2 x 2 subcells of that link cell, so a link cell's atoms are
+// one contiguous run of the sorted order.
+__host__ __device__ __forceinline__ int subcell_key(const CellGrid &g, int sx, int sy, int sz)
+{
+    const int blk = ((sz >> 1) * g.nc[1] + (sy >> 1)) * g.nc[0] + (sx >> 1);
+    return blk * 8 + ((sz & 1) * 4 + (sy & 1) * 2 + (sx & 1));
+}
 
 __device__ __forceinline__ int cell_coord(double x, double L, double invL, int nc)
 {
@@ -61,10 +73,10 @@ __global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, in
     if (p.y >= 0.5 * g.L[1]) p.y -= g.L[1];
     if (p.z >= 0.5 * g.L[2]) p.z -= g.L[2];
     pos[i] = p;
-    const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.nc[0]);
-    const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.nc[1]);
-    const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.nc[2]);
-    const int c = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.sub * g.nc[0]);
+    const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.sub * g.nc[1]);
+    const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.sub * g.nc[2]);
+    const int c = g.sub == 2 ? subcell_key(g, cx, cy, cz) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
     cell_of[i] = c;
     // atoms are mostly in cell order already: one atomic per run of equal
     // cells within the wavefront
@@ -73,17 +85,25 @@ __global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, in
     if ((int)(threadIdx.x & 63) == head) atomicAdd(&count[c], len);
 }
 
-// Exclusive scan of the cell counts in one workgroup of 1024 threads.
+// Exclusive scan of the per-key counts in one workgroup of 1024 threads, each
+// thread taking kScanRun consecutive keys per pass (subcell binning has
+// hundreds of thousands of keys at a million atoms).
+constexpr int kScanRun = 8;
 __global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *start, int32_t *fill, int n)
 {
     __shared__ int32_t part[1024];
     __shared__ int32_t carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = i < n ? count[i] : 0;
-        part[threadIdx.x] = v;
+    for (int base = 0; base < n; base += 1024 * kScanRun) {
+        const int i0 = base + threadIdx.x * kScanRun;
+        int v[kScanRun], sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanRun; k++) {
+            v[k] = i0 + k < n ? count[i0 + k] : 0;
+            sum += v[k];
+        }
+        part[threadIdx.x] = sum;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
             int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
@@ -91,16 +111,27 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *st
             part[threadIdx.x] += t;
             __syncthreads();
         }
-        const int incl = part[threadIdx.x];
-        if (i < n) {
-            start[i] = carry + incl - v;
-            fill[i] = carry + incl - v;
+        int run = carry + part[threadIdx.x] - sum;
+#pragma unroll
+        for (int k = 0; k < kScanRun; k++) {
+            if (i0 + k < n) {
+                start[i0 + k] = run;
+                fill[i0 + k] = run;
+            }
+            run += v[k];
         }
         __syncthreads();
-        if (threadIdx.x == 1023) carry += incl;
+        if (threadIdx.x == 1023) carry += part[1023];
         __syncthreads();
     }
     if (threadIdx.x == 0) start[n] = carry;
+}
+
+// Link-cell starts from the subcell starts (8 consecutive keys per cell).
+__global__ void k_block_start(const int32_t *key_start, int32_t *cell_start, int ncell)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= ncell) cell_start[b] = key_start[8 * b];
 }
 
 __global__ void k_scatter(const int32_t *cell_of, int32_t *fill, int32_t *perm, int n)
@@ -356,6 +387,235 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
     }
 }
 
+// List build over subcells of half the link-cell edge (used when every axis
+// has at least 3 link cells).  One workgroup per link cell (2 x 2 x 2
+// subcells); the 6 x 6 x 6 subcells around it are staged in LDS as float4
+// {x + sx, y + sy, z + sz, j}, grouped by subcell in x-fastest order, so that
+// for an atom the 5 subcells of each of its 25 (y, z) rows are one
+// contiguous run of candidates.  An atom is never more than 2 subcells from
+// a neighbour (the subcell edge is at least half of cutoff+padding), so each
+// atom tests 125 subcells' atoms instead of the 27 link cells' (about 4.6
+// link cells' worth of candidates instead of 27).
+//
+// One atom per wavefront at a time: the 64 lanes test 64 candidates of a run,
+// and the accepted ones are packed with a ballot into the atom's next list
+// entries, so the loops and the list counter are wavefront-uniform.  Entry e
+// of atom i sits at list_slot(i, e / G, e % G), the layout of the link-cell
+// build, and list lane g then holds entries e = g, g + G, ...  The accept
+// test (FP32 filter, FP64 decision within 2e-5 of the cutoff) is the link-cell
+// build's, so both give the same set.
+constexpr int kSubSide = 6, kSubStencil = kSubSide * kSubSide * kSubSide;
+constexpr int kSubAtomChunk = 256;  // atoms of a link cell whose counts persist across staging rounds
+template <int G>
+__global__ void __launch_bounds__(kListThreads) k_build_list_sub(const double4 *__restrict__ pos, CellGrid g,
+                                                                 const int32_t *__restrict__ key_start, double rc2,
+                                                                 int cap_lane, int32_t *__restrict__ nbr,
+                                                                 int32_t *__restrict__ cnt, int32_t *flags)
+{
+    static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
+    __shared__ float4 cand[kListStage];
+    __shared__ int s_start[kSubStencil], s_off[kSubStencil + 1];  // s_off: sizes, then their exclusive scan
+    __shared__ double s_sh[3][kSubSide];
+    __shared__ int s_lo[9];
+    __shared__ int s_acnt[kSubAtomChunk];
+    __shared__ int s_ovf, s_big, s_tot;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b >= g.nc[0] * g.nc[1] * g.nc[2]) return;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    constexpr int NW = kListThreads / 64;
+    const int bc[3] = {b % g.nc[0], (b / g.nc[0]) % g.nc[1], b / (g.nc[0] * g.nc[1])};
+    if (t == 0) {
+        s_ovf = 0;
+        s_big = 0;
+        s_tot = 0;
+    }
+    if (t < kSubStencil) {
+        int sc[3];
+        const int q[3] = {t % kSubSide, (t / kSubSide) % kSubSide, t / (kSubSide * kSubSide)};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int ns = 2 * g.nc[d], u = 2 * bc[d] - 2 + q[d];
+            sc[d] = (u + ns) % ns;  // ns >= 6: the 6 stencil subcells of an axis are distinct
+        }
+        const int key = subcell_key(g, sc[0], sc[1], sc[2]);
+        s_start[t] = key_start[key];
+        s_off[t] = key_start[key + 1] - key_start[key];
+    } else if (t < kSubStencil + 3 * kSubSide) {
+        const int d = (t - kSubStencil) / kSubSide, qd = (t - kSubStencil) % kSubSide;
+        const int ns = 2 * g.nc[d], u = 2 * bc[d] - 2 + qd, su = (u + ns) % ns;
+        s_sh[d][qd] = (double)(u - su) / ns * g.L[d];  // image shift: -L, 0 or +L
+    } else if (t < kSubStencil + 3 * kSubSide + 9) {
+        s_lo[t - kSubStencil - 3 * kSubSide] = key_start[8 * b + (t - kSubStencil - 3 * kSubSide)];
+    }
+    __syncthreads();
+    if (t < 64) {
+        // exclusive scan of the 216 subcell sizes in place: 4 per lane (each
+        // lane reads its own entries before writing them), then across lanes
+        int v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int q = 4 * t + k;
+            v[k] = q < kSubStencil ? s_off[q] : 0;
+            sum += v[k];
+        }
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (t >= o) incl += y;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int q = 4 * t + k;
+            if (q < kSubStencil) s_off[q] = run;
+            run += v[k];
+        }
+        if (t == 63) s_off[kSubStencil] = incl;
+    }
+    __syncthreads();
+    const int ncand = s_off[kSubStencil];
+    const int hs = s_lo[0], nat = s_lo[8] - hs;
+    const int cap_tot = cap_lane * G;
+    const float rc2_lo = (float)(rc2 * (1.0 - 2.0e-5)), rc2_hi = (float)(rc2 * (1.0 + 2.0e-5));
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    for (int a0 = 0; a0 < nat; a0 += kSubAtomChunk) {
+        const int a1 = min(nat, a0 + kSubAtomChunk);
+        for (int r0 = 0; r0 < ncand; r0 += kListStage) {
+            const int m = min(kListStage, ncand - r0);
+            const bool last = r0 + m == ncand;
+            // stage candidates r0 .. r0+m-1 (kept when one round holds them all)
+            if (!(r0 == 0 && last && a0 > 0)) {
+                __syncthreads();
+                constexpr int U = 6;
+                for (int k0 = t; k0 < m; k0 += U * kListThreads) {
+                    double4 pk[U];
+                    int jk[U], qk[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int k = k0 + u * kListThreads;
+                        qk[u] = 0;
+                        if (k < m) {
+                            const int kk = r0 + k;
+                            int lo = 0, hi = kSubStencil;  // s_off[lo] <= kk < s_off[hi]
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (s_off[mid] <= kk) lo = mid;
+                                else hi = mid;
+                            }
+                            qk[u] = lo;
+                            jk[u] = s_start[lo] + (kk - s_off[lo]);
+                            pk[u] = pos[jk[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int k = k0 + u * kListThreads;
+                        if (k < m) {
+                            const int q = qk[u];
+                            const int qx = q % kSubSide, qy = (q / kSubSide) % kSubSide, qz = q / (kSubSide * kSubSide);
+                            cand[k] = make_float4((float)(pk[u].x + s_sh[0][qx]), (float)(pk[u].y + s_sh[1][qy]),
+                                                  (float)(pk[u].z + s_sh[2][qz]), __int_as_float(jk[u]));
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            // the next atom's position is loaded while this atom is processed
+            int a = a0 + wid;
+            double4 pnext = a < a1 ? pos[hs + a] : make_double4(0.0, 0.0, 0.0, 0.0);
+            for (; a < a1; a += NW) {
+                const int i = hs + a;
+                const double4 pi = pnext;
+                if (a + NW < a1) pnext = pos[i + NW];
+                // the atom's subcell among the link cell's 8 (s_lo is non-decreasing)
+                int l = 0;
+#pragma unroll
+                for (int k = 1; k < 8; k++) l += s_lo[k] <= i;
+                const int lx = l & 1, ly = (l >> 1) & 1, lz = l >> 2;
+                const float pfx = (float)pi.x, pfy = (float)pi.y, pfz = (float)pi.z;
+                // lane r < 25 holds the staged candidate range of row r (its 5
+                // subcells in x at dz = r / 5, dy = r % 5), relative to the round
+                int rlo = 0, rhi = 0;
+                if (lane < 25) {
+                    const int qrow = ((lz + lane / 5) * kSubSide + (ly + lane % 5)) * kSubSide + lx;
+                    rlo = max(s_off[qrow], r0) - r0;
+                    rhi = min(s_off[qrow + 5], r0 + m) - r0;
+                }
+                int count = __builtin_amdgcn_readfirstlane(r0 == 0 ? 0 : s_acnt[a - a0]);
+                int32_t *const row_out = nbr + list_slot(i, 0, 0, cap_lane);
+                // one chunk of up to 64 candidates; accepted ones are packed into the
+                // next list entries (an overflowing list is rebuilt, so entries past
+                // the capacity all land on its last slot)
+                auto test = [&](const float4 e, const bool v) {
+                    const int j = __float_as_int(e.w);
+                    const float ex = pfx - e.x, ey = pfy - e.y, ez = pfz - e.z;
+                    const float r2f = ex * ex + ey * ey + ez * ez;
+                    bool in = v && r2f < rc2_lo && j != i;
+                    const bool near = v && r2f >= rc2_lo && r2f <= rc2_hi && j != i;
+                    if (__any(near)) {
+                        if (near) {
+                            // x_i - (x_j + s L) with the stencil's s is the minimum
+                            // image for a pair this close (well under L/2)
+                            const double4 pj = pos[j];
+                            const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
+                            const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
+                            const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
+                            const double rsq = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)),
+                                                         __dmul_rn(dz, dz));
+                            in = rsq <= rc2;
+                        }
+                    }
+                    const unsigned long long acc = __ballot(in);
+                    const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(acc >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)acc, 0u));
+                    if (in) row_out[min(count + pre, cap_tot - 1)] = j;
+                    count += __popcll(acc);
+                };
+                // five rows at a time: their first chunks' LDS reads are issued together
+#pragma unroll 1
+                for (int g5 = 0; g5 < 25; g5 += 5) {
+                    int lo[5], hi[5];
+                    bool v[5];
+                    float4 e[5];
+#pragma unroll
+                    for (int u = 0; u < 5; u++) {
+                        lo[u] = __builtin_amdgcn_readlane(rlo, g5 + u);
+                        hi[u] = __builtin_amdgcn_readlane(rhi, g5 + u);
+                        v[u] = lo[u] + lane < hi[u];
+                        e[u] = cand[v[u] ? lo[u] + lane : 0];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 5; u++) test(e[u], v[u]);
+#pragma unroll
+                    for (int u = 0; u < 5; u++)
+                        for (int k0 = lo[u] + 64; k0 < hi[u]; k0 += 64) {
+                            const bool vv = k0 + lane < hi[u];
+                            test(cand[vv ? k0 + lane : 0], vv);
+                        }
+                }
+                if (!last) {
+                    if (lane == 0) s_acnt[a - a0] = count;
+                    continue;
+                }
+                // per-lane counts of the G-way interleave: lane g holds entries g, g+G, ...
+                if (lane < G) cnt[(size_t)i * G + lane] = (count - lane + G - 1) / G;
+                if (lane == 0) {
+                    if (count > cap_tot) atomicMax(&s_ovf, (count + G - 1) / G);
+                    atomicMax(&s_big, count);
+                    atomicAdd(&s_tot, count);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (s_ovf) atomicMax(&flags[0], s_ovf);
+        atomicMax(&flags[1], s_big);
+        atomicAdd(&flags[3], s_tot);
+    }
+}
+
 // vnl_check (neighbours.F90:150-178): largest minimum-image displacement since
 // the list was built, as the bit pattern of a non-negative double.
 __global__ void k_max_disp(const double4 *pos, const double4 *xref, int n, CellGrid g,
@@ -383,6 +643,7 @@ static CellGrid make_grid(const Ctx *c)
         g.L[d] = c->L[d];
         g.invL[d] = c->invL[d];
     }
+    g.sub = 1;
     return g;
 }
 
@@ -408,22 +669,35 @@ int build_list(Ctx *c)
         if (c->nc[d] < 1) return fail(c, DLPX_ERR_LINK_CELLS, "too few link cells");
     }
     const int ncell = c->nc[0] * c->nc[1] * c->nc[2];
+    // subcells need 6 distinct subcells per axis around a link cell
+    const bool sub = c->list_subcells && c->nc[0] >= 3 && c->nc[1] >= 3 && c->nc[2] >= 3;
+    const int nkey = sub ? 8 * ncell : ncell;
     if (ncell + 1 > c->ncell_cap) {
         int r;
-        if ((r = grow(c, &c->cell_count, ncell + 1))) return r;
         if ((r = grow(c, &c->cell_start, ncell + 1))) return r;
-        if ((r = grow(c, &c->cell_fill, ncell + 1))) return r;
         c->ncell_cap = ncell + 1;
     }
+    if (nkey + 1 > c->nkey_cap) {
+        int r;
+        if ((r = grow(c, &c->cell_count, nkey + 1))) return r;
+        if ((r = grow(c, &c->cell_fill, nkey + 1))) return r;
+        if ((r = grow(c, &c->key_start, nkey + 1))) return r;
+        c->nkey_cap = nkey + 1;
+    }
     c->ncell = ncell;
-    const CellGrid g = make_grid(c);
+    CellGrid g = make_grid(c);
+    g.sub = sub ? 2 : 1;
     hipStream_t s = c->stream;
-    DLPX_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int32_t) * (ncell + 1), s));
+    DLPX_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int32_t) * (nkey + 1), s));
     const int b = 256, nb = ceil_div(n, b);
+    int32_t *kstart = sub ? c->key_start : c->cell_start;
     hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, c->pos, n, g, c->cell_of, c->cell_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->cell_count, c->cell_start, c->cell_fill, ncell);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->cell_count, kstart, c->cell_fill, nkey);
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, c->cell_of, c->cell_fill, c->perm, n);
-    hipLaunchKernelGGL(k_sort_cells, dim3(ncell), dim3(256), 0, s, c->perm, c->cell_start, ncell);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nkey), dim3(sub ? 64 : 256), 0, s, c->perm, kstart, nkey);
+    if (sub)
+        hipLaunchKernelGGL(k_block_start, dim3(ceil_div(ncell + 1, 256)), dim3(256), 0, s, c->key_start,
+                           c->cell_start, ncell);
     hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, c->perm, n, c->pos, c->pos_alt, c->typ, c->typ_alt,
                        c->gid, c->gid_alt, c->resident ? c->vel : nullptr, c->resident ? c->vel_alt : nullptr,
                        c->slot, c->xref);
@@ -443,9 +717,12 @@ int build_list(Ctx *c)
         }
         DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
-        hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, c->cap, g,
-                           c->cell_start,
-                           rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
+        if (sub)
+            hipLaunchKernelGGL(k_build_list_sub<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, g,
+                               c->key_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
+        else
+            hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, c->cap,
+                               g, c->cell_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
         DLPX_HIP(c, hipStreamSynchronize(s));

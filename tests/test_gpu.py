@@ -68,6 +68,24 @@ def test_pair_set_bit_exact(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["argon_4000_jit", "nacl_4096_k48"])
+def test_link_cell_and_subcell_builds_give_the_same_set(name, monkeypatch):
+    # boxes with >= 3 link cells per axis use the subcell build by default;
+    # DLPX_LIST_SUBCELLS=0 (read at context creation) selects the link-cell build
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    sets = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("DLPX_LIST_SUBCELLS", flag)
+        ctx = lib.Context(s)
+        npairs, ibig = ctx.link_cell_pairs(s.pos)
+        g = ctx.get_pairs()
+        assert npairs == len(g)
+        sets.append(({tuple(p) for p in g}, ibig))
+        ctx.close()
+    assert sets[0] == sets[1]
+
+
 @pytest.mark.parametrize("name", STATIC)
 def test_mixed_precision_error_budget(name):
     s, _ = fixtures.load(name)
@@ -165,7 +183,7 @@ def test_full_size_nacl_fp64_and_mixed(nacl64k):
 
 def _resident_forces_vs_oracle(s, steps, precision):
     """Forces of the last resident step against the oracle at the positions
-    that step ended with (resident steps use their own spreading kernel)."""
+    that step ended with."""
     ctx = lib.Context(s, precision=precision)
     ctx.md_load(s.pos, s.vel)
     ctx.md_run(steps, s.timestep)
