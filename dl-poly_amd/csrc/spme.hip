@@ -100,8 +100,6 @@ __device__ __forceinline__ void grid_coord(const SpmeParams &sp, int d, double x
     w = u - (double)i;
 }
 
-constexpr int kAtomsPerBlock = 64;
-
 // Unwrapped fractional grid coordinate of one axis (no folding): the base
 // grid point floor(u) - S + 1 may be negative or >= K; callers fold indices.
 __device__ __forceinline__ double grid_u(const SpmeParams &sp, int d, double x, double y, double z)
@@ -130,10 +128,23 @@ constexpr int kSlabBytes = 32768;
 // One workgroup per link cell gives only ~730 workgroups at 64k ions, so each
 // needs many wavefronts to keep the LDS atomics busy: 512 threads in FP32;
 // the FP64 build's register use makes 256 the better trade there (measured).
+//
+// The B-splines of up to spread_atoms() atoms are staged in LDS once and
+// serve every z-slab; a cell with more atoms restages them per slab.  Each
+// wavefront adds one atom at a time, lanes over the (x, y) points, with the
+// atom's S z-weights held in registers so the S atomic adds of a lane issue
+// back to back.
 template <typename R>
 constexpr int spread_threads()
 {
     return sizeof(R) == 4 ? 512 : 256;
+}
+template <typename R, int S>
+constexpr int spread_atoms()
+{
+    // 12 KB (FP32) or 18 KB (FP64) of staged splines: with the 32 KB slab,
+    // three workgroups fit in a CU's LDS
+    return (sizeof(R) == 8 ? 18432 : 12288) / (3 * S * (int)sizeof(R));
 }
 template <typename R, int S>
 __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const double4 *__restrict__ pos,
@@ -141,10 +152,12 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
                                                       SpmeParams sp, R *__restrict__ Q)
 {
     constexpr int kCap = kSlabBytes / (int)sizeof(double);
+    constexpr int NA = spread_atoms<R, S>();
+    constexpr int NT = spread_threads<R>();
     __shared__ double slab[kCap];
-    __shared__ R sM[3][kAtomsPerBlock][S];
-    __shared__ int sB[3][kAtomsPerBlock];
-    __shared__ R sQ[kAtomsPerBlock];
+    __shared__ R sM[3][NA][S];
+    __shared__ int sB[3][NA];
+    __shared__ R sQ[NA];
     __shared__ int wlo[3], whi[3];
     const int c = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x, wid = t >> 6, lane = t & 63;
     if (c >= ncell) return;
@@ -155,7 +168,7 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
         whi[t] = -0x7fffffff;
     }
     __syncthreads();
-    for (int k = t; k < n; k += blockDim.x) {
+    for (int k = t; k < n; k += NT) {
         const double4 p = pos[s0 + k];
         for (int d = 0; d < 3; d++) {
             const int b = (int)floor(grid_u(sp, d, p.x, p.y, p.z)) - S + 1;
@@ -170,51 +183,68 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
     const bool tile = ex <= sp.K[0] && ey <= sp.K[1] && ez <= sp.K[2] && ex * ey <= kCap;
     const int Wx = tile ? (int)ex : 0, Wy = tile ? (int)ey : 0, Wz = tile ? (int)ez : 1;
     const int Dz = tile ? min(Wz, kCap / (Wx * Wy)) : Wz;
+    const int plane = Wx * Wy;
+    // B-splines, base points and charges of atoms a0 .. a0+NA-1 into LDS
+    auto stage = [&](int a0) {
+        for (int k = t; k < 3 * NA; k += NT) {
+            const int d = k / NA, a = k % NA;
+            if (a0 + a < n) {
+                const double4 p = pos[s0 + a0 + a];
+                const double u = grid_u(sp, d, p.x, p.y, p.z);
+                const double fl = floor(u);
+                R M[S];
+                bspline<R, false, S>((R)(u - fl), M, nullptr);
+#pragma unroll
+                for (int m = 0; m < S; m++) sM[d][a][m] = M[m];
+                sB[d][a] = (int)fl - S + 1;
+                if (d == 0) sQ[a] = (R)p.w;
+            }
+        }
+    };
+    const bool once = n <= NA;
+    if (once) stage(0);  // made visible by the barrier after the slab clear
     for (int z0 = 0; z0 < Wz; z0 += Dz) {
         if (tile) {
-            for (int k = t; k < Wx * Wy * Dz; k += blockDim.x) slab[k] = 0.0;
+            if (z0 > 0) __syncthreads();  // the previous slab's flush has read the slab
+            for (int k = t; k < plane * Dz; k += NT) slab[k] = 0.0;
         }
-        for (int a0 = 0; a0 < n; a0 += kAtomsPerBlock) {
+        for (int a0 = 0; a0 < n; a0 += NA) {
             __syncthreads();
-            if (t < 3 * kAtomsPerBlock) {
-                const int d = t / kAtomsPerBlock, a = t % kAtomsPerBlock;
-                if (a0 + a < n) {
-                    const double4 p = pos[s0 + a0 + a];
-                    const double u = grid_u(sp, d, p.x, p.y, p.z);
-                    const double fl = floor(u);
-                    R M[S];
-                    bspline<R, false, S>((R)(u - fl), M, nullptr);
-#pragma unroll
-                    for (int k = 0; k < S; k++) sM[d][a][k] = M[k];
-                    sB[d][a] = (int)fl - S + 1;
-                    if (d == 0) sQ[a] = (R)p.w;
-                }
+            if (!once) {
+                stage(a0);
+                __syncthreads();
             }
-            __syncthreads();
-            const int na = min(kAtomsPerBlock, n - a0);
-            for (int a = wid; a < na; a += spread_threads<R>() / 64) {
+            const int na = min(NA, n - a0);
+            for (int a = wid; a < na; a += NT / 64) {
                 const R q = sQ[a];
                 if (fabs((double)q) <= kZeroPlus) continue;
-                const int bx = sB[0][a], by = sB[1][a], bz = sB[2][a];
+                // one atom per wavefront: its base point and z-range are scalars,
+                // so the per-z tests below are scalar branches, not exec masks
+                const int bx = __builtin_amdgcn_readfirstlane(sB[0][a]);
+                const int by = __builtin_amdgcn_readfirstlane(sB[1][a]);
+                const int bz = __builtin_amdgcn_readfirstlane(sB[2][a]);
                 // spline points of this atom inside the current z-slab
                 const int tz0 = tile ? max(0, z0 + lo2 - bz) : 0;
                 const int tz1 = tile ? min(S, z0 + lo2 + Dz - bz) : S;
                 if (tz0 >= tz1) continue;
+                R mz[S];
+#pragma unroll
+                for (int m = 0; m < S; m++) mz[m] = q * sM[2][a][m];
                 for (int pp = lane; pp < S * S; pp += 64) {
                     const int tx = pp % S, ty = pp / S;
                     const R mxy = sM[0][a][tx] * sM[1][a][ty];
                     if (tile) {
-                        const int lxy = (by + ty - lo1) * Wx + (bx + tx - lo0);
-                        const int plane = Wx * Wy;
-                        int l = (bz + tz0 - lo2 - z0) * plane + lxy;  // lz in [0, Dz)
-                        for (int tz = tz0; tz < tz1; tz++, l += plane)
-                            atomicAdd(&slab[l], (double)(q * sM[2][a][tz] * mxy));
+                        // slab index of spline point tz = 0 (points outside the slab are skipped)
+                        const int l0 = (bz - lo2 - z0) * plane + (by + ty - lo1) * Wx + (bx + tx - lo0);
+#pragma unroll
+                        for (int tz = 0; tz < S; tz++)
+                            if (tz >= tz0 && tz < tz1) atomicAdd(&slab[l0 + tz * plane], (double)(mz[tz] * mxy));
                     } else {
                         const int gx = wrapk(bx + tx, sp.K[0]), gy = wrapk(by + ty, sp.K[1]);
 #pragma unroll
                         for (int tz = 0; tz < S; tz++) {
                             const int gz = wrapk(bz + tz, sp.K[2]);
-                            unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], q * sM[2][a][tz] * mxy);
+                            unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], mz[tz] * mxy);
                         }
                     }
                 }
@@ -223,16 +253,34 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
         __syncthreads();
         if (!tile) break;  // the direct path did every z at once
         const int dz = min(Dz, Wz - z0);
+        // window coordinates within one period of the grid (always, for
+        // positions near the cell) wrap with one compare
+        const bool near = lo0 >= -sp.K[0] && lo1 >= -sp.K[1] && lo2 >= -sp.K[2] && lo0 + Wx <= 2 * sp.K[0] &&
+                          lo1 + Wy <= 2 * sp.K[1] && lo2 + Wz <= 2 * sp.K[2];
+        if (near && Wx <= 64) {
+            // one (y, z) row of the slab per group of 32 or 64 lanes, lanes over x
+            const int LW = Wx <= 32 ? 32 : 64, lx = lane & (LW - 1);
+            auto fold = [](int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); };
+            const int gx = fold(lo0 + lx, sp.K[0]);
+            for (int row = wid * (64 / LW) + lane / LW; row < Wy * dz; row += NT / LW) {
+                const int lz = row / Wy, ly = row - lz * Wy;
+                const R v = lx < Wx ? (R)slab[row * Wx + lx] : (R)0;
+                if (v == (R)0) continue;
+                const int gy = fold(lo1 + ly, sp.K[1]), gz = fold(lo2 + z0 + lz, sp.K[2]);
+                unsafeAtomicAdd(&Q[((size_t)gz * sp.K[1] + gy) * sp.K[0] + gx], v);
+            }
+            continue;
+        }
         const float iWx = 1.0f / (float)Wx, iWy = 1.0f / (float)Wy;
         // the float-reciprocal wrap is exact for window coordinates below 2^20
-        // in magnitude (always, for positions near the cell)
-        const bool near = abs(lo0) < (1 << 19) && abs(lo1) < (1 << 19) && abs(lo2) < (1 << 19);
-        for (int k = t; k < Wx * Wy * dz; k += blockDim.x) {
+        // in magnitude
+        const bool small = abs(lo0) < (1 << 19) && abs(lo1) < (1 << 19) && abs(lo2) < (1 << 19);
+        for (int k = t; k < plane * dz; k += NT) {
             const R v = (R)slab[k];
             if (v == (R)0) continue;
             int lx, ly, gx, gy, gz;
             const int lz = fdiv_floor(fdiv_floor(k, Wx, iWx, lx), Wy, iWy, ly);
-            if (near) {
+            if (small) {
                 (void)fdiv_floor(lo0 + lx, sp.K[0], sp.invK[0], gx);
                 (void)fdiv_floor(lo1 + ly, sp.K[1], sp.invK[1], gy);
                 (void)fdiv_floor(lo2 + z0 + lz, sp.K[2], sp.invK[2], gz);
