@@ -560,12 +560,17 @@ int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, c
     int r;
     if ((r = set_cell(c, cell))) return r;
     c->resident = true;
+    c->have_last_out = false;
     if ((r = upload(c, atoms, true, vx, vy, vz, mass, vstride))) return r;
     charges_summary(c, atoms);
     if ((r = build_list(c))) return r;
     // forces at the starting configuration (md_vv computes them before step 1)
     if ((r = evaluate(c))) return r;
-    return fetch_red(c);
+    if ((r = fetch_red(c))) return r;
+    // the starting configuration's output, until a run with energies replaces it
+    assemble(c, c->h_red, &c->last_out);
+    c->have_last_out = true;
+    return 0;
 }
 
 // What a captured step graph bakes in (buffers, sizes, launch shapes, the
@@ -687,9 +692,22 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
             e[2] = o.engsrp;
             e[3] = o.virsrp;
             e[4] = 0.5 * red[(size_t)kReduceSlots * s + R_KIN];
+            if (s == nsteps - 1) {
+                c->last_out = o;
+                c->have_last_out = true;
+            }
         }
     }
     if (nrebuilds) *nrebuilds = rebuilds;
+    return 0;
+}
+
+int dlpx_md_last_output(dlpx_ctx *ctx, dlpx_two_body_out *out)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out) return DLPX_ERR_BAD_ARG;
+    if (!c->have_last_out) return fail(c, DLPX_ERR_STATE, "no resident step with energies has run");
+    *out = c->last_out;
     return 0;
 }
 

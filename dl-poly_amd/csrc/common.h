@@ -160,6 +160,8 @@ struct Ctx {
     double *h_red = nullptr;         // pinned mirror of red_out
     double *energies_dev = nullptr;  // per-step energies (resident runs)
     int energies_cap = 0;
+    dlpx_two_body_out last_out{};    // the last step of the last resident run with energies
+    bool have_last_out = false;
 
     // timing
     bool timing = false;

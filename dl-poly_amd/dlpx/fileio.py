@@ -1,11 +1,25 @@
 """DL_POLY file formats used on the force path's boundary.
 
-Writers for CONFIG (``configuration.F90:846`` reader, list-directed reals),
-FIELD (``ffield.F90``; ``units internal`` so parameters are taken verbatim) and
-new-style CONTROL (``control.F90`` keyword table); readers for REVCON
-(``configuration.F90:2601-2615``, ``3g20.10`` records) and the binary REVIVE
-(``system.F90:2078-2093``: sequential unformatted records, stpval at record 4,
-stress at record 13).
+Readers and writers of the files a DL_POLY run of this path reads and
+writes, so runs of this library can be set up and judged with DL_POLY's own
+inputs and comparators:
+
+* CONFIG / REVCON: ``read_config`` (``configuration.F90:846`` reader:
+  title, ``levcfg imcon natms`` record, cell for imcon > 0, then per atom a
+  name/index record and position, velocity, force records by levcfg) and
+  ``write_config`` / ``write_revcon`` (``configuration.F90:2540-2615``:
+  ``(4i10,1p,2e16.7)`` header, ``(a8,i10)`` and ``3g20.10`` records);
+* FIELD: ``read_field`` (``ffield.F90:150-5365``, the subset this path
+  supports: energy units, molecule types of unbonded atoms, the ``lj`` and
+  ``bhm`` vdW forms) and ``write_field``;
+* CONTROL: ``read_control`` / ``write_control`` (new-style keyword file,
+  ``control.F90``);
+* STATIS: ``statis_header`` / ``statis_record`` (``statistics.F90:1020-1027,
+  1333-1347``: ``(i10,1p,e14.6,0p,i10,/,(1p,5e14.6))`` records), ``read_statis``
+  and ``statis_equal``, the relative-precision rule of the reference's test
+  harness (``utils/beetest:39-48``);
+* REVIVE (binary restart): ``read_revive`` (``system.F90:2078-2093``:
+  sequential unformatted records, stpval at record 4, stress at record 13).
 """
 import os
 import struct
@@ -13,6 +27,13 @@ import struct
 import numpy as np
 
 from .systems import VDW_NAMES
+
+
+# constants.F90:104,112-114,123: Boltzmann constant and energy units in
+# internal units (10 J/mol), pressure unit (internal -> katm)
+BOLTZ = 8.31451115e-1
+ENERGY_UNITS = {"internal": 1.0, "ev": 9648.530821, "kcal": 418.4, "kj": 100.0, "k": BOLTZ}
+PRSUNT = 1.63882576e-1
 
 
 def _num(x):
@@ -158,3 +179,333 @@ def read_revive(path):
                 engsrp=stpval[STP_ENGSRP], engcpe=stpval[STP_ENGCPE],
                 virsrp=stpval[STP_VIRSRP], vircpe=stpval[STP_VIRCPE],
                 engcfg=stpval[STP_ENGCFG])
+
+
+# ---------------------------------------------------------------- CONFIG --
+
+def read_config(path):
+    """CONFIG or REVCON (configuration.F90:846-1100 reader).  Returns dict(title,
+    levcfg, imcon, cell (9,), names (N,), index (N,), pos, vel, frc (N, 3))
+    in file order; vel/frc are zero when levcfg does not provide them."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    title = lines[0].rstrip()
+    head = lines[1].split()
+    levcfg, imcon = int(head[0]), int(head[1])
+    n_decl = int(head[2]) if len(head) > 2 else None
+    k = 2
+    cell = np.zeros(9)
+    if imcon > 0:
+        cell = np.array([float(v) for ln in lines[2:5] for v in ln.split()[:3]])
+        k = 5
+    per = 2 + (levcfg >= 1) + (levcfg >= 2)
+    names, index, pos, vel, frc = [], [], [], [], []
+    while k + per - 1 < len(lines) and lines[k].strip():
+        rec = lines[k].split()
+        names.append(rec[0])
+        index.append(int(rec[1]) if len(rec) > 1 else len(names))
+        pos.append([float(v) for v in lines[k + 1].split()[:3]])
+        vel.append([float(v) for v in lines[k + 2].split()[:3]] if levcfg >= 1 else [0.0] * 3)
+        frc.append([float(v) for v in lines[k + 3].split()[:3]] if levcfg >= 2 else [0.0] * 3)
+        k += per
+    if n_decl is not None and n_decl != len(names):
+        raise ValueError("%s declares %d atoms, holds %d" % (path, n_decl, len(names)))
+    return dict(title=title, levcfg=levcfg, imcon=imcon, cell=cell, names=names,
+                index=np.array(index, np.int64), pos=np.array(pos), vel=np.array(vel), frc=np.array(frc))
+
+
+def _g20(x):
+    # a 20-character field with 10 significant digits (the reference writes 3g20.10)
+    return "%20.10G" % float(x)
+
+
+def write_revcon(path, title, cell, names, pos, vel=None, frc=None, nstep=0, tstep=0.0, time=0.0, imcon=None):
+    """REVCON in the reference's layout (configuration.F90:2540-2615): levcfg 2
+    when forces are given, 1 with velocities only, 0 otherwise."""
+    n = len(names)
+    levcfg = 2 if frc is not None else (1 if vel is not None else 0)
+    if imcon is None:
+        imcon = 1 if _cubic(np.asarray(cell)) else (2 if _orthorhombic(np.asarray(cell)) else 3)
+    out = ["%-72s" % title[:72], "%10d%10d%10d%10d%16.7E%16.7E" % (levcfg, imcon, n, nstep, tstep, time)]
+    c = np.asarray(cell, np.float64).reshape(3, 3)
+    for r in range(3):
+        out.append("%20.10f%20.10f%20.10f" % tuple(c[r]))
+    for i in range(n):
+        out.append("%-8s%10d" % (names[i], i + 1))
+        out.append("".join(_g20(v) for v in pos[i]))
+        if levcfg >= 1:
+            out.append("".join(_g20(v) for v in vel[i]))
+        if levcfg >= 2:
+            out.append("".join(_g20(v) for v in frc[i]))
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+# ----------------------------------------------------------------- FIELD --
+
+_VDW_KEYS = {"lj": 2, "bhm": 5}
+# parameters that carry energy units, per vdW form (ffield.F90 vdw section:
+# parpot(1) always; bhm also parpot(4), parpot(5))
+_VDW_ENERGY_PARAMS = {2: (0,), 5: (0, 3, 4)}
+
+
+def read_field(path):
+    """FIELD (ffield.F90:150-5365), the subset of this path: energy units;
+    molecule types made of unbonded atoms (``name mass charge [repeat
+    [frozen]]``); the ``lj`` and ``bhm`` vdW forms.  Bonded terms, frozen
+    atoms, shells, other vdW forms and many-body terms raise ValueError.
+    Returns dict(title, units, molecules [(name, nummols, [(atom, mass, charge)])],
+    vdw [(atom1, atom2, key, params in internal units)])."""
+    with open(path) as f:
+        raw = [ln.split("#")[0].rstrip() for ln in f.read().splitlines()]
+    title = raw[0].strip()
+    lines = [ln for ln in raw[1:] if ln.strip()]
+    units = "internal"
+    unit = 1.0
+    molecules, vdw = [], []
+    i = 0
+
+    def words(ln):
+        return ln.split()
+
+    while i < len(lines):
+        w = words(lines[i])
+        key = w[0].lower()
+        if key == "units":
+            u = w[1].lower() if len(w) > 1 else "internal"
+            for name, val in (("ev", ENERGY_UNITS["ev"]), ("kcal", ENERGY_UNITS["kcal"]), ("kj", ENERGY_UNITS["kj"]),
+                              ("internal", 1.0)):
+                if u.startswith(name):
+                    units, unit = name, val
+                    break
+            else:
+                if u.startswith("k"):
+                    units, unit = "k", ENERGY_UNITS["k"]
+                else:
+                    raise ValueError("unknown energy units %r" % u)
+            i += 1
+        elif key.startswith("molecul"):
+            nmol = int(w[-1])
+            i += 1
+            for _ in range(nmol):
+                mname = lines[i].strip()
+                i += 1
+                nummols, atoms = None, []
+                while True:
+                    w = words(lines[i])
+                    k2 = w[0].lower()
+                    if k2.startswith("nummol"):
+                        nummols = int(w[-1])
+                        i += 1
+                    elif k2 == "atoms":
+                        na = int(w[-1])
+                        i += 1
+                        got = 0
+                        while got < na:
+                            a = words(lines[i])
+                            rep = int(a[3]) if len(a) > 3 else 1
+                            frozen = int(a[4]) if len(a) > 4 else 0
+                            if frozen:
+                                raise ValueError("frozen atoms are not supported")
+                            for _r in range(max(rep, 1)):
+                                atoms.append((a[0], float(a[1]), float(a[2])))
+                            got += max(rep, 1)
+                            i += 1
+                    elif k2 == "finish":
+                        i += 1
+                        break
+                    else:
+                        raise ValueError("FIELD section %r is not supported by this path" % w[0])
+                if nummols is None:
+                    raise ValueError("molecule %r has no nummols record" % mname)
+                molecules.append((mname, nummols, atoms))
+        elif key == "vdw":
+            nv = int(w[-1])
+            i += 1
+            for _ in range(nv):
+                a = words(lines[i])
+                form = a[2].lower()
+                if form not in _VDW_KEYS:
+                    raise ValueError("vdW form %r is not supported by this path" % a[2])
+                kind = _VDW_KEYS[form]
+                params = [float(v) for v in a[3:]]
+                for j in _VDW_ENERGY_PARAMS[kind]:
+                    params[j] *= unit
+                vdw.append((a[0], a[1], kind, tuple(params)))
+                i += 1
+        elif key == "close":
+            break
+        else:
+            raise ValueError("FIELD directive %r is not supported by this path" % w[0])
+    return dict(title=title, units=units, molecules=molecules, vdw=vdw)
+
+
+# --------------------------------------------------------------- CONTROL --
+
+_TIME_UNITS = {"ps": 1.0, "fs": 1.0e-3, "ns": 1.0e3}
+_LENGTH_UNITS = {"ang": 1.0, "nm": 10.0, "bohr": 0.52917721067}
+
+
+def read_control(path):
+    """New-style CONTROL (control.F90 keyword table), the keys this path uses:
+    title, ensemble (nve), timestep, time_run, time_equilibration,
+    stats_frequency, cutoff, padding, vdw_cutoff, coul_method (spme / off),
+    coul_dielectric_constant, spme_alpha, spme_kvec, spme_nsplines,
+    temperature.  Other keys are kept verbatim in ``other``."""
+    out = dict(title="", ensemble="nve", timestep=None, steps=0, equilibration=0, stats_frequency=1,
+               cutoff=None, padding=0.0, vdw_cutoff=None, coul_method="off", epsilon=1.0, alpha=None,
+               kvec=None, nsplines=8, temperature=0.0, other={})
+    with open(path) as f:
+        for ln in f:
+            ln = ln.split("#")[0].strip()
+            if not ln:
+                continue
+            w = ln.split()
+            key = w[0].lower()
+            vals = w[1:]
+
+            def length(v):
+                return float(v[0]) * _LENGTH_UNITS.get(v[1].lower(), 1.0) if len(v) > 1 else float(v[0])
+
+            if key == "title":
+                out["title"] = ln[len(w[0]):].strip()
+            elif key == "ensemble":
+                out["ensemble"] = vals[0].lower()
+            elif key == "timestep":
+                out["timestep"] = float(vals[0]) * (_TIME_UNITS.get(vals[1].lower(), 1.0) if len(vals) > 1 else 1.0)
+            elif key == "time_run":
+                out["steps"] = int(float(vals[0]))
+            elif key == "time_equilibration":
+                out["equilibration"] = int(float(vals[0]))
+            elif key == "stats_frequency":
+                out["stats_frequency"] = int(float(vals[0]))
+            elif key == "cutoff":
+                out["cutoff"] = length(vals)
+            elif key == "padding":
+                out["padding"] = length(vals)
+            elif key == "vdw_cutoff":
+                out["vdw_cutoff"] = length(vals)
+            elif key == "coul_method":
+                out["coul_method"] = vals[0].lower()
+            elif key == "coul_dielectric_constant":
+                out["epsilon"] = float(vals[0])
+            elif key == "spme_alpha":
+                out["alpha"] = float(vals[0])
+            elif key == "spme_kvec":
+                nums = ln[ln.index(w[1]):].replace("[", " ").replace("]", " ").split()
+                out["kvec"] = tuple(int(float(v)) for v in nums[:3])
+            elif key == "spme_nsplines":
+                out["nsplines"] = int(float(vals[0]))
+            elif key == "temperature":
+                out["temperature"] = float(vals[0])
+            else:
+                out["other"][key] = " ".join(vals)
+    if out["vdw_cutoff"] is None:
+        out["vdw_cutoff"] = out["cutoff"]
+    return out
+
+
+def system_from_files(directory):
+    """A ``System`` from DL_POLY's CONFIG, FIELD and CONTROL in a directory.
+    Atom types are the FIELD's unique atom names in order of first appearance
+    (sites%unique_atom); the CONFIG must list the atoms in FIELD order
+    (molecule type by molecule type, nummols copies each)."""
+    from .systems import System
+    cfg = read_config(os.path.join(directory, "CONFIG"))
+    fld = read_field(os.path.join(directory, "FIELD"))
+    ctl = read_control(os.path.join(directory, "CONTROL"))
+    if ctl["ensemble"] != "nve":
+        raise ValueError("ensemble %r: this path integrates NVE only" % ctl["ensemble"])
+    if cfg["imcon"] not in (1, 2):
+        raise ValueError("imcon %d: this path supports cubic and orthorhombic cells" % cfg["imcon"])
+    type_names, masses, charges, site_types = [], [], [], []
+    for (_m, nummols, atoms) in fld["molecules"]:
+        for _k in range(nummols):
+            for (nm, mass, q) in atoms:
+                if nm not in type_names:
+                    type_names.append(nm)
+                    masses.append(mass)
+                    charges.append(q)
+                t = type_names.index(nm)
+                if masses[t] != mass or charges[t] != q:
+                    raise ValueError("atom %r appears with different mass or charge" % nm)
+                site_types.append(t)
+    n = len(cfg["names"])
+    if len(site_types) != n:
+        raise ValueError("FIELD describes %d atoms, CONFIG holds %d" % (len(site_types), n))
+    for i, nm in enumerate(cfg["names"]):
+        if type_names[site_types[i]] != nm:
+            raise ValueError("CONFIG atom %d is %r, FIELD expects %r" % (i + 1, nm, type_names[site_types[i]]))
+    vdw = []
+    for (a1, a2, kind, params) in fld["vdw"]:
+        if a1 not in type_names or a2 not in type_names:
+            raise ValueError("vdW pair %s-%s names an unknown atom" % (a1, a2))
+        vdw.append((type_names.index(a1), type_names.index(a2), kind, params))
+    spme = ctl["coul_method"].startswith("spme")
+    if not spme and not ctl["coul_method"].startswith("off"):
+        raise ValueError("coul_method %r: this path has SPME or no electrostatics" % ctl["coul_method"])
+    if spme and (ctl["alpha"] is None or ctl["kvec"] is None):
+        raise ValueError("SPME needs spme_alpha and spme_kvec in CONTROL")
+    kw = {}
+    if spme:
+        kw = dict(alpha=ctl["alpha"], kvec=ctl["kvec"], nsplines=ctl["nsplines"])
+    return System(name=cfg["title"].strip() or fld["title"], cell=cfg["cell"].copy(), pos=cfg["pos"].copy(),
+                  vel=cfg["vel"].copy(), types=np.array(site_types, np.int32), type_names=type_names,
+                  masses=np.array(masses), charges=np.array(charges), vdw=vdw, cutoff=ctl["cutoff"],
+                  padding=ctl["padding"], vdw_cutoff=ctl["vdw_cutoff"], electrostatics=spme,
+                  epsilon=ctl["epsilon"], timestep=ctl["timestep"], temperature=ctl["temperature"],
+                  meta=dict(steps=ctl["steps"], stats_frequency=ctl["stats_frequency"], levcfg=cfg["levcfg"],
+                            imcon=cfg["imcon"]), **kw)
+
+
+# ---------------------------------------------------------------- STATIS --
+
+def statis_header(title, units="DL_POLY Internal UNITS (10 J/mol)"):
+    """The two header records (statistics.F90:1005-1026)."""
+    return "title: '%-72s'\nenergy unitS: %s\n" % (title[:72], units)
+
+
+def statis_record(nstep, time, values):
+    """One record: (i10,1p,e14.6,0p,i10,/,(1p,5e14.6)) of the values in
+    column order (statistics.F90:1337)."""
+    v = list(values)
+    out = ["%10d%14.6E%10d" % (nstep, time, len(v))]
+    for k in range(0, len(v), 5):
+        out.append("".join("%14.6E" % x for x in v[k:k + 5]))
+    return "\n".join(out) + "\n"
+
+
+def read_statis(path):
+    """Records of a STATIS file: dict nstep -> (time, values array)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    recs = {}
+    k = 2
+    while k < len(lines):
+        h = lines[k].split()
+        if not h:
+            k += 1
+            continue
+        nstep, time, num = int(h[0]), float(h[1]), int(h[2])
+        vals = []
+        k += 1
+        while len(vals) < num:
+            vals += [float(x) for x in lines[k].split()]
+            k += 1
+        recs[nstep] = (time, np.array(vals[:num]))
+    return recs
+
+
+def statis_equal(actual, expected, precision):
+    """utils/beetest:39-48 EqualToRelativePrecision: both zero, or an absolute
+    test when the expected value is zero, else |a - e| / |e| < precision."""
+    if actual == 0.0 and expected == 0.0:
+        return True
+    if expected == 0.0:
+        return abs(actual - expected) < abs(precision)
+    return abs((actual - expected) / expected) < abs(precision)
+
+
+# STATIS columns (1-based, as the reference's utils/Scripts read them)
+STATIS_COLUMNS = {"eng_tot": 1, "temp_tot": 2, "eng_cfg": 3, "eng_src": 4, "eng_cou": 5, "eng_pv": 10,
+                  "vir_cfg": 12, "vir_src": 13, "vir_cou": 14, "volume": 19, "press": 27}

@@ -23,6 +23,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
+           "dlpx_md_last_output",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -212,6 +213,14 @@ class Context:
         ptrs = [C.c_void_p(b + 8 * k) for k in range(9)]
         self._chk(self.lib.dlpx_md_store(self.h, *ptrs, C.c_int32(72), C.c_int32(0)))
         return out[:, 0:3].copy(), out[:, 3:6].copy(), out[:, 6:9].copy()
+
+    def md_last_output(self):
+        """dlpx_md_last_output: the two-body output of the last step of the last
+        md_run(..., energies=True)."""
+        o = TwoBodyOut()
+        self._chk(self.lib.dlpx_md_last_output(self.h, C.byref(o)))
+        return dict(buffer=np.array(o.buffer[:]), stress=np.array(o.stress[:]), engcpe=o.engcpe,
+                    vircpe=o.vircpe, engsrp=o.engsrp, virsrp=o.virsrp)
 
     def set_timing(self, on=True):
         self._chk(self.lib.dlpx_set_timing(self.h, C.c_int32(1 if on else 0)))

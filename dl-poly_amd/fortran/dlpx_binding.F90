@@ -48,7 +48,7 @@ Module dlpx_binding
   Public :: dlpx_create, dlpx_destroy, dlpx_version, dlpx_set_vdw_tables, dlpx_set_erfc_tables, &
             dlpx_generate_vdw_tables, dlpx_generate_erfc_tables, dlpx_set_lrc, dlpx_compute_lrc, &
             dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_md_load, &
-            dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_error_message
+            dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_md_last_output, dlpx_error_message
 
   Interface
     Integer(c_int) Function dlpx_create(params, ctx) Bind(C, name='dlpx_create')
@@ -182,6 +182,14 @@ Module dlpx_binding
       Type(c_ptr), Value :: ctx, x, y, z, vx, vy, vz, fx, fy, fz
       Integer(c_int32_t), Value :: stride, vstride
     End Function dlpx_md_store
+
+    !> two-body output (energies, virials, stress) of the last step of the last
+    !> dlpx_md_run that recorded energies
+    Integer(c_int) Function dlpx_md_last_output(ctx, out) Bind(C, name='dlpx_md_last_output')
+      Import :: c_int, c_ptr, dlpx_two_body_out
+      Type(c_ptr), Value :: ctx
+      Type(dlpx_two_body_out), Intent(Out) :: out
+    End Function dlpx_md_last_output
 
     !> dlpx_md_run over many independent contexts in lockstep
     Integer(c_int) Function dlpx_md_run_ensemble(ctxs, nctx, nsteps, tsteps, nrebuilds) &

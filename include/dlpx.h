@@ -143,6 +143,15 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
 int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                   double *fx, double *fy, double *fz, int32_t stride, int32_t vstride);
 
+/* The two-body output (energies, virials, stress and the 20-slot buffer, as
+ * dlpx_two_body_forces returns them) of the last step of the most recent
+ * dlpx_md_run call that recorded energies, or of the starting configuration
+ * after dlpx_md_load: what two_body_forces added into stats at that step
+ * (two_body.F90:766-790), for a host that writes the step's statistics
+ * (statistics.F90:1118-1160) without copying the configuration back.
+ * DLPX_ERR_STATE before any dlpx_md_load. */
+int dlpx_md_last_output(dlpx_ctx *ctx, dlpx_two_body_out *out);
+
 /* Batched ensemble: the same as dlpx_md_run(ctxs[k], nsteps, tsteps[k], NULL,
  * &nrebuilds[k]) for every k, with the systems stepped in lockstep so that
  * the steps of different systems run on the GPU at the same time (each

@@ -11,10 +11,12 @@ inputs and the reference's outputs:
   energies and virials (``statistics.F90:1118-1145``) and the configurational
   stress, at full double precision;
 * for trajectory cases, REVCON positions and velocities after ``steps``
-  velocity-Verlet NVE steps.
+  velocity-Verlet NVE steps;
+* ``statis/<case>.STATIS``: the reference's STATIS file itself (one record
+  per step), for the cases in STATIS_CASES.
 
 Run in the build container (it needs the reference binary):
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [case ...] [--statis-only]
 """
 import os
 import subprocess
@@ -29,6 +31,7 @@ from dlpx import fileio, systems  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
 OUT = os.path.dirname(os.path.abspath(__file__))
+STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10")
 
 
 def cases():
@@ -51,13 +54,19 @@ def cases():
     }
 
 
-def run_case(name, sysm, steps):
+def run_case(name, sysm, steps, statis_only=False):
     with tempfile.TemporaryDirectory() as d:
         fileio.write_inputs(d, sysm, steps=steps)
         r = subprocess.run([REF], cwd=d, capture_output=True, text=True, timeout=3600)
         if r.returncode != 0 or not os.path.exists(os.path.join(d, "REVCON")):
             out = open(os.path.join(d, "OUTPUT")).read()[-3000:] if os.path.exists(os.path.join(d, "OUTPUT")) else ""
             raise RuntimeError("reference failed for %s:\n%s\n%s" % (name, r.stdout[-2000:], out))
+        if name in STATIS_CASES:
+            os.makedirs(os.path.join(OUT, "statis"), exist_ok=True)
+            with open(os.path.join(d, "STATIS")) as src, open(os.path.join(OUT, "statis", name + ".STATIS"), "w") as dst:
+                dst.write(src.read())
+        if statis_only:
+            return
         rc = fileio.read_revcon(os.path.join(d, "REVCON"))
         rv = fileio.read_revive(os.path.join(d, "REVIVE"))
     vdw_kind = np.array([v[2] for v in sysm.vdw], np.int32)
@@ -82,11 +91,12 @@ def run_case(name, sysm, steps):
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: python oracle/build_ref.py")
-    only = set(sys.argv[1:])
+    statis_only = "--statis-only" in sys.argv[1:]
+    only = set(a for a in sys.argv[1:] if not a.startswith("--"))
     for name, (sysm, steps) in cases().items():
         if only and name not in only:
             continue
-        run_case(name, sysm, steps)
+        run_case(name, sysm, steps, statis_only=statis_only)
 
 
 if __name__ == "__main__":

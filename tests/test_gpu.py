@@ -5,12 +5,15 @@ Bars: FP64 forces RMS relative error <= 1e-10 against the oracle (and every
 printed REVCON digit of the reference), energies/virials <= 1e-11 relative,
 neighbour pair set bit-exact; mixed precision forces RMS relative error
 <= 1e-5, energies <= 1e-6."""
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
 import fixtures
 import oracle
-from dlpx import lib, systems
+from dlpx import fileio, lib, runner, systems
 
 pytestmark = pytest.mark.gpu
 
@@ -127,6 +130,37 @@ def test_nve_trajectory_matches_reference(name):
     assert abs(e[-1, 2] - ref["engsrp"]) <= 1e-10 * abs(ref["engsrp"])
     assert abs(e[-1, 0] - ref["engcpe"]) <= 1e-10 * max(abs(ref["engcpe"]), 1.0)
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [lib.FP64, lib.MIXED])
+@pytest.mark.parametrize("name", TRAJ)
+def test_dlpoly_inputs_run_to_the_reference_statis(name, precision):
+    """CONFIG/FIELD/CONTROL read back by the host, run on the GPU, and the
+    STATIS written for every step compared with the reference's STATIS for the
+    same inputs by the reference harness's rule (relative 1e-5,
+    utils/beetest:39-48): every column in FP64; the scalar columns (energies,
+    virials, temperature, pressure) in mixed precision, whose pressure-tensor
+    off-diagonals are too small for a relative test."""
+    s, ref = fixtures.load(name)
+    print("case", name, precision, flush=True)
+    want = fileio.read_statis(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "statis",
+                                           name + ".STATIS"))
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, s, steps=int(ref["steps"]))
+        res = runner.run_directory(d, precision=precision)
+        got = fileio.read_statis(res["statis"])
+        rc = fileio.read_config(res["revcon"])
+    assert sorted(got) == sorted(want)
+    ncol = len(want[0][1])
+    cols = range(ncol) if precision == lib.FP64 else [c - 1 for c in fileio.STATIS_COLUMNS.values()]
+    bad = [(step, k + 1, got[step][1][k], w[k]) for step, (_, w) in want.items() for k in cols
+           if not fileio.statis_equal(got[step][1][k], w[k], 1e-5)]
+    assert not bad, bad[:10]
+    # the written REVCON holds the run's last configuration
+    L = np.diag(s.cell.reshape(3, 3))
+    dx = rc["pos"] - ref["pos"]
+    dx -= L * np.round(dx / L)
+    assert np.abs(dx).max() <= (1e-8 if precision == lib.FP64 else 1e-4)
 
 
 def test_list_overflow_returns_error_106():
