@@ -321,9 +321,16 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
         const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
         const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
         const double bxy = n2y[k] * n2x[j];
-        const double wgt = (j == 0 || 2 * j == ip.K[0]) ? 1.0 : 2.0;  // Hermitian partner
+        // the other half of the spectrum: column j (0 < j < K1/2) stands for
+        // its Hermitian partner (-j, -ky, -kz) too, with the same |m| and
+        // |Q|^2.  The reference folds an index at K/2 to +K/2 (ewald_spole.F90:
+        // 1326-1345), so a partner component at a Nyquist index keeps its sign,
+        // which the off-diagonal stress sees.
+        const bool partner = !(j == 0 || 2 * j == ip.K[0]);
+        const int kkp = (2 * k == ip.K[1]) ? kk : -kk;
         for (int l = jt; l < n; l += TPL) {
             const int ll = (2 * l > ip.K[2]) ? l - ip.K[2] : l;
+            const int llp = (2 * l == ip.K[2]) ? ll : -ll;
             const double r3[3] = {ll * ip.rcell[2], ll * ip.rcell[5], ll * ip.rcell[8]};
             const double r2[3] = {r3[0] + kk * ip.rcell[1], r3[1] + kk * ip.rcell[4], r3[2] + kk * ip.rcell[7]};
             const double rp[3] = {r2[0] + j * ip.rcell[0], r2[1] + j * ip.rcell[3], r2[2] + j * ip.rcell[6]};
@@ -334,13 +341,27 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
                 const double fp = sqrt(k2) * ip.rcf;
                 const double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
                 const double hr = (double)h.x, hi = (double)h.y;
-                const double pv = wgt * fac * (hr * hr + hi * hi) * (-2.0 * ((1.0 + fp * fp) / k2));
-                acc[0] += rp[0] * rp[0] * pv;
-                acc[1] += rp[0] * rp[1] * pv;
-                acc[2] += rp[0] * rp[2] * pv;
-                acc[3] += rp[1] * rp[1] * pv;
-                acc[4] += rp[1] * rp[2] * pv;
-                acc[5] += rp[2] * rp[2] * pv;
+                const double pv = fac * (hr * hr + hi * hi) * (-2.0 * ((1.0 + fp * fp) / k2));
+                double s0 = rp[0] * rp[0], s1 = rp[0] * rp[1], s2 = rp[0] * rp[2];
+                double s3 = rp[1] * rp[1], s4 = rp[1] * rp[2], s5 = rp[2] * rp[2];
+                if (partner) {
+                    // orthorhombic cells: the partner has the same |m| (and so the same fac)
+                    const double q[3] = {-j * ip.rcell[0] + kkp * ip.rcell[1] + llp * ip.rcell[2],
+                                         -j * ip.rcell[3] + kkp * ip.rcell[4] + llp * ip.rcell[5],
+                                         -j * ip.rcell[6] + kkp * ip.rcell[7] + llp * ip.rcell[8]};
+                    s0 += q[0] * q[0];
+                    s1 += q[0] * q[1];
+                    s2 += q[0] * q[2];
+                    s3 += q[1] * q[1];
+                    s4 += q[1] * q[2];
+                    s5 += q[2] * q[2];
+                }
+                acc[0] += s0 * pv;
+                acc[1] += s1 * pv;
+                acc[2] += s2 * pv;
+                acc[3] += s3 * pv;
+                acc[4] += s4 * pv;
+                acc[5] += s5 * pv;
                 h.x = (T)(fac * hr);
                 h.y = (T)(fac * hi);
             } else {
