@@ -408,11 +408,12 @@ int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, 
     const int mg = table_size(c->prm.vdw_cutoff);
     std::vector<double> pot((size_t)nkeys * (mg + 1), 0.0), frc((size_t)nkeys * (mg + 1), 0.0);
     for (int k = 0; k < nkeys; k++) {
-        if (kind[k] == 2 || kind[k] == 5) {
+        if (vdw_kind_supported(kind[k])) {
             vdw_table(kind[k], params + 5 * k, c->prm.vdw_cutoff, mg, pot.data() + (size_t)k * (mg + 1),
                       frc.data() + (size_t)k * (mg + 1));
         } else if (kind[k] >= 0) {
-            return fail(c, DLPX_ERR_UNSUPPORTED, "only Lennard-Jones (2) and Born-Huggins-Meyer (5) are generated");
+            return fail(c, DLPX_ERR_UNSUPPORTED,
+                        "vdW kind " + std::to_string(kind[k]) + " is not generated (1, 2, 3, 4, 5, 6, 8 are)");
         }
     }
     std::vector<int32_t> ltp(kind, kind + nkeys);

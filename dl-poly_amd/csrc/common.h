@@ -198,6 +198,7 @@ int fail(Ctx *c, int code, const std::string &msg);
 
 // host table generation (tables.cpp)
 int table_size(double cut);
+bool vdw_kind_supported(int kind);
 void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc);
 void erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d);
 void bspline_norm2(int K, int S, double *norm2);

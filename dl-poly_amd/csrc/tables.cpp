@@ -13,21 +13,72 @@ static int nint_pos(double x) { return (int)std::floor(x + 0.5); }
 // bounds.F90:811,820 -- max(1004, nint(cut/delr_max) + 4), delr_max = 0.01 A
 int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 
-// two_body_potentials.F90:260-270 (Lennard-Jones) and 499-514 (Born-Huggins-Meyer):
-// returns U(r) and gamma(r) = -r dU/dr.
+// U(r) and gamma(r) = -r dU/dr of the vdW forms the library generates
+// (vdw.F90:64-113 keys; two_body_potentials.F90 closed forms):
+//   1 12-6       A/r^12 - B/r^6                          (:307-317)
+//   2 LJ         4 eps [(sig/r)^12 - (sig/r)^6]          (:260-270)
+//   3 n-m        e0/(n-m) [m (r0/r)^n - n (r0/r)^m]      (:330-345; integer powers)
+//   4 Buckingham A exp(-r/rho) - C/r^6                   (:471-485)
+//   5 BHM        A exp(B (sig-r)) - C/r^6 - D/r^8        (:499-514)
+//   6 12-10      A/r^12 - B/r^10                         (:530-546)
+//   8 Morse      e0 {[1 - exp(-k (r-r0))]^2 - 1}         (:416-428)
+bool vdw_kind_supported(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 4 || kind == 5 || kind == 6 || kind == 8; }
+
+static double ipow(double x, int n)
+{
+    double r = 1.0;
+    for (int k = 0; k < n; k++) r *= x;
+    return r;
+}
+
 static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
 {
-    if (kind == 2) {
-        double s = p[1] / r, s2 = s * s, s6 = s2 * s2 * s2;
-        *e = 4.0 * p[0] * s6 * (s6 - 1.0);
-        *g = 24.0 * p[0] * s6 * (2.0 * s6 - 1.0);
-    } else {
-        double ri2 = 1.0 / (r * r);
-        double t1 = p[0] * std::exp(p[1] * (p[2] - r));
-        double t2 = -p[3] * (ri2 * ri2 * ri2);
-        double t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
-        *e = t1 + t2 + t3;
-        *g = t1 * r * p[1] + 6.0 * t2 + 8.0 * t3;
+    switch (kind) {
+        case 1: {
+            const double r6 = ipow(1.0 / r, 6);
+            *e = (p[0] * r6 - p[1]) * r6;
+            *g = 6.0 * r6 * (2.0 * p[0] * r6 - p[1]);
+            break;
+        }
+        case 2: {
+            double s = p[1] / r, s2 = s * s, s6 = s2 * s2 * s2;
+            *e = 4.0 * p[0] * s6 * (s6 - 1.0);
+            *g = 24.0 * p[0] * s6 * (2.0 * s6 - 1.0);
+            break;
+        }
+        case 3: {
+            const double a = p[3] / r, b = 1.0 / (p[1] - p[2]);
+            const double rn = ipow(a, (int)p[1]), rm = ipow(a, (int)p[2]);
+            *e = p[0] * (p[2] * rn - p[1] * rm) * b;
+            *g = p[0] * p[2] * p[1] * (rn - rm) * b;
+            break;
+        }
+        case 4: {
+            const double b = r / p[1], t1 = p[0] * std::exp(-b), t2 = -p[2] / ipow(r, 6);
+            *e = t1 + t2;
+            *g = t1 * b + 6.0 * t2;
+            break;
+        }
+        case 6: {
+            const double ri2 = 1.0 / (r * r), f12 = p[0] * ipow(ri2, 6), f10 = -p[1] * ipow(ri2, 5);
+            *e = f12 + f10;
+            *g = 12.0 * f12 + 10.0 * f10;
+            break;
+        }
+        case 8: {
+            const double t = std::exp(-p[2] * (r - p[1]));
+            *e = p[0] * ((1.0 - t) * (1.0 - t) - 1.0);
+            *g = -2.0 * r * p[0] * p[2] * (1.0 - t) * t;
+            break;
+        }
+        default: {  // 5, Born-Huggins-Meyer
+            double ri2 = 1.0 / (r * r);
+            double t1 = p[0] * std::exp(p[1] * (p[2] - r));
+            double t2 = -p[3] * (ri2 * ri2 * ri2);
+            double t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
+            *e = t1 + t2 + t3;
+            *g = t1 * r * p[1] + 6.0 * t2 + 8.0 * t3;
+        }
     }
 }
 
@@ -145,7 +196,7 @@ void dcell(const double *a, double *b)
     if (y[i1] >= y[i2]) { b[7] = d[i1]; b[8] = d[i2]; } else { b[7] = d[i2]; b[8] = d[i1]; }
 }
 
-// vdw.F90:617-967 for Lennard-Jones (2) and Born-Huggins-Meyer (5)
+// vdw.F90:617-967 (the closed-form tails of the generated forms)
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
              double volm, double *elrc, double *vlrc)
 {
@@ -155,12 +206,50 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
         for (int j = 1; j <= i; j++, key++) {
             const double *p = params + 5 * key, r = rvdw;
             double eadd = 0.0, padd = 0.0;
-            if (kinds[key] == 2) {
-                eadd = 4.0 * p[0] * (std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / (3.0 * std::pow(r, 3)));
-                padd = 8.0 * p[0] * (6.0 * std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / std::pow(r, 3));
-            } else if (kinds[key] == 5) {
-                eadd = -p[3] / (3.0 * std::pow(r, 3)) - p[4] / (5.0 * std::pow(r, 5));
-                padd = -2.0 * p[3] / std::pow(r, 3) - 8.0 * p[4] / (5.0 * std::pow(r, 5));
+            switch (kinds[key]) {
+                case 1:
+                    eadd = p[0] / (9.0 * std::pow(r, 9)) - p[1] / (3.0 * std::pow(r, 3));
+                    padd = 12.0 * p[0] / (9.0 * std::pow(r, 9)) - 6.0 * p[1] / (3.0 * std::pow(r, 3));
+                    break;
+                case 2:
+                    eadd = 4.0 * p[0] * (std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / (3.0 * std::pow(r, 3)));
+                    padd = 8.0 * p[0] * (6.0 * std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) - std::pow(p[1], 6) / std::pow(r, 3));
+                    break;
+                case 3: {
+                    const double e0 = p[0], nr = p[1], mr = p[2], r0 = p[3];
+                    eadd = e0 / (nr - mr) *
+                           (mr * std::pow(r0, nr) / ((nr - 3.0) * std::pow(r, nr - 3.0)) -
+                            nr * std::pow(r0, mr) / ((mr - 3.0) * std::pow(r, mr - 3.0)));
+                    padd = e0 / (nr - mr) * nr * mr *
+                           (std::pow(r0, nr) / ((nr - 3.0) * std::pow(r, nr - 3.0)) -
+                            std::pow(r0, mr) / ((mr - 3.0) * std::pow(r, mr - 3.0)));
+                    break;
+                }
+                case 4:
+                    eadd = -p[2] / (3.0 * std::pow(r, 3));
+                    padd = -2.0 * p[2] / std::pow(r, 3);
+                    break;
+                case 5:
+                    eadd = -p[3] / (3.0 * std::pow(r, 3)) - p[4] / (5.0 * std::pow(r, 5));
+                    padd = -2.0 * p[3] / std::pow(r, 3) - 8.0 * p[4] / (5.0 * std::pow(r, 5));
+                    break;
+                case 6:
+                    eadd = p[0] / (9.0 * std::pow(r, 9)) - p[1] / (7.0 * std::pow(r, 7));
+                    padd = 12.0 * p[0] / (9.0 * std::pow(r, 9)) - 10.0 * p[1] / (7.0 * std::pow(r, 7));
+                    break;
+                case 8: {
+                    const double e0 = p[0], r0 = p[1], kk = p[2];
+                    if (kk > 1.17549435e-38) {
+                        const double t = std::exp(-kk * (r - r0)), k3 = kk * kk * kk, kr = kk * r;
+                        eadd = -2.0 * e0 * t / k3 * ((kr + 1) * (kr + 1) + 1) +
+                               e0 * t * t / (4.0 * k3) * ((kr + 1) * (kr + 1) + kk * kk * r * r);
+                        padd = -2.0 * e0 * t / k3 * (kr * kr * kr + 3 * kr * kr + 6 * kr + 6) +
+                               e0 * t * t / (4.0 * k3) * (4.0 * kr * kr * kr + 6 * kr * kr + 6 * kr + 3);
+                    }
+                    break;
+                }
+                default:
+                    break;
             }
             if (i != j) {
                 eadd *= 2.0;

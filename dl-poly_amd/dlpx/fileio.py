@@ -10,8 +10,9 @@ inputs and comparators:
   ``write_config`` / ``write_revcon`` (``configuration.F90:2540-2615``:
   ``(4i10,1p,2e16.7)`` header, ``(a8,i10)`` and ``3g20.10`` records);
 * FIELD: ``read_field`` (``ffield.F90:150-5365``, the subset this path
-  supports: energy units, molecule types of unbonded atoms, the ``lj`` and
-  ``bhm`` vdW forms) and ``write_field``;
+  supports: energy units, molecule types of unbonded atoms, the ``12-6``,
+  ``lj``, ``nm``, ``buck``, ``bhm``, ``hbnd`` and ``mors`` vdW forms) and
+  ``write_field``;
 * CONTROL: ``read_control`` / ``write_control`` (new-style keyword file,
   ``control.F90``);
 * STATIS: ``statis_header`` / ``statis_record`` (``statistics.F90:1020-1027,
@@ -243,16 +244,17 @@ def write_revcon(path, title, cell, names, pos, vel=None, frc=None, nstep=0, tst
 
 # ----------------------------------------------------------------- FIELD --
 
-_VDW_KEYS = {"lj": 2, "bhm": 5}
-# parameters that carry energy units, per vdW form (ffield.F90 vdw section:
-# parpot(1) always; bhm also parpot(4), parpot(5))
-_VDW_ENERGY_PARAMS = {2: (0,), 5: (0, 3, 4)}
+_VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors": 8}
+# parameters that carry energy units, per vdW form (ffield.F90:3757-3800:
+# parpot(1) always; 12-6 and hbnd parpot(2); buck parpot(3); bhm parpot(4:5))
+_VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,)}
 
 
 def read_field(path):
     """FIELD (ffield.F90:150-5365), the subset of this path: energy units;
     molecule types made of unbonded atoms (``name mass charge [repeat
-    [frozen]]``); the ``lj`` and ``bhm`` vdW forms.  Bonded terms, frozen
+    [frozen]]``); the vdW forms the library generates (``12-6``, ``lj``,
+    ``nm``, ``buck``, ``bhm``, ``hbnd``, ``mors``).  Bonded terms, frozen
     atoms, shells, other vdW forms and many-body terms raise ValueError.
     Returns dict(title, units, molecules [(name, nummols, [(atom, mass, charge)])],
     vdw [(atom1, atom2, key, params in internal units)])."""

@@ -18,10 +18,19 @@ import numpy as np
 R4PIE0 = 138935.4835
 BOLTZ = 8.31451115e-1
 
-# vdw.F90:64-113 potential keys used by the configs
+# vdw.F90:64-113 potential keys of the forms the library generates, their
+# FIELD keywords (ffield.F90 vdw section) and parameter counts
+VDW_12_6 = 1
 VDW_LENNARD_JONES = 2
+VDW_N_M = 3
+VDW_BUCKINGHAM = 4
 VDW_BORN_HUGGINS_MEYER = 5
-VDW_NAMES = {VDW_LENNARD_JONES: "lj", VDW_BORN_HUGGINS_MEYER: "bhm"}
+VDW_HYDROGEN_BOND = 6
+VDW_MORSE = 8
+VDW_NAMES = {VDW_12_6: "12-6", VDW_LENNARD_JONES: "lj", VDW_N_M: "nm", VDW_BUCKINGHAM: "buck",
+             VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors"}
+VDW_NPARAMS = {VDW_12_6: 2, VDW_LENNARD_JONES: 2, VDW_N_M: 4, VDW_BUCKINGHAM: 3, VDW_BORN_HUGGINS_MEYER: 5,
+               VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3}
 
 # Tosi-Fumi NaCl Born-Huggins-Meyer parameters, DL_POLY form
 # A*exp(B*(sigma-r)) - C/r^6 - D/r^8, converted from kJ/mol to internal
@@ -164,6 +173,45 @@ def argon(ncells=10, density=1.374, temperature=120.0, seed=20261202, epsilon=99
                   cutoff=cutoff, padding=padding, vdw_cutoff=cutoff, electrostatics=False,
                   alpha=0.0, kvec=(0, 0, 0), nsplines=8, timestep=timestep,
                   temperature=temperature, meta=dict(seed=seed, a=a, ncells=ncells))
+
+
+# three-species mixture: one vdW form per pair key (internal units, A)
+MIX_NAMES = ["Xa", "Xb", "Xc"]
+MIX_MASSES = (39.948, 20.18, 83.8)
+MIX_VDW = [
+    (0, 0, VDW_12_6, (4 * 99.6 * 3.405 ** 12, 4 * 99.6 * 3.405 ** 6)),
+    (1, 0, VDW_BUCKINGHAM, (3.2e7, 0.30, 4.28e5)),
+    (2, 0, VDW_HYDROGEN_BOND, (8.4e9, 6.3e8)),
+    (1, 1, VDW_N_M, (60.0, 12.0, 6.0, 3.1)),
+    (2, 1, VDW_MORSE, (80.0, 3.9, 1.6)),
+    (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
+]
+
+
+def mixture(ncells=10, a=5.78015, temperature=100.0, seed=20261202, charges=(0.3, -0.3, 0.0), kvec=(36, 36, 36),
+            alpha=0.35, cutoff=9.0, padding=0.5, timestep=0.004, jitter=0.1):
+    """Three species on an FCC lattice (4 sites per cell, edge ``a``), the
+    species drawn by a seeded permutation of the sites, with a different vdW
+    form for each pair (12-6, Buckingham, 12-10, n-m, Morse, Lennard-Jones)
+    and SPME Coulomb.  Atoms are listed species by species (the FIELD's
+    molecule order)."""
+    n = 4 * ncells ** 3
+    L = a * ncells
+    r_perm, r_jit, r_vel = _streams(seed, 3)
+    g = np.stack(np.meshgrid(np.arange(ncells), np.arange(ncells), np.arange(ncells),
+                             indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    sites = (g[:, None, :] + FCC[None, :, :] + 0.25).reshape(-1, 3) * a - 0.5 * L
+    sites = sites[r_perm.permutation(n)]
+    counts = [n // 3 + (1 if k < n % 3 else 0) for k in range(3)]
+    types = np.repeat(np.arange(3, dtype=np.int32), counts)
+    pos = _wrap(sites + r_jit.uniform(-jitter, jitter, size=sites.shape), L)
+    masses = np.array(MIX_MASSES)
+    vel = _maxwell(r_vel, masses[types], temperature) if temperature > 0 else np.zeros((n, 3))
+    return System(name="mix_%d" % n, cell=np.diag([L, L, L]).reshape(9).astype(np.float64), pos=pos, vel=vel,
+                  types=types, type_names=list(MIX_NAMES), masses=masses, charges=np.array(charges, np.float64),
+                  vdw=list(MIX_VDW), cutoff=cutoff, padding=padding, vdw_cutoff=cutoff, electrostatics=True,
+                  alpha=alpha, kvec=tuple(kvec), nsplines=8, timestep=timestep, temperature=temperature,
+                  meta=dict(seed=seed, a=a, ncells=ncells, jitter=jitter))
 
 
 def smooth235_at_least(n):

@@ -93,9 +93,13 @@ int dlpx_version(void);
 int dlpx_set_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, int32_t max_grid, const double *tab_potential,
                         const double *tab_force, const int32_t *ltp);
 int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, const double *erfc_deriv);
-/* The same tables generated in the library from potential parameters:
- * kind[key] (2 = Lennard-Jones eps,sig; 5 = Born-Huggins-Meyer A,B,sigma,C,D;
- * -1 none), params 5 per key.  Sizes follow bounds.F90:811,820. */
+/* The same tables generated in the library from potential parameters, for
+ * the vdW forms with closed-form long-range corrections (vdw.F90:64-113 keys,
+ * parameters in vdws%param order, two_body_potentials.F90): kind[key] 1 =
+ * 12-6 A,B; 2 = Lennard-Jones eps,sig; 3 = n-m e0,n,m,r0; 4 = Buckingham
+ * A,rho,C; 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B; 8 = Morse
+ * e0,r0,k; -1 none.  params 5 per key.  Other forms: tabulate them on the host
+ * and use dlpx_set_vdw_tables.  Sizes follow bounds.F90:811,820. */
 int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, const double *params);
 int dlpx_generate_erfc_tables(dlpx_ctx *ctx);
 /* vdW long-range corrections vdws%elrc / vdws%vlrc (vdw.F90:617-967).  Pass the

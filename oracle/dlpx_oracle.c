@@ -86,13 +86,42 @@ void orc_dcell(const double *a, double *b)
 /* ------------------------------------------------------------------ */
 /* potentials and tables                                               */
 /* ------------------------------------------------------------------ */
-/* two_body_potentials.F90:260-270 (LJ), 499-514 (BHM).  kind: 2 LJ, 5 BHM */
+/* U(r) and gamma = -r dU/dr, two_body_potentials.F90; kind = vdw.F90:64-113 key:
+ * 1 12-6 (:307-317), 2 LJ (:260-270), 3 n-m (:330-345, integer powers of r0/r),
+ * 4 Buckingham (:471-485), 5 BHM (:499-514), 6 12-10 (:530-546), 8 Morse (:416-428).
+ * Returns -1 for other kinds. */
+static double orc_ipow(double x, int n)
+{
+    double r = 1.0;
+    while (n-- > 0) r *= x;
+    return r;
+}
+
 int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
 {
+    if (kind == 1) { /* u = a/r^12 - b/r^6 */
+        double r6 = orc_ipow(1.0 / r, 6);
+        *e = (p[0] * r6 - p[1]) * r6;
+        *g = 6.0 * r6 * (2.0 * p[0] * r6 - p[1]);
+        return 0;
+    }
     if (kind == 2) {
         double sor6 = pow(p[1] / r, 6);
         *e = 4.0 * p[0] * sor6 * (sor6 - 1.0);
         *g = 24.0 * p[0] * sor6 * (2.0 * sor6 - 1.0);
+        return 0;
+    }
+    if (kind == 3) { /* u = e0/(n-m) [m (r0/r)^n - n (r0/r)^m], params e0, n, m, r0 */
+        double a = p[3] / r, b = 1.0 / (p[1] - p[2]);
+        double rn = orc_ipow(a, (int)p[1]), rm = orc_ipow(a, (int)p[2]);
+        *e = p[0] * (p[2] * rn - p[1] * rm) * b;
+        *g = p[0] * p[2] * p[1] * (rn - rm) * b;
+        return 0;
+    }
+    if (kind == 4) { /* u = A exp(-r/rho) - C/r^6, params A, rho, C */
+        double b = r / p[1], t1 = p[0] * exp(-b), t2 = -p[2] / pow(r, 6);
+        *e = t1 + t2;
+        *g = t1 * b + 6.0 * t2;
         return 0;
     }
     if (kind == 5) {
@@ -102,6 +131,19 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         double t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
         *e = t1 + t2 + t3;
         *g = t1 * r * p[1] + 6.0 * t2 + 8.0 * t3;
+        return 0;
+    }
+    if (kind == 6) { /* u = a/r^12 - b/r^10 */
+        double ri2 = 1.0 / (r * r);
+        double f12 = p[0] * pow(ri2, 6), f10 = -p[1] * pow(ri2, 5);
+        *e = f12 + f10;
+        *g = 12.0 * f12 + 10.0 * f10;
+        return 0;
+    }
+    if (kind == 8) { /* u = e0 {[1 - exp(-k (r - r0))]^2 - 1}, params e0, r0, k */
+        double t = exp(-p[2] * (r - p[1]));
+        *e = p[0] * ((1.0 - t) * (1.0 - t) - 1.0);
+        *g = -2.0 * r * p[0] * p[2] * (1.0 - t) * t;
         return 0;
     }
     return -1;
@@ -160,7 +202,7 @@ double orc_interp3(const double *tab0, double rdr, double r)
     return t1 + (t2 - t1) * ppp * 0.5;
 }
 
-/* vdw.F90:617-967 for LJ (2) and BHM (5): energy and virial long-range corrections.
+/* vdw.F90:617-967 for the forms of orc_vdw_eval: energy and virial long-range corrections.
  * kinds/params are indexed by pair key (0-based key = hi*(hi-1)/2+lo-1, 1-based types). */
 void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*5 per key*/,
                  double rvdw, double volm, double *elrc, double *vlrc)
@@ -172,14 +214,34 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
         for (int j = 1; j <= i; j++) {
             double eadd = 0.0, padd = 0.0, r = rvdw;
             const double *p = params + 5 * ivdw;
-            if (kinds[ivdw] == 2) {
+            int kd = kinds[ivdw];
+            if (kd == 1) { /* vdw.F90:697-707 */
+                eadd = p[0] / (9.0 * pow(r, 9)) - p[1] / (3.0 * pow(r, 3));
+                padd = 12.0 * p[0] / (9.0 * pow(r, 9)) - 6.0 * p[1] / (3.0 * pow(r, 3));
+            } else if (kd == 2) { /* :709-718 */
                 double eps = p[0], sig = p[1];
                 eadd = 4.0 * eps * (pow(sig, 12) / (9.0 * pow(r, 9)) - pow(sig, 6) / (3.0 * pow(r, 3)));
                 padd = 8.0 * eps * (6.0 * pow(sig, 12) / (9.0 * pow(r, 9)) - pow(sig, 6) / pow(r, 3));
-            } else if (kinds[ivdw] == 5) {
+            } else if (kd == 3) { /* :720-732 */
+                double e0 = p[0], nr = p[1], mr = p[2], r0 = p[3];
+                double tn = pow(r0, nr) / ((nr - 3.0) * pow(r, nr - 3.0)), tm = pow(r0, mr) / ((mr - 3.0) * pow(r, mr - 3.0));
+                eadd = e0 / (nr - mr) * (mr * tn - nr * tm);
+                padd = e0 / (nr - mr) * nr * mr * (tn - tm);
+            } else if (kd == 4) { /* :734-742: the dispersion tail only */
+                eadd = -p[2] / (3.0 * pow(r, 3));
+                padd = -2.0 * p[2] / pow(r, 3);
+            } else if (kd == 5) { /* :744-753 */
                 double c = p[3], d = p[4];
                 eadd = -c / (3.0 * pow(r, 3)) - d / (5.0 * pow(r, 5));
                 padd = -2.0 * c / pow(r, 3) - 8.0 * d / (5.0 * pow(r, 5));
+            } else if (kd == 6) { /* :755-764 */
+                eadd = p[0] / (9.0 * pow(r, 9)) - p[1] / (7.0 * pow(r, 7));
+                padd = 12.0 * p[0] / (9.0 * pow(r, 9)) - 10.0 * p[1] / (7.0 * pow(r, 7));
+            } else if (kd == 8 && p[2] > 1.17549435e-38) { /* :766-782 */
+                double e0 = p[0], r0 = p[1], kk = p[2], t = exp(-kk * (r - r0)), k3 = kk * kk * kk;
+                eadd = -2.0 * e0 * t / k3 * (pow(kk * r + 1, 2) + 1) + e0 * t * t / (4.0 * k3) * (pow(kk * r + 1, 2) + kk * kk * r * r);
+                padd = -2.0 * e0 * t / k3 * (pow(kk, 3) * pow(r, 3) + 3 * kk * kk * r * r + 6 * kk * r + 6) +
+                       e0 * t * t / (4.0 * k3) * (4.0 * pow(kk, 3) * pow(r, 3) + 6 * kk * kk * r * r + 6 * kk * r + 3);
             }
             if (i != j) { eadd *= 2.0; padd *= 2.0; }
             double denprd = twopi * (num_type[i - 1] * num_type[j - 1]) / (volm * volm);

@@ -19,7 +19,7 @@ NAMES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
 def load(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"))
     st = z["settings"]
-    vdw = [(int(t[0]), int(t[1]), int(k), tuple(p[:5] if k == 5 else p[:2]))
+    vdw = [(int(t[0]), int(t[1]), int(k), tuple(p[:systems.VDW_NPARAMS[int(k)]]))
            for t, k, p in zip(z["vdw_types"], z["vdw_kind"], z["vdw_params"])]
     s = systems.System(name=name, cell=z["cell"], pos=z["pos"], vel=z["vel"], types=z["types"],
                        type_names=[str(x) for x in z["type_names"]], masses=z["masses"],

@@ -31,7 +31,7 @@ from dlpx import fileio, systems  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
 OUT = os.path.dirname(os.path.abspath(__file__))
-STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10")
+STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10")
 
 
 def cases():
@@ -51,6 +51,9 @@ def cases():
         # short NVE trajectories: integrator + neighbour-list refresh
         "argon_4000_nve20": (systems.argon(), 20),
         "nacl_1000_nve10": (systems.nacl(ncells=5, kvec=(32, 32, 32)), 10),
+        # three species, one vdW form per pair (12-6, buck, 12-10, n-m, Morse, LJ) + SPME
+        "mix_4000_k36": (systems.mixture(), 0),
+        "mix_4000_nve10": (systems.mixture(), 10),
     }
 
 

@@ -78,7 +78,7 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
     if bad:
         text = text.replace("finish\nion", bad + "finish\nion", 1)
     else:
-        text = text.replace("C  C  lj   1.0   3.4", "C  C  buck 1.0 0.3 2.0")
+        text = text.replace("C  C  lj   1.0   3.4", "C  C  wca  1.0 3.4 0.0")
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "FIELD")
         open(p, "w").write(text)
@@ -138,7 +138,7 @@ def test_statis_equal_is_the_harness_rule():
     assert not fileio.statis_equal(0.0, 1e-9, 1e-5)
 
 
-@pytest.mark.parametrize("name", ["nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10"])
+@pytest.mark.parametrize("name", ["nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10"])
 def test_statis_step0_quantities_match_reference_file(name):
     """The STATIS assembly of one step (temperature, enthalpy, pressure,
     pressure tensor with the kinetic stress, cell angles, energies and

@@ -57,12 +57,18 @@ def _eval(kind, p, r):
     return e.value, g.value
 
 
-def test_vdw_known_answers():
-    # test_vdw.F90:40-63 at r = 1 A with parameters 1..7: virial = -gamma
-    e, g = _eval(2, [1, 2], 1.0)
-    assert e == 16128.0 and -g == -195072.0
-    e, g = _eval(5, [1, 2, 3, 4, 5], 1.0)
-    assert abs(e - 45.598150033144236) < 1e-12 and abs(-g - (-45.196300066288472)) < 1e-12
+# test_vdw.F90:40-63: energy and virial (-gamma) of each form at r = 1 A with
+# parameters 1, 2, 3, ... (index = vdW key)
+VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
+             4: (-2.3934693402873668, 17.696734670143684), 5: (45.598150033144236, -45.196300066288472),
+             6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847)}
+
+
+@pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
+def test_vdw_known_answers(kind):
+    e, g = _eval(kind, [1, 2, 3, 4, 5], 1.0)
+    ee, vv = VDW_KNOWN[kind]
+    assert abs(e - ee) <= 1e-12 * max(abs(ee), 1.0) and abs(-g - vv) <= 1e-12 * max(abs(vv), 1.0)
 
 
 @pytest.mark.parametrize("r", [2.1, 3.3, 7.9, 11.5])
@@ -236,20 +242,40 @@ def test_forces_are_minus_energy_gradient():
             assert abs(fd - r0["frc"][i, d]) < 1e-6 * np.abs(r0["frc"][i]).max() + 1e-4
 
 
-def test_vdw_lrc_matches_tail_integrals():
-    s = systems.nacl(ncells=5, kvec=(32, 32, 32))
+def _tail(kind, p):
+    """The part of U(r) whose tail the reference integrates (vdw.F90:697-782):
+    the whole form, except Buckingham and BHM (dispersion terms only)."""
+    if kind == 1:
+        return lambda r: p[0] / r ** 12 - p[1] / r ** 6
+    if kind == 2:
+        return lambda r: 4 * p[0] * ((p[1] / r) ** 12 - (p[1] / r) ** 6)
+    if kind == 3:
+        return lambda r: p[0] / (p[1] - p[2]) * (p[2] * (p[3] / r) ** p[1] - p[1] * (p[3] / r) ** p[2])
+    if kind == 4:
+        return lambda r: -p[2] / r ** 6
+    if kind == 5:
+        return lambda r: -p[3] / r ** 6 - p[4] / r ** 8
+    if kind == 6:
+        return lambda r: p[0] / r ** 12 - p[1] / r ** 10
+    return lambda r: p[0] * ((1 - math.exp(-p[2] * (r - p[1]))) ** 2 - 1)
+
+
+@pytest.mark.parametrize("make", [lambda: systems.nacl(ncells=5, kvec=(32, 32, 32)),
+                                  lambda: systems.mixture(ncells=6)])
+def test_vdw_lrc_matches_tail_integrals(make):
+    s = make()
     _, _, _, from_key = oracle.vdw_tables(s)
     elrc, vlrc = oracle.vdw_lrc(s, from_key)
     V = s.volume
-    n = np.bincount(s.types, minlength=2).astype(float)
+    n = np.bincount(s.types, minlength=s.ntypes).astype(float)
     rc = s.vdw_cutoff
     e = v = 0.0
     for (ti, tj, kind, p) in s.vdw:
-        A, B, s0, Cc, D = p
-        ut = lambda r: -Cc / r ** 6 - D / r ** 8   # the reference keeps only the dispersion tail
-        dut = lambda r: 6 * Cc / r ** 7 + 8 * D / r ** 9
+        ut = _tail(kind, p)
+        h = 1e-4
+        dut = lambda r: (ut(r + h) - ut(r - h)) / (2 * h)
         mult = 1.0 if ti == tj else 2.0
         e += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r * r * ut(r), rc, np.inf)[0]
         v += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r ** 3 * dut(r), rc, np.inf)[0]
-    assert abs(elrc - e) < 1e-10 * abs(e)
-    assert abs(vlrc - v) < 1e-10 * abs(v)
+    assert abs(elrc - e) < 1e-9 * abs(e)
+    assert abs(vlrc - v) < 1e-7 * abs(v)
