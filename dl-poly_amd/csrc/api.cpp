@@ -451,6 +451,16 @@ int dlpx_compute_lrc(dlpx_ctx *ctx, const double cell[9], const double *num_type
         return fail(c, DLPX_ERR_STATE, "tables were not generated here: pass the host's vdws%elrc/vlrc to dlpx_set_lrc");
     vdw_lrc(c->prm.ntypes, num_type, c->ltp.data(), c->gen_params.data(), c->prm.vdw_cutoff, c->volm, &c->elrc,
             &c->vlrc);
+    if (c->vdw_force_shift) c->elrc = c->vlrc = 0.0;  // vdw.F90:662: no correction with force-shifting
+    return 0;
+}
+
+int dlpx_set_vdw_force_shift(dlpx_ctx *ctx, int32_t on)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || (on != 0 && on != 1)) return DLPX_ERR_BAD_ARG;
+    c->vdw_force_shift = on != 0;
+    if (c->vdw_force_shift) c->elrc = c->vlrc = 0.0;
     return 0;
 }
 

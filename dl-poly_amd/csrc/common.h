@@ -73,6 +73,7 @@ struct Ctx {
     // resident steps replay one captured graph: forces, stage 2 and the next
     // stage 1; graph_key records what the capture baked in (buffers, sizes)
     bool use_graphs = true;
+    bool vdw_force_shift = false;    // vdws%l_force_shift
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
     hipGraphExec_t step_graph = nullptr;
     std::vector<uint64_t> graph_key;

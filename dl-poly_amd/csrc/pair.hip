@@ -23,6 +23,7 @@ struct PairParams {
     double rc, rvdw;
     double rdr_v, rdr_e;               // reciprocal table spacings
     double scaling;                    // r4pie0/eps
+    double r_rvdw;                     // 1/rvdw (force-shifting)
     int electro;
     int cap_lane;                      // list capacity per lane
 };
@@ -74,7 +75,7 @@ constexpr int kPairThreads = 512;
 // pair lane p walks list lane p % kLanesPerAtom, entries c = p / kLanesPerAtom,
 // c + GP / kLanesPerAtom, ...  More lanes per atom give more wavefronts for
 // the same atoms (latency hiding on small systems) at the same total work.
-template <typename R, int GP, int NT>
+template <typename R, int GP, int NT, bool FS>
 __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
@@ -142,9 +143,17 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
             if (key >= 0 && rrr < rvdw) {
                 R gf, ge;
                 interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
-                const R g = gf * rinv2;
+                R g = gf * rinv2;
+                R ev = ge;
+                if (FS) {
+                    // vdw.F90:1922,1961-1964 with the samples at r = rvdw (index max_grid-4)
+                    const R2 cut = tv[key * pp.mg1 + pp.mg1 - 5];
+                    const R r_rvdw = (R)pp.r_rvdw;
+                    g = g - cut.x * ((rrr * rinv2) * r_rvdw);
+                    ev = ev + cut.x * (rrr * r_rvdw - (R)1) - cut.y;
+                }
                 gam = g;
-                acc[0] += (double)ge;
+                acc[0] += (double)ev;
                 acc[1] -= (double)(g * (rrr * rrr));
             }
             // no |q_j| > Tiny test: such a q_j contributes exact (or subnormal) zeros
@@ -199,20 +208,29 @@ static int launch_pair(Ctx *c)
     pp.rdr_v = 1.0 / (c->prm.vdw_cutoff / (double)(c->max_grid - 4));
     pp.rdr_e = pp.electro ? 1.0 / (c->prm.cutoff / (double)(c->nsamples - 4)) : 0.0;
     pp.scaling = kR4pie0 / c->prm.epsilon;
+    pp.r_rvdw = 1.0 / c->prm.vdw_cutoff;
     pp.cap_lane = c->cap_lane;
     const size_t lds = sizeof(R2) * ((size_t)pp.nkeys * pp.mg1 + (pp.electro ? pp.ns1 : 0)) +
                        sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * (kPairThreads / 64);
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-#define DLPX_PAIR(GP)                                                                                        \
-    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads>));                                                            \
-    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads>), dim3(pair_blocks(c)), dim3(kPairThreads), lds, c->stream, \
-                       c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,             \
+#define DLPX_PAIR(GP, FS)                                                                                    \
+    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS>));                                                        \
+    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,       \
+                       c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,   \
                        (const R2 *)c->d_erfc, c->d_keytab, c->frc, c->red)
-    switch (c->pair_lanes) {
-        case 16: DLPX_PAIR(16); break;
-        case 8: DLPX_PAIR(8); break;
-        default: DLPX_PAIR(4); break;
+    if (c->vdw_force_shift) {
+        switch (c->pair_lanes) {
+            case 16: DLPX_PAIR(16, true); break;
+            case 8: DLPX_PAIR(8, true); break;
+            default: DLPX_PAIR(4, true); break;
+        }
+    } else {
+        switch (c->pair_lanes) {
+            case 16: DLPX_PAIR(16, false); break;
+            case 8: DLPX_PAIR(8, false); break;
+            default: DLPX_PAIR(4, false); break;
+        }
     }
 #undef DLPX_PAIR
     DLPX_HIP(c, hipGetLastError());

@@ -116,6 +116,8 @@ def write_control(path, sysm, steps=0, print_every=1, extra=()):
                 "spme_nsplines %d" % sysm.nsplines]
     else:
         out.append("coul_method off")
+    if getattr(sysm, "vdw_force_shift", False):
+        out.append("vdw_force_shift on")
     out += list(extra)
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
@@ -351,12 +353,12 @@ _LENGTH_UNITS = {"ang": 1.0, "nm": 10.0, "bohr": 0.52917721067}
 def read_control(path):
     """New-style CONTROL (control.F90 keyword table), the keys this path uses:
     title, ensemble (nve), timestep, time_run, time_equilibration,
-    stats_frequency, cutoff, padding, vdw_cutoff, coul_method (spme / off),
-    coul_dielectric_constant, spme_alpha, spme_kvec, spme_nsplines,
-    temperature.  Other keys are kept verbatim in ``other``."""
+    stats_frequency, cutoff, padding, vdw_cutoff, vdw_force_shift,
+    coul_method (spme / off), coul_dielectric_constant, spme_alpha, spme_kvec,
+    spme_nsplines, temperature.  Other keys are kept verbatim in ``other``."""
     out = dict(title="", ensemble="nve", timestep=None, steps=0, equilibration=0, stats_frequency=1,
                cutoff=None, padding=0.0, vdw_cutoff=None, coul_method="off", epsilon=1.0, alpha=None,
-               kvec=None, nsplines=8, temperature=0.0, other={})
+               kvec=None, nsplines=8, temperature=0.0, vdw_force_shift=False, other={})
     with open(path) as f:
         for ln in f:
             ln = ln.split("#")[0].strip()
@@ -400,6 +402,8 @@ def read_control(path):
                 out["nsplines"] = int(float(vals[0]))
             elif key == "temperature":
                 out["temperature"] = float(vals[0])
+            elif key == "vdw_force_shift":
+                out["vdw_force_shift"] = (vals[0].lower() if vals else "on") in ("on", "true", "yes", "1")
             else:
                 out["other"][key] = " ".join(vals)
     if out["vdw_cutoff"] is None:
@@ -456,6 +460,7 @@ def system_from_files(directory):
                   masses=np.array(masses), charges=np.array(charges), vdw=vdw, cutoff=ctl["cutoff"],
                   padding=ctl["padding"], vdw_cutoff=ctl["vdw_cutoff"], electrostatics=spme,
                   epsilon=ctl["epsilon"], timestep=ctl["timestep"], temperature=ctl["temperature"],
+                  vdw_force_shift=ctl["vdw_force_shift"],
                   meta=dict(steps=ctl["steps"], stats_frequency=ctl["stats_frequency"], levcfg=cfg["levcfg"],
                             imcon=cfg["imcon"]), **kw)
 

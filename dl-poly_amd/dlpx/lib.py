@@ -23,7 +23,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
-           "dlpx_md_last_output",
+           "dlpx_md_last_output", "dlpx_set_vdw_force_shift",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -93,6 +93,8 @@ class Context:
         self.cell = np.ascontiguousarray(sysm.cell, dtype=np.float64)
         self.q = np.ascontiguousarray(sysm.q)
         self.ltype = np.ascontiguousarray(sysm.types + 1, dtype=np.int32)
+        if getattr(sysm, "vdw_force_shift", False):
+            self._chk(self.lib.dlpx_set_vdw_force_shift(self.h, C.c_int32(1)))
         if tables is None:
             self._generate_tables()
         else:

@@ -65,6 +65,7 @@ class System:
     epsilon: float = 1.0             # electro%eps
     timestep: float = 0.002          # ps
     temperature: float = 0.0
+    vdw_force_shift: bool = False    # vdws%l_force_shift (CONTROL vdw_force_shift on)
     meta: dict = field(default_factory=dict)
 
     @property

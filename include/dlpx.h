@@ -107,6 +107,13 @@ int dlpx_generate_erfc_tables(dlpx_ctx *ctx);
  * with the number of atoms of each type. */
 int dlpx_set_lrc(dlpx_ctx *ctx, double elrc, double vlrc);
 int dlpx_compute_lrc(dlpx_ctx *ctx, const double cell[9], const double *num_type);
+/* vdW force-shifting (vdws%l_force_shift, CONTROL vdw_force_shift; vdw.F90:
+ * 1922,1961-1964): each pair's vdW gamma loses tab_force(rvdw)/(r rvdw) and its
+ * energy gains tab_force(rvdw) (r/rvdw - 1) - tab_potential(rvdw), with the
+ * table samples at r = rvdw (index max_grid-4).  The reference then applies no
+ * long-range correction (vdw.F90:662): switching it on zeroes elrc and vlrc,
+ * and dlpx_compute_lrc leaves them zero.  on: 0 or 1. */
+int dlpx_set_vdw_force_shift(dlpx_ctx *ctx, int32_t on);
 int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
 
 /* link_cell_pairs (neighbours.F90:356-1414): build the Verlet list of pairs

@@ -529,6 +529,7 @@ typedef struct {
     int nsamples;            /* erfc tables: nsamples values, tab[i-1] = f(i*dr) */
     const double *erfc_t, *erfc_d;
     double elrc, vlrc;
+    int vdw_fs;              /* vdW force-shifting (vdws%l_force_shift) */
 } orc_input;
 
 typedef struct {
@@ -742,12 +743,18 @@ int orc_two_body(const orc_input *in, orc_output *out)
             double gk1 = tf[l + 1], gk2 = tf[l + 2];
             double t1 = gk + (gk1 - gk) * ppp, t2 = gk1 + (gk2 - gk1) * (ppp - 1.0);
             double gamma = (t1 + (t2 - t1) * ppp * 0.5) * r_rsq;
+            /* vdw.F90:1922: force-shifting with the samples at r = rvdw (index max_grid-4) */
+            double r_rvdw = 1.0 / in->rvdw;
+            if (in->vdw_fs) gamma = gamma - tf[in->max_grid - 4] * (r_rrr * r_rvdw);
             double fx = gamma * xx, fy = gamma * yy, fz = gamma * zz;
             f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
             f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
             double vk = tp[l], vk1 = tp[l + 1], vk2 = tp[l + 2];
             t1 = vk + (vk1 - vk) * ppp; t2 = vk1 + (vk2 - vk1) * (ppp - 1.0);
-            engvdw += t1 + (t2 - t1) * ppp * 0.5;
+            double eng = t1 + (t2 - t1) * ppp * 0.5;
+            /* vdw.F90:1961-1964 */
+            if (in->vdw_fs) eng = eng + tf[in->max_grid - 4] * (rrr * r_rvdw - 1.0) - tp[in->max_grid - 4];
+            engvdw += eng;
             virvdw -= gamma * rsq;
             st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
             st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;

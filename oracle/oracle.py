@@ -53,7 +53,7 @@ class OrcInput(C.Structure):
                 ("vdw_frc", C.POINTER(C.c_double)),
                 ("nsamples", C.c_int), ("erfc_t", C.POINTER(C.c_double)),
                 ("erfc_d", C.POINTER(C.c_double)),
-                ("elrc", C.c_double), ("vlrc", C.c_double)]
+                ("elrc", C.c_double), ("vlrc", C.c_double), ("vdw_fs", C.c_int)]
 
 
 class OrcOutput(C.Structure):
@@ -137,7 +137,9 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
     L = lib()
     mg, pot, frc, from_key = vdw_tables(sysm, table_points)
     ns, et, ed = erfc_tables_exact(sysm, table_points) if exact_erfc else erfc_tables(sysm)
-    elrc, vlrc = vdw_lrc(sysm, from_key)
+    fs = bool(getattr(sysm, "vdw_force_shift", False))
+    # vdw.F90:662: no long-range correction with force-shifting
+    elrc, vlrc = (0.0, 0.0) if fs else vdw_lrc(sysm, from_key)
     xyz = np.ascontiguousarray(sysm.pos if pos is None else pos, dtype=np.float64)
     cell = np.ascontiguousarray(sysm.cell, dtype=np.float64)
     q = np.ascontiguousarray(sysm.q)
@@ -148,7 +150,8 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
                    rvdw=sysm.vdw_cutoff, alpha=sysm.alpha, epsq=sysm.epsilon,
                    kvec=(C.c_int * 3)(*sysm.kvec), nspl=sysm.nsplines,
                    electro=1 if sysm.electrostatics else 0, max_grid=mg, vdw_pot=_p(pot),
-                   vdw_frc=_p(frc), nsamples=ns, erfc_t=_p(et), erfc_d=_p(ed), elrc=elrc, vlrc=vlrc)
+                   vdw_frc=_p(frc), nsamples=ns, erfc_t=_p(et), erfc_d=_p(ed), elrc=elrc, vlrc=vlrc,
+                   vdw_fs=1 if fs else 0)
     f = np.zeros((sysm.natms, 3))
     out = OrcOutput(frc=_p(f))
     K = sysm.kvec

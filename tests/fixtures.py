@@ -27,7 +27,8 @@ def load(name):
                        vdw_cutoff=float(st[2]), electrostatics=bool(z["electrostatics"]),
                        alpha=float(st[3]), kvec=tuple(int(k) for k in z["kvec"]),
                        nsplines=int(z["nsplines"]), epsilon=float(st[4]), timestep=float(st[5]),
-                       temperature=float(st[6]))
+                       temperature=float(st[6]),
+                       vdw_force_shift=bool(z["vdw_force_shift"]) if "vdw_force_shift" in z.files else False)
     ref = dict(frc=z["ref_frc"], pos=z["ref_pos"], vel=z["ref_vel"], stress=z["ref_stress"],
                stpval=z["ref_stpval"], steps=int(z["steps"]),
                engsrp=z["ref_stpval"][STP_ENGSRP], engcpe=z["ref_stpval"][STP_ENGCPE],

@@ -34,6 +34,12 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10")
 
 
+def _fs(sysm):
+    sysm.vdw_force_shift = True
+    sysm.name += "_fs"
+    return sysm
+
+
 def cases():
     return {
         # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
@@ -54,6 +60,9 @@ def cases():
         # three species, one vdW form per pair (12-6, buck, 12-10, n-m, Morse, LJ) + SPME
         "mix_4000_k36": (systems.mixture(), 0),
         "mix_4000_nve10": (systems.mixture(), 10),
+        # vdW force-shifting (CONTROL vdw_force_shift on): no long-range correction
+        "argon_4000_jit_fs": (_fs(systems.argon(jitter=0.2)), 0),
+        "mix_4000_fs": (_fs(systems.mixture()), 0),
     }
 
 
@@ -86,6 +95,7 @@ def run_case(name, sysm, steps, statis_only=False):
                            sysm.timestep, sysm.temperature]),
         kvec=np.array(sysm.kvec, np.int32), nsplines=np.int32(sysm.nsplines),
         electrostatics=np.int32(sysm.electrostatics), steps=np.int32(steps),
+        vdw_force_shift=np.int32(sysm.vdw_force_shift),
         ref_frc=rc["frc"], ref_pos=rc["pos"], ref_vel=rc["vel"], ref_stpval=rv["stpval"],
         ref_stress=rv["stress"], ref_nstep=np.int32(rv["nstep"]))
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))
