@@ -94,6 +94,33 @@ int upload_tables(Ctx *c)
     c->d_keytab = nullptr;
     DLPX_HIP(c, hipMalloc((void **)&c->d_keytab, sizeof(int32_t) * kt.size()));
     DLPX_HIP(c, hipMemcpy(c->d_keytab, kt.data(), sizeof(int32_t) * kt.size(), hipMemcpyHostToDevice));
+    if (c->vdw_direct) {
+        // vdw_method direct: per key {form, 5 parameters, afs, bfs}; afs, bfs
+        // follow vdw_direct_fs_generate (vdw.F90:969-1048) under force-shifting
+        std::vector<double> d((size_t)8 * c->nkeys, 0.0);
+        for (int k = 0; k < c->nkeys; k++) {
+            const int kind = c->ltp[k];
+            if (!vdw_kind_supported(kind)) continue;
+            const double *pk = c->gen_params.data() + 5 * (size_t)k;
+            d[8 * k] = (double)kind;
+            for (int q = 0; q < 5; q++) d[8 * k + 1 + q] = pk[q];
+            if (c->vdw_force_shift) {
+                double z, dz;
+                vdw_form(kind, pk, c->prm.vdw_cutoff, &z, &dz);
+                d[8 * k + 6] = dz / c->prm.vdw_cutoff;
+                d[8 * k + 7] = -z - dz;
+            }
+        }
+        if (c->d_vdw_direct) (void)hipFree(c->d_vdw_direct);
+        c->d_vdw_direct = nullptr;
+        DLPX_HIP(c, hipMalloc(&c->d_vdw_direct, rs * d.size()));
+        if (mixed) {
+            std::vector<float> df(d.begin(), d.end());
+            DLPX_HIP(c, hipMemcpy(c->d_vdw_direct, df.data(), rs * df.size(), hipMemcpyHostToDevice));
+        } else {
+            DLPX_HIP(c, hipMemcpy(c->d_vdw_direct, d.data(), rs * d.size(), hipMemcpyHostToDevice));
+        }
+    }
     c->tables_dirty = false;
     return 0;
 }
@@ -350,6 +377,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
+                   c->d_vdw_direct,
                    c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -383,6 +411,7 @@ int dlpx_set_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, int32_t max_grid, const do
     c->vdw_frc.assign(tab_force, tab_force + m);
     c->ltp.assign(ltp, ltp + nkeys);
     c->gen_params.clear();
+    c->vdw_direct = false;  // host tables carry no analytic forms
     c->tables_dirty = true;
     return 0;
 }
@@ -417,8 +446,12 @@ int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, 
         }
     }
     std::vector<int32_t> ltp(kind, kind + nkeys);
+    const bool direct = c->vdw_direct;
     const int r = dlpx_set_vdw_tables(ctx, nkeys, mg, pot.data(), frc.data(), ltp.data());
-    if (r == 0) c->gen_params.assign(params, params + 5 * (size_t)nkeys);
+    if (r == 0) {
+        c->gen_params.assign(params, params + 5 * (size_t)nkeys);
+        c->vdw_direct = direct;  // the forms are known again
+    }
     return r;
 }
 
@@ -461,6 +494,18 @@ int dlpx_set_vdw_force_shift(dlpx_ctx *ctx, int32_t on)
     if (!c || (on != 0 && on != 1)) return DLPX_ERR_BAD_ARG;
     c->vdw_force_shift = on != 0;
     if (c->vdw_force_shift) c->elrc = c->vlrc = 0.0;
+    c->tables_dirty = true;  // the direct shift constants depend on it
+    return 0;
+}
+
+int dlpx_set_vdw_direct(dlpx_ctx *ctx, int32_t on)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || (on != 0 && on != 1)) return DLPX_ERR_BAD_ARG;
+    if (on && c->gen_params.empty())
+        return fail(c, DLPX_ERR_STATE, "vdw_method direct needs the forms: call dlpx_generate_vdw_tables first");
+    c->vdw_direct = on != 0;
+    c->tables_dirty = true;
     return 0;
 }
 
@@ -594,7 +639,7 @@ static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
             (uint64_t)c->xref,     (uint64_t)c->typ,     (uint64_t)c->nbr,     (uint64_t)c->cnt,
             (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
             (uint64_t)c->Q,        (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
-            (uint64_t)c->d_keytab, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
+            (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
             (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   t};
 }
 

@@ -74,6 +74,7 @@ struct Ctx {
     // stage 1; graph_key records what the capture baked in (buffers, sizes)
     bool use_graphs = true;
     bool vdw_force_shift = false;    // vdws%l_force_shift
+    bool vdw_direct = false;         // vdws%l_direct: analytic forms per pair
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
     hipGraphExec_t step_graph = nullptr;
     std::vector<uint64_t> graph_key;
@@ -140,6 +141,7 @@ struct Ctx {
     void *d_vdw = nullptr;           // nkeys x (max_grid+1) x {frc, pot}, FP64 or FP32
     void *d_erfc = nullptr;          // nsamples+1 x {deriv, erfc} (index 0 virtual)
     int32_t *d_keytab = nullptr;     // ntypes*ntypes -> key or -1
+    void *d_vdw_direct = nullptr;    // vdw_method direct: 8 per key {form, params[5], afs, bfs}, FP64 or FP32
 
     // SPME
     int K[3]{0, 0, 0};
@@ -200,6 +202,7 @@ int fail(Ctx *c, int code, const std::string &msg);
 // host table generation (tables.cpp)
 int table_size(double cut);
 bool vdw_kind_supported(int kind);
+void vdw_form(int kind, const double *p, double r, double *e, double *g);  // U(r), gamma = -r dU/dr
 void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc);
 void erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d);
 void bspline_norm2(int K, int S, double *norm2);

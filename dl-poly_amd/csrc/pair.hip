@@ -65,6 +65,71 @@ __device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
     e = t1 + (t2 - t1) * ppp * (R)0.5;
 }
 
+// vdw_method direct: U(r) and gamma = -r dU/dr of the generated forms
+// (two_body_potentials.F90, keys of vdw.F90:64-113), in R.
+template <typename R>
+__device__ __forceinline__ R rpow(R x, int n)
+{
+    R y = (R)1;
+    for (int k = 0; k < n; k++) y *= x;
+    return y;
+}
+template <typename R>
+__device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
+{
+    e = (R)0;
+    g = (R)0;
+    switch (kind) {
+        case 1: {  // 12-6 (:307-317)
+            const R r6 = rpow((R)1 / r, 6);
+            e = (p[0] * r6 - p[1]) * r6;
+            g = (R)6 * r6 * ((R)2 * p[0] * r6 - p[1]);
+            break;
+        }
+        case 2: {  // Lennard-Jones (:260-270)
+            const R s = p[1] / r, s2 = s * s, s6 = s2 * s2 * s2;
+            e = (R)4 * p[0] * s6 * (s6 - (R)1);
+            g = (R)24 * p[0] * s6 * ((R)2 * s6 - (R)1);
+            break;
+        }
+        case 3: {  // n-m, integer powers (:330-345)
+            const R a = p[3] / r, b = (R)1 / (p[1] - p[2]);
+            const R rn = rpow(a, (int)p[1]), rm = rpow(a, (int)p[2]);
+            e = p[0] * (p[2] * rn - p[1] * rm) * b;
+            g = p[0] * p[2] * p[1] * (rn - rm) * b;
+            break;
+        }
+        case 4: {  // Buckingham (:471-485)
+            const R b = r / p[1], t1 = p[0] * exp(-b), t2 = -p[2] / rpow(r, 6);
+            e = t1 + t2;
+            g = t1 * b + (R)6 * t2;
+            break;
+        }
+        case 5: {  // Born-Huggins-Meyer (:499-514)
+            const R ri2 = (R)1 / (r * r);
+            const R t1 = p[0] * exp(p[1] * (p[2] - r)), t2 = -p[3] * (ri2 * ri2 * ri2),
+                    t3 = -p[4] * (ri2 * ri2 * ri2 * ri2);
+            e = t1 + t2 + t3;
+            g = t1 * r * p[1] + (R)6 * t2 + (R)8 * t3;
+            break;
+        }
+        case 6: {  // 12-10 (:530-546)
+            const R ri2 = (R)1 / (r * r), f12 = p[0] * rpow(ri2, 6), f10 = -p[1] * rpow(ri2, 5);
+            e = f12 + f10;
+            g = (R)12 * f12 + (R)10 * f10;
+            break;
+        }
+        case 8: {  // Morse (:416-428)
+            const R t = exp(-p[2] * (r - p[1]));
+            e = p[0] * (((R)1 - t) * ((R)1 - t) - (R)1);
+            g = (R)-2 * r * p[0] * p[2] * ((R)1 - t) * t;
+            break;
+        }
+        default:
+            break;
+    }
+}
+
 // 512-thread workgroups: the LDS tables (about 77 KB in FP64 for three vdW
 // keys plus erfc) are loaded once per workgroup, so larger groups put more
 // wavefronts per CU behind one copy to hide the latency of the neighbour
@@ -75,23 +140,29 @@ constexpr int kPairThreads = 512;
 // pair lane p walks list lane p % kLanesPerAtom, entries c = p / kLanesPerAtom,
 // c + GP / kLanesPerAtom, ...  More lanes per atom give more wavefronts for
 // the same atoms (latency hiding on small systems) at the same total work.
-template <typename R, int GP, int NT, bool FS>
+template <typename R, int GP, int NT, bool FS, bool DIRECT>
 __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
                                               const typename R2T<R>::type *__restrict__ vdw_g,
                                               const typename R2T<R>::type *__restrict__ erfc_g,
-                                              const int32_t *__restrict__ keytab_g, double4 *__restrict__ frc,
-                                              double *__restrict__ red)
+                                              const int32_t *__restrict__ keytab_g, const R *__restrict__ direct_g,
+                                              double4 *__restrict__ frc, double *__restrict__ red)
 {
     using R2 = typename R2T<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
+    // LDS: vdW tables (none in direct mode), erfc table, key table, direct
+    // forms (8 per key), reduction scratch
     R2 *tv = (R2 *)smem;
-    R2 *te = tv + pp.nkeys * pp.mg1;
+    R2 *te = tv + (DIRECT ? 0 : pp.nkeys * pp.mg1);
     int32_t *kt = (int32_t *)(te + (pp.electro ? pp.ns1 : 0));
-    double *lred = (double *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
+    R *dv = (R *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
+    double *lred = (double *)(dv + (DIRECT ? 8 * pp.nkeys : 0));
 
-    for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tv[k] = vdw_g[k];
+    if (DIRECT)
+        for (int k = threadIdx.x; k < 8 * pp.nkeys; k += blockDim.x) dv[k] = direct_g[k];
+    else
+        for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tv[k] = vdw_g[k];
     if (pp.electro)
         for (int k = threadIdx.x; k < pp.ns1; k += blockDim.x) te[k] = erfc_g[k];
     for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = keytab_g[k];
@@ -142,10 +213,19 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
             const int key = ktrow[tj];
             if (key >= 0 && rrr < rvdw) {
                 R gf, ge;
-                interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
-                R g = gf * rinv2;
-                R ev = ge;
-                if (FS) {
+                R g, ev;
+                if (DIRECT) {
+                    // vdw.F90:1696-1699
+                    const R *dk = dv + 8 * key;
+                    vdw_form<R>((int)dk[0], dk + 1, rrr, ge, gf);
+                    ev = ge + dk[6] * rrr + dk[7];
+                    g = gf * rinv2 - dk[6] * (rrr * rinv2);
+                } else {
+                    interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
+                    g = gf * rinv2;
+                    ev = ge;
+                }
+                if (FS && !DIRECT) {
                     // vdw.F90:1922,1961-1964 with the samples at r = rvdw (index max_grid-4)
                     const R2 cut = tv[key * pp.mg1 + pp.mg1 - 5];
                     const R r_rvdw = (R)pp.r_rvdw;
@@ -210,16 +290,27 @@ static int launch_pair(Ctx *c)
     pp.scaling = kR4pie0 / c->prm.epsilon;
     pp.r_rvdw = 1.0 / c->prm.vdw_cutoff;
     pp.cap_lane = c->cap_lane;
-    const size_t lds = sizeof(R2) * ((size_t)pp.nkeys * pp.mg1 + (pp.electro ? pp.ns1 : 0)) +
-                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) + sizeof(double) * 10 * (kPairThreads / 64);
+    const bool direct = c->vdw_direct;
+    const size_t lds = sizeof(R2) * ((direct ? 0 : (size_t)pp.nkeys * pp.mg1) + (pp.electro ? pp.ns1 : 0)) +
+                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
+                       (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-#define DLPX_PAIR(GP, FS)                                                                                    \
-    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS>));                                                        \
-    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,       \
+#define DLPX_PAIR_K(GP, FS, DIR)                                                                             \
+    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR>));                                                   \
+    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,  \
                        c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,   \
-                       (const R2 *)c->d_erfc, c->d_keytab, c->frc, c->red)
-    if (c->vdw_force_shift) {
+                       (const R2 *)c->d_erfc, c->d_keytab, (const R *)c->d_vdw_direct, c->frc, c->red)
+#define DLPX_PAIR(GP, FS) DLPX_PAIR_K(GP, FS, false)
+    if (direct) {
+        // direct mode at the default 8 lanes per atom only
+        if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "vdw_method direct runs with 8 pair lanes");
+        if (c->vdw_force_shift) {
+            DLPX_PAIR_K(8, true, true);
+        } else {
+            DLPX_PAIR_K(8, false, true);
+        }
+    } else if (c->vdw_force_shift) {
         switch (c->pair_lanes) {
             case 16: DLPX_PAIR(16, true); break;
             case 8: DLPX_PAIR(8, true); break;
@@ -233,6 +324,7 @@ static int launch_pair(Ctx *c)
         }
     }
 #undef DLPX_PAIR
+#undef DLPX_PAIR_K
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

@@ -82,6 +82,8 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
     }
 }
 
+void vdw_form(int kind, const double *p, double r, double *e, double *g) { vdw_eval(kind, p, r, e, g); }
+
 // vdw.F90:1397-1451: samples at r_i = i * rvdw/(max_grid-4), i = 1..max_grid;
 // slot 0 holds Huge (never used inside the cutoff).
 void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc)

@@ -103,7 +103,7 @@ def write_control(path, sysm, steps=0, print_every=1, extra=()):
            "cutoff %s ang" % _num(sysm.cutoff),
            "padding %s ang" % _num(sysm.padding),
            "vdw_cutoff %s ang" % _num(sysm.vdw_cutoff),
-           "vdw_method tabulated",
+           "vdw_method %s" % ("direct" if getattr(sysm, "vdw_direct", False) else "tabulated"),
            "print_frequency %d steps" % print_every,
            "stats_frequency %d steps" % print_every,
            "time_job 100000 s",
@@ -353,12 +353,13 @@ _LENGTH_UNITS = {"ang": 1.0, "nm": 10.0, "bohr": 0.52917721067}
 def read_control(path):
     """New-style CONTROL (control.F90 keyword table), the keys this path uses:
     title, ensemble (nve), timestep, time_run, time_equilibration,
-    stats_frequency, cutoff, padding, vdw_cutoff, vdw_force_shift,
+    stats_frequency, cutoff, padding, vdw_cutoff, vdw_method (tabulated /
+    direct), vdw_force_shift,
     coul_method (spme / off), coul_dielectric_constant, spme_alpha, spme_kvec,
     spme_nsplines, temperature.  Other keys are kept verbatim in ``other``."""
     out = dict(title="", ensemble="nve", timestep=None, steps=0, equilibration=0, stats_frequency=1,
                cutoff=None, padding=0.0, vdw_cutoff=None, coul_method="off", epsilon=1.0, alpha=None,
-               kvec=None, nsplines=8, temperature=0.0, vdw_force_shift=False, other={})
+               kvec=None, nsplines=8, temperature=0.0, vdw_force_shift=False, vdw_method="tabulated", other={})
     with open(path) as f:
         for ln in f:
             ln = ln.split("#")[0].strip()
@@ -402,6 +403,8 @@ def read_control(path):
                 out["nsplines"] = int(float(vals[0]))
             elif key == "temperature":
                 out["temperature"] = float(vals[0])
+            elif key == "vdw_method":
+                out["vdw_method"] = vals[0].lower()
             elif key == "vdw_force_shift":
                 out["vdw_force_shift"] = (vals[0].lower() if vals else "on") in ("on", "true", "yes", "1")
             else:
@@ -420,6 +423,8 @@ def system_from_files(directory):
     cfg = read_config(os.path.join(directory, "CONFIG"))
     fld = read_field(os.path.join(directory, "FIELD"))
     ctl = read_control(os.path.join(directory, "CONTROL"))
+    if ctl["vdw_method"] not in ("tabulated", "direct"):
+        raise ValueError("vdw_method %r: this path has tabulated or direct vdW" % ctl["vdw_method"])
     if ctl["ensemble"] != "nve":
         raise ValueError("ensemble %r: this path integrates NVE only" % ctl["ensemble"])
     if cfg["imcon"] not in (1, 2):
@@ -461,6 +466,7 @@ def system_from_files(directory):
                   padding=ctl["padding"], vdw_cutoff=ctl["vdw_cutoff"], electrostatics=spme,
                   epsilon=ctl["epsilon"], timestep=ctl["timestep"], temperature=ctl["temperature"],
                   vdw_force_shift=ctl["vdw_force_shift"],
+                  vdw_direct=ctl["vdw_method"] == "direct",
                   meta=dict(steps=ctl["steps"], stats_frequency=ctl["stats_frequency"], levcfg=cfg["levcfg"],
                             imcon=cfg["imcon"]), **kw)
 

@@ -23,7 +23,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
-           "dlpx_md_last_output", "dlpx_set_vdw_force_shift",
+           "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -119,6 +119,8 @@ class Context:
             self._chk(self.lib.dlpx_generate_erfc_tables(self.h))
         num = np.bincount(s.types, minlength=s.ntypes).astype(np.float64)
         self._chk(self.lib.dlpx_compute_lrc(self.h, _ptr(self.cell), _ptr(num)))
+        if getattr(s, "vdw_direct", False):
+            self._chk(self.lib.dlpx_set_vdw_direct(self.h, C.c_int32(1)))
 
     def set_tables(self, max_grid, vdw_pot, vdw_frc, ltp, nsamples=0, erfc=None, erfc_d=None):
         pot = np.ascontiguousarray(vdw_pot, np.float64)

@@ -66,6 +66,7 @@ class System:
     timestep: float = 0.002          # ps
     temperature: float = 0.0
     vdw_force_shift: bool = False    # vdws%l_force_shift (CONTROL vdw_force_shift on)
+    vdw_direct: bool = False         # vdws%l_direct (CONTROL vdw_method direct)
     meta: dict = field(default_factory=dict)
 
     @property

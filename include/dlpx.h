@@ -114,6 +114,12 @@ int dlpx_compute_lrc(dlpx_ctx *ctx, const double cell[9], const double *num_type
  * long-range correction (vdw.F90:662): switching it on zeroes elrc and vlrc,
  * and dlpx_compute_lrc leaves them zero.  on: 0 or 1. */
 int dlpx_set_vdw_force_shift(dlpx_ctx *ctx, int32_t on);
+/* vdw_method direct (vdws%l_direct; vdw.F90:1590-1788 vdw_forces_direct): each
+ * pair evaluates its analytic form (the forms of dlpx_generate_vdw_tables,
+ * which must have been called) instead of interpolating the tables; with
+ * force-shifting, U gains afs r + bfs and gamma loses afs/r, afs =
+ * gamma(rvdw)/rvdw, bfs = -U(rvdw) - gamma(rvdw) (vdw.F90:969-1048).  on: 0 or 1. */
+int dlpx_set_vdw_direct(dlpx_ctx *ctx, int32_t on);
 int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
 
 /* link_cell_pairs (neighbours.F90:356-1414): build the Verlet list of pairs

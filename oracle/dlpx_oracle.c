@@ -530,6 +530,9 @@ typedef struct {
     const double *erfc_t, *erfc_d;
     double elrc, vlrc;
     int vdw_fs;              /* vdW force-shifting (vdws%l_force_shift) */
+    int vdw_direct;          /* vdw_method direct: analytic forms per pair (vdws%l_direct) */
+    const int *vdw_kind;     /* per key, for vdw_direct */
+    const double *vdw_param; /* 5 per key, for vdw_direct */
 } orc_input;
 
 typedef struct {
@@ -732,8 +735,34 @@ int orc_two_body(const orc_input *in, orc_output *out)
         double yy = in->xyz[3 * i + 1] - (in->xyz[3 * j + 1] + pairs[5 * p + 3] * L[1]);
         double zz = in->xyz[3 * i + 2] - (in->xyz[3 * j + 2] + pairs[5 * p + 4] * L[2]);
         double rrr = sqrt(xx * xx + yy * yy + zz * zz);
-        /* vdW */
-        if (in->max_grid > 0 && rrr < in->rvdw) {
+        /* vdW, direct: vdw.F90:1655-1700 with vdw_direct_fs_generate (:969-1048) */
+        if (in->vdw_direct && rrr < in->rvdw) {
+            int k = key_of(in->ltype[i], in->ltype[j]);
+            int kind = in->vdw_kind[k];
+            if (kind > 0) {
+                const double *pk = in->vdw_param + 5 * (long)k;
+                double afs = 0.0, bfs = 0.0, e, g;
+                if (in->vdw_fs) {
+                    double z, dz;
+                    orc_vdw_eval(kind, pk, in->rvdw, &z, &dz);
+                    afs = dz / in->rvdw;
+                    bfs = -z - dz;
+                }
+                double r_rrr = 1.0 / rrr, rsq = rrr * rrr, r_rsq = r_rrr * r_rrr;
+                orc_vdw_eval(kind, pk, rrr, &e, &g);
+                double eng = e + afs * rrr + bfs;
+                double gamma = g * r_rsq - afs * r_rrr;
+                double fx = gamma * xx, fy = gamma * yy, fz = gamma * zz;
+                f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
+                f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
+                engvdw += eng;
+                virvdw -= gamma * rsq;
+                st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
+                st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
+            }
+        }
+        /* vdW, tabulated */
+        if (!in->vdw_direct && in->max_grid > 0 && rrr < in->rvdw) {
             int k = key_of(in->ltype[i], in->ltype[j]);
             const double *tf = in->vdw_frc + (long)k * nrow, *tp = in->vdw_pot + (long)k * nrow;
             double r_rrr = 1.0 / rrr, rsq = rrr * rrr, r_rsq = r_rrr * r_rrr;

@@ -53,7 +53,8 @@ class OrcInput(C.Structure):
                 ("vdw_frc", C.POINTER(C.c_double)),
                 ("nsamples", C.c_int), ("erfc_t", C.POINTER(C.c_double)),
                 ("erfc_d", C.POINTER(C.c_double)),
-                ("elrc", C.c_double), ("vlrc", C.c_double), ("vdw_fs", C.c_int)]
+                ("elrc", C.c_double), ("vlrc", C.c_double), ("vdw_fs", C.c_int), ("vdw_direct", C.c_int),
+                ("vdw_kind", C.POINTER(C.c_int)), ("vdw_param", C.POINTER(C.c_double))]
 
 
 class OrcOutput(C.Structure):
@@ -138,6 +139,11 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
     mg, pot, frc, from_key = vdw_tables(sysm, table_points)
     ns, et, ed = erfc_tables_exact(sysm, table_points) if exact_erfc else erfc_tables(sysm)
     fs = bool(getattr(sysm, "vdw_force_shift", False))
+    direct = bool(getattr(sysm, "vdw_direct", False))
+    nkey = sysm.ntypes * (sysm.ntypes + 1) // 2
+    kinds = np.zeros(nkey, np.int32); params = np.zeros((nkey, 5))
+    for k, (kind, p) in from_key.items():
+        kinds[k] = kind; params[k] = p
     # vdw.F90:662: no long-range correction with force-shifting
     elrc, vlrc = (0.0, 0.0) if fs else vdw_lrc(sysm, from_key)
     xyz = np.ascontiguousarray(sysm.pos if pos is None else pos, dtype=np.float64)
@@ -151,7 +157,8 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
                    kvec=(C.c_int * 3)(*sysm.kvec), nspl=sysm.nsplines,
                    electro=1 if sysm.electrostatics else 0, max_grid=mg, vdw_pot=_p(pot),
                    vdw_frc=_p(frc), nsamples=ns, erfc_t=_p(et), erfc_d=_p(ed), elrc=elrc, vlrc=vlrc,
-                   vdw_fs=1 if fs else 0)
+                   vdw_fs=1 if fs else 0, vdw_direct=1 if direct else 0, vdw_kind=_p(kinds, C.c_int),
+                   vdw_param=_p(params))
     f = np.zeros((sysm.natms, 3))
     out = OrcOutput(frc=_p(f))
     K = sysm.kvec

@@ -40,6 +40,12 @@ def _fs(sysm):
     return sysm
 
 
+def _direct(sysm):
+    sysm.vdw_direct = True
+    sysm.name += "_direct"
+    return sysm
+
+
 def cases():
     return {
         # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
@@ -63,6 +69,9 @@ def cases():
         # vdW force-shifting (CONTROL vdw_force_shift on): no long-range correction
         "argon_4000_jit_fs": (_fs(systems.argon(jitter=0.2)), 0),
         "mix_4000_fs": (_fs(systems.mixture()), 0),
+        # vdw_method direct: analytic forms per pair, without and with force-shifting
+        "mix_4000_direct": (_direct(systems.mixture()), 0),
+        "mix_4000_direct_fs": (_direct(_fs(systems.mixture())), 0),
     }
 
 
@@ -96,6 +105,7 @@ def run_case(name, sysm, steps, statis_only=False):
         kvec=np.array(sysm.kvec, np.int32), nsplines=np.int32(sysm.nsplines),
         electrostatics=np.int32(sysm.electrostatics), steps=np.int32(steps),
         vdw_force_shift=np.int32(sysm.vdw_force_shift),
+        vdw_direct=np.int32(sysm.vdw_direct),
         ref_frc=rc["frc"], ref_pos=rc["pos"], ref_vel=rc["vel"], ref_stpval=rv["stpval"],
         ref_stress=rv["stress"], ref_nstep=np.int32(rv["nstep"]))
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))
