@@ -187,6 +187,7 @@ static int evaluate(Ctx *c)
         c->sp = c->stream;
         DLPX_HIP(c, hipEventRecord(c->ev[0], c->stream));
         if ((r = pair_forces(c))) return r;
+        if ((r = excl_forces(c))) return r;
         DLPX_HIP(c, hipEventRecord(c->ev[1], c->stream));
         if (c->prm.electrostatics) {
             if ((r = spme_forces(c))) return r;  // records ev[2..4]
@@ -202,6 +203,7 @@ static int evaluate(Ctx *c)
         DLPX_HIP(c, hipStreamWaitEvent(c->recip_stream, c->fork_ev, 0));
         if ((r = spme_forces(c))) return r;
         if ((r = pair_forces(c))) return r;
+        if ((r = excl_forces(c))) return r;
         DLPX_HIP(c, hipEventRecord(c->join_ev, c->recip_stream));
         DLPX_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
         c->sp = c->stream;
@@ -209,6 +211,8 @@ static int evaluate(Ctx *c)
         if ((r = pair_forces(c))) return r;
     }
     for (int k = R_ENGVDW; k <= R_S33; k++) c->red_rows[k] = pair_blocks(c);
+    if (c->excl_total > 0 && c->prm.electrostatics)
+        for (int k = R_ENGEX; k <= R_XS33; k++) c->red_rows[k] = excl_blocks(c);
     if ((r = reduce_partials(c))) return r;
     if (c->prm.electrostatics) {
         const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
@@ -251,8 +255,15 @@ static void assemble(Ctx *c, const double *red, dlpx_two_body_out *o)
     const double virrl = c->prm.electrostatics ? -(red[R_S11] + red[R_S22] + red[R_S33]) - virvdw : 0.0;
     double st[9] = {red[R_S11], red[R_S12], red[R_S13], red[R_S12], red[R_S22],
                     red[R_S23], red[R_S13], red[R_S23], red[R_S33]};
-    double engrc = 0.0, virrc = 0.0, engnz = 0.0, virnz = 0.0;
+    double engrc = 0.0, virrc = 0.0, engnz = 0.0, virnz = 0.0, engex = 0.0, virex = 0.0;
     if (c->prm.electrostatics) {
+        // Ewald exclusion correction: energy, and virial = -trace of its stress
+        // (ewald_spole.F90:615-630)
+        const double xs[9] = {red[R_XS11], red[R_XS12], red[R_XS13], red[R_XS12], red[R_XS22],
+                              red[R_XS23], red[R_XS13], red[R_XS23], red[R_XS33]};
+        engex = red[R_ENGEX];
+        virex = -(xs[0] + xs[4] + xs[8]);
+        for (int k = 0; k < 9; k++) st[k] += xs[k];
         const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
                              (kR4pie0 / c->prm.epsilon);
         const double eng = red[R_ERECIP] * scale;
@@ -277,8 +288,10 @@ static void assemble(Ctx *c, const double *red, dlpx_two_body_out *o)
     o->buffer[10] = virrc;
     o->buffer[11] = engrl;
     o->buffer[12] = virrl;
-    o->engcpe = engrc + engrl + engnz;
-    o->vircpe = virrc + virrl + virnz;
+    o->buffer[15] = engex;
+    o->buffer[16] = virex;
+    o->engcpe = engrc + engrl + engex + engnz;
+    o->vircpe = virrc + virrl + virex + virnz;
     o->engsrp = engvdw + c->elrc;
     o->virsrp = virvdw + c->vlrc;
     const double dnz = -virnz / 3.0, dlrc = -c->vlrc / 3.0;
@@ -377,7 +390,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
-                   c->d_vdw_direct,
+                   c->d_vdw_direct, c->excl_off, c->excl_ids,
                    c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -506,6 +519,56 @@ int dlpx_set_vdw_direct(dlpx_ctx *ctx, int32_t on)
         return fail(c, DLPX_ERR_STATE, "vdw_method direct needs the forms: call dlpx_generate_vdw_tables first");
     c->vdw_direct = on != 0;
     c->tables_dirty = true;
+    return 0;
+}
+
+int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int32_t *list_excl)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    if (c->resident) return fail(c, DLPX_ERR_STATE, "set the exclusions before dlpx_md_load");
+    if (natms < 0 || natms > c->cap || mxexcl < 0 || (natms > 0 && !list_excl))
+        return fail(c, DLPX_ERR_BAD_ARG, "exclusions: natms outside 0..natms_max, negative mxexcl or no array");
+    // rows as sorted, duplicate-free host indices (build_excl.F90 add_exclusion keeps one copy)
+    std::vector<int32_t> off(natms + 1, 0), ids;
+    const size_t ld = (size_t)mxexcl + 1;
+    for (int i = 0; i < natms; i++) {
+        const int32_t *row = list_excl + ld * i;
+        const int m = row[0];
+        if (m < 0 || m > mxexcl) return fail(c, DLPX_ERR_BAD_ARG, "exclusions: count outside 0..mxexcl");
+        std::vector<int32_t> r;
+        for (int k = 1; k <= m; k++) {
+            const int j = row[k] - 1;
+            if (j < 0 || j >= natms || j == i)
+                return fail(c, DLPX_ERR_BAD_ARG, "exclusions: atom " + std::to_string(i + 1) + " lists id " +
+                                                     std::to_string(row[k]));
+            r.push_back(j);
+        }
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        ids.insert(ids.end(), r.begin(), r.end());
+        off[i + 1] = (int32_t)ids.size();
+    }
+    // a pair is dropped from whichever end holds it, so both ends must list it
+    for (int i = 0; i < natms; i++)
+        for (int k = off[i]; k < off[i + 1]; k++) {
+            const int j = ids[k];
+            if (!std::binary_search(ids.begin() + off[j], ids.begin() + off[j + 1], i))
+                return fail(c, DLPX_ERR_BAD_ARG, "exclusions are not symmetric: " + std::to_string(i + 1) +
+                                                     " excludes " + std::to_string(j + 1) + " but not the reverse");
+        }
+    if (c->excl_off) (void)hipFree(c->excl_off);
+    if (c->excl_ids) (void)hipFree(c->excl_ids);
+    c->excl_off = c->excl_ids = nullptr;
+    c->excl_natms = c->excl_total = 0;
+    c->have_list = false;  // the list was built under the old exclusions
+    if (ids.empty()) return 0;
+    int r;
+    if ((r = dalloc(c, &c->excl_off, off.size())) || (r = dalloc(c, &c->excl_ids, ids.size()))) return r;
+    DLPX_HIP(c, hipMemcpy(c->excl_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+    DLPX_HIP(c, hipMemcpy(c->excl_ids, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice));
+    c->excl_natms = natms;
+    c->excl_total = (int)ids.size();
     return 0;
 }
 
@@ -640,7 +703,8 @@ static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
             (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
             (uint64_t)c->Q,        (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
             (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
-            (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   t};
+            (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
+            (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total, t};
 }
 
 // Capture forces (both streams), stage 2 of this step and stage 1 of the

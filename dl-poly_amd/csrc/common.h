@@ -17,7 +17,7 @@ constexpr double kZeroPlus = 2.2250738585072014e-308;     // Tiny(1.0_wp)
 constexpr double kHalfMinus = 0.49999999999999994;        // Nearest(0.5_wp, -1.0_wp)
 constexpr int kLanesPerAtom = 4;                          // lanes sharing one atom's list
 constexpr int kMaxSplines = 16;
-constexpr int kReduceSlots = 24;
+constexpr int kReduceSlots = 28;
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs of an
 // MI355X (workgroup b on XCD b % 8), each with its own L2.  Kernels whose
@@ -51,7 +51,8 @@ enum Red : int {
     R_FNETX, R_FNETY, R_FNETZ,                   // sum of unscaled reciprocal forces
     R_RS11, R_RS12, R_RS13, R_RS22, R_RS23, R_RS33,  // reciprocal stress (before scale)
     R_KIN,                                       // kinetic energy (2 x)
-    R_SPARE0, R_SPARE1, R_SPARE2
+    R_ENGEX,                                     // Ewald exclusion correction energy
+    R_XS11, R_XS12, R_XS13, R_XS22, R_XS23, R_XS33   // its stress
 };
 
 struct DevBuf {
@@ -102,6 +103,12 @@ struct Ctx {
     std::vector<double> erfc_t, erfc_d;    // nsamples
     double elrc = 0.0, vlrc = 0.0;
     bool tables_dirty = true;
+
+    // excluded pairs (neigh%list_excl), symmetric, as CSR over host indices
+    int excl_natms = 0;              // atoms the lists were given for (0: none)
+    int excl_total = 0;              // entries (each pair twice)
+    int32_t *excl_off = nullptr;     // excl_natms+1 row starts
+    int32_t *excl_ids = nullptr;     // excl_total host indices, ascending per row
 
     // per-atom device arrays (sorted order unless noted)
     double4 *pos = nullptr, *pos_alt = nullptr;     // x, y, z, q
@@ -216,6 +223,8 @@ int upload_tables(Ctx *c);
 int build_list(Ctx *c);                      // from c->pos (current order), sorts & permutes
 int vnl_check_dev(Ctx *c, int *update);
 int pair_forces(Ctx *c);                     // writes c->frc and reduction slots
+int excl_forces(Ctx *c);                     // adds the exclusion correction into c->frc
+int excl_blocks(const Ctx *c);
 int spme_forces(Ctx *c);                     // adds into c->frc and reduction slots
 int spme_setup(Ctx *c);
 int ensure_red_rows(Ctx *c, int rows);

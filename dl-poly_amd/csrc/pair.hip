@@ -337,4 +337,104 @@ int pair_forces(Ctx *c)
 
 int pair_blocks(const Ctx *c) { return xcd_grid(ceil_div((long)c->natms * c->pair_lanes, kPairThreads)); }
 
+// Ewald correction of the excluded pairs (ewald_spole.F90:479-679, called
+// from two_body.F90:551-592).  The reciprocal sum holds the whole Coulomb
+// interaction of every pair, so each excluded pair closer than the cutoff
+// gives back its smooth part, U = -q_i q_j erf(alpha r)/r, with erf from
+// Abramowitz & Stegun 7.1.26 and, below alpha r = 0.01, from its Taylor
+// series to x^9.  FP64 in both precision modes: an atom has a handful of
+// exclusions.  Thread i (sorted order) walks the exclusion row of atom i and
+// adds the forces on i alone, so every pair is met from both ends and its
+// energy and stress are halved.
+struct ExclParams {
+    double L[3], invL[3];
+    double rc, alpha, scaling;
+};
+
+__global__ void __launch_bounds__(256) k_excl(const double4 *__restrict__ pos, int n, const int32_t *__restrict__ gid,
+                                              const int32_t *__restrict__ slot, const int32_t *__restrict__ excl_off,
+                                              const int32_t *__restrict__ excl_ids, ExclParams ep,
+                                              double4 *__restrict__ frc, double *__restrict__ red)
+{
+    // A&S 7.1.26: erfc(x) ~ t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x)
+    constexpr double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027,
+                     a5 = 1.061405429, p = 0.3275911;
+    constexpr double sqrpi = 1.7724538509055160273;
+    __shared__ double lred[7 * (256 / 64)];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const int h = gid[i];
+        const int e0 = excl_off[h], e1 = excl_off[h + 1];
+        const double4 pi = pos[i];
+        if (e1 > e0 && fabs(pi.w) > kZeroPlus) {
+            const double chgea = pi.w * ep.scaling, alpha = ep.alpha;
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            for (int e = e0; e < e1; e++) {
+                const double4 pj = pos[slot[excl_ids[e]]];
+                const double dx = min_image_1(pi.x, pj.x, ep.L[0], ep.invL[0]);
+                const double dy = min_image_1(pi.y, pj.y, ep.L[1], ep.invL[1]);
+                const double dz = min_image_1(pi.z, pj.z, ep.L[2], ep.invL[2]);
+                const double rrr = sqrt(dx * dx + dy * dy + dz * dz);
+                if (!(fabs(pj.w) > kZeroPlus && rrr < ep.rc)) continue;
+                const double chgprd = pj.w * chgea;
+                const double rsq = rrr * rrr;
+                const double alpr = rrr * alpha, alpr2 = alpr * alpr;
+                double erfr, egamma;
+                if (alpr < 1.0e-2) {
+                    // erf(x)/x = 2/sqrt(pi) (1 - x^2/3 + x^4/10 - x^6/42 + x^8/216), and its r-derivative
+                    erfr = 2.0 * chgprd * (alpha / sqrpi) *
+                           (1.0 + alpr2 * (-(1.0 / 3.0) + alpr2 * (0.1 + alpr2 * (-(1.0 / 42.0) + alpr2 * (1.0 / 216.0)))));
+                    egamma = -4.0 * chgprd * (alpha * alpha * alpha / sqrpi) *
+                             ((1.0 / 3.0) + alpr2 * (-2.0 * 0.1 + alpr2 * (3.0 * (1.0 / 42.0) - 4.0 * alpr2 * (1.0 / 216.0))));
+                } else {
+                    const double ex = exp(-(alpr * alpr));
+                    const double tt = 1.0 / (1.0 + p * alpha * rrr);
+                    erfr = chgprd * (1.0 - tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * ex) / rrr;
+                    egamma = -(erfr - 2.0 * chgprd * (alpha / sqrpi) * ex) / rsq;
+                }
+                const double fxp = egamma * dx, fyp = egamma * dy, fzp = egamma * dz;
+                fx += fxp;
+                fy += fyp;
+                fz += fzp;
+                acc[0] -= erfr;
+                acc[1] += dx * fxp;
+                acc[2] += dx * fyp;
+                acc[3] += dx * fzp;
+                acc[4] += dy * fyp;
+                acc[5] += dy * fzp;
+                acc[6] += dz * fzp;
+            }
+            double4 f = frc[i];
+            f.x += fx;
+            f.y += fy;
+            f.z += fz;
+            frc[i] = f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] *= 0.5;  // every pair is met twice
+    block_sum<7>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ENGEX);
+}
+
+int excl_blocks(const Ctx *c) { return ceil_div(c->natms, 256); }
+
+int excl_forces(Ctx *c)
+{
+    if (c->excl_total == 0 || !c->prm.electrostatics) return 0;
+    if (c->excl_natms != c->natms) return fail(c, DLPX_ERR_STATE, "exclusion lists are for another atom count");
+    ExclParams ep;
+    for (int d = 0; d < 3; d++) {
+        ep.L[d] = c->L[d];
+        ep.invL[d] = c->invL[d];
+    }
+    ep.rc = c->prm.cutoff;
+    ep.alpha = c->prm.alpha;
+    ep.scaling = kR4pie0 / c->prm.epsilon;
+    hipLaunchKernelGGL(k_excl, dim3(excl_blocks(c)), dim3(256), 0, c->stream, c->pos, c->natms, c->gid, c->slot,
+                       c->excl_off, c->excl_ids, ep, c->frc, c->red);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
 }  // namespace dlpx

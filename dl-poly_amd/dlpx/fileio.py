@@ -72,16 +72,41 @@ def _cubic(cell):
 
 
 def write_field(path, sysm):
-    """One single-atom molecule type per atom type, in type order (the CONFIG
-    lists atoms grouped by type in the same order)."""
-    counts = np.bincount(sysm.types, minlength=sysm.ntypes)
-    order_ok = np.all(np.diff(sysm.types) >= 0)
-    if not order_ok:
-        raise ValueError("atoms must be grouped by type for the FIELD molecule layout")
-    out = [sysm.name, "units internal", "molecules %d" % sysm.ntypes]
-    for t, nm in enumerate(sysm.type_names):
-        out += [nm, "nummols %d" % counts[t], "atoms 1",
-                "%-8s %s %s" % (nm, _num(sysm.masses[t]), _num(sysm.charges[t])), "finish"]
+    """Without ``sysm.molecules``: one single-atom molecule type per atom type,
+    in type order (the CONFIG lists atoms grouped by type in the same order).
+    With them: each molecule type with its sites, and its bonds and angles as
+    zero-strength harmonic terms (``harm``), which exclude their pairs from the
+    two-body sum and add no force."""
+    mols = getattr(sysm, "molecules", None)
+    if mols:
+        out = [sysm.name, "units internal", "molecules %d" % len(mols)]
+        base = 0
+        for m in mols:
+            got = sysm.types[base:base + m.nummols * len(m.sites)]
+            if not np.array_equal(got, np.tile(np.asarray(m.sites, np.int32), m.nummols)):
+                raise ValueError("atoms do not follow molecule type %r's site order" % m.name)
+            base += m.nummols * len(m.sites)
+            out += [m.name, "nummols %d" % m.nummols, "atoms %d" % len(m.sites)]
+            for t in m.sites:
+                out.append("%-8s %s %s" % (sysm.type_names[t], _num(sysm.masses[t]), _num(sysm.charges[t])))
+            if m.bonds:
+                out.append("bonds %d" % len(m.bonds))
+                out += ["harm %d %d 0.0 1.0" % (a + 1, b + 1) for (a, b) in m.bonds]
+            if m.angles:
+                out.append("angles %d" % len(m.angles))
+                out += ["harm %d %d %d 0.0 109.47" % (a + 1, b + 1, c + 1) for (a, b, c) in m.angles]
+            out.append("finish")
+        if base != sysm.natms:
+            raise ValueError("molecules describe %d atoms, the system has %d" % (base, sysm.natms))
+    else:
+        counts = np.bincount(sysm.types, minlength=sysm.ntypes)
+        order_ok = np.all(np.diff(sysm.types) >= 0)
+        if not order_ok:
+            raise ValueError("atoms must be grouped by type for the FIELD molecule layout")
+        out = [sysm.name, "units internal", "molecules %d" % sysm.ntypes]
+        for t, nm in enumerate(sysm.type_names):
+            out += [nm, "nummols %d" % counts[t], "atoms 1",
+                    "%-8s %s %s" % (nm, _num(sysm.masses[t]), _num(sysm.charges[t])), "finish"]
     out.append("vdw %d" % len(sysm.vdw))
     for (ti, tj, key, params) in sysm.vdw:
         out.append("%-8s %-8s %4s %s" % (sysm.type_names[ti], sysm.type_names[tj], VDW_NAMES[key],
@@ -252,13 +277,40 @@ _VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors"
 _VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,)}
 
 
+def _bonded_block(lines, i, count, natoms, kind, mname):
+    """``count`` bond (natoms 2) or angle (natoms 3) records from lines[i]:
+    ``key i j [k] params``.  This path computes no bonded forces, so only
+    zero-strength harmonic terms are accepted: ``harm`` (excludes its pairs,
+    ffield.F90:1348-1357, 1592-1601) and ``-hrm`` (restrained: excludes
+    nothing).  Returns the site tuples (0-based) of the excluding terms."""
+    out = []
+    for _ in range(count):
+        w = lines[i].split()
+        i += 1
+        key = w[0].lower()[:4]
+        idx = [int(v) - 1 for v in w[1:1 + natoms]]
+        prm = [float(v) for v in w[1 + natoms:]]
+        if key not in ("harm", "-hrm"):
+            raise ValueError("%s %r in molecule %r: bonded forces are not part of this path" % (kind, w[0], mname))
+        if not prm or prm[0] != 0.0:
+            raise ValueError("%s in molecule %r has a nonzero force constant: bonded forces are not part of "
+                             "this path (only zero-strength terms, which just exclude pairs)" % (kind, mname))
+        if min(idx) < 0:
+            raise ValueError("%s in molecule %r names site %d" % (kind, mname, min(idx) + 1))
+        if key == "harm":
+            out.append(tuple(idx))
+    return out, i
+
+
 def read_field(path):
     """FIELD (ffield.F90:150-5365), the subset of this path: energy units;
-    molecule types made of unbonded atoms (``name mass charge [repeat
-    [frozen]]``); the vdW forms the library generates (``12-6``, ``lj``,
-    ``nm``, ``buck``, ``bhm``, ``hbnd``, ``mors``).  Bonded terms, frozen
-    atoms, shells, other vdW forms and many-body terms raise ValueError.
+    molecule types of atoms (``name mass charge [repeat [frozen]]``) with
+    zero-strength harmonic ``bonds`` and ``angles`` that only exclude pairs;
+    the vdW forms the library generates (``12-6``, ``lj``, ``nm``, ``buck``,
+    ``bhm``, ``hbnd``, ``mors``).  Bonded forces, constraints, frozen atoms,
+    shells, other vdW forms and many-body terms raise ValueError.
     Returns dict(title, units, molecules [(name, nummols, [(atom, mass, charge)])],
+    topology [(bonds, angles) per molecule type, 0-based sites],
     vdw [(atom1, atom2, key, params in internal units)])."""
     with open(path) as f:
         raw = [ln.split("#")[0].rstrip() for ln in f.read().splitlines()]
@@ -266,7 +318,7 @@ def read_field(path):
     lines = [ln for ln in raw[1:] if ln.strip()]
     units = "internal"
     unit = 1.0
-    molecules, vdw = [], []
+    molecules, vdw, topology = [], [], []
     i = 0
 
     def words(ln):
@@ -294,7 +346,7 @@ def read_field(path):
             for _ in range(nmol):
                 mname = lines[i].strip()
                 i += 1
-                nummols, atoms = None, []
+                nummols, atoms, bonds, angles = None, [], [], []
                 while True:
                     w = words(lines[i])
                     k2 = w[0].lower()
@@ -315,6 +367,10 @@ def read_field(path):
                                 atoms.append((a[0], float(a[1]), float(a[2])))
                             got += max(rep, 1)
                             i += 1
+                    elif k2 == "bonds":
+                        bonds, i = _bonded_block(lines, i + 1, int(w[-1]), 2, "bond", mname)
+                    elif k2 == "angles":
+                        angles, i = _bonded_block(lines, i + 1, int(w[-1]), 3, "angle", mname)
                     elif k2 == "finish":
                         i += 1
                         break
@@ -322,7 +378,12 @@ def read_field(path):
                         raise ValueError("FIELD section %r is not supported by this path" % w[0])
                 if nummols is None:
                     raise ValueError("molecule %r has no nummols record" % mname)
+                for t in bonds + angles:
+                    if max(t) >= len(atoms):
+                        raise ValueError("a bonded term of molecule %r names site %d of %d" % (mname, max(t) + 1,
+                                                                                             len(atoms)))
                 molecules.append((mname, nummols, atoms))
+                topology.append((bonds, angles))
         elif key == "vdw":
             nv = int(w[-1])
             i += 1
@@ -341,7 +402,7 @@ def read_field(path):
             break
         else:
             raise ValueError("FIELD directive %r is not supported by this path" % w[0])
-    return dict(title=title, units=units, molecules=molecules, vdw=vdw)
+    return dict(title=title, units=units, molecules=molecules, topology=topology, vdw=vdw)
 
 
 # --------------------------------------------------------------- CONTROL --
@@ -460,6 +521,11 @@ def system_from_files(directory):
     kw = {}
     if spme:
         kw = dict(alpha=ctl["alpha"], kvec=ctl["kvec"], nsplines=ctl["nsplines"])
+    if any(b or a for (b, a) in fld["topology"]):
+        from .systems import Molecule
+        kw["molecules"] = [Molecule(name=m, nummols=nm, sites=[type_names.index(x[0]) for x in atoms],
+                                    bonds=list(b), angles=list(a))
+                           for (m, nm, atoms), (b, a) in zip(fld["molecules"], fld["topology"])]
     return System(name=cfg["title"].strip() or fld["title"], cell=cfg["cell"].copy(), pos=cfg["pos"].copy(),
                   vel=cfg["vel"].copy(), types=np.array(site_types, np.int32), type_names=type_names,
                   masses=np.array(masses), charges=np.array(charges), vdw=vdw, cutoff=ctl["cutoff"],

@@ -23,7 +23,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
-           "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct",
+           "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -101,6 +101,23 @@ class Context:
             self.set_tables(**tables)
         if lrc is not None:
             self._chk(self.lib.dlpx_set_lrc(self.h, C.c_double(lrc[0]), C.c_double(lrc[1])))
+        excl = sysm.exclusions() if hasattr(sysm, "exclusions") else None
+        if excl is not None:
+            self.set_exclusions(excl)
+
+    def set_exclusions(self, excl):
+        """Excluded pairs: one sequence of 0-based partner indices per atom
+        (symmetric), or None to clear; passed in the reference's
+        list_excl(0:mxexcl, 1:natms) layout with 1-based ids."""
+        if excl is None:
+            self._chk(self.lib.dlpx_set_exclusions(self.h, C.c_int32(0), C.c_int32(0), None))
+            return
+        mx = max((len(e) for e in excl), default=0)
+        arr = np.zeros((len(excl), mx + 1), np.int32)
+        for i, e in enumerate(excl):
+            arr[i, 0] = len(e)
+            arr[i, 1:1 + len(e)] = np.asarray(e, np.int64) + 1
+        self._chk(self.lib.dlpx_set_exclusions(self.h, C.c_int32(len(excl)), C.c_int32(mx), _ptr(arr)))
 
     # -- tables -------------------------------------------------------
     def _generate_tables(self):

@@ -10,7 +10,7 @@ per-type mass/charge, the vdW pair table (``vdw.F90`` type keys) and the
 cutoffs / Ewald / SPME settings.
 """
 from dataclasses import dataclass, field
-from typing import List, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -45,6 +45,21 @@ NA_MASS, CL_MASS, AR_MASS = 22.98977, 35.453, 39.948
 
 
 @dataclass
+class Molecule:
+    """One FIELD molecule type: ``nummols`` copies of its sites (0-based atom
+    type per site, in CONFIG order) and the bonds and angles that make
+    exclusions (0-based site indices).  This path computes no bonded forces:
+    the terms only mark which pairs leave the two-body sum, as
+    build_excl_intra does (build_excl.F90:32-1195: a bond excludes its two
+    atoms, an angle all three pairs)."""
+    name: str
+    nummols: int
+    sites: List[int]
+    bonds: List[Tuple[int, int]] = field(default_factory=list)
+    angles: List[Tuple[int, int, int]] = field(default_factory=list)
+
+
+@dataclass
 class System:
     name: str
     cell: np.ndarray                 # (9,) a, b, c row vectors
@@ -68,6 +83,7 @@ class System:
     vdw_force_shift: bool = False    # vdws%l_force_shift (CONTROL vdw_force_shift on)
     vdw_direct: bool = False         # vdws%l_direct (CONTROL vdw_method direct)
     meta: dict = field(default_factory=dict)
+    molecules: Optional[List[Molecule]] = None  # None: one single-atom molecule type per atom type
 
     @property
     def natms(self):
@@ -89,6 +105,29 @@ class System:
     def volume(self):
         c = self.cell.reshape(3, 3)
         return float(abs(np.linalg.det(c)))
+
+    def exclusions(self):
+        """Excluded partners of every atom (sorted 0-based indices), as
+        neigh%list_excl holds them, or None when nothing is excluded."""
+        if not self.molecules:
+            return None
+        excl = [set() for _ in range(self.natms)]
+        base = 0
+        for m in self.molecules:
+            pairs = [tuple(b) for b in m.bonds]
+            for (a, b, c) in m.angles:
+                pairs += [(a, b), (a, c), (b, c)]
+            for k in range(m.nummols):
+                o = base + k * len(m.sites)
+                for (a, b) in pairs:
+                    excl[o + a].add(o + b)
+                    excl[o + b].add(o + a)
+            base += m.nummols * len(m.sites)
+        if base != self.natms:
+            raise ValueError("molecules describe %d atoms, the system has %d" % (base, self.natms))
+        if not any(excl):
+            return None
+        return [np.array(sorted(e), np.int64) for e in excl]
 
 
 def _streams(seed, n):
@@ -214,6 +253,89 @@ def mixture(ncells=10, a=5.78015, temperature=100.0, seed=20261202, charges=(0.3
                   vdw=list(MIX_VDW), cutoff=cutoff, padding=padding, vdw_cutoff=cutoff, electrostatics=True,
                   alpha=alpha, kvec=tuple(kvec), nsplines=8, timestep=timestep, temperature=temperature,
                   meta=dict(seed=seed, a=a, ncells=ncells, jitter=jitter))
+
+
+# three-site polar molecule (SPC-like charges and O-O Lennard-Jones, with
+# small H repulsions), internal units; the four-site variant adds a site "Dw"
+# 0.02 A from Ow that carries most of the oxygen charge (a Drude-like pair,
+# closer than 0.01/alpha, so its exclusion correction takes the series branch)
+MOL_NAMES = ["Ow", "Hw", "Dw"]
+MOL_MASSES = (15.9994, 1.008, 0.4)
+MOL_VDW = [
+    (0, 0, VDW_LENNARD_JONES, (65.02, 3.166)),
+    (1, 0, VDW_LENNARD_JONES, (15.0, 2.0)),
+    (1, 1, VDW_LENNARD_JONES, (5.0, 1.2)),
+]
+
+
+def _rotations(rng, n):
+    """n uniformly random rotation matrices (from unit quaternions)."""
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def molecular(ncells=6, a=4.93, temperature=300.0, seed=20261202, drude=False, kvec=(32, 32, 32), alpha=0.35,
+              cutoff=9.0, padding=0.5, timestep=0.001, bond=1.0, angle=109.47, drude_sep=0.02):
+    """Polar triatomic molecules (Ow, Hw, Hw) centred on an FCC lattice (4 per
+    cell, edge ``a``) with random orientations, bond length ``bond`` and angle
+    ``angle``; with ``drude`` a fourth site Dw sits ``drude_sep`` from Ow.
+    Two bonds (and the Dw bond) and the H-O-H angle (and the H-O-Dw angles)
+    exclude every intramolecular pair, so the molecules carry no intramolecular
+    two-body terms and the reciprocal sum's share of them is removed by the
+    Ewald exclusion correction."""
+    nmol = 4 * ncells ** 3
+    L = a * ncells
+    r_rot, r_vel, r_dir = _streams(seed, 3)
+    g = np.stack(np.meshgrid(np.arange(ncells), np.arange(ncells), np.arange(ncells),
+                             indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    centres = (g[:, None, :] + FCC[None, :, :] + 0.25).reshape(-1, 3) * a - 0.5 * L
+    th = np.radians(angle) / 2
+    local = [np.zeros(3), bond * np.array([np.sin(th), 0.0, np.cos(th)]),
+             bond * np.array([-np.sin(th), 0.0, np.cos(th)])]
+    sites, charges = [0, 1, 1], (-0.82, 0.41, 0.0)
+    bonds, angles = [(0, 1), (0, 2)], [(1, 0, 2)]
+    if drude:
+        local.append(drude_sep * np.array([0.0, 0.0, -1.0]))
+        sites, charges = [0, 1, 1, 2], (1.0, 0.41, -1.82)
+        bonds.append((0, 3))
+        angles += [(1, 0, 3), (2, 0, 3)]
+    rot = _rotations(r_rot, nmol)
+    pos = centres[:, None, :] + np.einsum("mab,sb->msa", rot, np.array(local))
+    pos = _wrap(pos.reshape(-1, 3), L)
+    types = np.tile(np.array(sites, np.int32), nmol)
+    ntypes = 3 if drude else 2
+    masses = np.array(MOL_MASSES[:ntypes])
+    if drude:
+        masses[0] -= MOL_MASSES[2]
+    vel = _maxwell(r_vel, masses[types], temperature) if temperature > 0 else np.zeros_like(pos)
+    mol = Molecule(name="polar4" if drude else "polar3", nummols=nmol, sites=sites, bonds=bonds, angles=angles)
+    return System(name="mol%d_%d" % (len(sites), pos.shape[0]), cell=np.diag([L, L, L]).reshape(9).astype(np.float64),
+                  pos=pos, vel=vel, types=types, type_names=MOL_NAMES[:ntypes], masses=masses,
+                  charges=np.array(charges[:ntypes], np.float64), vdw=list(MOL_VDW), cutoff=cutoff,
+                  padding=padding, vdw_cutoff=cutoff, electrostatics=True, alpha=alpha, kvec=tuple(kvec),
+                  nsplines=8, timestep=timestep, temperature=temperature,
+                  meta=dict(seed=seed, a=a, ncells=ncells, drude=drude), molecules=[mol])
+
+
+def molecules_to_json(mols):
+    import json
+    if not mols:
+        return ""
+    return json.dumps([dict(name=m.name, nummols=m.nummols, sites=list(m.sites), bonds=[list(b) for b in m.bonds],
+                            angles=[list(t) for t in m.angles]) for m in mols])
+
+
+def molecules_from_json(text):
+    import json
+    if not text:
+        return None
+    return [Molecule(name=d["name"], nummols=int(d["nummols"]), sites=[int(x) for x in d["sites"]],
+                     bonds=[tuple(b) for b in d["bonds"]], angles=[tuple(t) for t in d["angles"]])
+            for d in json.loads(text)]
 
 
 def smooth235_at_least(n):

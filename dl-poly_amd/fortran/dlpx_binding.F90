@@ -49,7 +49,7 @@ Module dlpx_binding
             dlpx_generate_vdw_tables, dlpx_generate_erfc_tables, dlpx_set_lrc, dlpx_compute_lrc, &
             dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_md_load, &
             dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_md_last_output, &
-            dlpx_set_vdw_force_shift, dlpx_set_vdw_direct, dlpx_error_message
+            dlpx_set_vdw_force_shift, dlpx_set_vdw_direct, dlpx_set_exclusions, dlpx_error_message
 
   Interface
     Integer(c_int) Function dlpx_create(params, ctx) Bind(C, name='dlpx_create')
@@ -205,6 +205,15 @@ Module dlpx_binding
       Type(c_ptr), Value :: ctx
       Integer(c_int32_t), Value :: on
     End Function dlpx_set_vdw_direct
+
+    !> excluded pairs: pass neigh%list_excl(0:mxexcl, 1:natms) as it is
+    Integer(c_int) Function dlpx_set_exclusions(ctx, natms, mxexcl, list_excl) &
+        Bind(C, name='dlpx_set_exclusions')
+      Import :: c_int, c_int32_t, c_ptr
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Value :: natms, mxexcl
+      Integer(c_int32_t), Intent(In) :: list_excl(0:mxexcl, *)
+    End Function dlpx_set_exclusions
 
     !> dlpx_md_run over many independent contexts in lockstep
     Integer(c_int) Function dlpx_md_run_ensemble(ctxs, nctx, nsteps, tsteps, nrebuilds) &

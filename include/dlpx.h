@@ -122,6 +122,19 @@ int dlpx_set_vdw_force_shift(dlpx_ctx *ctx, int32_t on);
 int dlpx_set_vdw_direct(dlpx_ctx *ctx, int32_t on);
 int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
 
+/* Excluded pairs, neigh%list_excl as build_excl_intra fills it from bonds,
+ * angles, dihedrals, constraints, rigid bodies and core-shell units
+ * (build_excl.F90:32-1195).  list_excl holds mxexcl+1 int32 per atom, laid
+ * out as the reference's list_excl(0:mxexcl, 1:natms): element k of atom i
+ * (0-based host index) at list_excl[(mxexcl+1)*i + k], k = 0 the count, then
+ * the partners' 1-based global ids (ltg).  Each pair must be listed from both
+ * ends.  Excluded pairs leave the Verlet list (neighbours.F90:1229-1250) and,
+ * with SPME, get the Ewald exclusion correction U = -q_i q_j erf(alpha r)/r
+ * for r < cutoff (ewald_spole.F90:479-679), reported in buffer slots 15-16
+ * (two_body.F90:723-724) and included in engcpe, vircpe and the stress.
+ * Call before dlpx_link_cell_pairs / dlpx_md_load; natms = 0 clears them. */
+int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int32_t *list_excl);
+
 /* link_cell_pairs (neighbours.F90:356-1414): build the Verlet list of pairs
  * within cutoff+padding from the given positions.  On success *npairs is the
  * number of unordered pairs and *ibig the largest per-atom count; with a fixed

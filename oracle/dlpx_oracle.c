@@ -17,6 +17,9 @@
  *   orc_bspline        partition of unity, derivative sums, symmetry, finite
  *                      differences, closed-form cubic B-spline
  *   orc_fft3d          naive DFT on small 2.3.5 sizes
+ *   orc_excl_pair      erf(alpha r)/r and its derivative against the C
+ *                      library's erf (both branches); REVCON/REVIVE of the
+ *                      reference for molecules with exclusions
  *   orc_two_body       REVCON forces and REVIVE energies/virials/stress of the
  *                      compiled reference on generated NaCl/argon systems;
  *                      Madelung constant of rocksalt; direct Ewald sum
@@ -533,6 +536,8 @@ typedef struct {
     int vdw_direct;          /* vdw_method direct: analytic forms per pair (vdws%l_direct) */
     const int *vdw_kind;     /* per key, for vdw_direct */
     const double *vdw_param; /* 5 per key, for vdw_direct */
+    const int32_t *excl_off; /* excluded pairs (neigh%list_excl) as CSR over atoms: N+1 row starts, */
+    const int32_t *excl_ids; /* then 0-based partners, both ends listed; NULL = none */
 } orc_input;
 
 typedef struct {
@@ -546,6 +551,38 @@ typedef struct {
     /* optional intermediates (NULL to skip): K1*K2*K3 reals each */
     double *charge_grid, *potential_grid;
 } orc_output;
+
+static int excluded(const orc_input *in, int i, int j)
+{
+    if (!in->excl_off) return 0;
+    for (int e = in->excl_off[i]; e < in->excl_off[i + 1]; e++)
+        if (in->excl_ids[e] == j) return 1;
+    return 0;
+}
+
+/* ewald_spole.F90:479-679 for one excluded pair at separation r = rrr <
+ * rcut, charge product chgprd (with the 1/(4 pi eps0 eps) scaling):
+ * the smooth erf(alpha r)/r part the reciprocal sum holds, taken back.  erf
+ * is A&S 7.1.26, or its Taylor series to x^9 below alpha r = 0.01. */
+void orc_excl_pair(double chgprd, double alpha, double rrr, double *erfr_out, double *egamma_out)
+{
+    const double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027, a5 = 1.061405429,
+                 pp = 0.3275911, sqrpi = sqrt(ORC_PI);
+    double rsq = rrr * rrr, alpr = rrr * alpha, alpr2 = alpr * alpr, erfr, egamma;
+    if (alpr < 1.0e-2) {
+        erfr = 2.0 * chgprd * (alpha / sqrpi) *
+               (1.0 + alpr2 * (-1.0 / 3.0 + alpr2 * (0.1 + alpr2 * (-1.0 / 42.0 + alpr2 / 216.0))));
+        egamma = -4.0 * chgprd * (alpha * alpha * alpha / sqrpi) *
+                 (1.0 / 3.0 + alpr2 * (-0.2 + alpr2 * (3.0 / 42.0 - 4.0 * alpr2 / 216.0)));
+    } else {
+        double e = exp(-(alpha * rrr) * (alpha * rrr));
+        double tt = 1.0 / (1.0 + pp * alpha * rrr);
+        erfr = chgprd * (1.0 - tt * (a1 + tt * (a2 + tt * (a3 + tt * (a4 + tt * a5)))) * e) / rrr;
+        egamma = -(erfr - 2.0 * chgprd * (alpha / sqrpi) * e) / rsq;
+    }
+    *erfr_out = erfr;
+    *egamma_out = egamma;
+}
 
 static int key_of(int ti, int tj)
 {
@@ -729,8 +766,11 @@ int orc_two_body(const orc_input *in, orc_output *out)
     double dlrpot = in->rvdw / (double)(in->max_grid - 4), rdr = 1.0 / dlrpot;
     double dr_e = in->rcut / (double)(in->nsamples - 4), rdr_e = 1.0 / dr_e;
     int nrow = in->max_grid + 1;
+    long nexcl_listed = 0;
     for (long p = 0; p < np; p++) {
         int i = pairs[5 * p], j = pairs[5 * p + 1];
+        /* neighbours.F90:1229-1250: excluded pairs sit past list(0,i), out of the two-body loop */
+        if (excluded(in, i, j)) { nexcl_listed++; continue; }
         double xx = in->xyz[3 * i] - (in->xyz[3 * j] + pairs[5 * p + 2] * L[0]);
         double yy = in->xyz[3 * i + 1] - (in->xyz[3 * j + 1] + pairs[5 * p + 3] * L[1]);
         double zz = in->xyz[3 * i + 2] - (in->xyz[3 * j + 2] + pairs[5 * p + 4] * L[2]);
@@ -805,8 +845,40 @@ int orc_two_body(const orc_input *in, orc_output *out)
         }
     }
     free(pairs);
+    out->npairs = np - nexcl_listed;
     const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int k = 0; k < 9; k++) out->stress[k] += st_v[map[k]] + st_c[map[k]];
+
+    /* ---- Ewald exclusion correction: two_body.F90:551-592, ewald_spole.F90:479-679 ----
+     * every excluded pair once (i < j), at its minimum image, if both charges
+     * are nonzero and r < rcut */
+    double engcpe_ex = 0.0, vircpe_ex = 0.0, st_x[6] = {0};
+    if (in->electro && in->excl_off) {
+        for (int i = 0; i < n; i++) {
+            double qi = in->chge[i];
+            if (!(fabs(qi) > ZERO_PLUS)) continue;
+            for (int e = in->excl_off[i]; e < in->excl_off[i + 1]; e++) {
+                int j = in->excl_ids[e];
+                if (j <= i) continue;
+                double qj = in->chge[j];
+                double dx[3];
+                int sh[3];
+                min_image(L, in->xyz + 3 * i, in->xyz + 3 * j, dx, sh);
+                double rrr = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+                if (!(fabs(qj) > ZERO_PLUS && rrr < in->rcut)) continue;
+                double erfr, egamma;
+                orc_excl_pair(qj * (qi * scaling), in->alpha, rrr, &erfr, &egamma);
+                double fx = egamma * dx[0], fy = egamma * dx[1], fz = egamma * dx[2];
+                f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
+                f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
+                engcpe_ex -= erfr;
+                vircpe_ex -= egamma * rrr * rrr;
+                st_x[0] += dx[0] * fx; st_x[1] += dx[0] * fy; st_x[2] += dx[0] * fz;
+                st_x[3] += dx[1] * fy; st_x[4] += dx[1] * fz; st_x[5] += dx[2] * fz;
+            }
+        }
+        for (int k = 0; k < 9; k++) out->stress[k] += st_x[map[k]];
+    }
 
     /* ---- bookkeeping: two_body.F90:689-790 ---- */
     double engcpe_nz = 0.0, vircpe_nz = 0.0;
@@ -819,8 +891,9 @@ int orc_two_body(const orc_input *in, orc_output *out)
     out->buffer[7] = engvdw; out->buffer[8] = virvdw;
     out->buffer[9] = engcpe_rc; out->buffer[10] = vircpe_rc;
     out->buffer[11] = engrl; out->buffer[12] = virrl;
-    out->engcpe = engcpe_rc + engrl + engcpe_nz;
-    out->vircpe = vircpe_rc + virrl + vircpe_nz;
+    out->buffer[15] = engcpe_ex; out->buffer[16] = vircpe_ex;
+    out->engcpe = engcpe_rc + engrl + engcpe_ex + engcpe_nz;
+    out->vircpe = vircpe_rc + virrl + vircpe_ex + vircpe_nz;
     double tmp = -vircpe_nz / 3.0;
     out->stress[0] += tmp; out->stress[4] += tmp; out->stress[8] += tmp;
     out->engsrp = engvdw + in->elrc;

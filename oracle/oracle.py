@@ -54,7 +54,8 @@ class OrcInput(C.Structure):
                 ("nsamples", C.c_int), ("erfc_t", C.POINTER(C.c_double)),
                 ("erfc_d", C.POINTER(C.c_double)),
                 ("elrc", C.c_double), ("vlrc", C.c_double), ("vdw_fs", C.c_int), ("vdw_direct", C.c_int),
-                ("vdw_kind", C.POINTER(C.c_int)), ("vdw_param", C.POINTER(C.c_double))]
+                ("vdw_kind", C.POINTER(C.c_int)), ("vdw_param", C.POINTER(C.c_double)),
+                ("excl_off", C.POINTER(C.c_int32)), ("excl_ids", C.POINTER(C.c_int32))]
 
 
 class OrcOutput(C.Structure):
@@ -159,6 +160,13 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
                    vdw_frc=_p(frc), nsamples=ns, erfc_t=_p(et), erfc_d=_p(ed), elrc=elrc, vlrc=vlrc,
                    vdw_fs=1 if fs else 0, vdw_direct=1 if direct else 0, vdw_kind=_p(kinds, C.c_int),
                    vdw_param=_p(params))
+    excl = sysm.exclusions() if hasattr(sysm, "exclusions") else None
+    if excl is not None:
+        eo = np.zeros(sysm.natms + 1, np.int32)
+        eo[1:] = np.cumsum([len(e) for e in excl])
+        ei = np.ascontiguousarray(np.concatenate(excl) if eo[-1] else np.zeros(1), dtype=np.int32)
+        inp.excl_off = _p(eo, C.c_int32)
+        inp.excl_ids = _p(ei, C.c_int32)
     f = np.zeros((sysm.natms, 3))
     out = OrcOutput(frc=_p(f))
     K = sysm.kvec
@@ -175,6 +183,14 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
     if grids and sysm.electrostatics:
         res["Q"] = Q.reshape(K[2], K[1], K[0]); res["Phi"] = P.reshape(K[2], K[1], K[0])
     return res
+
+
+def excl_pair(chgprd, alpha, r):
+    """ewald_spole.F90:479-679 for one excluded pair: (erfr, egamma), with
+    erfr = chgprd erf(alpha r)/r and egamma its force factor."""
+    e, g = C.c_double(), C.c_double()
+    lib().orc_excl_pair(C.c_double(chgprd), C.c_double(alpha), C.c_double(r), C.byref(e), C.byref(g))
+    return e.value, g.value
 
 
 def pairs(cell, xyz, rcut, method="cells"):

@@ -29,7 +29,8 @@ def load(name):
                        nsplines=int(z["nsplines"]), epsilon=float(st[4]), timestep=float(st[5]),
                        temperature=float(st[6]),
                        vdw_force_shift=bool(z["vdw_force_shift"]) if "vdw_force_shift" in z.files else False,
-                       vdw_direct=bool(z["vdw_direct"]) if "vdw_direct" in z.files else False)
+                       vdw_direct=bool(z["vdw_direct"]) if "vdw_direct" in z.files else False,
+                       molecules=systems.molecules_from_json(str(z["molecules"])) if "molecules" in z.files else None)
     ref = dict(frc=z["ref_frc"], pos=z["ref_pos"], vel=z["ref_vel"], stress=z["ref_stress"],
                stpval=z["ref_stpval"], steps=int(z["steps"]),
                engsrp=z["ref_stpval"][STP_ENGSRP], engcpe=z["ref_stpval"][STP_ENGCPE],

@@ -31,7 +31,7 @@ from dlpx import fileio, systems  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
 OUT = os.path.dirname(os.path.abspath(__file__))
-STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10")
+STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10", "mol3_2592_nve10")
 
 
 def _fs(sysm):
@@ -72,6 +72,13 @@ def cases():
         # vdw_method direct: analytic forms per pair, without and with force-shifting
         "mix_4000_direct": (_direct(systems.mixture()), 0),
         "mix_4000_direct_fs": (_direct(_fs(systems.mixture())), 0),
+        # polar triatomic molecules whose bonds and angle (zero-strength
+        # harmonic terms) exclude every intramolecular pair: Ewald exclusion
+        # correction (ewald_spole.F90:479-679)
+        "mol3_2592_k32": (systems.molecular(), 0),
+        "mol3_2592_nve10": (systems.molecular(), 10),
+        # with a fourth site 0.02 A from the oxygen: the correction's series branch
+        "mol4_3456_k32": (systems.molecular(drude=True), 0),
     }
 
 
@@ -106,6 +113,7 @@ def run_case(name, sysm, steps, statis_only=False):
         electrostatics=np.int32(sysm.electrostatics), steps=np.int32(steps),
         vdw_force_shift=np.int32(sysm.vdw_force_shift),
         vdw_direct=np.int32(sysm.vdw_direct),
+        molecules=np.array(systems.molecules_to_json(sysm.molecules)),
         ref_frc=rc["frc"], ref_pos=rc["pos"], ref_vel=rc["vel"], ref_stpval=rv["stpval"],
         ref_stress=rv["stress"], ref_nstep=np.int32(rv["nstep"]))
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))

@@ -72,7 +72,8 @@ def test_field_reader_units_and_repeats():
     assert lj2 == ("C", "C", 2, (100.0, 3.4))
 
 
-@pytest.mark.parametrize("bad", ["bonds 1\n1 2 harm 100.0 1.0\n", None])
+@pytest.mark.parametrize("bad", ["bonds 1\nharm 1 2 100.0 1.0\n", "bonds 1\nmors 1 2 0.0 1.0 2.0\n",
+                                 "constraints 1\n1 2 1.0\n", None])
 def test_field_reader_rejects_what_the_path_lacks(bad):
     text = FIELD_KJ
     if bad:
@@ -84,6 +85,53 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
         open(p, "w").write(text)
         with pytest.raises(ValueError):
             fileio.read_field(p)
+
+
+FIELD_MOL = """three-site molecules
+units internal
+molecules 1
+polar
+nummols 2
+atoms 3
+Ow 15.9994 -0.82
+Hw 1.008 0.41 2
+bonds 2
+harm 1 2 0.0 1.0
+-hrm 1 3 0.0 1.0
+angles 1
+harm 2 1 3 0.0 109.47
+finish
+vdw 1
+Ow Ow lj 65.02 3.166
+close
+"""
+
+
+def test_field_zero_strength_bonds_give_exclusions():
+    """bonds and angles mark excluded pairs (build_excl.F90:125-200): a bond
+    its two atoms, an angle all three pairs; a restrained term (-hrm)
+    excludes nothing."""
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "FIELD")
+        open(p, "w").write(FIELD_MOL)
+        f = fileio.read_field(p)
+    assert f["topology"] == [([(0, 1)], [(1, 0, 2)])]
+    s = systems.molecular(ncells=1)
+    s.molecules = [systems.Molecule(name="polar", nummols=2, sites=[0, 1, 1], bonds=[(0, 1)], angles=[(1, 0, 2)])]
+    s.pos, s.vel, s.types = s.pos[:6], s.vel[:6], s.types[:6]
+    ex = s.exclusions()
+    assert [list(e) for e in ex] == [[1, 2], [0, 2], [0, 1], [4, 5], [3, 5], [3, 4]]
+
+
+def test_molecular_inputs_round_trip():
+    s = systems.molecular(ncells=2, drude=True)
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, s, steps=3)
+        t = fileio.system_from_files(d)
+    assert np.array_equal(t.types, s.types) and t.type_names == s.type_names
+    assert [(m.nummols, m.sites, m.bonds, m.angles) for m in t.molecules] == \
+        [(m.nummols, m.sites, m.bonds, m.angles) for m in s.molecules]
+    assert all(np.array_equal(a, b) for a, b in zip(t.exclusions(), s.exclusions()))
 
 
 def test_control_reader_units():
@@ -138,7 +186,8 @@ def test_statis_equal_is_the_harness_rule():
     assert not fileio.statis_equal(0.0, 1e-9, 1e-5)
 
 
-@pytest.mark.parametrize("name", ["nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10"])
+@pytest.mark.parametrize("name", ["nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10",
+                                  "mol3_2592_nve10"])
 def test_statis_step0_quantities_match_reference_file(name):
     """The STATIS assembly of one step (temperature, enthalpy, pressure,
     pressure tensor with the kinetic stress, cell angles, energies and

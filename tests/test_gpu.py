@@ -33,17 +33,26 @@ def gpu_two_body(s, precision=lib.FP64, pos=None, **kw):
     return ctx, r
 
 
-def check_energies(r, ref, tol, coulomb_scale=False):
-    """Energies and virials within tol relative to the reference value.  With
-    coulomb_scale (mixed precision) the Coulomb terms are measured against the
-    largest Coulomb component, max(|engcpe|, |eng_recip|, |eng_self|): engcpe
-    is their sum, and where they cancel (a weakly charged mixture: +5.5e6
-    reciprocal, -6.6e6 self, -1.1e6 total) FP32 rounding of the components
-    is what bounds the total."""
+def coulomb_scales(o):
+    """The largest term of engcpe and of vircpe in an oracle result: engcpe is
+    the sum of the self, reciprocal, real-space and exclusion-correction
+    energies, which cancel where the charges are weak (a mixture: +5.5e6
+    reciprocal, -6.6e6 self, -1.1e6 total) or the molecules are polar (2e7
+    terms, 7e4 total); the rounding of the terms is what bounds the total.
+    The same terms bound vircpe: the real-space Coulomb virial is the
+    real-space stress trace less the vdW virial, and with exclusions the
+    reciprocal sum carries each molecule's own smooth interaction (the size of
+    the self and correction energies) until the FP64 correction removes it."""
+    b = o["buffer"]
+    eng = max(abs(o["eng_recip"]), abs(o["eng_self"]), abs(b[11]), abs(b[15]))
+    return dict(engcpe=eng, vircpe=max(eng, abs(b[8]), abs(b[10]), abs(b[12]), abs(b[16])))
+
+
+def check_energies(r, ref, tol, scales=None):
+    """Energies and virials within tol relative to the reference value, or to
+    scales[k] where that is larger (see coulomb_scales)."""
     for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
-        scale = max(abs(ref[k]), 1.0)
-        if coulomb_scale and k in ("engcpe", "vircpe"):
-            scale = max(scale, abs(ref.get("eng_recip", 0.0)), abs(ref.get("eng_self", 0.0)))
+        scale = max(abs(ref[k]), 1.0, (scales or {}).get(k, 0.0))
         assert abs(r[k] - ref[k]) <= tol * scale, (k, r[k], ref[k])
 
 
@@ -55,13 +64,13 @@ def test_fp64_matches_oracle_and_reference(name):
     ctx, r = gpu_two_body(s)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-10
     assert np.abs(r["frc"] - o["frc"]).max() <= 1e-10 * np.abs(o["frc"]).max()
-    check_energies(r, o, 1e-11)
+    check_energies(r, o, 1e-11, coulomb_scales(o))
     assert np.abs(r["stress"] - o["stress"]).max() <= 1e-11 * np.abs(o["stress"]).max()
     assert np.allclose(r["buffer"], o["buffer"], rtol=1e-11, atol=1e-9)
     # and the reference itself: all printed force digits, full-precision energies
     bound = fixtures.printed_digits_bound(ref["frc"]) + 1e-12 * np.abs(ref["frc"]).max()
     assert (np.abs(r["frc"] - ref["frc"]) / bound).max() <= 1.05
-    check_energies(r, ref, 1e-11)
+    check_energies(r, ref, 1e-11, coulomb_scales(o))
     ctx.close()
 
 
@@ -73,10 +82,17 @@ def test_pair_set_bit_exact(name):
     npairs, ibig = ctx.link_cell_pairs(s.pos)
     g = ctx.get_pairs()
     o = oracle.pairs(s.cell, s.pos, s.cutoff + s.padding, "cells")
-    assert npairs == len(o) == len(g)
-    assert {tuple(p) for p in g} == {tuple(p) for p in o}
+    # ibig counts every pair within reach, excluded ones too (neighbours.F90:1189)
     counts = np.bincount(o[:, :2].ravel(), minlength=s.natms)
     assert ibig == counts.max()
+    excl = s.exclusions()
+    if excl is not None:
+        # excluded pairs leave the list (neighbours.F90:1229-1250)
+        keep = np.array([p[1] not in set(excl[p[0]]) for p in o], bool)
+        assert (~keep).sum() > 0
+        o = o[keep]
+    assert npairs == len(o) == len(g)
+    assert {tuple(p) for p in g} == {tuple(p) for p in o}
     ctx.close()
 
 
@@ -105,7 +121,7 @@ def test_mixed_precision_error_budget(name):
     o = oracle.two_body(s)
     ctx, r = gpu_two_body(s, precision=lib.MIXED)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-5
-    check_energies(r, o, 1e-6, coulomb_scale=True)
+    check_energies(r, o, 1e-6, coulomb_scales(o))
     ctx.close()
 
 
@@ -149,7 +165,10 @@ def test_dlpoly_inputs_run_to_the_reference_statis(name, precision):
     same inputs by the reference harness's rule (relative 1e-5,
     utils/beetest:39-48): every column in FP64; the scalar columns (energies,
     virials, temperature, pressure) in mixed precision, whose pressure-tensor
-    off-diagonals are too small for a relative test."""
+    off-diagonals are too small for a relative test.  For polar molecules the
+    Coulomb energy and virial are small sums of large terms (7e4 of 2.4e7
+    terms): in mixed precision those two columns get the mixed error budget
+    of test_mixed_precision_error_budget (1e-6 of the largest term)."""
     s, ref = fixtures.load(name)
     print("case", name, precision, flush=True)
     want = fileio.read_statis(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "statis",
@@ -162,8 +181,16 @@ def test_dlpoly_inputs_run_to_the_reference_statis(name, precision):
     assert sorted(got) == sorted(want)
     ncol = len(want[0][1])
     cols = range(ncol) if precision == lib.FP64 else [c - 1 for c in fileio.STATIS_COLUMNS.values()]
+    budget = {}
+    if precision == lib.MIXED:
+        sc = coulomb_scales(oracle.two_body(s))
+        budget = {fileio.STATIS_COLUMNS["eng_cou"] - 1: 1e-6 * sc["engcpe"],
+                  fileio.STATIS_COLUMNS["vir_cou"] - 1: 1e-6 * sc["vircpe"]}
+
+    def ok(k, g, w):
+        return fileio.statis_equal(g, w, 1e-5) or (k in budget and abs(g - w) <= budget[k])
     bad = [(step, k + 1, got[step][1][k], w[k]) for step, (_, w) in want.items() for k in cols
-           if not fileio.statis_equal(got[step][1][k], w[k], 1e-5)]
+           if not ok(k, got[step][1][k], w[k])]
     assert not bad, bad[:10]
     # the written REVCON holds the run's last configuration
     L = np.diag(s.cell.reshape(3, 3))
@@ -218,7 +245,7 @@ def test_full_size_nacl_fp64_and_mixed(nacl64k):
     ctx.close()
     ctx, rm = gpu_two_body(s, precision=lib.MIXED)
     assert rms_rel(rm["frc"], o["frc"]) <= 1e-5
-    check_energies(rm, o, 1e-6, coulomb_scale=True)
+    check_energies(rm, o, 1e-6, coulomb_scales(o))
     # net force vanishes (pair forces antisymmetric, mean reciprocal force removed)
     assert np.abs(r["frc"].sum(0)).max() <= 1e-8 * np.abs(r["frc"]).max() * np.sqrt(s.natms)
     ctx.close()
@@ -236,7 +263,7 @@ def _resident_forces_vs_oracle(s, steps, precision):
     return rms_rel(f, o["frc"])
 
 
-@pytest.mark.parametrize("name", ["nacl_4096_k48", "nacl_1440_orth_q"])
+@pytest.mark.parametrize("name", ["nacl_4096_k48", "nacl_1440_orth_q", "mol4_3456_k32"])
 def test_resident_step_forces_match_oracle(name):
     s, _ = fixtures.load(name)
     print("case", name, flush=True)
