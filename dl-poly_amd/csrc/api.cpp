@@ -390,7 +390,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
-                   c->d_vdw_direct, c->excl_off, c->excl_ids,
+                   c->d_vdw_direct, c->excl_off, c->excl_ids, c->pp, c->Q2,
                    c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -572,6 +572,51 @@ int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int3
     return 0;
 }
 
+int dlpx_set_per_particle(dlpx_ctx *ctx, int32_t on)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || (on != 0 && on != 1)) return DLPX_ERR_BAD_ARG;
+    if (on && c->pair_lanes != 8)
+        return fail(c, DLPX_ERR_UNSUPPORTED, "per-particle output runs with 8 pair lanes (DLPX_PAIR_LANES)");
+    c->per_particle = on != 0;
+    c->pp_valid = false;
+    return 0;
+}
+
+int dlpx_add_per_particle(dlpx_ctx *ctx, double *energy, double *stress)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    if (!c->pp_valid)
+        return fail(c, DLPX_ERR_STATE, "no per-particle data: dlpx_set_per_particle(ctx, 1), then dlpx_two_body_forces");
+    const int n = c->natms;
+    std::vector<double> pp((size_t)n * kPPStride);
+    std::vector<double4> fr(c->prm.electrostatics ? n : 0);
+    std::vector<int32_t> gid(n);
+    DLPX_HIP(c, hipMemcpy(pp.data(), c->pp, sizeof(double) * pp.size(), hipMemcpyDeviceToHost));
+    DLPX_HIP(c, hipMemcpy(gid.data(), c->gid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (c->prm.electrostatics)
+        DLPX_HIP(c, hipMemcpy(fr.data(), c->frc_recip, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    // ewald_spole.F90:429-431,464-466 (reciprocal energy and stress scaled,
+    // the self energy shared evenly) and two_body.F90:783 (the vdW long-range
+    // correction shared evenly); the exclusion correction and the net-charge
+    // term have no per-particle part in the reference
+    const double scale = c->prm.electrostatics ? kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) *
+                                                     (0.5 * (1.0 / c->volm)) * (kR4pie0 / c->prm.epsilon)
+                                               : 0.0;
+    const double share = (c->prm.electrostatics ? c->self_energy : 0.0) / n + c->elrc / n;
+    const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // 3x3 element -> (xx, xy, xz, yy, yz, zz)
+    for (int i = 0; i < n; i++) {
+        const double *p = pp.data() + (size_t)i * kPPStride;
+        const int h = gid[i];
+        if (energy) energy[h] += p[0] + (c->prm.electrostatics ? fr[i].w * scale : 0.0) + share;
+        if (stress)
+            for (int k = 0; k < 9; k++)
+                stress[(size_t)9 * h + k] += p[1 + sym[k]] + (c->prm.electrostatics ? p[8 + sym[k]] * scale : 0.0);
+    }
+    return 0;
+}
+
 int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -655,7 +700,26 @@ int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
     if ((r = set_cell(c, cell))) return r;
     if ((r = upload(c, atoms, false, nullptr, nullptr, nullptr, nullptr))) return r;
     charges_summary(c, atoms);
-    if ((r = evaluate(c))) return r;
+    c->pp_valid = false;
+    if (c->per_particle) {
+        // per-atom records and the stress-grid copy, allocated on first use
+        if (c->cap > c->pp_cap) {
+            if (c->pp) (void)hipFree(c->pp);
+            c->pp = nullptr;
+            c->pp_cap = 0;
+            if ((r = dalloc(c, &c->pp, (size_t)c->cap * kPPStride))) return r;
+            c->pp_cap = c->cap;
+        }
+        if (c->prm.electrostatics && !c->Q2) {
+            const size_t rs = c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double);
+            DLPX_HIP(c, hipMalloc(&c->Q2, rs * c->nreal));
+        }
+        c->pp_now = true;
+    }
+    r = evaluate(c);
+    c->pp_now = false;
+    if (r) return r;
+    c->pp_valid = c->per_particle;
     if ((r = store_atoms(c, c->stage, false, false))) return r;
     const int n = c->natms;
     DLPX_HIP(c, hipMemcpyAsync(c->h_st, c->stage, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, c->stream));

@@ -18,6 +18,10 @@ constexpr double kHalfMinus = 0.49999999999999994;        // Nearest(0.5_wp, -1.
 constexpr int kLanesPerAtom = 4;                          // lanes sharing one atom's list
 constexpr int kMaxSplines = 16;
 constexpr int kReduceSlots = 28;
+// per-atom record of the per-particle outputs: [0] pair energy share,
+// [1..6] pair stress share (xx, xy, xz, yy, yz, zz), [8..13] reciprocal
+// stress (same order, before the SPME scale)
+constexpr int kPPStride = 16;
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs of an
 // MI355X (workgroup b on XCD b % 8), each with its own L2.  Kernels whose
@@ -158,6 +162,7 @@ struct Ctx {
     void *H = nullptr;               // complex half grid K3 x K2 x K1hp (x < K1h used)
     void *tw = nullptr;              // twiddles per axis: 3 x max(K) complex
     double *bsp_n2 = nullptr;        // |b|^2 per axis, K1 + K2 + K3 (FP64)
+    void *Q2 = nullptr;              // per-particle stress grid (a copy of the charge grid, then convolved)
     double self_energy = 0.0;
     double sumchg = 0.0;
     bool self_done = false;
@@ -172,6 +177,13 @@ struct Ctx {
     int energies_cap = 0;
     dlpx_two_body_out last_out{};    // the last step of the last resident run with energies
     bool have_last_out = false;
+
+    // per-particle energies and stresses (stats%collect_pp) of dlpx_two_body_forces
+    bool per_particle = false;       // requested (dlpx_set_per_particle)
+    bool pp_now = false;             // the evaluation under way computes them
+    bool pp_valid = false;           // the last dlpx_two_body_forces computed them
+    double *pp = nullptr;            // kPPStride doubles per atom (sorted order), see pair.hip
+    int pp_cap = 0;
 
     // timing
     bool timing = false;
@@ -241,10 +253,15 @@ struct InflParams {
     int K[3], K1h;
     double rcell[9];
     double cut2, test_fac, rcf;  // cut_off^2, (1e-6/rcf)^2, pi/alpha
+    double alpha;
+    // 0: the potential's influence function theta(m), with the reciprocal
+    // stress partials in reduction slots R_RS11..R_RS33; 1: the per-particle
+    // stress kernel B(m) f'(x) pi/(x alpha) (ewald_general.F90:1033-1057), no
+    // reductions
+    int mode;
 };
-// Q <- FFT-(theta . FFT+(Q)) in place on the real grid c->Q, with the
-// reciprocal stress partials in reduction slots R_RS11..R_RS33.
-int fft_convolve(Ctx *c, const InflParams &ip);
+// grid <- FFT-(kernel . FFT+(grid)) in place on a real grid (c->Q or a copy).
+int fft_convolve(Ctx *c, const InflParams &ip, void *grid);
 int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

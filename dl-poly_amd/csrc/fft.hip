@@ -337,7 +337,23 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
             const double bb1 = n2z[l] * bxy;
             const double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
             T2 h = res[lw * ns + l];
-            if (k2 <= ip.cut2 && k2 > ip.test_fac) {
+            if (ip.mode == 1) {
+                // ewald_general.F90:674-682 with stress_kernel (:1033-1057):
+                // B f'(x) pi / x / alpha, f(x) = exp(-x^2)/(sqrt(pi) x^2),
+                // f'(x) = -2 f/x - (2/x) exp(-x^2)/sqrt(pi) (spme.F90:421-436)
+                const double x = kPi * sqrt(k2) * (1.0 / ip.alpha);
+                if (k2 <= ip.cut2 && x > 1.0e-6) {
+                    const double ex = exp(-(x * x));
+                    const double energy = ex / (sqrpi * (x * x));
+                    const double fpd = (-2.0 / x * energy) - ((2.0 / x) * (1.0 / sqrpi) * ex);
+                    const double fac = fpd * kPi / x / ip.alpha;
+                    h.x = (T)(bb1 * (double)h.x * fac);
+                    h.y = (T)(bb1 * (double)h.y * fac);
+                } else {
+                    h.x = (T)0;
+                    h.y = (T)0;
+                }
+            } else if (k2 <= ip.cut2 && k2 > ip.test_fac) {
                 const double fp = sqrt(k2) * ip.rcf;
                 const double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
                 const double hr = (double)h.x, hi = (double)h.y;
@@ -374,7 +390,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
     __syncthreads();
     T2 *res2 = stockham<T, T2>(res, res == b0 ? b1 : b0, pl, stw, -1, lw, active, jt, TPL);
     store_lines<T2, LPB>(data, lm, n, L0, nl, res2);
-    block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
+    if (ip.mode == 0) block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
 }
 
 // x axis, forward: real lines 2P and 2P+1 of Q (n contiguous values each)
@@ -593,13 +609,14 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
                        ip, c->bsp_n2, c->red)
     DLPX_LPB_SWITCH(lpb, DLPX_Z)
 #undef DLPX_Z
-    for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
+    if (ip.mode == 0)
+        for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
 
 template <typename T>
-static int run_x(Ctx *c, bool forward)
+static int run_x(Ctx *c, bool forward, void *grid)
 {
     using T2 = typename C2T<T>::type;
     const int n = c->K[0], nlines = c->K[1] * c->K[2];
@@ -613,7 +630,7 @@ static int run_x(Ctx *c, bool forward)
     if (forward) {
 #define DLPX_R2C(L)                                                                                                 \
     DLPX_LDS_MAX(c, (k_fft_x_r2c<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)c->Q,           \
+    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)grid,           \
                        (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
@@ -621,7 +638,7 @@ static int run_x(Ctx *c, bool forward)
 #define DLPX_C2R(L)                                                                                                 \
     DLPX_LDS_MAX(c, (k_fft_x_c2r<T, L>));                                                                     \
     hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T2 *)c->H,          \
-                       (T *)c->Q, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn)
+                       (T *)grid, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn)
         DLPX_LPB_SWITCH(lpb, DLPX_C2R)
 #undef DLPX_C2R
     }
@@ -630,19 +647,19 @@ static int run_x(Ctx *c, bool forward)
 }
 
 template <typename T>
-static int convolve_t(Ctx *c, const InflParams &ip)
+static int convolve_t(Ctx *c, const InflParams &ip, void *grid)
 {
     int r;
-    if ((r = run_x<T>(c, true))) return r;
+    if ((r = run_x<T>(c, true, grid))) return r;
     if ((r = run_y<T>(c, +1))) return r;
     if ((r = run_z_conv<T>(c, ip))) return r;
     if ((r = run_y<T>(c, -1))) return r;
-    return run_x<T>(c, false);
+    return run_x<T>(c, false, grid);
 }
 
-int fft_convolve(Ctx *c, const InflParams &ip)
+int fft_convolve(Ctx *c, const InflParams &ip, void *grid)
 {
-    return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip) : convolve_t<double>(c, ip);
+    return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip, grid) : convolve_t<double>(c, ip, grid);
 }
 
 int conv_blocks(const Ctx *c)

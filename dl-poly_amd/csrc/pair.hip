@@ -140,14 +140,23 @@ constexpr int kPairThreads = 512;
 // pair lane p walks list lane p % kLanesPerAtom, entries c = p / kLanesPerAtom,
 // c + GP / kLanesPerAtom, ...  More lanes per atom give more wavefronts for
 // the same atoms (latency hiding on small systems) at the same total work.
-template <typename R, int GP, int NT, bool FS, bool DIRECT>
+//
+// PP (per-particle collection, stats%collect_pp): each atom also gets half of
+// every pair's energy and of r (x) f (vdw.F90:1987-1997, 1750-1760;
+// ewald_spole.F90:205-213), written to ppo[i*kPPStride + 0..6].  As in the
+// reference's tabulated vdW loop, whose pair energy is evaluated only on the
+// side that passes the halo tie-break (vdw.F90:1946-1966), the vdW energy of
+// a pair across the cell boundary (nonzero minimum-image shift) goes only to
+// the atom with the smaller global id (gid, the host index).
+template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false>
 __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
                                               const typename R2T<R>::type *__restrict__ vdw_g,
                                               const typename R2T<R>::type *__restrict__ erfc_g,
                                               const int32_t *__restrict__ keytab_g, const R *__restrict__ direct_g,
-                                              double4 *__restrict__ frc, double *__restrict__ red)
+                                              double4 *__restrict__ frc, double *__restrict__ red,
+                                              const int32_t *__restrict__ gid, double *__restrict__ ppo)
 {
     using R2 = typename R2T<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -176,6 +185,7 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     // engvdw, virvdw, engrl, (unused: the Coulomb virial is -tr(stress) - virvdw,
     // formed after the reduction), s11..s33
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double ppe = 0.0;  // PP: this atom's energy share (before the halving)
     if (i < n) {
         const double4 pi = pos[i];
         const int ti = typ[i];
@@ -192,13 +202,17 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
         int jn = c0 < m ? lst[(size_t)c0 * G] : i;
         double4 pn = pos[jn];
         int tn = typ[jn];
+        int gn = PP ? gid[jn] : 0;
+        const int gi = PP ? gid[i] : 0;
         jn = c0 + CS < m ? lst[(size_t)(c0 + CS) * G] : i;
         for (int c = c0; c < m; c += CS) {
             const double4 pj = pn;
             const int tj = tn;
+            const int gj = gn;
             if (c + CS < m) {
                 pn = pos[jn];
                 tn = typ[jn];
+                if (PP) gn = gid[jn];
                 if (c + 2 * CS < m) jn = lst[(size_t)(c + 2 * CS) * G];
             }
             const double dxd = min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]);
@@ -235,6 +249,10 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
                 gam = g;
                 acc[0] += (double)ev;
                 acc[1] -= (double)(g * (rrr * rrr));
+                if (PP) {
+                    const bool cross = dxd != pi.x - pj.x || dyd != pi.y - pj.y || dzd != pi.z - pj.z;
+                    if (DIRECT || !cross || gi < gj) ppe += (double)ev;
+                }
             }
             // no |q_j| > Tiny test: such a q_j contributes exact (or subnormal) zeros
             if (qi_on && rrr < rc) {
@@ -244,6 +262,7 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
                 const R eg = prefac * df;
                 gam += eg;
                 acc[2] += (double)(prefac * de);
+                if (PP) ppe += (double)(prefac * de);
             }
             const R fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
             fx += (double)fxp;
@@ -261,6 +280,14 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     fy = group_sum<GP>(fy);
     fz = group_sum<GP>(fz);
     if (i < n && p == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
+    if (PP) {
+        double v[7] = {ppe, acc[4], acc[5], acc[6], acc[7], acc[8], acc[9]};
+#pragma unroll
+        for (int k = 0; k < 7; k++) v[k] = group_sum<GP>(v[k]);
+        if (i < n && p == 0)
+#pragma unroll
+            for (int k = 0; k < 7; k++) ppo[(size_t)i * kPPStride + k] = 0.5 * v[k];
+    }
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] *= 0.5;  // every pair is met twice
     __syncthreads();
@@ -296,13 +323,27 @@ static int launch_pair(Ctx *c)
                        (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-#define DLPX_PAIR_K(GP, FS, DIR)                                                                             \
-    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR>));                                                   \
-    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR>), dim3(pair_blocks(c)), dim3(kPairThreads), lds,  \
-                       c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp, (const R2 *)c->d_vdw,   \
-                       (const R2 *)c->d_erfc, c->d_keytab, (const R *)c->d_vdw_direct, c->frc, c->red)
+#define DLPX_PAIR_KP(GP, FS, DIR, PPK)                                                                       \
+    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK>));                                              \
+    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR, PPK>), dim3(pair_blocks(c)), dim3(kPairThreads),  \
+                       lds, c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp,                   \
+                       (const R2 *)c->d_vdw, (const R2 *)c->d_erfc, c->d_keytab, (const R *)c->d_vdw_direct,    \
+                       c->frc, c->red, c->gid, c->pp)
+#define DLPX_PAIR_K(GP, FS, DIR) DLPX_PAIR_KP(GP, FS, DIR, false)
 #define DLPX_PAIR(GP, FS) DLPX_PAIR_K(GP, FS, false)
-    if (direct) {
+    if (c->pp_now) {
+        // per-particle collection at the default 8 lanes per atom only
+        if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "per-particle output runs with 8 pair lanes");
+        if (direct && c->vdw_force_shift) {
+            DLPX_PAIR_KP(8, true, true, true);
+        } else if (direct) {
+            DLPX_PAIR_KP(8, false, true, true);
+        } else if (c->vdw_force_shift) {
+            DLPX_PAIR_KP(8, true, false, true);
+        } else {
+            DLPX_PAIR_KP(8, false, false, true);
+        }
+    } else if (direct) {
         // direct mode at the default 8 lanes per atom only
         if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "vdw_method direct runs with 8 pair lanes");
         if (c->vdw_force_shift) {
@@ -325,6 +366,7 @@ static int launch_pair(Ctx *c)
     }
 #undef DLPX_PAIR
 #undef DLPX_PAIR_K
+#undef DLPX_PAIR_KP
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

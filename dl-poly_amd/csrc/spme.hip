@@ -37,8 +37,8 @@ struct SpmeParams {
 // values M[t] and first derivatives D[t] at the S grid points
 // floor(u) - S + 1 + t, t = 0..S-1, evaluated with the reference's recursion
 // (S even in 4..16; the reference forces an even order, control.F90:1651-1675).
-template <typename R, bool DERIV, int S>
-__device__ __forceinline__ void bspline(R w, R *M, R *D)
+template <typename R, bool DERIV, int S, bool DERIV2 = false>
+__device__ __forceinline__ void bspline(R w, R *M, R *D, R *D2 = nullptr)
 {
     R zd[S + 2];
 #pragma unroll
@@ -57,15 +57,18 @@ __device__ __forceinline__ void bspline(R w, R *M, R *D)
         }
         zd[S] = zd[S] * w * km1_rr;
     }
-    // l = 2 stage: raise to order S-1 (second derivatives are not needed)
+    // l = 2 stage: second derivatives from order S-2 (z_j - 2 z_j+1 + z_j+2,
+    // zd[S+1] = 0), then raise to order S-1
     {
         const R k_r = (R)(S - 1);
         const R km1_rr = (R)1 / (R)(S - 2);
 #pragma unroll
         for (int j = 1; j <= S - 1; j++) {
+            if (DERIV2) D2[j - 1] = (zd[j] - (R)2 * zd[j + 1]) + zd[j + 2];
             const R pt = w + (R)(S - j);
             zd[j] = (pt * zd[j] + (k_r - pt) * zd[j + 1]) * km1_rr;
         }
+        if (DERIV2) D2[S - 1] = zd[S];
         zd[S] = zd[S] * w * km1_rr;
     }
     // l = 1 stage: first derivatives from order S-1, then raise to order S
@@ -413,13 +416,112 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
     if (i < n && tx == 0) {
         // ewald_general.F90:800-855: F_i = -q sum dM.M.M.Phi * recip_kmax
         const double Fx_ = -q * fx * sp.rk[0], Fy_ = -q * fy * sp.rk[1], Fz_ = -q * fz * sp.rk[2];
-        frc_recip[i] = make_double4(Fx_, Fy_, Fz_, 0.0);
+        frc_recip[i] = make_double4(Fx_, Fy_, Fz_, q * e);  // w: the atom's energy (ewald_general.F90:852)
         acc[0] = q * e;
         acc[1] = Fx_;
         acc[2] = Fy_;
         acc[3] = Fz_;
     }
     block_sum<4>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ERECIP);
+}
+
+// Per-particle reciprocal stress (spme_calc_stress, ewald_general.F90:871-
+// 1009) from the stress grid G (the charge grid convolved with the stress
+// kernel): S_ab = -q sum (d_a d_b M_x M_y M_z) G kmax_a kmax_b / (4 pi alpha)
+// with second derivatives of the splines, for ab = 11, 21, 31, 22, 32, 33.
+// The lane layout of k_gather: lane tx takes grid column x; the (y, z) sums
+//   A = sum My Mz G, B = sum dMy Mz G, C = sum My dMz G,
+//   D = sum d2My Mz G, E = sum dMy dMz G, F = sum My d2Mz G
+// give S11 ~ d2Mx A, S21 ~ dMx B, S31 ~ dMx C, S22 ~ Mx D, S32 ~ Mx E,
+// S33 ~ Mx F.  Writes pp[i*kPPStride + 8..13] and the block sums of the six
+// components into R_RS11..R_RS33: with per-particle collection the reference
+// takes the reciprocal stress from these sums (ewald_general.F90:963,1002).
+template <typename R, int S>
+__global__ void __launch_bounds__(256) k_gather_stress(const double4 *__restrict__ pos, int n, SpmeParams sp,
+                                                       double c_fac, const R *__restrict__ Gs,
+                                                       double *__restrict__ pp, double *red)
+{
+    constexpr int G = gather_lanes(S), APB = 256 / G;
+    __shared__ R sM[3][APB][S];
+    __shared__ R sD[3][APB][S];
+    __shared__ R sD2[3][APB][S];
+    __shared__ int sB[3][APB];
+    __shared__ double sQ[APB];
+    __shared__ double lred[6 * 4];
+    const int a0 = xcd_block(blockIdx.x, gridDim.x) * APB;
+    const int t = threadIdx.x;
+    if (t < 3 * APB) {
+        const int d = t / APB, a = t % APB;
+        const int i = a0 + a;
+        if (i < n) {
+            const double4 p = pos[i];
+            int idx;
+            double w;
+            grid_coord(sp, d, p.x, p.y, p.z, idx, w);
+            R M[S], D[S], D2[S];
+            bspline<R, true, S, true>((R)w, M, D, D2);
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                sM[d][a][k] = M[k];
+                sD[d][a][k] = D[k];
+                sD2[d][a][k] = D2[k];
+            }
+            int b = idx - S + 1;
+            if (b < 0) b += sp.K[d];
+            sB[d][a] = b;
+            if (d == 0) sQ[a] = p.w;
+        }
+    }
+    __syncthreads();
+    const int a = t / G, tx = t % G, i = a0 + a;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n && tx < S) {
+        int gx = sB[0][a] + tx;
+        if (gx >= sp.K[0]) gx -= sp.K[0];
+        const int by = sB[1][a], bz = sB[2][a];
+        const size_t plane = (size_t)sp.K[0] * sp.K[1];
+        double A = 0, B = 0, C = 0, D = 0, E = 0, F = 0;
+        for (int tz = 0; tz < S; tz++) {
+            int gz = bz + tz;
+            if (gz >= sp.K[2]) gz -= sp.K[2];
+            const double mz = (double)sM[2][a][tz], dz = (double)sD[2][a][tz], d2z = (double)sD2[2][a][tz];
+            double a_ = 0, b_ = 0, d_ = 0;  // sum over y of My G, dMy G, d2My G
+            for (int ty = 0; ty < S; ty++) {
+                int gy = by + ty;
+                if (gy >= sp.K[1]) gy -= sp.K[1];
+                const double g = (double)Gs[(size_t)gz * plane + (size_t)gy * sp.K[0] + gx];
+                a_ += (double)sM[1][a][ty] * g;
+                b_ += (double)sD[1][a][ty] * g;
+                d_ += (double)sD2[1][a][ty] * g;
+            }
+            A += mz * a_;
+            B += mz * b_;
+            C += dz * a_;
+            D += mz * d_;
+            E += dz * b_;
+            F += d2z * a_;
+        }
+        const double mx = (double)sM[0][a][tx], dx = (double)sD[0][a][tx], d2x = (double)sD2[0][a][tx];
+        v[0] = d2x * A;
+        v[1] = dx * B;
+        v[2] = dx * C;
+        v[3] = mx * D;
+        v[4] = mx * E;
+        v[5] = mx * F;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = group_sum<G>(v[k]);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n && tx == 0) {
+        const double q = sQ[a];
+        const int ab[6][2] = {{0, 0}, {1, 0}, {2, 0}, {1, 1}, {2, 1}, {2, 2}};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            acc[k] = -(q * v[k]) * sp.rk[ab[k][0]] * sp.rk[ab[k][1]] * c_fac;
+            pp[(size_t)i * kPPStride + 8 + k] = acc[k];
+        }
+    }
+    block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
 }
 
 int gather_blocks(const Ctx *c)
@@ -581,7 +683,14 @@ static int spme_run(Ctx *c)
     const double m = std::min(std::min((double)c->K[0] * rcp[6], (double)c->K[1] * rcp[7]), (double)c->K[2] * rcp[8]);
     const double cut_off = 0.5 * 1.05 * m;
     ip.cut2 = cut_off * cut_off;
-    if ((r = fft_convolve(c, ip))) return r;
+    ip.alpha = c->prm.alpha;
+    ip.mode = 0;
+    if (c->pp_now) {
+        // keep the charge grid for the per-particle stress convolution
+        const size_t bytes = (c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double)) * c->nreal;
+        DLPX_HIP(c, hipMemcpyAsync(c->Q2, c->Q, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    if ((r = fft_convolve(c, ip, c->Q))) return r;
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[3], s));
     switch (sp.S) {
 #define DLPX_GATHER(SS)                                                                                     \
@@ -594,6 +703,26 @@ static int spme_run(Ctx *c)
         default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
     }
     for (int k = R_ERECIP; k <= R_FNETZ; k++) c->red_rows[k] = nb;
+    if (c->pp_now) {
+        // per-particle stresses: the saved charge grid convolved with the
+        // stress kernel, then gathered with second spline derivatives
+        InflParams ips = ip;
+        ips.mode = 1;
+        if ((r = fft_convolve(c, ips, c->Q2))) return r;
+        const double c_fac = 1.0 / (2.0 * 2.0 * kPi * c->prm.alpha);  // ewald_general.F90:932-933
+        switch (sp.S) {
+#define DLPX_GSTRESS(SS)                                                                                        \
+    case SS:                                                                                                    \
+        hipLaunchKernelGGL((k_gather_stress<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, c_fac,          \
+                           (const R *)c->Q2, c->pp, c->red);                                                    \
+        break;
+            DLPX_GSTRESS(4) DLPX_GSTRESS(6) DLPX_GSTRESS(8) DLPX_GSTRESS(10) DLPX_GSTRESS(12) DLPX_GSTRESS(14)
+            DLPX_GSTRESS(16)
+#undef DLPX_GSTRESS
+            default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
+        }
+        for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
+    }
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[4], s));
     DLPX_HIP(c, hipGetLastError());
     return 0;

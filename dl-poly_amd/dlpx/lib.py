@@ -24,6 +24,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
            "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
+           "dlpx_set_per_particle", "dlpx_add_per_particle",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -201,15 +202,23 @@ class Context:
         self._chk(self.lib.dlpx_vnl_check(self.h, C.byref(a), C.byref(u)))
         return bool(u.value)
 
-    def two_body_forces(self, pos, frc=None):
-        """Forces are added into frc (zeroed here when not given)."""
+    def two_body_forces(self, pos, frc=None, per_particle=False):
+        """Forces are added into frc (zeroed here when not given).  With
+        per_particle=True the result also holds pp_energy (N) and pp_stress
+        (N, 9; element a + 3b of atom i is stats%pp_stress(a+3b+1, i))."""
         if frc is None:
             frc = np.zeros((self.n, 3))
         a = self._atoms(pos, frc)
         o = TwoBodyOut()
+        self._chk(self.lib.dlpx_set_per_particle(self.h, C.c_int32(1 if per_particle else 0)))
         self._chk(self.lib.dlpx_two_body_forces(self.h, _ptr(self.cell), C.byref(a), C.byref(o)))
-        return dict(frc=frc, buffer=np.array(o.buffer[:]), stress=np.array(o.stress[:]), engcpe=o.engcpe,
-                    vircpe=o.vircpe, engsrp=o.engsrp, virsrp=o.virsrp)
+        res = dict(frc=frc, buffer=np.array(o.buffer[:]), stress=np.array(o.stress[:]), engcpe=o.engcpe,
+                   vircpe=o.vircpe, engsrp=o.engsrp, virsrp=o.virsrp)
+        if per_particle:
+            e, st = np.zeros(self.n), np.zeros((self.n, 9))
+            self._chk(self.lib.dlpx_add_per_particle(self.h, _ptr(e), _ptr(st)))
+            res["pp_energy"], res["pp_stress"] = e, st
+        return res
 
     def md_load(self, pos, vel):
         a = self._atoms(pos)

@@ -49,7 +49,8 @@ Module dlpx_binding
             dlpx_generate_vdw_tables, dlpx_generate_erfc_tables, dlpx_set_lrc, dlpx_compute_lrc, &
             dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_md_load, &
             dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_md_last_output, &
-            dlpx_set_vdw_force_shift, dlpx_set_vdw_direct, dlpx_set_exclusions, dlpx_error_message
+            dlpx_set_vdw_force_shift, dlpx_set_vdw_direct, dlpx_set_exclusions, dlpx_set_per_particle, &
+            dlpx_add_per_particle, dlpx_error_message
 
   Interface
     Integer(c_int) Function dlpx_create(params, ctx) Bind(C, name='dlpx_create')
@@ -214,6 +215,20 @@ Module dlpx_binding
       Integer(c_int32_t), Value :: natms, mxexcl
       Integer(c_int32_t), Intent(In) :: list_excl(0:mxexcl, *)
     End Function dlpx_set_exclusions
+
+    !> stats%collect_pp: on = 0 or 1 for the following dlpx_two_body_forces calls
+    Integer(c_int) Function dlpx_set_per_particle(ctx, on) Bind(C, name='dlpx_set_per_particle')
+      Import :: c_int, c_int32_t, c_ptr
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Value :: on
+    End Function dlpx_set_per_particle
+
+    !> adds the last call's values into stats%pp_energy(natms), stats%pp_stress(9, natms)
+    Integer(c_int) Function dlpx_add_per_particle(ctx, pp_energy, pp_stress) Bind(C, name='dlpx_add_per_particle')
+      Import :: c_int, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(InOut) :: pp_energy(*), pp_stress(9, *)
+    End Function dlpx_add_per_particle
 
     !> dlpx_md_run over many independent contexts in lockstep
     Integer(c_int) Function dlpx_md_run_ensemble(ctxs, nctx, nsteps, tsteps, nrebuilds) &

@@ -135,6 +135,25 @@ int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
  * Call before dlpx_link_cell_pairs / dlpx_md_load; natms = 0 clears them. */
 int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int32_t *list_excl);
 
+/* Per-particle energies and stresses, stats%collect_pp (statistics.F90:368-
+ * 402; used for HEATFLUX and per-particle output, drivers.F90:2117-2165).
+ * dlpx_set_per_particle(ctx, 1) makes each following dlpx_two_body_forces
+ * collect them as the reference does: half of every pair's energy and of
+ * r (x) f to each atom (vdw.F90:1750-1760,1987-1997, ewald_spole.F90:205-
+ * 213; a tabulated-vdW pair across the cell boundary gives its energy share
+ * only to the atom with the smaller global id, vdw.F90:1946-1966), the atom's
+ * reciprocal energy q sum M Phi and its stress from the stress-kernel grid
+ * (ewald_general.F90:578-715,871-1009,1033-1057), and the self energy and
+ * vdW long-range correction shared evenly (ewald_spole.F90:465,
+ * two_body.F90:783).  As in the reference, the reciprocal stress of those
+ * calls (and so vircpe and the stress output) is then the sum of the
+ * per-particle ones.  Needs the default 8 pair lanes.
+ * dlpx_add_per_particle adds the last call's values into pp_energy(natms)
+ * and pp_stress(9, natms) (column-major 3x3 per atom, element a+3b = r_a f_b
+ * summed); either pointer may be NULL. */
+int dlpx_set_per_particle(dlpx_ctx *ctx, int32_t on);
+int dlpx_add_per_particle(dlpx_ctx *ctx, double *pp_energy, double *pp_stress);
+
 /* link_cell_pairs (neighbours.F90:356-1414): build the Verlet list of pairs
  * within cutoff+padding from the given positions.  On success *npairs is the
  * number of unordered pairs and *ibig the largest per-atom count; with a fixed

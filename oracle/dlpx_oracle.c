@@ -550,6 +550,9 @@ typedef struct {
     long npairs;             /* pairs within rcut+padding */
     /* optional intermediates (NULL to skip): K1*K2*K3 reals each */
     double *charge_grid, *potential_grid;
+    /* per-particle energies (N) and stresses (9N, column-major 3x3 per atom)
+     * of stats%collect_pp (NULL to skip): overwritten */
+    double *pp_energy, *pp_stress;
 } orc_output;
 
 static int excluded(const orc_input *in, int i, int j)
@@ -584,6 +587,30 @@ void orc_excl_pair(double chgprd, double alpha, double rrr, double *erfr_out, do
     *egamma_out = egamma;
 }
 
+/* per-particle share of one pair i < j (vdw.F90:1987-1997, ewald_spole.F90:
+ * 205-213): half the energy and half of r (x) f (statistics.F90:2616-2625:
+ * element a + 3b = r_a f_b, f the force on i, r = x_i - x_j) to each atom.
+ * eng_j is the energy share j receives: the tabulated vdW loop evaluates the
+ * pair energy only on the side that passes the halo tie-break (vdw.F90:
+ * 1946-1966, ltg(i) < ltg(j) for a pair across the cell boundary), so there
+ * the other atom receives none. */
+static void pp_pair(orc_output *out, int i, int j, double eng, double eng_j, double xx, double yy, double zz,
+                    double fx, double fy, double fz)
+{
+    if (out->pp_energy) {
+        out->pp_energy[i] += eng * 0.5;
+        out->pp_energy[j] += eng_j * 0.5;
+    }
+    if (out->pp_stress) {
+        const double r[3] = {xx, yy, zz}, f[3] = {fx, fy, fz};
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                out->pp_stress[9 * i + a + 3 * b] += r[a] * f[b] * 0.5;
+                out->pp_stress[9 * j + a + 3 * b] += r[a] * f[b] * 0.5;
+            }
+    }
+}
+
 static int key_of(int ti, int tj)
 {
     int a = ti + 1, b = tj + 1, hi = a > b ? a : b, lo = a > b ? b : a;
@@ -602,6 +629,9 @@ int orc_two_body(const orc_input *in, orc_output *out)
     memset(out->stress, 0, sizeof(out->stress));
     double scaling = R4PIE0 / in->epsq;     /* two_body.F90:188 */
     double volm = fabs(L[0] * L[1] * L[2]);
+    double *ppe = out->pp_energy, *pps = out->pp_stress;
+    if (ppe) memset(ppe, 0, sizeof(double) * n);
+    if (pps) memset(pps, 0, sizeof(double) * 9 * n);
 
     /* ---- reciprocal space: ewald_spole.F90:244-477 ---- */
     double engcpe_rc = 0.0, vircpe_rc = 0.0, sumchg = 0.0, self = 0.0;
@@ -658,6 +688,12 @@ int orc_two_body(const orc_input *in, orc_output *out)
         double *G = malloc(sizeof(double) * 2 * KK);
         for (long g = 0; g < KK; g++) { G[2 * g] = Q[g]; G[2 * g + 1] = 0.0; }
         orc_fft3d(G, in->kvec, +1);
+        double *Gs = NULL;  /* FFT+(Q) kept for the per-particle stress grid */
+        double pst[6] = {0, 0, 0, 0, 0, 0};  /* sum of the per-particle stresses (unscaled) */
+        if (pps) {
+            Gs = malloc(sizeof(double) * 2 * KK);
+            memcpy(Gs, G, sizeof(double) * 2 * KK);
+        }
         double *n2x = malloc(sizeof(double) * K1), *n2y = malloc(sizeof(double) * K2), *n2z = malloc(sizeof(double) * K3);
         orc_bspline_norm2(K1, S, n2x); orc_bspline_norm2(K2, S, n2y); orc_bspline_norm2(K3, S, n2z);
         double rcp[10];
@@ -733,8 +769,75 @@ int orc_two_body(const orc_input *in, orc_output *out)
                 }
             }
             eng += et;
+            if (ppe) ppe[i] += et * scale;  /* ewald_general.F90:852, ewald_spole.F90:429,465 */
             fr[3 * i] -= cf[0]; fr[3 * i + 1] -= cf[1]; fr[3 * i + 2] -= cf[2];
         }
+        if (pps) {
+            /* per-particle reciprocal stress (ewald_general.F90:578-715 with
+             * stress_kernel :1033-1057, then spme_calc_stress :871-1009): the
+             * charge transform times B(m) f'(x) pi/(x alpha), x = pi|m|/alpha,
+             * f(x) = exp(-x^2)/(sqrt(pi) x^2) (spme.F90:233-247,421-436),
+             * transformed back, then gathered with second derivatives of the
+             * splines */
+            for (int l = 0; l < K3; l++) {
+                int ll = l; if (2 * ll > K3) ll -= K3;
+                for (int k = 0; k < K2; k++) {
+                    int kk = k; if (2 * kk > K2) kk -= K2;
+                    for (int j = 0; j < K1; j++) {
+                        int jj = j; if (2 * jj > K1) jj -= K1;
+                        /* the same summation order as the potential loop above */
+                        double r3[3] = {ll * rcell[2], ll * rcell[5], ll * rcell[8]};
+                        double r2[3] = {r3[0] + kk * rcell[1], r3[1] + kk * rcell[4], r3[2] + kk * rcell[7]};
+                        double rp[3] = {r2[0] + jj * rcell[0], r2[1] + jj * rcell[3], r2[2] + jj * rcell[6]};
+                        double bb1 = n2z[l] * n2y[k] * n2x[j];
+                        double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+                        long o = 2 * (((long)l * K2 + k) * K1 + j);
+                        double x = ORC_PI * sqrt(k2) * (1.0 / in->alpha);
+                        if (k2 <= cut_off_2 && x > 1.0e-6) {
+                            double energy = exp(-(x * x)) / (sqrpi * (x * x));
+                            double fpd = (-2.0 / x * energy) - ((2.0 / x) * (1.0 / sqrpi) * exp(-(x * x)));
+                            double fac = fpd * ORC_PI / x / in->alpha;
+                            Gs[o] = bb1 * Gs[o] * fac;
+                            Gs[o + 1] = bb1 * Gs[o + 1] * fac;
+                        } else {
+                            Gs[o] = 0.0; Gs[o + 1] = 0.0;
+                        }
+                    }
+                }
+            }
+            orc_fft3d(Gs, in->kvec, -1);
+            const double c_fac = 1.0 / (2.0 * 2.0 * ORC_PI * in->alpha);
+            for (int m = 0; m < 6; m++) pst[m] = 0.0;
+            for (int i = 0; i < n; i++) {
+                double qi = in->chge[i];
+                const double *Sx = spl + (3 * i + 0) * 3 * S, *Sy = spl + (3 * i + 1) * 3 * S, *Sz = spl + (3 * i + 2) * 3 * S;
+                /* lower triangle (1,1) (2,1) (3,1) (2,2) (3,2) (3,3) */
+                double st6[6] = {0, 0, 0, 0, 0, 0};
+                for (int tz = 0; tz < S; tz++) {
+                    int gz = ((idx[3 * i + 2] - S + 1 + tz) % K3 + K3) % K3;
+                    double z0 = qi * Sz[tz], z1 = qi * Sz[S + tz], z2 = qi * Sz[2 * S + tz];
+                    for (int ty = 0; ty < S; ty++) {
+                        int gy = ((idx[3 * i + 1] - S + 1 + ty) % K2 + K2) % K2;
+                        double y0 = Sy[ty], y1 = Sy[S + ty], y2 = Sy[2 * S + ty];
+                        double t11 = z0 * y0, t21 = z0 * y1, t31 = z1 * y0, t22 = z0 * y2, t32 = z1 * y1, t33 = z2 * y0;
+                        for (int tx = 0; tx < S; tx++) {
+                            int gx = ((idx[3 * i] - S + 1 + tx) % K1 + K1) % K1;
+                            double g = Gs[2 * (((long)gz * K2 + gy) * K1 + gx)];
+                            double x0 = Sx[tx], x1 = Sx[S + tx], x2 = Sx[2 * S + tx];
+                            double v[6] = {t11 * x2 * g, t21 * x1 * g, t31 * x1 * g, t22 * x0 * g, t32 * x0 * g, t33 * x0 * g};
+                            const int ab[6][2] = {{0, 0}, {1, 0}, {2, 0}, {1, 1}, {2, 1}, {2, 2}};
+                            for (int m = 0; m < 6; m++)
+                                st6[m] -= v[m] * rk[ab[m][0]] * rk[ab[m][1]] * c_fac;
+                        }
+                    }
+                }
+                const int map9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+                for (int k = 0; k < 9; k++) pps[9 * i + k] += st6[map9[k]] * scale;
+                for (int m = 0; m < 6; m++) pst[m] += st6[m];
+            }
+            free(Gs);
+        }
+        if (ppe) for (int i = 0; i < n; i++) ppe[i] += self / (double)n;  /* ewald_spole.F90:465 */
         for (int d = 0; d < 3; d++) { ftot[d] /= (double)n; out->recip_net[d] = ftot[d]; }
         for (int i = 0; i < n; i++)
             for (int d = 0; d < 3; d++) fr[3 * i + d] -= ftot[d];
@@ -744,6 +847,13 @@ int orc_two_body(const orc_input *in, orc_output *out)
         /* stress_temp(beta,alpha) column-major reshape => s9[(alpha-1)*3 + beta-1] */
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) s9[a * 3 + b] = st[a][b] * scale;
+        if (pps) {
+            /* with per-particle collection the reference's reciprocal stress
+             * is the sum of the per-particle ones (ewald_general.F90:963,1002
+             * overwrite the k-space sum) */
+            const int map9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+            for (int k = 0; k < 9; k++) s9[k] = pst[map9[k]] * scale;
+        }
         s9[0] += eng; s9[4] += eng; s9[8] += eng;
         for (int k = 0; k < 9; k++) out->stress[k] += s9[k];
         vircpe_rc = -(s9[0] + s9[4] + s9[8]);
@@ -799,6 +909,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
                 virvdw -= gamma * rsq;
                 st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
                 st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
+                pp_pair(out, i, j, eng, eng, xx, yy, zz, fx, fy, fz);
             }
         }
         /* vdW, tabulated */
@@ -827,6 +938,8 @@ int orc_two_body(const orc_input *in, orc_output *out)
             virvdw -= gamma * rsq;
             st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
             st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
+            const int cross = pairs[5 * p + 2] != 0 || pairs[5 * p + 3] != 0 || pairs[5 * p + 4] != 0;
+            pp_pair(out, i, j, eng, cross ? 0.0 : eng, xx, yy, zz, fx, fy, fz);
         }
         /* erfc-damped Coulomb */
         if (in->electro) {
@@ -837,8 +950,10 @@ int orc_two_body(const orc_input *in, orc_output *out)
                 double fx = eg * xx, fy = eg * yy, fz = eg * zz;
                 f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
                 f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
-                engrl += prefac * orc_interp3(in->erfc_t - 1, rdr_e, rrr);
+                double ec = prefac * orc_interp3(in->erfc_t - 1, rdr_e, rrr);
+                engrl += ec;
                 virrl -= eg * (rrr * rrr);
+                pp_pair(out, i, j, ec, ec, xx, yy, zz, fx, fy, fz);
                 st_c[0] += xx * fx; st_c[1] += xx * fy; st_c[2] += xx * fz;
                 st_c[3] += yy * fy; st_c[4] += yy * fz; st_c[5] += zz * fz;
             }
@@ -898,6 +1013,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
     out->stress[0] += tmp; out->stress[4] += tmp; out->stress[8] += tmp;
     out->engsrp = engvdw + in->elrc;
     out->virsrp = virvdw + in->vlrc;
+    if (ppe) for (int i = 0; i < n; i++) ppe[i] += in->elrc / (double)n;  /* two_body.F90:783 */
     tmp = -in->vlrc / 3.0;
     out->stress[0] += tmp; out->stress[4] += tmp; out->stress[8] += tmp;
     return 0;

@@ -64,7 +64,8 @@ class OrcOutput(C.Structure):
                 ("engsrp", C.c_double), ("virsrp", C.c_double), ("eng_self", C.c_double),
                 ("eng_recip", C.c_double), ("eng_nz", C.c_double), ("recip_net", C.c_double * 3),
                 ("npairs", C.c_long), ("charge_grid", C.POINTER(C.c_double)),
-                ("potential_grid", C.POINTER(C.c_double))]
+                ("potential_grid", C.POINTER(C.c_double)), ("pp_energy", C.POINTER(C.c_double)),
+                ("pp_stress", C.POINTER(C.c_double))]
 
 
 def nint(x):
@@ -127,7 +128,7 @@ def erfc_tables_exact(sysm, n=None):
     return n, et, ed
 
 
-def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
+def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None, per_particle=False):
     """One call of the restated two_body_forces (with its own pair search).
     exact_erfc=True swaps the reference's A&S erfc tables for the true erfc;
     table_points overrides the number of table samples (finer tables make the
@@ -135,7 +136,9 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
 
     Returns dict: frc (N,3), buffer (20), stress (9), engcpe, vircpe, engsrp,
     virsrp, eng_self, eng_recip, npairs and, with grids=True, the real charge
-    grid Q and potential grid Re(Phi) shaped (K3, K2, K1)."""
+    grid Q and potential grid Re(Phi) shaped (K3, K2, K1); with
+    per_particle=True, pp_energy (N) and pp_stress (N, 9; element a + 3b of
+    atom i is stats%pp_stress(a+3b+1, i))."""
     L = lib()
     mg, pot, frc, from_key = vdw_tables(sysm, table_points)
     ns, et, ed = erfc_tables_exact(sysm, table_points) if exact_erfc else erfc_tables(sysm)
@@ -173,6 +176,9 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
     if grids and sysm.electrostatics:
         Q = np.zeros(K[0] * K[1] * K[2]); P = np.zeros_like(Q)
         out.charge_grid = _p(Q); out.potential_grid = _p(P)
+    if per_particle:
+        ppe = np.zeros(sysm.natms); pps = np.zeros((sysm.natms, 9))
+        out.pp_energy = _p(ppe); out.pp_stress = _p(pps)
     rc = L.orc_two_body(C.byref(inp), C.byref(out))
     if rc != 0:
         raise RuntimeError("orc_two_body returned %d" % rc)
@@ -182,6 +188,8 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None):
                recip_net=np.array(out.recip_net[:]), npairs=out.npairs, elrc=elrc, vlrc=vlrc)
     if grids and sysm.electrostatics:
         res["Q"] = Q.reshape(K[2], K[1], K[0]); res["Phi"] = P.reshape(K[2], K[1], K[0])
+    if per_particle:
+        res["pp_energy"] = ppe; res["pp_stress"] = pps
     return res
 
 

@@ -43,3 +43,13 @@ def printed_digits_bound(ref, sig=10):
     a = np.abs(ref)
     e = np.floor(np.log10(np.where(a > 0, a, 1.0)))
     return 0.5 * 10.0 ** (e - (sig - 1))
+
+
+def heat_flux(s, pp_energy, pp_stress):
+    """statistics.F90:2643-2665 (internal energy units): (sum (e_i + m v^2/2) v
+    + sum S_i v) / V, S_i the 3x3 of pp_stress(:, i) (column-major)."""
+    v, m = s.vel, s.mass
+    ke = 0.5 * m * (v * v).sum(1)
+    ev = ((pp_energy + ke)[:, None] * v).sum(0)
+    S = pp_stress.reshape(-1, 3, 3).transpose(0, 2, 1)  # S[i, a, b] = pp_stress[i, a + 3b]
+    return (ev + np.einsum("iab,ib->a", S, v)) / s.volume

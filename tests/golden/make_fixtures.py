@@ -15,8 +15,19 @@ inputs and the reference's outputs:
 * ``statis/<case>.STATIS``: the reference's STATIS file itself (one record
   per step), for the cases in STATIS_CASES.
 
+* ``ref_heatflux``: for static cases, the step-0 record of the reference's
+  HEATFLUX file (CONTROL ``heat_flux on``; ``drivers.F90:2150-2158``,
+  ``statistics.F90:2643-2665``), which combines the per-particle energies
+  and stresses (stats%pp_energy, pp_stress) with the velocities, at 12
+  significant figures, with that run's REVIVE energies/virials
+  (``ref_pp_stpval``) and stress (``ref_pp_stress``): with per-particle
+  collection the reference's reciprocal stress is the sum of the per-particle
+  stresses (``ewald_general.F90:963,1002``), not the k-space sum, so its
+  virial and stress differ slightly.  ``--heatflux`` adds these to existing
+  fixtures after checking that the run reproduces their energies.
+
 Run in the build container (it needs the reference binary):
-    python tests/golden/make_fixtures.py [case ...] [--statis-only]
+    python tests/golden/make_fixtures.py [case ...] [--statis-only | --heatflux]
 """
 import os
 import subprocess
@@ -119,13 +130,40 @@ def run_case(name, sysm, steps, statis_only=False):
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))
 
 
+def add_heatflux(name, sysm):
+    """Run a static case with ``heat_flux on`` and store HEATFLUX's step-0
+    vector in its fixture (after checking the energies are the stored ones)."""
+    path = os.path.join(OUT, name + ".npz")
+    z = dict(np.load(path))
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, sysm, steps=0, extra=("heat_flux on",))
+        r = subprocess.run([REF], cwd=d, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0 or not os.path.exists(os.path.join(d, "HEATFLUX")):
+            raise RuntimeError("reference failed for %s (heat flux)" % name)
+        rv = fileio.read_revive(os.path.join(d, "REVIVE"))
+        rec = open(os.path.join(d, "HEATFLUX")).read().split()
+    assert int(rec[0]) == 0 and len(rec) == 6, rec
+    for k in (fileio.STP_ENGCPE, fileio.STP_ENGSRP):
+        assert rv["stpval"][k] == z["ref_stpval"][k], name
+    z["ref_heatflux"] = np.array([float(v) for v in rec[3:6]])
+    z["ref_pp_stpval"] = rv["stpval"]
+    z["ref_pp_stress"] = rv["stress"]
+    np.savez_compressed(path, **z)
+    print("%-28s heat flux %s" % (name, rec[3:6]))
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: python oracle/build_ref.py")
     statis_only = "--statis-only" in sys.argv[1:]
+    heatflux = "--heatflux" in sys.argv[1:]
     only = set(a for a in sys.argv[1:] if not a.startswith("--"))
     for name, (sysm, steps) in cases().items():
         if only and name not in only:
+            continue
+        if heatflux:
+            if steps == 0:
+                add_heatflux(name, sysm)
             continue
         run_case(name, sysm, steps, statis_only=statis_only)
 

@@ -125,6 +125,38 @@ def test_mixed_precision_error_budget(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", STATIC)
+def test_per_particle_matches_oracle_and_reference_heat_flux(name):
+    """stats%collect_pp through dlpx_set_per_particle / dlpx_add_per_particle:
+    every atom's energy and stress against the oracle (FP64, 1e-10 of the
+    largest), the reference's HEATFLUX record and its per-particle-mode stress;
+    the next call without collection is back on the k-space stress; mixed
+    precision within 1e-5 RMS."""
+    s, ref = fixtures.load(name)
+    print("case", name, flush=True)
+    z = np.load(os.path.join(fixtures.GOLDEN, name + ".npz"))
+    o = oracle.two_body(s, per_particle=True)
+    ctx = lib.Context(s)
+    ctx.link_cell_pairs(s.pos)
+    r = ctx.two_body_forces(s.pos, per_particle=True)
+    assert np.abs(r["pp_energy"] - o["pp_energy"]).max() <= 1e-10 * np.abs(o["pp_energy"]).max()
+    assert np.abs(r["pp_stress"] - o["pp_stress"]).max() <= 1e-10 * np.abs(o["pp_stress"]).max()
+    hf, want = fixtures.heat_flux(s, r["pp_energy"], r["pp_stress"]), z["ref_heatflux"]
+    assert np.abs(hf - want).max() <= 1e-10 * np.abs(want).max()
+    assert np.abs(r["stress"] - z["ref_pp_stress"]).max() <= 1e-11 * np.abs(z["ref_pp_stress"]).max()
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-10
+    r2 = ctx.two_body_forces(s.pos)
+    assert "pp_energy" not in r2
+    assert np.abs(r2["stress"] - ref["stress"]).max() <= 1e-11 * np.abs(ref["stress"]).max()
+    ctx.close()
+    ctx = lib.Context(s, precision=lib.MIXED)
+    ctx.link_cell_pairs(s.pos)
+    rm = ctx.two_body_forces(s.pos, per_particle=True)
+    assert rms_rel(rm["pp_energy"], o["pp_energy"]) <= 1e-5
+    assert rms_rel(rm["pp_stress"], o["pp_stress"]) <= 1e-5
+    ctx.close()
+
+
 def test_forces_added_to_existing_and_aos_stride():
     s, _ = fixtures.load("nacl_1000_k32")
     ctx = lib.Context(s)

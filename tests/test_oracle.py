@@ -2,6 +2,7 @@
 compiled reference's golden fixtures and against what the mathematics fixes.
 CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -9,7 +10,7 @@ from scipy import integrate, special
 
 import fixtures
 import oracle
-from dlpx import systems
+from dlpx import fileio, systems
 
 STATIC = [n for n in fixtures.NAMES if "nve" not in n]
 TRAJ = [n for n in fixtures.NAMES if "nve" in n]
@@ -315,3 +316,38 @@ def test_excl_pair_is_erf_over_r(r):
     else:
         assert abs(e - q * fe) <= q * 1.5e-7 / r
         assert abs(g - q * fg) <= q * 1.5e-7 / r ** 3
+
+
+@pytest.mark.parametrize("name", STATIC)
+def test_oracle_per_particle_matches_reference_heat_flux(name):
+    """Per-particle energies and stresses (stats%collect_pp) against the
+    reference's HEATFLUX record at step 0 (12 significant figures), which
+    weights every atom's values with its own velocity, and the reference's
+    stress and Coulomb virial of the same run, whose reciprocal part is the
+    sum of the per-particle stresses (ewald_general.F90:963,1002)."""
+    s, _ = fixtures.load(name)
+    z = np.load(os.path.join(fixtures.GOLDEN, name + ".npz"))
+    o = oracle.two_body(s, per_particle=True)
+    want = z["ref_heatflux"]
+    assert np.abs(fixtures.heat_flux(s, o["pp_energy"], o["pp_stress"]) - want).max() <= 1e-11 * np.abs(want).max()
+    st = z["ref_pp_stress"]
+    assert np.abs(o["stress"] - st).max() <= 1e-12 * np.abs(st).max()
+    vc = z["ref_pp_stpval"][fileio.STP_VIRCPE]
+    assert abs(o["vircpe"] - vc) <= 1e-12 * max(abs(vc), abs(o["buffer"][10]), abs(o["buffer"][16]), 1.0)
+
+
+def test_oracle_per_particle_sums():
+    """Sum rules of the per-particle split: with direct vdW (every pair's
+    energy shared by both atoms) the energies add up to the two-body energy
+    less the net-charge term, which has no per-particle part; without SPME the
+    stresses add up to the two-body stress less the long-range-correction
+    diagonal, which has none either."""
+    s, _ = fixtures.load("mix_4000_direct")
+    o = oracle.two_body(s, per_particle=True)
+    total = o["engsrp"] + o["engcpe"] - o["eng_nz"]
+    assert abs(o["pp_energy"].sum() - total) <= 1e-12 * abs(o["eng_self"])
+    s, _ = fixtures.load("argon_4000_jit")
+    o = oracle.two_body(s, per_particle=True)
+    want = o["stress"].reshape(3, 3) + np.eye(3) * o["vlrc"] / 3.0
+    got = o["pp_stress"].sum(0).reshape(3, 3).T
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
