@@ -37,7 +37,7 @@ VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP
 
 
 # kernels of each stage (mixed precision, spline order 8)
-KERNELS = {"pair": "k_pair<float, 8, 512, false, false>", "spread": "k_spread_cells<float, 8>",
+KERNELS = {"pair": "k_pair<float, 8, 512, false, false, false>", "spread": "k_spread_cells<float, 8>",
            "fft": "k_fft_x_r2c/k_fft_c2c/k_fft_z_conv/k_fft_x_c2r<float, 16>", "gather": "k_gather<float, 8>"}
 
 
