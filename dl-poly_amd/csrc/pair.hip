@@ -184,8 +184,11 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     double fx = 0.0, fy = 0.0, fz = 0.0;
     // engvdw, virvdw, engrl, (unused: the Coulomb virial is -tr(stress) - virvdw,
     // formed after the reduction), s11..s33
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double ppe = 0.0;  // PP: this atom's energy share (before the halving)
+    // per-lane partial sums over this lane's ~m/CS pairs in R (FP32 in mixed
+    // precision: one add per term instead of a conversion and an FP64 add),
+    // widened to FP64 before any sum across lanes; forces accumulate in FP64
+    R pacc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    R ppe = 0;  // PP: this atom's energy share (before the halving)
     if (i < n) {
         const double4 pi = pos[i];
         const int ti = typ[i];
@@ -247,11 +250,11 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
                     ev = ev + cut.x * (rrr * r_rvdw - (R)1) - cut.y;
                 }
                 gam = g;
-                acc[0] += (double)ev;
-                acc[1] -= (double)(g * (rrr * rrr));
+                pacc[0] += ev;
+                pacc[1] -= g * (rrr * rrr);
                 if (PP) {
                     const bool cross = dxd != pi.x - pj.x || dyd != pi.y - pj.y || dzd != pi.z - pj.z;
-                    if (DIRECT || !cross || gi < gj) ppe += (double)ev;
+                    if (DIRECT || !cross || gi < gj) ppe += ev;
                 }
             }
             // no |q_j| > Tiny test: such a q_j contributes exact (or subnormal) zeros
@@ -261,19 +264,19 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
                 const R prefac = qi * (R)pj.w;
                 const R eg = prefac * df;
                 gam += eg;
-                acc[2] += (double)(prefac * de);
-                if (PP) ppe += (double)(prefac * de);
+                pacc[2] += prefac * de;
+                if (PP) ppe += prefac * de;
             }
             const R fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
             fx += (double)fxp;
             fy += (double)fyp;
             fz += (double)fzp;
-            acc[4] += (double)(dx * fxp);
-            acc[5] += (double)(dx * fyp);
-            acc[6] += (double)(dx * fzp);
-            acc[7] += (double)(dy * fyp);
-            acc[8] += (double)(dy * fzp);
-            acc[9] += (double)(dz * fzp);
+            pacc[4] += dx * fxp;
+            pacc[5] += dx * fyp;
+            pacc[6] += dx * fzp;
+            pacc[7] += dy * fyp;
+            pacc[8] += dy * fzp;
+            pacc[9] += dz * fzp;
         }
     }
     fx = group_sum<GP>(fx);
@@ -281,15 +284,17 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
     fz = group_sum<GP>(fz);
     if (i < n && p == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
     if (PP) {
-        double v[7] = {ppe, acc[4], acc[5], acc[6], acc[7], acc[8], acc[9]};
+        double v[7] = {(double)ppe, (double)pacc[4], (double)pacc[5], (double)pacc[6], (double)pacc[7],
+                       (double)pacc[8], (double)pacc[9]};
 #pragma unroll
         for (int k = 0; k < 7; k++) v[k] = group_sum<GP>(v[k]);
         if (i < n && p == 0)
 #pragma unroll
             for (int k = 0; k < 7; k++) ppo[(size_t)i * kPPStride + k] = 0.5 * v[k];
     }
+    double acc[10];
 #pragma unroll
-    for (int k = 0; k < 10; k++) acc[k] *= 0.5;  // every pair is met twice
+    for (int k = 0; k < 10; k++) acc[k] = 0.5 * (double)pacc[k];  // every pair is met twice
     __syncthreads();
     block_sum<10>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ENGVDW);
 }
