@@ -172,8 +172,15 @@ static void charges_summary(Ctx *c, const dlpx_atoms *a)
     c->self_done = true;
 }
 
-// pair + SPME forces into c->frc (cell order); partial sums reduced to red_out
-static int evaluate(Ctx *c)
+double recip_scale(const Ctx *c)
+{
+    return kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) * (kR4pie0 / c->prm.epsilon);
+}
+
+// pair + SPME forces into c->frc (cell order); partial sums reduced to
+// red_out.  With defer_finish the reciprocal forces are left in frc_recip for
+// the velocity stage to add (nve_second), saving a pass over the atoms.
+static int evaluate(Ctx *c, bool defer_finish = false)
 {
     int r;
     if ((r = upload_tables(c))) return r;
@@ -214,10 +221,12 @@ static int evaluate(Ctx *c)
     if (c->excl_total > 0 && c->prm.electrostatics)
         for (int k = R_ENGEX; k <= R_XS33; k++) c->red_rows[k] = excl_blocks(c);
     if ((r = reduce_partials(c))) return r;
+    c->finish_in_vv2 = false;
     if (c->prm.electrostatics) {
-        const double scale = kPi * std::sqrt(kPi) * std::pow(c->prm.alpha, 1 - 3) * (0.5 * (1.0 / c->volm)) *
-                             (kR4pie0 / c->prm.epsilon);
-        if ((r = spme_finish(c, 2.0 * scale))) return r;
+        if (defer_finish)
+            c->finish_in_vv2 = true;
+        else if ((r = spme_finish(c, 2.0 * recip_scale(c))))
+            return r;
     }
     if (c->timing) {
         DLPX_HIP(c, hipEventRecord(c->ev[5], c->stream));
@@ -391,7 +400,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
                    c->d_vdw_direct, c->excl_off, c->excl_ids, c->pp, c->Q2,
-                   c->Q, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
+                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->h_st) (void)hipHostFree(c->h_st);
@@ -765,7 +774,7 @@ static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
     return {(uint64_t)c->pos,      (uint64_t)c->vel,     (uint64_t)c->frc,     (uint64_t)c->frc_recip,
             (uint64_t)c->xref,     (uint64_t)c->typ,     (uint64_t)c->nbr,     (uint64_t)c->cnt,
             (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
-            (uint64_t)c->Q,        (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
+            (uint64_t)c->Q,        (uint64_t)c->Phi,     (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
             (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
             (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
             (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total, t};
@@ -779,7 +788,7 @@ static int capture_step_graph(Ctx *c, double tstep)
     if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     c->step_graph = nullptr;
     DLPX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int r = evaluate(c);
+    int r = evaluate(c, true);
     if (!r) r = nve_second(c, tstep, true);
     if (!r) r = nve_first(c, tstep);
     hipGraph_t g = nullptr;
@@ -833,7 +842,7 @@ static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool en
         DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
         return 0;
     }
-    if ((r = evaluate(c))) return r;
+    if ((r = evaluate(c, true))) return r;
     if ((r = nve_second(c, tstep, true))) return r;
     if (c->timing && (r = collect_timing(c))) return r;
     if (energies) {

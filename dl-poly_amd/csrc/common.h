@@ -78,6 +78,7 @@ struct Ctx {
     // resident steps replay one captured graph: forces, stage 2 and the next
     // stage 1; graph_key records what the capture baked in (buffers, sizes)
     bool use_graphs = true;
+    bool finish_in_vv2 = false;      // the last evaluation left the reciprocal finish to stage 2
     bool vdw_force_shift = false;    // vdws%l_force_shift
     bool vdw_direct = false;         // vdws%l_direct: analytic forms per pair
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
@@ -158,7 +159,9 @@ struct Ctx {
     int K[3]{0, 0, 0};
     int K1h = 0, K1hp = 0;           // K1/2+1 and the padded row stride of H
     size_t nreal = 0, nhalf = 0;
-    void *Q = nullptr;               // real grid K3 x K2 x K1 (spread, then potential)
+    void *Q = nullptr;               // real charge grid K3 x K2 x K1; all zero between evaluations (the
+                                     // forward x pass clears each line it has read)
+    void *Phi = nullptr;             // real potential grid (the convolution's output)
     void *H = nullptr;               // complex half grid K3 x K2 x K1hp (x < K1h used)
     void *tw = nullptr;              // twiddles per axis: 3 x max(K) complex
     double *bsp_n2 = nullptr;        // |b|^2 per axis, K1 + K2 + K3 (FP64)
@@ -245,7 +248,8 @@ int pair_blocks(const Ctx *c);
 int gather_blocks(const Ctx *c);
 int nve_first(Ctx *c, double dt);            // also the vnl_check displacement (flags[4..6])
 double nve_first_displacement(Ctx *c);       // after the stream has reached nve_first
-int nve_second(Ctx *c, double dt, bool kinetic);
+int nve_second(Ctx *c, double dt, bool kinetic);  // also does a deferred reciprocal finish
+double recip_scale(const Ctx *c);                   // the SPME energy scale (ewald_spole.F90:429)
 
 // Influence function of the SPME convolution over the half spectrum
 // (x index 0..K1/2), ewald_spole.F90:1296-1380.
@@ -260,8 +264,9 @@ struct InflParams {
     // reductions
     int mode;
 };
-// grid <- FFT-(kernel . FFT+(grid)) in place on a real grid (c->Q or a copy).
-int fft_convolve(Ctx *c, const InflParams &ip, void *grid);
+// out <- FFT-(kernel . FFT+(in)) for real grids in and out (may be the same);
+// with clear_in the forward pass leaves in all zero.
+int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in);
 int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

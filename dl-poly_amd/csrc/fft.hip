@@ -395,11 +395,13 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
 
 // x axis, forward: real lines 2P and 2P+1 of Q (n contiguous values each)
 // packed as one complex line; output half spectra to H (K1h of a row of hs).
+// With clear, every grid value is set to zero once read (each is read by one
+// thread), so the charge grid is ready for the next spreading.
 template <typename T, int LPB>
-__global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__ Q, typename C2T<T>::type *H,
+__global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(T *Q, typename C2T<T>::type *H,
                                                            int nreal_lines, int K1h, int hs, FftPlan pl,
                                                            const typename C2T<T>::type *__restrict__ tw,
-                                                           FastDiv div_n, FastDiv div_h)
+                                                           FastDiv div_n, FastDiv div_h, bool clear)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -420,6 +422,19 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(const T *__restrict__
                 const int a = 2 * (P0 + ll), b = a + 1;
                 z[u].x = Q[(size_t)a * n + e];
                 z[u].y = b < nreal_lines ? Q[(size_t)b * n + e] : (T)0;
+            }
+        }
+        if (clear) {
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; u++) {
+                const int idx = i0 + u * kFftThreads;
+                if (idx < np * n) {
+                    int e;
+                    const int ll = fdivmod(idx, div_n, e);
+                    const int a = 2 * (P0 + ll), b = a + 1;
+                    Q[(size_t)a * n + e] = (T)0;
+                    if (b < nreal_lines) Q[(size_t)b * n + e] = (T)0;
+                }
             }
         }
 #pragma unroll
@@ -616,7 +631,7 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
 }
 
 template <typename T>
-static int run_x(Ctx *c, bool forward, void *grid)
+static int run_x(Ctx *c, bool forward, void *grid, bool clear = false)
 {
     using T2 = typename C2T<T>::type;
     const int n = c->K[0], nlines = c->K[1] * c->K[2];
@@ -630,8 +645,8 @@ static int run_x(Ctx *c, bool forward, void *grid)
     if (forward) {
 #define DLPX_R2C(L)                                                                                                 \
     DLPX_LDS_MAX(c, (k_fft_x_r2c<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T *)grid,           \
-                       (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh)
+    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T *)grid,                 \
+                       (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh, clear)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
     } else {
@@ -647,19 +662,20 @@ static int run_x(Ctx *c, bool forward, void *grid)
 }
 
 template <typename T>
-static int convolve_t(Ctx *c, const InflParams &ip, void *grid)
+static int convolve_t(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in)
 {
     int r;
-    if ((r = run_x<T>(c, true, grid))) return r;
+    if ((r = run_x<T>(c, true, in, clear_in))) return r;
     if ((r = run_y<T>(c, +1))) return r;
     if ((r = run_z_conv<T>(c, ip))) return r;
     if ((r = run_y<T>(c, -1))) return r;
-    return run_x<T>(c, false, grid);
+    return run_x<T>(c, false, out);
 }
 
-int fft_convolve(Ctx *c, const InflParams &ip, void *grid)
+int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in)
 {
-    return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip, grid) : convolve_t<double>(c, ip, grid);
+    return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip, in, out, clear_in)
+                                          : convolve_t<double>(c, ip, in, out, clear_in);
 }
 
 int conv_blocks(const Ctx *c)

@@ -70,16 +70,29 @@ __global__ void __launch_bounds__(kVv1Threads) k_vv1(double4 *pos, double4 *vel,
     }
 }
 
-// Stage 2: v += dt/2 F/m, and sum m v^2 per block
-__global__ void __launch_bounds__(256) k_vv2(double4 *vel, const double4 *frc, int n, double dt, double *red)
+// Stage 2: v += dt/2 F/m, and sum m v^2 per block.  With FIN the
+// reciprocal forces are first added to the total force as k_finish does:
+// frc += 2 scale (frc_recip - mean frc_recip) (ewald_general.F90:857-864).
+template <bool FIN>
+__global__ void __launch_bounds__(256) k_vv2(double4 *vel, double4 *frc, int n, double dt, double *red,
+                                             const double4 *__restrict__ frc_recip, double two_scale,
+                                             const double *__restrict__ red_out, double inv_n)
 {
     __shared__ double lred[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (i < n) {
         double4 v = vel[i];
+        double4 f = frc[i];
+        if (FIN) {
+            const double mx = red_out[R_FNETX] * inv_n, my = red_out[R_FNETY] * inv_n, mz = red_out[R_FNETZ] * inv_n;
+            const double4 r = frc_recip[i];
+            f.x += (r.x - mx) * two_scale;
+            f.y += (r.y - my) * two_scale;
+            f.z += (r.z - mz) * two_scale;
+            frc[i] = f;
+        }
         if (v.w > 1.0e-6) {
-            const double4 f = frc[i];
             const double tmp = (0.5 * dt) / v.w;
             v.x += tmp * f.x;
             v.y += tmp * f.y;
@@ -155,7 +168,13 @@ int nve_second(Ctx *c, double dt, bool /*kinetic*/)
     const int nb = ceil_div(c->natms, 256);
     int r;
     if ((r = ensure_red_rows(c, nb))) return r;
-    hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red);
+    if (c->finish_in_vv2)
+        hipLaunchKernelGGL(k_vv2<true>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
+                           c->frc_recip, 2.0 * recip_scale(c), c->red_out, 1.0 / (double)c->natms);
+    else
+        hipLaunchKernelGGL(k_vv2<false>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
+                           nullptr, 0.0, nullptr, 0.0);
+    c->finish_in_vv2 = false;
     c->red_rows[R_KIN] = nb;
     DLPX_HIP(c, hipGetLastError());
     return 0;

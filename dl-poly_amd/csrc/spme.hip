@@ -297,14 +297,6 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
     }
 }
 
-// Grid-stride clear of n grid values.
-template <typename R>
-__global__ void k_zero(R *__restrict__ q, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        q[i] = (R)0;
-}
-
 // Lanes per atom in the gather: the next power of two >= S, lane tx < S
 // taking the grid column x = base + tx.
 constexpr int gather_lanes(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : 16; }
@@ -619,6 +611,8 @@ int spme_setup(Ctx *c)
     c->nhalf = (size_t)c->K1hp * c->K[1] * c->K[2];
     const size_t rs = c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double);
     DLPX_HIP(c, hipMalloc(&c->Q, rs * c->nreal));
+    DLPX_HIP(c, hipMemset(c->Q, 0, rs * c->nreal));  // kept zero from here on by the forward x pass
+    DLPX_HIP(c, hipMalloc(&c->Phi, rs * c->nreal));
     DLPX_HIP(c, hipMalloc(&c->H, 2 * rs * c->nhalf));
     const int ksum = c->K[0] + c->K[1] + c->K[2];
     std::vector<double> n2(ksum);
@@ -655,10 +649,9 @@ static int spme_run(Ctx *c)
     const SpmeParams sp = spme_params(c);
     const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
     int r;
-    // ewald_general.F90:517-576 charge spreading; the grid is cleared with a
-    // kernel (this sequence is also captured into step graphs)
-    hipLaunchKernelGGL(k_zero<R>, dim3(std::min<size_t>((c->nreal + 1023) / 1024, 4096)), dim3(256), 0, s, (R *)c->Q,
-                       c->nreal);
+    // ewald_general.F90:517-576 charge spreading into the charge grid, which
+    // is zero here: set at allocation, and cleared by the forward x pass of
+    // each convolution after reading it
     switch (sp.S) {
 #define DLPX_SPREAD(SS)                                                                                    \
     case SS:                                                                                               \
@@ -690,12 +683,12 @@ static int spme_run(Ctx *c)
         const size_t bytes = (c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double)) * c->nreal;
         DLPX_HIP(c, hipMemcpyAsync(c->Q2, c->Q, bytes, hipMemcpyDeviceToDevice, s));
     }
-    if ((r = fft_convolve(c, ip, c->Q))) return r;
+    if ((r = fft_convolve(c, ip, c->Q, c->Phi, true))) return r;
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[3], s));
     switch (sp.S) {
 #define DLPX_GATHER(SS)                                                                                     \
     case SS:                                                                                                \
-        hipLaunchKernelGGL((k_gather<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, (const R *)c->Q,   \
+        hipLaunchKernelGGL((k_gather<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, (const R *)c->Phi, \
                            c->frc_recip, c->red);                                                           \
         break;
         DLPX_GATHER(4) DLPX_GATHER(6) DLPX_GATHER(8) DLPX_GATHER(10) DLPX_GATHER(12) DLPX_GATHER(14) DLPX_GATHER(16)
@@ -708,7 +701,7 @@ static int spme_run(Ctx *c)
         // stress kernel, then gathered with second spline derivatives
         InflParams ips = ip;
         ips.mode = 1;
-        if ((r = fft_convolve(c, ips, c->Q2))) return r;
+        if ((r = fft_convolve(c, ips, c->Q2, c->Q2, false))) return r;
         const double c_fac = 1.0 / (2.0 * 2.0 * kPi * c->prm.alpha);  // ewald_general.F90:932-933
         switch (sp.S) {
 #define DLPX_GSTRESS(SS)                                                                                        \
