@@ -157,6 +157,79 @@ def test_per_particle_matches_oracle_and_reference_heat_flux(name):
     ctx.close()
 
 
+def _tiny_nacl(n_pairs=4, L=26.0, seed=5):
+    """A few ions scattered (at least 2.4 A apart) in a box just over twice
+    the cutoff: fewer atoms than a wavefront, 2 link cells per axis (the
+    link-cell list path), most cells empty."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    while len(pos) < 2 * n_pairs:
+        p = rng.uniform(-L / 2, L / 2, 3)
+        if all(np.linalg.norm((p - q) - L * np.round((p - q) / L)) > 2.4 for q in pos):
+            pos.append(p)
+    base = systems.nacl(ncells=1, kvec=(24, 24, 24), cutoff=12.0, padding=0.5)
+    types = np.array([0, 1] * n_pairs, np.int32)
+    order = np.argsort(types, kind="stable")
+    vel = rng.normal(scale=5.0, size=(2 * n_pairs, 3))
+    return systems.System(name="tiny", cell=np.diag([L, L, L]).reshape(9), pos=np.array(pos)[order],
+                          vel=vel[order], types=types[order], type_names=base.type_names, masses=base.masses,
+                          charges=base.charges, vdw=base.vdw, cutoff=12.0, padding=0.5, vdw_cutoff=12.0,
+                          electrostatics=True, alpha=0.28, kvec=(24, 24, 24), nsplines=8, timestep=0.002)
+
+
+@pytest.mark.parametrize("n_pairs", [1, 4, 40])
+def test_tiny_systems_match_oracle(n_pairs):
+    """2 to 80 ions: partial wavefronts and workgroups, empty link cells, the
+    link-cell (not subcell) list build; FP64 to 1e-10, mixed to 1e-5, and
+    the pair set exactly."""
+    s = _tiny_nacl(n_pairs)
+    print("case tiny", s.natms, flush=True)
+    o = oracle.two_body(s)
+    ctx, r = gpu_two_body(s)
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-10
+    check_energies(r, o, 1e-11, coulomb_scales(o))
+    g = ctx.get_pairs()
+    op = oracle.pairs(s.cell, s.pos, s.cutoff + s.padding, "brute")
+    assert {tuple(p) for p in g} == {tuple(p) for p in op}
+    ctx.close()
+    ctx, rm = gpu_two_body(s, precision=lib.MIXED)
+    assert rms_rel(rm["frc"], o["frc"]) <= 1e-5
+    ctx.close()
+
+
+def test_half_empty_box_matches_oracle():
+    """Argon atoms kept only in the lower half of the box (z < 0): half the
+    link cells and subcells are empty."""
+    s, _ = fixtures.load("argon_4000_jit")
+    keep = s.pos[:, 2] < 0.0
+    s.pos, s.vel, s.types = s.pos[keep], s.vel[keep], s.types[keep]
+    print("case half-empty argon", s.natms, flush=True)
+    o = oracle.two_body(s)
+    ctx, r = gpu_two_body(s)
+    assert rms_rel(r["frc"], o["frc"]) <= 1e-10
+    check_energies(r, o, 1e-11)
+    ctx.close()
+
+
+def test_unfolded_positions_give_the_same_forces():
+    """Positions outside the cell (a third of the ions moved by a lattice
+    vector, as a host that does not fold them would pass them): the same
+    forces, energies and pair set as the folded configuration."""
+    s, _ = fixtures.load("nacl_1000_k32")
+    print("case unfolded nacl 1000", flush=True)
+    L = np.diag(s.cell.reshape(3, 3))
+    rng = np.random.default_rng(2)
+    shift = rng.integers(-1, 2, size=s.pos.shape) * (rng.random(s.natms) < 1 / 3)[:, None]
+    ctx, r = gpu_two_body(s)
+    ctx2, r2 = gpu_two_body(s, pos=s.pos + shift * L)
+    assert np.abs(r2["frc"] - r["frc"]).max() <= 1e-11 * np.abs(r["frc"]).max()
+    for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
+        assert abs(r2[k] - r[k]) <= 1e-12 * max(abs(r[k]), abs(r["buffer"][10]), 1.0), k
+    assert {tuple(p[:2]) for p in ctx.get_pairs()} == {tuple(p[:2]) for p in ctx2.get_pairs()}
+    ctx.close()
+    ctx2.close()
+
+
 def test_forces_added_to_existing_and_aos_stride():
     s, _ = fixtures.load("nacl_1000_k32")
     ctx = lib.Context(s)
