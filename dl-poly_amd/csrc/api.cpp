@@ -57,12 +57,36 @@ int upload_tables(Ctx *c)
     const bool mixed = c->prm.precision == DLPX_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     const int mg1 = c->max_grid + 1, ns1 = c->nsamples + 1;
-    std::vector<double> v((size_t)2 * c->nkeys * mg1), e((size_t)2 * std::max(ns1, 1), 0.0);
-    for (int k = 0; k < c->nkeys; k++)
-        for (int l = 0; l < mg1; l++) {
-            v[2 * ((size_t)k * mg1 + l)] = c->vdw_frc[(size_t)k * mg1 + l];
-            v[2 * ((size_t)k * mg1 + l) + 1] = c->vdw_pot[(size_t)k * mg1 + l];
+    // pair keys whose interactions are identical (same tables and, for direct
+    // evaluation, the same form and parameters) share one device row, so many
+    // atom types of few distinct interactions still fit the pair kernel's LDS
+    std::vector<int> row_of(c->nkeys, -1), key_of_row;
+    for (int k = 0; k < c->nkeys; k++) {
+        if (c->ltp[k] < 0) continue;
+        for (int q = 0; q < (int)key_of_row.size() && row_of[k] < 0; q++) {
+            const int k2 = key_of_row[q];
+            bool same = c->ltp[k2] == c->ltp[k] &&
+                        !std::memcmp(&c->vdw_frc[(size_t)k * mg1], &c->vdw_frc[(size_t)k2 * mg1], sizeof(double) * mg1) &&
+                        !std::memcmp(&c->vdw_pot[(size_t)k * mg1], &c->vdw_pot[(size_t)k2 * mg1], sizeof(double) * mg1);
+            if (same && c->vdw_direct)
+                same = !std::memcmp(&c->gen_params[5 * (size_t)k], &c->gen_params[5 * (size_t)k2], sizeof(double) * 5);
+            if (same) row_of[k] = q;
         }
+        if (row_of[k] < 0) {
+            row_of[k] = (int)key_of_row.size();
+            key_of_row.push_back(k);
+        }
+    }
+    c->nrows = (int)key_of_row.size();
+    const int nrows_alloc = std::max(c->nrows, 1);
+    std::vector<double> v((size_t)2 * nrows_alloc * mg1, 0.0), e((size_t)2 * std::max(ns1, 1), 0.0);
+    for (int q = 0; q < c->nrows; q++) {
+        const int k = key_of_row[q];
+        for (int l = 0; l < mg1; l++) {
+            v[2 * ((size_t)q * mg1 + l)] = c->vdw_frc[(size_t)k * mg1 + l];
+            v[2 * ((size_t)q * mg1 + l) + 1] = c->vdw_pot[(size_t)k * mg1 + l];
+        }
+    }
     for (int l = 1; l < ns1; l++) {  // sample l <-> r = l * spacing
         e[2 * l] = c->erfc_d[l - 1];
         e[2 * l + 1] = c->erfc_t[l - 1];
@@ -88,7 +112,7 @@ int upload_tables(Ctx *c)
             const int hi = std::max(a, b), lo = std::min(a, b);
             const int key = hi * (hi - 1) / 2 + lo - 1;
             // vdw.F90:1883: an undefined pair (tab_potential(0,k) ~ 0) is skipped
-            kt[(size_t)(a - 1) * nt + (b - 1)] = (c->ltp[key] < 0) ? -1 : key;
+            kt[(size_t)(a - 1) * nt + (b - 1)] = (c->ltp[key] < 0) ? -1 : row_of[key];
         }
     if (c->d_keytab) (void)hipFree(c->d_keytab);
     c->d_keytab = nullptr;
@@ -97,18 +121,19 @@ int upload_tables(Ctx *c)
     if (c->vdw_direct) {
         // vdw_method direct: per key {form, 5 parameters, afs, bfs}; afs, bfs
         // follow vdw_direct_fs_generate (vdw.F90:969-1048) under force-shifting
-        std::vector<double> d((size_t)8 * c->nkeys, 0.0);
-        for (int k = 0; k < c->nkeys; k++) {
+        std::vector<double> d((size_t)8 * nrows_alloc, 0.0);
+        for (int row = 0; row < c->nrows; row++) {
+            const int k = key_of_row[row];
             const int kind = c->ltp[k];
             if (!vdw_kind_supported(kind)) continue;
             const double *pk = c->gen_params.data() + 5 * (size_t)k;
-            d[8 * k] = (double)kind;
-            for (int q = 0; q < 5; q++) d[8 * k + 1 + q] = pk[q];
+            d[8 * row] = (double)kind;
+            for (int q = 0; q < 5; q++) d[8 * row + 1 + q] = pk[q];
             if (c->vdw_force_shift) {
                 double z, dz;
                 vdw_form(kind, pk, c->prm.vdw_cutoff, &z, &dz);
-                d[8 * k + 6] = dz / c->prm.vdw_cutoff;
-                d[8 * k + 7] = -z - dz;
+                d[8 * row + 6] = dz / c->prm.vdw_cutoff;
+                d[8 * row + 7] = -z - dz;
             }
         }
         if (c->d_vdw_direct) (void)hipFree(c->d_vdw_direct);
@@ -399,7 +424,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
-                   c->d_vdw_direct, c->excl_off, c->excl_ids, c->pp, c->Q2,
+                   c->d_vdw_direct, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
                    c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -581,6 +606,46 @@ int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int3
     return 0;
 }
 
+int dlpx_set_frozen(dlpx_ctx *ctx, int32_t natms, const int32_t *lfrzn)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return DLPX_ERR_BAD_ARG;
+    if (c->resident) return fail(c, DLPX_ERR_STATE, "set the frozen atoms before dlpx_md_load");
+    if (natms < 0 || natms > c->cap || (natms > 0 && !lfrzn))
+        return fail(c, DLPX_ERR_BAD_ARG, "frozen flags: natms outside 0..natms_max or no array");
+    std::vector<int32_t> f(natms);
+    int nf = 0;
+    for (int i = 0; i < natms; i++) {
+        f[i] = lfrzn[i] != 0 ? 1 : 0;
+        nf += f[i];
+    }
+    if (c->frz) (void)hipFree(c->frz);
+    c->frz = nullptr;
+    c->frz_natms = c->nfrozen = 0;
+    c->have_list = false;  // the list was built without these flags
+    if (nf == 0) return 0;
+    int r;
+    if ((r = dalloc(c, &c->frz, (size_t)natms))) return r;
+    DLPX_HIP(c, hipMemcpy(c->frz, f.data(), sizeof(int32_t) * natms, hipMemcpyHostToDevice));
+    c->frz_natms = natms;
+    c->nfrozen = nf;
+    return 0;
+}
+
+int dlpx_compute_lrc_frozen(dlpx_ctx *ctx, const double cell[9], const double *num_type, const double *num_frozen)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !num_type || !num_frozen) return DLPX_ERR_BAD_ARG;
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    if (c->gen_params.empty())
+        return fail(c, DLPX_ERR_STATE, "tables were not generated here: pass the host's vdws%elrc/vlrc to dlpx_set_lrc");
+    vdw_lrc(c->prm.ntypes, num_type, c->ltp.data(), c->gen_params.data(), c->prm.vdw_cutoff, c->volm, &c->elrc,
+            &c->vlrc, num_frozen);
+    if (c->vdw_force_shift) c->elrc = c->vlrc = 0.0;  // vdw.F90:662
+    return 0;
+}
+
 int dlpx_set_per_particle(dlpx_ctx *ctx, int32_t on)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -758,6 +823,7 @@ int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, c
     if ((r = build_list(c))) return r;
     // forces at the starting configuration (md_vv computes them before step 1)
     if ((r = evaluate(c))) return r;
+    if ((r = freeze_atoms(c))) return r;
     if ((r = fetch_red(c))) return r;
     // the starting configuration's output, until a run with energies replaces it
     assemble(c, c->h_red, &c->last_out);
@@ -777,7 +843,8 @@ static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
             (uint64_t)c->Q,        (uint64_t)c->Phi,     (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
             (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
             (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
-            (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total, t};
+            (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total,
+            (uint64_t)c->frz,      (uint64_t)c->nfrozen, t};
 }
 
 // Capture forces (both streams), stage 2 of this step and stage 1 of the

@@ -101,6 +101,7 @@ struct Ctx {
 
     // host tables
     int nkeys = 0, max_grid = 0;
+    int nrows = 0;                         // distinct table rows on the device (keys with equal tables share one)
     std::vector<double> vdw_pot, vdw_frc;  // nkeys x (max_grid+1)
     std::vector<int32_t> ltp;
     std::vector<double> gen_params;        // 5 per key when the tables were generated here
@@ -108,6 +109,11 @@ struct Ctx {
     std::vector<double> erfc_t, erfc_d;    // nsamples
     double elrc = 0.0, vlrc = 0.0;
     bool tables_dirty = true;
+
+    // frozen atoms (config%lfrzn), per host index
+    int frz_natms = 0;               // atoms the flags were given for (0: none)
+    int nfrozen = 0;
+    int32_t *frz = nullptr;          // 1 = frozen
 
     // excluded pairs (neigh%list_excl), symmetric, as CSR over host indices
     int excl_natms = 0;              // atoms the lists were given for (0: none)
@@ -231,7 +237,7 @@ void bspline_norm2(int K, int S, double *norm2);
 void invert_cell(const double *a, double *b, double *det);
 void dcell(const double *a, double *b);
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
-             double volm, double *elrc, double *vlrc);
+             double volm, double *elrc, double *vlrc, const double *num_frz = nullptr);
 
 // device stages (launch wrappers, one per .hip file)
 int upload_tables(Ctx *c);
@@ -249,6 +255,7 @@ int gather_blocks(const Ctx *c);
 int nve_first(Ctx *c, double dt);            // also the vnl_check displacement (flags[4..6])
 double nve_first_displacement(Ctx *c);       // after the stream has reached nve_first
 int nve_second(Ctx *c, double dt, bool kinetic);  // also does a deferred reciprocal finish
+int freeze_atoms(Ctx *c);                          // zero force and velocity of frozen atoms
 double recip_scale(const Ctx *c);                   // the SPME energy scale (ewald_spole.F90:429)
 
 // Influence function of the SPME convolution over the half spectrum

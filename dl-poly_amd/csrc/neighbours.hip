@@ -635,15 +635,17 @@ __global__ void k_max_disp(const double4 *pos, const double4 *xref, int n, CellG
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(r2));
 }
 
-// Excluded pairs leave the list, as link_cell_pairs moves every j found in
-// list_excl(:,i) past list(0,i) so that the two-body loop never sees it
-// (neighbours.F90:1229-1250).  Thread t = i*G + g compacts lane g of sorted
-// atom i in place, keeping the order of what stays; flags[3] (the entry
-// count) loses what was dropped.
+// Frozen-frozen and excluded pairs leave the list, as link_cell_pairs moves
+// them past list(0,i) so that the two-body loop never sees them
+// (neighbours.F90:1198-1250: a pair of two frozen atoms, lfrzn /= 0, and
+// every j found in list_excl(:,i)).  Thread t = i*G + g compacts lane g of
+// sorted atom i in place, keeping the order of what stays; flags[3] (the
+// entry count) loses what was dropped.  excl_off / frz may be null.
 __global__ void __launch_bounds__(256) k_list_exclude(int n, int cap_lane, int32_t *__restrict__ nbr,
                                                       int32_t *__restrict__ cnt, const int32_t *__restrict__ gid,
                                                       const int32_t *__restrict__ excl_off,
-                                                      const int32_t *__restrict__ excl_ids, int32_t *flags)
+                                                      const int32_t *__restrict__ excl_ids,
+                                                      const int32_t *__restrict__ frz, int32_t *flags)
 {
     constexpr int G = kLanesPerAtom;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -651,14 +653,15 @@ __global__ void __launch_bounds__(256) k_list_exclude(int n, int cap_lane, int32
     int dropped = 0;
     if (i < n) {
         const int h = gid[i];
-        const int e0 = excl_off[h], e1 = excl_off[h + 1];
-        if (e1 > e0) {
+        const int e0 = excl_off ? excl_off[h] : 0, e1 = excl_off ? excl_off[h + 1] : 0;
+        const bool fi = frz && frz[h] != 0;
+        if (e1 > e0 || fi) {
             const int m = min(cnt[t], cap_lane);
             int w = 0;
             for (int k = 0; k < m; k++) {
                 const int j = nbr[list_slot(i, k, g, cap_lane)];
                 const int hj = gid[j];
-                bool ex = false;
+                bool ex = fi && frz[hj] != 0;
                 for (int e = e0; e < e1; e++) ex = ex || excl_ids[e] == hj;
                 if (!ex) nbr[list_slot(i, w++, g, cap_lane)] = j;
             }
@@ -698,6 +701,9 @@ int build_list(Ctx *c)
     const int n = c->natms;
     if (c->excl_total > 0 && c->excl_natms != n)
         return fail(c, DLPX_ERR_STATE, "the exclusion lists were given for " + std::to_string(c->excl_natms) +
+                                           " atoms, the configuration has " + std::to_string(n));
+    if (c->nfrozen > 0 && c->frz_natms != n)
+        return fail(c, DLPX_ERR_STATE, "the frozen flags were given for " + std::to_string(c->frz_natms) +
                                            " atoms, the configuration has " + std::to_string(n));
     const double rc_ext = c->prm.cutoff + c->prm.padding;
     const double cut = rc_ext + 1.0e-6;  // neighbours.F90:415 (smalldr)
@@ -762,9 +768,10 @@ int build_list(Ctx *c)
         else
             hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, c->cap,
                                g, c->cell_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
-        if (c->excl_total > 0)
+        if (c->excl_total > 0 || c->nfrozen > 1)
             hipLaunchKernelGGL(k_list_exclude, dim3(ceil_div((long)n * G, 256)), dim3(256), 0, s, n, c->cap_lane,
-                               c->nbr, c->cnt, c->gid, c->excl_off, c->excl_ids, c->flags);
+                               c->nbr, c->cnt, c->gid, c->excl_total > 0 ? c->excl_off : nullptr, c->excl_ids,
+                               c->nfrozen > 1 ? c->frz : nullptr, c->flags);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
         DLPX_HIP(c, hipStreamSynchronize(s));

@@ -73,10 +73,14 @@ __global__ void __launch_bounds__(kVv1Threads) k_vv1(double4 *pos, double4 *vel,
 // Stage 2: v += dt/2 F/m, and sum m v^2 per block.  With FIN the
 // reciprocal forces are first added to the total force as k_finish does:
 // frc += 2 scale (frc_recip - mean frc_recip) (ewald_general.F90:857-864).
+// Frozen atoms (frz[gid[i]] /= 0; frz may be null) get zero force and
+// velocity, as freeze_atoms does after each force evaluation
+// (configuration.F90:3240-3258), so they neither move nor add kinetic energy.
 template <bool FIN>
 __global__ void __launch_bounds__(256) k_vv2(double4 *vel, double4 *frc, int n, double dt, double *red,
                                              const double4 *__restrict__ frc_recip, double two_scale,
-                                             const double *__restrict__ red_out, double inv_n)
+                                             const double *__restrict__ red_out, double inv_n,
+                                             const int32_t *__restrict__ gid, const int32_t *__restrict__ frz)
 {
     __shared__ double lred[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -92,6 +96,12 @@ __global__ void __launch_bounds__(256) k_vv2(double4 *vel, double4 *frc, int n, 
             f.z += (r.z - mz) * two_scale;
             frc[i] = f;
         }
+        if (frz && frz[gid[i]] != 0) {
+            f.x = f.y = f.z = 0.0;
+            v.x = v.y = v.z = 0.0;
+            frc[i] = f;
+            vel[i] = v;
+        }
         if (v.w > 1.0e-6) {
             const double tmp = (0.5 * dt) / v.w;
             v.x += tmp * f.x;
@@ -102,6 +112,29 @@ __global__ void __launch_bounds__(256) k_vv2(double4 *vel, double4 *frc, int n, 
         }
     }
     block_sum<1>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_KIN);
+}
+
+// freeze_atoms (configuration.F90:3240-3258): zero force and velocity of
+// frozen atoms (frz[gid[i]] /= 0)
+__global__ void k_freeze(double4 *vel, double4 *frc, int n, const int32_t *__restrict__ gid,
+                         const int32_t *__restrict__ frz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || frz[gid[i]] == 0) return;
+    double4 v = vel[i], f = frc[i];
+    v.x = v.y = v.z = 0.0;
+    f.x = f.y = f.z = 0.0;
+    vel[i] = v;
+    frc[i] = f;
+}
+
+int freeze_atoms(Ctx *c)
+{
+    if (c->nfrozen == 0) return 0;
+    hipLaunchKernelGGL(k_freeze, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, c->vel, c->frc, c->natms,
+                       c->gid, c->frz);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
 }
 
 // host-order staging (8 doubles per atom: x y z q vx vy vz m) -> cell order
@@ -168,12 +201,13 @@ int nve_second(Ctx *c, double dt, bool /*kinetic*/)
     const int nb = ceil_div(c->natms, 256);
     int r;
     if ((r = ensure_red_rows(c, nb))) return r;
+    const int32_t *frz = c->nfrozen > 0 ? c->frz : nullptr;
     if (c->finish_in_vv2)
         hipLaunchKernelGGL(k_vv2<true>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
-                           c->frc_recip, 2.0 * recip_scale(c), c->red_out, 1.0 / (double)c->natms);
+                           c->frc_recip, 2.0 * recip_scale(c), c->red_out, 1.0 / (double)c->natms, c->gid, frz);
     else
         hipLaunchKernelGGL(k_vv2<false>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
-                           nullptr, 0.0, nullptr, 0.0);
+                           nullptr, 0.0, nullptr, 0.0, c->gid, frz);
     c->finish_in_vv2 = false;
     c->red_rows[R_KIN] = nb;
     DLPX_HIP(c, hipGetLastError());

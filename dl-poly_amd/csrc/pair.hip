@@ -148,7 +148,10 @@ constexpr int kPairThreads = 512;
 // side that passes the halo tie-break (vdw.F90:1946-1966), the vdW energy of
 // a pair across the cell boundary (nonzero minimum-image shift) goes only to
 // the atom with the smaller global id (gid, the host index).
-template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false>
+//
+// GT: the vdW tables are read from global memory (through the caches) instead
+// of LDS, for pair-key sets whose tables do not fit next to the erfc table.
+template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false, bool GT = false>
 __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
                                               int n, int cap, const int32_t *__restrict__ nbr,
                                               const int32_t *__restrict__ cnt, PairParams pp,
@@ -160,18 +163,19 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
 {
     using R2 = typename R2T<R>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS: vdW tables (none in direct mode), erfc table, key table, direct
-    // forms (8 per key), reduction scratch
-    R2 *tv = (R2 *)smem;
-    R2 *te = tv + (DIRECT ? 0 : pp.nkeys * pp.mg1);
+    // LDS: vdW tables (none in direct mode or with GT), erfc table, key table,
+    // direct forms (8 per table row), reduction scratch
+    R2 *tvl = (R2 *)smem;
+    const R2 *tv = GT ? vdw_g : tvl;
+    R2 *te = tvl + ((DIRECT || GT) ? 0 : pp.nkeys * pp.mg1);
     int32_t *kt = (int32_t *)(te + (pp.electro ? pp.ns1 : 0));
     R *dv = (R *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
     double *lred = (double *)(dv + (DIRECT ? 8 * pp.nkeys : 0));
 
     if (DIRECT)
         for (int k = threadIdx.x; k < 8 * pp.nkeys; k += blockDim.x) dv[k] = direct_g[k];
-    else
-        for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tv[k] = vdw_g[k];
+    else if (!GT)
+        for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tvl[k] = vdw_g[k];
     if (pp.electro)
         for (int k = threadIdx.x; k < pp.ns1; k += blockDim.x) te[k] = erfc_g[k];
     for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = keytab_g[k];
@@ -305,7 +309,7 @@ static int launch_pair(Ctx *c)
     using R2 = typename R2T<R>::type;
     PairParams pp;
     pp.ntypes = c->prm.ntypes;
-    pp.nkeys = c->nkeys;
+    pp.nkeys = c->nrows;  // distinct table rows (the key table maps type pairs to them)
     pp.mg1 = c->max_grid + 1;
     pp.ns1 = c->nsamples + 1;
     for (int d = 0; d < 3; d++) {
@@ -323,20 +327,38 @@ static int launch_pair(Ctx *c)
     pp.r_rvdw = 1.0 / c->prm.vdw_cutoff;
     pp.cap_lane = c->cap_lane;
     const bool direct = c->vdw_direct;
-    const size_t lds = sizeof(R2) * ((direct ? 0 : (size_t)pp.nkeys * pp.mg1) + (pp.electro ? pp.ns1 : 0)) +
-                       sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
-                       (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
+    const size_t lds_rest = sizeof(R2) * (pp.electro ? pp.ns1 : 0) + sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
+                            (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
+    // the vdW tables go to LDS when they fit in its 160 KiB with the rest,
+    // else they are read from global memory
+    const size_t lds_tab = direct ? 0 : sizeof(R2) * (size_t)pp.nkeys * pp.mg1;
+    const bool gtab = !direct && lds_tab + lds_rest > 160 * 1024;
+    const size_t lds = (gtab ? 0 : lds_tab) + lds_rest;
     if (lds > 160 * 1024)
-        return fail(c, DLPX_ERR_UNSUPPORTED, "interpolation tables exceed the 160 KiB LDS of a CU");
-#define DLPX_PAIR_KP(GP, FS, DIR, PPK)                                                                       \
-    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK>));                                              \
-    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR, PPK>), dim3(pair_blocks(c)), dim3(kPairThreads),  \
-                       lds, c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp,                   \
+        return fail(c, DLPX_ERR_UNSUPPORTED, "the erfc and key tables exceed the 160 KiB LDS of a CU");
+    if (gtab && c->pair_lanes != 8)
+        return fail(c, DLPX_ERR_UNSUPPORTED, "vdW tables beyond the LDS run with 8 pair lanes");
+#define DLPX_PAIR_KG(GP, FS, DIR, PPK, GTK)                                                                  \
+    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                                         \
+    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>), dim3(pair_blocks(c)),                 \
+                       dim3(kPairThreads), lds, c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp,\
                        (const R2 *)c->d_vdw, (const R2 *)c->d_erfc, c->d_keytab, (const R *)c->d_vdw_direct,    \
                        c->frc, c->red, c->gid, c->pp)
+#define DLPX_PAIR_KP(GP, FS, DIR, PPK) DLPX_PAIR_KG(GP, FS, DIR, PPK, false)
 #define DLPX_PAIR_K(GP, FS, DIR) DLPX_PAIR_KP(GP, FS, DIR, false)
 #define DLPX_PAIR(GP, FS) DLPX_PAIR_K(GP, FS, false)
-    if (c->pp_now) {
+    if (gtab) {
+        // tabulated vdW from global memory, 8 lanes per atom
+        if (c->pp_now && c->vdw_force_shift) {
+            DLPX_PAIR_KG(8, true, false, true, true);
+        } else if (c->pp_now) {
+            DLPX_PAIR_KG(8, false, false, true, true);
+        } else if (c->vdw_force_shift) {
+            DLPX_PAIR_KG(8, true, false, false, true);
+        } else {
+            DLPX_PAIR_KG(8, false, false, false, true);
+        }
+    } else if (c->pp_now) {
         // per-particle collection at the default 8 lanes per atom only
         if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "per-particle output runs with 8 pair lanes");
         if (direct && c->vdw_force_shift) {
@@ -372,6 +394,7 @@ static int launch_pair(Ctx *c)
 #undef DLPX_PAIR
 #undef DLPX_PAIR_K
 #undef DLPX_PAIR_KP
+#undef DLPX_PAIR_KG
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

@@ -200,7 +200,7 @@ void dcell(const double *a, double *b)
 
 // vdw.F90:617-967 (the closed-form tails of the generated forms)
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
-             double volm, double *elrc, double *vlrc)
+             double volm, double *elrc, double *vlrc, const double *num_frz)
 {
     double plrc = 0.0, e = 0.0;
     int key = 0;
@@ -257,7 +257,9 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                 eadd *= 2.0;
                 padd *= 2.0;
             }
-            const double denprd = 2.0 * kPi * (num_type[i - 1] * num_type[j - 1]) / (volm * volm);
+            // frozen-frozen pairs carry no correction (vdw.F90:665-670,953)
+            const double frz = num_frz ? num_frz[i - 1] * num_frz[j - 1] : 0.0;
+            const double denprd = 2.0 * kPi * (num_type[i - 1] * num_type[j - 1] - frz) / (volm * volm);
             e += volm * denprd * eadd;
             plrc += denprd * padd / 3.0;
         }

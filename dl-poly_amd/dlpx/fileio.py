@@ -71,6 +71,18 @@ def _cubic(cell):
     return np.allclose(c, np.diag(np.diag(c))) and c[0, 0] == c[1, 1] == c[2, 2]
 
 
+def _site_frozen(sysm, start, nummols, nsites, name):
+    """Frozen flags of the sites of a molecule type whose nummols copies start
+    at atom ``start`` (a FIELD gives one flag per site, shared by all copies)."""
+    frz = getattr(sysm, "frozen", None)
+    if frz is None:
+        return [False] * nsites
+    f = np.asarray(frz, bool)[start:start + nummols * nsites].reshape(max(nummols, 1), nsites)
+    if not np.all(f == f[0]):
+        raise ValueError("molecule type %r: copies differ in which sites are frozen" % name)
+    return [bool(x) for x in f[0]]
+
+
 def write_field(path, sysm):
     """Without ``sysm.molecules``: one single-atom molecule type per atom type,
     in type order (the CONFIG lists atoms grouped by type in the same order).
@@ -87,8 +99,10 @@ def write_field(path, sysm):
                 raise ValueError("atoms do not follow molecule type %r's site order" % m.name)
             base += m.nummols * len(m.sites)
             out += [m.name, "nummols %d" % m.nummols, "atoms %d" % len(m.sites)]
-            for t in m.sites:
-                out.append("%-8s %s %s" % (sysm.type_names[t], _num(sysm.masses[t]), _num(sysm.charges[t])))
+            fz = _site_frozen(sysm, base - m.nummols * len(m.sites), m.nummols, len(m.sites), m.name)
+            for t, f in zip(m.sites, fz):
+                out.append("%-8s %s %s" % (sysm.type_names[t], _num(sysm.masses[t]), _num(sysm.charges[t])) +
+                           (" 1 1" if f else ""))
             if m.bonds:
                 out.append("bonds %d" % len(m.bonds))
                 out += ["harm %d %d 0.0 1.0" % (a + 1, b + 1) for (a, b) in m.bonds]
@@ -104,9 +118,13 @@ def write_field(path, sysm):
         if not order_ok:
             raise ValueError("atoms must be grouped by type for the FIELD molecule layout")
         out = [sysm.name, "units internal", "molecules %d" % sysm.ntypes]
+        base = 0
         for t, nm in enumerate(sysm.type_names):
+            fz = _site_frozen(sysm, base, int(counts[t]), 1, nm)
+            base += int(counts[t])
             out += [nm, "nummols %d" % counts[t], "atoms 1",
-                    "%-8s %s %s" % (nm, _num(sysm.masses[t]), _num(sysm.charges[t])), "finish"]
+                    "%-8s %s %s" % (nm, _num(sysm.masses[t]), _num(sysm.charges[t])) + (" 1 1" if fz[0] else ""),
+                    "finish"]
     out.append("vdw %d" % len(sysm.vdw))
     for (ti, tj, key, params) in sysm.vdw:
         out.append("%-8s %-8s %4s %s" % (sysm.type_names[ti], sysm.type_names[tj], VDW_NAMES[key],
@@ -304,13 +322,15 @@ def _bonded_block(lines, i, count, natoms, kind, mname):
 
 def read_field(path):
     """FIELD (ffield.F90:150-5365), the subset of this path: energy units;
-    molecule types of atoms (``name mass charge [repeat [frozen]]``) with
+    molecule types of atoms (``name mass charge [repeat [frozen]]``; frozen
+    atoms, config%lfrzn, are kept in place and pairs of them leave the list) with
     zero-strength harmonic ``bonds`` and ``angles`` that only exclude pairs;
     the vdW forms the library generates (``12-6``, ``lj``, ``nm``, ``buck``,
-    ``bhm``, ``hbnd``, ``mors``).  Bonded forces, constraints, frozen atoms,
-    shells, other vdW forms and many-body terms raise ValueError.
+    ``bhm``, ``hbnd``, ``mors``).  Bonded forces, constraints, shells, other
+    vdW forms and many-body terms raise ValueError.
     Returns dict(title, units, molecules [(name, nummols, [(atom, mass, charge)])],
-    topology [(bonds, angles) per molecule type, 0-based sites],
+    topology [(bonds, angles) per molecule type, 0-based sites], frozen [per
+    molecule type, a flag per site],
     vdw [(atom1, atom2, key, params in internal units)])."""
     with open(path) as f:
         raw = [ln.split("#")[0].rstrip() for ln in f.read().splitlines()]
@@ -318,7 +338,7 @@ def read_field(path):
     lines = [ln for ln in raw[1:] if ln.strip()]
     units = "internal"
     unit = 1.0
-    molecules, vdw, topology = [], [], []
+    molecules, vdw, topology, frozen = [], [], [], []
     i = 0
 
     def words(ln):
@@ -346,7 +366,7 @@ def read_field(path):
             for _ in range(nmol):
                 mname = lines[i].strip()
                 i += 1
-                nummols, atoms, bonds, angles = None, [], [], []
+                nummols, atoms, bonds, angles, frz = None, [], [], [], []
                 while True:
                     w = words(lines[i])
                     k2 = w[0].lower()
@@ -360,11 +380,10 @@ def read_field(path):
                         while got < na:
                             a = words(lines[i])
                             rep = int(a[3]) if len(a) > 3 else 1
-                            frozen = int(a[4]) if len(a) > 4 else 0
-                            if frozen:
-                                raise ValueError("frozen atoms are not supported")
+                            fflag = int(a[4]) if len(a) > 4 else 0
                             for _r in range(max(rep, 1)):
                                 atoms.append((a[0], float(a[1]), float(a[2])))
+                                frz.append(fflag != 0)
                             got += max(rep, 1)
                             i += 1
                     elif k2 == "bonds":
@@ -384,6 +403,7 @@ def read_field(path):
                                                                                              len(atoms)))
                 molecules.append((mname, nummols, atoms))
                 topology.append((bonds, angles))
+                frozen.append(frz)
         elif key == "vdw":
             nv = int(w[-1])
             i += 1
@@ -402,7 +422,7 @@ def read_field(path):
             break
         else:
             raise ValueError("FIELD directive %r is not supported by this path" % w[0])
-    return dict(title=title, units=units, molecules=molecules, topology=topology, vdw=vdw)
+    return dict(title=title, units=units, molecules=molecules, topology=topology, frozen=frozen, vdw=vdw)
 
 
 # --------------------------------------------------------------- CONTROL --
@@ -521,6 +541,11 @@ def system_from_files(directory):
     kw = {}
     if spme:
         kw = dict(alpha=ctl["alpha"], kvec=ctl["kvec"], nsplines=ctl["nsplines"])
+    site_frozen = []
+    for (_m, nummols, atoms), fz in zip(fld["molecules"], fld["frozen"]):
+        site_frozen += list(fz) * nummols
+    if any(site_frozen):
+        kw["frozen"] = np.array(site_frozen, bool)
     if any(b or a for (b, a) in fld["topology"]):
         from .systems import Molecule
         kw["molecules"] = [Molecule(name=m, nummols=nm, sites=[type_names.index(x[0]) for x in atoms],

@@ -24,7 +24,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
            "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
-           "dlpx_set_per_particle", "dlpx_add_per_particle",
+           "dlpx_set_per_particle", "dlpx_add_per_particle", "dlpx_set_frozen", "dlpx_compute_lrc_frozen",
            "dlpx_set_timing", "dlpx_get_timing"]
 
 
@@ -105,6 +105,10 @@ class Context:
         excl = sysm.exclusions() if hasattr(sysm, "exclusions") else None
         if excl is not None:
             self.set_exclusions(excl)
+        frz = getattr(sysm, "frozen", None)
+        if frz is not None and np.any(frz):
+            f = np.ascontiguousarray(np.asarray(frz, bool).astype(np.int32))
+            self._chk(self.lib.dlpx_set_frozen(self.h, C.c_int32(len(f)), _ptr(f)))
 
     def set_exclusions(self, excl):
         """Excluded pairs: one sequence of 0-based partner indices per atom
@@ -136,7 +140,12 @@ class Context:
         if s.electrostatics:
             self._chk(self.lib.dlpx_generate_erfc_tables(self.h))
         num = np.bincount(s.types, minlength=s.ntypes).astype(np.float64)
-        self._chk(self.lib.dlpx_compute_lrc(self.h, _ptr(self.cell), _ptr(num)))
+        frz = getattr(s, "frozen", None)
+        if frz is not None and np.any(frz):
+            nfz = np.bincount(s.types[np.asarray(frz, bool)], minlength=s.ntypes).astype(np.float64)
+            self._chk(self.lib.dlpx_compute_lrc_frozen(self.h, _ptr(self.cell), _ptr(num), _ptr(nfz)))
+        else:
+            self._chk(self.lib.dlpx_compute_lrc(self.h, _ptr(self.cell), _ptr(num)))
         if getattr(s, "vdw_direct", False):
             self._chk(self.lib.dlpx_set_vdw_direct(self.h, C.c_int32(1)))
 

@@ -27,14 +27,36 @@ def _cell_angles(cell):
     return (np.degrees(np.arccos(cos(b, c))), np.degrees(np.arccos(cos(a, c))), np.degrees(np.arccos(cos(a, b))))
 
 
+def start_velocities(s):
+    """Velocities as DL_POLY starts the run with them (set_temperature,
+    temperature.F90:579-620, with the default fixed_com): frozen atoms at rest
+    (freeze_atoms), and the centre-of-mass velocity of the other atoms
+    (getvom, kinetics.F90:365-419) subtracted from them."""
+    v = np.array(s.vel, dtype=np.float64, copy=True)
+    m = s.mass
+    frz = getattr(s, "frozen", None)
+    frz = np.asarray(frz, bool) if frz is not None else np.zeros(s.natms, bool)
+    v[frz] = 0.0
+    free = ~frz & (m > 1.0e-6)
+    tot = float(m[~frz].sum())
+    if tot > 0.0:
+        vom = (m[~frz, None] * v[~frz]).sum(0) / tot
+        v[free] -= vom
+    return v
+
+
 def statis_values(s, out, vel, msd_by_type, engunit=1.0):
     """STATIS columns of one step (statistics.F90:1118-1160, 1337 order):
     stpval(1:27), stpval(0), the pressure tensor (9) and the mean squared
     displacement per atom type.  ``out`` is a two-body output dict (engcpe,
     vircpe, engsrp, virsrp, stress); ``vel`` the velocities of the step."""
     m = s.mass
+    frz = getattr(s, "frozen", None)
+    nfrz = int(np.count_nonzero(frz)) if frz is not None else 0
+    if nfrz:
+        vel = np.where(np.asarray(frz, bool)[:, None], 0.0, vel)  # freeze_atoms (configuration.F90:3240-3258)
     engke = 0.5 * float(np.sum(m[:, None] * vel * vel))
-    degfre = 3 * s.natms - 3                                    # temperature.F90:143 (COM removed)
+    degfre = 3 * s.natms - 3 - 3 * nfrz                         # temperature.F90:110-143 (COM, frozen atoms)
     temp = 2.0 * engke / (fileio.BOLTZ * degfre)
     engcfg = out["engcpe"] + out["engsrp"]
     stpeng = engcfg + engke
@@ -73,8 +95,9 @@ def run_directory(directory, out_dir=None, precision=lib.FP64, steps=None):
     counts = np.bincount(s.types, minlength=ntypes).astype(np.float64)
 
     ctx = lib.Context(s, precision=precision)
+    v0 = start_velocities(s)
     try:
-        ctx.md_load(s.pos, s.vel)
+        ctx.md_load(s.pos, v0)
         # mean squared displacement as the reference accumulates it for real
         # dynamics (statistics.F90:1162-1176, 1240-1287): xto += v(t_n) dt at
         # every step n >= 1 with the step's full-step velocities, so the run
@@ -85,8 +108,8 @@ def run_directory(directory, out_dir=None, precision=lib.FP64, steps=None):
         with open(statis_path, "w") as f:
             f.write(fileio.statis_header(s.name, units_name))
             out = ctx.md_last_output()
-            f.write(fileio.statis_record(0, 0.0, statis_values(s, out, s.vel, np.zeros(ntypes), engunit)))
-            x, v, frc = s.pos, s.vel, None
+            f.write(fileio.statis_record(0, 0.0, statis_values(s, out, v0, np.zeros(ntypes), engunit)))
+            x, v, frc = s.pos, v0, None
             for step in range(1, nsteps + 1):
                 _, nreb = ctx.md_run(1, s.timestep, energies=True)
                 rebuilds += nreb

@@ -84,6 +84,7 @@ class System:
     vdw_direct: bool = False         # vdws%l_direct (CONTROL vdw_method direct)
     meta: dict = field(default_factory=dict)
     molecules: Optional[List[Molecule]] = None  # None: one single-atom molecule type per atom type
+    frozen: Optional[np.ndarray] = None  # (N,) bool, config%lfrzn (FIELD frozen flag); None: none
 
     @property
     def natms(self):
@@ -319,6 +320,35 @@ def molecular(ncells=6, a=4.93, temperature=300.0, seed=20261202, drude=False, k
                   padding=padding, vdw_cutoff=cutoff, electrostatics=True, alpha=alpha, kvec=tuple(kvec),
                   nsplines=8, timestep=timestep, temperature=temperature,
                   meta=dict(seed=seed, a=a, ncells=ncells, drude=drude), molecules=[mol])
+
+
+def frozen_slab(s, zmax):
+    """The same configuration with the atoms below z = zmax frozen
+    (config%lfrzn): each frozen species becomes its own atom type (name +
+    "f", same mass, charge and vdW parameters), since a FIELD flags frozen
+    sites per molecule type, and the atoms are regrouped by type."""
+    if s.molecules:
+        raise ValueError("frozen_slab works on single-atom molecule layouts")
+    nt = s.ntypes
+    frz = s.pos[:, 2] < zmax
+    types = np.where(frz, s.types + nt, s.types).astype(np.int32)
+    order = np.argsort(types, kind="stable")
+    species = list(range(nt)) * 2                      # type -> original species
+    par = {(max(a, b), min(a, b)): (k, p) for (a, b, k, p) in s.vdw}
+    vdw = []
+    for i in range(2 * nt):
+        for j in range(i + 1):
+            key = (max(species[i], species[j]), min(species[i], species[j]))
+            if key in par:
+                vdw.append((i, j, par[key][0], par[key][1]))
+    return System(name=s.name + "_frz", cell=s.cell.copy(), pos=s.pos[order], vel=s.vel[order], types=types[order],
+                  type_names=list(s.type_names) + [n + "f" for n in s.type_names],
+                  masses=np.concatenate([s.masses, s.masses]), charges=np.concatenate([s.charges, s.charges]),
+                  vdw=vdw, cutoff=s.cutoff, padding=s.padding, vdw_cutoff=s.vdw_cutoff,
+                  electrostatics=s.electrostatics, alpha=s.alpha, kvec=s.kvec, nsplines=s.nsplines,
+                  epsilon=s.epsilon, timestep=s.timestep, temperature=s.temperature,
+                  vdw_force_shift=s.vdw_force_shift, vdw_direct=s.vdw_direct, meta=dict(s.meta, frozen_below=zmax),
+                  frozen=frz[order])
 
 
 def molecules_to_json(mols):

@@ -50,7 +50,7 @@ Module dlpx_binding
             dlpx_get_lrc, dlpx_link_cell_pairs, dlpx_vnl_check, dlpx_two_body_forces, dlpx_md_load, &
             dlpx_md_run, dlpx_md_store, dlpx_md_run_ensemble, dlpx_md_last_output, &
             dlpx_set_vdw_force_shift, dlpx_set_vdw_direct, dlpx_set_exclusions, dlpx_set_per_particle, &
-            dlpx_add_per_particle, dlpx_error_message
+            dlpx_add_per_particle, dlpx_set_frozen, dlpx_compute_lrc_frozen, dlpx_error_message
 
   Interface
     Integer(c_int) Function dlpx_create(params, ctx) Bind(C, name='dlpx_create')
@@ -215,6 +215,22 @@ Module dlpx_binding
       Integer(c_int32_t), Value :: natms, mxexcl
       Integer(c_int32_t), Intent(In) :: list_excl(0:mxexcl, *)
     End Function dlpx_set_exclusions
+
+    !> frozen atoms: pass config%lfrzn(1:natms)
+    Integer(c_int) Function dlpx_set_frozen(ctx, natms, lfrzn) Bind(C, name='dlpx_set_frozen')
+      Import :: c_int, c_int32_t, c_ptr
+      Type(c_ptr), Value :: ctx
+      Integer(c_int32_t), Value :: natms
+      Integer(c_int32_t), Intent(In) :: lfrzn(*)
+    End Function dlpx_set_frozen
+
+    !> dlpx_compute_lrc with the number of frozen atoms of each type (vdw.F90:665-670)
+    Integer(c_int) Function dlpx_compute_lrc_frozen(ctx, cell, num_type, num_frozen) &
+        Bind(C, name='dlpx_compute_lrc_frozen')
+      Import :: c_int, c_ptr, c_double
+      Type(c_ptr), Value :: ctx
+      Real(c_double), Intent(In) :: cell(9), num_type(*), num_frozen(*)
+    End Function dlpx_compute_lrc_frozen
 
     !> stats%collect_pp: on = 0 or 1 for the following dlpx_two_body_forces calls
     Integer(c_int) Function dlpx_set_per_particle(ctx, on) Bind(C, name='dlpx_set_per_particle')

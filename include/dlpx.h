@@ -135,6 +135,20 @@ int dlpx_get_lrc(dlpx_ctx *ctx, double *elrc, double *vlrc);
  * Call before dlpx_link_cell_pairs / dlpx_md_load; natms = 0 clears them. */
 int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int32_t *list_excl);
 
+/* Frozen atoms, config%lfrzn (nonzero = frozen; 0-based host order, natms
+ * values).  A pair of two frozen atoms leaves the Verlet list
+ * (neighbours.F90:1198-1225), so it has no vdW or real-space Coulomb term;
+ * the reciprocal sum keeps it (DL_POLY 5.1 calls no frozen-pair Ewald
+ * correction).  In resident runs (dlpx_md_load / dlpx_md_run) frozen atoms
+ * get zero force and velocity after each force evaluation, as freeze_atoms
+ * does (configuration.F90:3240-3258); dlpx_two_body_forces leaves that to
+ * the host.  Call before dlpx_link_cell_pairs / dlpx_md_load; natms = 0
+ * clears them. */
+int dlpx_set_frozen(dlpx_ctx *ctx, int32_t natms, const int32_t *lfrzn);
+/* dlpx_compute_lrc with num_frozen[type] frozen atoms of each type: the
+ * frozen-frozen pairs carry no correction (vdw.F90:665-670,953). */
+int dlpx_compute_lrc_frozen(dlpx_ctx *ctx, const double cell[9], const double *num_type, const double *num_frozen);
+
 /* Per-particle energies and stresses, stats%collect_pp (statistics.F90:368-
  * 402; used for HEATFLUX and per-particle output, drivers.F90:2117-2165).
  * dlpx_set_per_particle(ctx, 1) makes each following dlpx_two_body_forces

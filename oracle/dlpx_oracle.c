@@ -208,7 +208,7 @@ double orc_interp3(const double *tab0, double rdr, double r)
 /* vdw.F90:617-967 for the forms of orc_vdw_eval: energy and virial long-range corrections.
  * kinds/params are indexed by pair key (0-based key = hi*(hi-1)/2+lo-1, 1-based types). */
 void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*5 per key*/,
-                 double rvdw, double volm, double *elrc, double *vlrc)
+                 double rvdw, double volm, double *elrc, double *vlrc, const double *num_frz /* NULL: none */)
 {
     const double twopi = 2.0 * ORC_PI;
     double plrc = 0.0, e = 0.0;
@@ -247,7 +247,9 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
                        e0 * t * t / (4.0 * k3) * (4.0 * pow(kk, 3) * pow(r, 3) + 6 * kk * kk * r * r + 6 * kk * r + 3);
             }
             if (i != j) { eadd *= 2.0; padd *= 2.0; }
-            double denprd = twopi * (num_type[i - 1] * num_type[j - 1]) / (volm * volm);
+            /* vdw.F90:953: frozen-frozen pairs carry no correction */
+            double frz = num_frz ? num_frz[i - 1] * num_frz[j - 1] : 0.0;
+            double denprd = twopi * (num_type[i - 1] * num_type[j - 1] - frz) / (volm * volm);
             e += volm * denprd * eadd;
             plrc += denprd * padd / 3.0;
             ivdw++;
@@ -538,6 +540,7 @@ typedef struct {
     const double *vdw_param; /* 5 per key, for vdw_direct */
     const int32_t *excl_off; /* excluded pairs (neigh%list_excl) as CSR over atoms: N+1 row starts, */
     const int32_t *excl_ids; /* then 0-based partners, both ends listed; NULL = none */
+    const int32_t *frozen;   /* per atom, nonzero = frozen (config%lfrzn); NULL = none */
 } orc_input;
 
 typedef struct {
@@ -879,8 +882,9 @@ int orc_two_body(const orc_input *in, orc_output *out)
     long nexcl_listed = 0;
     for (long p = 0; p < np; p++) {
         int i = pairs[5 * p], j = pairs[5 * p + 1];
-        /* neighbours.F90:1229-1250: excluded pairs sit past list(0,i), out of the two-body loop */
-        if (excluded(in, i, j)) { nexcl_listed++; continue; }
+        /* neighbours.F90:1198-1250: frozen-frozen and excluded pairs sit past
+         * list(0,i), out of the two-body loop */
+        if ((in->frozen && in->frozen[i] && in->frozen[j]) || excluded(in, i, j)) { nexcl_listed++; continue; }
         double xx = in->xyz[3 * i] - (in->xyz[3 * j] + pairs[5 * p + 2] * L[0]);
         double yy = in->xyz[3 * i + 1] - (in->xyz[3 * j + 1] + pairs[5 * p + 3] * L[1]);
         double zz = in->xyz[3 * i + 2] - (in->xyz[3 * j + 2] + pairs[5 * p + 4] * L[2]);

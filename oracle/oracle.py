@@ -55,7 +55,8 @@ class OrcInput(C.Structure):
                 ("erfc_d", C.POINTER(C.c_double)),
                 ("elrc", C.c_double), ("vlrc", C.c_double), ("vdw_fs", C.c_int), ("vdw_direct", C.c_int),
                 ("vdw_kind", C.POINTER(C.c_int)), ("vdw_param", C.POINTER(C.c_double)),
-                ("excl_off", C.POINTER(C.c_int32)), ("excl_ids", C.POINTER(C.c_int32))]
+                ("excl_off", C.POINTER(C.c_int32)), ("excl_ids", C.POINTER(C.c_int32)),
+                ("frozen", C.POINTER(C.c_int32))]
 
 
 class OrcOutput(C.Structure):
@@ -110,9 +111,14 @@ def vdw_lrc(sysm, from_key):
     for k, (kind, p) in from_key.items():
         kinds[k] = kind; params[k] = p
     num = np.bincount(sysm.types, minlength=sysm.ntypes).astype(np.float64)
+    frz = getattr(sysm, "frozen", None)
+    nfz = None
+    if frz is not None and np.any(frz):
+        nfz = np.bincount(sysm.types[np.asarray(frz, bool)], minlength=sysm.ntypes).astype(np.float64)
     e = C.c_double(); v = C.c_double()
     lib().orc_vdw_lrc(C.c_int(sysm.ntypes), _p(num), _p(kinds, C.c_int), _p(params),
-                      C.c_double(sysm.vdw_cutoff), C.c_double(sysm.volume), C.byref(e), C.byref(v))
+                      C.c_double(sysm.vdw_cutoff), C.c_double(sysm.volume), C.byref(e), C.byref(v),
+                      _p(nfz) if nfz is not None else None)
     return e.value, v.value
 
 
@@ -170,6 +176,10 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None, p
         ei = np.ascontiguousarray(np.concatenate(excl) if eo[-1] else np.zeros(1), dtype=np.int32)
         inp.excl_off = _p(eo, C.c_int32)
         inp.excl_ids = _p(ei, C.c_int32)
+    frz = getattr(sysm, "frozen", None)
+    if frz is not None and np.any(frz):
+        fz = np.ascontiguousarray(np.asarray(frz, bool).astype(np.int32))
+        inp.frozen = _p(fz, C.c_int32)
     f = np.zeros((sysm.natms, 3))
     out = OrcOutput(frc=_p(f))
     K = sysm.kvec
@@ -241,11 +251,28 @@ def nve(sysm, steps, pos=None, vel=None):
     x = np.ascontiguousarray(sysm.pos if pos is None else pos, dtype=np.float64).copy()
     v = np.ascontiguousarray(sysm.vel if vel is None else vel, dtype=np.float64).copy()
     m = np.ascontiguousarray(sysm.mass)
-    res = two_body(sysm, pos=x)
-    f = res["frc"].copy()
+    frz = getattr(sysm, "frozen", None)
+    frz = np.asarray(frz, bool) if frz is not None else np.zeros(sysm.natms, bool)
+    if vel is None:
+        # set_temperature (temperature.F90:579-620, getvom kinetics.F90:365-419):
+        # frozen atoms at rest, the free atoms' centre-of-mass velocity removed
+        v[frz] = 0.0
+        mf = m[~frz]
+        if mf.sum() > 0.0:
+            vom = (mf[:, None] * v[~frz]).sum(0) / mf.sum()
+            v[~frz & (m > 1.0e-6)] -= vom
+
+    def forces():
+        # freeze_atoms (configuration.F90:3240-3258) after each evaluation
+        r = two_body(sysm, pos=x)
+        fr = r["frc"].copy()
+        fr[frz] = 0.0
+        v[frz] = 0.0
+        return r, fr
+
+    res, f = forces()
     for _ in range(steps):
         L.orc_nve_first(C.c_int(sysm.natms), C.c_double(sysm.timestep), _p(m), _p(x), _p(v), _p(f))
-        res = two_body(sysm, pos=x)
-        f = res["frc"].copy()
+        res, f = forces()
         L.orc_nve_second(C.c_int(sysm.natms), C.c_double(sysm.timestep), _p(m), _p(v), _p(f))
     return x, v, f, res

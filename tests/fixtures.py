@@ -30,7 +30,8 @@ def load(name):
                        temperature=float(st[6]),
                        vdw_force_shift=bool(z["vdw_force_shift"]) if "vdw_force_shift" in z.files else False,
                        vdw_direct=bool(z["vdw_direct"]) if "vdw_direct" in z.files else False,
-                       molecules=systems.molecules_from_json(str(z["molecules"])) if "molecules" in z.files else None)
+                       molecules=systems.molecules_from_json(str(z["molecules"])) if "molecules" in z.files else None,
+                       frozen=(z["frozen"].astype(bool) if "frozen" in z.files and z["frozen"].size else None))
     ref = dict(frc=z["ref_frc"], pos=z["ref_pos"], vel=z["ref_vel"], stress=z["ref_stress"],
                stpval=z["ref_stpval"], steps=int(z["steps"]),
                engsrp=z["ref_stpval"][STP_ENGSRP], engcpe=z["ref_stpval"][STP_ENGCPE],
@@ -48,8 +49,19 @@ def printed_digits_bound(ref, sig=10):
 def heat_flux(s, pp_energy, pp_stress):
     """statistics.F90:2643-2665 (internal energy units): (sum (e_i + m v^2/2) v
     + sum S_i v) / V, S_i the 3x3 of pp_stress(:, i) (column-major)."""
-    v, m = s.vel, s.mass
+    from dlpx import runner
+    v, m = runner.start_velocities(s), s.mass  # as the reference starts the run (frozen at rest, no COM drift)
     ke = 0.5 * m * (v * v).sum(1)
     ev = ((pp_energy + ke)[:, None] * v).sum(0)
     S = pp_stress.reshape(-1, 3, 3).transpose(0, 2, 1)  # S[i, a, b] = pp_stress[i, a + 3b]
     return (ev + np.einsum("iab,ib->a", S, v)) / s.volume
+
+
+def frozen_zeroed(s, frc):
+    """Forces as the reference reports them: freeze_atoms zeroes those of
+    frozen atoms after the force evaluation (configuration.F90:3240-3258)."""
+    if s.frozen is None:
+        return frc
+    f = np.array(frc, copy=True)
+    f[s.frozen] = 0.0
+    return f

@@ -42,7 +42,8 @@ from dlpx import fileio, systems  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
 OUT = os.path.dirname(os.path.abspath(__file__))
-STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10", "mol3_2592_nve10")
+STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10", "mol3_2592_nve10",
+                "nacl_1000_frz_nve10")
 
 
 def _fs(sysm):
@@ -90,6 +91,12 @@ def cases():
         "mol3_2592_nve10": (systems.molecular(), 10),
         # with a fourth site 0.02 A from the oxygen: the correction's series branch
         "mol4_3456_k32": (systems.molecular(drude=True), 0),
+        # the ions below z = -7 A frozen (FIELD frozen flag): frozen-frozen
+        # pairs leave the list, the vdW correction drops them, frozen atoms
+        # keep zero force and velocity (neighbours.F90:1198-1225,
+        # vdw.F90:665-670, configuration.F90:3240-3258)
+        "nacl_1000_frz": (systems.frozen_slab(systems.nacl(ncells=5, kvec=(32, 32, 32)), -7.0), 0),
+        "nacl_1000_frz_nve10": (systems.frozen_slab(systems.nacl(ncells=5, kvec=(32, 32, 32)), -7.0), 10),
     }
 
 
@@ -125,6 +132,7 @@ def run_case(name, sysm, steps, statis_only=False):
         vdw_force_shift=np.int32(sysm.vdw_force_shift),
         vdw_direct=np.int32(sysm.vdw_direct),
         molecules=np.array(systems.molecules_to_json(sysm.molecules)),
+        frozen=(np.asarray(sysm.frozen, np.int8) if sysm.frozen is not None else np.zeros(0, np.int8)),
         ref_frc=rc["frc"], ref_pos=rc["pos"], ref_vel=rc["vel"], ref_stpval=rv["stpval"],
         ref_stress=rv["stress"], ref_nstep=np.int32(rv["nstep"]))
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))

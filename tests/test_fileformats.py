@@ -123,6 +123,21 @@ def test_field_zero_strength_bonds_give_exclusions():
     assert [list(e) for e in ex] == [[1, 2], [0, 2], [0, 1], [4, 5], [3, 5], [3, 4]]
 
 
+def test_frozen_flags_round_trip_and_start_velocities():
+    """FIELD frozen flags (``name mass charge repeat frozen``) per atom type,
+    and the run's starting velocities: frozen atoms at rest and no
+    centre-of-mass drift among the others (temperature.F90:579-620)."""
+    s = systems.frozen_slab(systems.nacl(ncells=3, kvec=(24, 24, 24), cutoff=8.0, padding=0.3), -2.0)
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, s, steps=2)
+        t = fileio.system_from_files(d)
+    assert np.array_equal(t.frozen, s.frozen) and t.type_names == ["Na", "Cl", "Naf", "Clf"]
+    v = runner.start_velocities(s)
+    assert np.all(v[s.frozen] == 0.0)
+    p = (s.mass[:, None] * v).sum(0)
+    assert np.abs(p).max() <= 1e-12 * (s.mass[:, None] * np.abs(v)).sum()
+
+
 def test_molecular_inputs_round_trip():
     s = systems.molecular(ncells=2, drude=True)
     with tempfile.TemporaryDirectory() as d:
@@ -187,7 +202,7 @@ def test_statis_equal_is_the_harness_rule():
 
 
 @pytest.mark.parametrize("name", ["nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10",
-                                  "mol3_2592_nve10"])
+                                  "mol3_2592_nve10", "nacl_1000_frz_nve10"])
 def test_statis_step0_quantities_match_reference_file(name):
     """The STATIS assembly of one step (temperature, enthalpy, pressure,
     pressure tensor with the kinetic stress, cell angles, energies and
@@ -196,7 +211,7 @@ def test_statis_step0_quantities_match_reference_file(name):
     s, _ = fixtures.load(name)
     ref = fileio.read_statis(os.path.join(GOLDEN, name + ".STATIS"))
     o = oracle.two_body(s)
-    vals = runner.statis_values(s, o, s.vel, np.zeros(s.ntypes))
+    vals = runner.statis_values(s, o, runner.start_velocities(s), np.zeros(s.ntypes))
     t0, want = ref[0]
     assert t0 == 0.0 and len(vals) == len(want)
     bad = [(k + 1, vals[k], want[k]) for k in range(len(want)) if not fileio.statis_equal(vals[k], want[k], 1e-5)]

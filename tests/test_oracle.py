@@ -28,7 +28,7 @@ def test_oracle_matches_reference_fixture(name):
     s, ref = fixtures.load(name)
     r = oracle.two_body(s)
     # forces: every printed digit of the reference (10 significant figures)
-    assert _force_margin(r["frc"], ref["frc"]) <= 1.01
+    assert _force_margin(fixtures.frozen_zeroed(s, r["frc"]), ref["frc"]) <= 1.01
     # energies and virials from the binary REVIVE record (full precision),
     # relative to the largest term of the sum: with exclusions engcpe is a
     # small difference of the self, reciprocal and correction energies
