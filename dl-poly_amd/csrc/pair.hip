@@ -125,6 +125,38 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
             g = (R)-2 * r * p[0] * p[2] * ((R)1 - t) * t;
             break;
         }
+        case 9:  // WCA, zero beyond cut = 2^(1/6) sig + d (:559-576)
+            if (r < p[3] || fabs(r - p[2]) < (R)1.0e-10) {
+                const R s6 = rpow(p[1] / (r - p[2]), 6);
+                e = (R)4 * p[0] * s6 * (s6 - (R)1) + p[0];
+                g = (R)24 * p[0] * s6 * ((R)2 * s6 - (R)1) * r / (r - p[2]);
+            }
+            break;
+        case 10:  // DPD, zero beyond rc (:591-610)
+            if (r < p[1]) {
+                const R x = r / p[1], h = (R)0.5 * p[0] * p[1] * ((R)1 - x);
+                e = h * ((R)1 - x);
+                g = (R)2 * h * x;
+            }
+            break;
+        case 12: {  // Lennard-Jones cohesive (:283-293)
+            const R s6 = rpow(p[1] / r, 6);
+            e = (R)4 * p[0] * s6 * (s6 - p[2]);
+            g = (R)24 * p[0] * s6 * ((R)2 * s6 - p[2]);
+            break;
+        }
+        case 13: {  // Morse + c/r^12 (:442-456)
+            const R t = exp(-p[2] * (r - p[1])), c12 = p[3] / rpow(r, 12);
+            e = p[0] * t * (t - (R)2) + c12;
+            g = (R)-2 * r * p[0] * p[2] * ((R)1 - t) * t + (R)12 * c12;
+            break;
+        }
+        case 14: {  // Rydberg (:689-702)
+            const R x = r / p[2], t = exp(-x);
+            e = (p[0] + p[1] * r) * t;
+            g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
+            break;
+        }
         default:
             break;
     }

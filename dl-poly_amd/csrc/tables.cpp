@@ -22,7 +22,16 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //   5 BHM        A exp(B (sig-r)) - C/r^6 - D/r^8        (:499-514)
 //   6 12-10      A/r^12 - B/r^10                         (:530-546)
 //   8 Morse      e0 {[1 - exp(-k (r-r0))]^2 - 1}         (:416-428)
-bool vdw_kind_supported(int kind) { return kind == 1 || kind == 2 || kind == 3 || kind == 4 || kind == 5 || kind == 6 || kind == 8; }
+//   9 WCA        4 eps [(sig/(r-d))^12 - (sig/(r-d))^6] + eps below
+//                cut = 2^(1/6) sig + d, else 0           (:559-576)
+//  10 DPD        a rc (1 - r/rc)^2 / 2 below rc          (:591-610)
+//  12 LJ cohesive 4 eps [(sig/r)^12 - c (sig/r)^6]       (:283-293)
+//  13 Morse + 12 e0 {[1 - exp(-k (r-r0))]^2 - 1} + c/r^12 (:442-456)
+//  14 Rydberg    (a + b r) exp(-r/c)                     (:689-702)
+bool vdw_kind_supported(int kind)
+{
+    return (kind >= 1 && kind <= 6) || (kind >= 8 && kind <= 10) || (kind >= 12 && kind <= 14);
+}
 
 static double ipow(double x, int n)
 {
@@ -69,6 +78,42 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
             const double t = std::exp(-p[2] * (r - p[1]));
             *e = p[0] * ((1.0 - t) * (1.0 - t) - 1.0);
             *g = -2.0 * r * p[0] * p[2] * (1.0 - t) * t;
+            break;
+        }
+        case 9: {  // params eps, sig, d, cut
+            *e = *g = 0.0;
+            if (r < p[3] || std::fabs(r - p[2]) < 1.0e-10) {
+                const double s = p[1] / (r - p[2]), s6 = ipow(s, 6);
+                *e = 4.0 * p[0] * s6 * (s6 - 1.0) + p[0];
+                *g = 24.0 * p[0] * s6 * (2.0 * s6 - 1.0) * r / (r - p[2]);
+            }
+            break;
+        }
+        case 10: {  // params a, rc
+            *e = *g = 0.0;
+            if (r < p[1]) {
+                const double x = r / p[1], h = 0.5 * p[0] * p[1] * (1.0 - x);
+                *e = h * (1.0 - x);
+                *g = 2.0 * h * x;
+            }
+            break;
+        }
+        case 12: {  // params eps, sig, c
+            const double s6 = ipow(p[1] / r, 6);
+            *e = 4.0 * p[0] * s6 * (s6 - p[2]);
+            *g = 24.0 * p[0] * s6 * (2.0 * s6 - p[2]);
+            break;
+        }
+        case 13: {  // params e0, r0, k, c
+            const double t = std::exp(-p[2] * (r - p[1])), c12 = p[3] / ipow(r, 12);
+            *e = p[0] * t * (t - 2.0) + c12;
+            *g = -2.0 * r * p[0] * p[2] * (1.0 - t) * t + 12.0 * c12;
+            break;
+        }
+        case 14: {  // params a, b, c
+            const double x = r / p[2], t = std::exp(-x);
+            *e = (p[0] + p[1] * r) * t;
+            *g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
             break;
         }
         default: {  // 5, Born-Huggins-Meyer
@@ -198,7 +243,8 @@ void dcell(const double *a, double *b)
     if (y[i1] >= y[i2]) { b[7] = d[i1]; b[8] = d[i2]; } else { b[7] = d[i2]; b[8] = d[i1]; }
 }
 
-// vdw.F90:617-967 (the closed-form tails of the generated forms)
+// vdw.F90:617-967 (the closed-form tails of the generated forms; WCA and DPD
+// have none)
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
              double volm, double *elrc, double *vlrc, const double *num_frz)
 {
@@ -239,7 +285,22 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                     eadd = p[0] / (9.0 * std::pow(r, 9)) - p[1] / (7.0 * std::pow(r, 7));
                     padd = 12.0 * p[0] / (9.0 * std::pow(r, 9)) - 10.0 * p[1] / (7.0 * std::pow(r, 7));
                     break;
-                case 8: {
+                case 12:  // vdw.F90:798-808
+                    eadd = 4.0 * p[0] * (std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) -
+                                         p[2] * std::pow(p[1], 6) / (3.0 * std::pow(r, 3)));
+                    padd = 8.0 * p[0] * (6.0 * std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) -
+                                         p[2] * std::pow(p[1], 6) / std::pow(r, 3));
+                    break;
+                case 14: {  // vdw.F90:832-845: tail integrals of (a + b r) exp(-r/c)
+                    const double a = p[0], b = p[1], cc = p[2], t = std::exp(-r / cc);
+                    eadd = (b * cc * r * r * r + (3 * b * cc * cc + a * cc) * r * r + (6 * b * cc * cc * cc + 2 * a * cc * cc) * r +
+                            6 * b * cc * cc * cc * cc + 2 * a * cc * cc * cc) * t;
+                    padd = (b * r * r * r * r + (3 * b * cc + a) * r * r * r + (9 * b * cc * cc + 3 * a * cc) * r * r +
+                            (18 * b * cc * cc * cc + 6 * a * cc * cc) * r + 18 * b * cc * cc * cc * cc + 6 * a * cc * cc * cc) * t;
+                    break;
+                }
+                case 8:
+                case 13: {  // Morse; Morse + c/r^12 adds that term's tail (vdw.F90:810-830)
                     const double e0 = p[0], r0 = p[1], kk = p[2];
                     if (kk > 1.17549435e-38) {
                         const double t = std::exp(-kk * (r - r0)), k3 = kk * kk * kk, kr = kk * r;
@@ -247,6 +308,11 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                                e0 * t * t / (4.0 * k3) * ((kr + 1) * (kr + 1) + kk * kk * r * r);
                         padd = -2.0 * e0 * t / k3 * (kr * kr * kr + 3 * kr * kr + 6 * kr + 6) +
                                e0 * t * t / (4.0 * k3) * (4.0 * kr * kr * kr + 6 * kr * kr + 6 * kr + 3);
+                        if (kinds[key] == 13) {
+                            const double s9 = p[3] / (9.0 * std::pow(r, 9));
+                            eadd += s9;
+                            padd += 12.0 * s9;
+                        }
                     }
                     break;
                 }

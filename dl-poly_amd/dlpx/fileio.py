@@ -11,7 +11,8 @@ inputs and comparators:
   ``(4i10,1p,2e16.7)`` header, ``(a8,i10)`` and ``3g20.10`` records);
 * FIELD: ``read_field`` (``ffield.F90:150-5365``, the subset this path
   supports: energy units, molecule types of unbonded atoms, the ``12-6``,
-  ``lj``, ``nm``, ``buck``, ``bhm``, ``hbnd`` and ``mors`` vdW forms) and
+  ``lj``, ``nm``, ``buck``, ``bhm``, ``hbnd``, ``mors``, ``wca``, ``dpd``,
+  ``ljc``, ``mstw`` and ``ryd`` vdW forms) and
   ``write_field``;
 * CONTROL: ``read_control`` / ``write_control`` (new-style keyword file,
   ``control.F90``);
@@ -27,7 +28,7 @@ import struct
 
 import numpy as np
 
-from .systems import VDW_NAMES
+from .systems import VDW_NAMES, VDW_NPARAMS, VDW_WCA, wca_params
 
 
 # constants.F90:104,112-114,123: Boltzmann constant and energy units in
@@ -128,7 +129,7 @@ def write_field(path, sysm):
     out.append("vdw %d" % len(sysm.vdw))
     for (ti, tj, key, params) in sysm.vdw:
         out.append("%-8s %-8s %4s %s" % (sysm.type_names[ti], sysm.type_names[tj], VDW_NAMES[key],
-                                         " ".join(_num(p) for p in params)))
+                                         " ".join(_num(p) for p in params[:_VDW_FIELD_NPARAMS[key]])))
     out.append("close")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
@@ -289,10 +290,15 @@ def write_revcon(path, title, cell, names, pos, vel=None, frc=None, nstep=0, tst
 
 # ----------------------------------------------------------------- FIELD --
 
-_VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors": 8}
-# parameters that carry energy units, per vdW form (ffield.F90:3757-3800:
-# parpot(1) always; 12-6 and hbnd parpot(2); buck parpot(3); bhm parpot(4:5))
-_VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,)}
+_VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors": 8, "wca": 9, "dpd": 10,
+             "ljc": 12, "mstw": 13, "ryd": 14}
+# parameters that carry energy units, per vdW form (ffield.F90:3757-3805:
+# parpot(1) always; 12-6, hbnd and ryd parpot(2); buck parpot(3); bhm
+# parpot(4:5); mstw parpot(4))
+_VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,), 9: (0,), 10: (0,),
+                      12: (0,), 13: (0, 3), 14: (0, 1)}
+# parameters a FIELD record gives (WCA's cut is derived, ffield.F90:3788)
+_VDW_FIELD_NPARAMS = {**VDW_NPARAMS, VDW_WCA: 3}
 
 
 def _bonded_block(lines, i, count, natoms, kind, mname):
@@ -414,8 +420,11 @@ def read_field(path):
                     raise ValueError("vdW form %r is not supported by this path" % a[2])
                 kind = _VDW_KEYS[form]
                 params = [float(v) for v in a[3:]]
+                params += [0.0] * (_VDW_FIELD_NPARAMS[kind] - len(params))  # absent words read as 0
                 for j in _VDW_ENERGY_PARAMS[kind]:
                     params[j] *= unit
+                if kind == VDW_WCA:
+                    params = list(wca_params(*params[:3]))
                 vdw.append((a[0], a[1], kind, tuple(params)))
                 i += 1
         elif key == "close":

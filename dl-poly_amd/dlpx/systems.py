@@ -27,10 +27,26 @@ VDW_BUCKINGHAM = 4
 VDW_BORN_HUGGINS_MEYER = 5
 VDW_HYDROGEN_BOND = 6
 VDW_MORSE = 8
+VDW_WCA = 9                 # params eps, sig, d, cut = 2^(1/6) sig + d
+VDW_DPD = 10                # params a, rc
+VDW_LENNARD_JONES_COHESIVE = 12
+VDW_MORSE_12 = 13
+VDW_RYDBERG = 14
 VDW_NAMES = {VDW_12_6: "12-6", VDW_LENNARD_JONES: "lj", VDW_N_M: "nm", VDW_BUCKINGHAM: "buck",
-             VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors"}
+             VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors", VDW_WCA: "wca",
+             VDW_DPD: "dpd", VDW_LENNARD_JONES_COHESIVE: "ljc", VDW_MORSE_12: "mstw", VDW_RYDBERG: "ryd"}
 VDW_NPARAMS = {VDW_12_6: 2, VDW_LENNARD_JONES: 2, VDW_N_M: 4, VDW_BUCKINGHAM: 3, VDW_BORN_HUGGINS_MEYER: 5,
-               VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3}
+               VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3, VDW_WCA: 4, VDW_DPD: 2, VDW_LENNARD_JONES_COHESIVE: 3,
+               VDW_MORSE_12: 4, VDW_RYDBERG: 3}
+
+
+def wca_params(eps, sig, d=0.0):
+    """WCA parameters as the FIELD reader completes them (ffield.F90:3784-3788):
+    sig taken positive, d limited to sig/2 from above, cut = 2^(1/6) sig + d."""
+    sig = abs(sig)
+    if d > sig / 2.0:
+        d = sig / 2.0
+    return (eps, sig, d, 2.0 ** (1.0 / 6.0) * sig + d)
 
 # Tosi-Fumi NaCl Born-Huggins-Meyer parameters, DL_POLY form
 # A*exp(B*(sigma-r)) - C/r^6 - D/r^8, converted from kJ/mol to internal
@@ -228,15 +244,27 @@ MIX_VDW = [
     (2, 1, VDW_MORSE, (80.0, 3.9, 1.6)),
     (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
 ]
+# the other analytic forms: LJ cohesive, Morse + c/r^12, Rydberg, WCA (cut
+# 4.2 A, inside the nearest-neighbour shell) and DPD with rc 9.5 A, beyond a
+# 9 A vdW cutoff, so the force-shift exception for DPD shows
+MIX2_VDW = [
+    (0, 0, VDW_LENNARD_JONES_COHESIVE, (99.6, 3.405, 0.8)),
+    (1, 0, VDW_MORSE_12, (80.0, 3.9, 1.6, 1.0e7)),
+    (2, 0, VDW_RYDBERG, (2.0e5, -1.0e3, 0.4)),
+    (1, 1, VDW_WCA, wca_params(120.0, 3.3, 0.5)),
+    (2, 1, VDW_DPD, (300.0, 9.5)),
+    (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
+]
 
 
 def mixture(ncells=10, a=5.78015, temperature=100.0, seed=20261202, charges=(0.3, -0.3, 0.0), kvec=(36, 36, 36),
-            alpha=0.35, cutoff=9.0, padding=0.5, timestep=0.004, jitter=0.1):
+            alpha=0.35, cutoff=9.0, padding=0.5, timestep=0.004, jitter=0.1, vdw=None):
     """Three species on an FCC lattice (4 sites per cell, edge ``a``), the
     species drawn by a seeded permutation of the sites, with a different vdW
-    form for each pair (12-6, Buckingham, 12-10, n-m, Morse, Lennard-Jones)
-    and SPME Coulomb.  Atoms are listed species by species (the FIELD's
-    molecule order)."""
+    form for each pair (by default 12-6, Buckingham, 12-10, n-m, Morse,
+    Lennard-Jones; ``vdw=MIX2_VDW`` gives the other six forms) and SPME
+    Coulomb.  Atoms are listed species by species (the FIELD's molecule
+    order)."""
     n = 4 * ncells ** 3
     L = a * ncells
     r_perm, r_jit, r_vel = _streams(seed, 3)
@@ -251,7 +279,8 @@ def mixture(ncells=10, a=5.78015, temperature=100.0, seed=20261202, charges=(0.3
     vel = _maxwell(r_vel, masses[types], temperature) if temperature > 0 else np.zeros((n, 3))
     return System(name="mix_%d" % n, cell=np.diag([L, L, L]).reshape(9).astype(np.float64), pos=pos, vel=vel,
                   types=types, type_names=list(MIX_NAMES), masses=masses, charges=np.array(charges, np.float64),
-                  vdw=list(MIX_VDW), cutoff=cutoff, padding=padding, vdw_cutoff=cutoff, electrostatics=True,
+                  vdw=list(MIX_VDW if vdw is None else vdw), cutoff=cutoff, padding=padding, vdw_cutoff=cutoff,
+                  electrostatics=True,
                   alpha=alpha, kvec=tuple(kvec), nsplines=8, timestep=timestep, temperature=temperature,
                   meta=dict(seed=seed, a=a, ncells=ncells, jitter=jitter))
 

@@ -149,6 +149,42 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         *g = -2.0 * r * p[0] * p[2] * (1.0 - t) * t;
         return 0;
     }
+    if (kind == 9) { /* WCA (two_body_potentials.F90:559-576), params eps, sig, d, cut */
+        *e = 0.0; *g = 0.0;
+        if (r < p[3] || fabs(r - p[2]) < 1.0e-10) {
+            double sr6 = pow(p[1] / (r - p[2]), 6);
+            *e = 4.0 * p[0] * sr6 * (sr6 - 1.0) + p[0];
+            *g = 24.0 * p[0] * sr6 * (2.0 * sr6 - 1.0) * r / (r - p[2]);
+        }
+        return 0;
+    }
+    if (kind == 10) { /* DPD (:591-610), params a, rc */
+        *e = 0.0; *g = 0.0;
+        if (r < p[1]) {
+            double t2 = r / p[1], t1 = 0.5 * p[0] * p[1] * (1.0 - t2);
+            *e = t1 * (1.0 - t2);
+            *g = 2.0 * t1 * t2;
+        }
+        return 0;
+    }
+    if (kind == 12) { /* LJ cohesive (:283-293), params eps, sig, c */
+        double sor6 = pow(p[1] / r, 6);
+        *e = 4.0 * p[0] * sor6 * (sor6 - p[2]);
+        *g = 24.0 * p[0] * sor6 * (2.0 * sor6 - p[2]);
+        return 0;
+    }
+    if (kind == 13) { /* Morse + c/r^12 (:442-456), params e0, r0, k, c */
+        double t1 = exp(-p[2] * (r - p[1])), t2 = p[3] * pow(r, -12);
+        *e = p[0] * t1 * (t1 - 2.0) + t2;
+        *g = -2.0 * r * p[0] * p[2] * (1.0 - t1) * t1 + 12.0 * t2;
+        return 0;
+    }
+    if (kind == 14) { /* Rydberg (:689-702), params a, b, c */
+        double kk = r / p[2], t1 = exp(-kk);
+        *e = (p[0] + p[1] * r) * t1;
+        *g = kk * t1 * (p[0] - p[1] * p[2] + p[1] * r);
+        return 0;
+    }
     return -1;
 }
 
@@ -240,11 +276,21 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
             } else if (kd == 6) { /* :755-764 */
                 eadd = p[0] / (9.0 * pow(r, 9)) - p[1] / (7.0 * pow(r, 7));
                 padd = 12.0 * p[0] / (9.0 * pow(r, 9)) - 10.0 * p[1] / (7.0 * pow(r, 7));
-            } else if (kd == 8 && p[2] > 1.17549435e-38) { /* :766-782 */
+            } else if ((kd == 8 || kd == 13) && p[2] > 1.17549435e-38) { /* :766-782, 810-830 */
                 double e0 = p[0], r0 = p[1], kk = p[2], t = exp(-kk * (r - r0)), k3 = kk * kk * kk;
                 eadd = -2.0 * e0 * t / k3 * (pow(kk * r + 1, 2) + 1) + e0 * t * t / (4.0 * k3) * (pow(kk * r + 1, 2) + kk * kk * r * r);
                 padd = -2.0 * e0 * t / k3 * (pow(kk, 3) * pow(r, 3) + 3 * kk * kk * r * r + 6 * kk * r + 6) +
                        e0 * t * t / (4.0 * k3) * (4.0 * pow(kk, 3) * pow(r, 3) + 6 * kk * kk * r * r + 6 * kk * r + 3);
+                if (kd == 13) { double s9 = p[3] / (9.0 * pow(r, 9)); eadd += s9; padd += 12.0 * s9; }
+            } else if (kd == 12) { /* :798-808 */
+                eadd = 4.0 * p[0] * (pow(p[1], 12) / (9.0 * pow(r, 9)) - p[2] * pow(p[1], 6) / (3.0 * pow(r, 3)));
+                padd = 8.0 * p[0] * (6.0 * pow(p[1], 12) / (9.0 * pow(r, 9)) - p[2] * pow(p[1], 6) / pow(r, 3));
+            } else if (kd == 14) { /* :832-845 */
+                double a = p[0], b = p[1], c = p[2], t = exp(-r / c);
+                eadd = (b * c * pow(r, 3) + (3 * b * c * c + a * c) * r * r + (6 * b * pow(c, 3) + 2 * a * c * c) * r +
+                        6 * b * pow(c, 4) + 2 * a * pow(c, 3)) * t;
+                padd = (b * pow(r, 4) + (3 * b * c + a) * pow(r, 3) + (9 * b * c * c + 3 * a * c) * r * r +
+                        (18 * b * pow(c, 3) + 6 * a * c * c) * r + 18 * b * pow(c, 4) + 6 * a * pow(c, 3)) * t;
             }
             if (i != j) { eadd *= 2.0; padd *= 2.0; }
             /* vdw.F90:953: frozen-frozen pairs carry no correction */
@@ -896,7 +942,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
             if (kind > 0) {
                 const double *pk = in->vdw_param + 5 * (long)k;
                 double afs = 0.0, bfs = 0.0, e, g;
-                if (in->vdw_fs) {
+                if (in->vdw_fs && kind != 10) { /* DPD: vdw.F90:1021-1025 */
                     double z, dz;
                     orc_vdw_eval(kind, pk, in->rvdw, &z, &dz);
                     afs = dz / in->rvdw;

@@ -58,6 +58,11 @@ def _direct(sysm):
     return sysm
 
 
+def _mix2(sysm):
+    sysm.name = sysm.name.replace("mix", "mix2")
+    return sysm
+
+
 def cases():
     return {
         # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
@@ -84,6 +89,12 @@ def cases():
         # vdw_method direct: analytic forms per pair, without and with force-shifting
         "mix_4000_direct": (_direct(systems.mixture()), 0),
         "mix_4000_direct_fs": (_direct(_fs(systems.mixture())), 0),
+        # the other analytic forms (LJ cohesive, Morse + c/r^12, Rydberg, WCA,
+        # DPD, LJ): tabulated, force-shifted, direct, direct force-shifted
+        "mix2_4000_k36": (_mix2(systems.mixture(vdw=systems.MIX2_VDW)), 0),
+        "mix2_4000_fs": (_fs(_mix2(systems.mixture(vdw=systems.MIX2_VDW))), 0),
+        "mix2_4000_direct": (_direct(_mix2(systems.mixture(vdw=systems.MIX2_VDW))), 0),
+        "mix2_4000_direct_fs": (_direct(_fs(_mix2(systems.mixture(vdw=systems.MIX2_VDW)))), 0),
         # polar triatomic molecules whose bonds and angle (zero-strength
         # harmonic terms) exclude every intramolecular pair: Ewald exclusion
         # correction (ewald_spole.F90:479-679)

@@ -79,12 +79,32 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
     if bad:
         text = text.replace("finish\nion", bad + "finish\nion", 1)
     else:
-        text = text.replace("C  C  lj   1.0   3.4", "C  C  wca  1.0 3.4 0.0")
+        text = text.replace("C  C  lj   1.0   3.4", "C  C  zbl  1.0 3.4 0.0")
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "FIELD")
         open(p, "w").write(text)
         with pytest.raises(ValueError):
             fileio.read_field(p)
+
+
+@pytest.mark.parametrize("rec,want", [
+    # ffield.F90:3784-3788: sig made positive, d limited to sig/2 from above,
+    # cut = 2^(1/6) sig + d; energies in kJ/mol x 100
+    ("wca  1.0 -3.0 2.0", (100.0, 3.0, 1.5, 2 ** (1 / 6) * 3.0 + 1.5)),
+    ("wca  1.0 3.0 -2.0", (100.0, 3.0, -2.0, 2 ** (1 / 6) * 3.0 - 2.0)),
+    ("wca  1.0 3.0", (100.0, 3.0, 0.0, 2 ** (1 / 6) * 3.0)),
+    ("dpd  2.0 4.0", (200.0, 4.0)),
+    ("ljc  1.0 3.4 0.5", (100.0, 3.4, 0.5)),
+    ("mstw 1.0 3.4 1.5 2.0", (100.0, 3.4, 1.5, 200.0)),
+    ("ryd  1.0 2.0 0.3", (100.0, 200.0, 0.3)),
+])
+def test_field_reader_completes_other_forms(rec, want):
+    text = FIELD_KJ.replace("C  C  lj   1.0   3.4", "C  C  " + rec)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "FIELD")
+        open(p, "w").write(text)
+        got = [v for v in fileio.read_field(p)["vdw"] if v[0] == "C" and v[1] == "C"][0]
+    assert got[3] == pytest.approx(want, rel=1e-15, abs=0)
 
 
 FIELD_MOL = """three-site molecules

@@ -67,7 +67,9 @@ def _eval(kind, p, r):
 # parameters 1, 2, 3, ... (index = vdW key)
 VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
              4: (-2.3934693402873668, 17.696734670143684), 5: (45.598150033144236, -45.196300066288472),
-             6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847)}
+             6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847), 9: (1.0, 12.0), 10: (0.25, -0.5),
+             12: (15616.0, -192000.0), 13: (367.25771964635982, -2348.0595394172847),
+             14: (2.1495939317213679, 0.71653131057378916)}
 
 
 @pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
@@ -263,11 +265,20 @@ def _tail(kind, p):
         return lambda r: -p[3] / r ** 6 - p[4] / r ** 8
     if kind == 6:
         return lambda r: p[0] / r ** 12 - p[1] / r ** 10
+    if kind in (9, 10):  # the correction has no case for WCA and DPD (vdw.F90:681-933)
+        return lambda r: 0.0
+    if kind == 12:
+        return lambda r: 4 * p[0] * ((p[1] / r) ** 12 - p[2] * (p[1] / r) ** 6)
+    if kind == 13:
+        return lambda r: p[0] * ((1 - math.exp(-p[2] * (r - p[1]))) ** 2 - 1) + p[3] / r ** 12
+    if kind == 14:
+        return lambda r: (p[0] + p[1] * r) * math.exp(-r / p[2])
     return lambda r: p[0] * ((1 - math.exp(-p[2] * (r - p[1]))) ** 2 - 1)
 
 
 @pytest.mark.parametrize("make", [lambda: systems.nacl(ncells=5, kvec=(32, 32, 32)),
-                                  lambda: systems.mixture(ncells=6)])
+                                  lambda: systems.mixture(ncells=6),
+                                  lambda: systems.mixture(ncells=6, vdw=systems.MIX2_VDW)])
 def test_vdw_lrc_matches_tail_integrals(make):
     s = make()
     _, _, _, from_key = oracle.vdw_tables(s)
