@@ -37,8 +37,11 @@ VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP
 
 
 # kernels of each stage (mixed precision, spline order 8)
-KERNELS = {"pair": "k_pair<float, 8, 512, false, false, false>", "spread": "k_spread_cells<float, 8>",
+KERNELS = {"pair": "k_pair<float, 8, 512, false, false, false, false>", "spread": "k_spread_cells<float, 8>",
            "fft": "k_fft_x_r2c/k_fft_c2c/k_fft_z_conv/k_fft_x_c2r<float, 16>", "gather": "k_gather<float, 8>"}
+# the FFT stage's kernels and their launches per stage
+FFT_KERNELS = [("k_fft_x_r2c<float, 16>", 1), ("k_fft_c2c<float, 16>", 2), ("k_fft_z_conv<float, 16>", 1),
+               ("k_fft_x_c2r<float, 16>", 1)]
 
 
 def traffic_from_profile(workload, kernel):
@@ -242,7 +245,7 @@ def force_error(s, md_steps=50):
     res = {}
     for prec in (lib.FP64, lib.MIXED):
         ctx = lib.Context(s, precision=prec)
-        ctx.link_cell_pairs(pos)
+        np_list, _ = ctx.link_cell_pairs(pos)
         res[prec] = ctx.two_body_forces(pos)
         if prec == lib.FP64:
             rc = max(s.cutoff, s.vdw_cutoff)
@@ -255,7 +258,7 @@ def force_error(s, md_steps=50):
     v64 = res[lib.FP64]["vircpe"] + res[lib.FP64]["virsrp"]
     vmx = res[lib.MIXED]["vircpe"] + res[lib.MIXED]["virsrp"]
     return dict(force_rms_rel=err, energy_rel=abs(emx - e64) / abs(e64), virial_rel=abs(vmx - v64) / abs(v64),
-                npairs_rc=np_rc)
+                npairs_rc=np_rc, npairs_list=int(np_list))
 
 
 def vs_reference(s, steps, revcon):
@@ -363,7 +366,8 @@ def main():
                           achieved_tflops=flops / t / 1e12, achieved_gbs=byts / t / 1e9, tmin_frac=tmin / t)
         tmin_sum += tmin
         tmeas_sum += t
-    dom = max(stages, key=lambda k: stages[k]["ms"])
+    # the dominant single kernel (the FFT stage is five launches of four kernels)
+    dom = max((k for k in stages if k != "fft"), key=lambda k: stages[k]["ms"])
     d = stages[dom]
     flops, byts = work[dom]
     t = d["ms"] * 1e-3
@@ -375,6 +379,17 @@ def main():
     roof["kernel"] = KERNELS[dom]
     roof["traffic"] = traffic_from_profile(args.workload, roof["kernel"])
     roof["stage_fraction_sum_tmin_over_tmeasured"] = tmin_sum / tmeas_sum if tmeas_sum else None
+    if "fft" in stages:
+        tr = [traffic_from_profile(args.workload, k) for k, _ in FFT_KERNELS]
+        stages["fft"]["traffic"] = (sum(t * m for t, (_, m) in zip(tr, FFT_KERNELS))
+                                    if all(t is not None for t in tr) else None)
+    # the list build, per rebuild (28 B per atom read + 4 B per stored entry;
+    # HBM-bound), outside the per-step sum above
+    t_list = ms[5] / max(runs[lib.MIXED]["timed_rebuilds"], 1) * 1e-3
+    if runs[lib.MIXED]["timed_rebuilds"] > 0:
+        b_list = 28.0 * s.natms + 4.0 * err["npairs_list"]
+        stages["list_per_rebuild"] = dict(ms=t_list * 1e3, bound="hbm", achieved_gbs=b_list / t_list / 1e9,
+                                          tmin_frac=b_list / HBM_PEAK / t_list, entries=err["npairs_list"])
 
     drift = None
     if args.drift_steps > 0:

@@ -171,13 +171,27 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
         whi[t] = -0x7fffffff;
     }
     __syncthreads();
+    // window bounds; a cell whose atoms all fit the staging arrays has their
+    // B-splines, base points and charges staged in the same pass (one read of
+    // the positions)
+    const bool once = n <= NA;
     for (int k = t; k < n; k += NT) {
         const double4 p = pos[s0 + k];
         for (int d = 0; d < 3; d++) {
-            const int b = (int)floor(grid_u(sp, d, p.x, p.y, p.z)) - S + 1;
+            const double u = grid_u(sp, d, p.x, p.y, p.z);
+            const double fl = floor(u);
+            const int b = (int)fl - S + 1;
             atomicMin(&wlo[d], b);
             atomicMax(&whi[d], b);
+            if (once) {
+                R M[S];
+                bspline<R, false, S>((R)(u - fl), M, nullptr);
+#pragma unroll
+                for (int m = 0; m < S; m++) sM[d][k][m] = M[m];
+                sB[d][k] = b;
+            }
         }
+        if (once) sQ[k] = (R)p.w;
     }
     __syncthreads();
     const int lo0 = wlo[0], lo1 = wlo[1], lo2 = wlo[2];
@@ -204,8 +218,6 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
             }
         }
     };
-    const bool once = n <= NA;
-    if (once) stage(0);  // made visible by the barrier after the slab clear
     for (int z0 = 0; z0 < Wz; z0 += Dz) {
         if (tile) {
             if (z0 > 0) __syncthreads();  // the previous slab's flush has read the slab
