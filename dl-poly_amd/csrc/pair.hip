@@ -157,6 +157,19 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
             g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
             break;
         }
+        case 22: {  // Sanderson (:865-879)
+            const R x = (r - p[1]) / p[2], t = p[0] * exp(-x * x);
+            e = -t;
+            g = (R)-2 * (r - p[1]) * r * t / (p[2] * p[2]);
+            break;
+        }
+        case 23:  // nDPD, zero beyond rc (:623-642)
+            if (r < p[3]) {
+                const R x = r / p[3], t1 = p[0] * p[3] * ((R)1 - x), t0 = p[1] * pow((R)1 - x, p[2] - (R)1);
+                e = t1 * ((R)1 - x) * (t0 / (p[2] + (R)1) - (R)0.5);
+                g = t1 * x * (t0 - (R)1);
+            }
+            break;
         default:
             break;
     }

@@ -28,9 +28,14 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //  12 LJ cohesive 4 eps [(sig/r)^12 - c (sig/r)^6]       (:283-293)
 //  13 Morse + 12 e0 {[1 - exp(-k (r-r0))]^2 - 1} + c/r^12 (:442-456)
 //  14 Rydberg    (a + b r) exp(-r/c)                     (:689-702)
+//  22 Sanderson  -A exp(-[(r-L)/d]^2)                    (:865-879)
+//  23 nDPD       a rc (1-r/rc)^2 [b (1-r/rc)^(n-1)/(n+1) - 1/2]
+//                below rc                                (:623-642)
+// (no long-range correction for 9, 10, 22 and 23: vdw.F90:681-933)
 bool vdw_kind_supported(int kind)
 {
-    return (kind >= 1 && kind <= 6) || (kind >= 8 && kind <= 10) || (kind >= 12 && kind <= 14);
+    return (kind >= 1 && kind <= 6) || (kind >= 8 && kind <= 10) || (kind >= 12 && kind <= 14) || kind == 22 ||
+           kind == 23;
 }
 
 static double ipow(double x, int n)
@@ -114,6 +119,21 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
             const double x = r / p[2], t = std::exp(-x);
             *e = (p[0] + p[1] * r) * t;
             *g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
+            break;
+        }
+        case 22: {  // params A, L, d
+            const double x = (r - p[1]) / p[2], t = p[0] * std::exp(-x * x);
+            *e = -t;
+            *g = -2.0 * (r - p[1]) * r * t / (p[2] * p[2]);
+            break;
+        }
+        case 23: {  // params a, b, n, rc
+            *e = *g = 0.0;
+            if (r < p[3]) {
+                const double x = r / p[3], t1 = p[0] * p[3] * (1.0 - x), t0 = p[1] * std::pow(1.0 - x, p[2] - 1.0);
+                *e = t1 * (1.0 - x) * (t0 / (p[2] + 1.0) - 0.5);
+                *g = t1 * x * (t0 - 1.0);
+            }
             break;
         }
         default: {  // 5, Born-Huggins-Meyer

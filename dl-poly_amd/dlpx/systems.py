@@ -32,12 +32,15 @@ VDW_DPD = 10                # params a, rc
 VDW_LENNARD_JONES_COHESIVE = 12
 VDW_MORSE_12 = 13
 VDW_RYDBERG = 14
+VDW_SANDERSON = 22          # params A, L, d
+VDW_NDPD = 23               # params a, b, n, rc
 VDW_NAMES = {VDW_12_6: "12-6", VDW_LENNARD_JONES: "lj", VDW_N_M: "nm", VDW_BUCKINGHAM: "buck",
              VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors", VDW_WCA: "wca",
-             VDW_DPD: "dpd", VDW_LENNARD_JONES_COHESIVE: "ljc", VDW_MORSE_12: "mstw", VDW_RYDBERG: "ryd"}
+             VDW_DPD: "dpd", VDW_LENNARD_JONES_COHESIVE: "ljc", VDW_MORSE_12: "mstw", VDW_RYDBERG: "ryd",
+             VDW_SANDERSON: "sand", VDW_NDPD: "ndpd"}
 VDW_NPARAMS = {VDW_12_6: 2, VDW_LENNARD_JONES: 2, VDW_N_M: 4, VDW_BUCKINGHAM: 3, VDW_BORN_HUGGINS_MEYER: 5,
                VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3, VDW_WCA: 4, VDW_DPD: 2, VDW_LENNARD_JONES_COHESIVE: 3,
-               VDW_MORSE_12: 4, VDW_RYDBERG: 3}
+               VDW_MORSE_12: 4, VDW_RYDBERG: 3, VDW_SANDERSON: 3, VDW_NDPD: 4}
 
 
 def wca_params(eps, sig, d=0.0):
@@ -242,6 +245,16 @@ MIX_VDW = [
     (2, 0, VDW_HYDROGEN_BOND, (8.4e9, 6.3e8)),
     (1, 1, VDW_N_M, (60.0, 12.0, 6.0, 3.1)),
     (2, 1, VDW_MORSE, (80.0, 3.9, 1.6)),
+    (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
+]
+# Sanderson and nDPD (rc 9.5 A, beyond a 9 A vdW cutoff) beside three
+# repulsive forms
+MIX3_VDW = [
+    (0, 0, VDW_LENNARD_JONES, (99.6, 3.405)),
+    (1, 0, VDW_SANDERSON, (200.0, 4.0, 1.0)),
+    (2, 0, VDW_NDPD, (300.0, 2.0, 3.0, 9.5)),
+    (1, 1, VDW_LENNARD_JONES, (120.0, 3.3)),
+    (2, 1, VDW_BUCKINGHAM, (3.2e7, 0.30, 4.28e5)),
     (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
 ]
 # the other analytic forms: LJ cohesive, Morse + c/r^12, Rydberg, WCA (cut

@@ -185,6 +185,21 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         *g = kk * t1 * (p[0] - p[1] * p[2] + p[1] * r);
         return 0;
     }
+    if (kind == 22) { /* Sanderson (:865-879), params A, L, d */
+        double b = pow((r - p[1]) / p[2], 2.0), t = p[0] * exp(-b);
+        *e = -t;
+        *g = -2.0 * (r - p[1]) * r * t / pow(p[2], 2.0);
+        return 0;
+    }
+    if (kind == 23) { /* nDPD (:623-642), params a, b, n, rc */
+        *e = 0.0; *g = 0.0;
+        if (r < p[3]) {
+            double t2 = r / p[3], t1 = p[0] * p[3] * (1.0 - t2), t0 = p[1] * pow(1.0 - t2, p[2] - 1.0);
+            *e = t1 * (1.0 - t2) * (t0 / (p[2] + 1.0) - 0.5);
+            *g = t1 * t2 * (t0 - 1.0);
+        }
+        return 0;
+    }
     return -1;
 }
 
@@ -942,7 +957,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
             if (kind > 0) {
                 const double *pk = in->vdw_param + 5 * (long)k;
                 double afs = 0.0, bfs = 0.0, e, g;
-                if (in->vdw_fs && kind != 10) { /* DPD: vdw.F90:1021-1025 */
+                if (in->vdw_fs && kind != 10 && kind != 23) { /* DPD, nDPD: vdw.F90:1021-1030 */
                     double z, dz;
                     orc_vdw_eval(kind, pk, in->rvdw, &z, &dz);
                     afs = dz / in->rvdw;

@@ -63,6 +63,11 @@ def _mix2(sysm):
     return sysm
 
 
+def _mix3(sysm):
+    sysm.name = sysm.name.replace("mix", "mix3")
+    return sysm
+
+
 def cases():
     return {
         # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
@@ -95,6 +100,11 @@ def cases():
         "mix2_4000_fs": (_fs(_mix2(systems.mixture(vdw=systems.MIX2_VDW))), 0),
         "mix2_4000_direct": (_direct(_mix2(systems.mixture(vdw=systems.MIX2_VDW))), 0),
         "mix2_4000_direct_fs": (_direct(_fs(_mix2(systems.mixture(vdw=systems.MIX2_VDW)))), 0),
+        # Sanderson and nDPD: tabulated and direct, with and without force-shifting
+        "mix3_4000_k36": (_mix3(systems.mixture(vdw=systems.MIX3_VDW)), 0),
+        "mix3_4000_fs": (_fs(_mix3(systems.mixture(vdw=systems.MIX3_VDW))), 0),
+        "mix3_4000_direct": (_direct(_mix3(systems.mixture(vdw=systems.MIX3_VDW))), 0),
+        "mix3_4000_direct_fs": (_direct(_fs(_mix3(systems.mixture(vdw=systems.MIX3_VDW)))), 0),
         # polar triatomic molecules whose bonds and angle (zero-strength
         # harmonic terms) exclude every intramolecular pair: Ewald exclusion
         # correction (ewald_spole.F90:479-679)

@@ -69,7 +69,8 @@ VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
              4: (-2.3934693402873668, 17.696734670143684), 5: (45.598150033144236, -45.196300066288472),
              6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847), 9: (1.0, 12.0), 10: (0.25, -0.5),
              12: (15616.0, -192000.0), 13: (367.25771964635982, -2348.0595394172847),
-             14: (2.1495939317213679, 0.71653131057378916)}
+             14: (2.1495939317213679, 0.71653131057378916), 22: (-0.8948393168143698, -0.1988531815143044),
+             23: (-0.4921875, -0.09375)}
 
 
 @pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
@@ -265,7 +266,7 @@ def _tail(kind, p):
         return lambda r: -p[3] / r ** 6 - p[4] / r ** 8
     if kind == 6:
         return lambda r: p[0] / r ** 12 - p[1] / r ** 10
-    if kind in (9, 10):  # the correction has no case for WCA and DPD (vdw.F90:681-933)
+    if kind in (9, 10, 22, 23):  # no correction for WCA, DPD, Sanderson, nDPD (vdw.F90:681-933)
         return lambda r: 0.0
     if kind == 12:
         return lambda r: 4 * p[0] * ((p[1] / r) ** 12 - p[2] * (p[1] / r) ** 6)
@@ -278,7 +279,8 @@ def _tail(kind, p):
 
 @pytest.mark.parametrize("make", [lambda: systems.nacl(ncells=5, kvec=(32, 32, 32)),
                                   lambda: systems.mixture(ncells=6),
-                                  lambda: systems.mixture(ncells=6, vdw=systems.MIX2_VDW)])
+                                  lambda: systems.mixture(ncells=6, vdw=systems.MIX2_VDW),
+                                  lambda: systems.mixture(ncells=6, vdw=systems.MIX3_VDW)])
 def test_vdw_lrc_matches_tail_integrals(make):
     s = make()
     _, _, _, from_key = oracle.vdw_tables(s)
