@@ -129,7 +129,7 @@ int upload_tables(Ctx *c)
             const double *pk = c->gen_params.data() + 5 * (size_t)k;
             d[8 * row] = (double)kind;
             for (int q = 0; q < 5; q++) d[8 * row + 1 + q] = pk[q];
-            if (c->vdw_force_shift && kind != 10 && kind != 23) {  // DPD, nDPD: none (vdw.F90:1021-1030)
+            if (c->vdw_force_shift && kind != 7 && kind != 10 && kind != 23) {  // none for n-m shift, DPD, nDPD (vdw.F90:1015-1030)
                 double z, dz;
                 vdw_form(kind, pk, c->prm.vdw_cutoff, &z, &dz);
                 d[8 * row + 6] = dz / c->prm.vdw_cutoff;

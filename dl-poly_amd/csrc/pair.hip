@@ -157,6 +157,19 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
             g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
             break;
         }
+        case 7:  // n-m shifted and force-corrected at r_trunc (:360-400)
+            if (r <= p[4]) {
+                const R n = p[1], m = p[2], ni = rint(n), mi = rint(m), t = n - m, b = (R)1 / t;
+                const R c = p[4] / p[3], ci = p[3] / p[4];
+                const R beta = c * pow((pow(c, mi + (R)1) - (R)1) / (pow(c, ni + (R)1) - (R)1), b);
+                const R alpha = -t / (m * pow(beta, ni) * ((R)1 + (n * ci - n - (R)1) * pow(ci, ni)) -
+                                      n * pow(beta, mi) * ((R)1 + (m * ci - m - (R)1) * pow(ci, mi)));
+                const R e1 = p[0] * alpha, a = p[3] / r, bn = pow(beta * ci, ni) - pow(beta * ci, mi);
+                e = e1 * (m * pow(beta, ni) * (pow(a, ni) - pow(ci, ni)) - n * pow(beta, mi) * (pow(a, mi) - pow(ci, mi)) +
+                          n * m * ((r / p[4] - (R)1) * bn)) * b;
+                g = e1 * m * n * (pow(beta, ni) * pow(a, ni) - pow(beta, mi) * pow(a, mi) - r / p[4] * bn) * b;
+            }
+            break;
         case 22: {  // Sanderson (:865-879)
             const R x = (r - p[1]) / p[2], t = p[0] * exp(-x * x);
             e = -t;

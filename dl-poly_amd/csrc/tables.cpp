@@ -21,6 +21,7 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //   4 Buckingham A exp(-r/rho) - C/r^6                   (:471-485)
 //   5 BHM        A exp(B (sig-r)) - C/r^6 - D/r^8        (:499-514)
 //   6 12-10      A/r^12 - B/r^10                         (:530-546)
+//   7 n-m shift  n-m shifted and force-corrected at r_trunc (:360-400)
 //   8 Morse      e0 {[1 - exp(-k (r-r0))]^2 - 1}         (:416-428)
 //   9 WCA        4 eps [(sig/(r-d))^12 - (sig/(r-d))^6] + eps below
 //                cut = 2^(1/6) sig + d, else 0           (:559-576)
@@ -31,11 +32,10 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //  22 Sanderson  -A exp(-[(r-L)/d]^2)                    (:865-879)
 //  23 nDPD       a rc (1-r/rc)^2 [b (1-r/rc)^(n-1)/(n+1) - 1/2]
 //                below rc                                (:623-642)
-// (no long-range correction for 9, 10, 22 and 23: vdw.F90:681-933)
+// (no long-range correction for 7, 9, 10, 22 and 23: vdw.F90:681-933)
 bool vdw_kind_supported(int kind)
 {
-    return (kind >= 1 && kind <= 6) || (kind >= 8 && kind <= 10) || (kind >= 12 && kind <= 14) || kind == 22 ||
-           kind == 23;
+    return (kind >= 1 && kind <= 10) || (kind >= 12 && kind <= 14) || kind == 22 || kind == 23;
 }
 
 static double ipow(double x, int n)
@@ -119,6 +119,22 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
             const double x = r / p[2], t = std::exp(-x);
             *e = (p[0] + p[1] * r) * t;
             *g = x * t * (p[0] - p[1] * p[2] + p[1] * r);
+            break;
+        }
+        case 7: {  // params e0, n, m, r0, r_trunc; n and m rounded for the powers
+            *e = *g = 0.0;
+            if (r <= p[4]) {
+                const double n = p[1], m = p[2], ni = std::round(n), mi = std::round(m), t = n - m, b = 1.0 / t;
+                const double c = p[4] / p[3], ci = p[3] / p[4];
+                const double beta = c * std::pow((std::pow(c, mi + 1.0) - 1.0) / (std::pow(c, ni + 1.0) - 1.0), b);
+                const double alpha = -t / (m * std::pow(beta, ni) * (1.0 + (n * ci - n - 1.0) * std::pow(ci, ni)) -
+                                           n * std::pow(beta, mi) * (1.0 + (m * ci - m - 1.0) * std::pow(ci, mi)));
+                const double e1 = p[0] * alpha, a = p[3] / r;
+                const double bn = std::pow(beta * ci, ni) - std::pow(beta * ci, mi);
+                *e = e1 * (m * std::pow(beta, ni) * (std::pow(a, ni) - std::pow(ci, ni)) -
+                           n * std::pow(beta, mi) * (std::pow(a, mi) - std::pow(ci, mi)) + n * m * ((r / p[4] - 1.0) * bn)) * b;
+                *g = e1 * m * n * (std::pow(beta, ni) * std::pow(a, ni) - std::pow(beta, mi) * std::pow(a, mi) - r / p[4] * bn) * b;
+            }
             break;
         }
         case 22: {  // params A, L, d

@@ -97,7 +97,8 @@ int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, co
  * the vdW forms with closed-form long-range corrections (vdw.F90:64-113 keys,
  * parameters in vdws%param order, two_body_potentials.F90): kind[key] 1 =
  * 12-6 A,B; 2 = Lennard-Jones eps,sig; 3 = n-m e0,n,m,r0; 4 = Buckingham
- * A,rho,C; 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B; 8 = Morse
+ * A,rho,C; 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B;
+ * 7 = shifted n-m e0,n,m,r0,r_trunc; 8 = Morse
  * e0,r0,k; 9 = WCA eps,sig,d,cut (cut = 2^(1/6) sig + d, ffield.F90:3788);
  * 10 = DPD a,rc; 12 = LJ cohesive eps,sig,c; 13 = Morse + c/r^12 e0,r0,k,c;
  * 14 = Rydberg a,b,c; 22 = Sanderson A,L,d; 23 = nDPD a,b,n,rc; -1 none.  params 5 per key.  Other forms: tabulate them on the host

@@ -185,6 +185,23 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         *g = kk * t1 * (p[0] - p[1] * p[2] + p[1] * r);
         return 0;
     }
+    if (kind == 7) { /* n-m shift (:360-400), params e0, n, m, r0, r_trunc; Nint of n, m as powers */
+        *e = 0.0; *g = 0.0;
+        if (r <= p[4]) {
+            double n = p[1], m = p[2], n_int = round(n), m_int = round(m);
+            double t = n - m, b = 1.0 / t, c = p[4] / p[3], c_inv = p[3] / p[4];
+            double beta = c * pow((pow(c, m_int + 1.0) - 1.0) / (pow(c, n_int + 1.0) - 1.0), b);
+            double alpha = -t / (m * pow(beta, n_int) * (1.0 + (n * c_inv - n - 1.0) * pow(c_inv, n_int)) -
+                                 n * pow(beta, m_int) * (1.0 + (m * c_inv - m - 1.0) * pow(c_inv, m_int)));
+            double e1 = p[0] * alpha, a = p[3] / r;
+            *e = e1 * (m * pow(beta, n_int) * (pow(a, n_int) - pow(c_inv, n_int)) -
+                       n * pow(beta, m_int) * (pow(a, m_int) - pow(c_inv, m_int)) +
+                       n * m * ((r / p[4] - 1.0) * (pow(beta * c_inv, n_int) - pow(beta * c_inv, m_int)))) * b;
+            *g = e1 * m * n * (pow(beta, n_int) * pow(a, n_int) - pow(beta, m_int) * pow(a, m_int) -
+                               r / p[4] * (pow(beta * c_inv, n_int) - pow(beta * c_inv, m_int))) * b;
+        }
+        return 0;
+    }
     if (kind == 22) { /* Sanderson (:865-879), params A, L, d */
         double b = pow((r - p[1]) / p[2], 2.0), t = p[0] * exp(-b);
         *e = -t;
@@ -957,7 +974,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
             if (kind > 0) {
                 const double *pk = in->vdw_param + 5 * (long)k;
                 double afs = 0.0, bfs = 0.0, e, g;
-                if (in->vdw_fs && kind != 10 && kind != 23) { /* DPD, nDPD: vdw.F90:1021-1030 */
+                if (in->vdw_fs && kind != 7 && kind != 10 && kind != 23) { /* vdw.F90:1015-1030 */
                     double z, dz;
                     orc_vdw_eval(kind, pk, in->rvdw, &z, &dz);
                     afs = dz / in->rvdw;

@@ -70,7 +70,7 @@ VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
              6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847), 9: (1.0, 12.0), 10: (0.25, -0.5),
              12: (15616.0, -192000.0), 13: (367.25771964635982, -2348.0595394172847),
              14: (2.1495939317213679, 0.71653131057378916), 22: (-0.8948393168143698, -0.1988531815143044),
-             23: (-0.4921875, -0.09375)}
+             23: (-0.4921875, -0.09375), 7: (4567.8016528926664, -16899.173553719396)}
 
 
 @pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
@@ -266,7 +266,7 @@ def _tail(kind, p):
         return lambda r: -p[3] / r ** 6 - p[4] / r ** 8
     if kind == 6:
         return lambda r: p[0] / r ** 12 - p[1] / r ** 10
-    if kind in (9, 10, 22, 23):  # no correction for WCA, DPD, Sanderson, nDPD (vdw.F90:681-933)
+    if kind in (7, 9, 10, 22, 23):  # no correction for these forms (vdw.F90:681-933)
         return lambda r: 0.0
     if kind == 12:
         return lambda r: 4 * p[0] * ((p[1] / r) ** 12 - p[2] * (p[1] / r) ** 6)
