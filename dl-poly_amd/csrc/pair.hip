@@ -170,6 +170,13 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
                 g = e1 * m * n * (pow(beta, ni) * pow(a, ni) - pow(beta, mi) * pow(a, mi) - r / p[4] * bn) * b;
             }
             break;
+        case 11: {  // AMOEBA 14-7 (:657-676)
+            const R rho = r / p[1], t1 = (R)1 / ((R)0.07 + rho), t2 = (R)1 / ((R)0.12 + rpow(rho, 7));
+            const R t3 = p[0] * rpow((R)1.07 * t1, 7), t = t3 * ((R)1.12 * t2 - (R)2);
+            e = t;
+            g = (R)7 * (t1 * t + (R)1.12 * t3 * t2 * t2 * rpow(rho, 6)) * rho;
+            break;
+        }
         case 22: {  // Sanderson (:865-879)
             const R x = (r - p[1]) / p[2], t = p[0] * exp(-x * x);
             e = -t;

@@ -21,6 +21,8 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //   4 Buckingham A exp(-r/rho) - C/r^6                   (:471-485)
 //   5 BHM        A exp(B (sig-r)) - C/r^6 - D/r^8        (:499-514)
 //   6 12-10      A/r^12 - B/r^10                         (:530-546)
+//  11 AMOEBA 14-7 eps [1.07/(r/sig + 0.07)]^7 [1.12/((r/sig)^7 + 0.12) - 2]
+//                                                        (:657-676)
 //   7 n-m shift  n-m shifted and force-corrected at r_trunc (:360-400)
 //   8 Morse      e0 {[1 - exp(-k (r-r0))]^2 - 1}         (:416-428)
 //   9 WCA        4 eps [(sig/(r-d))^12 - (sig/(r-d))^6] + eps below
@@ -35,7 +37,7 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 // (no long-range correction for 7, 9, 10, 22 and 23: vdw.F90:681-933)
 bool vdw_kind_supported(int kind)
 {
-    return (kind >= 1 && kind <= 10) || (kind >= 12 && kind <= 14) || kind == 22 || kind == 23;
+    return (kind >= 1 && kind <= 14) || kind == 22 || kind == 23;
 }
 
 static double ipow(double x, int n)
@@ -101,6 +103,13 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
                 *e = h * (1.0 - x);
                 *g = 2.0 * h * x;
             }
+            break;
+        }
+        case 11: {  // params eps, sig
+            const double rho = r / p[1], t1 = 1.0 / (0.07 + rho), t2 = 1.0 / (0.12 + std::pow(rho, 7));
+            const double t3 = p[0] * std::pow(1.07 * t1, 7), t = t3 * (1.12 * t2 - 2.0);
+            *e = t;
+            *g = 7.0 * (t1 * t + 1.12 * t3 * t2 * t2 * std::pow(rho, 6)) * rho;
             break;
         }
         case 12: {  // params eps, sig, c
@@ -281,6 +290,40 @@ void dcell(const double *a, double *b)
 
 // vdw.F90:617-967 (the closed-form tails of the generated forms; WCA and DPD
 // have none)
+// AMOEBA 14-7 tail integrals as vdw.F90:522-613 evaluates them: Simpson
+// panels of width 2h (h = rw/(r0 n), n = 10000) over chunks of n steps in the
+// reduced distance x = r/r0, from rw/r0 onwards, chunk after chunk until one
+// changes the sum by at most prec (3 prec/h), each chunk starting again at
+// the previous chunk's last panel (that panel counts twice, as there).
+static double mm3(double x, double A, double B, double eps)
+{
+    const double ia = 1.0 / (A + x), ib = 1.0 / (B + std::pow(x, 7));
+    return eps * std::pow((1.0 + A) * ia, 7) * ((1.0 + B) * ib - 2.0);
+}
+static double dmm3(double x, double r0, double A, double B, double eps)
+{
+    const double ia = 1.0 / (A + x), ib = 1.0 / (B + std::pow(x, 7));
+    return -7.0 * (mm3(x, A, B, eps) * ia - std::pow((1.0 + A) * ia, 7) * ((1.0 + B) * ib * ib) * std::pow(x, 6)) / r0;
+}
+static double mm3_tail(double r0, double A, double B, double eps, double rw, double prec, bool deriv)
+{
+    const int n = 10000;
+    const double xs = rw / r0, h = (2.0 * xs - xs) / (double)n;
+    double s = 0.0, sold = HUGE_VAL;
+    for (int j = 1; std::fabs(s - sold) * h / 3.0 > prec; j++) {
+        sold = s;
+        for (int i = (j - 1) * n; i <= j * n; i += 2) {
+            const double x = xs + i * h, x1 = x + h, x2 = x + 2.0 * h;
+            if (deriv)
+                s += x * x * x * dmm3(x, r0, A, B, eps) + 4.0 * x1 * x1 * x1 * dmm3(x1, r0, A, B, eps) +
+                     x2 * x2 * x2 * dmm3(x2, r0, A, B, eps);
+            else
+                s += x * x * mm3(x, A, B, eps) + 4.0 * x1 * x1 * mm3(x1, A, B, eps) + x2 * x2 * mm3(x2, A, B, eps);
+        }
+    }
+    return deriv ? s * h * std::pow(r0, 4) / 3.0 : s * h * std::pow(r0, 3) / 3.0;
+}
+
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
              double volm, double *elrc, double *vlrc, const double *num_frz)
 {
@@ -320,6 +363,10 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                 case 6:
                     eadd = p[0] / (9.0 * std::pow(r, 9)) - p[1] / (7.0 * std::pow(r, 7));
                     padd = 12.0 * p[0] / (9.0 * std::pow(r, 9)) - 10.0 * p[1] / (7.0 * std::pow(r, 7));
+                    break;
+                case 11:  // vdw.F90:784-796
+                    eadd = mm3_tail(p[1], 0.07, 0.12, p[0], r, 1.0e-12, false);
+                    padd = -mm3_tail(p[1], 0.07, 0.12, p[0], r, 1.0e-12, true);
                     break;
                 case 12:  // vdw.F90:798-808
                     eadd = 4.0 * p[0] * (std::pow(p[1], 12) / (9.0 * std::pow(r, 9)) -

@@ -12,7 +12,7 @@ inputs and comparators:
 * FIELD: ``read_field`` (``ffield.F90:150-5365``, the subset this path
   supports: energy units, molecule types of unbonded atoms, the ``12-6``,
   ``lj``, ``nm``, ``buck``, ``bhm``, ``hbnd``, ``mors``, ``wca``, ``dpd``,
-  ``ljc``, ``mstw``, ``ryd``, ``sand``, ``ndpd`` and ``snm`` vdW forms) and
+  ``ljc``, ``mstw``, ``ryd``, ``sand``, ``ndpd``, ``snm`` and ``14-7`` vdW forms) and
   ``write_field``;
 * CONTROL: ``read_control`` / ``write_control`` (new-style keyword file,
   ``control.F90``);
@@ -291,12 +291,12 @@ def write_revcon(path, title, cell, names, pos, vel=None, frc=None, nstep=0, tst
 # ----------------------------------------------------------------- FIELD --
 
 _VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors": 8, "wca": 9, "dpd": 10,
-             "ljc": 12, "mstw": 13, "ryd": 14, "sand": 22, "ndpd": 23, "snm": 7}
+             "ljc": 12, "mstw": 13, "ryd": 14, "sand": 22, "ndpd": 23, "snm": 7, "14-7": 11}
 # parameters that carry energy units, per vdW form (ffield.F90:3757-3805:
 # parpot(1) always; 12-6, hbnd and ryd parpot(2); buck parpot(3); bhm
 # parpot(4:5); mstw parpot(4))
 _VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,), 9: (0,), 10: (0,),
-                      12: (0,), 13: (0, 3), 14: (0, 1), 22: (0,), 23: (0,), 7: (0,)}
+                      12: (0,), 13: (0, 3), 14: (0, 1), 22: (0,), 23: (0,), 7: (0,), 11: (0,)}
 # parameters a FIELD record gives (WCA's cut is derived, ffield.F90:3788)
 _VDW_FIELD_NPARAMS = {**VDW_NPARAMS, VDW_WCA: 3}
 

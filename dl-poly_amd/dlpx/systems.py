@@ -30,6 +30,7 @@ VDW_N_M_SHIFT = 7           # params e0, n, m, r0, r_trunc
 VDW_MORSE = 8
 VDW_WCA = 9                 # params eps, sig, d, cut = 2^(1/6) sig + d
 VDW_DPD = 10                # params a, rc
+VDW_AMOEBA = 11             # params eps, sig (14-7)
 VDW_LENNARD_JONES_COHESIVE = 12
 VDW_MORSE_12 = 13
 VDW_RYDBERG = 14
@@ -38,10 +39,11 @@ VDW_NDPD = 23               # params a, b, n, rc
 VDW_NAMES = {VDW_12_6: "12-6", VDW_LENNARD_JONES: "lj", VDW_N_M: "nm", VDW_BUCKINGHAM: "buck",
              VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors", VDW_WCA: "wca",
              VDW_DPD: "dpd", VDW_LENNARD_JONES_COHESIVE: "ljc", VDW_MORSE_12: "mstw", VDW_RYDBERG: "ryd",
-             VDW_SANDERSON: "sand", VDW_NDPD: "ndpd", VDW_N_M_SHIFT: "snm"}
+             VDW_SANDERSON: "sand", VDW_NDPD: "ndpd", VDW_N_M_SHIFT: "snm",
+             VDW_AMOEBA: "14-7"}
 VDW_NPARAMS = {VDW_12_6: 2, VDW_LENNARD_JONES: 2, VDW_N_M: 4, VDW_BUCKINGHAM: 3, VDW_BORN_HUGGINS_MEYER: 5,
                VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3, VDW_WCA: 4, VDW_DPD: 2, VDW_LENNARD_JONES_COHESIVE: 3,
-               VDW_MORSE_12: 4, VDW_RYDBERG: 3, VDW_SANDERSON: 3, VDW_NDPD: 4, VDW_N_M_SHIFT: 5}
+               VDW_MORSE_12: 4, VDW_RYDBERG: 3, VDW_SANDERSON: 3, VDW_NDPD: 4, VDW_N_M_SHIFT: 5, VDW_AMOEBA: 2}
 
 
 def wca_params(eps, sig, d=0.0):
@@ -248,10 +250,10 @@ MIX_VDW = [
     (2, 1, VDW_MORSE, (80.0, 3.9, 1.6)),
     (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
 ]
-# Sanderson, nDPD (rc 9.5 A, beyond a 9 A vdW cutoff) and the shifted n-m
-# form (truncated at 8.5 A) beside LJ and Buckingham pairs
+# AMOEBA 14-7, Sanderson, nDPD (rc 9.5 A, beyond a 9 A vdW cutoff) and the
+# shifted n-m form (truncated at 8.5 A) beside LJ and Buckingham pairs
 MIX3_VDW = [
-    (0, 0, VDW_LENNARD_JONES, (99.6, 3.405)),
+    (0, 0, VDW_AMOEBA, (99.6, 3.9)),
     (1, 0, VDW_SANDERSON, (200.0, 4.0, 1.0)),
     (2, 0, VDW_NDPD, (300.0, 2.0, 3.0, 9.5)),
     (1, 1, VDW_N_M_SHIFT, (120.0, 12.0, 6.0, 3.7, 8.5)),

@@ -100,7 +100,7 @@ int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, co
  * A,rho,C; 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B;
  * 7 = shifted n-m e0,n,m,r0,r_trunc; 8 = Morse
  * e0,r0,k; 9 = WCA eps,sig,d,cut (cut = 2^(1/6) sig + d, ffield.F90:3788);
- * 10 = DPD a,rc; 12 = LJ cohesive eps,sig,c; 13 = Morse + c/r^12 e0,r0,k,c;
+ * 10 = DPD a,rc; 11 = AMOEBA 14-7 eps,sig; 12 = LJ cohesive eps,sig,c; 13 = Morse + c/r^12 e0,r0,k,c;
  * 14 = Rydberg a,b,c; 22 = Sanderson A,L,d; 23 = nDPD a,b,n,rc; -1 none.  params 5 per key.  Other forms: tabulate them on the host
  * and use dlpx_set_vdw_tables.  Sizes follow bounds.F90:811,820. */
 int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, const double *params);

@@ -167,6 +167,13 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         }
         return 0;
     }
+    if (kind == 11) { /* AMOEBA 14-7 (:657-676), params eps, sig */
+        double rho = r / p[1], t1 = 1.0 / (0.07 + rho), t2 = 1.0 / (0.12 + pow(rho, 7));
+        double t3 = p[0] * pow(1.07 * t1, 7), t = t3 * (1.12 * t2 - 2.0);
+        *e = t;
+        *g = 7.0 * (t1 * t + 1.12 * t3 * pow(t2, 2) * pow(rho, 6)) * rho;
+        return 0;
+    }
     if (kind == 12) { /* LJ cohesive (:283-293), params eps, sig, c */
         double sor6 = pow(p[1] / r, 6);
         *e = 4.0 * p[0] * sor6 * (sor6 - p[2]);
@@ -275,6 +282,40 @@ double orc_interp3(const double *tab0, double rdr, double r)
 
 /* vdw.F90:617-967 for the forms of orc_vdw_eval: energy and virial long-range corrections.
  * kinds/params are indexed by pair key (0-based key = hi*(hi-1)/2+lo-1, 1-based types). */
+/* vdw.F90:522-613: the AMOEBA tail integrals by the reference's chunked
+ * Simpson rule (chunks of n = 10000 steps of h = rw/(r0 n) in x = r/r0 from
+ * rw/r0, each restarting at the previous chunk's last panel, until a chunk
+ * adds at most 3 prec/h) */
+static double orc_mm3(double x, double A, double B, double eps)
+{
+    double ia = 1.0 / (A + x), ib = 1.0 / (B + pow(x, 7));
+    return eps * pow((1.0 + A) * ia, 7) * ((1.0 + B) * ib - 2.0);
+}
+static double orc_dmm3(double x, double r0, double A, double B, double eps)
+{
+    double ia = 1.0 / (A + x), ib = 1.0 / (B + pow(x, 7));
+    return -7.0 * (orc_mm3(x, A, B, eps) * ia - pow((1.0 + A) * ia, 7) * ((1.0 + B) * pow(ib, 2)) * pow(x, 6)) / r0;
+}
+static double orc_mm3_int(double r0, double A, double B, double eps, double rw, double prec, int deriv)
+{
+    int n = 10000, j = 1;
+    double is = rw / r0, ie = 2.0 * is, h = (ie - is) / (double)n, s = 0.0, sold = HUGE_VAL;
+    while (fabs(s - sold) * h / 3.0 > prec) {
+        sold = s;
+        for (int i = (j - 1) * n; i <= j * n; i += 2) {
+            double x = is + i * h;
+            if (deriv)
+                s += pow(x, 3) * orc_dmm3(x, r0, A, B, eps) + 4.0 * pow(x + h, 3) * orc_dmm3(x + h, r0, A, B, eps) +
+                     pow(x + 2.0 * h, 3) * orc_dmm3(x + 2.0 * h, r0, A, B, eps);
+            else
+                s += x * x * orc_mm3(x, A, B, eps) + 4.0 * pow(x + h, 2) * orc_mm3(x + h, A, B, eps) +
+                     pow(x + 2.0 * h, 2) * orc_mm3(x + 2.0 * h, A, B, eps);
+        }
+        j++;
+    }
+    return deriv ? s * h * pow(r0, 4) / 3.0 : s * h * pow(r0, 3) / 3.0;
+}
+
 void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*5 per key*/,
                  double rvdw, double volm, double *elrc, double *vlrc, const double *num_frz /* NULL: none */)
 {
@@ -314,6 +355,9 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
                 padd = -2.0 * e0 * t / k3 * (pow(kk, 3) * pow(r, 3) + 3 * kk * kk * r * r + 6 * kk * r + 6) +
                        e0 * t * t / (4.0 * k3) * (4.0 * pow(kk, 3) * pow(r, 3) + 6 * kk * kk * r * r + 6 * kk * r + 3);
                 if (kd == 13) { double s9 = p[3] / (9.0 * pow(r, 9)); eadd += s9; padd += 12.0 * s9; }
+            } else if (kd == 11) { /* :784-796 */
+                eadd = orc_mm3_int(p[1], 0.07, 0.12, p[0], r, 1.0e-12, 0);
+                padd = -orc_mm3_int(p[1], 0.07, 0.12, p[0], r, 1.0e-12, 1);
             } else if (kd == 12) { /* :798-808 */
                 eadd = 4.0 * p[0] * (pow(p[1], 12) / (9.0 * pow(r, 9)) - p[2] * pow(p[1], 6) / (3.0 * pow(r, 3)));
                 padd = 8.0 * p[0] * (6.0 * pow(p[1], 12) / (9.0 * pow(r, 9)) - p[2] * pow(p[1], 6) / pow(r, 3));

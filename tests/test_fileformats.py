@@ -100,6 +100,7 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
     ("sand 2.0 4.0 1.0", (200.0, 4.0, 1.0)),
     ("ndpd 3.0 2.0 3.0 9.5", (300.0, 2.0, 3.0, 9.5)),
     ("snm  1.0 12.0 6.0 3.7 8.5", (100.0, 12.0, 6.0, 3.7, 8.5)),
+    ("14-7 1.0 3.9", (100.0, 3.9)),
 ])
 def test_field_reader_completes_other_forms(rec, want):
     text = FIELD_KJ.replace("C  C  lj   1.0   3.4", "C  C  " + rec)

@@ -70,7 +70,8 @@ VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
              6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847), 9: (1.0, 12.0), 10: (0.25, -0.5),
              12: (15616.0, -192000.0), 13: (367.25771964635982, -2348.0595394172847),
              14: (2.1495939317213679, 0.71653131057378916), 22: (-0.8948393168143698, -0.1988531815143044),
-             23: (-0.4921875, -0.09375), 7: (4567.8016528926664, -16899.173553719396)}
+             23: (-0.4921875, -0.09375), 7: (4567.8016528926664, -16899.173553719396),
+             11: (555.50928442334305, -3719.0014773362891)}
 
 
 @pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
@@ -268,6 +269,8 @@ def _tail(kind, p):
         return lambda r: p[0] / r ** 12 - p[1] / r ** 10
     if kind in (7, 9, 10, 22, 23):  # no correction for these forms (vdw.F90:681-933)
         return lambda r: 0.0
+    if kind == 11:
+        return lambda r: p[0] * (1.07 / (r / p[1] + 0.07)) ** 7 * (1.12 / ((r / p[1]) ** 7 + 0.12) - 2.0)
     if kind == 12:
         return lambda r: 4 * p[0] * ((p[1] / r) ** 12 - p[2] * (p[1] / r) ** 6)
     if kind == 13:
@@ -296,8 +299,16 @@ def test_vdw_lrc_matches_tail_integrals(make):
         mult = 1.0 if ti == tj else 2.0
         e += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r * r * ut(r), rc, np.inf)[0]
         v += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r ** 3 * dut(r), rc, np.inf)[0]
-    assert abs(elrc - e) < 1e-9 * abs(e)
-    assert abs(vlrc - v) < 1e-7 * abs(v)
+    # AMOEBA's corrections are the reference's chunked Simpson sums, whose
+    # chunks each recount one panel (vdw.F90:552-566): the energy lies a few
+    # 1e-6 above the integral.  Its virial integrand (dmm3, vdw.F90:579) is not
+    # the derivative of the form (no eps and the opposite sign on the x^6
+    # term), so that one is checked only through the reference fixtures of the
+    # third mixture, which pin both values (engsrp, virsrp).
+    amoeba = any(k == 11 for (_, _, k, _) in s.vdw)
+    assert abs(elrc - e) < (2e-5 if amoeba else 1e-9) * abs(e)
+    if not amoeba:
+        assert abs(vlrc - v) < 1e-7 * abs(v)
 
 
 def _erf_over_r_series(a, r, terms=30):
