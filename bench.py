@@ -97,33 +97,10 @@ def workload(name):
     return s, desc
 
 
-def ensemble_systems(nsys=256, seed=20261202):
-    """The batched-ensemble workload: nsys systems drawn from a seed, half LJ
-    argon and half molten NaCl (shuffled), sizes uniform in 1,000-20,000 atoms
-    rounded to whole lattices (4 m^3 argon, 8 m^3 NaCl), cubic boxes at the
-    densities of the single-system workloads, NaCl meshes the smallest
-    2^a 3^b 5^c size of at least one point per A, clamped to 32..64."""
-    rng = np.random.default_rng(seed)
-    kinds = ["argon"] * (nsys // 2) + ["nacl"] * (nsys - nsys // 2)
-    rng.shuffle(kinds)
-    out = []
-    for i, kind in enumerate(kinds):
-        n = int(rng.integers(1000, 20001))
-        if kind == "argon":
-            m = max(4, int(round((n / 4.0) ** (1.0 / 3.0))))
-            out.append(systems.argon(ncells=m, seed=seed + 1 + i))
-        else:
-            m = max(5, int(round((n / 8.0) ** (1.0 / 3.0))))
-            L = m * 5.64 * 1.08
-            K = min(64, max(32, systems.smooth235_at_least(int(math.ceil(L)))))
-            out.append(systems.nacl(ncells=m, kvec=(K, K, K), seed=seed + 1 + i))
-    return out
-
-
 def run_ensemble(args):
     """Mixed-precision ensemble (dlpx_md_run_ensemble over all systems) against
     the per-system FP64 loop (dlpx_md_run one system after another)."""
-    syss = ensemble_systems()
+    syss = systems.ensemble_systems()
     natoms = sum(s.natms for s in syss)
     log("ensemble: %d systems, %d atoms (%d argon, %d NaCl)" % (len(syss), natoms,
         sum(1 for s in syss if not s.electrostatics), sum(1 for s in syss if s.electrostatics)))

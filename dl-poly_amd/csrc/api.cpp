@@ -12,6 +12,10 @@
 namespace dlpx {
 
 int load_atoms(Ctx *c, const double *st_dev, const int32_t *st_t_dev, bool identity, bool with_vel);
+struct Ensemble;
+uint64_t next_serial();
+void ensemble_free(void *e);
+int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool energies, std::vector<int> &rebuilds);
 int store_atoms(Ctx *c, double *st_dev, bool with_pos, bool with_vel);
 int spme_finish(Ctx *c, double two_scale);
 
@@ -342,6 +346,113 @@ static int fetch_red(Ctx *c)
     return collect_timing(c);
 }
 
+// What a captured step graph bakes in (buffers, sizes, launch shapes, the
+// time step): a change of any of them forces a new capture.
+std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
+{
+    uint64_t t;
+    memcpy(&t, &tstep, sizeof(t));
+    return {(uint64_t)c->pos,      (uint64_t)c->vel,     (uint64_t)c->frc,     (uint64_t)c->frc_recip,
+            (uint64_t)c->xref,     (uint64_t)c->typ,     (uint64_t)c->nbr,     (uint64_t)c->cnt,
+            (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
+            (uint64_t)c->Q,        (uint64_t)c->Phi,     (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
+            (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
+            (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
+            (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total,
+            (uint64_t)c->frz,      (uint64_t)c->nfrozen, t};
+}
+
+// Capture forces (both streams), stage 2 of this step and stage 1 of the
+// next into one executable graph.  Only called after an eager step, so every
+// buffer is sized and every kernel attribute set before the capture.
+static int capture_step_graph(Ctx *c, double tstep)
+{
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+    c->step_graph = nullptr;
+    DLPX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = evaluate(c, true);
+    if (!r) r = nve_second(c, tstep, true);
+    if (!r) r = nve_first(c, tstep);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, DLPX_ERR_HIP + (int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->step_graph = nullptr;
+        return fail(c, DLPX_ERR_HIP + (int)ei, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    c->graph_key = step_graph_key(c, tstep);
+    return 0;
+}
+
+// Start of a resident run: clear the counters of the fused stage-1 kernel
+// and do stage 1 of the first step.
+static int md_begin(Ctx *c, int nsteps, double tstep)
+{
+    // displacement maximum and workgroup counter of the fused stage-1 kernel
+    DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
+    return nsteps > 0 ? nve_first(c, tstep) : 0;
+}
+
+// Step s of a resident run (nve.F90:127-215 with refresh_mappings/vnl_check
+// and the force call, drivers.F90:2105-2199): wait for this context's stage 1,
+// rebuild the list if an atom moved by padding/2, then forces, stage 2 and
+// the next step's stage 1.  With graphs, steps 2..n-1 replay one captured
+// graph.  Returns with the step's work queued on c->stream.
+static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool energies, int &rebuilds)
+{
+    int r;
+    int update = 1;
+    if (c->prm.padding > 0.0) {
+        DLPX_HIP(c, hipStreamSynchronize(c->stream));
+        update = nve_first_displacement(c) >= kHalfMinus * c->prm.padding ? 1 : 0;
+    }
+    if (update) {
+        if ((r = build_list(c))) return r;
+        rebuilds++;
+    }
+    const bool last = s == nsteps - 1;
+    if (graphs && s > 0 && !last) {
+        // forces + stage 2 of this step + stage 1 of the next, as one graph
+        if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
+            if ((r = capture_step_graph(c, tstep))) return r;
+        DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
+        return 0;
+    }
+    if ((r = evaluate(c, true))) return r;
+    if ((r = nve_second(c, tstep, true))) return r;
+    if (c->timing && (r = collect_timing(c))) return r;
+    if (energies) {
+        if ((r = reduce_partials(c))) return r;
+        DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
+                                   sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (!last && (r = nve_first(c, tstep))) return r;
+    return 0;
+}
+
+// A resident run with host-decided rebuilds (the list regrows when it
+// overflows), from the current positions and velocities: the list is built
+// and the forces evaluated first, as at dlpx_md_load.  Per-step records go to
+// energies_dev when energies is set.
+int md_fallback(Ctx *c, int nsteps, double tstep, bool energies, int &rebuilds)
+{
+    int r;
+    if ((r = build_list(c))) return r;
+    if ((r = evaluate(c))) return r;
+    if ((r = freeze_atoms(c))) return r;
+    rebuilds = 0;
+    if ((r = md_begin(c, nsteps, tstep))) return r;
+    for (int s = 0; s < nsteps; s++)
+        if ((r = md_step(c, s, nsteps, tstep, false, energies, rebuilds))) return r;
+    return 0;
+}
+
 }  // namespace dlpx
 
 using namespace dlpx;
@@ -397,10 +508,12 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         (r = dalloc(c, &c->xref, n)) || (r = dalloc(c, &c->typ, n)) || (r = dalloc(c, &c->typ_alt, n)) ||
         (r = dalloc(c, &c->gid, n)) || (r = dalloc(c, &c->gid_alt, n)) || (r = dalloc(c, &c->slot, n)) ||
         (r = dalloc(c, &c->stage, 9 * n)) || (r = dalloc(c, &c->stage_i, n)) || (r = dalloc(c, &c->cell_of, n)) ||
-        (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, 8)) ||
+        (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, F_WORDS)) ||
         (r = dalloc(c, &c->red_out, kReduceSlots)))
         return bail(r);
     if (hipMemset(c->vel, 0, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (hipMemset(c->flags, 0, sizeof(int32_t) * F_WORDS) != hipSuccess) return bail(DLPX_ERR_HIP);
+    c->serial = next_serial();
     // host words the GPU writes directly (the displacement maximum of each step)
     if (hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (hipHostGetDevicePointer((void **)&c->h_flags_dev, c->h_flags, 0) != hipSuccess) return bail(DLPX_ERR_HIP);
@@ -421,11 +534,14 @@ int dlpx_destroy(dlpx_ctx *ctx)
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->recip_stream) (void)hipStreamSynchronize(c->recip_stream);
+    if (c->ens) ensemble_free(c->ens);
+    if (c->ens_multi) ensemble_free(c->ens_multi);
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
                    c->d_vdw_direct, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
-                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev};
+                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev,
+                   c->snap_d, c->snap_i};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->h_st) (void)hipHostFree(c->h_st);
@@ -831,93 +947,39 @@ int dlpx_md_load(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms, c
     return 0;
 }
 
-// What a captured step graph bakes in (buffers, sizes, launch shapes, the
-// time step): a change of any of them forces a new capture.
-static std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
+// Room for nsteps per-step records of the two-body output.
+static int energies_room(Ctx *c, int nsteps)
 {
-    uint64_t t;
-    memcpy(&t, &tstep, sizeof(t));
-    return {(uint64_t)c->pos,      (uint64_t)c->vel,     (uint64_t)c->frc,     (uint64_t)c->frc_recip,
-            (uint64_t)c->xref,     (uint64_t)c->typ,     (uint64_t)c->nbr,     (uint64_t)c->cnt,
-            (uint64_t)c->cell_start, (uint64_t)c->red,   (uint64_t)c->red_out, (uint64_t)c->red_cap,
-            (uint64_t)c->Q,        (uint64_t)c->Phi,     (uint64_t)c->H,       (uint64_t)c->d_vdw,   (uint64_t)c->d_erfc,
-            (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
-            (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
-            (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total,
-            (uint64_t)c->frz,      (uint64_t)c->nfrozen, t};
-}
-
-// Capture forces (both streams), stage 2 of this step and stage 1 of the
-// next into one executable graph.  Only called after an eager step, so every
-// buffer is sized and every kernel attribute set before the capture.
-static int capture_step_graph(Ctx *c, double tstep)
-{
-    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
-    c->step_graph = nullptr;
-    DLPX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int r = evaluate(c, true);
-    if (!r) r = nve_second(c, tstep, true);
-    if (!r) r = nve_first(c, tstep);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (r) {
-        if (g) (void)hipGraphDestroy(g);
-        return r;
-    }
-    if (e != hipSuccess) return fail(c, DLPX_ERR_HIP + (int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) {
-        c->step_graph = nullptr;
-        return fail(c, DLPX_ERR_HIP + (int)ei, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-    }
-    c->graph_key = step_graph_key(c, tstep);
+    if (nsteps <= c->energies_cap) return 0;
+    if (c->energies_dev) (void)hipFree(c->energies_dev);
+    c->energies_dev = nullptr;
+    c->energies_cap = 0;
+    DLPX_HIP(c, hipMalloc((void **)&c->energies_dev, sizeof(double) * kReduceSlots * (size_t)nsteps));
+    c->energies_cap = nsteps;
     return 0;
 }
 
-// Start of a resident run: clear the counters of the fused stage-1 kernel
-// and do stage 1 of the first step.
-static int md_begin(Ctx *c, int nsteps, double tstep)
+// The per-step records of a run as (engcpe, vircpe, engsrp, virsrp, kinetic)
+// per step; the last step's output is kept for dlpx_md_last_output.
+static int collect_energies(Ctx *c, int nsteps, double *energies)
 {
-    // displacement maximum and workgroup counter of the fused stage-1 kernel
-    DLPX_HIP(c, hipMemsetAsync(c->flags + 4, 0, sizeof(int32_t) * 4, c->stream));
-    return nsteps > 0 ? nve_first(c, tstep) : 0;
-}
-
-// Step s of a resident run (nve.F90:127-215 with refresh_mappings/vnl_check
-// and the force call, drivers.F90:2105-2199): wait for this context's stage 1,
-// rebuild the list if an atom moved by padding/2, then forces, stage 2 and
-// the next step's stage 1.  With graphs, steps 2..n-1 replay one captured
-// graph.  Returns with the step's work queued on c->stream.
-static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool energies, int &rebuilds)
-{
-    int r;
-    int update = 1;
-    if (c->prm.padding > 0.0) {
-        DLPX_HIP(c, hipStreamSynchronize(c->stream));
-        update = nve_first_displacement(c) >= kHalfMinus * c->prm.padding ? 1 : 0;
+    if (nsteps <= 0) return 0;
+    std::vector<double> red((size_t)kReduceSlots * nsteps);
+    DLPX_HIP(c, hipMemcpy(red.data(), c->energies_dev, sizeof(double) * red.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < nsteps; s++) {
+        dlpx_two_body_out o;
+        assemble(c, red.data() + (size_t)kReduceSlots * s, &o);
+        double *e = energies + (size_t)5 * s;
+        e[0] = o.engcpe;
+        e[1] = o.vircpe;
+        e[2] = o.engsrp;
+        e[3] = o.virsrp;
+        e[4] = 0.5 * red[(size_t)kReduceSlots * s + R_KIN];
+        if (s == nsteps - 1) {
+            c->last_out = o;
+            c->have_last_out = true;
+        }
     }
-    if (update) {
-        if ((r = build_list(c))) return r;
-        rebuilds++;
-    }
-    const bool last = s == nsteps - 1;
-    if (graphs && s > 0 && !last) {
-        // forces + stage 2 of this step + stage 1 of the next, as one graph
-        if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
-            if ((r = capture_step_graph(c, tstep))) return r;
-        DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
-        return 0;
-    }
-    if ((r = evaluate(c, true))) return r;
-    if ((r = nve_second(c, tstep, true))) return r;
-    if (c->timing && (r = collect_timing(c))) return r;
-    if (energies) {
-        if ((r = reduce_partials(c))) return r;
-        DLPX_HIP(c, hipMemcpyAsync(c->energies_dev + (size_t)kReduceSlots * s, c->red_out,
-                                   sizeof(double) * kReduceSlots, hipMemcpyDeviceToDevice, c->stream));
-    }
-    if (!last && (r = nve_first(c, tstep))) return r;
     return 0;
 }
 
@@ -927,37 +989,25 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
     if (!c || nsteps < 0) return DLPX_ERR_BAD_ARG;
     if (!c->resident || !c->have_list) return fail(c, DLPX_ERR_STATE, "call dlpx_md_load first");
     int r, rebuilds = 0;
-    if (energies && nsteps > c->energies_cap) {
-        if (c->energies_dev) (void)hipFree(c->energies_dev);
-        c->energies_dev = nullptr;
-        DLPX_HIP(c, hipMalloc((void **)&c->energies_dev, sizeof(double) * kReduceSlots * (size_t)nsteps));
-        c->energies_cap = nsteps;
+    if (energies && (r = energies_room(c, nsteps))) return r;
+    c->run_dt = tstep;
+    if (c->use_graphs && !c->timing) {
+        // device-decided rebuilds and captured steps: a batch of one
+        std::vector<Ctx *> cs{c};
+        std::vector<int> reb;
+        Ensemble *E = reinterpret_cast<Ensemble *>(c->ens);
+        r = ensemble_run(cs, E, nsteps, energies != nullptr, reb);
+        c->ens = E;
+        if (r) return r;
+        rebuilds = reb.empty() ? 0 : reb[0];
+    } else {
+        // stage timing (or DLPX_GRAPHS=0): eager steps, host-decided rebuilds
+        if ((r = md_begin(c, nsteps, tstep))) return r;
+        for (int s = 0; s < nsteps; s++)
+            if ((r = md_step(c, s, nsteps, tstep, false, energies != nullptr, rebuilds))) return r;
+        DLPX_HIP(c, hipStreamSynchronize(c->stream));
     }
-    // per-step energies and stage timing need host work between the kernels
-    // of a step; otherwise steps after the first replay one captured graph
-    const bool graphs = c->use_graphs && !energies && !c->timing;
-    if ((r = md_begin(c, nsteps, tstep))) return r;
-    for (int s = 0; s < nsteps; s++)
-        if ((r = md_step(c, s, nsteps, tstep, graphs, energies != nullptr, rebuilds))) return r;
-    DLPX_HIP(c, hipStreamSynchronize(c->stream));
-    if (energies && nsteps > 0) {
-        std::vector<double> red((size_t)kReduceSlots * nsteps);
-        DLPX_HIP(c, hipMemcpy(red.data(), c->energies_dev, sizeof(double) * red.size(), hipMemcpyDeviceToHost));
-        for (int s = 0; s < nsteps; s++) {
-            dlpx_two_body_out o;
-            assemble(c, red.data() + (size_t)kReduceSlots * s, &o);
-            double *e = energies + (size_t)5 * s;
-            e[0] = o.engcpe;
-            e[1] = o.vircpe;
-            e[2] = o.engsrp;
-            e[3] = o.virsrp;
-            e[4] = 0.5 * red[(size_t)kReduceSlots * s + R_KIN];
-            if (s == nsteps - 1) {
-                c->last_out = o;
-                c->have_last_out = true;
-            }
-        }
-    }
+    if (energies && (r = collect_energies(c, nsteps, energies))) return r;
     if (nrebuilds) *nrebuilds = rebuilds;
     return 0;
 }
@@ -971,8 +1021,8 @@ int dlpx_md_last_output(dlpx_ctx *ctx, dlpx_two_body_out *out)
     return 0;
 }
 
-int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
-                         int32_t *nrebuilds)
+int dlpx_md_run_ensemble_energies(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
+                                  double *energies, int32_t *nrebuilds)
 {
     if (!ctxs || nctx < 0 || nsteps < 0 || !tsteps) return DLPX_ERR_BAD_ARG;
     std::vector<Ctx *> cs(nctx);
@@ -984,24 +1034,41 @@ int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, co
         for (int j = 0; j < k; j++)
             if (cs[j] == cs[k]) return fail(cs[k], DLPX_ERR_BAD_ARG, "a context appears twice in the ensemble");
     }
+    if (nctx == 0) return 0;
     std::vector<int> rebuilds(nctx, 0);
     int r;
-    for (int k = 0; k < nctx; k++)
-        if ((r = md_begin(cs[k], nsteps, tsteps[k]))) return r;
-    // lockstep over systems: while the host waits for one system's stage 1,
-    // the steps queued for the others run on the GPU
-    for (int s = 0; s < nsteps; s++)
-        for (int k = 0; k < nctx; k++) {
-            Ctx *c = cs[k];
-            const bool graphs = c->use_graphs && !c->timing;
-            if ((r = md_step(c, s, nsteps, tsteps[k], graphs, false, rebuilds[k]))) return r;
-        }
+    bool batched = true;
     for (int k = 0; k < nctx; k++) {
-        Ctx *c = cs[k];
-        DLPX_HIP(c, hipStreamSynchronize(c->stream));
+        cs[k]->run_dt = tsteps[k];
+        if (energies && (r = energies_room(cs[k], nsteps))) return r;
+        batched = batched && cs[k]->use_graphs && !cs[k]->timing && cs[k]->prm.precision == cs[0]->prm.precision;
+    }
+    if (batched) {
+        // every stage one launch over all systems, steps replaying one graph
+        Ensemble *E = reinterpret_cast<Ensemble *>(cs[0]->ens_multi);
+        r = ensemble_run(cs, E, nsteps, energies != nullptr, rebuilds);
+        cs[0]->ens_multi = E;
+        if (r) return r;
+    } else {
+        // eager lockstep over systems with host-decided rebuilds
+        for (int k = 0; k < nctx; k++)
+            if ((r = md_begin(cs[k], nsteps, tsteps[k]))) return r;
+        for (int s = 0; s < nsteps; s++)
+            for (int k = 0; k < nctx; k++)
+                if ((r = md_step(cs[k], s, nsteps, tsteps[k], false, energies != nullptr, rebuilds[k]))) return r;
+        for (int k = 0; k < nctx; k++) DLPX_HIP(cs[k], hipStreamSynchronize(cs[k]->stream));
+    }
+    for (int k = 0; k < nctx; k++) {
+        if (energies && (r = collect_energies(cs[k], nsteps, energies + (size_t)5 * nsteps * k))) return r;
         if (nrebuilds) nrebuilds[k] = rebuilds[k];
     }
     return 0;
+}
+
+int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
+                         int32_t *nrebuilds)
+{
+    return dlpx_md_run_ensemble_energies(ctxs, nctx, nsteps, tsteps, nullptr, nrebuilds);
 }
 
 int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy, double *vz, double *fx,

@@ -37,6 +37,55 @@ __device__ __forceinline__ int xcd_block(int b, int nb_launched)
     return (b % kXcds) * (nb_launched / kXcds) + b / kXcds;
 }
 
+// Batched launches: the workgroups of one launch serve many independent
+// systems.  boff (device memory) holds nsys+1 non-decreasing workgroup
+// offsets; logical workgroup lb belongs to the system s with
+// boff[s] <= lb < boff[s+1] and is that system's workgroup lb - boff[s].  The
+// search runs on wavefront-uniform values, so it is scalar code.  A kernel
+// body written for one system takes its arguments as a struct plus its
+// logical workgroup index and count; a single-system kernel passes its own
+// kernel arguments, a batched one the struct of the system found here.
+__device__ __forceinline__ int batch_find(const int32_t *__restrict__ boff, int nsys, int lb)
+{
+    int lo = 0, hi = nsys;  // boff[lo] <= lb < boff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (boff[mid] <= lb) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The (system, workgroup, workgroup count) of a batched launch of
+// xcd_grid(boff[nsys]) workgroups, XCD-aware over the whole launch; false for
+// the padding workgroups past the last system.
+struct BatchPos {
+    int s, b, nb;
+};
+__device__ __forceinline__ bool batch_pos(const int32_t *__restrict__ boff, int nsys, BatchPos &bp)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    if (lb >= boff[nsys]) return false;
+    bp.s = batch_find(boff, nsys, lb);
+    bp.b = lb - boff[bp.s];
+    bp.nb = boff[bp.s + 1] - boff[bp.s];
+    return true;
+}
+
+// Words of Ctx::flags (int32, device).
+enum Flag : int {
+    F_OVF = 0,      // largest per-lane count of an overflowing list build (0: none)
+    F_IBIG = 1,     // largest per-atom count of the last build
+    F_OVF_ANY = 2,  // sticky overflow of any device-decided build in a resident run
+    F_PAIRS = 3,    // stored entries of the last build
+    F_DMAX = 4,     // [4..5] displacement maximum (double bits), stage 1
+    F_DONE = 6,     // workgroups of stage 1 finished
+    F_UPD = 7,      // 1: the list is rebuilt before the coming force evaluation
+    F_NREB = 8,     // rebuilds decided on the device since the run began
+    F_STEP = 9,     // per-step energy records written since the run began
+    F_WORDS = 16
+};
+
 // Verlet list layout: atom-major, the G lanes of one atom interleaved, so the
 // entries of one atom (and of one workgroup's atoms) are contiguous in HBM.
 // Lane g of atom i holds its c-th neighbour at this index.
@@ -194,6 +243,14 @@ struct Ctx {
     double *pp = nullptr;            // kPPStride doubles per atom (sorted order), see pair.hip
     int pp_cap = 0;
 
+    // resident runs
+    double run_dt = 0.0;             // time step of the run under way
+    uint64_t serial = 0;             // unique per context in the process (ensemble cache keys)
+    void *ens = nullptr;             // this context's own batched step (an Ensemble of one)
+    void *ens_multi = nullptr;       // the last ensemble run led by this context
+    int32_t *snap_i = nullptr;       // device copies of the state at the start of a run
+    double4 *snap_d = nullptr;       // (restored if a device-decided list build overflows)
+
     // timing
     bool timing = false;
     bool timing_pending = false;
@@ -201,6 +258,50 @@ struct Ctx {
     hipEvent_t ev[8]{};
     double stage_ms[8]{};      // pair, spread, fft+influence, gather, finish, list build
 };
+
+// One batched launch (see batch_find): a device array of nsys per-system
+// argument structs, the systems' workgroup offsets, the launch's workgroup
+// count (rounded up to a multiple of the XCD count when xcd) and its dynamic
+// LDS bytes.  Owned by the ensemble that prepared it.
+struct BatchLaunch {
+    void *args = nullptr;
+    int32_t *boff = nullptr;
+    int nsys = 0, total = 0;
+    size_t lds = 0;
+};
+// the batched kernels of one list build (neighbours.hip)
+struct ListBatch {
+    BatchLaunch wrap, scan, scat, sort, perm, copy, bsub, blink, excl;
+};
+// the batched passes of one SPME convolution (fft.hip) and of the
+// reciprocal-space stage (spme.hip)
+struct FftBatch {
+    BatchLaunch r2c, yf, zc, yb, c2r;
+    int lpb_x = 0, lpb_y = 0, lpb_z = 0;
+    int lpb_cap = 16;            // largest lines per workgroup to use (tuning knob)
+    std::vector<int> conv_rows;  // z-pass workgroups (reduction rows) per system
+};
+struct SpmeBatch {
+    BatchLaunch spread, gather;
+    FftBatch fft;
+    int S = 0;                     // spline order (0: no system has electrostatics)
+    std::vector<int> gather_rows;  // gather workgroups per system
+};
+// the pair kernel's launches, one per template variant present
+struct PairBatch {
+    struct Group {
+        BatchLaunch bl;
+        int gp = 8;
+        bool fs = false, direct = false, pp = false, gt = false;
+    };
+    std::vector<Group> groups;
+};
+struct RedRowsHost {
+    int rows[kReduceSlots];
+};
+int batch_upload(Ctx *c, BatchLaunch &bl, const void *args, size_t bytes_each, const std::vector<int> &nblocks,
+                 bool xcd);
+void batch_free(BatchLaunch &bl);
 
 // error helpers
 int fail(Ctx *c, int code, const std::string &msg);
@@ -275,6 +376,31 @@ struct InflParams {
 // with clear_in the forward pass leaves in all zero.
 int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in);
 int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
+
+// batched stages (one launch per stage over the systems of an ensemble)
+int vv1_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl);
+int vv1_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
+int vv2_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl);
+int vv2_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
+int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb);
+int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb);
+void list_batch_free(ListBatch &lb);
+int pair_prepare(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::vector<int> &rows);
+int pair_launch(Ctx *c, hipStream_t s, const PairBatch &pb);
+int pair_blocks_exact(const Ctx *c);
+int excl_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl);
+int excl_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
+InflParams infl_params(const Ctx *c);
+int fft_prepare(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips, FftBatch &fb);
+int fft_launch(Ctx *c, hipStream_t s, const FftBatch &fb);
+int spme_prepare(const std::vector<Ctx *> &cs, SpmeBatch &sb);
+int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb);
+int reduce_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
+                   BatchLaunch &bl, int32_t **flag_ptrs);
+int reduce_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
+int record_next_launch(Ctx *c, hipStream_t s, int32_t *const *flag_ptrs, int nsys);
+int vv2_blocks(const Ctx *c);
+int excl_blocks(const Ctx *c);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

@@ -270,22 +270,30 @@ __device__ __forceinline__ void load_twiddles(const T2 *tw, int n, T2 *stw)
 }
 
 // Complex-to-complex transforms of strided lines (the y axis), in place.
+template <typename T>
+struct C2CArgs {
+    typename C2T<T>::type *data;
+    LineMap lm;
+    FftPlan pl;
+    const typename C2T<T>::type *tw;
+    int sign;
+};
 template <typename T, int LPB>
-__global__ void __launch_bounds__(kFftThreads) k_fft_c2c(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
-                                                         const typename C2T<T>::type *__restrict__ tw, int sign)
+__device__ __forceinline__ void c2c_body(const C2CArgs<T> &a, int bid)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = pl.n, ns = n + 1;
+    const LineMap &lm = a.lm;
+    const int n = a.pl.n, ns = n + 1;
     T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
-    const int L0 = blockIdx.x * LPB, nl = min(LPB, lm.nl - L0);
-    load_twiddles(tw, n, stw);
-    load_lines<T2, LPB>(data, lm, n, L0, nl, b0);
+    const int L0 = bid * LPB, nl = min(LPB, lm.nl - L0);
+    load_twiddles(a.tw, n, stw);
+    load_lines<T2, LPB>(a.data, lm, n, L0, nl, b0);
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
     const int lw = threadIdx.x / TPL;
-    T2 *res = stockham<T, T2>(b0, b1, pl, stw, sign, lw, lw < nl && line_ok(lm, L0 + lw), threadIdx.x % TPL, TPL);
-    store_lines<T2, LPB>(data, lm, n, L0, nl, res);
+    T2 *res = stockham<T, T2>(b0, b1, a.pl, stw, a.sign, lw, lw < nl && line_ok(lm, L0 + lw), threadIdx.x % TPL, TPL);
+    store_lines<T2, LPB>(a.data, lm, n, L0, nl, res);
 }
 
 constexpr int kConvRedBytes = 6 * (kFftThreads / 64) * sizeof(double);
@@ -294,20 +302,31 @@ constexpr int kConvRedBytes = 6 * (kFftThreads / 64) * sizeof(double);
 // each z line, multiply by theta(m) (ewald_spole.F90:1296-1380) while summing
 // the reciprocal stress, inverse transform, all in LDS.  Line L is the column
 // (kx, ky) = (a, b) of the half spectrum (see LineMap).
+template <typename T>
+struct ZConvArgs {
+    typename C2T<T>::type *data;
+    LineMap lm;
+    FftPlan pl;
+    const typename C2T<T>::type *tw;
+    InflParams ip;
+    const double *n2;
+    double *red;
+};
 template <typename T, int LPB>
-__global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::type *data, LineMap lm, FftPlan pl,
-                                                            const typename C2T<T>::type *__restrict__ tw,
-                                                            InflParams ip, const double *__restrict__ n2,
-                                                            double *__restrict__ red)
+__device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     double *lred = (double *)smem;  // 6 values x wavefronts
+    const LineMap &lm = a.lm;
+    const FftPlan &pl = a.pl;
+    const InflParams &ip = a.ip;
+    const double *__restrict__ n2 = a.n2;
     const int n = pl.n, ns = n + 1;
     T2 *b0 = (T2 *)(smem + kConvRedBytes), *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
-    const int L0 = blockIdx.x * LPB, nl = min(LPB, lm.nl - L0);
-    load_twiddles(tw, n, stw);
-    load_lines<T2, LPB>(data, lm, n, L0, nl, b0);
+    const int L0 = bid * LPB, nl = min(LPB, lm.nl - L0);
+    load_twiddles(a.tw, n, stw);
+    load_lines<T2, LPB>(a.data, lm, n, L0, nl, b0);
     __syncthreads();
     constexpr int TPL = kFftThreads / LPB;
     const int lw = threadIdx.x / TPL, jt = threadIdx.x % TPL;
@@ -389,28 +408,41 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_z_conv(typename C2T<T>::typ
     }
     __syncthreads();
     T2 *res2 = stockham<T, T2>(res, res == b0 ? b1 : b0, pl, stw, -1, lw, active, jt, TPL);
-    store_lines<T2, LPB>(data, lm, n, L0, nl, res2);
-    if (ip.mode == 0) block_sum<6>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_RS11);
+    store_lines<T2, LPB>(a.data, lm, n, L0, nl, res2);
+    if (ip.mode == 0) block_sum<6>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_RS11);
 }
 
 // x axis, forward: real lines 2P and 2P+1 of Q (n contiguous values each)
 // packed as one complex line; output half spectra to H (K1h of a row of hs).
 // With clear, every grid value is set to zero once read (each is read by one
 // thread), so the charge grid is ready for the next spreading.
+template <typename T>
+struct R2CArgs {
+    T *Q;
+    typename C2T<T>::type *H;
+    int nreal_lines, K1h, hs;
+    FftPlan pl;
+    const typename C2T<T>::type *tw;
+    FastDiv div_n, div_h;
+    int clear;
+};
 template <typename T, int LPB>
-__global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(T *Q, typename C2T<T>::type *H,
-                                                           int nreal_lines, int K1h, int hs, FftPlan pl,
-                                                           const typename C2T<T>::type *__restrict__ tw,
-                                                           FastDiv div_n, FastDiv div_h, bool clear)
+__device__ __forceinline__ void r2c_body(const R2CArgs<T> &a, int bid)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
+    T *__restrict__ Q = a.Q;
+    T2 *__restrict__ H = a.H;
+    const int nreal_lines = a.nreal_lines, K1h = a.K1h, hs = a.hs;
+    const FftPlan &pl = a.pl;
+    const FastDiv div_n = a.div_n, div_h = a.div_h;
+    const bool clear = a.clear != 0;
     const int n = pl.n, ns = n + 1;
     T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
     const int npairs = (nreal_lines + 1) / 2;
-    const int P0 = blockIdx.x * LPB;
+    const int P0 = bid * LPB;
     const int np = min(LPB, npairs - P0);
-    load_twiddles(tw, n, stw);
+    load_twiddles(a.tw, n, stw);
     for (int i0 = threadIdx.x; i0 < np * n; i0 += kLoadBatch * kFftThreads) {
         T2 z[kLoadBatch];
 #pragma unroll
@@ -469,20 +501,31 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_r2c(T *Q, typename C2T<T>
 
 // x axis, inverse: rebuild the full Hermitian line of two half spectra,
 // transform with sign -1 and write the real parts to Q.
+template <typename T>
+struct C2RArgs {
+    const typename C2T<T>::type *H;
+    T *Q;
+    int nreal_lines, K1h, hs;
+    FftPlan pl;
+    const typename C2T<T>::type *tw;
+    FastDiv div_n;
+};
 template <typename T, int LPB>
-__global__ void __launch_bounds__(kFftThreads) k_fft_x_c2r(const typename C2T<T>::type *__restrict__ H, T *Q,
-                                                           int nreal_lines, int K1h, int hs, FftPlan pl,
-                                                           const typename C2T<T>::type *__restrict__ tw,
-                                                           FastDiv div_n)
+__device__ __forceinline__ void c2r_body(const C2RArgs<T> &a, int bid)
 {
     using T2 = typename C2T<T>::type;
     extern __shared__ __align__(16) unsigned char smem[];
+    const T2 *__restrict__ H = a.H;
+    T *__restrict__ Q = a.Q;
+    const int nreal_lines = a.nreal_lines, K1h = a.K1h, hs = a.hs;
+    const FftPlan &pl = a.pl;
+    const FastDiv div_n = a.div_n;
     const int n = pl.n, ns = n + 1;
     T2 *b0 = (T2 *)smem, *b1 = b0 + LPB * ns, *stw = b1 + LPB * ns;
     const int npairs = (nreal_lines + 1) / 2;
-    const int P0 = blockIdx.x * LPB;
+    const int P0 = bid * LPB;
     const int np = min(LPB, npairs - P0);
-    load_twiddles(tw, n, stw);
+    load_twiddles(a.tw, n, stw);
     for (int i0 = threadIdx.x; i0 < np * n; i0 += kLoadBatch * kFftThreads) {
         T2 A[kLoadBatch], B[kLoadBatch];
 #pragma unroll
@@ -534,6 +577,30 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_x_c2r(const typename C2T<T>
     }
 }
 
+// Single-system kernels (one 3D grid) and batched ones (the grids of many
+// systems, one argument struct each, see batch_find).  FFT workgroups take
+// lines in grid order: no XCD renumbering.
+#define DLPX_FFT_KERNELS(NAME, ARGS, BODY)                                                                 \
+    template <typename T, int LPB>                                                                         \
+    __global__ void __launch_bounds__(kFftThreads) NAME(ARGS<T> a)                                         \
+    {                                                                                                      \
+        BODY<T, LPB>(a, blockIdx.x);                                                                       \
+    }                                                                                                      \
+    template <typename T, int LPB>                                                                         \
+    __global__ void __launch_bounds__(kFftThreads) NAME##_batch(const ARGS<T> *__restrict__ av,            \
+                                                                const int32_t *__restrict__ boff, int nsys) \
+    {                                                                                                      \
+        const int lb = blockIdx.x;                                                                         \
+        if (lb >= boff[nsys]) return;                                                                      \
+        const int s = batch_find(boff, nsys, lb);                                                          \
+        BODY<T, LPB>(av[s], lb - boff[s]);                                                                 \
+    }
+DLPX_FFT_KERNELS(k_fft_c2c, C2CArgs, c2c_body)
+DLPX_FFT_KERNELS(k_fft_z_conv, ZConvArgs, z_conv_body)
+DLPX_FFT_KERNELS(k_fft_x_r2c, R2CArgs, r2c_body)
+DLPX_FFT_KERNELS(k_fft_x_c2r, C2RArgs, c2r_body)
+#undef DLPX_FFT_KERNELS
+
 // Lines per workgroup: the largest of 16, 8, 4, 2, 1 whose two Stockham
 // buffers and twiddles fit 72 KB of LDS (two workgroups per CU); 16 lines give
 // 128 B (FP32) or 256 B (FP64) contiguous runs in the y and z passes.
@@ -560,31 +627,31 @@ static int pick_lpb(int n, size_t extra)
         default: CALL(1); break;   \
     }
 
+// Lines per workgroup of each pass for one context (0: a length too large
+// for LDS).
 template <typename T>
-static int run_y(Ctx *c, int sign)
+static int xy_lpb(int n)
+{
+    return pick_lpb<T>(n, 0);
+}
+
+template <typename T>
+static C2CArgs<T> y_args(const Ctx *c, int sign, int lpb)
 {
     using T2 = typename C2T<T>::type;
     const int K1h = c->K1h, K2 = c->K[1], K3 = c->K[2];
-    const int lpb = pick_lpb<T>(K2, 0);
-    if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
-    LineMap lm;
-    lm.na = K1h;
-    lm.na_pad = (K1h + lpb - 1) / lpb * lpb;
-    lm.nl = lm.na_pad * K3;
-    lm.sa = 1;
-    lm.sb = (long)c->K1hp * K2;
-    lm.se = c->K1hp;
-    const FftPlan pl = make_plan(K2);
-    const T2 *tw = (const T2 *)c->tw + c->K[0];
-    const size_t lds = fft_lds<T>(lpb, K2, 0);
-    const int nb = ceil_div(lm.nl, lpb);
-#define DLPX_Y(L)                                                                                                   \
-    DLPX_LDS_MAX(c, (k_fft_c2c<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw, sign)
-    DLPX_LPB_SWITCH(lpb, DLPX_Y)
-#undef DLPX_Y
-    DLPX_HIP(c, hipGetLastError());
-    return 0;
+    C2CArgs<T> a;
+    a.lm.na = K1h;
+    a.lm.na_pad = (K1h + lpb - 1) / lpb * lpb;
+    a.lm.nl = a.lm.na_pad * K3;
+    a.lm.sa = 1;
+    a.lm.sb = (long)c->K1hp * K2;
+    a.lm.se = c->K1hp;
+    a.pl = make_plan(K2);
+    a.tw = (const T2 *)c->tw + c->K[0];
+    a.data = (T2 *)c->H;
+    a.sign = sign;
+    return a;
 }
 
 static LineMap z_lines(const Ctx *c, int lpb)
@@ -606,22 +673,86 @@ static int z_lpb(const Ctx *c)
 }
 
 template <typename T>
-static int run_z_conv(Ctx *c, const InflParams &ip)
+static ZConvArgs<T> z_args(const Ctx *c, const InflParams &ip, int lpb)
 {
     using T2 = typename C2T<T>::type;
-    const int K3 = c->K[2];
-    const size_t extra = kConvRedBytes;
+    ZConvArgs<T> a;
+    a.lm = z_lines(c, lpb);
+    a.pl = make_plan(c->K[2]);
+    a.tw = (const T2 *)c->tw + c->K[0] + c->K[1];
+    a.data = (T2 *)c->H;
+    a.ip = ip;
+    a.n2 = c->bsp_n2;
+    a.red = c->red;
+    return a;
+}
+
+template <typename T>
+static R2CArgs<T> r2c_args(const Ctx *c, void *grid, bool clear)
+{
+    using T2 = typename C2T<T>::type;
+    R2CArgs<T> a;
+    const int n = c->K[0];
+    a.Q = (T *)grid;
+    a.H = (T2 *)c->H;
+    a.nreal_lines = c->K[1] * c->K[2];
+    a.K1h = c->K1h;
+    a.hs = c->K1hp;
+    a.pl = make_plan(n);
+    a.tw = (const T2 *)c->tw;
+    a.div_n = make_div(n);
+    a.div_h = make_div(c->K1h);
+    a.clear = clear ? 1 : 0;
+    return a;
+}
+
+template <typename T>
+static C2RArgs<T> c2r_args(const Ctx *c, void *grid)
+{
+    using T2 = typename C2T<T>::type;
+    C2RArgs<T> a;
+    const int n = c->K[0];
+    a.H = (const T2 *)c->H;
+    a.Q = (T *)grid;
+    a.nreal_lines = c->K[1] * c->K[2];
+    a.K1h = c->K1h;
+    a.hs = c->K1hp;
+    a.pl = make_plan(n);
+    a.tw = (const T2 *)c->tw;
+    a.div_n = make_div(n);
+    return a;
+}
+
+static int x_blocks(const Ctx *c, int lpb) { return ceil_div((c->K[1] * c->K[2] + 1) / 2, lpb); }
+
+template <typename T>
+static int run_y(Ctx *c, int sign)
+{
+    const int lpb = xy_lpb<T>(c->K[1]);
+    if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    const C2CArgs<T> a = y_args<T>(c, sign, lpb);
+    const size_t lds = fft_lds<T>(lpb, c->K[1], 0);
+    const int nb = ceil_div(a.lm.nl, lpb);
+#define DLPX_Y(L)                                                                                                   \
+    DLPX_LDS_MAX(c, (k_fft_c2c<T, L>));                                                                     \
+    hipLaunchKernelGGL((k_fft_c2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, a)
+    DLPX_LPB_SWITCH(lpb, DLPX_Y)
+#undef DLPX_Y
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int run_z_conv(Ctx *c, const InflParams &ip)
+{
     const int lpb = z_lpb<T>(c);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
-    const LineMap lm = z_lines(c, lpb);
-    const FftPlan pl = make_plan(K3);
-    const T2 *tw = (const T2 *)c->tw + c->K[0] + c->K[1];
-    const size_t lds = fft_lds<T>(lpb, K3, extra);
-    const int nb = ceil_div(lm.nl, lpb);
+    const ZConvArgs<T> a = z_args<T>(c, ip, lpb);
+    const size_t lds = fft_lds<T>(lpb, c->K[2], kConvRedBytes);
+    const int nb = ceil_div(a.lm.nl, lpb);
 #define DLPX_Z(L)                                                                                                   \
     DLPX_LDS_MAX(c, (k_fft_z_conv<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T2 *)c->H, lm, pl, tw,   \
-                       ip, c->bsp_n2, c->red)
+    hipLaunchKernelGGL((k_fft_z_conv<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, a)
     DLPX_LPB_SWITCH(lpb, DLPX_Z)
 #undef DLPX_Z
     if (ip.mode == 0)
@@ -633,27 +764,23 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
 template <typename T>
 static int run_x(Ctx *c, bool forward, void *grid, bool clear = false)
 {
-    using T2 = typename C2T<T>::type;
-    const int n = c->K[0], nlines = c->K[1] * c->K[2];
-    const int lpb = pick_lpb<T>(n, 0);
+    const int n = c->K[0];
+    const int lpb = xy_lpb<T>(n);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
-    const int npairs = (nlines + 1) / 2;
-    const FftPlan pl = make_plan(n);
     const size_t lds = fft_lds<T>(lpb, n, 0);
-    const int nb = ceil_div(npairs, lpb);
-    const FastDiv dn = make_div(n), dh = make_div(c->K1h);
+    const int nb = x_blocks(c, lpb);
     if (forward) {
+        const R2CArgs<T> a = r2c_args<T>(c, grid, clear);
 #define DLPX_R2C(L)                                                                                                 \
     DLPX_LDS_MAX(c, (k_fft_x_r2c<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (T *)grid,                 \
-                       (T2 *)c->H, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn, dh, clear)
+    hipLaunchKernelGGL((k_fft_x_r2c<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, a)
         DLPX_LPB_SWITCH(lpb, DLPX_R2C)
 #undef DLPX_R2C
     } else {
+        const C2RArgs<T> a = c2r_args<T>(c, grid);
 #define DLPX_C2R(L)                                                                                                 \
     DLPX_LDS_MAX(c, (k_fft_x_c2r<T, L>));                                                                     \
-    hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, (const T2 *)c->H,          \
-                       (T *)grid, nlines, c->K1h, c->K1hp, pl, (const T2 *)c->tw, dn)
+    hipLaunchKernelGGL((k_fft_x_c2r<T, L>), dim3(nb), dim3(kFftThreads), lds, c->sp, a)
         DLPX_LPB_SWITCH(lpb, DLPX_C2R)
 #undef DLPX_C2R
     }
@@ -675,7 +802,104 @@ static int convolve_t(Ctx *c, const InflParams &ip, void *in, void *out, bool cl
 int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in)
 {
     return c->prm.precision == DLPX_MIXED ? convolve_t<float>(c, ip, in, out, clear_in)
-                                          : convolve_t<double>(c, ip, in, out, clear_in);
+                                          : convolve_t<double>(c, ip, in, out, clear_in)
+;
+}
+
+// Batched convolution Q -> Phi (influence function mode 0, the forward pass
+// clearing Q) over the systems with electrostatics: one launch per pass
+// over every system's lines, with one lines-per-workgroup value per pass
+// (the smallest any system needs, so every system's lines fit its LDS).
+template <typename T>
+static int fft_prepare_t(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips, FftBatch &fb)
+{
+    const int cap = fb.lpb_cap > 0 ? fb.lpb_cap : 16;
+    int lx = cap, ly = cap, lz = cap;
+    for (Ctx *c : cs) {
+        if (!c->prm.electrostatics) continue;
+        const int a = xy_lpb<T>(c->K[0]), b = xy_lpb<T>(c->K[1]), z = z_lpb<T>(c);
+        if (!a || !b || !z) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+        lx = std::min(lx, a);
+        ly = std::min(ly, b);
+        lz = std::min(lz, z);
+    }
+    std::vector<R2CArgs<T>> r2c;
+    std::vector<C2CArgs<T>> yf, yb;
+    std::vector<ZConvArgs<T>> zc;
+    std::vector<C2RArgs<T>> c2r;
+    std::vector<int> nbx, nby, nbz;
+    size_t ldx = 0, ldy = 0, ldz = 0;
+    for (size_t k = 0; k < cs.size(); k++) {
+        Ctx *c = cs[k];
+        const bool on = c->prm.electrostatics != 0;
+        r2c.push_back(r2c_args<T>(c, c->Q, true));
+        c2r.push_back(c2r_args<T>(c, c->Phi));
+        yf.push_back(y_args<T>(c, +1, ly));
+        yb.push_back(y_args<T>(c, -1, ly));
+        zc.push_back(z_args<T>(c, ips[k], lz));
+        nbx.push_back(on ? x_blocks(c, lx) : 0);
+        nby.push_back(on ? ceil_div(yf.back().lm.nl, ly) : 0);
+        nbz.push_back(on ? ceil_div(zc.back().lm.nl, lz) : 0);
+        if (on) {
+            ldx = std::max(ldx, fft_lds<T>(lx, c->K[0], 0));
+            ldy = std::max(ldy, fft_lds<T>(ly, c->K[1], 0));
+            ldz = std::max(ldz, fft_lds<T>(lz, c->K[2], kConvRedBytes));
+        }
+    }
+    Ctx *c0 = cs[0];
+    int r;
+    if ((r = batch_upload(c0, fb.r2c, r2c.data(), sizeof(R2CArgs<T>), nbx, false)) ||
+        (r = batch_upload(c0, fb.yf, yf.data(), sizeof(C2CArgs<T>), nby, false)) ||
+        (r = batch_upload(c0, fb.zc, zc.data(), sizeof(ZConvArgs<T>), nbz, false)) ||
+        (r = batch_upload(c0, fb.yb, yb.data(), sizeof(C2CArgs<T>), nby, false)) ||
+        (r = batch_upload(c0, fb.c2r, c2r.data(), sizeof(C2RArgs<T>), nbx, false)))
+        return r;
+    fb.r2c.lds = fb.c2r.lds = ldx;
+    fb.yf.lds = fb.yb.lds = ldy;
+    fb.zc.lds = ldz;
+    fb.lpb_x = lx;
+    fb.lpb_y = ly;
+    fb.lpb_z = lz;
+    fb.conv_rows.assign(nbz.begin(), nbz.end());
+    return 0;
+}
+
+int fft_prepare(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips, FftBatch &fb)
+{
+    return cs[0]->prm.precision == DLPX_MIXED ? fft_prepare_t<float>(cs, ips, fb) : fft_prepare_t<double>(cs, ips, fb);
+}
+
+template <typename T>
+static int fft_launch_t(Ctx *c, hipStream_t s, const FftBatch &fb)
+{
+    if (fb.r2c.total == 0) return 0;
+#define DLPX_FB(KERNEL, ARGS, BL, L)                                                                         \
+    DLPX_LDS_MAX(c, (KERNEL<T, L>));                                                                         \
+    hipLaunchKernelGGL((KERNEL<T, L>), dim3((BL).total), dim3(kFftThreads), (BL).lds, s, (const ARGS<T> *)(BL).args, \
+                       (BL).boff, (BL).nsys)
+#define DLPX_FB_R2C(L) DLPX_FB(k_fft_x_r2c_batch, R2CArgs, fb.r2c, L)
+#define DLPX_FB_YF(L) DLPX_FB(k_fft_c2c_batch, C2CArgs, fb.yf, L)
+#define DLPX_FB_Z(L) DLPX_FB(k_fft_z_conv_batch, ZConvArgs, fb.zc, L)
+#define DLPX_FB_YB(L) DLPX_FB(k_fft_c2c_batch, C2CArgs, fb.yb, L)
+#define DLPX_FB_C2R(L) DLPX_FB(k_fft_x_c2r_batch, C2RArgs, fb.c2r, L)
+    DLPX_LPB_SWITCH(fb.lpb_x, DLPX_FB_R2C)
+    DLPX_LPB_SWITCH(fb.lpb_y, DLPX_FB_YF)
+    DLPX_LPB_SWITCH(fb.lpb_z, DLPX_FB_Z)
+    DLPX_LPB_SWITCH(fb.lpb_y, DLPX_FB_YB)
+    DLPX_LPB_SWITCH(fb.lpb_x, DLPX_FB_C2R)
+#undef DLPX_FB_R2C
+#undef DLPX_FB_YF
+#undef DLPX_FB_Z
+#undef DLPX_FB_YB
+#undef DLPX_FB_C2R
+#undef DLPX_FB
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int fft_launch(Ctx *c, hipStream_t s, const FftBatch &fb)
+{
+    return c->prm.precision == DLPX_MIXED ? fft_launch_t<float>(c, s, fb) : fft_launch_t<double>(c, s, fb);
 }
 
 int conv_blocks(const Ctx *c)

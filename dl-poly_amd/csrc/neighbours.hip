@@ -11,6 +11,7 @@
 // atom's entries, and one workgroup's atoms, are contiguous.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
@@ -58,13 +59,27 @@ __device__ __forceinline__ void run_of(int key, int &head, int &len)
     len = min(next, nact) - head;
 }
 
+// Every list-build kernel takes `upd`: null on the host-decided path, else
+// the device word that stage 1 set (F_UPD), and a zero there means no
+// rebuild is due, so the kernel returns at once.  That lets a captured step
+// graph hold the whole rebuild without a host round trip per step.
+__device__ __forceinline__ bool skip_build(const int32_t *upd) { return upd && *upd == 0; }
+
 // Fold positions into the primary cell (relocate_particles / pbcshift) and
 // count atoms per link cell.
-__global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, int32_t *count)
+struct WrapArgs {
+    double4 *pos;
+    int n;
+    CellGrid g;
+    int32_t *cell_of, *count;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void wrap_bin_body(const WrapArgs &a, int bid)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 p = pos[i];
+    const CellGrid &g = a.g;
+    const int i = bid * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double4 p = a.pos[i];
     p.x -= g.L[0] * floor(p.x * g.invL[0] + 0.5);
     p.y -= g.L[1] * floor(p.y * g.invL[1] + 0.5);
     p.z -= g.L[2] * floor(p.z * g.invL[2] + 0.5);
@@ -72,27 +87,36 @@ __global__ void k_wrap_bin(double4 *pos, int n, CellGrid g, int32_t *cell_of, in
     if (p.x >= 0.5 * g.L[0]) p.x -= g.L[0];
     if (p.y >= 0.5 * g.L[1]) p.y -= g.L[1];
     if (p.z >= 0.5 * g.L[2]) p.z -= g.L[2];
-    pos[i] = p;
+    a.pos[i] = p;
     const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.sub * g.nc[0]);
     const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.sub * g.nc[1]);
     const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.sub * g.nc[2]);
     const int c = g.sub == 2 ? subcell_key(g, cx, cy, cz) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
-    cell_of[i] = c;
+    a.cell_of[i] = c;
     // atoms are mostly in cell order already: one atomic per run of equal
     // cells within the wavefront
     int head, len;
     run_of(c, head, len);
-    if ((int)(threadIdx.x & 63) == head) atomicAdd(&count[c], len);
+    if ((int)(threadIdx.x & 63) == head) atomicAdd(&a.count[c], len);
 }
 
 // Exclusive scan of the per-key counts in one workgroup of 1024 threads, each
 // thread taking kScanRun consecutive keys per pass (subcell binning has
-// hundreds of thousands of keys at a million atoms).
+// hundreds of thousands of keys at a million atoms).  The counts are zeroed
+// once read, ready for the next build.  With subcells (cell_start not null)
+// the link-cell starts are the starts of every 8th key.
 constexpr int kScanRun = 8;
-__global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *start, int32_t *fill, int n)
+struct ScanArgs {
+    int32_t *count, *start, *fill;
+    int n;
+    int32_t *cell_start;  // may be null
+    const int32_t *upd;
+};
+__device__ __forceinline__ void scan_body(const ScanArgs &a)
 {
     __shared__ int32_t part[1024];
     __shared__ int32_t carry;
+    const int n = a.n;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < n; base += 1024 * kScanRun) {
@@ -100,9 +124,12 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *st
         int v[kScanRun], sum = 0;
 #pragma unroll
         for (int k = 0; k < kScanRun; k++) {
-            v[k] = i0 + k < n ? count[i0 + k] : 0;
+            v[k] = i0 + k < n ? a.count[i0 + k] : 0;
             sum += v[k];
         }
+#pragma unroll
+        for (int k = 0; k < kScanRun; k++)
+            if (i0 + k < n) a.count[i0 + k] = 0;
         part[threadIdx.x] = sum;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
@@ -115,8 +142,9 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *st
 #pragma unroll
         for (int k = 0; k < kScanRun; k++) {
             if (i0 + k < n) {
-                start[i0 + k] = run;
-                fill[i0 + k] = run;
+                a.start[i0 + k] = run;
+                a.fill[i0 + k] = run;
+                if (a.cell_start && ((i0 + k) & 7) == 0) a.cell_start[(i0 + k) >> 3] = run;
             }
             run += v[k];
         }
@@ -124,45 +152,53 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t *count, int32_t *st
         if (threadIdx.x == 1023) carry += part[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) start[n] = carry;
+    if (threadIdx.x == 0) {
+        a.start[n] = carry;
+        if (a.cell_start) a.cell_start[n >> 3] = carry;
+    }
 }
 
-// Link-cell starts from the subcell starts (8 consecutive keys per cell).
-__global__ void k_block_start(const int32_t *key_start, int32_t *cell_start, int ncell)
+struct ScatterArgs {
+    const int32_t *cell_of;
+    int32_t *fill, *perm;
+    int n;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void scatter_body(const ScatterArgs &a, int bid)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b <= ncell) cell_start[b] = key_start[8 * b];
-}
-
-__global__ void k_scatter(const int32_t *cell_of, int32_t *fill, int32_t *perm, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = cell_of[i];
+    const int i = bid * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int c = a.cell_of[i];
     int head, len;
     run_of(c, head, len);
     const int lane = threadIdx.x & 63;
     int base = 0;
-    if (lane == head) base = atomicAdd(&fill[c], len);
+    if (lane == head) base = atomicAdd(&a.fill[c], len);
     base = __shfl(base, head, 64);
-    perm[base + (lane - head)] = i;
+    a.perm[base + (lane - head)] = i;
 }
 
 // Make the order inside each cell deterministic (ascending previous index):
 // one workgroup per cell, every member's rank is the number of smaller
 // members (indices are distinct), so the sort is one parallel pass in LDS.
 constexpr int kSortLds = 2048;
-__global__ void __launch_bounds__(256) k_sort_cells(int32_t *perm, const int32_t *start, int ncell)
+struct SortArgs {
+    int32_t *perm;
+    const int32_t *start;
+    int nkey;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void sort_cells_body(const SortArgs &a, int c)
 {
     __shared__ int32_t v[kSortLds];
-    const int c = blockIdx.x;
-    const int s = start[c], n = start[c + 1] - s;
+    int32_t *perm = a.perm;
+    const int s = a.start[c], n = a.start[c + 1] - s;
     if (n <= 1) return;
     if (n > kSortLds) {  // a very crowded cell: serial insertion sort
         if (threadIdx.x == 0)
-            for (int a = s + 1; a < s + n; a++) {
-                const int x = perm[a];
-                int b = a - 1;
+            for (int q = s + 1; q < s + n; q++) {
+                const int x = perm[q];
+                int b = q - 1;
                 while (b >= s && perm[b] > x) {
                     perm[b + 1] = perm[b];
                     b--;
@@ -181,34 +217,58 @@ __global__ void __launch_bounds__(256) k_sort_cells(int32_t *perm, const int32_t
     }
 }
 
-__global__ void k_permute(const int32_t *perm, int n, const double4 *pos, double4 *pos_new,
-                          const int32_t *typ, int32_t *typ_new, const int32_t *gid, int32_t *gid_new,
-                          const double4 *vel, double4 *vel_new, int32_t *slot, double4 *xref)
+struct PermArgs {
+    const int32_t *perm;
+    int n;
+    const double4 *pos;
+    double4 *pos_new;
+    const int32_t *typ;
+    int32_t *typ_new;
+    const int32_t *gid;
+    int32_t *gid_new;
+    const double4 *vel;  // may be null
+    double4 *vel_new;
+    int32_t *slot;
+    double4 *xref;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void permute_body(const PermArgs &a, int bid)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int o = perm[i];
-    const double4 p = pos[o];
-    pos_new[i] = p;
-    xref[i] = p;
-    typ_new[i] = typ[o];
-    const int g = gid[o];
-    gid_new[i] = g;
-    slot[g] = i;
-    if (vel) vel_new[i] = vel[o];
+    const int i = bid * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int o = a.perm[i];
+    const double4 p = a.pos[o];
+    a.pos_new[i] = p;
+    a.xref[i] = p;
+    a.typ_new[i] = a.typ[o];
+    const int g = a.gid[o];
+    a.gid_new[i] = g;
+    a.slot[g] = i;
+    if (a.vel) a.vel_new[i] = a.vel[o];
 }
 
 // The permuted arrays back into the primary ones, so device pointers stay
 // the same across rebuilds (captured step graphs keep them as arguments).
-__global__ void k_copy_back(int n, const double4 *pos_alt, double4 *pos, const int32_t *typ_alt, int32_t *typ,
-                            const int32_t *gid_alt, int32_t *gid, const double4 *vel_alt, double4 *vel)
+struct CopyArgs {
+    int n;
+    const double4 *pos_alt;
+    double4 *pos;
+    const int32_t *typ_alt;
+    int32_t *typ;
+    const int32_t *gid_alt;
+    int32_t *gid;
+    const double4 *vel_alt;  // may be null
+    double4 *vel;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void copy_back_body(const CopyArgs &a, int bid)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    pos[i] = pos_alt[i];
-    typ[i] = typ_alt[i];
-    gid[i] = gid_alt[i];
-    if (vel) vel[i] = vel_alt[i];
+    const int i = bid * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    a.pos[i] = a.pos_alt[i];
+    a.typ[i] = a.typ_alt[i];
+    a.gid[i] = a.gid_alt[i];
+    if (a.vel) a.vel[i] = a.vel_alt[i];
 }
 
 // One workgroup per link cell.  The atoms of the (up to) 27 stencil cells
@@ -229,18 +289,44 @@ __global__ void k_copy_back(int n, const double4 *pos_alt, double4 *pos, const i
 // Stencils with more than kListStage candidates are processed in rounds.
 constexpr int kListThreads = 512;
 constexpr int kListStage = 3072;  // 48 KB of float4
+// The workgroup's partial counters into the build's flags (one global
+// atomic each: a single address hit by every atom would serialise tens of
+// thousands of atomics); an overflow also sets the sticky F_OVF_ANY that a
+// resident run checks once at its end.
+__device__ __forceinline__ void build_flags(int32_t *flags, int ovf, int big, int tot)
+{
+    if (ovf) {
+        atomicMax(&flags[F_OVF], ovf);
+        atomicMax(&flags[F_OVF_ANY], ovf);
+    }
+    atomicMax(&flags[F_IBIG], big);
+    atomicAdd(&flags[F_PAIRS], tot);
+}
+
+struct BuildArgs {
+    const double4 *pos;
+    CellGrid g;
+    const int32_t *start;  // link-cell starts (k_build_list) or subcell-key starts (k_build_list_sub)
+    double rc2;
+    int cap_lane;
+    int32_t *nbr, *cnt, *flags;
+    const int32_t *upd;
+};
 template <int G>
-__global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__restrict__ pos, int cap, CellGrid g,
-                                                             const int32_t *__restrict__ start, double rc2,
-                                                             int cap_lane, int32_t *__restrict__ nbr,
-                                                             int32_t *__restrict__ cnt, int32_t *flags)
+__device__ __forceinline__ void build_list_body(const BuildArgs &a, int c)
 {
     static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
     __shared__ float4 cand[kListStage];
     __shared__ int s_start[27], s_off[28];
     __shared__ double s_sh[27][3];
     __shared__ int s_ovf, s_big, s_tot;  // workgroup partials of flags[0], [1], [3]
-    const int c = xcd_block(blockIdx.x, gridDim.x);
+    const double4 *__restrict__ pos = a.pos;
+    const CellGrid &g = a.g;
+    const int32_t *__restrict__ start = a.start;
+    const double rc2 = a.rc2;
+    const int cap_lane = a.cap_lane;
+    int32_t *__restrict__ nbr = a.nbr;
+    int32_t *__restrict__ cnt = a.cnt;
     if (c >= g.nc[0] * g.nc[1] * g.nc[2]) return;
     const int t = threadIdx.x;
     if (t == 0) {
@@ -377,14 +463,8 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
             atomicAdd(&s_tot, tot);
         }
     }
-    // one global atomic per workgroup and flag: a single address hit by every
-    // atom would serialise tens of thousands of atomics
     __syncthreads();
-    if (t == 0) {
-        if (s_ovf) atomicMax(&flags[0], s_ovf);
-        atomicMax(&flags[1], s_big);
-        atomicAdd(&flags[3], s_tot);
-    }
+    if (t == 0) build_flags(a.flags, s_ovf, s_big, s_tot);
 }
 
 // List build over subcells of half the link-cell edge (used when every axis
@@ -407,10 +487,7 @@ __global__ void __launch_bounds__(kListThreads) k_build_list(const double4 *__re
 constexpr int kSubSide = 6, kSubStencil = kSubSide * kSubSide * kSubSide;
 constexpr int kSubAtomChunk = 256;  // atoms of a link cell whose counts persist across staging rounds
 template <int G>
-__global__ void __launch_bounds__(kListThreads) k_build_list_sub(const double4 *__restrict__ pos, CellGrid g,
-                                                                 const int32_t *__restrict__ key_start, double rc2,
-                                                                 int cap_lane, int32_t *__restrict__ nbr,
-                                                                 int32_t *__restrict__ cnt, int32_t *flags)
+__device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
 {
     static_assert(G == kLanesPerAtom, "list_slot assumes kLanesPerAtom lanes per atom");
     __shared__ float4 cand[kListStage];
@@ -419,7 +496,13 @@ __global__ void __launch_bounds__(kListThreads) k_build_list_sub(const double4 *
     __shared__ int s_lo[9];
     __shared__ int s_acnt[kSubAtomChunk];
     __shared__ int s_ovf, s_big, s_tot;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const double4 *__restrict__ pos = a.pos;
+    const CellGrid &g = a.g;
+    const int32_t *__restrict__ key_start = a.start;
+    const double rc2 = a.rc2;
+    const int cap_lane = a.cap_lane;
+    int32_t *__restrict__ nbr = a.nbr;
+    int32_t *__restrict__ cnt = a.cnt;
     if (b >= g.nc[0] * g.nc[1] * g.nc[2]) return;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     constexpr int NW = kListThreads / 64;
@@ -609,11 +692,7 @@ __global__ void __launch_bounds__(kListThreads) k_build_list_sub(const double4 *
         }
     }
     __syncthreads();
-    if (t == 0) {
-        if (s_ovf) atomicMax(&flags[0], s_ovf);
-        atomicMax(&flags[1], s_big);
-        atomicAdd(&flags[3], s_tot);
-    }
+    if (t == 0) build_flags(a.flags, s_ovf, s_big, s_tot);
 }
 
 // vnl_check (neighbours.F90:150-178): largest minimum-image displacement since
@@ -639,40 +718,83 @@ __global__ void k_max_disp(const double4 *pos, const double4 *xref, int n, CellG
 // them past list(0,i) so that the two-body loop never sees them
 // (neighbours.F90:1198-1250: a pair of two frozen atoms, lfrzn /= 0, and
 // every j found in list_excl(:,i)).  Thread t = i*G + g compacts lane g of
-// sorted atom i in place, keeping the order of what stays; flags[3] (the
+// sorted atom i in place, keeping the order of what stays; F_PAIRS (the
 // entry count) loses what was dropped.  excl_off / frz may be null.
-__global__ void __launch_bounds__(256) k_list_exclude(int n, int cap_lane, int32_t *__restrict__ nbr,
-                                                      int32_t *__restrict__ cnt, const int32_t *__restrict__ gid,
-                                                      const int32_t *__restrict__ excl_off,
-                                                      const int32_t *__restrict__ excl_ids,
-                                                      const int32_t *__restrict__ frz, int32_t *flags)
+struct ExclListArgs {
+    int n, cap_lane;
+    int32_t *nbr, *cnt;
+    const int32_t *gid, *excl_off, *excl_ids, *frz;
+    int32_t *flags;
+    const int32_t *upd;
+};
+__device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid)
 {
     constexpr int G = kLanesPerAtom;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bid * blockDim.x + threadIdx.x;
     const int i = t / G, g = t % G;
     int dropped = 0;
-    if (i < n) {
-        const int h = gid[i];
-        const int e0 = excl_off ? excl_off[h] : 0, e1 = excl_off ? excl_off[h + 1] : 0;
-        const bool fi = frz && frz[h] != 0;
+    if (i < a.n) {
+        const int h = a.gid[i];
+        const int e0 = a.excl_off ? a.excl_off[h] : 0, e1 = a.excl_off ? a.excl_off[h + 1] : 0;
+        const bool fi = a.frz && a.frz[h] != 0;
         if (e1 > e0 || fi) {
-            const int m = min(cnt[t], cap_lane);
+            const int m = min(a.cnt[t], a.cap_lane);
             int w = 0;
             for (int k = 0; k < m; k++) {
-                const int j = nbr[list_slot(i, k, g, cap_lane)];
-                const int hj = gid[j];
-                bool ex = fi && frz[hj] != 0;
-                for (int e = e0; e < e1; e++) ex = ex || excl_ids[e] == hj;
-                if (!ex) nbr[list_slot(i, w++, g, cap_lane)] = j;
+                const int j = a.nbr[list_slot(i, k, g, a.cap_lane)];
+                const int hj = a.gid[j];
+                bool ex = fi && a.frz[hj] != 0;
+                for (int e = e0; e < e1; e++) ex = ex || a.excl_ids[e] == hj;
+                if (!ex) a.nbr[list_slot(i, w++, g, a.cap_lane)] = j;
             }
             dropped = m - w;
-            if (cnt[t] <= cap_lane) cnt[t] = w;  // an overflowed lane is rebuilt anyway
+            if (a.cnt[t] <= a.cap_lane) a.cnt[t] = w;  // an overflowed lane is rebuilt anyway
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dropped += __shfl_xor(dropped, o, 64);
-    if ((threadIdx.x & 63) == 0 && dropped) atomicSub(&flags[3], dropped);
+    if ((threadIdx.x & 63) == 0 && dropped) atomicSub(&a.flags[F_PAIRS], dropped);
 }
+
+// Single-system kernels (arguments by value) and batched ones (argument
+// structs from device memory, one per system, see batch_find) around the
+// bodies above; every one first honours the rebuild gate.
+#define DLPX_LIST_KERNELS(NAME, ARGS, NT, CALL_SINGLE, CALL_BATCH, XCD)                                      \
+    __global__ void __launch_bounds__(NT) NAME(ARGS a)                                                    \
+    {                                                                                                     \
+        if (skip_build(a.upd)) return;                                                                    \
+        CALL_SINGLE;                                                                                      \
+    }                                                                                                     \
+    __global__ void __launch_bounds__(NT) NAME##_batch(const ARGS *__restrict__ av,                       \
+                                                       const int32_t *__restrict__ boff, int nsys)        \
+    {                                                                                                     \
+        BatchPos bp;                                                                                      \
+        if (XCD) {                                                                                        \
+            if (!batch_pos(boff, nsys, bp)) return;                                                       \
+        } else {                                                                                          \
+            if ((int)blockIdx.x >= boff[nsys]) return;                                                    \
+            bp.s = batch_find(boff, nsys, blockIdx.x);                                                    \
+            bp.b = blockIdx.x - boff[bp.s];                                                               \
+        }                                                                                                 \
+        const ARGS &a = av[bp.s];                                                                          \
+        if (skip_build(a.upd)) return;                                                                    \
+        CALL_BATCH;                                                                                       \
+    }
+DLPX_LIST_KERNELS(k_wrap_bin, WrapArgs, 256, wrap_bin_body(a, blockIdx.x), wrap_bin_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_scan, ScanArgs, 1024, scan_body(a), scan_body(a), false)
+DLPX_LIST_KERNELS(k_scatter, ScatterArgs, 256, scatter_body(a, blockIdx.x), scatter_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_sort_cells, SortArgs, 256, sort_cells_body(a, blockIdx.x), sort_cells_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_permute, PermArgs, 256, permute_body(a, blockIdx.x), permute_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_copy_back, CopyArgs, 256, copy_back_body(a, blockIdx.x), copy_back_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_list_exclude, ExclListArgs, 256, list_exclude_body(a, blockIdx.x), list_exclude_body(a, bp.b),
+                  false)
+DLPX_LIST_KERNELS(k_build_list, BuildArgs, kListThreads,
+                  build_list_body<kLanesPerAtom>(a, xcd_block(blockIdx.x, gridDim.x)),
+                  build_list_body<kLanesPerAtom>(a, bp.b), true)
+DLPX_LIST_KERNELS(k_build_list_sub, BuildArgs, kListThreads,
+                  build_list_sub_body<kLanesPerAtom>(a, xcd_block(blockIdx.x, gridDim.x)),
+                  build_list_sub_body<kLanesPerAtom>(a, bp.b), true)
+#undef DLPX_LIST_KERNELS
 
 static CellGrid make_grid(const Ctx *c)
 {
@@ -695,8 +817,23 @@ static int grow(Ctx *c, T **p, size_t count)
     return 0;
 }
 
-// Bin, sort, permute and build the list from c->pos (current order).
-int build_list(Ctx *c)
+// The arguments of every kernel of one list build of a context, and its
+// launch shapes.  Sizes the cell arrays (so it is never called inside a
+// stream capture before an eager build has sized them).
+struct ListChain {
+    WrapArgs wrap;
+    ScanArgs scan;
+    ScatterArgs scat;
+    SortArgs sort;
+    PermArgs perm;
+    CopyArgs copy;
+    BuildArgs build;
+    ExclListArgs excl;
+    bool sub, excl_on;
+    int n, ncell, nkey;
+};
+
+static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
 {
     const int n = c->natms;
     if (c->excl_total > 0 && c->excl_natms != n)
@@ -727,28 +864,71 @@ int build_list(Ctx *c)
         if ((r = grow(c, &c->cell_count, nkey + 1))) return r;
         if ((r = grow(c, &c->cell_fill, nkey + 1))) return r;
         if ((r = grow(c, &c->key_start, nkey + 1))) return r;
+        // the scan zeroes the counts it has read, so they are zero between builds
+        DLPX_HIP(c, hipMemset(c->cell_count, 0, sizeof(int32_t) * (nkey + 1)));
         c->nkey_cap = nkey + 1;
     }
     c->ncell = ncell;
     CellGrid g = make_grid(c);
     g.sub = sub ? 2 : 1;
-    hipStream_t s = c->stream;
-    DLPX_HIP(c, hipMemsetAsync(c->cell_count, 0, sizeof(int32_t) * (nkey + 1), s));
-    const int b = 256, nb = ceil_div(n, b);
     int32_t *kstart = sub ? c->key_start : c->cell_start;
-    hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, c->pos, n, g, c->cell_of, c->cell_count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, c->cell_count, kstart, c->cell_fill, nkey);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, c->cell_of, c->cell_fill, c->perm, n);
-    hipLaunchKernelGGL(k_sort_cells, dim3(nkey), dim3(sub ? 64 : 256), 0, s, c->perm, kstart, nkey);
-    if (sub)
-        hipLaunchKernelGGL(k_block_start, dim3(ceil_div(ncell + 1, 256)), dim3(256), 0, s, c->key_start,
-                           c->cell_start, ncell);
-    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, c->perm, n, c->pos, c->pos_alt, c->typ, c->typ_alt,
-                       c->gid, c->gid_alt, c->resident ? c->vel : nullptr, c->resident ? c->vel_alt : nullptr,
-                       c->slot, c->xref);
-    hipLaunchKernelGGL(k_copy_back, dim3(nb), dim3(b), 0, s, n, c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt,
-                       c->gid, c->resident ? c->vel_alt : nullptr, c->resident ? c->vel : nullptr);
+    lc.n = n;
+    lc.ncell = ncell;
+    lc.nkey = nkey;
+    lc.sub = sub;
+    lc.wrap = WrapArgs{c->pos, n, g, c->cell_of, c->cell_count, upd};
+    lc.scan = ScanArgs{c->cell_count, kstart, c->cell_fill, nkey, sub ? c->cell_start : nullptr, upd};
+    lc.scat = ScatterArgs{c->cell_of, c->cell_fill, c->perm, n, upd};
+    lc.sort = SortArgs{c->perm, kstart, nkey, upd};
+    const bool res = c->resident;
+    lc.perm = PermArgs{c->perm, n,      c->pos,   c->pos_alt, c->typ,  c->typ_alt,
+                       c->gid,  c->gid_alt, res ? c->vel : nullptr, res ? c->vel_alt : nullptr,
+                       c->slot, c->xref, upd};
+    lc.copy = CopyArgs{n, c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt, c->gid,
+                       res ? c->vel_alt : nullptr, res ? c->vel : nullptr, upd};
+    lc.build = BuildArgs{c->pos, g, sub ? c->key_start : c->cell_start, rc_ext * rc_ext, c->cap_lane,
+                         c->nbr, c->cnt, c->flags, upd};
+    lc.excl_on = c->excl_total > 0 || c->nfrozen > 1;
+    lc.excl = ExclListArgs{n,      c->cap_lane, c->nbr, c->cnt, c->gid,
+                           c->excl_total > 0 ? c->excl_off : nullptr, c->excl_ids,
+                           c->nfrozen > 1 ? c->frz : nullptr, c->flags, upd};
+    return 0;
+}
 
+// Bin, sort and permute the atoms of a list chain (everything before the
+// list kernel itself).
+static void launch_sort(const ListChain &lc, hipStream_t s)
+{
+    const int b = 256, nb = ceil_div(lc.n, b);
+    hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, lc.wrap);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, lc.scan);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, lc.scat);
+    hipLaunchKernelGGL(k_sort_cells, dim3(lc.nkey), dim3(lc.sub ? 64 : 256), 0, s, lc.sort);
+    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, lc.perm);
+    hipLaunchKernelGGL(k_copy_back, dim3(nb), dim3(b), 0, s, lc.copy);
+}
+
+static void launch_build(const ListChain &lc, hipStream_t s)
+{
+    if (lc.sub)
+        hipLaunchKernelGGL(k_build_list_sub, dim3(xcd_grid(lc.ncell)), dim3(kListThreads), 0, s, lc.build);
+    else
+        hipLaunchKernelGGL(k_build_list, dim3(xcd_grid(lc.ncell)), dim3(kListThreads), 0, s, lc.build);
+    if (lc.excl_on)
+        hipLaunchKernelGGL(k_list_exclude, dim3(ceil_div((long)lc.n * kLanesPerAtom, 256)), dim3(256), 0, s, lc.excl);
+}
+
+// Bin, sort, permute and build the list from c->pos (current order); the
+// host waits for the build and regrows an overflowing list.
+int build_list(Ctx *c)
+{
+    ListChain lc;
+    int r;
+    if ((r = list_chain(c, nullptr, lc))) return r;
+    const int n = c->natms;
+    const double rc_ext = c->prm.cutoff + c->prm.padding;
+    hipStream_t s = c->stream;
+    launch_sort(lc, s);
     constexpr int G = kLanesPerAtom;
     for (int attempt = 0; attempt < 2; attempt++) {
         if (c->cap_lane == 0) {
@@ -757,21 +937,13 @@ int build_list(Ctx *c)
             const double est = 4.18879020478639 * rho * rc_ext * rc_ext * rc_ext;  // 4/3 pi r^3 rho
             const int per_atom = c->prm.max_list > 0 ? c->prm.max_list : (int)(1.5 * est) + 64;
             c->cap_lane = (per_atom + G - 1) / G;
-            int r;
             if ((r = grow(c, &c->nbr, (size_t)c->cap_lane * c->cap * G))) return r;
         }
+        lc.build.cap_lane = lc.excl.cap_lane = c->cap_lane;
+        lc.build.nbr = lc.excl.nbr = c->nbr;
         DLPX_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int32_t) * 4, s));
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[6], s));
-        if (sub)
-            hipLaunchKernelGGL(k_build_list_sub<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, g,
-                               c->key_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
-        else
-            hipLaunchKernelGGL(k_build_list<G>, dim3(xcd_grid(ncell)), dim3(kListThreads), 0, s, c->pos, c->cap,
-                               g, c->cell_start, rc_ext * rc_ext, c->cap_lane, c->nbr, c->cnt, c->flags);
-        if (c->excl_total > 0 || c->nfrozen > 1)
-            hipLaunchKernelGGL(k_list_exclude, dim3(ceil_div((long)n * G, 256)), dim3(256), 0, s, n, c->cap_lane,
-                               c->nbr, c->cnt, c->gid, c->excl_total > 0 ? c->excl_off : nullptr, c->excl_ids,
-                               c->nfrozen > 1 ? c->frz : nullptr, c->flags);
+        launch_build(lc, s);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
         DLPX_HIP(c, hipStreamSynchronize(s));
@@ -780,24 +952,98 @@ int build_list(Ctx *c)
             DLPX_HIP(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
             c->stage_ms[5] += ms;
         }
-        if (c->h_flags[0] == 0) break;  // no lane overflowed
+        if (c->h_flags[F_OVF] == 0) break;  // no lane overflowed
         if (c->prm.max_list > 0) {
             c->have_list = false;
             return fail(c, DLPX_ERR_LIST_OVERFLOW,
-                        "neighbour list overflow: an atom has " + std::to_string(c->h_flags[1]) +
+                        "neighbour list overflow: an atom has " + std::to_string(c->h_flags[F_IBIG]) +
                             " neighbours, max_list is " + std::to_string(c->prm.max_list));
         }
         // grow with margin and rebuild
-        c->cap_lane = c->h_flags[0] + c->h_flags[0] / 4 + 8;
-        int r;
+        c->cap_lane = c->h_flags[F_OVF] + c->h_flags[F_OVF] / 4 + 8;
         if ((r = grow(c, &c->nbr, (size_t)c->cap_lane * c->cap * G))) return r;
     }
-    if (c->h_flags[0] != 0) {
+    if (c->h_flags[F_OVF] != 0) {
         c->have_list = false;
         return fail(c, DLPX_ERR_LIST_OVERFLOW, "neighbour list overflow after regrowth");
     }
     c->have_list = true;
     return 0;
+}
+
+// Batched list builds of an ensemble, each gated by its context's device
+// rebuild decision (F_UPD).  One launch per kernel of the chain over every
+// system; the link-cell and subcell list kernels each take the systems that
+// use them.
+int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb)
+{
+    std::vector<WrapArgs> wrap;
+    std::vector<ScanArgs> scan;
+    std::vector<ScatterArgs> scat;
+    std::vector<SortArgs> sort;
+    std::vector<PermArgs> perm;
+    std::vector<CopyArgs> copy;
+    std::vector<BuildArgs> bsub, blink;
+    std::vector<ExclListArgs> excl;
+    std::vector<int> nb_atoms, nb_one, nb_keys, nb_sub, nb_link, nb_excl;
+    int r;
+    for (Ctx *c : cs) {
+        ListChain lc;
+        if ((r = list_chain(c, c->flags + F_UPD, lc))) return r;
+        if (c->cap_lane == 0 || !c->nbr) return fail(c, DLPX_ERR_STATE, "no list to rebuild on the device");
+        wrap.push_back(lc.wrap);
+        scan.push_back(lc.scan);
+        scat.push_back(lc.scat);
+        sort.push_back(lc.sort);
+        perm.push_back(lc.perm);
+        copy.push_back(lc.copy);
+        excl.push_back(lc.excl);
+        nb_atoms.push_back(ceil_div(lc.n, 256));
+        nb_one.push_back(1);
+        nb_keys.push_back(lc.nkey);
+        nb_sub.push_back(lc.sub ? lc.ncell : 0);
+        nb_link.push_back(lc.sub ? 0 : lc.ncell);
+        nb_excl.push_back(lc.excl_on ? ceil_div((long)lc.n * kLanesPerAtom, 256) : 0);
+        bsub.push_back(lc.build);
+        blink.push_back(lc.build);
+    }
+    Ctx *c0 = cs[0];
+    if ((r = batch_upload(c0, lb.wrap, wrap.data(), sizeof(WrapArgs), nb_atoms, false)) ||
+        (r = batch_upload(c0, lb.scan, scan.data(), sizeof(ScanArgs), nb_one, false)) ||
+        (r = batch_upload(c0, lb.scat, scat.data(), sizeof(ScatterArgs), nb_atoms, false)) ||
+        (r = batch_upload(c0, lb.sort, sort.data(), sizeof(SortArgs), nb_keys, false)) ||
+        (r = batch_upload(c0, lb.perm, perm.data(), sizeof(PermArgs), nb_atoms, false)) ||
+        (r = batch_upload(c0, lb.copy, copy.data(), sizeof(CopyArgs), nb_atoms, false)) ||
+        (r = batch_upload(c0, lb.bsub, bsub.data(), sizeof(BuildArgs), nb_sub, true)) ||
+        (r = batch_upload(c0, lb.blink, blink.data(), sizeof(BuildArgs), nb_link, true)) ||
+        (r = batch_upload(c0, lb.excl, excl.data(), sizeof(ExclListArgs), nb_excl, false)))
+        return r;
+    return 0;
+}
+
+int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb)
+{
+#define DLPX_LB(K, BL, ARGS, NT)                                                                          \
+    if ((BL).total > 0)                                                                                  \
+        hipLaunchKernelGGL(K, dim3((BL).total), dim3(NT), 0, s, (const ARGS *)(BL).args, (BL).boff, (BL).nsys)
+    DLPX_LB(k_wrap_bin_batch, lb.wrap, WrapArgs, 256);
+    DLPX_LB(k_scan_batch, lb.scan, ScanArgs, 1024);
+    DLPX_LB(k_scatter_batch, lb.scat, ScatterArgs, 256);
+    DLPX_LB(k_sort_cells_batch, lb.sort, SortArgs, 256);
+    DLPX_LB(k_permute_batch, lb.perm, PermArgs, 256);
+    DLPX_LB(k_copy_back_batch, lb.copy, CopyArgs, 256);
+    DLPX_LB(k_build_list_sub_batch, lb.bsub, BuildArgs, kListThreads);
+    DLPX_LB(k_build_list_batch, lb.blink, BuildArgs, kListThreads);
+    DLPX_LB(k_list_exclude_batch, lb.excl, ExclListArgs, 256);
+#undef DLPX_LB
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+void list_batch_free(ListBatch &lb)
+{
+    for (BatchLaunch *b : {&lb.wrap, &lb.scan, &lb.scat, &lb.sort, &lb.perm, &lb.copy, &lb.bsub, &lb.blink, &lb.excl})
+        batch_free(*b);
 }
 
 int vnl_check_dev(Ctx *c, int *update)

@@ -2,6 +2,7 @@
 // host-order <-> cell-order transfers of per-atom data.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "common.h"
 #include "device_util.h"
@@ -11,42 +12,55 @@ namespace dlpx {
 // Stage 1: v += dt/2 F/m ; x += dt v  (atoms of mass <= 1e-6 stay frozen),
 // fused with the displacement test of vnl_check (neighbours.F90:150-178):
 // the largest minimum-image |x - x_ref|^2 goes to *dmax (as the bit pattern
-// of a non-negative double), and the last workgroup to finish copies it to
-// host memory and resets *dmax and *done for the next step.
+// of a non-negative double).  The last workgroup to finish resets *dmax and
+// *done for the next step and publishes the result: to host memory
+// (host_dmax, the host then decides), and/or as the device's own rebuild
+// decision (upd: 1 when max |dx| >= Nearest(0.5,-1) padding, neighbours.F90:
+// 157-182, or always with no padding), which the list-build kernels read;
+// a decided rebuild also clears the build's counters and counts itself.
 struct Box {
     double L[3], invL[3];
+};
+struct Vv1Args {
+    double4 *pos, *vel;
+    const double4 *frc, *xref;
+    int n;
+    double dt;
+    Box b;
+    unsigned long long *dmax;
+    unsigned int *done;
+    unsigned long long *host_dmax;  // may be null
+    int32_t *flags;                 // F_UPD and the build counters; null: no device decision
+    double half_pad;                // Nearest(0.5,-1) * padding (0: rebuild every step)
 };
 // 1024-thread workgroups: the maximum is reduced in LDS first, so only one
 // atomic per workgroup (63 at 64k atoms) meets the single device word.
 constexpr int kVv1Threads = 1024;
-__global__ void __launch_bounds__(kVv1Threads) k_vv1(double4 *pos, double4 *vel, const double4 *frc, int n,
-                                                     double dt, const double4 *__restrict__ xref, Box b,
-                                                     unsigned long long *dmax, unsigned int *done,
-                                                     unsigned long long *host_dmax)
+__device__ __forceinline__ void vv1_body(const Vv1Args &a, int bid, int nb)
 {
     __shared__ double wmax[kVv1Threads / 64];
     __shared__ bool last;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * blockDim.x + threadIdx.x;
     double r2 = 0.0;
-    if (i < n) {
-        double4 v = vel[i];
-        double4 p = pos[i];
+    if (i < a.n) {
+        double4 v = a.vel[i];
+        double4 p = a.pos[i];
         if (v.w > 1.0e-6) {
-            const double4 f = frc[i];
-            const double tmp = (0.5 * dt) / v.w;
+            const double4 f = a.frc[i];
+            const double tmp = (0.5 * a.dt) / v.w;
             v.x += tmp * f.x;
             v.y += tmp * f.y;
             v.z += tmp * f.z;
-            p.x += dt * v.x;
-            p.y += dt * v.y;
-            p.z += dt * v.z;
-            vel[i] = v;
-            pos[i] = p;
+            p.x += a.dt * v.x;
+            p.y += a.dt * v.y;
+            p.z += a.dt * v.z;
+            a.vel[i] = v;
+            a.pos[i] = p;
         }
-        const double4 q = xref[i];
-        const double dx = min_image_1(p.x, q.x, b.L[0], b.invL[0]);
-        const double dy = min_image_1(p.y, q.y, b.L[1], b.invL[1]);
-        const double dz = min_image_1(p.z, q.z, b.L[2], b.invL[2]);
+        const double4 q = a.xref[i];
+        const double dx = min_image_1(p.x, q.x, a.b.L[0], a.b.invL[0]);
+        const double dy = min_image_1(p.y, q.y, a.b.L[1], a.b.invL[1]);
+        const double dz = min_image_1(p.z, q.z, a.b.L[2], a.b.invL[2]);
         r2 = dx * dx + dy * dy + dz * dz;
     }
 #pragma unroll
@@ -56,62 +70,102 @@ __global__ void __launch_bounds__(kVv1Threads) k_vv1(double4 *pos, double4 *vel,
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wmax[w]);
-        atomicMax(dmax, (unsigned long long)__double_as_longlong(m));
+        atomicMax(a.dmax, (unsigned long long)__double_as_longlong(m));
         __threadfence();
-        last = atomicAdd(done, 1u) == gridDim.x - 1;
+        last = atomicAdd(a.done, 1u) == (unsigned)nb - 1;
     }
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
-        *host_dmax = atomicAdd(dmax, 0ull);
-        __threadfence_system();
-        *dmax = 0ull;
-        *done = 0u;
+        const unsigned long long bits = atomicAdd(a.dmax, 0ull);
+        if (a.host_dmax) {
+            *a.host_dmax = bits;
+            __threadfence_system();
+        }
+        if (a.flags) {
+            const int upd = (a.half_pad <= 0.0 || sqrt(__longlong_as_double((long long)bits)) >= a.half_pad) ? 1 : 0;
+            a.flags[F_UPD] = upd;
+            if (upd) {
+                a.flags[F_OVF] = 0;
+                a.flags[F_IBIG] = 0;
+                a.flags[F_PAIRS] = 0;
+                a.flags[F_NREB] += 1;
+            }
+        }
+        *a.dmax = 0ull;
+        *a.done = 0u;
     }
 }
+__global__ void __launch_bounds__(kVv1Threads) k_vv1(Vv1Args a) { vv1_body(a, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kVv1Threads) k_vv1_batch(const Vv1Args *__restrict__ av,
+                                                           const int32_t *__restrict__ boff, int nsys)
+{
+    const int lb = blockIdx.x;  // stage 1 reads atoms in order: no XCD renumbering
+    if (lb >= boff[nsys]) return;
+    const int s = batch_find(boff, nsys, lb);
+    vv1_body(av[s], lb - boff[s], boff[s + 1] - boff[s]);
+}
 
-// Stage 2: v += dt/2 F/m, and sum m v^2 per block.  With FIN the
+// Stage 2: v += dt/2 F/m, and sum m v^2 per block.  With fin the
 // reciprocal forces are first added to the total force as k_finish does:
 // frc += 2 scale (frc_recip - mean frc_recip) (ewald_general.F90:857-864).
 // Frozen atoms (frz[gid[i]] /= 0; frz may be null) get zero force and
 // velocity, as freeze_atoms does after each force evaluation
 // (configuration.F90:3240-3258), so they neither move nor add kinetic energy.
-template <bool FIN>
-__global__ void __launch_bounds__(256) k_vv2(double4 *vel, double4 *frc, int n, double dt, double *red,
-                                             const double4 *__restrict__ frc_recip, double two_scale,
-                                             const double *__restrict__ red_out, double inv_n,
-                                             const int32_t *__restrict__ gid, const int32_t *__restrict__ frz)
+struct Vv2Args {
+    double4 *vel, *frc;
+    int n;
+    double dt;
+    double *red;
+    const double4 *frc_recip;  // fin only
+    double two_scale;
+    const double *red_out;
+    double inv_n;
+    const int32_t *gid, *frz;
+    int fin;
+};
+__device__ __forceinline__ void vv2_body(const Vv2Args &a, int bid)
 {
     __shared__ double lred[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
-    if (i < n) {
-        double4 v = vel[i];
-        double4 f = frc[i];
-        if (FIN) {
-            const double mx = red_out[R_FNETX] * inv_n, my = red_out[R_FNETY] * inv_n, mz = red_out[R_FNETZ] * inv_n;
-            const double4 r = frc_recip[i];
-            f.x += (r.x - mx) * two_scale;
-            f.y += (r.y - my) * two_scale;
-            f.z += (r.z - mz) * two_scale;
-            frc[i] = f;
+    if (i < a.n) {
+        double4 v = a.vel[i];
+        double4 f = a.frc[i];
+        if (a.fin) {
+            const double mx = a.red_out[R_FNETX] * a.inv_n, my = a.red_out[R_FNETY] * a.inv_n,
+                         mz = a.red_out[R_FNETZ] * a.inv_n;
+            const double4 r = a.frc_recip[i];
+            f.x += (r.x - mx) * a.two_scale;
+            f.y += (r.y - my) * a.two_scale;
+            f.z += (r.z - mz) * a.two_scale;
+            a.frc[i] = f;
         }
-        if (frz && frz[gid[i]] != 0) {
+        if (a.frz && a.frz[a.gid[i]] != 0) {
             f.x = f.y = f.z = 0.0;
             v.x = v.y = v.z = 0.0;
-            frc[i] = f;
-            vel[i] = v;
+            a.frc[i] = f;
+            a.vel[i] = v;
         }
         if (v.w > 1.0e-6) {
-            const double tmp = (0.5 * dt) / v.w;
+            const double tmp = (0.5 * a.dt) / v.w;
             v.x += tmp * f.x;
             v.y += tmp * f.y;
             v.z += tmp * f.z;
-            vel[i] = v;
+            a.vel[i] = v;
             acc[0] = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
         }
     }
-    block_sum<1>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_KIN);
+    block_sum<1>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_KIN);
+}
+__global__ void __launch_bounds__(256) k_vv2(Vv2Args a) { vv2_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_vv2_batch(const Vv2Args *__restrict__ av, const int32_t *__restrict__ boff,
+                                                   int nsys)
+{
+    const int lb = blockIdx.x;
+    if (lb >= boff[nsys]) return;
+    const int s = batch_find(boff, nsys, lb);
+    vv2_body(av[s], lb - boff[s]);
 }
 
 // freeze_atoms (configuration.F90:3240-3258): zero force and velocity of
@@ -172,19 +226,34 @@ __global__ void k_store(double *__restrict__ st, int n, const int32_t *__restric
     r[6] = f.x; r[7] = f.y; r[8] = f.z;
 }
 
+Vv1Args vv1_args(Ctx *c, double dt, bool device_decides)
+{
+    Vv1Args a;
+    a.pos = c->pos;
+    a.vel = c->vel;
+    a.frc = c->frc;
+    a.xref = c->xref;
+    a.n = c->natms;
+    a.dt = dt;
+    for (int d = 0; d < 3; d++) {
+        a.b.L[d] = c->L[d];
+        a.b.invL[d] = c->invL[d];
+    }
+    // device words F_DMAX (maximum) and F_DONE (workgroup count) are zero
+    // between launches
+    a.dmax = (unsigned long long *)(c->flags + F_DMAX);
+    a.done = (unsigned int *)(c->flags + F_DONE);
+    a.host_dmax = device_decides ? nullptr : (unsigned long long *)(c->h_flags_dev + 8);
+    a.flags = device_decides ? c->flags : nullptr;
+    a.half_pad = kHalfMinus * c->prm.padding;
+    return a;
+}
+int vv1_blocks(const Ctx *c) { return ceil_div(c->natms, kVv1Threads); }
+
 int nve_first(Ctx *c, double dt)
 {
-    Box b;
-    for (int d = 0; d < 3; d++) {
-        b.L[d] = c->L[d];
-        b.invL[d] = c->invL[d];
-    }
-    // device words flags[4..5] (maximum) and flags[6] (workgroup count) are
-    // zero between launches; the result lands in host word h_flags[8..9]
-    hipLaunchKernelGGL(k_vv1, dim3(ceil_div(c->natms, kVv1Threads)), dim3(kVv1Threads), 0, c->stream, c->pos,
-                       c->vel, c->frc,
-                       c->natms, dt, c->xref, b, (unsigned long long *)(c->flags + 4), (unsigned int *)(c->flags + 6),
-                       (unsigned long long *)(c->h_flags_dev + 8));
+    // the result lands in host word h_flags[8..9]
+    hipLaunchKernelGGL(k_vv1, dim3(vv1_blocks(c)), dim3(kVv1Threads), 0, c->stream, vv1_args(c, dt, false));
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -196,20 +265,69 @@ double nve_first_displacement(Ctx *c)
     return std::sqrt(r2);
 }
 
+Vv2Args vv2_args(Ctx *c, double dt, bool fin)
+{
+    Vv2Args a;
+    a.vel = c->vel;
+    a.frc = c->frc;
+    a.n = c->natms;
+    a.dt = dt;
+    a.red = c->red;
+    a.fin = fin ? 1 : 0;
+    a.frc_recip = fin ? c->frc_recip : nullptr;
+    a.two_scale = fin ? 2.0 * recip_scale(c) : 0.0;
+    a.red_out = fin ? c->red_out : nullptr;
+    a.inv_n = 1.0 / (double)c->natms;
+    a.gid = c->gid;
+    a.frz = c->nfrozen > 0 ? c->frz : nullptr;
+    return a;
+}
+int vv2_blocks(const Ctx *c) { return ceil_div(c->natms, 256); }
+
 int nve_second(Ctx *c, double dt, bool /*kinetic*/)
 {
-    const int nb = ceil_div(c->natms, 256);
+    const int nb = vv2_blocks(c);
     int r;
     if ((r = ensure_red_rows(c, nb))) return r;
-    const int32_t *frz = c->nfrozen > 0 ? c->frz : nullptr;
-    if (c->finish_in_vv2)
-        hipLaunchKernelGGL(k_vv2<true>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
-                           c->frc_recip, 2.0 * recip_scale(c), c->red_out, 1.0 / (double)c->natms, c->gid, frz);
-    else
-        hipLaunchKernelGGL(k_vv2<false>, dim3(nb), dim3(256), 0, c->stream, c->vel, c->frc, c->natms, dt, c->red,
-                           nullptr, 0.0, nullptr, 0.0, c->gid, frz);
+    hipLaunchKernelGGL(k_vv2, dim3(nb), dim3(256), 0, c->stream, vv2_args(c, dt, c->finish_in_vv2));
     c->finish_in_vv2 = false;
     c->red_rows[R_KIN] = nb;
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+// Batched stages 1 and 2 over the systems of an ensemble (each with its own
+// time step, Ctx::run_dt); stage 1 leaves the rebuild decision on the device.
+int vv1_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl)
+{
+    std::vector<Vv1Args> a;
+    std::vector<int> nb;
+    for (Ctx *c : cs) {
+        a.push_back(vv1_args(c, c->run_dt, true));
+        nb.push_back(vv1_blocks(c));
+    }
+    return batch_upload(cs[0], bl, a.data(), sizeof(Vv1Args), nb, false);
+}
+int vv1_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
+{
+    hipLaunchKernelGGL(k_vv1_batch, dim3(bl.total), dim3(kVv1Threads), 0, s, (const Vv1Args *)bl.args, bl.boff,
+                       bl.nsys);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+int vv2_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl)
+{
+    std::vector<Vv2Args> a;
+    std::vector<int> nb;
+    for (Ctx *c : cs) {
+        a.push_back(vv2_args(c, c->run_dt, c->prm.electrostatics != 0));
+        nb.push_back(vv2_blocks(c));
+    }
+    return batch_upload(cs[0], bl, a.data(), sizeof(Vv2Args), nb, false);
+}
+int vv2_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
+{
+    hipLaunchKernelGGL(k_vv2_batch, dim3(bl.total), dim3(256), 0, s, (const Vv2Args *)bl.args, bl.boff, bl.nsys);
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

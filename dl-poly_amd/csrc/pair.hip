@@ -216,17 +216,39 @@ constexpr int kPairThreads = 512;
 //
 // GT: the vdW tables are read from global memory (through the caches) instead
 // of LDS, for pair-key sets whose tables do not fit next to the erfc table.
+template <typename R>
+struct PairArgs {
+    const double4 *pos;
+    const int32_t *typ;
+    int n;
+    const int32_t *nbr, *cnt;
+    PairParams pp;
+    const typename R2T<R>::type *vdw_g, *erfc_g;
+    const int32_t *keytab_g;
+    const R *direct_g;
+    double4 *frc;
+    double *red;
+    const int32_t *gid;
+    double *ppo;
+};
 template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false, bool GT = false>
-__global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, const int32_t *__restrict__ typ,
-                                              int n, int cap, const int32_t *__restrict__ nbr,
-                                              const int32_t *__restrict__ cnt, PairParams pp,
-                                              const typename R2T<R>::type *__restrict__ vdw_g,
-                                              const typename R2T<R>::type *__restrict__ erfc_g,
-                                              const int32_t *__restrict__ keytab_g, const R *__restrict__ direct_g,
-                                              double4 *__restrict__ frc, double *__restrict__ red,
-                                              const int32_t *__restrict__ gid, double *__restrict__ ppo)
+__device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
 {
     using R2 = typename R2T<R>::type;
+    const double4 *__restrict__ pos = a.pos;
+    const int32_t *__restrict__ typ = a.typ;
+    const int n = a.n;
+    const int32_t *__restrict__ nbr = a.nbr;
+    const int32_t *__restrict__ cnt = a.cnt;
+    const PairParams &pp = a.pp;
+    const R2 *__restrict__ vdw_g = a.vdw_g;
+    const R2 *__restrict__ erfc_g = a.erfc_g;
+    const int32_t *__restrict__ keytab_g = a.keytab_g;
+    const R *__restrict__ direct_g = a.direct_g;
+    double4 *__restrict__ frc = a.frc;
+    double *__restrict__ red = a.red;
+    const int32_t *__restrict__ gid = a.gid;
+    double *__restrict__ ppo = a.ppo;
     extern __shared__ __align__(16) unsigned char smem[];
     // LDS: vdW tables (none in direct mode or with GT), erfc table, key table,
     // direct forms (8 per table row), reduction scratch
@@ -248,7 +270,7 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
 
     constexpr int G = kLanesPerAtom, CS = GP / G;
     static_assert(GP % G == 0 && GP <= 64, "GP must be a multiple of the list lanes");
-    const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int tid = bid * blockDim.x + threadIdx.x;
     const int i = tid / GP, p = tid % GP, lane = p % G, c0 = p / G;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     // engvdw, virvdw, engrl, (unused: the Coulomb virial is -tr(stress) - virvdw,
@@ -365,14 +387,40 @@ __global__ void __launch_bounds__(NT) k_pair(const double4 *__restrict__ pos, co
 #pragma unroll
     for (int k = 0; k < 10; k++) acc[k] = 0.5 * (double)pacc[k];  // every pair is met twice
     __syncthreads();
-    block_sum<10>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ENGVDW);
+    block_sum<10>(acc, lred, red + (size_t)bid * kReduceSlots + R_ENGVDW);
+}
+template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false, bool GT = false>
+__global__ void __launch_bounds__(NT) k_pair(PairArgs<R> a)
+{
+    pair_body<R, GP, NT, FS, DIRECT, PP, GT>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false, bool GT = false>
+__global__ void __launch_bounds__(NT) k_pair_batch(const PairArgs<R> *__restrict__ av,
+                                                    const int32_t *__restrict__ boff, int nsys)
+{
+    BatchPos bp;
+    if (!batch_pos(boff, nsys, bp)) return;
+    const PairArgs<R> &a = av[bp.s];
+    pair_body<R, GP, NT, FS, DIRECT, PP, GT>(a, bp.b);
 }
 
+// The pair kernel's arguments, dynamic LDS and template variant for one
+// context.  Variant: lanes per atom (4, 8, 16), force-shifting, direct
+// evaluation, per-particle collection, vdW tables from global memory.
+struct PairVariant {
+    int gp;
+    bool fs, direct, pp, gt;
+    bool operator==(const PairVariant &o) const
+    {
+        return gp == o.gp && fs == o.fs && direct == o.direct && pp == o.pp && gt == o.gt;
+    }
+};
+
 template <typename R>
-static int launch_pair(Ctx *c)
+static int pair_setup(Ctx *c, PairArgs<R> &a, size_t &lds, PairVariant &v, int lanes)
 {
     using R2 = typename R2T<R>::type;
-    PairParams pp;
+    PairParams &pp = a.pp;
     pp.ntypes = c->prm.ntypes;
     pp.nkeys = c->nrows;  // distinct table rows (the key table maps type pairs to them)
     pp.mg1 = c->max_grid + 1;
@@ -391,6 +439,19 @@ static int launch_pair(Ctx *c)
     pp.scaling = kR4pie0 / c->prm.epsilon;
     pp.r_rvdw = 1.0 / c->prm.vdw_cutoff;
     pp.cap_lane = c->cap_lane;
+    a.pos = c->pos;
+    a.typ = c->typ;
+    a.n = c->natms;
+    a.nbr = c->nbr;
+    a.cnt = c->cnt;
+    a.vdw_g = (const R2 *)c->d_vdw;
+    a.erfc_g = (const R2 *)c->d_erfc;
+    a.keytab_g = c->d_keytab;
+    a.direct_g = (const R *)c->d_vdw_direct;
+    a.frc = c->frc;
+    a.red = c->red;
+    a.gid = c->gid;
+    a.ppo = c->pp;
     const bool direct = c->vdw_direct;
     const size_t lds_rest = sizeof(R2) * (pp.electro ? pp.ns1 : 0) + sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
                             (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
@@ -398,68 +459,56 @@ static int launch_pair(Ctx *c)
     // else they are read from global memory
     const size_t lds_tab = direct ? 0 : sizeof(R2) * (size_t)pp.nkeys * pp.mg1;
     const bool gtab = !direct && lds_tab + lds_rest > 160 * 1024;
-    const size_t lds = (gtab ? 0 : lds_tab) + lds_rest;
+    lds = (gtab ? 0 : lds_tab) + lds_rest;
     if (lds > 160 * 1024)
         return fail(c, DLPX_ERR_UNSUPPORTED, "the erfc and key tables exceed the 160 KiB LDS of a CU");
-    if (gtab && c->pair_lanes != 8)
-        return fail(c, DLPX_ERR_UNSUPPORTED, "vdW tables beyond the LDS run with 8 pair lanes");
-#define DLPX_PAIR_KG(GP, FS, DIR, PPK, GTK)                                                                  \
-    DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                                         \
-    hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>), dim3(pair_blocks(c)),                 \
-                       dim3(kPairThreads), lds, c->stream, c->pos, c->typ, c->natms, c->cap, c->nbr, c->cnt, pp,\
-                       (const R2 *)c->d_vdw, (const R2 *)c->d_erfc, c->d_keytab, (const R *)c->d_vdw_direct,    \
-                       c->frc, c->red, c->gid, c->pp)
-#define DLPX_PAIR_KP(GP, FS, DIR, PPK) DLPX_PAIR_KG(GP, FS, DIR, PPK, false)
-#define DLPX_PAIR_K(GP, FS, DIR) DLPX_PAIR_KP(GP, FS, DIR, false)
-#define DLPX_PAIR(GP, FS) DLPX_PAIR_K(GP, FS, false)
-    if (gtab) {
-        // tabulated vdW from global memory, 8 lanes per atom
-        if (c->pp_now && c->vdw_force_shift) {
-            DLPX_PAIR_KG(8, true, false, true, true);
-        } else if (c->pp_now) {
-            DLPX_PAIR_KG(8, false, false, true, true);
-        } else if (c->vdw_force_shift) {
-            DLPX_PAIR_KG(8, true, false, false, true);
-        } else {
-            DLPX_PAIR_KG(8, false, false, false, true);
-        }
-    } else if (c->pp_now) {
-        // per-particle collection at the default 8 lanes per atom only
-        if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "per-particle output runs with 8 pair lanes");
-        if (direct && c->vdw_force_shift) {
-            DLPX_PAIR_KP(8, true, true, true);
-        } else if (direct) {
-            DLPX_PAIR_KP(8, false, true, true);
-        } else if (c->vdw_force_shift) {
-            DLPX_PAIR_KP(8, true, false, true);
-        } else {
-            DLPX_PAIR_KP(8, false, false, true);
-        }
-    } else if (direct) {
-        // direct mode at the default 8 lanes per atom only
-        if (c->pair_lanes != 8) return fail(c, DLPX_ERR_UNSUPPORTED, "vdw_method direct runs with 8 pair lanes");
-        if (c->vdw_force_shift) {
-            DLPX_PAIR_K(8, true, true);
-        } else {
-            DLPX_PAIR_K(8, false, true);
-        }
-    } else if (c->vdw_force_shift) {
-        switch (c->pair_lanes) {
-            case 16: DLPX_PAIR(16, true); break;
-            case 8: DLPX_PAIR(8, true); break;
-            default: DLPX_PAIR(4, true); break;
-        }
-    } else {
-        switch (c->pair_lanes) {
-            case 16: DLPX_PAIR(16, false); break;
-            case 8: DLPX_PAIR(8, false); break;
-            default: DLPX_PAIR(4, false); break;
-        }
-    }
-#undef DLPX_PAIR
-#undef DLPX_PAIR_K
-#undef DLPX_PAIR_KP
-#undef DLPX_PAIR_KG
+    // per-particle output, direct mode and global tables run with 8 lanes per atom
+    v = PairVariant{(gtab || c->pp_now || direct) ? 8 : lanes, c->vdw_force_shift, direct, c->pp_now, gtab};
+    return 0;
+}
+
+// Calls CALL(GP, FS, DIRECT, PP, GT) with the compile-time values of v.
+#define DLPX_PAIR_VARIANTS(v, CALL)                                                          \
+    do {                                                                                     \
+        if ((v).gt) {                                                                        \
+            if ((v).pp) { if ((v).fs) CALL(8, true, false, true, true); else CALL(8, false, false, true, true); } \
+            else { if ((v).fs) CALL(8, true, false, false, true); else CALL(8, false, false, false, true); }      \
+        } else if ((v).pp) {                                                                 \
+            if ((v).direct) { if ((v).fs) CALL(8, true, true, true, false); else CALL(8, false, true, true, false); } \
+            else { if ((v).fs) CALL(8, true, false, true, false); else CALL(8, false, false, true, false); }         \
+        } else if ((v).direct) {                                                             \
+            if ((v).fs) CALL(8, true, true, false, false); else CALL(8, false, true, false, false); \
+        } else if ((v).fs) {                                                                 \
+            switch ((v).gp) {                                                                \
+                case 16: CALL(16, true, false, false, false); break;                         \
+                case 8: CALL(8, true, false, false, false); break;                           \
+                default: CALL(4, true, false, false, false); break;                          \
+            }                                                                                \
+        } else {                                                                             \
+            switch ((v).gp) {                                                                \
+                case 16: CALL(16, false, false, false, false); break;                        \
+                case 8: CALL(8, false, false, false, false); break;                          \
+                default: CALL(4, false, false, false, false); break;                         \
+            }                                                                                \
+        }                                                                                    \
+    } while (0)
+
+template <typename R>
+static int launch_pair(Ctx *c)
+{
+    PairArgs<R> a;
+    size_t lds;
+    PairVariant v;
+    int r;
+    if ((r = pair_setup<R>(c, a, lds, v, c->pair_lanes))) return r;
+#define DLPX_PAIR1(GP, FS, DIR, PPK, GTK)                                                                \
+    do {                                                                                                 \
+        DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                                \
+        hipLaunchKernelGGL((k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>), dim3(pair_blocks(c)),         \
+                           dim3(kPairThreads), lds, c->stream, a);                                       \
+    } while (0)
+    DLPX_PAIR_VARIANTS(v, DLPX_PAIR1);
+#undef DLPX_PAIR1
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -470,7 +519,83 @@ int pair_forces(Ctx *c)
     return launch_pair<double>(c);
 }
 
-int pair_blocks(const Ctx *c) { return xcd_grid(ceil_div((long)c->natms * c->pair_lanes, kPairThreads)); }
+// Batched pair forces: one launch per variant present among the systems
+// (every system of the benchmark ensembles shares one).
+template <typename R>
+static int pair_prepare_t(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::vector<int> &rows)
+{
+    std::vector<PairVariant> vs;
+    std::vector<std::vector<PairArgs<R>>> args;
+    std::vector<std::vector<int>> nbs;
+    std::vector<size_t> lds_max;
+    int r;
+    for (Ctx *c : cs) {
+        PairArgs<R> a;
+        size_t lds;
+        PairVariant v;
+        if ((r = pair_setup<R>(c, a, lds, v, lanes))) return r;
+        size_t g = 0;
+        while (g < vs.size() && !(vs[g] == v)) g++;
+        if (g == vs.size()) {
+            vs.push_back(v);
+            args.emplace_back();
+            nbs.emplace_back();
+            lds_max.push_back(0);
+        }
+        args[g].push_back(a);
+        nbs[g].push_back(ceil_div((long)c->natms * v.gp, kPairThreads));
+        rows.push_back(nbs[g].back());
+        lds_max[g] = std::max(lds_max[g], lds);
+    }
+    pb.groups.resize(vs.size());
+    for (size_t g = 0; g < vs.size(); g++) {
+        PairBatch::Group &gr = pb.groups[g];
+        gr.gp = vs[g].gp;
+        gr.fs = vs[g].fs;
+        gr.direct = vs[g].direct;
+        gr.pp = vs[g].pp;
+        gr.gt = vs[g].gt;
+        if ((r = batch_upload(cs[0], gr.bl, args[g].data(), sizeof(PairArgs<R>), nbs[g], true))) return r;
+        gr.bl.lds = lds_max[g];
+    }
+    return 0;
+}
+
+int pair_prepare(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::vector<int> &rows)
+{
+    rows.clear();
+    if (cs[0]->prm.precision == DLPX_MIXED) return pair_prepare_t<float>(cs, pb, lanes, rows);
+    return pair_prepare_t<double>(cs, pb, lanes, rows);
+}
+
+template <typename R>
+static int pair_launch_t(Ctx *c, hipStream_t s, const PairBatch &pb)
+{
+    for (const PairBatch::Group &gr : pb.groups) {
+        const PairVariant v{gr.gp, gr.fs, gr.direct, gr.pp, gr.gt};
+        const size_t lds = gr.bl.lds;
+#define DLPX_PAIRB(GP, FS, DIR, PPK, GTK)                                                                   \
+    do {                                                                                                    \
+        DLPX_LDS_MAX(c, (k_pair_batch<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                             \
+        hipLaunchKernelGGL((k_pair_batch<R, GP, kPairThreads, FS, DIR, PPK, GTK>), dim3(gr.bl.total),         \
+                           dim3(kPairThreads), lds, s, (const PairArgs<R> *)gr.bl.args, gr.bl.boff,           \
+                           gr.bl.nsys);                                                                     \
+    } while (0)
+        DLPX_PAIR_VARIANTS(v, DLPX_PAIRB);
+#undef DLPX_PAIRB
+    }
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int pair_launch(Ctx *c, hipStream_t s, const PairBatch &pb)
+{
+    if (c->prm.precision == DLPX_MIXED) return pair_launch_t<float>(c, s, pb);
+    return pair_launch_t<double>(c, s, pb);
+}
+
+int pair_blocks_exact(const Ctx *c) { return ceil_div((long)c->natms * c->pair_lanes, kPairThreads); }
+int pair_blocks(const Ctx *c) { return xcd_grid(pair_blocks_exact(c)); }
 
 // Ewald correction of the excluded pairs (ewald_spole.F90:479-679, called
 // from two_body.F90:551-592).  The reciprocal sum holds the whole Coulomb
@@ -486,17 +611,30 @@ struct ExclParams {
     double rc, alpha, scaling;
 };
 
-__global__ void __launch_bounds__(256) k_excl(const double4 *__restrict__ pos, int n, const int32_t *__restrict__ gid,
-                                              const int32_t *__restrict__ slot, const int32_t *__restrict__ excl_off,
-                                              const int32_t *__restrict__ excl_ids, ExclParams ep,
-                                              double4 *__restrict__ frc, double *__restrict__ red)
+struct ExclArgs {
+    const double4 *pos;
+    int n;
+    const int32_t *gid, *slot, *excl_off, *excl_ids;
+    ExclParams ep;
+    double4 *frc;
+    double *red;
+};
+__device__ __forceinline__ void excl_body(const ExclArgs &a, int bid)
 {
     // A&S 7.1.26: erfc(x) ~ t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-x^2), t = 1/(1 + p x)
     constexpr double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741, a4 = -1.453152027,
                      a5 = 1.061405429, p = 0.3275911;
     constexpr double sqrpi = 1.7724538509055160273;
+    const double4 *__restrict__ pos = a.pos;
+    const int n = a.n;
+    const int32_t *__restrict__ gid = a.gid;
+    const int32_t *__restrict__ slot = a.slot;
+    const int32_t *__restrict__ excl_off = a.excl_off;
+    const int32_t *__restrict__ excl_ids = a.excl_ids;
+    const ExclParams &ep = a.ep;
+    double4 *__restrict__ frc = a.frc;
     __shared__ double lred[7 * (256 / 64)];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = bid * blockDim.x + threadIdx.x;
     double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (i < n) {
         const int h = gid[i];
@@ -549,25 +687,69 @@ __global__ void __launch_bounds__(256) k_excl(const double4 *__restrict__ pos, i
     }
 #pragma unroll
     for (int k = 0; k < 7; k++) acc[k] *= 0.5;  // every pair is met twice
-    block_sum<7>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ENGEX);
+    block_sum<7>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_ENGEX);
+}
+__global__ void __launch_bounds__(256) k_excl(ExclArgs a) { excl_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_excl_batch(const ExclArgs *__restrict__ av, const int32_t *__restrict__ boff,
+                                                    int nsys)
+{
+    const int lb = blockIdx.x;
+    if (lb >= boff[nsys]) return;
+    const int s = batch_find(boff, nsys, lb);
+    const ExclArgs &a = av[s];
+    excl_body(a, lb - boff[s]);
 }
 
 int excl_blocks(const Ctx *c) { return ceil_div(c->natms, 256); }
+
+static ExclArgs excl_args(const Ctx *c)
+{
+    ExclArgs a;
+    for (int d = 0; d < 3; d++) {
+        a.ep.L[d] = c->L[d];
+        a.ep.invL[d] = c->invL[d];
+    }
+    a.ep.rc = c->prm.cutoff;
+    a.ep.alpha = c->prm.alpha;
+    a.ep.scaling = kR4pie0 / c->prm.epsilon;
+    a.pos = c->pos;
+    a.n = c->natms;
+    a.gid = c->gid;
+    a.slot = c->slot;
+    a.excl_off = c->excl_off;
+    a.excl_ids = c->excl_ids;
+    a.frc = c->frc;
+    a.red = c->red;
+    return a;
+}
 
 int excl_forces(Ctx *c)
 {
     if (c->excl_total == 0 || !c->prm.electrostatics) return 0;
     if (c->excl_natms != c->natms) return fail(c, DLPX_ERR_STATE, "exclusion lists are for another atom count");
-    ExclParams ep;
-    for (int d = 0; d < 3; d++) {
-        ep.L[d] = c->L[d];
-        ep.invL[d] = c->invL[d];
+    hipLaunchKernelGGL(k_excl, dim3(excl_blocks(c)), dim3(256), 0, c->stream, excl_args(c));
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+// Batched exclusion corrections over the systems that have exclusions.
+int excl_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl)
+{
+    std::vector<ExclArgs> a;
+    std::vector<int> nb;
+    for (Ctx *c : cs) {
+        const bool on = c->excl_total > 0 && c->prm.electrostatics;
+        if (on && c->excl_natms != c->natms) return fail(c, DLPX_ERR_STATE, "exclusion lists are for another atom count");
+        a.push_back(excl_args(c));
+        nb.push_back(on ? excl_blocks(c) : 0);
     }
-    ep.rc = c->prm.cutoff;
-    ep.alpha = c->prm.alpha;
-    ep.scaling = kR4pie0 / c->prm.epsilon;
-    hipLaunchKernelGGL(k_excl, dim3(excl_blocks(c)), dim3(256), 0, c->stream, c->pos, c->natms, c->gid, c->slot,
-                       c->excl_off, c->excl_ids, ep, c->frc, c->red);
+    return batch_upload(cs[0], bl, a.data(), sizeof(ExclArgs), nb, false);
+}
+
+int excl_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
+{
+    if (bl.total == 0) return 0;
+    hipLaunchKernelGGL(k_excl_batch, dim3(bl.total), dim3(256), 0, s, (const ExclArgs *)bl.args, bl.boff, bl.nsys);
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

@@ -149,10 +149,16 @@ constexpr int spread_atoms()
     // three workgroups fit in a CU's LDS
     return (sizeof(R) == 8 ? 18432 : 12288) / (3 * S * (int)sizeof(R));
 }
+template <typename R>
+struct SpreadArgs {
+    const double4 *pos;
+    const int32_t *cell_start;
+    int ncell;
+    SpmeParams sp;
+    R *Q;
+};
 template <typename R, int S>
-__global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const double4 *__restrict__ pos,
-                                                      const int32_t *__restrict__ cell_start, int ncell,
-                                                      SpmeParams sp, R *__restrict__ Q)
+__device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
 {
     constexpr int kCap = kSlabBytes / (int)sizeof(double);
     constexpr int NA = spread_atoms<R, S>();
@@ -162,8 +168,12 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
     __shared__ int sB[3][NA];
     __shared__ R sQ[NA];
     __shared__ int wlo[3], whi[3];
-    const int c = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x, wid = t >> 6, lane = t & 63;
-    if (c >= ncell) return;
+    const double4 *__restrict__ pos = sa.pos;
+    const int32_t *__restrict__ cell_start = sa.cell_start;
+    const SpmeParams &sp = sa.sp;
+    R *__restrict__ Q = sa.Q;
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    if (c >= sa.ncell) return;
     const int s0 = cell_start[c], n = cell_start[c + 1] - s0;
     if (n <= 0) return;
     if (t < 3) {
@@ -309,6 +319,22 @@ __global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(const doub
     }
 }
 
+template <typename R, int S>
+__global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells(SpreadArgs<R> a)
+{
+    spread_body<R, S>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <typename R, int S>
+__global__ void __launch_bounds__(spread_threads<R>()) k_spread_cells_batch(const SpreadArgs<R> *__restrict__ av,
+                                                                            const int32_t *__restrict__ boff,
+                                                                            int nsys)
+{
+    BatchPos bp;
+    if (!batch_pos(boff, nsys, bp)) return;
+    const SpreadArgs<R> &a = av[bp.s];
+    spread_body<R, S>(a, bp.b);
+}
+
 // Lanes per atom in the gather: the next power of two >= S, lane tx < S
 // taking the grid column x = base + tx.
 constexpr int gather_lanes(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : 16; }
@@ -322,10 +348,17 @@ constexpr int gather_lanes(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : 16; }
 //   Fz = sum_y M_y sum_z dM_z Phi,
 // then e = M_x E, fx = dM_x E, fy = M_x Fy, fz = M_x Fz per lane.  The z sums
 // run in R, everything after them in FP64.
+template <typename R>
+struct GatherArgs {
+    const double4 *pos;
+    int n;
+    SpmeParams sp;
+    const R *Phi;
+    double4 *frc_recip;
+    double *red;
+};
 template <typename R, int S>
-__global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos, int n, SpmeParams sp,
-                                                const R *__restrict__ Phi, double4 *__restrict__ frc_recip,
-                                                double *red)
+__device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
 {
     constexpr int G = gather_lanes(S), APB = 256 / G;
     __shared__ R sM[3][APB][S];
@@ -333,7 +366,12 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
     __shared__ int sB[3][APB];
     __shared__ double sQ[APB];
     __shared__ double lred[4 * 4];
-    const int a0 = xcd_block(blockIdx.x, gridDim.x) * APB;
+    const double4 *__restrict__ pos = ga.pos;
+    const int n = ga.n;
+    const SpmeParams &sp = ga.sp;
+    const R *__restrict__ Phi = ga.Phi;
+    double4 *__restrict__ frc_recip = ga.frc_recip;
+    const int a0 = bid * APB;
     const int t = threadIdx.x;
     if (t < 3 * APB) {
         const int d = t / APB, a = t % APB;
@@ -426,7 +464,21 @@ __global__ void __launch_bounds__(256) k_gather(const double4 *__restrict__ pos,
         acc[2] = Fy_;
         acc[3] = Fz_;
     }
-    block_sum<4>(acc, lred, red + (size_t)blockIdx.x * kReduceSlots + R_ERECIP);
+    block_sum<4>(acc, lred, ga.red + (size_t)bid * kReduceSlots + R_ERECIP);
+}
+template <typename R, int S>
+__global__ void __launch_bounds__(256) k_gather(GatherArgs<R> a)
+{
+    gather_body<R, S>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <typename R, int S>
+__global__ void __launch_bounds__(256) k_gather_batch(const GatherArgs<R> *__restrict__ av,
+                                                      const int32_t *__restrict__ boff, int nsys)
+{
+    BatchPos bp;
+    if (!batch_pos(boff, nsys, bp)) return;
+    const GatherArgs<R> &a = av[bp.s];
+    gather_body<R, S>(a, bp.b);
 }
 
 // Per-particle reciprocal stress (spme_calc_stress, ewald_general.F90:871-
@@ -554,19 +606,56 @@ struct RedRows {
     int rows[kReduceSlots];
 };
 // one workgroup per column; fixed summation order (deterministic)
-__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ red, RedRows rr, double *out)
+__device__ __forceinline__ void reduce_body(const double *__restrict__ red, const int *rows, double *out, int col,
+                                            double *rec)
 {
     __shared__ double part[256];
-    const int col = blockIdx.x;
     double s = 0.0;
-    for (int r = threadIdx.x; r < rr.rows[col]; r += blockDim.x) s += red[(size_t)r * kReduceSlots + col];
+    for (int r = threadIdx.x; r < rows[col]; r += blockDim.x) s += red[(size_t)r * kReduceSlots + col];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[col] = part[0];
+    if (threadIdx.x == 0) {
+        out[col] = part[0];
+        if (rec) rec[col] = part[0];
+    }
+}
+__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ red, RedRows rr, double *out)
+{
+    reduce_body(red, rr.rows, out, blockIdx.x, nullptr);
+}
+
+// Batched reductions: workgroup b sums column b % kReduceSlots of system
+// b / kReduceSlots.  With a per-step record (energies not null) the totals
+// are also written to record F_STEP of the system's energies, and
+// k_record_next then advances F_STEP.
+struct ReduceArgs {
+    const double *red;
+    RedRows rr;
+    double *out;
+    double *energies;  // may be null
+    const int32_t *flags;
+    int ecap;          // records the energies array holds
+};
+__global__ void __launch_bounds__(256) k_reduce_batch(const ReduceArgs *__restrict__ av, int nsys)
+{
+    const int s = blockIdx.x / kReduceSlots, col = blockIdx.x % kReduceSlots;
+    if (s >= nsys) return;
+    const ReduceArgs &a = av[s];
+    double *rec = nullptr;
+    if (a.energies) {
+        const int k = a.flags[F_STEP];
+        if (k < a.ecap) rec = a.energies + (size_t)k * kReduceSlots;
+    }
+    reduce_body(a.red, a.rr.rows, a.out, col, rec);
+}
+__global__ void k_record_next(int32_t *const *__restrict__ flags, int nsys)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) flags[s][F_STEP] += 1;
 }
 
 int ensure_red_rows(Ctx *c, int rows)
@@ -653,30 +742,9 @@ int spme_setup(Ctx *c)
     return 0;
 }
 
-template <typename R>
-static int spme_run(Ctx *c)
+// ewald_spole.F90:1296-1306 cutoff and convergence factors
+InflParams infl_params(const Ctx *c)
 {
-    const int n = c->natms;
-    hipStream_t s = c->sp;  // the evaluation's reciprocal-space stream
-    const SpmeParams sp = spme_params(c);
-    const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
-    int r;
-    // ewald_general.F90:517-576 charge spreading into the charge grid, which
-    // is zero here: set at allocation, and cleared by the forward x pass of
-    // each convolution after reading it
-    switch (sp.S) {
-#define DLPX_SPREAD(SS)                                                                                    \
-    case SS:                                                                                               \
-        hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(spread_threads<R>()), 0, \
-                           s, c->pos,                                                                      \
-                           c->cell_start, c->ncell, sp, (R *)c->Q);                                        \
-        break;
-        DLPX_SPREAD(4) DLPX_SPREAD(6) DLPX_SPREAD(8) DLPX_SPREAD(10) DLPX_SPREAD(12) DLPX_SPREAD(14) DLPX_SPREAD(16)
-#undef DLPX_SPREAD
-        default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
-    }
-    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[2], s));
-    // ewald_spole.F90:1296-1306 cutoff and convergence factors
     InflParams ip;
     for (int d = 0; d < 3; d++) ip.K[d] = c->K[d];
     ip.K1h = c->K1h;
@@ -690,6 +758,50 @@ static int spme_run(Ctx *c)
     ip.cut2 = cut_off * cut_off;
     ip.alpha = c->prm.alpha;
     ip.mode = 0;
+    return ip;
+}
+
+template <typename R>
+static SpreadArgs<R> spread_args(const Ctx *c)
+{
+    return SpreadArgs<R>{c->pos, c->cell_start, c->ncell, spme_params(c), (R *)c->Q};
+}
+template <typename R>
+static GatherArgs<R> gather_args(const Ctx *c)
+{
+    return GatherArgs<R>{c->pos, c->natms, spme_params(c), (const R *)c->Phi, c->frc_recip, c->red};
+}
+
+#define DLPX_S_SWITCH(S, CALL)                                                                             \
+    switch (S) {                                                                                           \
+        case 4: CALL(4); break;                                                                            \
+        case 6: CALL(6); break;                                                                            \
+        case 8: CALL(8); break;                                                                            \
+        case 10: CALL(10); break;                                                                          \
+        case 12: CALL(12); break;                                                                          \
+        case 14: CALL(14); break;                                                                          \
+        case 16: CALL(16); break;                                                                          \
+        default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");           \
+    }
+
+template <typename R>
+static int spme_run(Ctx *c)
+{
+    const int n = c->natms;
+    hipStream_t s = c->sp;  // the evaluation's reciprocal-space stream
+    const SpmeParams sp = spme_params(c);
+    const int nb = gather_blocks(c);  // evaluate() has sized the partial-sum rows
+    int r;
+    // ewald_general.F90:517-576 charge spreading into the charge grid, which
+    // is zero here: set at allocation, and cleared by the forward x pass of
+    // each convolution after reading it
+    const SpreadArgs<R> spa = spread_args<R>(c);
+#define DLPX_SPREAD(SS) \
+    hipLaunchKernelGGL((k_spread_cells<R, SS>), dim3(xcd_grid(c->ncell)), dim3(spread_threads<R>()), 0, s, spa)
+    DLPX_S_SWITCH(sp.S, DLPX_SPREAD)
+#undef DLPX_SPREAD
+    if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[2], s));
+    InflParams ip = infl_params(c);
     if (c->pp_now) {
         // keep the charge grid for the per-particle stress convolution
         const size_t bytes = (c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double)) * c->nreal;
@@ -697,16 +809,10 @@ static int spme_run(Ctx *c)
     }
     if ((r = fft_convolve(c, ip, c->Q, c->Phi, true))) return r;
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[3], s));
-    switch (sp.S) {
-#define DLPX_GATHER(SS)                                                                                     \
-    case SS:                                                                                                \
-        hipLaunchKernelGGL((k_gather<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, (const R *)c->Phi, \
-                           c->frc_recip, c->red);                                                           \
-        break;
-        DLPX_GATHER(4) DLPX_GATHER(6) DLPX_GATHER(8) DLPX_GATHER(10) DLPX_GATHER(12) DLPX_GATHER(14) DLPX_GATHER(16)
+    const GatherArgs<R> ga = gather_args<R>(c);
+#define DLPX_GATHER(SS) hipLaunchKernelGGL((k_gather<R, SS>), dim3(nb), dim3(256), 0, s, ga)
+    DLPX_S_SWITCH(sp.S, DLPX_GATHER)
 #undef DLPX_GATHER
-        default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
-    }
     for (int k = R_ERECIP; k <= R_FNETZ; k++) c->red_rows[k] = nb;
     if (c->pp_now) {
         // per-particle stresses: the saved charge grid convolved with the
@@ -715,20 +821,123 @@ static int spme_run(Ctx *c)
         ips.mode = 1;
         if ((r = fft_convolve(c, ips, c->Q2, c->Q2, false))) return r;
         const double c_fac = 1.0 / (2.0 * 2.0 * kPi * c->prm.alpha);  // ewald_general.F90:932-933
-        switch (sp.S) {
 #define DLPX_GSTRESS(SS)                                                                                        \
-    case SS:                                                                                                    \
-        hipLaunchKernelGGL((k_gather_stress<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, c_fac,          \
-                           (const R *)c->Q2, c->pp, c->red);                                                    \
-        break;
-            DLPX_GSTRESS(4) DLPX_GSTRESS(6) DLPX_GSTRESS(8) DLPX_GSTRESS(10) DLPX_GSTRESS(12) DLPX_GSTRESS(14)
-            DLPX_GSTRESS(16)
+    hipLaunchKernelGGL((k_gather_stress<R, SS>), dim3(nb), dim3(256), 0, s, c->pos, n, sp, c_fac,              \
+                       (const R *)c->Q2, c->pp, c->red)
+        DLPX_S_SWITCH(sp.S, DLPX_GSTRESS)
 #undef DLPX_GSTRESS
-            default: return fail(c, DLPX_ERR_UNSUPPORTED, "spline order must be even and in 4..16");
-        }
         for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
     }
     if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[4], s));
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int gather_blocks_exact(const Ctx *c) { return ceil_div(c->natms, 256 / gather_lanes(c->prm.nsplines)); }
+
+// Batched reciprocal space over the systems with electrostatics: spreading,
+// the convolution passes and the gather, one launch each.  Every system of
+// the batch must have the same spline order.
+template <typename R>
+static int spme_prepare_t(const std::vector<Ctx *> &cs, SpmeBatch &sb)
+{
+    std::vector<SpreadArgs<R>> spa;
+    std::vector<GatherArgs<R>> ga;
+    std::vector<InflParams> ips;
+    std::vector<int> nbs, nbg;
+    int S = 0;
+    for (Ctx *c : cs) {
+        const bool on = c->prm.electrostatics != 0;
+        if (on) {
+            if (S && c->prm.nsplines != S)
+                return fail(c, DLPX_ERR_UNSUPPORTED, "systems of one batch must share the spline order");
+            S = c->prm.nsplines;
+        }
+        spa.push_back(spread_args<R>(c));
+        ga.push_back(gather_args<R>(c));
+        ips.push_back(on ? infl_params(c) : InflParams{});
+        nbs.push_back(on ? c->ncell : 0);
+        nbg.push_back(on ? gather_blocks_exact(c) : 0);
+    }
+    sb.S = S;
+    int r;
+    if ((r = batch_upload(cs[0], sb.spread, spa.data(), sizeof(SpreadArgs<R>), nbs, true)) ||
+        (r = batch_upload(cs[0], sb.gather, ga.data(), sizeof(GatherArgs<R>), nbg, true)))
+        return r;
+    sb.gather_rows = nbg;
+    return fft_prepare(cs, ips, sb.fft);
+}
+
+int spme_prepare(const std::vector<Ctx *> &cs, SpmeBatch &sb)
+{
+    if (cs[0]->prm.precision == DLPX_MIXED) return spme_prepare_t<float>(cs, sb);
+    return spme_prepare_t<double>(cs, sb);
+}
+
+template <typename R>
+static int spme_launch_t(Ctx *c, hipStream_t s, const SpmeBatch &sb)
+{
+    if (sb.S == 0) return 0;
+    const BatchLaunch &bs = sb.spread, &bg = sb.gather;
+#define DLPX_SPREADB(SS)                                                                                         \
+    hipLaunchKernelGGL((k_spread_cells_batch<R, SS>), dim3(bs.total), dim3(spread_threads<R>()), 0, s,          \
+                       (const SpreadArgs<R> *)bs.args, bs.boff, bs.nsys)
+    DLPX_S_SWITCH(sb.S, DLPX_SPREADB)
+#undef DLPX_SPREADB
+    int r;
+    if ((r = fft_launch(c, s, sb.fft))) return r;
+#define DLPX_GATHERB(SS)                                                                                         \
+    hipLaunchKernelGGL((k_gather_batch<R, SS>), dim3(bg.total), dim3(256), 0, s, (const GatherArgs<R> *)bg.args, \
+                       bg.boff, bg.nsys)
+    DLPX_S_SWITCH(sb.S, DLPX_GATHERB)
+#undef DLPX_GATHERB
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb)
+{
+    if (c->prm.precision == DLPX_MIXED) return spme_launch_t<float>(c, s, sb);
+    return spme_launch_t<double>(c, s, sb);
+}
+
+// Batched reductions (and the optional per-step records) of an ensemble.
+int reduce_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
+                   BatchLaunch &bl, int32_t **flag_ptrs)
+{
+    std::vector<ReduceArgs> a;
+    std::vector<int> nb;
+    std::vector<int32_t *> fl;
+    for (size_t k = 0; k < cs.size(); k++) {
+        Ctx *c = cs[k];
+        ReduceArgs r;
+        r.red = c->red;
+        for (int q = 0; q < kReduceSlots; q++) r.rr.rows[q] = rows[k].rows[q];
+        r.out = c->red_out;
+        r.energies = energies ? c->energies_dev : nullptr;
+        r.flags = c->flags;
+        r.ecap = c->energies_cap;
+        a.push_back(r);
+        nb.push_back(kReduceSlots);
+        fl.push_back(c->flags);
+    }
+    int r;
+    if ((r = batch_upload(cs[0], bl, a.data(), sizeof(ReduceArgs), nb, false))) return r;
+    DLPX_HIP(cs[0], hipMemcpy(flag_ptrs, fl.data(), sizeof(int32_t *) * fl.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int reduce_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
+{
+    hipLaunchKernelGGL(k_reduce_batch, dim3(bl.nsys * kReduceSlots), dim3(256), 0, s, (const ReduceArgs *)bl.args,
+                       bl.nsys);
+    DLPX_HIP(c, hipGetLastError());
+    return 0;
+}
+
+int record_next_launch(Ctx *c, hipStream_t s, int32_t *const *flag_ptrs, int nsys)
+{
+    hipLaunchKernelGGL(k_record_next, dim3(ceil_div(nsys, 256)), dim3(256), 0, s, flag_ptrs, nsys);
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

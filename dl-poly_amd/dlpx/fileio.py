@@ -297,6 +297,7 @@ _VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors"
 # parpot(4:5); mstw parpot(4))
 _VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,), 9: (0,), 10: (0,),
                       12: (0,), 13: (0, 3), 14: (0, 1), 22: (0,), 23: (0,), 7: (0,), 11: (0,)}
+_ZERO_PLUS = 2.2250738585072014e-308  # Tiny(1.0_wp), the reference's zero_plus
 # parameters a FIELD record gives (WCA's cut is derived, ffield.F90:3788)
 _VDW_FIELD_NPARAMS = {**VDW_NPARAMS, VDW_WCA: 3}
 
@@ -420,7 +421,18 @@ def read_field(path):
                     raise ValueError("vdW form %r is not supported by this path" % a[2])
                 kind = _VDW_KEYS[form]
                 params = [float(v) for v in a[3:]]
-                params += [0.0] * (_VDW_FIELD_NPARAMS[kind] - len(params))  # absent words read as 0
+                # absent words read as 0; words past the form's own parameters
+                # are read by the reference (vdws%max_param of them) but unused
+                params += [0.0] * (_VDW_FIELD_NPARAMS[kind] - len(params))
+                params = params[:_VDW_FIELD_NPARAMS[kind]]
+                if kind == 4 and abs(params[1]) <= _ZERO_PLUS:
+                    # ffield.F90:3745-3751: a zero rho is 1 when A is zero too
+                    # (pure dispersion), else error 467
+                    if abs(params[0]) <= _ZERO_PLUS:
+                        params[1] = 1.0
+                    else:
+                        raise ValueError("buck vdW pair %s-%s: rho is zero with A nonzero (DL_POLY error 467)"
+                                         % (a[0], a[1]))
                 for j in _VDW_ENERGY_PARAMS[kind]:
                     params[j] *= unit
                 if kind == VDW_WCA:

@@ -22,7 +22,7 @@ FP64, MIXED = 0, 1
 EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dlpx_set_vdw_tables",
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
            "dlpx_compute_lrc", "dlpx_get_lrc", "dlpx_link_cell_pairs", "dlpx_get_pairs", "dlpx_vnl_check",
-           "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_store",
+           "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_run_ensemble_energies", "dlpx_md_store",
            "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
            "dlpx_set_per_particle", "dlpx_add_per_particle", "dlpx_set_frozen", "dlpx_compute_lrc_frozen",
            "dlpx_set_timing", "dlpx_get_timing"]
@@ -281,21 +281,24 @@ class Context:
             pass
 
 
-def md_run_ensemble(contexts, nsteps, tsteps=None):
-    """dlpx_md_run_ensemble: nsteps resident NVE steps of every context (each
-    loaded with md_load), stepped in lockstep so different systems overlap
-    on the GPU.  tsteps defaults to each system's own time step.  Returns the
-    list rebuilds per system."""
+def md_run_ensemble(contexts, nsteps, tsteps=None, energies=False):
+    """dlpx_md_run_ensemble(_energies): nsteps resident NVE steps of every
+    context (each loaded with md_load), every stage one launch over all the
+    systems.  tsteps defaults to each system's own time step.  Returns the
+    list rebuilds per system, and with energies also an array (nsys, nsteps,
+    5) of per-step (engcpe, vircpe, engsrp, virsrp, kinetic energy)."""
     n = len(contexts)
     if n == 0:
-        return np.zeros(0, np.int32)
+        return (np.zeros(0, np.int32), np.zeros((0, nsteps, 5))) if energies else np.zeros(0, np.int32)
     ts = np.ascontiguousarray([c.sys.timestep for c in contexts] if tsteps is None else tsteps, np.float64)
     hs = (C.c_void_p * n)(*[c.h for c in contexts])
     nreb = np.zeros(n, np.int32)
     lib = contexts[0].lib
-    rc = lib.dlpx_md_run_ensemble(hs, C.c_int32(n), C.c_int32(nsteps), _ptr(ts), _ptr(nreb))
+    e = np.zeros((n, nsteps, 5), np.float64)
+    rc = lib.dlpx_md_run_ensemble_energies(hs, C.c_int32(n), C.c_int32(nsteps), _ptr(ts),
+                                           _ptr(e) if energies else None, _ptr(nreb))
     if rc != 0:
         bad = [c for c in contexts if c.lib.dlpx_last_error(c.h)]
         msg = lib.dlpx_last_error(bad[0].h).decode() if bad else "dlpx_md_run_ensemble failed"
         raise DlpxError(rc, msg)
-    return nreb
+    return (nreb, e) if energies else nreb

@@ -9,6 +9,7 @@ carry for the two-body + SPME path: cell (row vectors a, b, c as in
 per-type mass/charge, the vdW pair table (``vdw.F90`` type keys) and the
 cutoffs / Ewald / SPME settings.
 """
+import math
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
 
@@ -431,3 +432,26 @@ def pair_key(ti, tj):
     a, b = ti + 1, tj + 1
     hi, lo = max(a, b), min(a, b)
     return hi * (hi - 1) // 2 + lo - 1
+
+
+def ensemble_systems(nsys=256, seed=20261202):
+    """The batched-ensemble workload: nsys systems drawn from a seed, half LJ
+    argon and half molten NaCl (shuffled), sizes uniform in 1,000-20,000 atoms
+    rounded to whole lattices (4 m^3 argon, 8 m^3 NaCl), cubic boxes at the
+    densities of the single-system workloads, NaCl meshes the smallest
+    2^a 3^b 5^c size of at least one point per A, clamped to 32..64."""
+    rng = np.random.default_rng(seed)
+    kinds = ["argon"] * (nsys // 2) + ["nacl"] * (nsys - nsys // 2)
+    rng.shuffle(kinds)
+    out = []
+    for i, kind in enumerate(kinds):
+        n = int(rng.integers(1000, 20001))
+        if kind == "argon":
+            m = max(4, int(round((n / 4.0) ** (1.0 / 3.0))))
+            out.append(argon(ncells=m, seed=seed + 1 + i))
+        else:
+            m = max(5, int(round((n / 8.0) ** (1.0 / 3.0))))
+            L = m * 5.64 * 1.08
+            K = min(64, max(32, smooth235_at_least(int(math.ceil(L)))))
+            out.append(nacl(ncells=m, kvec=(K, K, K), seed=seed + 1 + i))
+    return out

@@ -219,14 +219,26 @@ int dlpx_md_store(dlpx_ctx *ctx, double *x, double *y, double *z, double *vx, do
 int dlpx_md_last_output(dlpx_ctx *ctx, dlpx_two_body_out *out);
 
 /* Batched ensemble: the same as dlpx_md_run(ctxs[k], nsteps, tsteps[k], NULL,
- * &nrebuilds[k]) for every k, with the systems stepped in lockstep so that
- * the steps of different systems run on the GPU at the same time (each
- * context has its own streams).  Every context must have been loaded with
- * dlpx_md_load, be on the same device and appear once.  No counterpart in
- * DL_POLY, which runs one system per process: it serves hosts that run many
- * small independent systems. */
+ * &nrebuilds[k]) for every k, with every stage of a step one launch over all
+ * the systems: per-system argument tables in device memory, each workgroup
+ * finding its system; the list rebuild of each system decided on the device
+ * by its own stage 1; the steps replaying one captured graph, with no host
+ * round trip until the run ends.  Launch shapes (pair-kernel lanes per atom,
+ * FFT lines per workgroup) are chosen per size class by timing candidates on
+ * the first steps of a long enough run, and cached (DLPX_AUTOTUNE=0 disables
+ * that; DLPX_TUNE_CACHE names a cache file).  Every context must have been
+ * loaded with dlpx_md_load, be on the same device, appear once and share the
+ * precision mode; systems with electrostatics must share the spline order.
+ * The contexts' work runs on the first context's streams.  No counterpart in
+ * DL_POLY, which runs one system per process (calculate_forces,
+ * drivers.F90:573-929): it serves hosts that run many small independent
+ * systems.  dlpx_md_run_ensemble_energies also records each system's per-step
+ * energies as dlpx_md_run does, system k's step s at energies[5*(nsteps*k +
+ * s) + {0..4}] (energies may be NULL). */
 int dlpx_md_run_ensemble(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
                          int32_t *nrebuilds);
+int dlpx_md_run_ensemble_energies(dlpx_ctx *const *ctxs, int32_t nctx, int32_t nsteps, const double *tsteps,
+                                  double *energies, int32_t *nrebuilds);
 
 /* Per-stage device time (ms, HIP events on the context's stream), summed over
  * the force evaluations since dlpx_set_timing(ctx, 1): ms[0] pair forces,

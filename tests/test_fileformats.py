@@ -72,6 +72,16 @@ def test_field_reader_units_and_repeats():
     assert lj2 == ("C", "C", 2, (100.0, 3.4))
 
 
+def test_field_reader_rejects_buckingham_with_zero_rho():
+    """ffield.F90:3745-3751: rho = 0 with A nonzero is DL_POLY's error 467."""
+    text = FIELD_KJ.replace("C  C  lj   1.0   3.4", "C  C  buck 2.0 0.0 5.0")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "FIELD")
+        open(p, "w").write(text)
+        with pytest.raises(ValueError, match="467"):
+            fileio.read_field(p)
+
+
 @pytest.mark.parametrize("bad", ["bonds 1\nharm 1 2 100.0 1.0\n", "bonds 1\nmors 1 2 0.0 1.0 2.0\n",
                                  "constraints 1\n1 2 1.0\n", None])
 def test_field_reader_rejects_what_the_path_lacks(bad):
@@ -101,6 +111,11 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
     ("ndpd 3.0 2.0 3.0 9.5", (300.0, 2.0, 3.0, 9.5)),
     ("snm  1.0 12.0 6.0 3.7 8.5", (100.0, 12.0, 6.0, 3.7, 8.5)),
     ("14-7 1.0 3.9", (100.0, 3.9)),
+    # ffield.F90:3745-3751: rho = 0 with A = 0 (pure dispersion) reads as rho = 1
+    ("buck 0.0 0.0 5.0", (0.0, 1.0, 500.0)),
+    # words past the form's own parameters are read and ignored
+    ("bhm  1.0 3.0 2.5 5.0 6.0 7.0", (100.0, 3.0, 2.5, 500.0, 600.0)),
+    ("snm  1.0 12.0 6.0 3.7 8.5 9.9", (100.0, 12.0, 6.0, 3.7, 8.5)),
 ])
 def test_field_reader_completes_other_forms(rec, want):
     text = FIELD_KJ.replace("C  C  lj   1.0   3.4", "C  C  " + rec)

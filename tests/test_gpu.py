@@ -532,6 +532,59 @@ def test_ensemble_matches_separate_runs():
         ctx.close()
 
 
+def _ensemble_sample(syss, n=32):
+    """Indices of n systems of the ensemble: the smallest and largest of each
+    kind, one per NaCl mesh edge, the rest spread evenly over each kind."""
+    pick = []
+    for kind in (False, True):
+        idx = [k for k, s in enumerate(syss) if bool(s.electrostatics) == kind]
+        idx.sort(key=lambda k: syss[k].natms)
+        want = [idx[0], idx[-1]]
+        if kind:
+            for K in sorted(set(syss[k].kvec[0] for k in idx)):
+                want.append(next(k for k in idx if syss[k].kvec[0] == K))
+        for k in np.linspace(0, len(idx) - 1, n // 2).astype(int):
+            if len(set(want)) >= n // 2:
+                break
+            want.append(idx[k])
+        pick += list(dict.fromkeys(want))[: n // 2]
+    return pick
+
+
+def test_ensemble256_sampled_systems_match_oracle():
+    """The 256-system batched ensemble (systems.ensemble_systems(), seed
+    20261202: half argon, half NaCl with meshes 40-64 including the odd 45),
+    stepped as one batch: after a few resident steps, the forces and the last
+    step's energies of 32 sampled systems against the oracle at the positions
+    reached.  FP64 forces <= 1e-10 RMS relative, energies <= 1e-11 of their
+    largest term; mixed forces <= 1e-5, energies <= 1e-6."""
+    import concurrent.futures as cf
+    syss = systems.ensemble_systems()
+    pick = _ensemble_sample(syss)
+    assert len(pick) == 32
+    steps = 4
+    for precision, tf, te in ((lib.FP64, 1e-10, 1e-11), (lib.MIXED, 1e-5, 1e-6)):
+        print("case ensemble256 precision", precision, flush=True)
+        ctxs = []
+        for s in syss:
+            ctx = lib.Context(s, precision=precision)
+            ctx.md_load(s.pos, s.vel)
+            ctxs.append(ctx)
+        nreb, e = lib.md_run_ensemble(ctxs, steps, energies=True)
+        states = {k: ctxs[k].md_store() for k in pick}
+        for c in ctxs:
+            c.close()
+        with cf.ThreadPoolExecutor(8) as ex:
+            orc = dict(zip(pick, ex.map(lambda k: oracle.two_body(syss[k], pos=states[k][0]), pick)))
+        for k in pick:
+            x, v, f = states[k]
+            o = orc[k]
+            err = rms_rel(f, o["frc"])
+            assert err <= tf, (k, syss[k].natms, err)
+            last = dict(zip(("engcpe", "vircpe", "engsrp", "virsrp"), e[k, -1, :4]))
+            check_energies(last, o, te, coulomb_scales(o) if syss[k].electrostatics else None)
+
+
 def test_nve_energy_conservation_fp64_and_mixed():
     s = systems.nacl(ncells=8, kvec=(48, 48, 48), seed=3)
     for prec, tol in ((lib.FP64, 2e-5), (lib.MIXED, 5e-5)):

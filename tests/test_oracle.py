@@ -293,22 +293,50 @@ def test_vdw_lrc_matches_tail_integrals(make):
     rc = s.vdw_cutoff
     e = v = 0.0
     for (ti, tj, kind, p) in s.vdw:
+        mult = 1.0 if ti == tj else 2.0
+        fac = mult * 2 * np.pi * n[ti] * n[tj] / V
+        if kind == 11:
+            # AMOEBA: the reference's own quadrature (vdw.F90:522-613, called
+            # at :795-796), whose chunks each recount one panel, so the energy
+            # lies a few 1e-6 above the integral, and whose virial integrand is
+            # not the derivative of the form
+            e += fac * _amoeba_sum(p[1], p[0], rc, virial=False)
+            v += fac * _amoeba_sum(p[1], p[0], rc, virial=True)
+            continue
         ut = _tail(kind, p)
         h = 1e-4
         dut = lambda r: (ut(r + h) - ut(r - h)) / (2 * h)
-        mult = 1.0 if ti == tj else 2.0
-        e += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r * r * ut(r), rc, np.inf)[0]
-        v += mult * 2 * np.pi * n[ti] * n[tj] / V * integrate.quad(lambda r: r ** 3 * dut(r), rc, np.inf)[0]
-    # AMOEBA's corrections are the reference's chunked Simpson sums, whose
-    # chunks each recount one panel (vdw.F90:552-566): the energy lies a few
-    # 1e-6 above the integral.  Its virial integrand (dmm3, vdw.F90:579) is not
-    # the derivative of the form (no eps and the opposite sign on the x^6
-    # term), so that one is checked only through the reference fixtures of the
-    # third mixture, which pin both values (engsrp, virsrp).
-    amoeba = any(k == 11 for (_, _, k, _) in s.vdw)
-    assert abs(elrc - e) < (2e-5 if amoeba else 1e-9) * abs(e)
-    if not amoeba:
-        assert abs(vlrc - v) < 1e-7 * abs(v)
+        e += fac * integrate.quad(lambda r: r * r * ut(r), rc, np.inf)[0]
+        v += fac * integrate.quad(lambda r: r ** 3 * dut(r), rc, np.inf)[0]
+    assert abs(elrc - e) < 1e-9 * abs(e)
+    assert abs(vlrc - v) < 1e-7 * abs(v)
+
+
+def _amoeba_sum(r0, eps, rw, virial, a=0.07, b=0.12, prec=1.0e-12):
+    """intRadMM3 / intRaddMM3 of vdw.F90:522-613 in numpy: Simpson panels of
+    width h = (rw/r0)/10000 over chunks of 10000 steps from x = rw/r0, each
+    chunk's first panel starting where the last one ended (so one panel per
+    chunk is counted twice), until a chunk adds less than prec.  Returns the
+    energy integral, or the virial one (the reference's padd with its sign
+    flipped, which enters vlrc with the energy's sign)."""
+    def mm3(x):
+        return eps * ((1.0 + a) / (a + x)) ** 7 * ((1.0 + b) / (b + x ** 7) - 2.0)
+
+    def dmm3(x):
+        ia, ib = 1.0 / (a + x), 1.0 / (b + x ** 7)
+        return -7.0 * (mm3(x) * ia - ((1.0 + a) * ia) ** 7 * ((1.0 + b) * ib ** 2) * x ** 6) / r0
+
+    f, pw, sc = (dmm3, 3, 4) if virial else (mm3, 2, 3)
+    n = 10000
+    xs = rw / r0
+    h = (2 * xs - xs) / n
+    s, sold, j = 0.0, np.finfo(float).max, 1
+    while abs(s - sold) * h / 3.0 > prec:
+        sold = s
+        x = xs + np.arange((j - 1) * n, j * n + 1, 2) * h
+        s += float(np.sum(x ** pw * f(x) + 4.0 * (x + h) ** pw * f(x + h) + (x + 2.0 * h) ** pw * f(x + 2.0 * h)))
+        j += 1
+    return s * h * r0 ** sc / 3.0
 
 
 def _erf_over_r_series(a, r, terms=30):
