@@ -104,11 +104,11 @@ int upload_tables(Ctx *c)
         std::vector<float> vf(v.begin(), v.end()), ef(e.begin(), e.end());
         for (auto &x : vf)
             if (!std::isfinite(x)) x = 3.0e38f;  // slot 0 holds Huge
-        DLPX_HIP(c, hipMemcpy(c->d_vdw, vf.data(), rs * vf.size(), hipMemcpyHostToDevice));
-        DLPX_HIP(c, hipMemcpy(c->d_erfc, ef.data(), rs * ef.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, dev_upload(c->stream, c->d_vdw, vf.data(), rs * vf.size()));
+        DLPX_HIP(c, dev_upload(c->stream, c->d_erfc, ef.data(), rs * ef.size()));
     } else {
-        DLPX_HIP(c, hipMemcpy(c->d_vdw, v.data(), rs * v.size(), hipMemcpyHostToDevice));
-        DLPX_HIP(c, hipMemcpy(c->d_erfc, e.data(), rs * e.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, dev_upload(c->stream, c->d_vdw, v.data(), rs * v.size()));
+        DLPX_HIP(c, dev_upload(c->stream, c->d_erfc, e.data(), rs * e.size()));
     }
     std::vector<int32_t> kt((size_t)nt * nt);
     for (int a = 1; a <= nt; a++)
@@ -121,7 +121,7 @@ int upload_tables(Ctx *c)
     if (c->d_keytab) (void)hipFree(c->d_keytab);
     c->d_keytab = nullptr;
     DLPX_HIP(c, hipMalloc((void **)&c->d_keytab, sizeof(int32_t) * kt.size()));
-    DLPX_HIP(c, hipMemcpy(c->d_keytab, kt.data(), sizeof(int32_t) * kt.size(), hipMemcpyHostToDevice));
+    DLPX_HIP(c, dev_upload(c->stream, c->d_keytab, kt.data(), sizeof(int32_t) * kt.size()));
     if (c->vdw_direct) {
         // vdw_method direct: per key {form, 5 parameters, afs, bfs}; afs, bfs
         // follow vdw_direct_fs_generate (vdw.F90:969-1048) under force-shifting
@@ -145,9 +145,9 @@ int upload_tables(Ctx *c)
         DLPX_HIP(c, hipMalloc(&c->d_vdw_direct, rs * d.size()));
         if (mixed) {
             std::vector<float> df(d.begin(), d.end());
-            DLPX_HIP(c, hipMemcpy(c->d_vdw_direct, df.data(), rs * df.size(), hipMemcpyHostToDevice));
+            DLPX_HIP(c, dev_upload(c->stream, c->d_vdw_direct, df.data(), rs * df.size()));
         } else {
-            DLPX_HIP(c, hipMemcpy(c->d_vdw_direct, d.data(), rs * d.size(), hipMemcpyHostToDevice));
+            DLPX_HIP(c, dev_upload(c->stream, c->d_vdw_direct, d.data(), rs * d.size()));
         }
     }
     c->tables_dirty = false;
@@ -511,8 +511,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, F_WORDS)) ||
         (r = dalloc(c, &c->red_out, kReduceSlots)))
         return bail(r);
-    if (hipMemset(c->vel, 0, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
-    if (hipMemset(c->flags, 0, sizeof(int32_t) * F_WORDS) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (dev_zero(c->stream, c->vel, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
+    if (dev_zero(c->stream, c->flags, sizeof(int32_t) * F_WORDS) != hipSuccess) return bail(DLPX_ERR_HIP);
     c->serial = next_serial();
     // host words the GPU writes directly (the displacement maximum of each step)
     if (hipHostMalloc((void **)&c->h_flags, 64, hipHostMallocMapped) != hipSuccess) return bail(DLPX_ERR_HIP);
@@ -715,8 +715,8 @@ int dlpx_set_exclusions(dlpx_ctx *ctx, int32_t natms, int32_t mxexcl, const int3
     if (ids.empty()) return 0;
     int r;
     if ((r = dalloc(c, &c->excl_off, off.size())) || (r = dalloc(c, &c->excl_ids, ids.size()))) return r;
-    DLPX_HIP(c, hipMemcpy(c->excl_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
-    DLPX_HIP(c, hipMemcpy(c->excl_ids, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice));
+    DLPX_HIP(c, dev_upload(c->stream, c->excl_off, off.data(), sizeof(int32_t) * off.size()));
+    DLPX_HIP(c, dev_upload(c->stream, c->excl_ids, ids.data(), sizeof(int32_t) * ids.size()));
     c->excl_natms = natms;
     c->excl_total = (int)ids.size();
     return 0;
@@ -742,7 +742,7 @@ int dlpx_set_frozen(dlpx_ctx *ctx, int32_t natms, const int32_t *lfrzn)
     if (nf == 0) return 0;
     int r;
     if ((r = dalloc(c, &c->frz, (size_t)natms))) return r;
-    DLPX_HIP(c, hipMemcpy(c->frz, f.data(), sizeof(int32_t) * natms, hipMemcpyHostToDevice));
+    DLPX_HIP(c, dev_upload(c->stream, c->frz, f.data(), sizeof(int32_t) * natms));
     c->frz_natms = natms;
     c->nfrozen = nf;
     return 0;

@@ -39,8 +39,8 @@ int batch_upload(Ctx *c, BatchLaunch &bl, const void *args, size_t bytes_each, c
     bl.total = xcd ? xcd_grid(off[nsys]) : off[nsys];
     DLPX_HIP(c, hipMalloc(&bl.args, std::max<size_t>(bytes_each * nsys, 1)));
     DLPX_HIP(c, hipMalloc((void **)&bl.boff, sizeof(int32_t) * (nsys + 1)));
-    DLPX_HIP(c, hipMemcpy(bl.args, args, bytes_each * nsys, hipMemcpyHostToDevice));
-    DLPX_HIP(c, hipMemcpy(bl.boff, off.data(), sizeof(int32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    DLPX_HIP(c, dev_upload(c->stream, bl.args, args, bytes_each * nsys));
+    DLPX_HIP(c, dev_upload(c->stream, bl.boff, off.data(), sizeof(int32_t) * (nsys + 1)));
     return 0;
 }
 
@@ -408,6 +408,10 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         int32_t fl[F_WORDS];
         DLPX_HIP(c, hipMemcpy(fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost));
         rebuilds[k] = fl[F_NREB];
+        if (fl[F_ERR] != 0) {
+            c->have_list = false;
+            return fail(c, DLPX_ERR_STATE, "internal error: a list build met an atom index out of range");
+        }
         c->have_list = true;
         c->finish_in_vv2 = false;
         if (fl[F_OVF_ANY] != 0) {

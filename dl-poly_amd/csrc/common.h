@@ -83,6 +83,7 @@ enum Flag : int {
     F_UPD = 7,      // 1: the list is rebuilt before the coming force evaluation
     F_NREB = 8,     // rebuilds decided on the device since the run began
     F_STEP = 9,     // per-step energy records written since the run began
+    F_ERR = 10,     // sticky: a list-build kernel met an index outside the atoms (internal error)
     F_WORDS = 16
 };
 
@@ -302,6 +303,21 @@ struct RedRowsHost {
 int batch_upload(Ctx *c, BatchLaunch &bl, const void *args, size_t bytes_each, const std::vector<int> &nblocks,
                  bool xcd);
 void batch_free(BatchLaunch &bl);
+
+// Device memory clears and host-to-device copies for set-up code: issued on
+// the given stream and waited for before returning.  The context streams are
+// non-blocking, so the null stream's hipMemset/hipMemcpy would not be ordered
+// with the kernels queued on them.  Never used inside a stream capture.
+inline hipError_t dev_zero(hipStream_t s, void *p, size_t bytes)
+{
+    const hipError_t e = hipMemsetAsync(p, 0, bytes, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+inline hipError_t dev_upload(hipStream_t s, void *dst, const void *src, size_t bytes)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
 
 // error helpers
 int fail(Ctx *c, int code, const std::string &msg);

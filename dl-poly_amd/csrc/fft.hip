@@ -29,6 +29,7 @@ static FftPlan make_plan(int n)
     FftPlan p;
     p.n = n;
     p.np = 0;
+    if (n <= 0) return p;  // no mesh (a system without electrostatics)
     int m = n;
     while (m % 4 == 0) { p.rad[p.np++] = 4; m /= 4; }
     while (m % 2 == 0) { p.rad[p.np++] = 2; m /= 2; }
@@ -832,11 +833,13 @@ static int fft_prepare_t(const std::vector<Ctx *> &cs, const std::vector<InflPar
     for (size_t k = 0; k < cs.size(); k++) {
         Ctx *c = cs[k];
         const bool on = c->prm.electrostatics != 0;
-        r2c.push_back(r2c_args<T>(c, c->Q, true));
-        c2r.push_back(c2r_args<T>(c, c->Phi));
-        yf.push_back(y_args<T>(c, +1, ly));
-        yb.push_back(y_args<T>(c, -1, ly));
-        zc.push_back(z_args<T>(c, ips[k], lz));
+        // a system without electrostatics has no mesh: zero workgroups and
+        // an empty argument struct
+        r2c.push_back(on ? r2c_args<T>(c, c->Q, true) : R2CArgs<T>{});
+        c2r.push_back(on ? c2r_args<T>(c, c->Phi) : C2RArgs<T>{});
+        yf.push_back(on ? y_args<T>(c, +1, ly) : C2CArgs<T>{});
+        yb.push_back(on ? y_args<T>(c, -1, ly) : C2CArgs<T>{});
+        zc.push_back(on ? z_args<T>(c, ips[k], lz) : ZConvArgs<T>{});
         nbx.push_back(on ? x_blocks(c, lx) : 0);
         nby.push_back(on ? ceil_div(yf.back().lm.nl, ly) : 0);
         nbz.push_back(on ? ceil_div(zc.back().lm.nl, lz) : 0);

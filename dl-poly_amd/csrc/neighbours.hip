@@ -163,6 +163,7 @@ struct ScatterArgs {
     int32_t *fill, *perm;
     int n;
     const int32_t *upd;
+    int32_t *flags;  // F_ERR on an inconsistent count
 };
 __device__ __forceinline__ void scatter_body(const ScatterArgs &a, int bid)
 {
@@ -175,7 +176,9 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs &a, int bid)
     int base = 0;
     if (lane == head) base = atomicAdd(&a.fill[c], len);
     base = __shfl(base, head, 64);
-    a.perm[base + (lane - head)] = i;
+    const int dst = base + (lane - head);
+    if ((unsigned)dst < (unsigned)a.n) a.perm[dst] = i;
+    else a.flags[F_ERR] = 1;
 }
 
 // Make the order inside each cell deterministic (ascending previous index):
@@ -231,19 +234,28 @@ struct PermArgs {
     int32_t *slot;
     double4 *xref;
     const int32_t *upd;
+    int32_t *flags;  // F_ERR on an index outside the atoms
 };
 __device__ __forceinline__ void permute_body(const PermArgs &a, int bid)
 {
     const int i = bid * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    // perm and gid hold indices below n by construction; the test keeps a
+    // broken count (a kernel of the chain that did not run) from turning into
+    // stray memory accesses, and is reported as an internal error
     const int o = a.perm[i];
+    if ((unsigned)o >= (unsigned)a.n) {
+        a.flags[F_ERR] = 1;
+        return;
+    }
     const double4 p = a.pos[o];
     a.pos_new[i] = p;
     a.xref[i] = p;
     a.typ_new[i] = a.typ[o];
     const int g = a.gid[o];
     a.gid_new[i] = g;
-    a.slot[g] = i;
+    if ((unsigned)g < (unsigned)a.n) a.slot[g] = i;
+    else a.flags[F_ERR] = 1;
     if (a.vel) a.vel_new[i] = a.vel[o];
 }
 
@@ -865,7 +877,7 @@ static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
         if ((r = grow(c, &c->cell_fill, nkey + 1))) return r;
         if ((r = grow(c, &c->key_start, nkey + 1))) return r;
         // the scan zeroes the counts it has read, so they are zero between builds
-        DLPX_HIP(c, hipMemset(c->cell_count, 0, sizeof(int32_t) * (nkey + 1)));
+        DLPX_HIP(c, dev_zero(c->stream, c->cell_count, sizeof(int32_t) * (nkey + 1)));
         c->nkey_cap = nkey + 1;
     }
     c->ncell = ncell;
@@ -878,12 +890,12 @@ static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
     lc.sub = sub;
     lc.wrap = WrapArgs{c->pos, n, g, c->cell_of, c->cell_count, upd};
     lc.scan = ScanArgs{c->cell_count, kstart, c->cell_fill, nkey, sub ? c->cell_start : nullptr, upd};
-    lc.scat = ScatterArgs{c->cell_of, c->cell_fill, c->perm, n, upd};
+    lc.scat = ScatterArgs{c->cell_of, c->cell_fill, c->perm, n, upd, c->flags};
     lc.sort = SortArgs{c->perm, kstart, nkey, upd};
     const bool res = c->resident;
     lc.perm = PermArgs{c->perm, n,      c->pos,   c->pos_alt, c->typ,  c->typ_alt,
                        c->gid,  c->gid_alt, res ? c->vel : nullptr, res ? c->vel_alt : nullptr,
-                       c->slot, c->xref, upd};
+                       c->slot, c->xref, upd, c->flags};
     lc.copy = CopyArgs{n, c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt, c->gid,
                        res ? c->vel_alt : nullptr, res ? c->vel : nullptr, upd};
     lc.build = BuildArgs{c->pos, g, sub ? c->key_start : c->cell_start, rc_ext * rc_ext, c->cap_lane,
@@ -946,7 +958,12 @@ int build_list(Ctx *c)
         launch_build(lc, s);
         if (c->timing) DLPX_HIP(c, hipEventRecord(c->ev[7], s));
         DLPX_HIP(c, hipMemcpyAsync(c->h_flags, c->flags, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
+        DLPX_HIP(c, hipMemcpyAsync(c->h_flags + 10, c->flags + F_ERR, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         DLPX_HIP(c, hipStreamSynchronize(s));
+        if (c->h_flags[10] != 0) {
+            c->have_list = false;
+            return fail(c, DLPX_ERR_STATE, "internal error: the list build met an atom index out of range");
+        }
         if (c->timing) {
             float ms;
             DLPX_HIP(c, hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));

@@ -712,7 +712,7 @@ int spme_setup(Ctx *c)
     c->nhalf = (size_t)c->K1hp * c->K[1] * c->K[2];
     const size_t rs = c->prm.precision == DLPX_MIXED ? sizeof(float) : sizeof(double);
     DLPX_HIP(c, hipMalloc(&c->Q, rs * c->nreal));
-    DLPX_HIP(c, hipMemset(c->Q, 0, rs * c->nreal));  // kept zero from here on by the forward x pass
+    DLPX_HIP(c, dev_zero(c->stream, c->Q, rs * c->nreal));  // kept zero from here on by the forward x pass
     DLPX_HIP(c, hipMalloc(&c->Phi, rs * c->nreal));
     DLPX_HIP(c, hipMalloc(&c->H, 2 * rs * c->nhalf));
     const int ksum = c->K[0] + c->K[1] + c->K[2];
@@ -721,7 +721,7 @@ int spme_setup(Ctx *c)
     bspline_norm2(c->K[1], S, n2.data() + c->K[0]);
     bspline_norm2(c->K[2], S, n2.data() + c->K[0] + c->K[1]);
     DLPX_HIP(c, hipMalloc((void **)&c->bsp_n2, sizeof(double) * ksum));
-    DLPX_HIP(c, hipMemcpy(c->bsp_n2, n2.data(), sizeof(double) * ksum, hipMemcpyHostToDevice));
+    DLPX_HIP(c, dev_upload(c->stream, c->bsp_n2, n2.data(), sizeof(double) * ksum));
     // twiddles exp(+2 pi i m/K) per axis, in the grid precision
     std::vector<double> twd(2 * (size_t)ksum);
     size_t o = 0;
@@ -734,10 +734,10 @@ int spme_setup(Ctx *c)
     if (c->prm.precision == DLPX_MIXED) {
         std::vector<float> twf(twd.begin(), twd.end());
         DLPX_HIP(c, hipMalloc(&c->tw, sizeof(float) * twf.size()));
-        DLPX_HIP(c, hipMemcpy(c->tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, dev_upload(c->stream, c->tw, twf.data(), sizeof(float) * twf.size()));
     } else {
         DLPX_HIP(c, hipMalloc(&c->tw, sizeof(double) * twd.size()));
-        DLPX_HIP(c, hipMemcpy(c->tw, twd.data(), sizeof(double) * twd.size(), hipMemcpyHostToDevice));
+        DLPX_HIP(c, dev_upload(c->stream, c->tw, twd.data(), sizeof(double) * twd.size()));
     }
     return 0;
 }
@@ -853,8 +853,8 @@ static int spme_prepare_t(const std::vector<Ctx *> &cs, SpmeBatch &sb)
                 return fail(c, DLPX_ERR_UNSUPPORTED, "systems of one batch must share the spline order");
             S = c->prm.nsplines;
         }
-        spa.push_back(spread_args<R>(c));
-        ga.push_back(gather_args<R>(c));
+        spa.push_back(on ? spread_args<R>(c) : SpreadArgs<R>{});
+        ga.push_back(on ? gather_args<R>(c) : GatherArgs<R>{});
         ips.push_back(on ? infl_params(c) : InflParams{});
         nbs.push_back(on ? c->ncell : 0);
         nbg.push_back(on ? gather_blocks_exact(c) : 0);
@@ -923,7 +923,7 @@ int reduce_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> 
     }
     int r;
     if ((r = batch_upload(cs[0], bl, a.data(), sizeof(ReduceArgs), nb, false))) return r;
-    DLPX_HIP(cs[0], hipMemcpy(flag_ptrs, fl.data(), sizeof(int32_t *) * fl.size(), hipMemcpyHostToDevice));
+    DLPX_HIP(cs[0], dev_upload(cs[0]->stream, flag_ptrs, fl.data(), sizeof(int32_t *) * fl.size()));
     return 0;
 }
 
