@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -453,6 +454,38 @@ int md_fallback(Ctx *c, int nsteps, double tstep, bool energies, int &rebuilds)
     return 0;
 }
 
+// Scratch of dlpx_two_body_forces_noalloc: the partial-sum rows of one force
+// evaluation (one row of kReduceSlots doubles per workgroup of a reducing
+// kernel) followed by their kReduceSlots column totals.  kScratchRows rows
+// (7.3 MB) cover up to about a million atoms at spline order 8, where the
+// gather has one workgroup per 32 atoms.  Each device holds its own copy
+// until the process ends; every call on a device shares it.  Host code uses
+// the address hipGetSymbolAddress gives for the device, never the array's
+// name (a host-side stand-in).
+constexpr int kScratchRows = 32768;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(size_t)kScratchRows * kReduceSlots + kReduceSlots];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (dlpx_create runs on it).
+static int scratch_address(Ctx *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, DLPX_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[c->device]) return 0;
+    void *p = nullptr;
+    DLPX_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[c->device] = (double *)p;
+    return 0;
+}
+
+static double *scratch_of(const Ctx *c)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+}
+
 }  // namespace dlpx
 
 using namespace dlpx;
@@ -524,6 +557,7 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     if (hipHostMalloc((void **)&c->h_t, sizeof(int32_t) * n, hipHostMallocDefault) != hipSuccess)
         return bail(DLPX_ERR_HIP);
     if (p->electrostatics && (r = spme_setup(c))) return bail(r);
+    if ((r = scratch_address(c))) return bail(r);
     *out = reinterpret_cast<dlpx_ctx *>(c);
     return 0;
 }
@@ -828,6 +862,12 @@ int dlpx_link_cell_pairs(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
     r = build_list(c);
     if (ibig) *ibig = c->h_flags[1];
     if (r) return r;
+    // tables already given go to the device with the list, so that the force
+    // calls after it need no allocation of their own
+    const int nt = c->prm.ntypes;
+    if (c->tables_dirty && c->nkeys == nt * (nt + 1) / 2 && (!c->prm.electrostatics || c->nsamples > 0) &&
+        (r = upload_tables(c)))
+        return r;
     if (npairs) *npairs = (int64_t)c->h_flags[3] / 2;
     return 0;
 }
@@ -916,6 +956,64 @@ int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
     if ((r = fetch_red(c))) return r;
     const int st = atoms->stride;
     for (int i = 0; i < n; i++) {
+        const double *f = c->h_st + (size_t)9 * i + 6;
+        *(double *)((char *)atoms->fx + (size_t)st * i) += f[0];
+        *(double *)((char *)atoms->fy + (size_t)st * i) += f[1];
+        *(double *)((char *)atoms->fz + (size_t)st * i) += f[2];
+    }
+    assemble(c, c->h_red, out);
+    return 0;
+}
+
+int dlpx_two_body_forces_noalloc(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms,
+                                 dlpx_two_body_out *out)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out || !atoms) return DLPX_ERR_BAD_ARG;
+    if (!c->have_list) return fail(c, DLPX_ERR_STATE, "call dlpx_link_cell_pairs first");
+    if (!atoms->fx || !atoms->fy || !atoms->fz) return fail(c, DLPX_ERR_BAD_ARG, "force arrays");
+    if (c->per_particle)
+        return fail(c, DLPX_ERR_UNSUPPORTED, "per-particle output needs buffers of its own: use dlpx_two_body_forces");
+    if (c->tables_dirty)
+        return fail(c, DLPX_ERR_STATE, "tables changed since the list build: call dlpx_link_cell_pairs first");
+    double *scr = scratch_of(c);
+    if (!scr) return fail(c, DLPX_ERR_STATE, "no scratch address for this device");
+    int r;
+    if ((r = set_cell(c, cell))) return r;
+    if ((r = upload(c, atoms, false, nullptr, nullptr, nullptr, nullptr))) return r;
+    const int rows = std::max({pair_blocks(c), gather_blocks(c), conv_blocks(c), excl_blocks(c)});
+    if (rows > kScratchRows)
+        return fail(c, DLPX_ERR_BAD_ARG,
+                    "the evaluation needs " + std::to_string(rows) + " partial-sum rows, beyond the " +
+                        std::to_string(kScratchRows) + " of the allocation-free scratch (about " +
+                        std::to_string(32 * kScratchRows) + " atoms at spline order 8)");
+    charges_summary(c, atoms);
+    c->pp_valid = false;
+    // the evaluation writes its partial sums and totals into the scratch: every
+    // reducing kernel writes each row it owns before the reduction reads it,
+    // so nothing needs clearing between calls
+    double *const red = c->red, *const red_out = c->red_out;
+    const int red_cap = c->red_cap;
+    c->red = scr;
+    c->red_cap = kScratchRows;
+    c->red_out = scr + (size_t)kScratchRows * kReduceSlots;
+    r = evaluate(c);
+    if (!r) r = store_atoms(c, c->stage, false, false);
+    if (!r) {
+        const hipError_t e = hipMemcpyAsync(c->h_st, c->stage, sizeof(double) * 9 * c->natms,
+                                            hipMemcpyDeviceToHost, c->stream);
+        r = e != hipSuccess ? fail(c, DLPX_ERR_HIP + (int)e, std::string("hipMemcpyAsync: ") + hipGetErrorString(e))
+                            : fetch_red(c);  // waits for the stream: the scratch is free again on return
+    }
+    c->red = red;
+    c->red_cap = red_cap;
+    c->red_out = red_out;
+    if (r) {
+        (void)hipStreamSynchronize(c->stream);
+        return r;
+    }
+    const int st = atoms->stride;
+    for (int i = 0; i < c->natms; i++) {
         const double *f = c->h_st + (size_t)9 * i + 6;
         *(double *)((char *)atoms->fx + (size_t)st * i) += f[0];
         *(double *)((char *)atoms->fy + (size_t)st * i) += f[1];

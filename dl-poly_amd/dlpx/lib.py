@@ -25,7 +25,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_run_ensemble_energies", "dlpx_md_store",
            "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
            "dlpx_set_per_particle", "dlpx_add_per_particle", "dlpx_set_frozen", "dlpx_compute_lrc_frozen",
-           "dlpx_set_timing", "dlpx_get_timing"]
+           "dlpx_set_timing", "dlpx_get_timing", "dlpx_two_body_forces_noalloc"]
 
 
 class DlpxError(RuntimeError):
@@ -109,6 +109,22 @@ class Context:
         if frz is not None and np.any(frz):
             f = np.ascontiguousarray(np.asarray(frz, bool).astype(np.int32))
             self._chk(self.lib.dlpx_set_frozen(self.h, C.c_int32(len(f)), _ptr(f)))
+
+    def set_system(self, sysm):
+        """Use the context for another configuration of the same species, with
+        at most the natms_max it was created for (DL_POLY's natms changes the
+        same way between list builds): the host-side arrays switch to sysm and
+        the vdW long-range correction is recomputed for its composition and
+        volume.  Call link_cell_pairs next."""
+        if sysm.ntypes != self.sys.ntypes:
+            raise ValueError("set_system needs the same atom types")
+        self.sys = sysm
+        self.n = sysm.natms
+        self.cell = np.ascontiguousarray(sysm.cell, dtype=np.float64)
+        self.q = np.ascontiguousarray(sysm.q)
+        self.ltype = np.ascontiguousarray(sysm.types + 1, dtype=np.int32)
+        num = np.bincount(sysm.types, minlength=sysm.ntypes).astype(np.float64)
+        self._chk(self.lib.dlpx_compute_lrc(self.h, _ptr(self.cell), _ptr(num)))
 
     def set_exclusions(self, excl):
         """Excluded pairs: one sequence of 0-based partner indices per atom
@@ -211,16 +227,18 @@ class Context:
         self._chk(self.lib.dlpx_vnl_check(self.h, C.byref(a), C.byref(u)))
         return bool(u.value)
 
-    def two_body_forces(self, pos, frc=None, per_particle=False):
+    def two_body_forces(self, pos, frc=None, per_particle=False, noalloc=False):
         """Forces are added into frc (zeroed here when not given).  With
         per_particle=True the result also holds pp_energy (N) and pp_stress
-        (N, 9; element a + 3b of atom i is stats%pp_stress(a+3b+1, i))."""
+        (N, 9; element a + 3b of atom i is stats%pp_stress(a+3b+1, i)).  With
+        noalloc=True the call goes through dlpx_two_body_forces_noalloc."""
         if frc is None:
             frc = np.zeros((self.n, 3))
         a = self._atoms(pos, frc)
         o = TwoBodyOut()
         self._chk(self.lib.dlpx_set_per_particle(self.h, C.c_int32(1 if per_particle else 0)))
-        self._chk(self.lib.dlpx_two_body_forces(self.h, _ptr(self.cell), C.byref(a), C.byref(o)))
+        entry = self.lib.dlpx_two_body_forces_noalloc if noalloc else self.lib.dlpx_two_body_forces
+        self._chk(entry(self.h, _ptr(self.cell), C.byref(a), C.byref(o)))
         res = dict(frc=frc, buffer=np.array(o.buffer[:]), stress=np.array(o.stress[:]), engcpe=o.engcpe,
                    vircpe=o.vircpe, engsrp=o.engsrp, virsrp=o.virsrp)
         if per_particle:

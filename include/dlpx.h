@@ -191,6 +191,18 @@ int dlpx_vnl_check(dlpx_ctx *ctx, const dlpx_atoms *atoms, int32_t *update);
  * in dlpx_two_body_out. */
 int dlpx_two_body_forces(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms,
                          dlpx_two_body_out *out);
+/* dlpx_two_body_forces without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in DL_POLY.  Same inputs, outputs and
+ * results; the partial sums live in a fixed scratch array of the library
+ * (7.3 MB per device: up to about a million atoms at spline order 8; larger
+ * systems return DLPX_ERR_BAD_ARG).  Preconditions: the list was built with
+ * dlpx_link_cell_pairs after the last table change (DLPX_ERR_STATE
+ * otherwise), and per-particle output is off (DLPX_ERR_UNSUPPORTED).  Every
+ * context on a device shares that device's scratch, so calls that overlap in
+ * time on one device (from several host threads) race: serialise them. */
+int dlpx_two_body_forces_noalloc(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *atoms,
+                                 dlpx_two_body_out *out);
 
 /* Resident mode: positions, velocities and forces stay in HBM between steps.
  * dlpx_md_load copies a configuration in: vx/vy/vz and mass are read with
