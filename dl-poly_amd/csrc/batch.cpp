@@ -337,7 +337,10 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         tune_load();
         auto it = g_tune.find(cls);
         if (it != g_tune.end()) sh = it->second;
-        else tune = tune_enabled() && !energies && nsteps >= 40;
+        // a batch of several systems is tuned on its own first steps; a single
+        // system keeps the default shape (8 pair lanes per atom, 16 FFT lines
+        // per workgroup), the best one measured at 4k-1.7M atoms
+        else tune = tune_enabled() && !energies && nsteps >= 40 && cs.size() > 1;
     }
     for (Ctx *c : cs)
         if ((r = upload_tables(c))) return r;

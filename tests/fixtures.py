@@ -36,7 +36,32 @@ def load(name):
                stpval=z["ref_stpval"], steps=int(z["steps"]),
                engsrp=z["ref_stpval"][STP_ENGSRP], engcpe=z["ref_stpval"][STP_ENGCPE],
                virsrp=z["ref_stpval"][STP_VIRSRP], vircpe=z["ref_stpval"][STP_VIRCPE])
+    # the reference's step-0 state at full precision (static cases; see
+    # tests/golden/ref_dump.F90): two_body_forces' own forces, its energies,
+    # stress and vdW long-range corrections, and its vdW tables
+    if "ref_frc64" in z.files:
+        d = z["ref_dump_stats"]
+        ref.update(frc64=z["ref_frc64"], dump_engcpe=d[0], dump_vircpe=d[1], dump_engsrp=d[2],
+                   dump_virsrp=d[3], dump_stress=d[4:13], elrc=d[13], vlrc=d[14])
+    if "ref_tab_pot" in z.files:
+        ref.update(tab_pot=z["ref_tab_pot"], tab_frc=z["ref_tab_frc"], tab_list=z["ref_tab_list"])
     return s, ref
+
+
+def reference_tables(s, ref):
+    """The reference's own vdW tables laid out per type-pair key, as
+    dlpx_set_vdw_tables takes them: (max_grid, pot, frc, ltp), with ltp the
+    vdW form of each key (-1 for a pair without a potential)."""
+    nkey = s.ntypes * (s.ntypes + 1) // 2
+    m = ref["tab_pot"].shape[1]
+    pot, frc, ltp = np.zeros((nkey, m)), np.zeros((nkey, m)), np.full(nkey, -1, np.int32)
+    for (ti, tj, kind, _) in s.vdw:
+        a, b = ti + 1, tj + 1
+        hi, lo = max(a, b), min(a, b)
+        key = hi * (hi - 1) // 2 + lo - 1
+        k = int(ref["tab_list"][key]) - 1
+        pot[key], frc[key], ltp[key] = ref["tab_pot"][k], ref["tab_frc"][k], kind
+    return m - 1, pot, frc, ltp
 
 
 def printed_digits_bound(ref, sig=10):

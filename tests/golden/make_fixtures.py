@@ -26,8 +26,20 @@ inputs and the reference's outputs:
   virial and stress differ slightly.  ``--heatflux`` adds these to existing
   fixtures after checking that the run reproduces their energies.
 
+* for static cases, from the reference linked with the dump hook of
+  ``tests/golden/ref_dump.F90`` (``oracle/_ref/DLPOLY_DUMP.Z``, built by
+  ``tests/golden/build_ref_dump.py``): ``ref_frc64``, the forces after the
+  reference's first ``two_body_forces`` call at full double precision;
+  ``ref_dump_stats``, its engcpe, vircpe, engsrp, virsrp, stress(1:9) and
+  the vdW long-range corrections elrc, vlrc;
+  and, where the reference tabulates vdW, its own tables ``ref_tab_pot`` /
+  ``ref_tab_frc`` (one row per potential, samples 0..max_grid) with
+  ``ref_tab_list`` (vdws%list: type-pair key -> potential).  ``--dump``
+  adds these to existing fixtures after checking that the run reproduces
+  their energies.
+
 Run in the build container (it needs the reference binary):
-    python tests/golden/make_fixtures.py [case ...] [--statis-only | --heatflux]
+    python tests/golden/make_fixtures.py [case ...] [--statis-only | --heatflux | --dump]
 """
 import os
 import subprocess
@@ -41,6 +53,7 @@ sys.path.insert(0, os.path.join(ROOT, "dl-poly_amd"))
 from dlpx import fileio, systems  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "DLPOLY_DUMP.Z")
 OUT = os.path.dirname(os.path.abspath(__file__))
 STATIS_CASES = ("nacl_1000_k32", "argon_4000_nve20", "nacl_1000_nve10", "mix_4000_nve10", "mol3_2592_nve10",
                 "nacl_1000_frz_nve10")
@@ -74,6 +87,9 @@ def cases():
         "nacl_1000_k32": (systems.nacl(ncells=5, kvec=(32, 32, 32)), 0),
         # odd 2.3.5 mesh (30 = 2.3.5, 36 = 2^2.3^2, 40 = 2^3.5) and spline order 6
         "nacl_1000_k30_36_40_s6": (systems.nacl(ncells=5, kvec=(30, 36, 40), nsplines=6, seed=7), 0),
+        # odd mesh edges on every axis (45 = 3^2.5, 27 = 3^3, 25 = 5^2): no
+        # Nyquist plane in the half spectrum of x, odd y and z transforms
+        "nacl_1000_k45_27_25_s6": (systems.nacl(ncells=5, kvec=(45, 27, 25), nsplines=6, seed=17), 0),
         # the 4,096-ion NaCl with a 48^3 mesh
         "nacl_4096_k48": (systems.nacl(ncells=8, kvec=(48, 48, 48), seed=11), 0),
         # orthorhombic box, non-neutral charges (net-charge correction, two_body.F90:689-696)
@@ -159,6 +175,69 @@ def run_case(name, sysm, steps, statis_only=False):
     print("%-28s N=%-6d steps=%-3d engsrp=%.10e engcpe=%.10e" % (name, sysm.natms, steps, rv["engsrp"], rv["engcpe"]))
 
 
+def read_dump(path):
+    """The DLPX_DUMP stream of tests/golden/ref_dump.F90."""
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:4], "<i4")[0])
+    rec = np.frombuffer(raw[4:4 + 28 * n], dtype=np.dtype([("ltg", "<i4"), ("f", "<f8", 3)]))
+    off = 4 + 28 * n
+    stats = np.frombuffer(raw[off:off + 15 * 8], "<f8").copy()
+    off += 15 * 8
+    n_vdw, max_grid, nkey, have_tab = (int(v) for v in np.frombuffer(raw[off:off + 16], "<i4"))
+    off += 16
+    frc = np.zeros((n, 3))
+    frc[rec["ltg"] - 1] = rec["f"]
+    out = dict(frc=frc, stats=stats)
+    if have_tab:
+        out["list"] = np.frombuffer(raw[off:off + 4 * nkey], "<i4").copy()
+        off += 4 * nkey
+        m = max_grid + 1
+        tab = np.frombuffer(raw[off:off + 16 * m * n_vdw], "<f8").reshape(n_vdw, 2, m)
+        out["tab_pot"], out["tab_frc"] = tab[:, 0, :].copy(), tab[:, 1, :].copy()
+        off += 16 * m * n_vdw
+    assert off == len(raw), (off, len(raw))
+    return out
+
+
+def dump_run(name, sysm):
+    """The static case through DLPOLY_DUMP.Z: full-precision step-0 forces,
+    energies, stress and vdW tables of the reference."""
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, sysm, steps=0)
+        r = subprocess.run([REF_DUMP], cwd=d, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0 or not os.path.exists(os.path.join(d, "DLPX_DUMP")):
+            raise RuntimeError("reference dump failed for %s:\n%s" % (name, r.stdout[-2000:]))
+        dump = read_dump(os.path.join(d, "DLPX_DUMP"))
+        rv = fileio.read_revive(os.path.join(d, "REVIVE"))
+    return dump, rv
+
+
+def add_dump(name, sysm):
+    """Store the reference's full-precision step-0 state in a static fixture
+    (after checking the run reproduces its REVIVE energies and REVCON forces)."""
+    path = os.path.join(OUT, name + ".npz")
+    z = dict(np.load(path))
+    dump, rv = dump_run(name, sysm)
+    for k in (fileio.STP_ENGCPE, fileio.STP_ENGSRP):
+        assert rv["stpval"][k] == z["ref_stpval"][k], name
+    bound = np.abs(z["ref_frc"]).max() * 1e-9
+    # REVCON holds frozen atoms' forces zeroed after the evaluation
+    # (configuration.F90:3240-3258); the dump is two_body_forces' own output
+    frc = dump["frc"].copy()
+    if sysm.frozen is not None:
+        frc[np.asarray(sysm.frozen, bool)] = 0.0
+    assert np.abs(frc - z["ref_frc"]).max() <= bound, name
+    z["ref_frc64"] = dump["frc"]
+    z["ref_dump_stats"] = dump["stats"]
+    for k, name_z in (("list", "ref_tab_list"), ("tab_pot", "ref_tab_pot"), ("tab_frc", "ref_tab_frc")):
+        z.pop(name_z, None)
+        z.pop("ref_tab_" + k, None)
+        if k in dump:
+            z[name_z] = dump[k]
+    np.savez_compressed(path, **z)
+    print("%-28s full-precision forces%s" % (name, ", vdW tables" if "tab_pot" in dump else ""))
+
+
 def add_heatflux(name, sysm):
     """Run a static case with ``heat_flux on`` and store HEATFLUX's step-0
     vector in its fixture (after checking the energies are the stored ones)."""
@@ -186,6 +265,7 @@ def main():
         sys.exit("build the reference first: python oracle/build_ref.py")
     statis_only = "--statis-only" in sys.argv[1:]
     heatflux = "--heatflux" in sys.argv[1:]
+    dump = "--dump" in sys.argv[1:]
     only = set(a for a in sys.argv[1:] if not a.startswith("--"))
     for name, (sysm, steps) in cases().items():
         if only and name not in only:
@@ -193,6 +273,10 @@ def main():
         if heatflux:
             if steps == 0:
                 add_heatflux(name, sysm)
+            continue
+        if dump:
+            if steps == 0:
+                add_dump(name, sysm)
             continue
         run_case(name, sysm, steps, statis_only=statis_only)
 

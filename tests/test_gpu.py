@@ -71,7 +71,34 @@ def test_fp64_matches_oracle_and_reference(name):
     bound = fixtures.printed_digits_bound(ref["frc"]) + 1e-12 * np.abs(ref["frc"]).max()
     assert (np.abs(fixtures.frozen_zeroed(s, r["frc"]) - ref["frc"]) / bound).max() <= 1.05
     check_energies(r, ref, 1e-11, coulomb_scales(o))
+    # the reference's own forces at full precision (its two_body_forces output)
+    assert rms_rel(r["frc"], ref["frc64"]) <= 1e-12
+    assert np.abs(r["frc"] - ref["frc64"]).max() <= 1e-11 * np.abs(ref["frc64"]).max()
     ctx.close()
+
+
+@pytest.mark.parametrize("name", [n for n in STATIC if "tab_pot" in fixtures.load(n)[1]])
+def test_reference_vdw_tables_through_set_vdw_tables(name):
+    """The host-table route: DL_POLY's own vdws%tab_potential / tab_force
+    (dumped from the reference run) and its vdws%elrc / vlrc given through
+    dlpx_set_vdw_tables and dlpx_set_lrc, as a Fortran host passes them.
+    FP64 forces against the reference's full-precision forces (1e-12 RMS),
+    energies against its two_body_forces output; mixed within 1e-5."""
+    s, ref = fixtures.load(name)
+    print("case", name, flush=True)
+    mg, pot, frc, ltp = fixtures.reference_tables(s, ref)
+    tables = dict(max_grid=mg, vdw_pot=pot, vdw_frc=frc, ltp=ltp)
+    scales = coulomb_scales(oracle.two_body(s)) if s.electrostatics else None
+    for precision, tf in ((lib.FP64, 1e-12), (lib.MIXED, 1e-5)):
+        ctx = lib.Context(s, precision=precision, tables=tables, lrc=(ref["elrc"], ref["vlrc"]))
+        if s.electrostatics:
+            ctx._chk(ctx.lib.dlpx_generate_erfc_tables(ctx.h))
+        ctx.link_cell_pairs(s.pos)
+        r = ctx.two_body_forces(s.pos)
+        ctx.close()
+        assert rms_rel(r["frc"], ref["frc64"]) <= tf
+        dump = {k: ref["dump_" + k] for k in ("engcpe", "vircpe", "engsrp", "virsrp")}
+        check_energies(r, dump, 1e-11 if precision == lib.FP64 else 1e-6, scales)
 
 
 @pytest.mark.parametrize("name", STATIC)

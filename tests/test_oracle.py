@@ -40,6 +40,50 @@ def test_oracle_matches_reference_fixture(name):
     assert np.abs(r["stress"] - ref["stress"]).max() <= 1e-12 * np.abs(ref["stress"]).max()
 
 
+@pytest.mark.parametrize("name", STATIC)
+def test_oracle_matches_reference_full_precision(name):
+    """The reference's own step-0 forces, dumped at full double precision
+    after its two_body_forces call (tests/golden/ref_dump.F90), not only
+    REVCON's 10 printed digits: RMS relative 1e-12, every component within
+    1e-11 of the largest; the dumped energies against the oracle's."""
+    s, ref = fixtures.load(name)
+    r = oracle.two_body(s)
+    f, f64 = r["frc"], ref["frc64"]
+    assert float(np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum())) <= 1e-12
+    assert np.abs(f - f64).max() <= 1e-11 * np.abs(f64).max()
+    b = r["buffer"]
+    comps = dict(engcpe=(r["eng_self"], r["eng_recip"], b[11], b[15]), vircpe=(b[10], b[12], b[16]))
+    for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
+        scale = max([abs(ref[k]), 1.0] + [abs(x) for x in comps.get(k, ())])
+        assert abs(r[k] - ref["dump_" + k]) <= 1e-12 * scale, k
+    assert np.abs(r["stress"] - ref["dump_stress"]).max() <= 1e-12 * np.abs(ref["dump_stress"]).max()
+
+
+@pytest.mark.parametrize("name", [n for n in STATIC if "tab_pot" in fixtures.load(n)[1]])
+def test_oracle_vdw_tables_match_reference_tables(name):
+    """Every sample of the oracle's generated vdW tables against the
+    reference's own vdws%tab_potential / tab_force (vdw.F90:1397-1576) for
+    the forms of the fixture: equal to a few ulps (sample 0 holds Huge in
+    both), and the long-range corrections against vdws%elrc / vlrc."""
+    s, ref = fixtures.load(name)
+    mg, pot, frc, from_key = oracle.vdw_tables(s)
+    rmg, rpot, rfrc, ltp = fixtures.reference_tables(s, ref)
+    assert mg == rmg
+    for key in from_key:
+        for mine, theirs in ((pot[key], rpot[key]), (frc[key], rfrc[key])):
+            # rounding of each sample (powers of up to r^-14), with a floor of 1e-13 of the table's
+            # scale for samples where terms cancel (a zero crossing, the
+            # shifted n-m form's correction)
+            floor = 1e-13 * np.abs(theirs[len(theirs) // 10:]).max()
+            d = np.abs(mine[1:] - theirs[1:])
+            assert (d <= 1e-14 * np.abs(theirs[1:]) + floor).all(), (key, from_key[key][0], d.max())
+    if not s.vdw_force_shift:
+        # 1e-11: the AMOEBA correction is a Simpson sum of over 10^4 terms
+        elrc, vlrc = oracle.vdw_lrc(s, from_key)
+        assert abs(elrc - ref["elrc"]) <= 1e-11 * max(abs(ref["elrc"]), 1e-300)
+        assert abs(vlrc - ref["vlrc"]) <= 1e-11 * max(abs(ref["vlrc"]), 1e-300)
+
+
 @pytest.mark.parametrize("name", TRAJ)
 def test_oracle_nve_matches_reference_trajectory(name):
     s, ref = fixtures.load(name)
