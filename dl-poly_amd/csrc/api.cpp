@@ -363,34 +363,6 @@ std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
             (uint64_t)c->frz,      (uint64_t)c->nfrozen, t};
 }
 
-// Capture forces (both streams), stage 2 of this step and stage 1 of the
-// next into one executable graph.  Only called after an eager step, so every
-// buffer is sized and every kernel attribute set before the capture.
-static int capture_step_graph(Ctx *c, double tstep)
-{
-    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
-    c->step_graph = nullptr;
-    DLPX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int r = evaluate(c, true);
-    if (!r) r = nve_second(c, tstep, true);
-    if (!r) r = nve_first(c, tstep);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (r) {
-        if (g) (void)hipGraphDestroy(g);
-        return r;
-    }
-    if (e != hipSuccess) return fail(c, DLPX_ERR_HIP + (int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) {
-        c->step_graph = nullptr;
-        return fail(c, DLPX_ERR_HIP + (int)ei, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-    }
-    c->graph_key = step_graph_key(c, tstep);
-    return 0;
-}
-
 // Start of a resident run: clear the counters of the fused stage-1 kernel
 // and do stage 1 of the first step.
 static int md_begin(Ctx *c, int nsteps, double tstep)
@@ -400,12 +372,12 @@ static int md_begin(Ctx *c, int nsteps, double tstep)
     return nsteps > 0 ? nve_first(c, tstep) : 0;
 }
 
-// Step s of a resident run (nve.F90:127-215 with refresh_mappings/vnl_check
-// and the force call, drivers.F90:2105-2199): wait for this context's stage 1,
-// rebuild the list if an atom moved by padding/2, then forces, stage 2 and
-// the next step's stage 1.  With graphs, steps 2..n-1 replay one captured
-// graph.  Returns with the step's work queued on c->stream.
-static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool energies, int &rebuilds)
+// Step s of a host-decided resident run (nve.F90:127-215 with
+// refresh_mappings/vnl_check and the force call, drivers.F90:2105-2199): wait
+// for this context's stage 1, rebuild the list if an atom moved by padding/2,
+// then forces, stage 2 and the next step's stage 1.  Returns with the step's
+// work queued on c->stream.
+static int md_step(Ctx *c, int s, int nsteps, double tstep, bool energies, int &rebuilds)
 {
     int r;
     int update = 1;
@@ -418,13 +390,6 @@ static int md_step(Ctx *c, int s, int nsteps, double tstep, bool graphs, bool en
         rebuilds++;
     }
     const bool last = s == nsteps - 1;
-    if (graphs && s > 0 && !last) {
-        // forces + stage 2 of this step + stage 1 of the next, as one graph
-        if (!c->step_graph || c->graph_key != step_graph_key(c, tstep))
-            if ((r = capture_step_graph(c, tstep))) return r;
-        DLPX_HIP(c, hipGraphLaunch(c->step_graph, c->stream));
-        return 0;
-    }
     if ((r = evaluate(c, true))) return r;
     if ((r = nve_second(c, tstep, true))) return r;
     if (c->timing && (r = collect_timing(c))) return r;
@@ -450,7 +415,7 @@ int md_fallback(Ctx *c, int nsteps, double tstep, bool energies, int &rebuilds)
     rebuilds = 0;
     if ((r = md_begin(c, nsteps, tstep))) return r;
     for (int s = 0; s < nsteps; s++)
-        if ((r = md_step(c, s, nsteps, tstep, false, energies, rebuilds))) return r;
+        if ((r = md_step(c, s, nsteps, tstep, energies, rebuilds))) return r;
     return 0;
 }
 
@@ -584,7 +549,6 @@ int dlpx_destroy(dlpx_ctx *ctx)
     if (c->h_red) (void)hipHostFree(c->h_red);
     for (int k = 0; k < 8; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
-    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->recip_stream) (void)hipStreamDestroy(c->recip_stream);
@@ -1102,7 +1066,7 @@ int dlpx_md_run(dlpx_ctx *ctx, int32_t nsteps, double tstep, double *energies, i
         // stage timing (or DLPX_GRAPHS=0): eager steps, host-decided rebuilds
         if ((r = md_begin(c, nsteps, tstep))) return r;
         for (int s = 0; s < nsteps; s++)
-            if ((r = md_step(c, s, nsteps, tstep, false, energies != nullptr, rebuilds))) return r;
+            if ((r = md_step(c, s, nsteps, tstep, energies != nullptr, rebuilds))) return r;
         DLPX_HIP(c, hipStreamSynchronize(c->stream));
     }
     if (energies && (r = collect_energies(c, nsteps, energies))) return r;
@@ -1153,7 +1117,7 @@ int dlpx_md_run_ensemble_energies(dlpx_ctx *const *ctxs, int32_t nctx, int32_t n
             if ((r = md_begin(cs[k], nsteps, tsteps[k]))) return r;
         for (int s = 0; s < nsteps; s++)
             for (int k = 0; k < nctx; k++)
-                if ((r = md_step(cs[k], s, nsteps, tsteps[k], false, energies != nullptr, rebuilds[k]))) return r;
+                if ((r = md_step(cs[k], s, nsteps, tsteps[k], energies != nullptr, rebuilds[k]))) return r;
         for (int k = 0; k < nctx; k++) DLPX_HIP(cs[k], hipStreamSynchronize(cs[k]->stream));
     }
     for (int k = 0; k < nctx; k++) {

@@ -66,25 +66,22 @@ struct Ensemble {
     Shape shape;
     bool energies = false;
     bool electro = false;
-    BatchLaunch vv1, vv2, excl, red;
+    BatchLaunch step, excl;
     ListBatch list;
     PairBatch pair;
     SpmeBatch spme;
-    int32_t **flag_ptrs = nullptr;  // device: each system's flags
     hipGraphExec_t graph = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 
     ~Ensemble() { release(); }
     void release()
     {
-        for (BatchLaunch *b : {&vv1, &vv2, &excl, &red, &spme.spread, &spme.gather, &spme.fft.r2c, &spme.fft.yf,
-                               &spme.fft.zc, &spme.fft.yb, &spme.fft.c2r})
+        for (BatchLaunch *b : {&step, &excl, &spme.spread, &spme.gather, &spme.fft.r2c, &spme.fft.yf, &spme.fft.zc,
+                               &spme.fft.yb, &spme.fft.c2r})
             batch_free(*b);
         for (auto &g : pair.groups) batch_free(g.bl);
         pair.groups.clear();
         list_batch_free(list);
-        if (flag_ptrs) (void)hipFree(flag_ptrs);
-        flag_ptrs = nullptr;
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         if (fork) (void)hipEventDestroy(fork);
@@ -135,14 +132,14 @@ static int ens_prepare(Ensemble &E, bool energies, const Shape &sh)
     // records the buffer's address (upper bounds: the z pass has at most
     // K1 K2 workgroups, the pair kernel at most natms * 16 / 512)
     for (Ctx *c : cs) {
-        const int ub = std::max({ceil_div((long)c->natms * 16, 512), vv2_blocks(c), excl_blocks(c),
+        const int ub = std::max({ceil_div((long)c->natms * 16, 512), step_blocks(c), excl_blocks(c),
                                  c->prm.electrostatics ? c->K[0] * c->K[1] : 1,
                                  c->prm.electrostatics ? ceil_div(c->natms, 16) : 1});
         if ((r = ensure_red_rows(c, ub))) return r;
     }
     std::vector<int> npair;
-    if ((r = list_prepare(cs, E.list)) || (r = vv1_prepare(cs, E.vv1)) || (r = vv2_prepare(cs, E.vv2)) ||
-        (r = excl_prepare(cs, E.excl)) || (r = pair_prepare(cs, E.pair, sh.pair_lanes, npair)))
+    if ((r = list_prepare(cs, E.list)) || (r = excl_prepare(cs, E.excl)) ||
+        (r = pair_prepare(cs, E.pair, sh.pair_lanes, npair)))
         return r;
     E.spme.fft.lpb_cap = sh.fft_lpb;
     if (E.electro && (r = spme_prepare(cs, E.spme))) return r;
@@ -151,7 +148,7 @@ static int ens_prepare(Ensemble &E, bool energies, const Shape &sh)
         Ctx *c = cs[k];
         for (int q = 0; q < kReduceSlots; q++) rows[k].rows[q] = 0;
         for (int q = R_ENGVDW; q <= R_S33; q++) rows[k].rows[q] = npair[k];
-        rows[k].rows[R_KIN] = vv2_blocks(c);
+        rows[k].rows[R_KIN] = step_blocks(c);
         const int nex = (c->excl_total > 0 && c->prm.electrostatics) ? excl_blocks(c) : 0;
         for (int q = R_ENGEX; q <= R_XS33; q++) rows[k].rows[q] = nex;
         if (c->prm.electrostatics) {
@@ -161,8 +158,7 @@ static int ens_prepare(Ensemble &E, bool energies, const Shape &sh)
         for (int q = 0; q < kReduceSlots; q++)
             if (rows[k].rows[q] > c->red_cap) return fail(c, DLPX_ERR_STATE, "partial-sum rows exceed the buffer");
     }
-    DLPX_HIP(c0, hipMalloc((void **)&E.flag_ptrs, sizeof(int32_t *) * cs.size()));
-    if ((r = reduce_prepare(cs, rows, energies, E.red, E.flag_ptrs))) return r;
+    if ((r = step_prepare(cs, rows, energies, E.step))) return r;
     DLPX_HIP(c0, hipEventCreateWithFlags(&E.fork, hipEventDisableTiming));
     DLPX_HIP(c0, hipEventCreateWithFlags(&E.join, hipEventDisableTiming));
     E.key = ens_key(cs, energies, sh);
@@ -170,7 +166,8 @@ static int ens_prepare(Ensemble &E, bool energies, const Shape &sh)
 }
 
 // Queue one step on the first context's streams: list builds (gated),
-// forces, reductions, stage 2, and (with vv1) stage 1 of the next step.
+// forces, then one kernel for stage 2, the step's reductions and (with vv1)
+// stage 1 of the next step.
 static int ens_enqueue(Ensemble &E, bool with_vv1)
 {
     Ctx *c0 = E.cs[0];
@@ -188,14 +185,7 @@ static int ens_enqueue(Ensemble &E, bool with_vv1)
         DLPX_HIP(c0, hipEventRecord(E.join, s2));
         DLPX_HIP(c0, hipStreamWaitEvent(s, E.join, 0));
     }
-    if ((r = reduce_launch(c0, s, E.red))) return r;
-    if ((r = vv2_launch(c0, s, E.vv2))) return r;
-    if (E.energies) {
-        if ((r = reduce_launch(c0, s, E.red))) return r;
-        if ((r = record_next_launch(c0, s, E.flag_ptrs, (int)E.cs.size()))) return r;
-    }
-    if (with_vv1 && (r = vv1_launch(c0, s, E.vv1))) return r;
-    return 0;
+    return step_launch(c0, s, E.step, true, with_vv1);
 }
 
 static int ens_capture(Ensemble &E)
@@ -324,6 +314,9 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         DLPX_HIP(c, hipMemsetAsync(c->flags + F_OVF_ANY, 0, sizeof(int32_t), c0->stream));
         DLPX_HIP(c, hipMemsetAsync(c->flags + F_NREB, 0, sizeof(int32_t) * 2, c0->stream));
         DLPX_HIP(c, hipMemsetAsync(c->flags + F_DMAX, 0, sizeof(int32_t) * 3, c0->stream));
+        // the totals that the gather and the step kernel accumulate atomically
+        DLPX_HIP(c, hipMemsetAsync(c->red_out + R_ERECIP, 0, sizeof(double) * (R_FNETZ - R_ERECIP + 1), c0->stream));
+        DLPX_HIP(c, hipMemsetAsync(c->red_out + R_KIN, 0, sizeof(double), c0->stream));
     }
     for (Ctx *c : cs)
         if (c != c0) DLPX_HIP(c, hipStreamSynchronize(c->stream));  // the snapshots
@@ -353,7 +346,7 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
     int s = 0;
     // stage 1 of the first step (every system decides its rebuild)
     if ((r = ready(sh))) return r;
-    if ((r = vv1_launch(c0, c0->stream, E->vv1))) return r;
+    if ((r = step_launch(c0, c0->stream, E->step, false, true))) return r;
     if (tune) {
         // each candidate: one eager step (kernel attributes), a capture, then
         // timed replays; the fastest median replay wins

@@ -84,6 +84,7 @@ enum Flag : int {
     F_NREB = 8,     // rebuilds decided on the device since the run began
     F_STEP = 9,     // per-step energy records written since the run began
     F_ERR = 10,     // sticky: a list-build kernel met an index outside the atoms (internal error)
+    F_WDONE = 12,   // workgroups of the list binning finished (zero between launches)
     F_WORDS = 16
 };
 
@@ -125,15 +126,13 @@ struct Ctx {
     int pair_lanes = kLanesPerAtom;  // lanes per atom in the pair kernel: 4, 8 or 16
     hipStream_t sp = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    // resident steps replay one captured graph: forces, stage 2 and the next
-    // stage 1; graph_key records what the capture baked in (buffers, sizes)
+    // resident runs replay one captured step graph (batch.cpp) unless
+    // DLPX_GRAPHS=0 or stage timing asks for eager steps
     bool use_graphs = true;
     bool finish_in_vv2 = false;      // the last evaluation left the reciprocal finish to stage 2
     bool vdw_force_shift = false;    // vdws%l_force_shift
     bool vdw_direct = false;         // vdws%l_direct: analytic forms per pair
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
-    hipGraphExec_t step_graph = nullptr;
-    std::vector<uint64_t> graph_key;
     int device = 0;
 
     // box (orthorhombic)
@@ -272,7 +271,8 @@ struct BatchLaunch {
 };
 // the batched kernels of one list build (neighbours.hip)
 struct ListBatch {
-    BatchLaunch wrap, scan, scat, sort, perm, copy, bsub, blink, excl;
+    BatchLaunch wrap, scat, sort, bsub, blink, excl;
+    int32_t *gate = nullptr;  // one system: its F_UPD word, else null (per-system gates only)
 };
 // the batched passes of one SPME convolution (fft.hip) and of the
 // reciprocal-space stage (spme.hip)
@@ -394,10 +394,6 @@ int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_i
 int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 // batched stages (one launch per stage over the systems of an ensemble)
-int vv1_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl);
-int vv1_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
-int vv2_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl);
-int vv2_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
 int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb);
 int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb);
 void list_batch_free(ListBatch &lb);
@@ -411,12 +407,14 @@ int fft_prepare(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips
 int fft_launch(Ctx *c, hipStream_t s, const FftBatch &fb);
 int spme_prepare(const std::vector<Ctx *> &cs, SpmeBatch &sb);
 int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb);
-int reduce_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
-                   BatchLaunch &bl, int32_t **flag_ptrs);
-int reduce_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl);
-int record_next_launch(Ctx *c, hipStream_t s, int32_t *const *flag_ptrs, int nsys);
 int vv2_blocks(const Ctx *c);
 int excl_blocks(const Ctx *c);
+// one kernel per step of a batched run: stage 2 of the step, its reductions
+// and per-step record, stage 1 of the next step (nve.hip)
+int step_blocks(const Ctx *c);
+int step_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
+                 BatchLaunch &bl);
+int step_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl, bool stage2, bool stage1);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

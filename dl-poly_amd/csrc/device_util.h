@@ -23,10 +23,11 @@ __device__ __forceinline__ double group_sum(double v)
     return v;
 }
 
-// Block-wide sum of NV values per thread; the totals land in out[0..NV) of
-// thread 0.  lds must hold NV * (blockDim.x / 64) doubles.
+// Block-wide sum of NV values per thread; total k lands in out[k], written
+// by thread k, which also returns it (other threads return 0).  lds must hold
+// NV * (blockDim.x / 64) doubles.
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds, double *out)
+__device__ __forceinline__ double block_sum(double (&v)[NV], double *lds, double *out)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
@@ -35,11 +36,47 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds, double *
         if (lane == 0) lds[k * nw + wid] = s;
     }
     __syncthreads();
+    double total = 0.0;
     if (threadIdx.x < NV) {
-        double s = 0.0;
-        for (int w = 0; w < nw; w++) s += lds[threadIdx.x * nw + w];
-        out[threadIdx.x] = s;
+        for (int w = 0; w < nw; w++) total += lds[threadIdx.x * nw + w];
+        out[threadIdx.x] = total;
     }
+    return total;
+}
+
+// Wait until this thread's memory operations, the fire-and-forget atomics
+// included, have completed, so that an atomic issued after it (a workgroup
+// counter) cannot overtake them.  Cross-workgroup results here travel only
+// through device-scope atomics, which need no cache writeback; an
+// agent-scope fence would write back the XCD's L2 in every workgroup.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+// Coherent read of a value other workgroups of the same launch built with
+// device-scope atomics.
+template <typename T>
+__device__ __forceinline__ T load_coherent(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of column col over rows 0..rows-1 of a row-major buffer of partial
+// sums (stride doubles per row), in a fixed order, by a whole workgroup of NT
+// threads (a power of two); lds holds NT doubles.  Every thread gets the
+// total.
+template <int NT>
+__device__ __forceinline__ double column_sum(const double *red, int rows, int stride, int col, double *lds)
+{
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += NT) s += red[(size_t)r * stride + col];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double t = lds[0];
+    __syncthreads();
+    return t;
 }
 
 // floor(x / d) and the remainder x - d floor(x / d) in [0, d), for |x| < 2^20,

@@ -65,74 +65,43 @@ __device__ __forceinline__ void run_of(int key, int &head, int &len)
 // graph hold the whole rebuild without a host round trip per step.
 __device__ __forceinline__ bool skip_build(const int32_t *upd) { return upd && *upd == 0; }
 
-// Fold positions into the primary cell (relocate_particles / pbcshift) and
-// count atoms per link cell.
-struct WrapArgs {
-    double4 *pos;
-    int n;
-    CellGrid g;
-    int32_t *cell_of, *count;
-    const int32_t *upd;
-};
-__device__ __forceinline__ void wrap_bin_body(const WrapArgs &a, int bid)
-{
-    const CellGrid &g = a.g;
-    const int i = bid * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    double4 p = a.pos[i];
-    p.x -= g.L[0] * floor(p.x * g.invL[0] + 0.5);
-    p.y -= g.L[1] * floor(p.y * g.invL[1] + 0.5);
-    p.z -= g.L[2] * floor(p.z * g.invL[2] + 0.5);
-    // guard the rounding edge x == L/2 after folding
-    if (p.x >= 0.5 * g.L[0]) p.x -= g.L[0];
-    if (p.y >= 0.5 * g.L[1]) p.y -= g.L[1];
-    if (p.z >= 0.5 * g.L[2]) p.z -= g.L[2];
-    a.pos[i] = p;
-    const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.sub * g.nc[0]);
-    const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.sub * g.nc[1]);
-    const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.sub * g.nc[2]);
-    const int c = g.sub == 2 ? subcell_key(g, cx, cy, cz) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
-    a.cell_of[i] = c;
-    // atoms are mostly in cell order already: one atomic per run of equal
-    // cells within the wavefront
-    int head, len;
-    run_of(c, head, len);
-    if ((int)(threadIdx.x & 63) == head) atomicAdd(&a.count[c], len);
-}
+// The sort before a list build runs as three kernels:
+//   k_wrap_scan     fold positions into the primary cell (relocate_particles /
+//                   pbcshift), bin them, and in the last workgroup to finish
+//                   turn the counts into start offsets;
+//   k_scatter_copy  place each atom's index in its key's run, and copy the
+//                   atom data to the spare arrays;
+//   k_sort_permute  order each key's run by the previous index and gather the
+//                   atoms from the spare arrays into the primary ones.
+// The primary arrays keep their addresses (captured step graphs hold them).
 
-// Exclusive scan of the per-key counts in one workgroup of 1024 threads, each
+// Exclusive scan of the per-key counts by one workgroup of 1024 threads, each
 // thread taking kScanRun consecutive keys per pass (subcell binning has
 // hundreds of thousands of keys at a million atoms).  The counts are zeroed
 // once read, ready for the next build.  With subcells (cell_start not null)
 // the link-cell starts are the starts of every 8th key.
 constexpr int kScanRun = 8;
-struct ScanArgs {
-    int32_t *count, *start, *fill;
-    int n;
-    int32_t *cell_start;  // may be null
-    const int32_t *upd;
-};
-__device__ __forceinline__ void scan_body(const ScanArgs &a)
+constexpr int kWrapThreads = 1024;
+__device__ __forceinline__ void scan_keys(int32_t *count, int32_t *start, int32_t *fill, int n, int32_t *cell_start)
 {
-    __shared__ int32_t part[1024];
+    __shared__ int32_t part[kWrapThreads];
     __shared__ int32_t carry;
-    const int n = a.n;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 1024 * kScanRun) {
+    for (int base = 0; base < n; base += kWrapThreads * kScanRun) {
         const int i0 = base + threadIdx.x * kScanRun;
         int v[kScanRun], sum = 0;
 #pragma unroll
         for (int k = 0; k < kScanRun; k++) {
-            v[k] = i0 + k < n ? a.count[i0 + k] : 0;
+            v[k] = i0 + k < n ? load_coherent(&count[i0 + k]) : 0;
             sum += v[k];
         }
 #pragma unroll
         for (int k = 0; k < kScanRun; k++)
-            if (i0 + k < n) a.count[i0 + k] = 0;
+            if (i0 + k < n) count[i0 + k] = 0;
         part[threadIdx.x] = sum;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
+        for (int o = 1; o < kWrapThreads; o <<= 1) {
             int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
             __syncthreads();
             part[threadIdx.x] += t;
@@ -142,20 +111,68 @@ __device__ __forceinline__ void scan_body(const ScanArgs &a)
 #pragma unroll
         for (int k = 0; k < kScanRun; k++) {
             if (i0 + k < n) {
-                a.start[i0 + k] = run;
-                a.fill[i0 + k] = run;
-                if (a.cell_start && ((i0 + k) & 7) == 0) a.cell_start[(i0 + k) >> 3] = run;
+                start[i0 + k] = run;
+                fill[i0 + k] = run;
+                if (cell_start && ((i0 + k) & 7) == 0) cell_start[(i0 + k) >> 3] = run;
             }
             run += v[k];
         }
         __syncthreads();
-        if (threadIdx.x == 1023) carry += part[1023];
+        if (threadIdx.x == kWrapThreads - 1) carry += part[kWrapThreads - 1];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        a.start[n] = carry;
-        if (a.cell_start) a.cell_start[n >> 3] = carry;
+        start[n] = carry;
+        if (cell_start) cell_start[n >> 3] = carry;
     }
+}
+
+struct WrapArgs {
+    double4 *pos;
+    int n;
+    CellGrid g;
+    int32_t *cell_of, *count;
+    int32_t *start, *fill;   // key starts (and the scatter's fill pointers)
+    int nkey;
+    int32_t *cell_start;     // subcells: link-cell starts; else null (start is them)
+    unsigned int *done;      // workgroups finished (zero between builds)
+    const int32_t *upd;
+};
+__device__ __forceinline__ void wrap_scan_body(const WrapArgs &a, int bid, int nb)
+{
+    __shared__ bool last;
+    const CellGrid &g = a.g;
+    const int i = bid * blockDim.x + threadIdx.x;
+    if (i < a.n) {
+        double4 p = a.pos[i];
+        p.x -= g.L[0] * floor(p.x * g.invL[0] + 0.5);
+        p.y -= g.L[1] * floor(p.y * g.invL[1] + 0.5);
+        p.z -= g.L[2] * floor(p.z * g.invL[2] + 0.5);
+        // floor(x/L + 0.5) can leave x == L/2 by rounding
+        if (p.x >= 0.5 * g.L[0]) p.x -= g.L[0];
+        if (p.y >= 0.5 * g.L[1]) p.y -= g.L[1];
+        if (p.z >= 0.5 * g.L[2]) p.z -= g.L[2];
+        a.pos[i] = p;
+        const int cx = cell_coord(p.x, g.L[0], g.invL[0], g.sub * g.nc[0]);
+        const int cy = cell_coord(p.y, g.L[1], g.invL[1], g.sub * g.nc[1]);
+        const int cz = cell_coord(p.z, g.L[2], g.invL[2], g.sub * g.nc[2]);
+        const int c = g.sub == 2 ? subcell_key(g, cx, cy, cz) : (cz * g.nc[1] + cy) * g.nc[0] + cx;
+        a.cell_of[i] = c;
+        // atoms are mostly in cell order already: one atomic per run of equal
+        // cells within the wavefront
+        int head, len;
+        run_of(c, head, len);
+        if ((int)(threadIdx.x & 63) == head) atomicAdd(&a.count[c], len);
+    }
+    // the last workgroup to finish scans the counts (device-scope atomics,
+    // read coherently)
+    wait_memory();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == (unsigned)nb - 1;
+    __syncthreads();
+    if (!last) return;
+    scan_keys(a.count, a.start, a.fill, a.nkey, a.cell_start);
+    if (threadIdx.x == 0) __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct ScatterArgs {
@@ -164,11 +181,24 @@ struct ScatterArgs {
     int n;
     const int32_t *upd;
     int32_t *flags;  // F_ERR on an inconsistent count
+    // the atom data, copied to the spare arrays for k_sort_permute to gather from
+    const double4 *pos;
+    double4 *pos_alt;
+    const int32_t *typ;
+    int32_t *typ_alt;
+    const int32_t *gid;
+    int32_t *gid_alt;
+    const double4 *vel;  // may be null
+    double4 *vel_alt;
 };
-__device__ __forceinline__ void scatter_body(const ScatterArgs &a, int bid)
+__device__ __forceinline__ void scatter_copy_body(const ScatterArgs &a, int bid)
 {
     const int i = bid * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    a.pos_alt[i] = a.pos[i];
+    a.typ_alt[i] = a.typ[i];
+    a.gid_alt[i] = a.gid[i];
+    if (a.vel) a.vel_alt[i] = a.vel[i];
     const int c = a.cell_of[i];
     int head, len;
     run_of(c, head, len);
@@ -181,24 +211,57 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs &a, int bid)
     else a.flags[F_ERR] = 1;
 }
 
-// Make the order inside each cell deterministic (ascending previous index):
-// one workgroup per cell, every member's rank is the number of smaller
-// members (indices are distinct), so the sort is one parallel pass in LDS.
+// Make the order inside each key deterministic (ascending previous index):
+// one workgroup per key, every member's rank is the number of smaller
+// members (indices are distinct), so the sort is one parallel pass in LDS;
+// then the workgroup gathers its atoms from the spare arrays.  A key with
+// more than kSortLds members (a very crowded cell) is sorted by one thread in
+// global memory.
 constexpr int kSortLds = 2048;
 struct SortArgs {
     int32_t *perm;
     const int32_t *start;
-    int nkey;
+    int nkey, n;
     const int32_t *upd;
+    int32_t *flags;  // F_ERR on an index outside the atoms
+    const double4 *pos_alt;
+    double4 *pos;
+    const int32_t *typ_alt;
+    int32_t *typ;
+    const int32_t *gid_alt;
+    int32_t *gid;
+    const double4 *vel_alt;  // may be null
+    double4 *vel;
+    int32_t *slot;
+    double4 *xref;
 };
-__device__ __forceinline__ void sort_cells_body(const SortArgs &a, int c)
+__device__ __forceinline__ void permute_one(const SortArgs &a, int i, int o)
 {
-    __shared__ int32_t v[kSortLds];
+    // perm and gid hold indices below n by construction; the test keeps a
+    // broken count from turning into stray memory accesses, and is reported
+    // as an internal error
+    if ((unsigned)o >= (unsigned)a.n) {
+        a.flags[F_ERR] = 1;
+        return;
+    }
+    const double4 p = a.pos_alt[o];
+    a.pos[i] = p;
+    a.xref[i] = p;
+    a.typ[i] = a.typ_alt[o];
+    const int g = a.gid_alt[o];
+    a.gid[i] = g;
+    if ((unsigned)g < (unsigned)a.n) a.slot[g] = i;
+    else a.flags[F_ERR] = 1;
+    if (a.vel) a.vel[i] = a.vel_alt[o];
+}
+__device__ __forceinline__ void sort_permute_body(const SortArgs &a, int c)
+{
+    __shared__ int32_t v[kSortLds], sorted[kSortLds];
     int32_t *perm = a.perm;
     const int s = a.start[c], n = a.start[c + 1] - s;
-    if (n <= 1) return;
-    if (n > kSortLds) {  // a very crowded cell: serial insertion sort
-        if (threadIdx.x == 0)
+    if (n <= 0) return;
+    if (n > kSortLds) {
+        if (threadIdx.x == 0) {
             for (int q = s + 1; q < s + n; q++) {
                 const int x = perm[q];
                 int b = q - 1;
@@ -208,6 +271,13 @@ __device__ __forceinline__ void sort_cells_body(const SortArgs &a, int c)
                 }
                 perm[b + 1] = x;
             }
+        }
+        // the sorted run was written by thread 0: make it visible to the
+        // workgroup past its L1
+        __threadfence();
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            permute_one(a, s + k, __hip_atomic_load(&perm[s + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         return;
     }
     for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] = perm[s + k];
@@ -216,71 +286,11 @@ __device__ __forceinline__ void sort_cells_body(const SortArgs &a, int c)
         const int x = v[k];
         int rank = 0;
         for (int j = 0; j < n; j++) rank += v[j] < x;
+        sorted[rank] = x;
         perm[s + rank] = x;
     }
-}
-
-struct PermArgs {
-    const int32_t *perm;
-    int n;
-    const double4 *pos;
-    double4 *pos_new;
-    const int32_t *typ;
-    int32_t *typ_new;
-    const int32_t *gid;
-    int32_t *gid_new;
-    const double4 *vel;  // may be null
-    double4 *vel_new;
-    int32_t *slot;
-    double4 *xref;
-    const int32_t *upd;
-    int32_t *flags;  // F_ERR on an index outside the atoms
-};
-__device__ __forceinline__ void permute_body(const PermArgs &a, int bid)
-{
-    const int i = bid * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    // perm and gid hold indices below n by construction; the test keeps a
-    // broken count (a kernel of the chain that did not run) from turning into
-    // stray memory accesses, and is reported as an internal error
-    const int o = a.perm[i];
-    if ((unsigned)o >= (unsigned)a.n) {
-        a.flags[F_ERR] = 1;
-        return;
-    }
-    const double4 p = a.pos[o];
-    a.pos_new[i] = p;
-    a.xref[i] = p;
-    a.typ_new[i] = a.typ[o];
-    const int g = a.gid[o];
-    a.gid_new[i] = g;
-    if ((unsigned)g < (unsigned)a.n) a.slot[g] = i;
-    else a.flags[F_ERR] = 1;
-    if (a.vel) a.vel_new[i] = a.vel[o];
-}
-
-// The permuted arrays back into the primary ones, so device pointers stay
-// the same across rebuilds (captured step graphs keep them as arguments).
-struct CopyArgs {
-    int n;
-    const double4 *pos_alt;
-    double4 *pos;
-    const int32_t *typ_alt;
-    int32_t *typ;
-    const int32_t *gid_alt;
-    int32_t *gid;
-    const double4 *vel_alt;  // may be null
-    double4 *vel;
-    const int32_t *upd;
-};
-__device__ __forceinline__ void copy_back_body(const CopyArgs &a, int bid)
-{
-    const int i = bid * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    a.pos[i] = a.pos_alt[i];
-    a.typ[i] = a.typ_alt[i];
-    a.gid[i] = a.gid_alt[i];
-    if (a.vel) a.vel[i] = a.vel_alt[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) permute_one(a, s + k, sorted[k]);
 }
 
 // One workgroup per link cell.  The atoms of the (up to) 27 stencil cells
@@ -770,7 +780,10 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
 
 // Single-system kernels (arguments by value) and batched ones (argument
 // structs from device memory, one per system, see batch_find) around the
-// bodies above; every one first honours the rebuild gate.
+// bodies above; every one first honours the rebuild gate.  A batched launch
+// may also take one gate for the whole batch as a kernel argument (a batch
+// of one system: its F_UPD), which a skipped launch reads without first
+// fetching its argument struct.
 #define DLPX_LIST_KERNELS(NAME, ARGS, NT, CALL_SINGLE, CALL_BATCH, XCD)                                      \
     __global__ void __launch_bounds__(NT) NAME(ARGS a)                                                    \
     {                                                                                                     \
@@ -778,8 +791,10 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
         CALL_SINGLE;                                                                                      \
     }                                                                                                     \
     __global__ void __launch_bounds__(NT) NAME##_batch(const ARGS *__restrict__ av,                       \
-                                                       const int32_t *__restrict__ boff, int nsys)        \
+                                                       const int32_t *__restrict__ boff, int nsys,        \
+                                                       const int32_t *__restrict__ gate)                  \
     {                                                                                                     \
+        if (skip_build(gate)) return;                                                                     \
         BatchPos bp;                                                                                      \
         if (XCD) {                                                                                        \
             if (!batch_pos(boff, nsys, bp)) return;                                                       \
@@ -787,17 +802,16 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
             if ((int)blockIdx.x >= boff[nsys]) return;                                                    \
             bp.s = batch_find(boff, nsys, blockIdx.x);                                                    \
             bp.b = blockIdx.x - boff[bp.s];                                                               \
+            bp.nb = boff[bp.s + 1] - boff[bp.s];                                                          \
         }                                                                                                 \
         const ARGS &a = av[bp.s];                                                                          \
         if (skip_build(a.upd)) return;                                                                    \
         CALL_BATCH;                                                                                       \
     }
-DLPX_LIST_KERNELS(k_wrap_bin, WrapArgs, 256, wrap_bin_body(a, blockIdx.x), wrap_bin_body(a, bp.b), false)
-DLPX_LIST_KERNELS(k_scan, ScanArgs, 1024, scan_body(a), scan_body(a), false)
-DLPX_LIST_KERNELS(k_scatter, ScatterArgs, 256, scatter_body(a, blockIdx.x), scatter_body(a, bp.b), false)
-DLPX_LIST_KERNELS(k_sort_cells, SortArgs, 256, sort_cells_body(a, blockIdx.x), sort_cells_body(a, bp.b), false)
-DLPX_LIST_KERNELS(k_permute, PermArgs, 256, permute_body(a, blockIdx.x), permute_body(a, bp.b), false)
-DLPX_LIST_KERNELS(k_copy_back, CopyArgs, 256, copy_back_body(a, blockIdx.x), copy_back_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_wrap_scan, WrapArgs, kWrapThreads, wrap_scan_body(a, blockIdx.x, gridDim.x),
+                  wrap_scan_body(a, bp.b, bp.nb), false)
+DLPX_LIST_KERNELS(k_scatter_copy, ScatterArgs, 256, scatter_copy_body(a, blockIdx.x), scatter_copy_body(a, bp.b), false)
+DLPX_LIST_KERNELS(k_sort_permute, SortArgs, 256, sort_permute_body(a, blockIdx.x), sort_permute_body(a, bp.b), false)
 DLPX_LIST_KERNELS(k_list_exclude, ExclListArgs, 256, list_exclude_body(a, blockIdx.x), list_exclude_body(a, bp.b),
                   false)
 DLPX_LIST_KERNELS(k_build_list, BuildArgs, kListThreads,
@@ -834,11 +848,8 @@ static int grow(Ctx *c, T **p, size_t count)
 // stream capture before an eager build has sized them).
 struct ListChain {
     WrapArgs wrap;
-    ScanArgs scan;
     ScatterArgs scat;
     SortArgs sort;
-    PermArgs perm;
-    CopyArgs copy;
     BuildArgs build;
     ExclListArgs excl;
     bool sub, excl_on;
@@ -888,16 +899,15 @@ static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
     lc.ncell = ncell;
     lc.nkey = nkey;
     lc.sub = sub;
-    lc.wrap = WrapArgs{c->pos, n, g, c->cell_of, c->cell_count, upd};
-    lc.scan = ScanArgs{c->cell_count, kstart, c->cell_fill, nkey, sub ? c->cell_start : nullptr, upd};
-    lc.scat = ScatterArgs{c->cell_of, c->cell_fill, c->perm, n, upd, c->flags};
-    lc.sort = SortArgs{c->perm, kstart, nkey, upd};
     const bool res = c->resident;
-    lc.perm = PermArgs{c->perm, n,      c->pos,   c->pos_alt, c->typ,  c->typ_alt,
-                       c->gid,  c->gid_alt, res ? c->vel : nullptr, res ? c->vel_alt : nullptr,
-                       c->slot, c->xref, upd, c->flags};
-    lc.copy = CopyArgs{n, c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt, c->gid,
-                       res ? c->vel_alt : nullptr, res ? c->vel : nullptr, upd};
+    lc.wrap = WrapArgs{c->pos,    n,     g, c->cell_of, c->cell_count, kstart, c->cell_fill, nkey,
+                       sub ? c->cell_start : nullptr, (unsigned int *)(c->flags + F_WDONE), upd};
+    lc.scat = ScatterArgs{c->cell_of, c->cell_fill, c->perm,    n,          upd,
+                          c->flags,   c->pos,       c->pos_alt, c->typ,     c->typ_alt,
+                          c->gid,     c->gid_alt,   res ? c->vel : nullptr, res ? c->vel_alt : nullptr};
+    lc.sort = SortArgs{c->perm,    kstart, nkey,   n,      upd,     c->flags,
+                       c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt, c->gid,
+                       res ? c->vel_alt : nullptr, res ? c->vel : nullptr, c->slot, c->xref};
     lc.build = BuildArgs{c->pos, g, sub ? c->key_start : c->cell_start, rc_ext * rc_ext, c->cap_lane,
                          c->nbr, c->cnt, c->flags, upd};
     lc.excl_on = c->excl_total > 0 || c->nfrozen > 1;
@@ -911,13 +921,9 @@ static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
 // list kernel itself).
 static void launch_sort(const ListChain &lc, hipStream_t s)
 {
-    const int b = 256, nb = ceil_div(lc.n, b);
-    hipLaunchKernelGGL(k_wrap_bin, dim3(nb), dim3(b), 0, s, lc.wrap);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, lc.scan);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(b), 0, s, lc.scat);
-    hipLaunchKernelGGL(k_sort_cells, dim3(lc.nkey), dim3(lc.sub ? 64 : 256), 0, s, lc.sort);
-    hipLaunchKernelGGL(k_permute, dim3(nb), dim3(b), 0, s, lc.perm);
-    hipLaunchKernelGGL(k_copy_back, dim3(nb), dim3(b), 0, s, lc.copy);
+    hipLaunchKernelGGL(k_wrap_scan, dim3(ceil_div(lc.n, kWrapThreads)), dim3(kWrapThreads), 0, s, lc.wrap);
+    hipLaunchKernelGGL(k_scatter_copy, dim3(ceil_div(lc.n, 256)), dim3(256), 0, s, lc.scat);
+    hipLaunchKernelGGL(k_sort_permute, dim3(lc.nkey), dim3(lc.sub ? 64 : 256), 0, s, lc.sort);
 }
 
 static void launch_build(const ListChain &lc, hipStream_t s)
@@ -995,28 +1001,22 @@ int build_list(Ctx *c)
 int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb)
 {
     std::vector<WrapArgs> wrap;
-    std::vector<ScanArgs> scan;
     std::vector<ScatterArgs> scat;
     std::vector<SortArgs> sort;
-    std::vector<PermArgs> perm;
-    std::vector<CopyArgs> copy;
     std::vector<BuildArgs> bsub, blink;
     std::vector<ExclListArgs> excl;
-    std::vector<int> nb_atoms, nb_one, nb_keys, nb_sub, nb_link, nb_excl;
+    std::vector<int> nb_wrap, nb_atoms, nb_keys, nb_sub, nb_link, nb_excl;
     int r;
     for (Ctx *c : cs) {
         ListChain lc;
         if ((r = list_chain(c, c->flags + F_UPD, lc))) return r;
         if (c->cap_lane == 0 || !c->nbr) return fail(c, DLPX_ERR_STATE, "no list to rebuild on the device");
         wrap.push_back(lc.wrap);
-        scan.push_back(lc.scan);
         scat.push_back(lc.scat);
         sort.push_back(lc.sort);
-        perm.push_back(lc.perm);
-        copy.push_back(lc.copy);
         excl.push_back(lc.excl);
+        nb_wrap.push_back(ceil_div(lc.n, kWrapThreads));
         nb_atoms.push_back(ceil_div(lc.n, 256));
-        nb_one.push_back(1);
         nb_keys.push_back(lc.nkey);
         nb_sub.push_back(lc.sub ? lc.ncell : 0);
         nb_link.push_back(lc.sub ? 0 : lc.ncell);
@@ -1025,16 +1025,16 @@ int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb)
         blink.push_back(lc.build);
     }
     Ctx *c0 = cs[0];
-    if ((r = batch_upload(c0, lb.wrap, wrap.data(), sizeof(WrapArgs), nb_atoms, false)) ||
-        (r = batch_upload(c0, lb.scan, scan.data(), sizeof(ScanArgs), nb_one, false)) ||
+    if ((r = batch_upload(c0, lb.wrap, wrap.data(), sizeof(WrapArgs), nb_wrap, false)) ||
         (r = batch_upload(c0, lb.scat, scat.data(), sizeof(ScatterArgs), nb_atoms, false)) ||
         (r = batch_upload(c0, lb.sort, sort.data(), sizeof(SortArgs), nb_keys, false)) ||
-        (r = batch_upload(c0, lb.perm, perm.data(), sizeof(PermArgs), nb_atoms, false)) ||
-        (r = batch_upload(c0, lb.copy, copy.data(), sizeof(CopyArgs), nb_atoms, false)) ||
         (r = batch_upload(c0, lb.bsub, bsub.data(), sizeof(BuildArgs), nb_sub, true)) ||
         (r = batch_upload(c0, lb.blink, blink.data(), sizeof(BuildArgs), nb_link, true)) ||
         (r = batch_upload(c0, lb.excl, excl.data(), sizeof(ExclListArgs), nb_excl, false)))
         return r;
+    // a batch of one system gates every kernel on its own decision word, read
+    // first thing by each workgroup
+    lb.gate = cs.size() == 1 ? c0->flags + F_UPD : nullptr;
     return 0;
 }
 
@@ -1042,13 +1042,11 @@ int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb)
 {
 #define DLPX_LB(K, BL, ARGS, NT)                                                                          \
     if ((BL).total > 0)                                                                                  \
-        hipLaunchKernelGGL(K, dim3((BL).total), dim3(NT), 0, s, (const ARGS *)(BL).args, (BL).boff, (BL).nsys)
-    DLPX_LB(k_wrap_bin_batch, lb.wrap, WrapArgs, 256);
-    DLPX_LB(k_scan_batch, lb.scan, ScanArgs, 1024);
-    DLPX_LB(k_scatter_batch, lb.scat, ScatterArgs, 256);
-    DLPX_LB(k_sort_cells_batch, lb.sort, SortArgs, 256);
-    DLPX_LB(k_permute_batch, lb.perm, PermArgs, 256);
-    DLPX_LB(k_copy_back_batch, lb.copy, CopyArgs, 256);
+        hipLaunchKernelGGL(K, dim3((BL).total), dim3(NT), 0, s, (const ARGS *)(BL).args, (BL).boff, (BL).nsys, \
+                           (const int32_t *)lb.gate)
+    DLPX_LB(k_wrap_scan_batch, lb.wrap, WrapArgs, kWrapThreads);
+    DLPX_LB(k_scatter_copy_batch, lb.scat, ScatterArgs, 256);
+    DLPX_LB(k_sort_permute_batch, lb.sort, SortArgs, 256);
     DLPX_LB(k_build_list_sub_batch, lb.bsub, BuildArgs, kListThreads);
     DLPX_LB(k_build_list_batch, lb.blink, BuildArgs, kListThreads);
     DLPX_LB(k_list_exclude_batch, lb.excl, ExclListArgs, 256);
@@ -1059,8 +1057,8 @@ int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb)
 
 void list_batch_free(ListBatch &lb)
 {
-    for (BatchLaunch *b : {&lb.wrap, &lb.scan, &lb.scat, &lb.sort, &lb.perm, &lb.copy, &lb.bsub, &lb.blink, &lb.excl})
-        batch_free(*b);
+    for (BatchLaunch *b : {&lb.wrap, &lb.scat, &lb.sort, &lb.bsub, &lb.blink, &lb.excl}) batch_free(*b);
+    lb.gate = nullptr;
 }
 
 int vnl_check_dev(Ctx *c, int *update)

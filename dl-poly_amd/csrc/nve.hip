@@ -12,12 +12,10 @@ namespace dlpx {
 // Stage 1: v += dt/2 F/m ; x += dt v  (atoms of mass <= 1e-6 stay frozen),
 // fused with the displacement test of vnl_check (neighbours.F90:150-178):
 // the largest minimum-image |x - x_ref|^2 goes to *dmax (as the bit pattern
-// of a non-negative double).  The last workgroup to finish resets *dmax and
-// *done for the next step and publishes the result: to host memory
-// (host_dmax, the host then decides), and/or as the device's own rebuild
-// decision (upd: 1 when max |dx| >= Nearest(0.5,-1) padding, neighbours.F90:
-// 157-182, or always with no padding), which the list-build kernels read;
-// a decided rebuild also clears the build's counters and counts itself.
+// of a non-negative double).  The last workgroup to finish copies it to host
+// memory (host_dmax: the host decides the rebuild) and resets *dmax and
+// *done for the next step.  Host-decided runs only; a batched run decides on
+// the device (k_step_batch).
 struct Box {
     double L[3], invL[3];
 };
@@ -29,9 +27,7 @@ struct Vv1Args {
     Box b;
     unsigned long long *dmax;
     unsigned int *done;
-    unsigned long long *host_dmax;  // may be null
-    int32_t *flags;                 // F_UPD and the build counters; null: no device decision
-    double half_pad;                // Nearest(0.5,-1) * padding (0: rebuild every step)
+    unsigned long long *host_dmax;
 };
 // 1024-thread workgroups: the maximum is reduced in LDS first, so only one
 // atomic per workgroup (63 at 64k atoms) meets the single device word.
@@ -77,35 +73,13 @@ __device__ __forceinline__ void vv1_body(const Vv1Args &a, int bid, int nb)
     __syncthreads();
     if (last && threadIdx.x == 0) {
         __threadfence();
-        const unsigned long long bits = atomicAdd(a.dmax, 0ull);
-        if (a.host_dmax) {
-            *a.host_dmax = bits;
-            __threadfence_system();
-        }
-        if (a.flags) {
-            const int upd = (a.half_pad <= 0.0 || sqrt(__longlong_as_double((long long)bits)) >= a.half_pad) ? 1 : 0;
-            a.flags[F_UPD] = upd;
-            if (upd) {
-                a.flags[F_OVF] = 0;
-                a.flags[F_IBIG] = 0;
-                a.flags[F_PAIRS] = 0;
-                a.flags[F_NREB] += 1;
-            }
-        }
+        *a.host_dmax = atomicAdd(a.dmax, 0ull);
+        __threadfence_system();
         *a.dmax = 0ull;
         *a.done = 0u;
     }
 }
 __global__ void __launch_bounds__(kVv1Threads) k_vv1(Vv1Args a) { vv1_body(a, blockIdx.x, gridDim.x); }
-__global__ void __launch_bounds__(kVv1Threads) k_vv1_batch(const Vv1Args *__restrict__ av,
-                                                           const int32_t *__restrict__ boff, int nsys)
-{
-    const int lb = blockIdx.x;  // stage 1 reads atoms in order: no XCD renumbering
-    if (lb >= boff[nsys]) return;
-    const int s = batch_find(boff, nsys, lb);
-    vv1_body(av[s], lb - boff[s], boff[s + 1] - boff[s]);
-}
-
 // Stage 2: v += dt/2 F/m, and sum m v^2 per block.  With fin the
 // reciprocal forces are first added to the total force as k_finish does:
 // frc += 2 scale (frc_recip - mean frc_recip) (ewald_general.F90:857-864).
@@ -159,15 +133,6 @@ __device__ __forceinline__ void vv2_body(const Vv2Args &a, int bid)
     block_sum<1>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_KIN);
 }
 __global__ void __launch_bounds__(256) k_vv2(Vv2Args a) { vv2_body(a, blockIdx.x); }
-__global__ void __launch_bounds__(256) k_vv2_batch(const Vv2Args *__restrict__ av, const int32_t *__restrict__ boff,
-                                                   int nsys)
-{
-    const int lb = blockIdx.x;
-    if (lb >= boff[nsys]) return;
-    const int s = batch_find(boff, nsys, lb);
-    vv2_body(av[s], lb - boff[s]);
-}
-
 // freeze_atoms (configuration.F90:3240-3258): zero force and velocity of
 // frozen atoms (frz[gid[i]] /= 0)
 __global__ void k_freeze(double4 *vel, double4 *frc, int n, const int32_t *__restrict__ gid,
@@ -226,7 +191,7 @@ __global__ void k_store(double *__restrict__ st, int n, const int32_t *__restric
     r[6] = f.x; r[7] = f.y; r[8] = f.z;
 }
 
-Vv1Args vv1_args(Ctx *c, double dt, bool device_decides)
+static Vv1Args vv1_args(Ctx *c, double dt)
 {
     Vv1Args a;
     a.pos = c->pos;
@@ -243,9 +208,7 @@ Vv1Args vv1_args(Ctx *c, double dt, bool device_decides)
     // between launches
     a.dmax = (unsigned long long *)(c->flags + F_DMAX);
     a.done = (unsigned int *)(c->flags + F_DONE);
-    a.host_dmax = device_decides ? nullptr : (unsigned long long *)(c->h_flags_dev + 8);
-    a.flags = device_decides ? c->flags : nullptr;
-    a.half_pad = kHalfMinus * c->prm.padding;
+    a.host_dmax = (unsigned long long *)(c->h_flags_dev + 8);
     return a;
 }
 int vv1_blocks(const Ctx *c) { return ceil_div(c->natms, kVv1Threads); }
@@ -253,7 +216,7 @@ int vv1_blocks(const Ctx *c) { return ceil_div(c->natms, kVv1Threads); }
 int nve_first(Ctx *c, double dt)
 {
     // the result lands in host word h_flags[8..9]
-    hipLaunchKernelGGL(k_vv1, dim3(vv1_blocks(c)), dim3(kVv1Threads), 0, c->stream, vv1_args(c, dt, false));
+    hipLaunchKernelGGL(k_vv1, dim3(vv1_blocks(c)), dim3(kVv1Threads), 0, c->stream, vv1_args(c, dt));
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -265,7 +228,7 @@ double nve_first_displacement(Ctx *c)
     return std::sqrt(r2);
 }
 
-Vv2Args vv2_args(Ctx *c, double dt, bool fin)
+static Vv2Args vv2_args(Ctx *c, double dt, bool fin)
 {
     Vv2Args a;
     a.vel = c->vel;
@@ -296,38 +259,204 @@ int nve_second(Ctx *c, double dt, bool /*kinetic*/)
     return 0;
 }
 
-// Batched stages 1 and 2 over the systems of an ensemble (each with its own
-// time step, Ctx::run_dt); stage 1 leaves the rebuild decision on the device.
-int vv1_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl)
+// One launch per step of a resident batched run (every system of the batch):
+// S2, stage 2 of the step just evaluated: the reciprocal finish (frc +=
+// 2 scale (frc_recip - mean), the mean from the gather's atomic totals in
+// red_out), frozen atoms, the second half-kick and m v^2; the step's partial
+// sums reduced into red_out (workgroup b takes columns b, b + nb, ...; the
+// kinetic energy is added atomically, the reciprocal energy and net force
+// came from the gather), and the step's record when records are kept; and
+// S1, stage 1 of the next step: half-kick, drift, and the displacement test,
+// whose largest value the last workgroup to finish turns into the rebuild
+// decision (as k_vv1).  Workgroups exchange data only through device-scope
+// atomics; the last one also clears the atomic totals for the next step.
+struct StepArgs {
+    double4 *pos, *vel, *frc;
+    const double4 *frc_recip;  // null: no reciprocal part
+    const double4 *xref;
+    const int32_t *gid, *frz;  // frz may be null
+    int n;
+    double dt, two_scale, inv_n, half_pad;
+    Box b;
+    double *red, *red_out;
+    int rows[kReduceSlots];    // partial rows per column of the step
+    double *energies;          // per-step records (kReduceSlots each) or null
+    int ecap;
+    int32_t *flags;
+};
+__device__ __forceinline__ bool atomic_column(int col) { return col == R_KIN || (col >= R_ERECIP && col <= R_FNETZ); }
+template <bool S2, bool S1>
+__device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
 {
-    std::vector<Vv1Args> a;
-    std::vector<int> nb;
-    for (Ctx *c : cs) {
-        a.push_back(vv1_args(c, c->run_dt, true));
-        nb.push_back(vv1_blocks(c));
+    __shared__ double lsum[kVv1Threads];
+    __shared__ double wmax[kVv1Threads / 64];
+    __shared__ double lred[kVv1Threads / 64];
+    __shared__ bool last;
+    const int t = threadIdx.x;
+    // the step's record index (the last workgroup advances it after every
+    // workgroup has read it)
+    const int rec = (S2 && a.energies) ? a.flags[F_STEP] : a.ecap;
+    double *record = rec < a.ecap ? a.energies + (size_t)rec * kReduceSlots : nullptr;
+    if (S2) {
+        // the columns the step's kernels filled with partial rows
+        for (int col = bid; col < kReduceSlots; col += nb) {
+            if (atomic_column(col)) continue;
+            const double v = column_sum<kVv1Threads>(a.red, a.rows[col], kReduceSlots, col, lsum);
+            if (t == 0) {
+                a.red_out[col] = v;
+                if (record) record[col] = v;
+            }
+        }
     }
-    return batch_upload(cs[0], bl, a.data(), sizeof(Vv1Args), nb, false);
-}
-int vv1_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
-{
-    hipLaunchKernelGGL(k_vv1_batch, dim3(bl.total), dim3(kVv1Threads), 0, s, (const Vv1Args *)bl.args, bl.boff,
-                       bl.nsys);
-    DLPX_HIP(c, hipGetLastError());
-    return 0;
-}
-int vv2_prepare(const std::vector<Ctx *> &cs, BatchLaunch &bl)
-{
-    std::vector<Vv2Args> a;
-    std::vector<int> nb;
-    for (Ctx *c : cs) {
-        a.push_back(vv2_args(c, c->run_dt, c->prm.electrostatics != 0));
-        nb.push_back(vv2_blocks(c));
+    const int i = bid * blockDim.x + t;
+    double ke = 0.0, r2 = 0.0;
+    if (i < a.n) {
+        double4 v = a.vel[i];
+        double4 f = a.frc[i];
+        if (S2) {
+            if (a.frc_recip) {
+                const double mx = a.red_out[R_FNETX] * a.inv_n, my = a.red_out[R_FNETY] * a.inv_n,
+                             mz = a.red_out[R_FNETZ] * a.inv_n;
+                const double4 r = a.frc_recip[i];
+                f.x += (r.x - mx) * a.two_scale;
+                f.y += (r.y - my) * a.two_scale;
+                f.z += (r.z - mz) * a.two_scale;
+            }
+            if (a.frz && a.frz[a.gid[i]] != 0) {
+                f.x = f.y = f.z = 0.0;
+                v.x = v.y = v.z = 0.0;
+            }
+            a.frc[i] = f;
+            if (v.w > 1.0e-6) {
+                const double tmp = (0.5 * a.dt) / v.w;
+                v.x += tmp * f.x;
+                v.y += tmp * f.y;
+                v.z += tmp * f.z;
+                ke = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+            }
+        }
+        if (S1) {
+            double4 p = a.pos[i];
+            if (v.w > 1.0e-6) {
+                const double tmp = (0.5 * a.dt) / v.w;
+                v.x += tmp * f.x;
+                v.y += tmp * f.y;
+                v.z += tmp * f.z;
+                p.x += a.dt * v.x;
+                p.y += a.dt * v.y;
+                p.z += a.dt * v.z;
+                a.pos[i] = p;
+            }
+            const double4 q = a.xref[i];
+            const double dx = min_image_1(p.x, q.x, a.b.L[0], a.b.invL[0]);
+            const double dy = min_image_1(p.y, q.y, a.b.L[1], a.b.invL[1]);
+            const double dz = min_image_1(p.z, q.z, a.b.L[2], a.b.invL[2]);
+            r2 = dx * dx + dy * dy + dz * dz;
+        }
+        a.vel[i] = v;
     }
-    return batch_upload(cs[0], bl, a.data(), sizeof(Vv2Args), nb, false);
+    if (S2) {
+        double acc[1] = {ke};
+        const double tot = block_sum<1>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_KIN);
+        if (t == 0) atomicAdd(&a.red_out[R_KIN], tot);
+    }
+    if (S1) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+        if ((t & 63) == 0) wmax[t >> 6] = r2;
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (S1) {
+            double m = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, wmax[w]);
+            atomicMax((unsigned long long *)(a.flags + F_DMAX), (unsigned long long)__double_as_longlong(m));
+        }
+        wait_memory();
+        last = atomicAdd((unsigned int *)(a.flags + F_DONE), 1u) == (unsigned)nb - 1;
+    }
+    __syncthreads();
+    if (!last || t != 0) return;
+    if (S2) {
+        // the atomic totals into the record, then cleared for the next step
+        for (int col = 0; col < kReduceSlots; col++)
+            if (atomic_column(col)) {
+                const double v = load_coherent(&a.red_out[col]);
+                if (record) record[col] = v;
+                __hip_atomic_store(&a.red_out[col], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        if (a.energies) a.flags[F_STEP] = rec + 1;
+    }
+    if (S1) {
+        const unsigned long long bits = load_coherent((unsigned long long *)(a.flags + F_DMAX));
+        const int upd = (a.half_pad <= 0.0 || sqrt(__longlong_as_double((long long)bits)) >= a.half_pad) ? 1 : 0;
+        a.flags[F_UPD] = upd;
+        if (upd) {
+            a.flags[F_OVF] = 0;
+            a.flags[F_IBIG] = 0;
+            a.flags[F_PAIRS] = 0;
+            a.flags[F_NREB] += 1;
+        }
+        __hip_atomic_store((unsigned long long *)(a.flags + F_DMAX), 0ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __hip_atomic_store((unsigned int *)(a.flags + F_DONE), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-int vv2_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
+template <bool S2, bool S1>
+__global__ void __launch_bounds__(kVv1Threads) k_step_batch(const StepArgs *__restrict__ av,
+                                                            const int32_t *__restrict__ boff, int nsys)
 {
-    hipLaunchKernelGGL(k_vv2_batch, dim3(bl.total), dim3(256), 0, s, (const Vv2Args *)bl.args, bl.boff, bl.nsys);
+    const int lb = blockIdx.x;  // atoms in order: no XCD renumbering
+    if (lb >= boff[nsys]) return;
+    const int s = batch_find(boff, nsys, lb);
+    step_body<S2, S1>(av[s], lb - boff[s], boff[s + 1] - boff[s]);
+}
+
+int step_blocks(const Ctx *c) { return ceil_div(c->natms, kVv1Threads); }
+
+int step_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
+                 BatchLaunch &bl)
+{
+    std::vector<StepArgs> av;
+    std::vector<int> nb;
+    for (size_t k = 0; k < cs.size(); k++) {
+        Ctx *c = cs[k];
+        StepArgs a{};
+        a.pos = c->pos;
+        a.vel = c->vel;
+        a.frc = c->frc;
+        a.frc_recip = c->prm.electrostatics ? c->frc_recip : nullptr;
+        a.xref = c->xref;
+        a.gid = c->gid;
+        a.frz = c->nfrozen > 0 ? c->frz : nullptr;
+        a.n = c->natms;
+        a.dt = c->run_dt;
+        a.two_scale = c->prm.electrostatics ? 2.0 * recip_scale(c) : 0.0;
+        a.inv_n = 1.0 / (double)c->natms;
+        a.half_pad = kHalfMinus * c->prm.padding;
+        for (int d = 0; d < 3; d++) {
+            a.b.L[d] = c->L[d];
+            a.b.invL[d] = c->invL[d];
+        }
+        a.red = c->red;
+        a.red_out = c->red_out;
+        for (int q = 0; q < kReduceSlots; q++) a.rows[q] = rows[k].rows[q];
+        a.energies = energies ? c->energies_dev : nullptr;
+        a.ecap = c->energies_cap;
+        a.flags = c->flags;
+        av.push_back(a);
+        nb.push_back(step_blocks(c));
+    }
+    return batch_upload(cs[0], bl, av.data(), sizeof(StepArgs), nb, false);
+}
+
+int step_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl, bool stage2, bool stage1)
+{
+    const dim3 g(bl.total), b(kVv1Threads);
+    const StepArgs *av = (const StepArgs *)bl.args;
+    if (stage2 && stage1) hipLaunchKernelGGL((k_step_batch<true, true>), g, b, 0, s, av, bl.boff, bl.nsys);
+    else if (stage2) hipLaunchKernelGGL((k_step_batch<true, false>), g, b, 0, s, av, bl.boff, bl.nsys);
+    else if (stage1) hipLaunchKernelGGL((k_step_batch<false, true>), g, b, 0, s, av, bl.boff, bl.nsys);
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }

@@ -356,6 +356,11 @@ struct GatherArgs {
     const R *Phi;
     double4 *frc_recip;
     double *red;
+    // batched runs: the energy and net-force columns are also added into
+    // acc_out[R_ERECIP..R_FNETZ] (zero before the launch) with one atomic
+    // each per workgroup, so the velocity stage needs no reduction launch to
+    // subtract the mean reciprocal force (null: rows only)
+    double *acc_out;
 };
 template <typename R, int S>
 __device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
@@ -464,7 +469,8 @@ __device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
         acc[2] = Fy_;
         acc[3] = Fz_;
     }
-    block_sum<4>(acc, lred, ga.red + (size_t)bid * kReduceSlots + R_ERECIP);
+    const double tot = block_sum<4>(acc, lred, ga.red + (size_t)bid * kReduceSlots + R_ERECIP);
+    if (ga.acc_out && t < 4 && tot != 0.0) atomicAdd(&ga.acc_out[R_ERECIP + t], tot);
 }
 template <typename R, int S>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs<R> a)
@@ -610,52 +616,15 @@ __device__ __forceinline__ void reduce_body(const double *__restrict__ red, cons
                                             double *rec)
 {
     __shared__ double part[256];
-    double s = 0.0;
-    for (int r = threadIdx.x; r < rows[col]; r += blockDim.x) s += red[(size_t)r * kReduceSlots + col];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
-    }
+    const double v = column_sum<256>(red, rows[col], kReduceSlots, col, part);
     if (threadIdx.x == 0) {
-        out[col] = part[0];
-        if (rec) rec[col] = part[0];
+        out[col] = v;
+        if (rec) rec[col] = v;
     }
 }
 __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ red, RedRows rr, double *out)
 {
     reduce_body(red, rr.rows, out, blockIdx.x, nullptr);
-}
-
-// Batched reductions: workgroup b sums column b % kReduceSlots of system
-// b / kReduceSlots.  With a per-step record (energies not null) the totals
-// are also written to record F_STEP of the system's energies, and
-// k_record_next then advances F_STEP.
-struct ReduceArgs {
-    const double *red;
-    RedRows rr;
-    double *out;
-    double *energies;  // may be null
-    const int32_t *flags;
-    int ecap;          // records the energies array holds
-};
-__global__ void __launch_bounds__(256) k_reduce_batch(const ReduceArgs *__restrict__ av, int nsys)
-{
-    const int s = blockIdx.x / kReduceSlots, col = blockIdx.x % kReduceSlots;
-    if (s >= nsys) return;
-    const ReduceArgs &a = av[s];
-    double *rec = nullptr;
-    if (a.energies) {
-        const int k = a.flags[F_STEP];
-        if (k < a.ecap) rec = a.energies + (size_t)k * kReduceSlots;
-    }
-    reduce_body(a.red, a.rr.rows, a.out, col, rec);
-}
-__global__ void k_record_next(int32_t *const *__restrict__ flags, int nsys)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsys) flags[s][F_STEP] += 1;
 }
 
 int ensure_red_rows(Ctx *c, int rows)
@@ -769,7 +738,7 @@ static SpreadArgs<R> spread_args(const Ctx *c)
 template <typename R>
 static GatherArgs<R> gather_args(const Ctx *c)
 {
-    return GatherArgs<R>{c->pos, c->natms, spme_params(c), (const R *)c->Phi, c->frc_recip, c->red};
+    return GatherArgs<R>{c->pos, c->natms, spme_params(c), (const R *)c->Phi, c->frc_recip, c->red, nullptr};
 }
 
 #define DLPX_S_SWITCH(S, CALL)                                                                             \
@@ -855,6 +824,7 @@ static int spme_prepare_t(const std::vector<Ctx *> &cs, SpmeBatch &sb)
         }
         spa.push_back(on ? spread_args<R>(c) : SpreadArgs<R>{});
         ga.push_back(on ? gather_args<R>(c) : GatherArgs<R>{});
+        ga.back().acc_out = on ? c->red_out : nullptr;
         ips.push_back(on ? infl_params(c) : InflParams{});
         nbs.push_back(on ? c->ncell : 0);
         nbg.push_back(on ? gather_blocks_exact(c) : 0);
@@ -899,47 +869,6 @@ int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb)
 {
     if (c->prm.precision == DLPX_MIXED) return spme_launch_t<float>(c, s, sb);
     return spme_launch_t<double>(c, s, sb);
-}
-
-// Batched reductions (and the optional per-step records) of an ensemble.
-int reduce_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
-                   BatchLaunch &bl, int32_t **flag_ptrs)
-{
-    std::vector<ReduceArgs> a;
-    std::vector<int> nb;
-    std::vector<int32_t *> fl;
-    for (size_t k = 0; k < cs.size(); k++) {
-        Ctx *c = cs[k];
-        ReduceArgs r;
-        r.red = c->red;
-        for (int q = 0; q < kReduceSlots; q++) r.rr.rows[q] = rows[k].rows[q];
-        r.out = c->red_out;
-        r.energies = energies ? c->energies_dev : nullptr;
-        r.flags = c->flags;
-        r.ecap = c->energies_cap;
-        a.push_back(r);
-        nb.push_back(kReduceSlots);
-        fl.push_back(c->flags);
-    }
-    int r;
-    if ((r = batch_upload(cs[0], bl, a.data(), sizeof(ReduceArgs), nb, false))) return r;
-    DLPX_HIP(cs[0], dev_upload(cs[0]->stream, flag_ptrs, fl.data(), sizeof(int32_t *) * fl.size()));
-    return 0;
-}
-
-int reduce_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl)
-{
-    hipLaunchKernelGGL(k_reduce_batch, dim3(bl.nsys * kReduceSlots), dim3(256), 0, s, (const ReduceArgs *)bl.args,
-                       bl.nsys);
-    DLPX_HIP(c, hipGetLastError());
-    return 0;
-}
-
-int record_next_launch(Ctx *c, hipStream_t s, int32_t *const *flag_ptrs, int nsys)
-{
-    hipLaunchKernelGGL(k_record_next, dim3(ceil_div(nsys, 256)), dim3(256), 0, s, flag_ptrs, nsys);
-    DLPX_HIP(c, hipGetLastError());
-    return 0;
 }
 
 int spme_forces(Ctx *c)
