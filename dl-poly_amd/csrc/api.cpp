@@ -507,8 +507,9 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         (r = dalloc(c, &c->gid, n)) || (r = dalloc(c, &c->gid_alt, n)) || (r = dalloc(c, &c->slot, n)) ||
         (r = dalloc(c, &c->stage, 9 * n)) || (r = dalloc(c, &c->stage_i, n)) || (r = dalloc(c, &c->cell_of, n)) ||
         (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, F_WORDS)) ||
-        (r = dalloc(c, &c->red_out, kReduceSlots)))
+        (r = dalloc(c, &c->red_out, kReduceSlots)) || (r = dalloc(c, &c->acc, 4 * kAccSlots)))
         return bail(r);
+    if (dev_zero(c->stream, c->acc, sizeof(double) * 4 * kAccSlots) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (dev_zero(c->stream, c->vel, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (dev_zero(c->stream, c->flags, sizeof(int32_t) * F_WORDS) != hipSuccess) return bail(DLPX_ERR_HIP);
     c->serial = next_serial();
@@ -539,7 +540,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
                    c->d_vdw_direct, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
-                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->energies_dev,
+                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->acc, c->energies_dev,
                    c->snap_d, c->snap_i};
     for (void *p : dev)
         if (p) (void)hipFree(p);

@@ -315,7 +315,7 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         DLPX_HIP(c, hipMemsetAsync(c->flags + F_NREB, 0, sizeof(int32_t) * 2, c0->stream));
         DLPX_HIP(c, hipMemsetAsync(c->flags + F_DMAX, 0, sizeof(int32_t) * 3, c0->stream));
         // the totals that the gather and the step kernel accumulate atomically
-        DLPX_HIP(c, hipMemsetAsync(c->red_out + R_ERECIP, 0, sizeof(double) * (R_FNETZ - R_ERECIP + 1), c0->stream));
+        DLPX_HIP(c, hipMemsetAsync(c->acc, 0, sizeof(double) * 4 * kAccSlots, c0->stream));
         DLPX_HIP(c, hipMemsetAsync(c->red_out + R_KIN, 0, sizeof(double), c0->stream));
     }
     for (Ctx *c : cs)

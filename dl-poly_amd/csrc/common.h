@@ -22,6 +22,10 @@ constexpr int kReduceSlots = 28;
 // [1..6] pair stress share (xx, xy, xz, yy, yz, zz), [8..13] reciprocal
 // stress (same order, before the SPME scale)
 constexpr int kPPStride = 16;
+// slots of the gather's atomic energy and net-force totals in batched runs:
+// workgroup b adds into slot b % kAccSlots, so no address takes more than a
+// few dozen atomics
+constexpr int kAccSlots = 64;
 
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs of an
 // MI355X (workgroup b on XCD b % 8), each with its own L2.  Kernels whose
@@ -230,6 +234,7 @@ struct Ctx {
     int red_cap = 0;
     int red_rows[kReduceSlots]{};    // rows written per column by the last evaluation
     double *red_out = nullptr;       // kReduceSlots totals (device)
+    double *acc = nullptr;           // kAccSlots x 4 atomic totals of the gather (batched runs), zero between steps
     double *h_red = nullptr;         // pinned mirror of red_out
     double *energies_dev = nullptr;  // per-step energies (resident runs)
     int energies_cap = 0;

@@ -262,10 +262,11 @@ int nve_second(Ctx *c, double dt, bool /*kinetic*/)
 // One launch per step of a resident batched run (every system of the batch):
 // S2, stage 2 of the step just evaluated: the reciprocal finish (frc +=
 // 2 scale (frc_recip - mean), the mean from the gather's atomic totals in
-// red_out), frozen atoms, the second half-kick and m v^2; the step's partial
+// acc), frozen atoms, the second half-kick and m v^2; the step's partial
 // sums reduced into red_out (workgroup b takes columns b, b + nb, ...; the
 // kinetic energy is added atomically, the reciprocal energy and net force
-// came from the gather), and the step's record when records are kept; and
+// come from the gather's slots), and the step's record when records are
+// kept; and
 // S1, stage 1 of the next step: half-kick, drift, and the displacement test,
 // whose largest value the last workgroup to finish turns into the rebuild
 // decision (as k_vv1).  Workgroups exchange data only through device-scope
@@ -279,6 +280,7 @@ struct StepArgs {
     double dt, two_scale, inv_n, half_pad;
     Box b;
     double *red, *red_out;
+    double *acc;               // the gather's kAccSlots x 4 totals (null: no reciprocal part)
     int rows[kReduceSlots];    // partial rows per column of the step
     double *energies;          // per-step records (kReduceSlots each) or null
     int ecap;
@@ -291,8 +293,20 @@ __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
     __shared__ double lsum[kVv1Threads];
     __shared__ double wmax[kVv1Threads / 64];
     __shared__ double lred[kVv1Threads / 64];
+    __shared__ double recip[4];  // reciprocal energy and net force (unscaled)
     __shared__ bool last;
     const int t = threadIdx.x;
+    if (S2) {
+        // the gather's slot totals, summed in a fixed order by the first wavefront
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (a.acc && t < kAccSlots)
+            for (int k = 0; k < 4; k++) v[k] = a.acc[4 * t + k];
+        if (t < 64)
+            for (int k = 0; k < 4; k++) v[k] = wave_sum(v[k]);
+        if (t == 0)
+            for (int k = 0; k < 4; k++) recip[k] = v[k];
+        __syncthreads();
+    }
     // the step's record index (the last workgroup advances it after every
     // workgroup has read it)
     const int rec = (S2 && a.energies) ? a.flags[F_STEP] : a.ecap;
@@ -315,8 +329,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
         double4 f = a.frc[i];
         if (S2) {
             if (a.frc_recip) {
-                const double mx = a.red_out[R_FNETX] * a.inv_n, my = a.red_out[R_FNETY] * a.inv_n,
-                             mz = a.red_out[R_FNETZ] * a.inv_n;
+                const double mx = recip[1] * a.inv_n, my = recip[2] * a.inv_n, mz = recip[3] * a.inv_n;
                 const double4 r = a.frc_recip[i];
                 f.x += (r.x - mx) * a.two_scale;
                 f.y += (r.y - my) * a.two_scale;
@@ -376,15 +389,24 @@ __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
         last = atomicAdd((unsigned int *)(a.flags + F_DONE), 1u) == (unsigned)nb - 1;
     }
     __syncthreads();
-    if (!last || t != 0) return;
+    if (!last) return;
     if (S2) {
-        // the atomic totals into the record, then cleared for the next step
-        for (int col = 0; col < kReduceSlots; col++)
-            if (atomic_column(col)) {
-                const double v = load_coherent(&a.red_out[col]);
-                if (record) record[col] = v;
-                __hip_atomic_store(&a.red_out[col], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+        // the gather's slots cleared for the next step (every workgroup has
+        // read them)
+        if (a.acc && t < 4 * kAccSlots)
+            __hip_atomic_store(&a.acc[t], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (t != 0) return;
+    if (S2) {
+        // the atomic totals into red_out and the record; the kinetic one is
+        // cleared for the next step
+        const double kin = load_coherent(&a.red_out[R_KIN]);
+        __hip_atomic_store(&a.red_out[R_KIN], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < 4; k++) a.red_out[R_ERECIP + k] = recip[k];
+        if (record) {
+            record[R_KIN] = kin;
+            for (int k = 0; k < 4; k++) record[R_ERECIP + k] = recip[k];
+        }
         if (a.energies) a.flags[F_STEP] = rec + 1;
     }
     if (S1) {
@@ -440,6 +462,7 @@ int step_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &r
         }
         a.red = c->red;
         a.red_out = c->red_out;
+        a.acc = c->prm.electrostatics ? c->acc : nullptr;
         for (int q = 0; q < kReduceSlots; q++) a.rows[q] = rows[k].rows[q];
         a.energies = energies ? c->energies_dev : nullptr;
         a.ecap = c->energies_cap;

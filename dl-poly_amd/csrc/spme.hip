@@ -356,11 +356,11 @@ struct GatherArgs {
     const R *Phi;
     double4 *frc_recip;
     double *red;
-    // batched runs: the energy and net-force columns are also added into
-    // acc_out[R_ERECIP..R_FNETZ] (zero before the launch) with one atomic
+    // batched runs: the energy and net-force totals are also added into
+    // acc[4 (bid % kAccSlots) + 0..3] (zero before the launch), one atomic
     // each per workgroup, so the velocity stage needs no reduction launch to
     // subtract the mean reciprocal force (null: rows only)
-    double *acc_out;
+    double *acc;
 };
 template <typename R, int S>
 __device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
@@ -470,7 +470,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
         acc[3] = Fz_;
     }
     const double tot = block_sum<4>(acc, lred, ga.red + (size_t)bid * kReduceSlots + R_ERECIP);
-    if (ga.acc_out && t < 4 && tot != 0.0) atomicAdd(&ga.acc_out[R_ERECIP + t], tot);
+    if (ga.acc && t < 4 && tot != 0.0) atomicAdd(&ga.acc[4 * (bid % kAccSlots) + t], tot);
 }
 template <typename R, int S>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs<R> a)
@@ -824,7 +824,7 @@ static int spme_prepare_t(const std::vector<Ctx *> &cs, SpmeBatch &sb)
         }
         spa.push_back(on ? spread_args<R>(c) : SpreadArgs<R>{});
         ga.push_back(on ? gather_args<R>(c) : GatherArgs<R>{});
-        ga.back().acc_out = on ? c->red_out : nullptr;
+        ga.back().acc = on ? c->acc : nullptr;
         ips.push_back(on ? infl_params(c) : InflParams{});
         nbs.push_back(on ? c->ncell : 0);
         nbg.push_back(on ? gather_blocks_exact(c) : 0);
