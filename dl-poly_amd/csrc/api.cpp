@@ -74,7 +74,8 @@ int upload_tables(Ctx *c)
                         !std::memcmp(&c->vdw_frc[(size_t)k * mg1], &c->vdw_frc[(size_t)k2 * mg1], sizeof(double) * mg1) &&
                         !std::memcmp(&c->vdw_pot[(size_t)k * mg1], &c->vdw_pot[(size_t)k2 * mg1], sizeof(double) * mg1);
             if (same && c->vdw_direct)
-                same = !std::memcmp(&c->gen_params[5 * (size_t)k], &c->gen_params[5 * (size_t)k2], sizeof(double) * 5);
+                same = !std::memcmp(&c->gen_params[kVdwParams * (size_t)k], &c->gen_params[kVdwParams * (size_t)k2],
+                                    sizeof(double) * kVdwParams);
             if (same) row_of[k] = q;
         }
         if (row_of[k] < 0) {
@@ -124,21 +125,24 @@ int upload_tables(Ctx *c)
     DLPX_HIP(c, hipMalloc((void **)&c->d_keytab, sizeof(int32_t) * kt.size()));
     DLPX_HIP(c, dev_upload(c->stream, c->d_keytab, kt.data(), sizeof(int32_t) * kt.size()));
     if (c->vdw_direct) {
-        // vdw_method direct: per key {form, 5 parameters, afs, bfs}; afs, bfs
-        // follow vdw_direct_fs_generate (vdw.F90:969-1048) under force-shifting
-        std::vector<double> d((size_t)8 * nrows_alloc, 0.0);
+        // vdw_method direct: per row {form, prepared parameters, afs, bfs}; afs,
+        // bfs follow vdw_direct_fs_generate (vdw.F90:969-1048) under force-shifting
+        constexpr int S = kDirectStride;
+        std::vector<double> d((size_t)S * nrows_alloc, 0.0);
         for (int row = 0; row < c->nrows; row++) {
             const int k = key_of_row[row];
             const int kind = c->ltp[k];
             if (!vdw_kind_supported(kind)) continue;
-            const double *pk = c->gen_params.data() + 5 * (size_t)k;
-            d[8 * row] = (double)kind;
-            for (int q = 0; q < 5; q++) d[8 * row + 1 + q] = pk[q];
-            if (c->vdw_force_shift && kind != 7 && kind != 10 && kind != 23) {  // none for n-m shift, DPD, nDPD (vdw.F90:1015-1030)
+            double q[kVdwParams];
+            vdw_prepare(kind, c->gen_params.data() + kVdwParams * (size_t)k, c->prm.vdw_cutoff, q);
+            d[S * row] = (double)kind;
+            for (int j = 0; j < kVdwParams; j++) d[S * row + 1 + j] = q[j];
+            // none for n-m shift, DPD, nDPD, LJF and 12-6-MDF (vdw.F90:1015-1040)
+            if (c->vdw_force_shift && kind != 7 && kind != 10 && kind != 23 && kind != 21 && kind != 20) {
                 double z, dz;
-                vdw_form(kind, pk, c->prm.vdw_cutoff, &z, &dz);
-                d[8 * row + 6] = dz / c->prm.vdw_cutoff;
-                d[8 * row + 7] = -z - dz;
+                vdw_form(kind, q, c->prm.vdw_cutoff, &z, &dz);
+                d[S * row + S - 2] = dz / c->prm.vdw_cutoff;
+                d[S * row + S - 1] = -z - dz;
             }
         }
         if (c->d_vdw_direct) (void)hipFree(c->d_vdw_direct);
@@ -600,18 +604,18 @@ int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, 
     std::vector<double> pot((size_t)nkeys * (mg + 1), 0.0), frc((size_t)nkeys * (mg + 1), 0.0);
     for (int k = 0; k < nkeys; k++) {
         if (vdw_kind_supported(kind[k])) {
-            vdw_table(kind[k], params + 5 * k, c->prm.vdw_cutoff, mg, pot.data() + (size_t)k * (mg + 1),
-                      frc.data() + (size_t)k * (mg + 1));
+            vdw_table(kind[k], params + kVdwParams * (size_t)k, c->prm.vdw_cutoff, mg,
+                      pot.data() + (size_t)k * (mg + 1), frc.data() + (size_t)k * (mg + 1));
         } else if (kind[k] >= 0) {
             return fail(c, DLPX_ERR_UNSUPPORTED,
-                        "vdW kind " + std::to_string(kind[k]) + " is not generated (1, 2, 3, 4, 5, 6, 8 are)");
+                        "vdW kind " + std::to_string(kind[k]) + " is not generated (1-24 are; 0 is tabulated input)");
         }
     }
     std::vector<int32_t> ltp(kind, kind + nkeys);
     const bool direct = c->vdw_direct;
     const int r = dlpx_set_vdw_tables(ctx, nkeys, mg, pot.data(), frc.data(), ltp.data());
     if (r == 0) {
-        c->gen_params.assign(params, params + 5 * (size_t)nkeys);
+        c->gen_params.assign(params, params + kVdwParams * (size_t)nkeys);
         c->vdw_direct = direct;  // the forms are known again
     }
     return r;

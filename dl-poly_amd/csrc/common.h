@@ -157,7 +157,7 @@ struct Ctx {
     int nrows = 0;                         // distinct table rows on the device (keys with equal tables share one)
     std::vector<double> vdw_pot, vdw_frc;  // nkeys x (max_grid+1)
     std::vector<int32_t> ltp;
-    std::vector<double> gen_params;        // 5 per key when the tables were generated here
+    std::vector<double> gen_params;        // kVdwParams per key when the tables were generated here
     int nsamples = 0;
     std::vector<double> erfc_t, erfc_d;    // nsamples
     double elrc = 0.0, vlrc = 0.0;
@@ -212,7 +212,7 @@ struct Ctx {
     void *d_vdw = nullptr;           // nkeys x (max_grid+1) x {frc, pot}, FP64 or FP32
     void *d_erfc = nullptr;          // nsamples+1 x {deriv, erfc} (index 0 virtual)
     int32_t *d_keytab = nullptr;     // ntypes*ntypes -> key or -1
-    void *d_vdw_direct = nullptr;    // vdw_method direct: 8 per key {form, params[5], afs, bfs}, FP64 or FP32
+    void *d_vdw_direct = nullptr;    // vdw_method direct: kDirectStride per row, FP64 or FP32
 
     // SPME
     int K[3]{0, 0, 0};
@@ -350,9 +350,16 @@ int fail(Ctx *c, int code, const std::string &msg);
     } while (0)
 
 // host table generation (tables.cpp)
+// vdW parameters per pair key: vdws%param(1:max_param, k), max_param = 7
+// (bounds.F90:666), in internal units as the FIELD reader leaves them
+constexpr int kVdwParams = 7;
+// device row of vdw_method direct: {form, prepared params[7], afs, bfs}
+constexpr int kDirectStride = 3 + kVdwParams;
 int table_size(double cut);
 bool vdw_kind_supported(int kind);
-void vdw_form(int kind, const double *p, double r, double *e, double *g);  // U(r), gamma = -r dU/dr
+// params of one key -> the constants the evaluators take (vdw_prepare)
+void vdw_prepare(int kind, const double *p, double rvdw, double *q);
+void vdw_form(int kind, const double *q, double r, double *e, double *g);  // U(r), gamma = -r dU/dr
 void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc);
 void erfc_tables(double rc, double alpha, int n, double *erfc_t, double *erfc_d);
 void bspline_norm2(int K, int S, double *norm2);

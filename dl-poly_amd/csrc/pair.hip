@@ -30,9 +30,8 @@ struct PairParams {
 
 // 3-point Newton interpolation of DL_POLY (numerics.F90:250-290) on the
 // .x (force) and .y (energy) members of consecutive samples.
-// r and 1/r^2 from r^2: FP64 with IEEE square root and division; FP32 with
-// the hardware reciprocal square root (about one ulp), well inside the mixed
-// path's error budget.
+// r and 1/r^2 from r^2: FP64 with IEEE square root and division; FP32 from
+// the hardware reciprocal square root refined by one Newton step.
 __device__ __forceinline__ void r_rinv2(double rsq, double &r, double &rinv2)
 {
     r = sqrt(rsq);
@@ -41,15 +40,34 @@ __device__ __forceinline__ void r_rinv2(double rsq, double &r, double &rinv2)
 }
 __device__ __forceinline__ void r_rinv2(float rsq, float &r, float &rinv2)
 {
-    const float ri = __builtin_amdgcn_rsqf(rsq);
+    const float ri0 = __builtin_amdgcn_rsqf(rsq);
+    // one Newton step on 1/sqrt(rsq): the hardware estimate's error leans one
+    // way, which would bias every table lookup (r) and every pair's force
+    // and virial (1/r^2) alike
+    const float u = __builtin_fmaf(-(rsq * ri0), ri0, 1.0f);
+    const float ri = __builtin_fmaf(0.5f * ri0, u, ri0);
     r = rsq * ri;
     rinv2 = ri * ri;
 }
 
-template <typename R, typename R2>
-__device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R &f, R &e)
+// The table coordinate r/dr: FP64 as the reference forms it (r * rdr);
+// FP32 with the reciprocal spacing split into a float and its rounding
+// remainder (rdr_lo), so the coordinate carries no systematic offset from
+// the float rounding of rdr, which would move every lookup of a table the
+// same way and bias the sums of steep forms (about 3e-7 of a ZBL sum).
+template <typename R>
+__device__ __forceinline__ R table_coord(R r, R rdr, R rdr_lo)
 {
-    const R s = r * rdr;
+    if constexpr (sizeof(R) == 4)
+        return __builtin_fmaf(r, rdr, r * rdr_lo);
+    else
+        return r * rdr;
+}
+
+template <typename R, typename R2>
+__device__ __forceinline__ void interp2(const R2 *tab, R r, R rdr, R rdr_lo, R &f, R &e)
+{
+    const R s = table_coord(r, rdr, rdr_lo);
     const int l = (int)s;
     const R ppp = s - (R)l;
     const R2 a = tab[l], b = tab[l + 1], c = tab[l + 2];
@@ -74,12 +92,119 @@ __device__ __forceinline__ R rpow(R x, int n)
     for (int k = 0; k < n; k++) y *= x;
     return y;
 }
+// pieces of the composite forms (kk = Z1 Z2 r4pie0 and a = inverse
+// screening length prepared on the host)
+template <typename R>
+__device__ __forceinline__ void zbl_form(R kk, R a, R r, R &e, R &g)
+{
+    const R b[4] = {(R)0.18175, (R)0.50986, (R)0.28022, (R)0.02817};
+    const R c[4] = {(R)3.1998, (R)0.94229, (R)0.40290, (R)0.20162};
+    const R x = r * a;
+    R s = (R)0, sc = (R)0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const R t = b[i] * exp(-x * c[i]);
+        s += t;
+        sc += c[i] * t;
+    }
+    e = kk * s * ((R)1 / r);
+    g = e + a * kk * sc;
+}
+template <typename R>
+__device__ __forceinline__ void fermi_form(R rm, R ic, R r, R &f, R &df)
+{
+    const R c = (R)1 / ic;
+    const R t = exp(-fabs(r - rm) * c) * (R)0.5;
+    f = r < rm ? (R)1 - t : t;
+    df = r * c * t;
+}
+template <typename R>
+__device__ __forceinline__ void mdf_form(R ri, R rc, R r, R &m, R &dm)
+{
+    m = r < ri ? (R)1 : (R)0;
+    dm = (R)0;
+    if (r >= ri && r <= rc) {
+        const R w = rc - ri, rci = w * w * w * w * w, u = rc - r;
+        m = u * u * u * ((R)10 * ri * ri - (R)5 * rc * ri - (R)15 * r * ri + rc * rc + (R)3 * r * rc + (R)6 * r * r) / rci;
+        dm = (R)30 * r * (r - rc) * (r - rc) * (r - ri) * (r - ri) / rci;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void morse_form(const R *p, R r, R &e, R &g)
+{
+    const R t = exp(-p[2] * (r - p[1]));
+    e = p[0] * (((R)1 - t) * ((R)1 - t) - (R)1);
+    g = (R)-2 * r * p[0] * p[2] * ((R)1 - t) * t;
+}
+template <typename R>
+__device__ __forceinline__ void buck_form(const R *p, R r, R &e, R &g)
+{
+    const R b = r / p[1], t1 = p[0] * exp(-b), t2 = -p[2] / rpow(r, 6);
+    e = t1 + t2;
+    g = t1 * b + (R)6 * t2;
+}
+
 template <typename R>
 __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
 {
     e = (R)0;
     g = (R)0;
     switch (kind) {
+        case 15:  // ZBL (:716-741)
+            zbl_form(p[0], p[1], r, e, g);
+            break;
+        case 16:    // ZBL switched to Morse (:787-808)
+        case 17: {  // ZBL switched to Buckingham (:826-847)
+            R z, dz, f, df, m, dm;
+            zbl_form(p[0], p[1], r, z, dz);
+            fermi_form(p[2], p[3], r, f, df);
+            if (kind == 16)
+                morse_form(p + 4, r, m, dm);
+            else
+                buck_form(p + 4, r, m, dm);
+            e = f * z + ((R)1 - f) * m;
+            g = f * dz + df * z + ((R)1 - f) * dm - df * m;
+            break;
+        }
+        case 18:    // LJ x MDF (:959-976)
+        case 19:    // Buckingham x MDF (:991-1009)
+        case 20: {  // 12-6 x MDF (:1025-1042)
+            R u, du, m, dm;
+            if (kind == 18) {
+                const R s = p[1] / r, s2 = s * s, s6 = s2 * s2 * s2;
+                u = (R)4 * p[0] * s6 * (s6 - (R)1);
+                du = (R)24 * p[0] * s6 * ((R)2 * s6 - (R)1);
+                mdf_form(p[2], p[3], r, m, dm);
+            } else if (kind == 19) {
+                buck_form(p, r, u, du);
+                mdf_form(p[3], p[4], r, m, dm);
+            } else {
+                const R r6 = rpow((R)1 / r, 6);
+                u = (p[0] * r6 - p[1]) * r6;
+                du = (R)6 * r6 * ((R)2 * p[0] * r6 - p[1]);
+                mdf_form(p[2], p[3], r, m, dm);
+            }
+            e = u * m;
+            g = du * m + dm * u;
+            break;
+        }
+        case 21:  // LJF, zero beyond rc (:924-945)
+            if (r * r <= p[2]) {
+                const R ir = (R)1 / (r * r), st = p[1] * ir, rct = p[2] * ir, x = p[0] * (rct - (R)1) * (rct - (R)1);
+                e = x * (st - (R)1);
+                g = (R)4 * p[0] * rct * (rct - (R)1) * (st - (R)1) + (R)2 * x * st;
+            }
+            break;
+        case 24: {  // Stillinger-Weber two-body part, zero beyond aa sig (:1058-1082)
+            const R rs = p[6] * p[3];
+            if (r < rs) {
+                const R ex = p[3] / (r - rs), pr = p[3] / r, c = p[2] * pow(pr, p[4]), prq = pow(pr, p[5]);
+                const R xe = exp(ex), t = p[1] * p[0] * (c - prq) * xe;
+                e = t;
+                g = p[1] * p[0] * (p[4] * c - p[5] * prq) * xe + t * r * ex / (r - rs);
+            }
+            break;
+        }
         case 1: {  // 12-6 (:307-317)
             const R r6 = rpow((R)1 / r, 6);
             e = (p[0] * r6 - p[1]) * r6;
@@ -99,12 +224,9 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
             g = p[0] * p[2] * p[1] * (rn - rm) * b;
             break;
         }
-        case 4: {  // Buckingham (:471-485)
-            const R b = r / p[1], t1 = p[0] * exp(-b), t2 = -p[2] / rpow(r, 6);
-            e = t1 + t2;
-            g = t1 * b + (R)6 * t2;
+        case 4:  // Buckingham (:471-485)
+            buck_form(p, r, e, g);
             break;
-        }
         case 5: {  // Born-Huggins-Meyer (:499-514)
             const R ri2 = (R)1 / (r * r);
             const R t1 = p[0] * exp(p[1] * (p[2] - r)), t2 = -p[3] * (ri2 * ri2 * ri2),
@@ -119,12 +241,9 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
             g = (R)12 * f12 + (R)10 * f10;
             break;
         }
-        case 8: {  // Morse (:416-428)
-            const R t = exp(-p[2] * (r - p[1]));
-            e = p[0] * (((R)1 - t) * ((R)1 - t) - (R)1);
-            g = (R)-2 * r * p[0] * p[2] * ((R)1 - t) * t;
+        case 8:  // Morse (:416-428)
+            morse_form(p, r, e, g);
             break;
-        }
         case 9:  // WCA, zero beyond cut = 2^(1/6) sig + d (:559-576)
             if (r < p[3] || fabs(r - p[2]) < (R)1.0e-10) {
                 const R s6 = rpow(p[1] / (r - p[2]), 6);
@@ -257,10 +376,10 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
     R2 *te = tvl + ((DIRECT || GT) ? 0 : pp.nkeys * pp.mg1);
     int32_t *kt = (int32_t *)(te + (pp.electro ? pp.ns1 : 0));
     R *dv = (R *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
-    double *lred = (double *)(dv + (DIRECT ? 8 * pp.nkeys : 0));
+    double *lred = (double *)(dv + (DIRECT ? kDirectStride * pp.nkeys : 0));
 
     if (DIRECT)
-        for (int k = threadIdx.x; k < 8 * pp.nkeys; k += blockDim.x) dv[k] = direct_g[k];
+        for (int k = threadIdx.x; k < kDirectStride * pp.nkeys; k += blockDim.x) dv[k] = direct_g[k];
     else if (!GT)
         for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tvl[k] = vdw_g[k];
     if (pp.electro)
@@ -287,6 +406,7 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
         const bool qi_on = fabs(pi.w * pp.scaling) >= kZeroPlus && pp.electro;
         const int m = min(cnt[(size_t)i * G + lane], pp.cap_lane);
         const R rvdw = (R)pp.rvdw, rc = (R)pp.rc, rdr_v = (R)pp.rdr_v, rdr_e = (R)pp.rdr_e;
+        const R rdr_v_lo = (R)(pp.rdr_v - (double)rdr_v), rdr_e_lo = (R)(pp.rdr_e - (double)rdr_e);
         const int32_t *ktrow = kt + ti * pp.ntypes;
         const int32_t *lst = nbr + list_slot(i, 0, lane, pp.cap_lane);  // entry c at lst[c * G]
         const R rmax2 = (R)pp.rmax2;
@@ -324,12 +444,13 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
                 R g, ev;
                 if (DIRECT) {
                     // vdw.F90:1696-1699
-                    const R *dk = dv + 8 * key;
+                    const R *dk = dv + kDirectStride * key;
+                    const R afs = dk[kDirectStride - 2], bfs = dk[kDirectStride - 1];
                     vdw_form<R>((int)dk[0], dk + 1, rrr, ge, gf);
-                    ev = ge + dk[6] * rrr + dk[7];
-                    g = gf * rinv2 - dk[6] * (rrr * rinv2);
+                    ev = ge + afs * rrr + bfs;
+                    g = gf * rinv2 - afs * (rrr * rinv2);
                 } else {
-                    interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, gf, ge);
+                    interp2<R, R2>(tv + key * pp.mg1, rrr, rdr_v, rdr_v_lo, gf, ge);
                     g = gf * rinv2;
                     ev = ge;
                 }
@@ -338,7 +459,10 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
                     const R2 cut = tv[key * pp.mg1 + pp.mg1 - 5];
                     const R r_rvdw = (R)pp.r_rvdw;
                     g = g - cut.x * ((rrr * rinv2) * r_rvdw);
-                    ev = ev + cut.x * (rrr * r_rvdw - (R)1) - cut.y;
+                    // r/rvdw - 1 with 1/rvdw split like the table coordinate
+                    const R rs1 = sizeof(R) == 4 ? fma(rrr, (R)(pp.r_rvdw - (double)r_rvdw), fma(rrr, r_rvdw, (R)-1))
+                                                 : rrr * r_rvdw - (R)1;
+                    ev = ev + cut.x * rs1 - cut.y;
                 }
                 gam = g;
                 pacc[0] += ev;
@@ -351,7 +475,7 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
             // no |q_j| > Tiny test: such a q_j contributes exact (or subnormal) zeros
             if (qi_on && rrr < rc) {
                 R df, de;
-                interp2<R, R2>(te, rrr, rdr_e, df, de);
+                interp2<R, R2>(te, rrr, rdr_e, rdr_e_lo, df, de);
                 const R prefac = qi * (R)pj.w;
                 const R eg = prefac * df;
                 gam += eg;
@@ -454,7 +578,7 @@ static int pair_setup(Ctx *c, PairArgs<R> &a, size_t &lds, PairVariant &v, int l
     a.ppo = c->pp;
     const bool direct = c->vdw_direct;
     const size_t lds_rest = sizeof(R2) * (pp.electro ? pp.ns1 : 0) + sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
-                            (direct ? sizeof(R) * 8 * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
+                            (direct ? sizeof(R) * kDirectStride * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
     // the vdW tables go to LDS when they fit in its 160 KiB with the rest,
     // else they are read from global memory
     const size_t lds_tab = direct ? 0 : sizeof(R2) * (size_t)pp.nkeys * pp.mg1;

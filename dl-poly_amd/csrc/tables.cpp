@@ -31,13 +31,44 @@ int table_size(double cut) { return std::max(1004, nint_pos(cut / 0.01) + 4); }
 //  12 LJ cohesive 4 eps [(sig/r)^12 - c (sig/r)^6]       (:283-293)
 //  13 Morse + 12 e0 {[1 - exp(-k (r-r0))]^2 - 1} + c/r^12 (:442-456)
 //  14 Rydberg    (a + b r) exp(-r/c)                     (:689-702)
+//  15 ZBL        Z1 Z2 r4pie0/r sum_i b_i exp(-c_i r/a), 1/a = (Z1^0.23 +
+//                Z2^0.23)/(0.88534 a_B)                  (:716-741)
+//  16 ZBL-Morse  f z + (1-f) Morse, f the Fermi switch at rm of width ic
+//                (:754-774 switch, :787-808)
+//  17 ZBL-Buck   f z + (1-f) Buckingham                  (:826-847)
+//  18 LJ-MDF     LJ(eps, sig) m(r), m the mollified damping from ri to the
+//                vdW cutoff (:893-911 damping, :959-976)
+//  19 Buck-MDF   Buckingham(A, rho, C) m(r)              (:991-1009)
+//  20 12-6-MDF   (a/r^12 - b/r^6) m(r)                   (:1025-1042)
+//  21 LJF        ea (rc^2/r^2 - 1)^2 (sig^2/r^2 - 1) below rc (:924-945)
 //  22 Sanderson  -A exp(-[(r-L)/d]^2)                    (:865-879)
 //  23 nDPD       a rc (1-r/rc)^2 [b (1-r/rc)^(n-1)/(n+1) - 1/2]
 //                below rc                                (:623-642)
-// (no long-range correction for 7, 9, 10, 22 and 23: vdw.F90:681-933)
+//  24 Stillinger-Weber two-body part A eps [B (sig/r)^p - (sig/r)^q]
+//                exp(sig/(r - aa sig)) below aa sig      (:1058-1082)
+// (no long-range correction for 7, 9, 10, 21-24: vdw.F90:681-933)
 bool vdw_kind_supported(int kind)
 {
-    return (kind >= 1 && kind <= 14) || kind == 22 || kind == 23;
+    return kind >= 1 && kind <= 24;
+}
+
+// The constants the evaluators take, from one key's parameters (the FIELD
+// reader's internal units): the ZBL forms carry Z1 Z2 r4pie0 and the inverse
+// screening length in slots 0 and 1 (the two quantities zbl_energy forms from
+// Z1 and Z2 on every call, two_body_potentials.F90:725-727); the MDF forms
+// get the vdW cutoff as the damping's outer radius, as potential_cutoffs
+// sets it (vdw.F90:205-230).
+void vdw_prepare(int kind, const double *p, double rvdw, double *q)
+{
+    for (int k = 0; k < kVdwParams; k++) q[k] = p[k];
+    if (kind == 15 || kind == 16 || kind == 17) {
+        q[0] = p[0] * p[1] * kR4pie0;
+        q[1] = (std::pow(p[0], 0.23) + std::pow(p[1], 0.23)) / (0.52917721067 * 0.88534);
+    } else if (kind == 18 || kind == 20) {
+        q[3] = rvdw;
+    } else if (kind == 19) {
+        q[4] = rvdw;
+    }
 }
 
 static double ipow(double x, int n)
@@ -47,9 +78,96 @@ static double ipow(double x, int n)
     return r;
 }
 
+// ZBL screened Coulomb with kk = Z1 Z2 r4pie0 and a the inverse screening length
+static void zbl(double kk, double a, double r, double *e, double *g)
+{
+    static const double b[4] = {0.18175, 0.50986, 0.28022, 0.02817}, c[4] = {3.1998, 0.94229, 0.40290, 0.20162};
+    const double x = r * a;
+    double s = 0.0, sc = 0.0;
+    for (int i = 0; i < 4; i++) {
+        const double t = b[i] * std::exp(-x * c[i]);
+        s += t;
+        sc += c[i] * t;
+    }
+    *e = kk * s * (1.0 / r);
+    *g = *e + a * kk * sc;
+}
+
+// Fermi switch of the ZBL mixtures: 1 - exp(-(rm-r)/ic)/2 below rm,
+// exp(-(r-rm)/ic)/2 above
+static void fermi(double rm, double ic, double r, double *f, double *df)
+{
+    const double c = 1.0 / ic;
+    const double t = (r < rm ? std::exp(-(rm - r) * c) : std::exp(-(r - rm) * c)) * 0.5;
+    *f = r < rm ? 1.0 - t : t;
+    *df = r * c * t;
+}
+
+// mollified damping function: 1 below ri, 0 above rc, a quintic between
+static void mdf(double ri, double rc, double r, double *m, double *dm)
+{
+    if (r < ri) {
+        *m = 1.0;
+        *dm = 0.0;
+    } else if (r > rc) {
+        *m = *dm = 0.0;
+    } else {
+        const double w = rc - ri, rci = w * w * w * w * w, u = rc - r;
+        *m = u * u * u * (10.0 * ri * ri - 5.0 * rc * ri - 15.0 * r * ri + rc * rc + 3.0 * r * rc + 6.0 * r * r) / rci;
+        *dm = 30.0 * r * (r - rc) * (r - rc) * (r - ri) * (r - ri) / rci;
+    }
+}
+
 static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
 {
     switch (kind) {
+        case 15:  // kk, 1/a
+            zbl(p[0], p[1], r, e, g);
+            break;
+        case 16:  // kk, 1/a, rm, ic, e0, r0, k
+        case 17: {  // kk, 1/a, rm, ic, A, rho, C
+            double z, dz, f, df, m, dm;
+            zbl(p[0], p[1], r, &z, &dz);
+            fermi(p[2], p[3], r, &f, &df);
+            vdw_eval(kind == 16 ? 8 : 4, p + 4, r, &m, &dm);
+            *e = f * z + (1.0 - f) * m;
+            *g = f * dz + df * z + (1.0 - f) * dm - df * m;
+            break;
+        }
+        case 18:  // eps, sig, ri, rc
+        case 19:  // A, rho, C, ri, rc
+        case 20: {  // a, b, ri, rc
+            double u, du, m, dm;
+            const int base = kind == 18 ? 2 : kind == 19 ? 4 : 1;
+            const int nb = kind == 19 ? 3 : 2;
+            vdw_eval(base, p, r, &u, &du);
+            mdf(p[nb], p[nb + 1], r, &m, &dm);
+            *e = u * m;
+            *g = du * m + dm * u;
+            break;
+        }
+        case 21: {  // ea, sig^2, rc^2
+            const double r2 = r * r;
+            *e = *g = 0.0;
+            if (r2 <= p[2]) {
+                const double ir = 1.0 / r2, st = p[1] * ir, rct = p[2] * ir, x = p[0] * (rct - 1.0) * (rct - 1.0);
+                *e = x * (st - 1.0);
+                *g = 4.0 * p[0] * rct * (rct - 1.0) * (st - 1.0) + 2.0 * x * st;
+            }
+            break;
+        }
+        case 24: {  // eps, A, B, sig, p, q, aa
+            *e = *g = 0.0;
+            const double rs = p[6] * p[3];
+            if (r < rs) {
+                const double ex = p[3] / (r - rs), pr = p[3] / r, c = p[2] * std::pow(pr, p[4]);
+                const double prq = std::pow(pr, p[5]), xe = std::exp(ex);
+                const double t = p[1] * p[0] * (c - prq) * xe;
+                *e = t;
+                *g = p[1] * p[0] * (p[4] * c - p[5] * prq) * xe + t * r * ex / (r - rs);
+            }
+            break;
+        }
         case 1: {
             const double r6 = ipow(1.0 / r, 6);
             *e = (p[0] * r6 - p[1]) * r6;
@@ -172,14 +290,16 @@ static void vdw_eval(int kind, const double *p, double r, double *e, double *g)
     }
 }
 
-void vdw_form(int kind, const double *p, double r, double *e, double *g) { vdw_eval(kind, p, r, e, g); }
+void vdw_form(int kind, const double *q, double r, double *e, double *g) { vdw_eval(kind, q, r, e, g); }
 
 // vdw.F90:1397-1451: samples at r_i = i * rvdw/(max_grid-4), i = 1..max_grid;
-// slot 0 holds Huge (never used inside the cutoff).
+// slot 0 holds Huge (never used inside the cutoff).  p: the key's parameters.
 void vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc)
 {
+    double q[kVdwParams];
+    vdw_prepare(kind, p, rvdw, q);
     const double dlrpot = rvdw / (double)(max_grid - 4);
-    for (int i = 1; i <= max_grid; i++) vdw_eval(kind, p, (double)i * dlrpot, &pot[i], &frc[i]);
+    for (int i = 1; i <= max_grid; i++) vdw_eval(kind, q, (double)i * dlrpot, &pot[i], &frc[i]);
     pot[0] = frc[0] = 1.79769313486231570815e308;
 }
 
@@ -324,6 +444,33 @@ static double mm3_tail(double r0, double A, double B, double eps, double rw, dou
     return deriv ? s * h * std::pow(r0, 4) / 3.0 : s * h * std::pow(r0, 3) / 3.0;
 }
 
+// The ZBL and MDF tail integrals as vdw.F90:288-520 evaluates them: the same
+// chunked Simpson rule in r itself (h = rw/n from rw, weight r^2 for the
+// energy and for gamma alike).  A cap on the chunks keeps parameters whose
+// integrand never settles from looping for ever.
+template <typename F>
+static double tail_simpson(F f, double rw, double prec)
+{
+    const int n = 10000;
+    const double h = (2.0 * rw - rw) / (double)n;
+    double s = 0.0, sold = HUGE_VAL;
+    for (int j = 1; std::fabs(s - sold) * h / 3.0 > prec && j <= 100000; j++) {
+        sold = s;
+        for (int i = (j - 1) * n; i <= j * n; i += 2) {
+            const double x = rw + i * h, x1 = x + h, x2 = x + 2.0 * h;
+            s += x * x * f(x) + 4.0 * (x1 * x1) * f(x1) + (x2 * x2) * f(x2);
+        }
+    }
+    return s * h / 3.0;
+}
+
+// energy and gamma tails of one prepared form
+static void tail_pair(int kind, const double *q, double rw, double *eadd, double *padd)
+{
+    *eadd = tail_simpson([&](double x) { double e, g; vdw_eval(kind, q, x, &e, &g); return e; }, rw, 1.0e-12);
+    *padd = tail_simpson([&](double x) { double e, g; vdw_eval(kind, q, x, &e, &g); return g; }, rw, 1.0e-12);
+}
+
 void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params, double rvdw,
              double volm, double *elrc, double *vlrc, const double *num_frz)
 {
@@ -331,7 +478,7 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
     int key = 0;
     for (int i = 1; i <= ntypes; i++)
         for (int j = 1; j <= i; j++, key++) {
-            const double *p = params + 5 * key, r = rvdw;
+            const double *p = params + kVdwParams * (size_t)key, r = rvdw;
             double eadd = 0.0, padd = 0.0;
             switch (kinds[key]) {
                 case 1:
@@ -383,9 +530,12 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                     break;
                 }
                 case 8:
-                case 13: {  // Morse; Morse + c/r^12 adds that term's tail (vdw.F90:810-830)
-                    const double e0 = p[0], r0 = p[1], kk = p[2];
-                    if (kk > 1.17549435e-38) {
+                case 13:
+                case 16: {  // Morse; Morse + c/r^12 adds that term's tail (vdw.F90:810-830); ZBL-Morse
+                            // keeps only the Morse tail (:859-876)
+                    const double *pm = kinds[key] == 16 ? p + 4 : p;
+                    const double e0 = pm[0], r0 = pm[1], kk = pm[2];
+                    if (kk > 2.2250738585072014e-308) {  // Tiny(kk)
                         const double t = std::exp(-kk * (r - r0)), k3 = kk * kk * kk, kr = kk * r;
                         eadd = -2.0 * e0 * t / k3 * ((kr + 1) * (kr + 1) + 1) +
                                e0 * t * t / (4.0 * k3) * ((kr + 1) * (kr + 1) + kk * kk * r * r);
@@ -399,7 +549,40 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
                     }
                     break;
                 }
-                default:
+                case 15: {
+                    // vdw.F90:846-857: the reference hands Z1 Z2 r4pie0 and 1/a
+                    // to intRadZBL, which sets them as the Z1 and Z2 of a ZBL
+                    // object (:288-330), so its integrand is the ZBL of those
+                    // two numbers; the same integrand here
+                    double q1[kVdwParams], q2[kVdwParams];
+                    vdw_prepare(15, p, r, q1);
+                    vdw_prepare(15, q1, r, q2);
+                    tail_pair(15, q2, r, &eadd, &padd);
+                    break;
+                }
+                case 17: {  // vdw.F90:878-890: the Buckingham tail, exponential term included
+                    const double A = p[4], r0 = p[5], cc = p[6], t = A * std::exp(-r / r0);
+                    eadd = (r * r + 2 * r0 * r + 2 * r0 * r0) * t * r0 - cc / (3.0 * r * r * r);
+                    padd = (r * r * r + 3 * r0 * r * r + 6 * r0 * r0 * r + 6 * r0 * r0 * r0) * t - 2.0 * cc / (r * r * r);
+                    break;
+                }
+                case 18:
+                case 19:
+                case 20: {
+                    // vdw.F90:892-921 with intRadMDF (:376-447): the damping's
+                    // outer radius is the cutoff the integral starts from, so the
+                    // tail is zero unless ri lies beyond it; for LJ-MDF the call
+                    // passes eps and sig in each other's places (:403-407)
+                    double q[kVdwParams] = {p[0], p[1], p[2], p[3], p[4], p[5], p[6]};
+                    if (kinds[key] == 18) {
+                        q[0] = p[1];
+                        q[1] = p[0];
+                    }
+                    q[kinds[key] == 19 ? 4 : 3] = r;
+                    tail_pair(kinds[key], q, r, &eadd, &padd);
+                    break;
+                }
+                default:  // 7, 9, 10, 21-24: none
                     break;
             }
             if (i != j) {

@@ -28,7 +28,7 @@ import struct
 
 import numpy as np
 
-from .systems import VDW_NAMES, VDW_NPARAMS, VDW_WCA, wca_params
+from .systems import VDW_LJF, VDW_NAMES, VDW_NPARAMS, VDW_WCA, ljf_field_words, ljf_params, wca_params
 
 
 # constants.F90:104,112-114,123: Boltzmann constant and energy units in
@@ -129,7 +129,8 @@ def write_field(path, sysm):
     out.append("vdw %d" % len(sysm.vdw))
     for (ti, tj, key, params) in sysm.vdw:
         out.append("%-8s %-8s %4s %s" % (sysm.type_names[ti], sysm.type_names[tj], VDW_NAMES[key],
-                                         " ".join(_num(p) for p in params[:_VDW_FIELD_NPARAMS[key]])))
+                                         " ".join(_num(p) for p in (ljf_field_words(params) if key == VDW_LJF
+                                                                     else params[:_VDW_FIELD_NPARAMS[key]]))))
     out.append("close")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
@@ -291,12 +292,16 @@ def write_revcon(path, title, cell, names, pos, vel=None, frc=None, nstep=0, tst
 # ----------------------------------------------------------------- FIELD --
 
 _VDW_KEYS = {"12-6": 1, "lj": 2, "nm": 3, "buck": 4, "bhm": 5, "hbnd": 6, "mors": 8, "wca": 9, "dpd": 10,
-             "ljc": 12, "mstw": 13, "ryd": 14, "sand": 22, "ndpd": 23, "snm": 7, "14-7": 11}
-# parameters that carry energy units, per vdW form (ffield.F90:3757-3805:
-# parpot(1) always; 12-6, hbnd and ryd parpot(2); buck parpot(3); bhm
-# parpot(4:5); mstw parpot(4))
+             "ljc": 12, "mstw": 13, "ryd": 14, "sand": 22, "ndpd": 23, "snm": 7, "14-7": 11, "zbl": 15,
+             "zbls": 16, "zblb": 17, "mlj": 18, "mbuc": 19, "m126": 20, "ljf": 21, "sw": 24}
+# parameters that carry energy units, per vdW form (ffield.F90:3757-3834:
+# parpot(1) except for the ZBL forms, whose parpot(1) is a charge; 12-6,
+# hbnd, ryd and m126 parpot(2); buck and mbuc parpot(3); bhm parpot(4:5);
+# mstw parpot(4); zbls parpot(5); zblb parpot(5) and parpot(7)); LJF converts
+# its own (ljf_params)
 _VDW_ENERGY_PARAMS = {1: (0, 1), 2: (0,), 3: (0,), 4: (0, 2), 5: (0, 3, 4), 6: (0, 1), 8: (0,), 9: (0,), 10: (0,),
-                      12: (0,), 13: (0, 3), 14: (0, 1), 22: (0,), 23: (0,), 7: (0,), 11: (0,)}
+                      12: (0,), 13: (0, 3), 14: (0, 1), 22: (0,), 23: (0,), 7: (0,), 11: (0,), 15: (), 16: (4,),
+                      17: (4, 6), 18: (0,), 19: (0, 2), 20: (0, 1), 21: (), 24: (0,)}
 _ZERO_PLUS = 2.2250738585072014e-308  # Tiny(1.0_wp), the reference's zero_plus
 # parameters a FIELD record gives (WCA's cut is derived, ffield.F90:3788)
 _VDW_FIELD_NPARAMS = {**VDW_NPARAMS, VDW_WCA: 3}
@@ -433,8 +438,13 @@ def read_field(path):
                     else:
                         raise ValueError("buck vdW pair %s-%s: rho is zero with A nonzero (DL_POLY error 467)"
                                          % (a[0], a[1]))
+                if kind in (15, 16, 17):
+                    # parpot(1) is scaled to internal units and back (ffield.F90:3757,3806-3816)
+                    params[0] = params[0] * unit / unit
                 for j in _VDW_ENERGY_PARAMS[kind]:
                     params[j] *= unit
+                if kind == VDW_LJF:
+                    params = list(ljf_params(params[0] * unit, params[1], params[2]))
                 if kind == VDW_WCA:
                     params = list(wca_params(*params[:3]))
                 if kind == 7 and params[1] <= params[2]:  # ffield.F90:3753-3755, errors 470 and 468

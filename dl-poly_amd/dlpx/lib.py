@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libdlpx.so")
 
 FP64, MIXED = 0, 1
+VDW_MAX_PARAM = 7  # parameters per vdW key (bounds.F90:666 vdws%max_param)
 
 EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dlpx_set_vdw_tables",
            "dlpx_set_erfc_tables", "dlpx_generate_vdw_tables", "dlpx_generate_erfc_tables", "dlpx_set_lrc",
@@ -145,7 +146,7 @@ class Context:
         s = self.sys
         nkeys = s.ntypes * (s.ntypes + 1) // 2
         kind = np.full(nkeys, -1, np.int32)
-        params = np.zeros((nkeys, 5))
+        params = np.zeros((nkeys, VDW_MAX_PARAM))  # vdws%param(1:max_param, key)
         for (ti, tj, k, prm) in s.vdw:
             a, b = ti + 1, tj + 1
             hi, lo = max(a, b), min(a, b)

@@ -35,16 +35,43 @@ VDW_AMOEBA = 11             # params eps, sig (14-7)
 VDW_LENNARD_JONES_COHESIVE = 12
 VDW_MORSE_12 = 13
 VDW_RYDBERG = 14
+VDW_ZBL = 15                # params Z1, Z2
+VDW_ZBL_SWITCH_MORSE = 16   # params Z1, Z2, rm, ic, e0, r0, k
+VDW_ZBL_SWITCH_BUCKINGHAM = 17  # params Z1, Z2, rm, ic, A, rho, C
+VDW_LJ_MDF = 18             # params eps, sig, ri (damped from ri to the vdW cutoff)
+VDW_BUCKINGHAM_MDF = 19     # params A, rho, C, ri
+VDW_126_MDF = 20            # params a, b, ri
+VDW_LJF = 21                # params ea, sig^2, rc^2 (ljf_params converts eps, sig, rc)
 VDW_SANDERSON = 22          # params A, L, d
 VDW_NDPD = 23               # params a, b, n, rc
+VDW_SW = 24                 # params eps, A, B, sig, p, q, aa
 VDW_NAMES = {VDW_12_6: "12-6", VDW_LENNARD_JONES: "lj", VDW_N_M: "nm", VDW_BUCKINGHAM: "buck",
              VDW_BORN_HUGGINS_MEYER: "bhm", VDW_HYDROGEN_BOND: "hbnd", VDW_MORSE: "mors", VDW_WCA: "wca",
              VDW_DPD: "dpd", VDW_LENNARD_JONES_COHESIVE: "ljc", VDW_MORSE_12: "mstw", VDW_RYDBERG: "ryd",
              VDW_SANDERSON: "sand", VDW_NDPD: "ndpd", VDW_N_M_SHIFT: "snm",
-             VDW_AMOEBA: "14-7"}
+             VDW_AMOEBA: "14-7", VDW_ZBL: "zbl", VDW_ZBL_SWITCH_MORSE: "zbls", VDW_ZBL_SWITCH_BUCKINGHAM: "zblb",
+             VDW_LJ_MDF: "mlj", VDW_BUCKINGHAM_MDF: "mbuc", VDW_126_MDF: "m126", VDW_LJF: "ljf", VDW_SW: "sw"}
 VDW_NPARAMS = {VDW_12_6: 2, VDW_LENNARD_JONES: 2, VDW_N_M: 4, VDW_BUCKINGHAM: 3, VDW_BORN_HUGGINS_MEYER: 5,
                VDW_HYDROGEN_BOND: 2, VDW_MORSE: 3, VDW_WCA: 4, VDW_DPD: 2, VDW_LENNARD_JONES_COHESIVE: 3,
-               VDW_MORSE_12: 4, VDW_RYDBERG: 3, VDW_SANDERSON: 3, VDW_NDPD: 4, VDW_N_M_SHIFT: 5, VDW_AMOEBA: 2}
+               VDW_MORSE_12: 4, VDW_RYDBERG: 3, VDW_SANDERSON: 3, VDW_NDPD: 4, VDW_N_M_SHIFT: 5, VDW_AMOEBA: 2,
+               VDW_ZBL: 2, VDW_ZBL_SWITCH_MORSE: 7, VDW_ZBL_SWITCH_BUCKINGHAM: 7, VDW_LJ_MDF: 3,
+               VDW_BUCKINGHAM_MDF: 4, VDW_126_MDF: 3, VDW_LJF: 3, VDW_SW: 7}
+
+
+def ljf_params(eps, sig, rc):
+    """LJF parameters as the FIELD reader converts them (ffield.F90:3818-3829):
+    ea = eps * x/4 * (3/(x-1))^3 with x = (rc/sig)^2, then sig^2 and rc^2."""
+    x = (rc / sig) ** 2
+    a = 0.25 * x * (3.0 / (x - 1.0)) ** 3
+    return (eps * a, sig ** 2, rc ** 2)
+
+
+def ljf_field_words(params):
+    """The FIELD words (eps, sig, rc) of converted LJF parameters."""
+    ea, sig2, rc2 = params[:3]
+    sig, rc = math.sqrt(sig2), math.sqrt(rc2)
+    x = (rc / sig) ** 2
+    return (ea / (0.25 * x * (3.0 / (x - 1.0)) ** 3), sig, rc)
 
 
 def wca_params(eps, sig, d=0.0):
@@ -273,13 +300,37 @@ MIX2_VDW = [
     (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
 ]
 
+# the ZBL family (bare, switched to Morse at 4 A and to Buckingham at 3.8 A,
+# near the nearest-neighbour distance so both sides of each switch count),
+# LJF (rc 7 A) and the Stillinger-Weber two-body part (cut 5.4 A)
+MIX4_VDW = [
+    (0, 0, VDW_ZBL, (18.0, 18.0)),
+    (1, 0, VDW_ZBL_SWITCH_MORSE, (10.0, 18.0, 4.0, 0.3, 80.0, 3.9, 1.6)),
+    (2, 0, VDW_ZBL_SWITCH_BUCKINGHAM, (36.0, 18.0, 3.8, 0.35, 3.2e7, 0.30, 4.28e5)),
+    (1, 1, VDW_LJF, ljf_params(120.0, 3.25, 7.0)),
+    (2, 1, VDW_SW, (200.0, 7.049556277, 0.6022245584, 3.0, 4.0, 0.0, 1.8)),
+    (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
+]
+# the damped (MDF) forms from ri = 7, 6.5 and 7.5 A to the 9 A cutoff, LJF
+# with rc 9.5 A (beyond the cutoff, so the direct force-shift exception for
+# LJF shows) and a bare ZBL pair
+MIX5_VDW = [
+    (0, 0, VDW_LJ_MDF, (99.6, 3.405, 7.0)),
+    (1, 0, VDW_BUCKINGHAM_MDF, (3.2e7, 0.30, 4.28e5, 6.5)),
+    (2, 0, VDW_126_MDF, (4 * 99.6 * 3.405 ** 12, 4 * 99.6 * 3.405 ** 6, 7.5)),
+    (1, 1, VDW_LJF, ljf_params(120.0, 3.3, 9.5)),
+    (2, 1, VDW_ZBL, (10.0, 36.0)),
+    (2, 2, VDW_LENNARD_JONES, (140.0, 3.6)),
+]
+
 
 def mixture(ncells=10, a=5.78015, temperature=100.0, seed=20261202, charges=(0.3, -0.3, 0.0), kvec=(36, 36, 36),
             alpha=0.35, cutoff=9.0, padding=0.5, timestep=0.004, jitter=0.1, vdw=None):
     """Three species on an FCC lattice (4 sites per cell, edge ``a``), the
     species drawn by a seeded permutation of the sites, with a different vdW
     form for each pair (by default 12-6, Buckingham, 12-10, n-m, Morse,
-    Lennard-Jones; ``vdw=MIX2_VDW`` gives the other six forms) and SPME
+    Lennard-Jones; ``vdw=MIX2_VDW``, ``MIX3_VDW``, ``MIX4_VDW`` and
+    ``MIX5_VDW`` give the other forms) and SPME
     Coulomb.  Atoms are listed species by species (the FIELD's molecule
     order)."""
     n = 4 * ncells ** 3

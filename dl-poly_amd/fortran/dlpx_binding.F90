@@ -92,6 +92,7 @@ Module dlpx_binding
       Real(c_double), Intent(In) :: erfc(*), erfc_deriv(*)
     End Function dlpx_set_erfc_tables
 
+    !> params: vdws%param(1:7, key) for every pair key (7 per key)
     Integer(c_int) Function dlpx_generate_vdw_tables(ctx, nkeys, kind, params) &
         Bind(C, name='dlpx_generate_vdw_tables')
       Import :: c_int, c_int32_t, c_ptr, c_double

@@ -34,7 +34,7 @@ Program dlpx_host_demo
   Real(c_double), Allocatable, Target :: vx(:), vy(:), vz(:), weight(:)
   Integer(c_int32_t) :: nreb
   Real(c_double), Target :: cell(9)
-  Real(c_double) :: a, L, basis(3, 4), params(5, 3), num_type(2)
+  Real(c_double) :: a, L, basis(3, 4), params(7, 3), num_type(2)
   Integer(c_int32_t) :: kinds(3), ibig
   Integer(c_int64_t) :: npairs
   Type(dlpx_params) :: prm
@@ -104,9 +104,11 @@ Program dlpx_host_demo
 
   ! Tosi-Fumi BHM, keys (Na,Na) (Na,Cl) (Cl,Cl), internal units
   kinds = [5, 5, 5]
-  params(:, 1) = [2544.4_c_double, 3.15457_c_double, 2.340_c_double, 10117.0_c_double, 4818.0_c_double]
-  params(:, 2) = [2035.5_c_double, 3.15457_c_double, 2.755_c_double, 67448.0_c_double, 83708.0_c_double]
-  params(:, 3) = [1526.6_c_double, 3.15457_c_double, 3.170_c_double, 698570.0_c_double, 1403160.0_c_double]
+  ! vdws%param(1:7, key), the reference's own layout (max_param = 7)
+  params = 0.0_c_double
+  params(1:5, 1) = [2544.4_c_double, 3.15457_c_double, 2.340_c_double, 10117.0_c_double, 4818.0_c_double]
+  params(1:5, 2) = [2035.5_c_double, 3.15457_c_double, 2.755_c_double, 67448.0_c_double, 83708.0_c_double]
+  params(1:5, 3) = [1526.6_c_double, 3.15457_c_double, 3.170_c_double, 698570.0_c_double, 1403160.0_c_double]
   Call check(dlpx_generate_vdw_tables(ctx, 3_c_int32_t, kinds, params))
   Call check(dlpx_generate_erfc_tables(ctx))
   num_type = [Real(natms / 2, c_double), Real(natms / 2, c_double)]

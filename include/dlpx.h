@@ -93,16 +93,24 @@ int dlpx_version(void);
 int dlpx_set_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, int32_t max_grid, const double *tab_potential,
                         const double *tab_force, const int32_t *ltp);
 int dlpx_set_erfc_tables(dlpx_ctx *ctx, int32_t nsamples, const double *erfc, const double *erfc_deriv);
-/* The same tables generated in the library from potential parameters, for
- * the vdW forms with closed-form long-range corrections (vdw.F90:64-113 keys,
- * parameters in vdws%param order, two_body_potentials.F90): kind[key] 1 =
- * 12-6 A,B; 2 = Lennard-Jones eps,sig; 3 = n-m e0,n,m,r0; 4 = Buckingham
- * A,rho,C; 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B;
- * 7 = shifted n-m e0,n,m,r0,r_trunc; 8 = Morse
- * e0,r0,k; 9 = WCA eps,sig,d,cut (cut = 2^(1/6) sig + d, ffield.F90:3788);
- * 10 = DPD a,rc; 11 = AMOEBA 14-7 eps,sig; 12 = LJ cohesive eps,sig,c; 13 = Morse + c/r^12 e0,r0,k,c;
- * 14 = Rydberg a,b,c; 22 = Sanderson A,L,d; 23 = nDPD a,b,n,rc; -1 none.  params 5 per key.  Other forms: tabulate them on the host
- * and use dlpx_set_vdw_tables.  Sizes follow bounds.F90:811,820. */
+/* The same tables generated in the library from potential parameters
+ * (vdw_generate, vdw.F90:1397-1576): kind[key] is the vdw.F90:64-113 key of
+ * each pair key, -1 none; params holds vdws%param(1:7, key), 7 doubles per
+ * key in the reference's own layout (bounds.F90:666 max_param = 7), in the
+ * internal units the FIELD reader leaves (ffield.F90:3757-3839; LJF's
+ * converted ea, sig^2, rc^2).  Forms (two_body_potentials.F90): 1 = 12-6 A,B;
+ * 2 = Lennard-Jones eps,sig; 3 = n-m e0,n,m,r0; 4 = Buckingham A,rho,C;
+ * 5 = Born-Huggins-Meyer A,B,sigma,C,D; 6 = 12-10 A,B; 7 = shifted n-m
+ * e0,n,m,r0,r_trunc; 8 = Morse e0,r0,k; 9 = WCA eps,sig,d,cut (cut =
+ * 2^(1/6) sig + d, ffield.F90:3788); 10 = DPD a,rc; 11 = AMOEBA 14-7 eps,sig;
+ * 12 = LJ cohesive eps,sig,c; 13 = Morse + c/r^12 e0,r0,k,c; 14 = Rydberg
+ * a,b,c; 15 = ZBL Z1,Z2; 16 = ZBL switched to Morse Z1,Z2,rm,ic,e0,r0,k;
+ * 17 = ZBL switched to Buckingham Z1,Z2,rm,ic,A,rho,C; 18 = LJ x MDF
+ * eps,sig,ri; 19 = Buckingham x MDF A,rho,C,ri; 20 = 12-6 x MDF a,b,ri (the
+ * damping ends at the vdW cutoff, vdw.F90:205-230); 21 = LJF ea,sig^2,rc^2;
+ * 22 = Sanderson A,L,d; 23 = nDPD a,b,n,rc; 24 = Stillinger-Weber two-body
+ * eps,A,B,sig,p,q,aa.  Tabulated (TABLE file) potentials: use
+ * dlpx_set_vdw_tables.  Sizes follow bounds.F90:811,820. */
 int dlpx_generate_vdw_tables(dlpx_ctx *ctx, int32_t nkeys, const int32_t *kind, const double *params);
 int dlpx_generate_erfc_tables(dlpx_ctx *ctx);
 /* vdW long-range corrections vdws%elrc / vdws%vlrc (vdw.F90:617-967).  Pass the

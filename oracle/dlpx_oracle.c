@@ -9,7 +9,9 @@
  * it restates.
  *
  * Pinning (what fixes each function independently of this file):
- *   orc_vdw_table      closed forms of LJ/BHM; test_vdw.F90:40-63 known answers
+ *   orc_vdw_table      closed forms of LJ/BHM; test_vdw.F90:46-59 known answers
+ *                      (keys 1-24); the reference's own tables and vdW long-range
+ *                      corrections (dumped at full precision) for keys 1-24
  *   orc_erfc_tables    A&S 7.1.26 coefficients; exact Gaussian derivative
  *   orc_interp3        exact for quadratic tables
  *   orc_pairs_brute    O(N^2) minimum-image search (the link-cell set is
@@ -215,6 +217,68 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
         *g = -2.0 * (r - p[1]) * r * t / pow(p[2], 2.0);
         return 0;
     }
+    if (kind >= 15 && kind <= 17) {
+        /* ZBL (:716-741) from Z1, Z2; the ZBL-Morse (:787-808) and ZBL-Buckingham
+         * (:826-847) mixtures through the Fermi switch (:754-774) at rm = p[2],
+         * width ic = p[3], with the Morse / Buckingham parameters in p[4..6] */
+        static const double zb[4] = {0.18175, 0.50986, 0.28022, 0.02817}, zc[4] = {3.1998, 0.94229, 0.40290, 0.20162};
+        double a = (pow(p[0], 0.23) + pow(p[1], 0.23)) / (0.52917721067 * 0.88534);
+        double kk = p[0] * p[1] * R4PIE0, x = r * a, ir = 1.0 / r, z = 0.0, dz = 0.0;
+        for (int i = 0; i < 4; i++) {
+            double t1 = zb[i] * exp(-x * zc[i]);
+            z += t1;
+            dz -= zc[i] * t1;
+        }
+        z = kk * z * ir;
+        dz = z - a * kk * dz;
+        if (kind == 15) { *e = z; *g = dz; return 0; }
+        double c = 1.0 / p[3], f, df, t, m, dm;
+        if (r < p[2]) { t = exp(-(p[2] - r) * c) * 0.5; f = 1.0 - t; df = r * c * t; }
+        else { t = exp(-(r - p[2]) * c) * 0.5; f = t; df = r * c * t; }
+        orc_vdw_eval(kind == 16 ? 8 : 4, p + 4, r, &m, &dm);
+        *e = f * z + (1.0 - f) * m;
+        *g = f * dz + df * z + (1.0 - f) * dm - df * m;
+        return 0;
+    }
+    if (kind >= 18 && kind <= 20) {
+        /* LJ (:959-976), Buckingham (:991-1009) or 12-6 (:1025-1042) times the
+         * mollified damping (:893-911) from ri to rc; params eps, sig, ri, rc /
+         * A, rho, C, ri, rc / a, b, ri, rc (rc: the vdW cutoff, orc_vdw_params) */
+        int nb = kind == 19 ? 3 : 2;
+        double u, du, m, dm, ri = p[nb], rc = p[nb + 1];
+        orc_vdw_eval(kind == 18 ? 2 : kind == 19 ? 4 : 1, p, r, &u, &du);
+        if (r < ri) { m = 1.0; dm = 0.0; }
+        else if (r > rc) { m = 0.0; dm = 0.0; }
+        else {
+            double rci = pow(rc - ri, 5);
+            m = pow(rc - r, 3) * (10.0 * ri * ri - 5.0 * rc * ri - 15.0 * r * ri + rc * rc + 3.0 * r * rc + 6 * r * r) / rci;
+            dm = 30.0 * r * pow(r - rc, 2) * pow(r - ri, 2) / rci;
+        }
+        *e = u * m;
+        *g = du * m + dm * u;
+        return 0;
+    }
+    if (kind == 21) { /* LJF (:924-945), params ea, sig^2, rc^2 */
+        double r2 = r * r;
+        *e = 0.0; *g = 0.0;
+        if (!(r2 > p[2])) {
+            double ir = 1.0 / r2, st = p[1] * ir, rct = p[2] * ir, x = p[0] * pow(rct - 1.0, 2);
+            *e = x * (st - 1.0);
+            *g = 4.0 * p[0] * rct * (rct - 1.0) * (st - 1.0) + 2.0 * x * st;
+        }
+        return 0;
+    }
+    if (kind == 24) { /* Stillinger-Weber two-body part (:1058-1082), params eps, A, B, sig, p, q, aa */
+        double ee = p[3] / (r - p[6] * p[3]);
+        *e = 0.0; *g = 0.0;
+        if (r < p[6] * p[3]) {
+            double p_r = p[3] / r, c = p[2] * pow(p_r, p[4]), exp_e = exp(ee);
+            double t = p[1] * p[0] * (c - pow(p_r, p[5])) * exp_e;
+            *e = t;
+            *g = p[1] * p[0] * (p[4] * c - p[5] * pow(p_r, p[5])) * exp_e + t * r * ee / (r - p[6] * p[3]);
+        }
+        return 0;
+    }
     if (kind == 23) { /* nDPD (:623-642), params a, b, n, rc */
         *e = 0.0; *g = 0.0;
         if (r < p[3]) {
@@ -229,12 +293,22 @@ int orc_vdw_eval(int kind, const double *p, double r, double *e, double *g)
 
 /* vdw.F90:1397-1576: tables at r_i = i*dlrpot, i = 1..max_grid,
  * dlrpot = rvdw/(max_grid-4); index 0 holds Huge. Arrays have max_grid+1 slots. */
+/* vdw.F90:205-230 (potential_cutoffs): the damped forms take the vdW cutoff
+ * as the damping's outer radius; q gets the key's 7 parameters with it */
+void orc_vdw_params(int kind, const double *p, double rvdw, double *q)
+{
+    memcpy(q, p, 7 * sizeof(double));
+    if (kind == 18 || kind == 20) q[3] = rvdw;
+    if (kind == 19) q[4] = rvdw;
+}
+
 int orc_vdw_table(int kind, const double *p, double rvdw, int max_grid, double *pot, double *frc)
 {
-    double dlrpot = rvdw / (double)(max_grid - 4);
+    double dlrpot = rvdw / (double)(max_grid - 4), q[7];
+    orc_vdw_params(kind, p, rvdw, q);
     for (int i = 1; i <= max_grid; i++) {
         double r = (double)i * dlrpot, e, g;
-        if (orc_vdw_eval(kind, p, r, &e, &g)) return -1;
+        if (orc_vdw_eval(kind, q, r, &e, &g)) return -1;
         pot[i] = e;
         frc[i] = g;
     }
@@ -316,7 +390,29 @@ static double orc_mm3_int(double r0, double A, double B, double eps, double rw, 
     return deriv ? s * h * pow(r0, 4) / 3.0 : s * h * pow(r0, 3) / 3.0;
 }
 
-void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*5 per key*/,
+/* vdw.F90:288-520: intRadZBL / intdRadZBL / intRadMDF / intdRadMDF, the same
+ * chunked Simpson rule in r (h = rw/n from rw, weight r^2) over U (deriv 0)
+ * or gamma (deriv 1) of a form with completed parameters q */
+static double orc_rad_int(int kind, const double *q, double rw, double prec, int deriv)
+{
+    int n = 10000, j = 1;
+    double is = rw, ie = 2 * is, h = (ie - is) / (double)n, s = 0.0, sold = HUGE_VAL;
+    while (fabs(s - sold) * h / 3.0 > prec && j <= 100000) {
+        sold = s;
+        for (int i = (j - 1) * n; i <= j * n; i += 2) {
+            double x = is + i * h, e0, g0, e1, g1, e2, g2;
+            orc_vdw_eval(kind, q, x, &e0, &g0);
+            orc_vdw_eval(kind, q, x + h, &e1, &g1);
+            orc_vdw_eval(kind, q, x + 2.0 * h, &e2, &g2);
+            s += x * x * (deriv ? g0 : e0) + 4.0 * pow(x + h, 2) * (deriv ? g1 : e1) +
+                 pow(x + 2.0 * h, 2) * (deriv ? g2 : e2);
+        }
+        j++;
+    }
+    return s * h / 3.0;
+}
+
+void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double *params /*7 per key*/,
                  double rvdw, double volm, double *elrc, double *vlrc, const double *num_frz /* NULL: none */)
 {
     const double twopi = 2.0 * ORC_PI;
@@ -325,7 +421,7 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
     for (int i = 1; i <= ntypes; i++) {
         for (int j = 1; j <= i; j++) {
             double eadd = 0.0, padd = 0.0, r = rvdw;
-            const double *p = params + 5 * ivdw;
+            const double *p = params + 7 * ivdw;
             int kd = kinds[ivdw];
             if (kd == 1) { /* vdw.F90:697-707 */
                 eadd = p[0] / (9.0 * pow(r, 9)) - p[1] / (3.0 * pow(r, 3));
@@ -349,8 +445,10 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
             } else if (kd == 6) { /* :755-764 */
                 eadd = p[0] / (9.0 * pow(r, 9)) - p[1] / (7.0 * pow(r, 7));
                 padd = 12.0 * p[0] / (9.0 * pow(r, 9)) - 10.0 * p[1] / (7.0 * pow(r, 7));
-            } else if ((kd == 8 || kd == 13) && p[2] > 1.17549435e-38) { /* :766-782, 810-830 */
-                double e0 = p[0], r0 = p[1], kk = p[2], t = exp(-kk * (r - r0)), k3 = kk * kk * kk;
+            } else if ((kd == 8 || kd == 13 || kd == 16) && p[kd == 16 ? 6 : 2] > ZERO_PLUS) {
+                /* :766-782, 810-830; :859-876 (ZBL-Morse: the Morse tail of p[4..6]) */
+                const double *pm = kd == 16 ? p + 4 : p;
+                double e0 = pm[0], r0 = pm[1], kk = pm[2], t = exp(-kk * (r - r0)), k3 = kk * kk * kk;
                 eadd = -2.0 * e0 * t / k3 * (pow(kk * r + 1, 2) + 1) + e0 * t * t / (4.0 * k3) * (pow(kk * r + 1, 2) + kk * kk * r * r);
                 padd = -2.0 * e0 * t / k3 * (pow(kk, 3) * pow(r, 3) + 3 * kk * kk * r * r + 6 * kk * r + 6) +
                        e0 * t * t / (4.0 * k3) * (4.0 * pow(kk, 3) * pow(r, 3) + 6 * kk * kk * r * r + 6 * kk * r + 3);
@@ -367,6 +465,28 @@ void orc_vdw_lrc(int ntypes, const double *num_type, const int *kinds, const dou
                         6 * b * pow(c, 4) + 2 * a * pow(c, 3)) * t;
                 padd = (b * pow(r, 4) + (3 * b * c + a) * pow(r, 3) + (9 * b * c * c + 3 * a * c) * r * r +
                         (18 * b * pow(c, 3) + 6 * a * c * c) * r + 18 * b * pow(c, 4) + 6 * a * pow(c, 3)) * t;
+            }
+            else if (kd == 15) {
+                /* :846-857: kk = Z1 Z2 r4pie0 and the inverse screening length go
+                 * to intRadZBL, which stores them as the Z1, Z2 of its ZBL */
+                double q[7] = {0}, z1 = p[0], z2 = p[1];
+                q[0] = z1 * z2 * R4PIE0;
+                q[1] = (pow(z1, 0.23) + pow(z2, 0.23)) / (0.52917721067 * 0.88534);
+                eadd = orc_rad_int(15, q, r, 1e-12, 0);
+                padd = orc_rad_int(15, q, r, 1e-12, 1);
+            } else if (kd == 17) { /* :878-890 */
+                double A = p[4], r0 = p[5], c = p[6], t = A * exp(-r / r0);
+                eadd = (r * r + 2 * r0 * r + 2 * r0 * r0) * t * r0 - c / (3.0 * pow(r, 3));
+                padd = (pow(r, 3) + 3 * r0 * r * r + 6 * r0 * r0 * r + 6 * pow(r0, 3)) * t - 2.0 * c / pow(r, 3);
+            } else if (kd >= 18 && kd <= 20) {
+                /* :892-921: intRadMDF(pot, a, b, c, ri, rw) with rc = rw; "mlj"
+                 * sets LJ sigma = a and epsilon = b (:403-407), a = param(1) */
+                double q[7] = {0};
+                if (kd == 18) { q[0] = p[1]; q[1] = p[0]; q[2] = p[2]; q[3] = r; }
+                if (kd == 19) { q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[3]; q[4] = r; }
+                if (kd == 20) { q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = r; }
+                eadd = orc_rad_int(kd, q, r, 1e-12, 0);
+                padd = orc_rad_int(kd, q, r, 1e-12, 1);
             }
             if (i != j) { eadd *= 2.0; padd *= 2.0; }
             /* vdw.F90:953: frozen-frozen pairs carry no correction */
@@ -659,7 +779,7 @@ typedef struct {
     int vdw_fs;              /* vdW force-shifting (vdws%l_force_shift) */
     int vdw_direct;          /* vdw_method direct: analytic forms per pair (vdws%l_direct) */
     const int *vdw_kind;     /* per key, for vdw_direct */
-    const double *vdw_param; /* 5 per key, for vdw_direct */
+    const double *vdw_param; /* 7 per key, for vdw_direct */
     const int32_t *excl_off; /* excluded pairs (neigh%list_excl) as CSR over atoms: N+1 row starts, */
     const int32_t *excl_ids; /* then 0-based partners, both ends listed; NULL = none */
     const int32_t *frozen;   /* per atom, nonzero = frozen (config%lfrzn); NULL = none */
@@ -678,6 +798,8 @@ typedef struct {
     /* per-particle energies (N) and stresses (9N, column-major 3x3 per atom)
      * of stats%collect_pp (NULL to skip): overwritten */
     double *pp_energy, *pp_stress;
+    double engvdw_abs;       /* sum of |vdW pair energy|: the scale of engsrp's rounding */
+    double virvdw_abs;       /* sum of |vdW pair virial|: the scale of virsrp's rounding */
 } orc_output;
 
 static int excluded(const orc_input *in, int i, int j)
@@ -996,7 +1118,7 @@ int orc_two_body(const orc_input *in, orc_output *out)
     int32_t *pairs = malloc(sizeof(int32_t) * 5 * (cap > 0 ? cap : 1));
     orc_pairs_cells(n, c, in->xyz, rcut_ext, pairs, cap);
     out->npairs = np;
-    double engvdw = 0.0, virvdw = 0.0, engrl = 0.0, virrl = 0.0;
+    double engvdw = 0.0, virvdw = 0.0, engrl = 0.0, virrl = 0.0, engvdw_abs = 0.0, virvdw_abs = 0.0;
     double st_v[6] = {0}, st_c[6] = {0};
     double dlrpot = in->rvdw / (double)(in->max_grid - 4), rdr = 1.0 / dlrpot;
     double dr_e = in->rcut / (double)(in->nsamples - 4), rdr_e = 1.0 / dr_e;
@@ -1016,9 +1138,10 @@ int orc_two_body(const orc_input *in, orc_output *out)
             int k = key_of(in->ltype[i], in->ltype[j]);
             int kind = in->vdw_kind[k];
             if (kind > 0) {
-                const double *pk = in->vdw_param + 5 * (long)k;
-                double afs = 0.0, bfs = 0.0, e, g;
-                if (in->vdw_fs && kind != 7 && kind != 10 && kind != 23) { /* vdw.F90:1015-1030 */
+                double pk[7], afs = 0.0, bfs = 0.0, e, g;
+                orc_vdw_params(kind, in->vdw_param + 7 * (long)k, in->rvdw, pk);
+                /* vdw.F90:1015-1040: none for n-m shift, DPD, nDPD, LJF, 12-6-MDF */
+                if (in->vdw_fs && kind != 7 && kind != 10 && kind != 23 && kind != 21 && kind != 20) {
                     double z, dz;
                     orc_vdw_eval(kind, pk, in->rvdw, &z, &dz);
                     afs = dz / in->rvdw;
@@ -1032,7 +1155,9 @@ int orc_two_body(const orc_input *in, orc_output *out)
                 f[3 * i] += fx; f[3 * i + 1] += fy; f[3 * i + 2] += fz;
                 f[3 * j] -= fx; f[3 * j + 1] -= fy; f[3 * j + 2] -= fz;
                 engvdw += eng;
+                engvdw_abs += fabs(eng);
                 virvdw -= gamma * rsq;
+                virvdw_abs += fabs(gamma * rsq);
                 st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
                 st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
                 pp_pair(out, i, j, eng, eng, xx, yy, zz, fx, fy, fz);
@@ -1061,7 +1186,9 @@ int orc_two_body(const orc_input *in, orc_output *out)
             /* vdw.F90:1961-1964 */
             if (in->vdw_fs) eng = eng + tf[in->max_grid - 4] * (rrr * r_rvdw - 1.0) - tp[in->max_grid - 4];
             engvdw += eng;
+            engvdw_abs += fabs(eng);
             virvdw -= gamma * rsq;
+            virvdw_abs += fabs(gamma * rsq);
             st_v[0] += xx * fx; st_v[1] += xx * fy; st_v[2] += xx * fz;
             st_v[3] += yy * fy; st_v[4] += yy * fz; st_v[5] += zz * fz;
             const int cross = pairs[5 * p + 2] != 0 || pairs[5 * p + 3] != 0 || pairs[5 * p + 4] != 0;
@@ -1130,6 +1257,8 @@ int orc_two_body(const orc_input *in, orc_output *out)
     }
     out->eng_nz = engcpe_nz;
     out->buffer[7] = engvdw; out->buffer[8] = virvdw;
+    out->engvdw_abs = engvdw_abs;
+    out->virvdw_abs = virvdw_abs;
     out->buffer[9] = engcpe_rc; out->buffer[10] = vircpe_rc;
     out->buffer[11] = engrl; out->buffer[12] = virrl;
     out->buffer[15] = engcpe_ex; out->buffer[16] = vircpe_ex;

@@ -66,7 +66,8 @@ class OrcOutput(C.Structure):
                 ("eng_recip", C.c_double), ("eng_nz", C.c_double), ("recip_net", C.c_double * 3),
                 ("npairs", C.c_long), ("charge_grid", C.POINTER(C.c_double)),
                 ("potential_grid", C.POINTER(C.c_double)), ("pp_energy", C.POINTER(C.c_double)),
-                ("pp_stress", C.POINTER(C.c_double))]
+                ("pp_stress", C.POINTER(C.c_double)), ("engvdw_abs", C.c_double),
+                ("virvdw_abs", C.c_double)]
 
 
 def nint(x):
@@ -88,7 +89,7 @@ def vdw_tables(sysm, max_grid=None):
         a, b = ti + 1, tj + 1
         hi, lo = max(a, b), min(a, b)
         k = hi * (hi - 1) // 2 + lo - 1
-        p = np.zeros(5); p[:len(params)] = params
+        p = np.zeros(7); p[:len(params)] = params
         pp = np.ascontiguousarray(p)
         rp = np.zeros(mg + 1); rf = np.zeros(mg + 1)
         if lib().orc_vdw_table(C.c_int(kind), _p(pp), C.c_double(sysm.vdw_cutoff), C.c_int(mg), _p(rp), _p(rf)):
@@ -107,7 +108,7 @@ def erfc_tables(sysm):
 
 def vdw_lrc(sysm, from_key):
     nkey = sysm.ntypes * (sysm.ntypes + 1) // 2
-    kinds = np.zeros(nkey, np.int32); params = np.zeros((nkey, 5))
+    kinds = np.zeros(nkey, np.int32); params = np.zeros((nkey, 7))
     for k, (kind, p) in from_key.items():
         kinds[k] = kind; params[k] = p
     num = np.bincount(sysm.types, minlength=sysm.ntypes).astype(np.float64)
@@ -151,7 +152,7 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None, p
     fs = bool(getattr(sysm, "vdw_force_shift", False))
     direct = bool(getattr(sysm, "vdw_direct", False))
     nkey = sysm.ntypes * (sysm.ntypes + 1) // 2
-    kinds = np.zeros(nkey, np.int32); params = np.zeros((nkey, 5))
+    kinds = np.zeros(nkey, np.int32); params = np.zeros((nkey, 7))
     for k, (kind, p) in from_key.items():
         kinds[k] = kind; params[k] = p
     # vdw.F90:662: no long-range correction with force-shifting
@@ -195,7 +196,8 @@ def two_body(sysm, pos=None, grids=False, exact_erfc=False, table_points=None, p
     res = dict(frc=f, buffer=np.array(out.buffer[:]), stress=np.array(out.stress[:]),
                engcpe=out.engcpe, vircpe=out.vircpe, engsrp=out.engsrp, virsrp=out.virsrp,
                eng_self=out.eng_self, eng_recip=out.eng_recip, eng_nz=out.eng_nz,
-               recip_net=np.array(out.recip_net[:]), npairs=out.npairs, elrc=elrc, vlrc=vlrc)
+               recip_net=np.array(out.recip_net[:]), npairs=out.npairs, elrc=elrc, vlrc=vlrc,
+               engvdw_abs=out.engvdw_abs, virvdw_abs=out.virvdw_abs)
     if grids and sysm.electrostatics:
         res["Q"] = Q.reshape(K[2], K[1], K[0]); res["Phi"] = P.reshape(K[2], K[1], K[0])
     if per_particle:

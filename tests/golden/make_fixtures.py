@@ -81,6 +81,16 @@ def _mix3(sysm):
     return sysm
 
 
+def _mix4(sysm):
+    sysm.name = sysm.name.replace("mix", "mix4")
+    return sysm
+
+
+def _mix5(sysm):
+    sysm.name = sysm.name.replace("mix", "mix5")
+    return sysm
+
+
 def cases():
     return {
         # 1000 ions, cubic, K=32, S=8: the reference settings at a small size
@@ -121,6 +131,17 @@ def cases():
         "mix3_4000_fs": (_fs(_mix3(systems.mixture(vdw=systems.MIX3_VDW))), 0),
         "mix3_4000_direct": (_direct(_mix3(systems.mixture(vdw=systems.MIX3_VDW))), 0),
         "mix3_4000_direct_fs": (_direct(_fs(_mix3(systems.mixture(vdw=systems.MIX3_VDW)))), 0),
+        # ZBL, ZBL-Morse, ZBL-Buckingham, LJF, Stillinger-Weber: tabulated and
+        # direct, with and without force-shifting
+        "mix4_4000_k36": (_mix4(systems.mixture(vdw=systems.MIX4_VDW)), 0),
+        "mix4_4000_fs": (_fs(_mix4(systems.mixture(vdw=systems.MIX4_VDW))), 0),
+        "mix4_4000_direct": (_direct(_mix4(systems.mixture(vdw=systems.MIX4_VDW))), 0),
+        "mix4_4000_direct_fs": (_direct(_fs(_mix4(systems.mixture(vdw=systems.MIX4_VDW)))), 0),
+        # LJ-MDF, Buckingham-MDF, 12-6-MDF, LJF beyond the cutoff, ZBL
+        "mix5_4000_k36": (_mix5(systems.mixture(vdw=systems.MIX5_VDW)), 0),
+        "mix5_4000_fs": (_fs(_mix5(systems.mixture(vdw=systems.MIX5_VDW))), 0),
+        "mix5_4000_direct": (_direct(_mix5(systems.mixture(vdw=systems.MIX5_VDW))), 0),
+        "mix5_4000_direct_fs": (_direct(_fs(_mix5(systems.mixture(vdw=systems.MIX5_VDW)))), 0),
         # polar triatomic molecules whose bonds and angle (zero-strength
         # harmonic terms) exclude every intramolecular pair: Ewald exclusion
         # correction (ewald_spole.F90:479-679)
@@ -154,7 +175,7 @@ def run_case(name, sysm, steps, statis_only=False):
         rv = fileio.read_revive(os.path.join(d, "REVIVE"))
     vdw_kind = np.array([v[2] for v in sysm.vdw], np.int32)
     vdw_types = np.array([[v[0], v[1]] for v in sysm.vdw], np.int32)
-    vdw_params = np.zeros((len(sysm.vdw), 5))
+    vdw_params = np.zeros((len(sysm.vdw), 7))
     for i, v in enumerate(sysm.vdw):
         vdw_params[i, :len(v[3])] = v[3]
     np.savez_compressed(

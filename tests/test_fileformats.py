@@ -72,6 +72,33 @@ def test_field_reader_units_and_repeats():
     assert lj2 == ("C", "C", 2, (100.0, 3.4))
 
 
+def test_field_reader_converts_the_zbl_mdf_ljf_and_sw_forms():
+    """ffield.F90:3760-3839 in kJ/mol: parpot(1) scaled except for the ZBL
+    forms (a charge); zbls parpot(5), zblb parpot(5) and (7), mbuc (1) and
+    (3), m126 (1) and (2), sw (1); LJF's eps times x/4 (3/(x-1))^3 with
+    x = (rc/sig)^2, then sig^2 and rc^2."""
+    recs = ["A  C  zbl  14.0 8.0", "B  C  zbls 14.0 8.0 1.5 0.2 2.0 3.0 1.5",
+            "A  B  zblb 14.0 8.0 1.5 0.2 3.0 0.3 4.0", "A  A  mlj  0.5 3.0 6.0", "B  A  mbuc 3.0 0.3 4.0 6.5",
+            "C  B  m126 5.0 6.0 7.0", "C  A  ljf  2.0 2.0 5.0", "B  B  sw   2.0 7.0 0.6 2.1 4.0 0.0 1.8"]
+    head = FIELD_KJ[:FIELD_KJ.index("vdw 3")]
+    text = head + "vdw %d\n" % len(recs) + "\n".join(recs) + "\nclose\n"
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "FIELD")
+        open(p, "w").write(text)
+        f = fileio.read_field(p)
+    got = {(a, b): (k, prm) for (a, b, k, prm) in f["vdw"]}
+    assert got[("A", "C")] == (15, (14.0, 8.0))
+    assert got[("B", "C")] == (16, (14.0, 8.0, 1.5, 0.2, 200.0, 3.0, 1.5))
+    assert got[("A", "B")] == (17, (14.0, 8.0, 1.5, 0.2, 300.0, 0.3, 400.0))
+    assert got[("A", "A")] == (18, (50.0, 3.0, 6.0))
+    assert got[("B", "A")] == (19, (300.0, 0.3, 400.0, 6.5))
+    assert got[("C", "B")] == (20, (500.0, 600.0, 7.0))
+    x = (5.0 / 2.0) ** 2
+    k, (ea, s2, rc2) = got[("C", "A")]
+    assert k == 21 and (s2, rc2) == (4.0, 25.0) and abs(ea - 200.0 * 0.25 * x * (3.0 / (x - 1.0)) ** 3) <= 1e-13 * ea
+    assert got[("B", "B")] == (24, (200.0, 7.0, 0.6, 2.1, 4.0, 0.0, 1.8))
+
+
 def test_field_reader_rejects_buckingham_with_zero_rho():
     """ffield.F90:3745-3751: rho = 0 with A nonzero is DL_POLY's error 467."""
     text = FIELD_KJ.replace("C  C  lj   1.0   3.4", "C  C  buck 2.0 0.0 5.0")
@@ -89,7 +116,7 @@ def test_field_reader_rejects_what_the_path_lacks(bad):
     if bad:
         text = text.replace("finish\nion", bad + "finish\nion", 1)
     else:
-        text = text.replace("C  C  lj   1.0   3.4", "C  C  zbl  1.0 3.4 0.0")
+        text = text.replace("C  C  lj   1.0   3.4", "C  C  tab")  # TABLE-file potentials
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "FIELD")
         open(p, "w").write(text)

@@ -33,24 +33,31 @@ def gpu_two_body(s, precision=lib.FP64, pos=None, **kw):
     return ctx, r
 
 
-def coulomb_scales(o):
-    """The largest term of engcpe and of vircpe in an oracle result: engcpe is
-    the sum of the self, reciprocal, real-space and exclusion-correction
-    energies, which cancel where the charges are weak (a mixture: +5.5e6
-    reciprocal, -6.6e6 self, -1.1e6 total) or the molecules are polar (2e7
-    terms, 7e4 total); the rounding of the terms is what bounds the total.
-    The same terms bound vircpe: the real-space Coulomb virial is the
-    real-space stress trace less the vdW virial, and with exclusions the
+def energy_scales(o, mixed=False):
+    """The largest term of engcpe, vircpe, engsrp and virsrp in an oracle
+    result: engcpe is the sum of the self, reciprocal, real-space and
+    exclusion-correction energies, which cancel where the charges are weak (a
+    mixture: +5.5e6 reciprocal, -6.6e6 self, -1.1e6 total) or the molecules
+    are polar (2e7 terms, 7e4 total); the rounding of the terms is what bounds
+    the total.  The same terms bound vircpe: the real-space Coulomb virial is
+    the real-space stress trace less the vdW virial, and with exclusions the
     reciprocal sum carries each molecule's own smooth interaction (the size of
-    the self and correction energies) until the FP64 correction removes it."""
+    the self and correction energies) until the FP64 correction removes it.
+    engsrp and virsrp are likewise sums of pair terms of both signs (repulsive
+    ZBL pairs beside attractive ones, a molecule's repulsive and attractive
+    neighbours): their rounding scales with the summed magnitudes, of which
+    1e-2 (FP64: summation order) or all (mixed: every pair term carries its
+    own FP32 rounding) bound the error with the check's relative tolerance."""
     b = o["buffer"]
     eng = max(abs(o["eng_recip"]), abs(o["eng_self"]), abs(b[11]), abs(b[15]))
-    return dict(engcpe=eng, vircpe=max(eng, abs(b[8]), abs(b[10]), abs(b[12]), abs(b[16])))
+    f = 1.0 if mixed else 1e-2
+    return dict(engcpe=eng, vircpe=max(eng, abs(b[8]), abs(b[10]), abs(b[12]), abs(b[16])),
+                engsrp=f * o["engvdw_abs"], virsrp=f * o["virvdw_abs"])
 
 
 def check_energies(r, ref, tol, scales=None):
     """Energies and virials within tol relative to the reference value, or to
-    scales[k] where that is larger (see coulomb_scales)."""
+    scales[k] where that is larger (see energy_scales)."""
     for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
         scale = max(abs(ref[k]), 1.0, (scales or {}).get(k, 0.0))
         assert abs(r[k] - ref[k]) <= tol * scale, (k, r[k], ref[k])
@@ -64,13 +71,13 @@ def test_fp64_matches_oracle_and_reference(name):
     ctx, r = gpu_two_body(s)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-10
     assert np.abs(r["frc"] - o["frc"]).max() <= 1e-10 * np.abs(o["frc"]).max()
-    check_energies(r, o, 1e-11, coulomb_scales(o))
+    check_energies(r, o, 1e-11, energy_scales(o))
     assert np.abs(r["stress"] - o["stress"]).max() <= 1e-11 * np.abs(o["stress"]).max()
     assert np.allclose(r["buffer"], o["buffer"], rtol=1e-11, atol=1e-9)
     # and the reference itself: all printed force digits, full-precision energies
     bound = fixtures.printed_digits_bound(ref["frc"]) + 1e-12 * np.abs(ref["frc"]).max()
     assert (np.abs(fixtures.frozen_zeroed(s, r["frc"]) - ref["frc"]) / bound).max() <= 1.05
-    check_energies(r, ref, 1e-11, coulomb_scales(o))
+    check_energies(r, ref, 1e-11, energy_scales(o))
     # the reference's own forces at full precision (its two_body_forces output)
     assert rms_rel(r["frc"], ref["frc64"]) <= 1e-12
     assert np.abs(r["frc"] - ref["frc64"]).max() <= 1e-11 * np.abs(ref["frc64"]).max()
@@ -88,8 +95,9 @@ def test_reference_vdw_tables_through_set_vdw_tables(name):
     print("case", name, flush=True)
     mg, pot, frc, ltp = fixtures.reference_tables(s, ref)
     tables = dict(max_grid=mg, vdw_pot=pot, vdw_frc=frc, ltp=ltp)
-    scales = coulomb_scales(oracle.two_body(s)) if s.electrostatics else None
+    o = oracle.two_body(s)
     for precision, tf in ((lib.FP64, 1e-12), (lib.MIXED, 1e-5)):
+        scales = energy_scales(o, mixed=precision == lib.MIXED)
         ctx = lib.Context(s, precision=precision, tables=tables, lrc=(ref["elrc"], ref["vlrc"]))
         if s.electrostatics:
             ctx._chk(ctx.lib.dlpx_generate_erfc_tables(ctx.h))
@@ -153,7 +161,7 @@ def test_mixed_precision_error_budget(name):
     o = oracle.two_body(s)
     ctx, r = gpu_two_body(s, precision=lib.MIXED)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-5
-    check_energies(r, o, 1e-6, coulomb_scales(o))
+    check_energies(r, o, 1e-6, energy_scales(o, mixed=True))
     ctx.close()
 
 
@@ -219,7 +227,7 @@ def test_tiny_systems_match_oracle(n_pairs):
     o = oracle.two_body(s)
     ctx, r = gpu_two_body(s)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-10
-    check_energies(r, o, 1e-11, coulomb_scales(o))
+    check_energies(r, o, 1e-11, energy_scales(o))
     g = ctx.get_pairs()
     op = oracle.pairs(s.cell, s.pos, s.cutoff + s.padding, "brute")
     assert {tuple(p) for p in g} == {tuple(p) for p in op}
@@ -262,7 +270,7 @@ def test_many_types_read_vdw_tables_from_global_memory():
     o = oracle.two_body(s)
     ctx, r = gpu_two_body(s)
     assert rms_rel(r["frc"], o["frc"]) <= 1e-10
-    check_energies(r, o, 1e-11, coulomb_scales(o))
+    check_energies(r, o, 1e-11, energy_scales(o))
     ctx.close()
     ctx, rm = gpu_two_body(s, precision=lib.MIXED)
     assert rms_rel(rm["frc"], o["frc"]) <= 1e-5
@@ -363,7 +371,7 @@ def test_dlpoly_inputs_run_to_the_reference_statis(name, precision):
     if precision == lib.MIXED:
         # the mixed budget of the Coulomb energy and virial, carried into
         # every column that contains them (statistics.F90:1096-1098,1118-1160)
-        sc = coulomb_scales(oracle.two_body(s))
+        sc = energy_scales(oracle.two_body(s), mixed=True)
         de, dv = 1e-6 * sc["engcpe"], 1e-6 * sc["vircpe"]
         col = {k: v - 1 for k, v in fileio.STATIS_COLUMNS.items()}
         budget = {col["eng_cou"]: de, col["eng_cfg"]: de, col["eng_tot"]: de, col["vir_cou"]: dv,
@@ -428,7 +436,7 @@ def test_full_size_nacl_fp64_and_mixed(nacl64k):
     ctx.close()
     ctx, rm = gpu_two_body(s, precision=lib.MIXED)
     assert rms_rel(rm["frc"], o["frc"]) <= 1e-5
-    check_energies(rm, o, 1e-6, coulomb_scales(o))
+    check_energies(rm, o, 1e-6, energy_scales(o, mixed=True))
     # net force vanishes (pair forces antisymmetric, mean reciprocal force removed)
     assert np.abs(r["frc"].sum(0)).max() <= 1e-8 * np.abs(r["frc"]).max() * np.sqrt(s.natms)
     ctx.close()
@@ -609,7 +617,7 @@ def test_ensemble256_sampled_systems_match_oracle():
             err = rms_rel(f, o["frc"])
             assert err <= tf, (k, syss[k].natms, err)
             last = dict(zip(("engcpe", "vircpe", "engsrp", "virsrp"), e[k, -1, :4]))
-            check_energies(last, o, te, coulomb_scales(o) if syss[k].electrostatics else None)
+            check_energies(last, o, te, energy_scales(o, mixed=precision == lib.MIXED))
 
 
 def test_nve_energy_conservation_fp64_and_mixed():

@@ -31,9 +31,13 @@ def test_oracle_matches_reference_fixture(name):
     assert _force_margin(fixtures.frozen_zeroed(s, r["frc"]), ref["frc"]) <= 1.01
     # energies and virials from the binary REVIVE record (full precision),
     # relative to the largest term of the sum: with exclusions engcpe is a
-    # small difference of the self, reciprocal and correction energies
+    # small difference of the self, reciprocal and correction energies, and
+    # engsrp can be a small sum of large pair energies of both signs (the
+    # ZBL forms' repulsion against attractive pairs); 1e-14 of the summed
+    # magnitudes covers the rounding of a different summation order
     b = r["buffer"]
-    comps = dict(engcpe=(r["eng_self"], r["eng_recip"], b[11], b[15]), vircpe=(b[10], b[12], b[16]))
+    comps = dict(engcpe=(r["eng_self"], r["eng_recip"], b[11], b[15]), vircpe=(b[10], b[12], b[16]),
+                 engsrp=(1e-2 * r["engvdw_abs"],))
     for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
         scale = max([abs(ref[k]), 1.0] + [abs(x) for x in comps.get(k, ())])
         assert abs(r[k] - ref[k]) <= 1e-12 * scale, k
@@ -52,7 +56,8 @@ def test_oracle_matches_reference_full_precision(name):
     assert float(np.sqrt(((f - f64) ** 2).sum() / (f64 ** 2).sum())) <= 1e-12
     assert np.abs(f - f64).max() <= 1e-11 * np.abs(f64).max()
     b = r["buffer"]
-    comps = dict(engcpe=(r["eng_self"], r["eng_recip"], b[11], b[15]), vircpe=(b[10], b[12], b[16]))
+    comps = dict(engcpe=(r["eng_self"], r["eng_recip"], b[11], b[15]), vircpe=(b[10], b[12], b[16]),
+                 engsrp=(1e-2 * r["engvdw_abs"],))
     for k in ("engsrp", "engcpe", "virsrp", "vircpe"):
         scale = max([abs(ref[k]), 1.0] + [abs(x) for x in comps.get(k, ())])
         assert abs(r[k] - ref["dump_" + k]) <= 1e-12 * scale, k
@@ -98,29 +103,41 @@ def test_oracle_nve_matches_reference_trajectory(name):
         assert abs(r[k] - ref[k]) <= 1e-11 * abs(ref[k]), k
 
 
-def _eval(kind, p, r):
+def _eval(kind, p, r, rvdw=None):
+    """U and gamma of one form; with rvdw, the parameters are first completed
+    as potential_cutoffs does (the damped forms' outer radius)."""
     import ctypes as C
     e, g = C.c_double(), C.c_double()
-    pp = np.ascontiguousarray(np.asarray(p, np.float64))
+    pp = np.zeros(7)
+    pp[:len(p)] = p
+    if rvdw is not None:
+        q = np.zeros(7)
+        oracle.lib().orc_vdw_params(C.c_int(kind), pp.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(rvdw),
+                                    q.ctypes.data_as(C.POINTER(C.c_double)))
+        pp = q
     oracle.lib().orc_vdw_eval(C.c_int(kind), pp.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(r),
                               C.byref(e), C.byref(g))
     return e.value, g.value
 
 
-# test_vdw.F90:40-63: energy and virial (-gamma) of each form at r = 1 A with
-# parameters 1, 2, 3, ... (index = vdW key)
+# test_vdw.F90:46-59: energy and virial (-gamma) of each form at r = 1 A with
+# parameters 1, 2, ..., 7 and a 10 A cutoff (index = vdW key)
 VDW_KNOWN = {1: (-1.0, 0.0), 2: (16128.0, -195072.0), 3: (80.0, -288.0),
              4: (-2.3934693402873668, 17.696734670143684), 5: (45.598150033144236, -45.196300066288472),
              6: (-1.0, 8.0), 8: (363.25771964635982, -2300.0595394172847), 9: (1.0, 12.0), 10: (0.25, -0.5),
              12: (15616.0, -192000.0), 13: (367.25771964635982, -2348.0595394172847),
              14: (2.1495939317213679, 0.71653131057378916), 22: (-0.8948393168143698, -0.1988531815143044),
              23: (-0.4921875, -0.09375), 7: (4567.8016528926664, -16899.173553719396),
-             11: (555.50928442334305, -3719.0014773362891)}
+             11: (555.50928442334305, -3719.0014773362891),
+             15: (16882.652704957309, -50044.066263312896), 16: (3.8142266913481757E+030, -5.2445617006037446E+031),
+             17: (11761.890149027380, -36135.103419546060), 18: (16128.0, -195072.0),
+             19: (-2.3934693402873668, 17.696734670143684), 20: (-1.0, 0.0), 21: (4.0, -40.0),
+             24: (-1765.997274794514, 15903.665444480088)}
 
 
 @pytest.mark.parametrize("kind", sorted(VDW_KNOWN))
 def test_vdw_known_answers(kind):
-    e, g = _eval(kind, [1, 2, 3, 4, 5], 1.0)
+    e, g = _eval(kind, [1, 2, 3, 4, 5, 6, 7], 1.0, rvdw=10.0)
     ee, vv = VDW_KNOWN[kind]
     assert abs(e - ee) <= 1e-12 * max(abs(ee), 1.0) and abs(-g - vv) <= 1e-12 * max(abs(vv), 1.0)
 
