@@ -353,6 +353,13 @@ static int fetch_red(Ctx *c)
 
 // What a captured step graph bakes in (buffers, sizes, launch shapes, the
 // time step): a change of any of them forces a new capture.
+static uint64_t bits(double x)
+{
+    uint64_t u;
+    memcpy(&u, &x, sizeof(u));
+    return u;
+}
+
 std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
 {
     uint64_t t;
@@ -364,7 +371,9 @@ std::vector<uint64_t> step_graph_key(const Ctx *c, double tstep)
             (uint64_t)c->d_keytab, (uint64_t)c->d_vdw_direct, (uint64_t)c->cap_lane, (uint64_t)c->ncell,  (uint64_t)c->natms,
             (uint64_t)c->nc[0],    (uint64_t)c->nc[1],   (uint64_t)c->nc[2],   (uint64_t)c->gid,
             (uint64_t)c->slot,     (uint64_t)c->excl_off, (uint64_t)c->excl_ids, (uint64_t)c->excl_total,
-            (uint64_t)c->frz,      (uint64_t)c->nfrozen, t};
+            (uint64_t)c->frz,      (uint64_t)c->nfrozen, t, (uint64_t)c->theta,
+            // the cell: pair, spreading and influence arguments carry it
+            bits(c->cell[0]), bits(c->cell[4]), bits(c->cell[8])};
 }
 
 // Start of a resident run: clear the counters of the fused stage-1 kernel
@@ -498,6 +507,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     if (const char *e = std::getenv("DLPX_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
     // the list build bins by half-edge subcells unless DLPX_LIST_SUBCELLS=0
     if (const char *e = std::getenv("DLPX_LIST_SUBCELLS")) c->list_subcells = std::atoi(e) != 0;
+    // DLPX_FFT_LINES=0: the generic Stockham FFT kernels for every length
+    if (const char *e = std::getenv("DLPX_FFT_LINES")) c->fft_lines = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
@@ -544,7 +555,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
                    c->d_vdw_direct, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
-                   c->Q, c->Phi, c->H, c->tw, c->bsp_n2, c->red, c->red_out, c->acc, c->energies_dev,
+                   c->Q, c->Phi, c->H, c->theta, c->tw, c->bsp_n2, c->red, c->red_out, c->acc, c->energies_dev,
                    c->snap_d, c->snap_i};
     for (void *p : dev)
         if (p) (void)hipFree(p);

@@ -76,9 +76,9 @@ struct Ensemble {
     ~Ensemble() { release(); }
     void release()
     {
-        for (BatchLaunch *b : {&step, &excl, &spme.spread, &spme.gather, &spme.fft.r2c, &spme.fft.yf, &spme.fft.zc,
-                               &spme.fft.yb, &spme.fft.c2r})
-            batch_free(*b);
+        for (BatchLaunch *b : {&step, &excl, &spme.spread, &spme.gather}) batch_free(*b);
+        for (auto &g : spme.fft.groups) batch_free(g.bl);
+        spme.fft.groups.clear();
         for (auto &g : pair.groups) batch_free(g.bl);
         pair.groups.clear();
         list_batch_free(list);

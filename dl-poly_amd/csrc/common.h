@@ -137,6 +137,7 @@ struct Ctx {
     bool vdw_force_shift = false;    // vdws%l_force_shift
     bool vdw_direct = false;         // vdws%l_direct: analytic forms per pair
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
+    bool fft_lines = true;           // compile-time-length line kernels for the FFT lengths that have them
     int device = 0;
 
     // box (orthorhombic)
@@ -224,6 +225,9 @@ struct Ctx {
     void *H = nullptr;               // complex half grid K3 x K2 x K1hp (x < K1h used)
     void *tw = nullptr;              // twiddles per axis: 3 x max(K) complex
     double *bsp_n2 = nullptr;        // |b|^2 per axis, K1 + K2 + K3 (FP64)
+    void *theta = nullptr;           // {theta(m), its stress weight} on the half spectrum (layout of H), grid precision
+    double theta_key[13]{};          // K1..K3, rcell, alpha it was computed for
+    bool theta_valid = false;
     void *Q2 = nullptr;              // per-particle stress grid (a copy of the charge grid, then convolved)
     double self_energy = 0.0;
     double sumchg = 0.0;
@@ -280,12 +284,19 @@ struct ListBatch {
     int32_t *gate = nullptr;  // one system: its F_UPD word, else null (per-system gates only)
 };
 // the batched passes of one SPME convolution (fft.hip) and of the
-// reciprocal-space stage (spme.hip)
+// reciprocal-space stage (spme.hip).  Each pass is one launch per group of
+// systems: a group per line length with compiled line kernels
+// (fft_lines.hip), one group for the other systems (generic kernels).
+struct FftGroup {
+    BatchLaunch bl;
+    int kind = 0;  // 0 forward x, 1 forward y, 2 z with the influence function, 3 inverse y, 4 inverse x
+    int n = 0;     // line length of a line-kernel group; 0: generic kernels
+    int lpb = 0;   // lines (x-line pairs) per workgroup
+};
 struct FftBatch {
-    BatchLaunch r2c, yf, zc, yb, c2r;
-    int lpb_x = 0, lpb_y = 0, lpb_z = 0;
-    int lpb_cap = 16;            // largest lines per workgroup to use (tuning knob)
-    std::vector<int> conv_rows;  // z-pass workgroups (reduction rows) per system
+    std::vector<FftGroup> groups;  // in launch order
+    int lpb_cap = 16;              // largest generic lines per workgroup to use (tuning knob)
+    std::vector<int> conv_rows;    // z-pass workgroups (reduction rows) per system
 };
 struct SpmeBatch {
     BatchLaunch spread, gather;
@@ -403,6 +414,9 @@ struct InflParams {
 // out <- FFT-(kernel . FFT+(in)) for real grids in and out (may be the same);
 // with clear_in the forward pass leaves in all zero.
 int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in);
+// theta(m) of ip into c->theta, launched on s when the mesh, cell or alpha
+// differ from those of the table's last computation
+int theta_update(Ctx *c, const InflParams &ip, hipStream_t s);
 int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 // batched stages (one launch per stage over the systems of an ensemble)

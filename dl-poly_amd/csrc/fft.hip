@@ -12,139 +12,9 @@
 #include <algorithm>
 #include <vector>
 
-#include "common.h"
-#include "device_util.h"
+#include "fft_common.h"
 
 namespace dlpx {
-
-struct FftPlan {
-    int n, np;
-    int rad[24];
-    int ls[24];        // Ls, the product of the radices of the passes before p
-    float inv_ls[24];  // 1/Ls
-};
-
-static FftPlan make_plan(int n)
-{
-    FftPlan p;
-    p.n = n;
-    p.np = 0;
-    if (n <= 0) return p;  // no mesh (a system without electrostatics)
-    int m = n;
-    while (m % 4 == 0) { p.rad[p.np++] = 4; m /= 4; }
-    while (m % 2 == 0) { p.rad[p.np++] = 2; m /= 2; }
-    while (m % 3 == 0) { p.rad[p.np++] = 3; m /= 3; }
-    while (m % 5 == 0) { p.rad[p.np++] = 5; m /= 5; }
-    int ls = 1;
-    for (int q = 0; q < p.np; q++) {
-        p.ls[q] = ls;
-        p.inv_ls[q] = 1.0f / (float)ls;
-        ls *= p.rad[q];
-    }
-    return p;
-}
-
-// x / d and x mod d for 0 <= x < 2^20 from a float reciprocal: the quotient
-// estimate is within one of the truth, and one correction step fixes it.
-struct FastDiv {
-    int d;
-    float inv;
-};
-static FastDiv make_div(int d) { return FastDiv{d, 1.0f / (float)d}; }
-__device__ __forceinline__ int fdivmod(int x, FastDiv f, int &r)
-{
-    int q = (int)((float)x * f.inv);
-    r = x - q * f.d;
-    if (r < 0) {
-        r += f.d;
-        q--;
-    } else if (r >= f.d) {
-        r -= f.d;
-        q++;
-    }
-    return q;
-}
-
-template <typename T> struct C2T;
-template <> struct C2T<double> { using type = double2; };
-template <> struct C2T<float> { using type = float2; };
-
-template <typename T2>
-__device__ __forceinline__ T2 cmul(T2 a, T2 b)
-{
-    T2 r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
-    return r;
-}
-template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-// multiply by s*i
-template <typename T2> __device__ __forceinline__ T2 cmuli(T2 a, int s) { T2 r; r.x = -s * a.y; r.y = s * a.x; return r; }
-
-// In-place 4-point DFT with kernel exp(sign 2 pi i jk/4).
-template <typename T2>
-__device__ __forceinline__ void dft4(T2 &a, T2 &b, T2 &c, T2 &d, int sign)
-{
-    const T2 s02 = cadd(a, c), d02 = csub(a, c);
-    const T2 s13 = cadd(b, d), d13 = cmuli(csub(b, d), sign);
-    a = cadd(s02, s13);
-    c = csub(s02, s13);
-    b = cadd(d02, d13);
-    d = csub(d02, d13);
-}
-
-// In-place DFT of length R (2, 3, 4, 5) with kernel exp(sign 2 pi i jk/R).
-template <typename T, typename T2, int R>
-__device__ __forceinline__ void dft_small(T2 *x, int sign)
-{
-    if (R == 2) {
-        T2 a = x[0], b = x[1];
-        x[0] = cadd(a, b);
-        x[1] = csub(a, b);
-    } else if (R == 4) {
-        T2 s02 = cadd(x[0], x[2]), d02 = csub(x[0], x[2]);
-        T2 s13 = cadd(x[1], x[3]), d13 = cmuli(csub(x[1], x[3]), sign);
-        x[0] = cadd(s02, s13);
-        x[2] = csub(s02, s13);
-        x[1] = cadd(d02, d13);
-        x[3] = csub(d02, d13);
-    } else if (R == 3) {
-        const T h = (T)0.86602540378443864676;  // sqrt(3)/2
-        T2 t = cadd(x[1], x[2]), d = csub(x[1], x[2]);
-        T2 a;
-        a.x = x[0].x - (T)0.5 * t.x;
-        a.y = x[0].y - (T)0.5 * t.y;
-        T2 b = cmuli(d, sign);
-        b.x *= h;
-        b.y *= h;
-        x[0] = cadd(x[0], t);
-        x[1] = cadd(a, b);
-        x[2] = csub(a, b);
-    } else {  // 5
-        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
-        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
-        T2 t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
-        T2 d1 = csub(x[1], x[4]), d2 = csub(x[2], x[3]);
-        T2 a1, a2, p1, p2;
-        a1.x = x[0].x + c1 * t1.x + c2 * t2.x;
-        a1.y = x[0].y + c1 * t1.y + c2 * t2.y;
-        a2.x = x[0].x + c2 * t1.x + c1 * t2.x;
-        a2.y = x[0].y + c2 * t1.y + c1 * t2.y;
-        // s1 d1 + s2 d2 and s2 d1 - s1 d2, then times sign*i
-        p1.x = s1 * d1.x + s2 * d2.x;
-        p1.y = s1 * d1.y + s2 * d2.y;
-        p2.x = s2 * d1.x - s1 * d2.x;
-        p2.y = s2 * d1.y - s1 * d2.y;
-        const T2 b1 = cmuli(p1, sign), b2 = cmuli(p2, sign);
-        x[0].x = x[0].x + t1.x + t2.x;
-        x[0].y = x[0].y + t1.y + t2.y;
-        x[1] = cadd(a1, b1);
-        x[4] = csub(a1, b1);
-        x[2] = cadd(a2, b2);
-        x[3] = csub(a2, b2);
-    }
-}
 
 // 512 threads per workgroup: the y and z passes have few workgroups (one per
 // 16 lines) and are latency-bound, so each needs many wavefronts in flight.
@@ -204,27 +74,6 @@ __device__ T2 *stockham(T2 *buf0, T2 *buf1, const FftPlan &pl, const T2 *tw, int
     return src;
 }
 
-// Strided lines of the half-spectrum array.  Line L is (a, b) = (L % na_pad,
-// L / na_pad); it exists when a < na, starts at a * sa + b * sb and its
-// elements are se apart.  na_pad (na rounded up to the lines per workgroup)
-// keeps every workgroup's lines on one aligned run of x columns, and the
-// array's row stride K1hp (K1/2+1 rounded up to 16) keeps those runs on whole
-// 128-byte lines: with rows of K1/2+1 elements every run straddled two.
-struct LineMap {
-    int nl;        // number of line slots, na_pad * (number of b)
-    int na, na_pad;
-    long sa, sb;
-    long se;
-};
-
-__device__ __forceinline__ bool line_ok(const LineMap &lm, int L) { return L < lm.nl && L % lm.na_pad < lm.na; }
-
-template <typename T2>
-__device__ __forceinline__ size_t line_base(const LineMap &lm, int L)
-{
-    return (size_t)(L % lm.na_pad) * lm.sa + (size_t)(L / lm.na_pad) * lm.sb;
-}
-
 // LPB strided lines per workgroup: thread t moves elements of line t % LPB,
 // so the LPB lines (adjacent x columns) are read and written as contiguous
 // runs of LPB elements.
@@ -271,14 +120,6 @@ __device__ __forceinline__ void load_twiddles(const T2 *tw, int n, T2 *stw)
 }
 
 // Complex-to-complex transforms of strided lines (the y axis), in place.
-template <typename T>
-struct C2CArgs {
-    typename C2T<T>::type *data;
-    LineMap lm;
-    FftPlan pl;
-    const typename C2T<T>::type *tw;
-    int sign;
-};
 template <typename T, int LPB>
 __device__ __forceinline__ void c2c_body(const C2CArgs<T> &a, int bid)
 {
@@ -297,22 +138,11 @@ __device__ __forceinline__ void c2c_body(const C2CArgs<T> &a, int bid)
     store_lines<T2, LPB>(a.data, lm, n, L0, nl, res);
 }
 
-constexpr int kConvRedBytes = 6 * (kFftThreads / 64) * sizeof(double);
 
 // The z axis with the influence function in between: forward transform of
 // each z line, multiply by theta(m) (ewald_spole.F90:1296-1380) while summing
 // the reciprocal stress, inverse transform, all in LDS.  Line L is the column
 // (kx, ky) = (a, b) of the half spectrum (see LineMap).
-template <typename T>
-struct ZConvArgs {
-    typename C2T<T>::type *data;
-    LineMap lm;
-    FftPlan pl;
-    const typename C2T<T>::type *tw;
-    InflParams ip;
-    const double *n2;
-    double *red;
-};
 template <typename T, int LPB>
 __device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
 {
@@ -341,6 +171,8 @@ __device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
         const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
         const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
         const double bxy = n2y[k] * n2x[j];
+        const T2 *__restrict__ th = a.theta + line_base<T2>(lm, L);
+        const double rcf2 = ip.rcf * ip.rcf;
         // the other half of the spectrum: column j (0 < j < K1/2) stands for
         // its Hermitian partner (-j, -ky, -kz) too, with the same |m| and
         // |Q|^2.  The reference folds an index at K/2 to +K/2 (ewald_spole.F90:
@@ -354,10 +186,10 @@ __device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
             const double r3[3] = {ll * ip.rcell[2], ll * ip.rcell[5], ll * ip.rcell[8]};
             const double r2[3] = {r3[0] + kk * ip.rcell[1], r3[1] + kk * ip.rcell[4], r3[2] + kk * ip.rcell[7]};
             const double rp[3] = {r2[0] + j * ip.rcell[0], r2[1] + j * ip.rcell[3], r2[2] + j * ip.rcell[6]};
-            const double bb1 = n2z[l] * bxy;
             const double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
             T2 h = res[lw * ns + l];
             if (ip.mode == 1) {
+                const double bb1 = n2z[l] * bxy;
                 // ewald_general.F90:674-682 with stress_kernel (:1033-1057):
                 // B f'(x) pi / x / alpha, f(x) = exp(-x^2)/(sqrt(pi) x^2),
                 // f'(x) = -2 f/x - (2/x) exp(-x^2)/sqrt(pi) (spme.F90:421-436)
@@ -373,11 +205,11 @@ __device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
                     h.x = (T)0;
                     h.y = (T)0;
                 }
-            } else if (k2 <= ip.cut2 && k2 > ip.test_fac) {
-                const double fp = sqrt(k2) * ip.rcf;
-                const double fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
+            } else if (const double fac = (double)th[(size_t)l * lm.se].x; fac != 0.0) {
+                // theta is zero outside the cutoff sphere and at m = 0;
+                // (1 + fp^2)/k2 = 1/k2 + (pi/alpha)^2 with fp = |m| pi/alpha
                 const double hr = (double)h.x, hi = (double)h.y;
-                const double pv = fac * (hr * hr + hi * hi) * (-2.0 * ((1.0 + fp * fp) / k2));
+                const double pv = fac * (hr * hr + hi * hi) * (-2.0 * (1.0 / k2 + rcf2));
                 double s0 = rp[0] * rp[0], s1 = rp[0] * rp[1], s2 = rp[0] * rp[2];
                 double s3 = rp[1] * rp[1], s4 = rp[1] * rp[2], s5 = rp[2] * rp[2];
                 if (partner) {
@@ -417,16 +249,6 @@ __device__ __forceinline__ void z_conv_body(const ZConvArgs<T> &a, int bid)
 // packed as one complex line; output half spectra to H (K1h of a row of hs).
 // With clear, every grid value is set to zero once read (each is read by one
 // thread), so the charge grid is ready for the next spreading.
-template <typename T>
-struct R2CArgs {
-    T *Q;
-    typename C2T<T>::type *H;
-    int nreal_lines, K1h, hs;
-    FftPlan pl;
-    const typename C2T<T>::type *tw;
-    FastDiv div_n, div_h;
-    int clear;
-};
 template <typename T, int LPB>
 __device__ __forceinline__ void r2c_body(const R2CArgs<T> &a, int bid)
 {
@@ -502,15 +324,6 @@ __device__ __forceinline__ void r2c_body(const R2CArgs<T> &a, int bid)
 
 // x axis, inverse: rebuild the full Hermitian line of two half spectra,
 // transform with sign -1 and write the real parts to Q.
-template <typename T>
-struct C2RArgs {
-    const typename C2T<T>::type *H;
-    T *Q;
-    int nreal_lines, K1h, hs;
-    FftPlan pl;
-    const typename C2T<T>::type *tw;
-    FastDiv div_n;
-};
 template <typename T, int LPB>
 __device__ __forceinline__ void c2r_body(const C2RArgs<T> &a, int bid)
 {
@@ -576,6 +389,63 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T> &a, int bid)
         Q[(size_t)a * n + e] = z.x;
         if (b < nreal_lines) Q[(size_t)b * n + e] = z.y;
     }
+}
+
+// The influence function theta(m) = B(m) exp(-pi^2 m^2/alpha^2) /
+// (sqrt(pi) (pi |m|/alpha)^2) of ewald_spole.F90:1296-1380 on the half
+// spectrum, in the layout of H (x fastest, row stride K1hp), and its stress
+// weight theta(m) (-2) (1 + fp^2)/|m|^2 = theta(m) (-2) (1/|m|^2 + (pi/alpha)^2)
+// (ewald_spole.F90:1366-1374); both zero outside the spherical cutoff and at
+// m = 0.  They depend only on the mesh, the cell and alpha, so the z pass
+// reads them instead of forming them per element and step.
+template <typename T>
+__global__ void __launch_bounds__(256) k_theta(typename C2T<T>::type *__restrict__ th, InflParams ip,
+                                               const double *__restrict__ n2, int K1hp)
+{
+    const long total = (long)ip.K1h * ip.K[1] * ip.K[2];
+    const double sqrpi = 1.7724538509055159;
+    const double *n2x = n2, *n2y = n2 + ip.K[0], *n2z = n2 + ip.K[0] + ip.K[1];
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int j = (int)(e % ip.K1h);
+        const long rest = e / ip.K1h;
+        const int k = (int)(rest % ip.K[1]), l = (int)(rest / ip.K[1]);
+        const int kk = (2 * k > ip.K[1]) ? k - ip.K[1] : k;
+        const int ll = (2 * l > ip.K[2]) ? l - ip.K[2] : l;
+        const double r3[3] = {ll * ip.rcell[2], ll * ip.rcell[5], ll * ip.rcell[8]};
+        const double r2[3] = {r3[0] + kk * ip.rcell[1], r3[1] + kk * ip.rcell[4], r3[2] + kk * ip.rcell[7]};
+        const double rp[3] = {r2[0] + j * ip.rcell[0], r2[1] + j * ip.rcell[3], r2[2] + j * ip.rcell[6]};
+        const double bb1 = n2z[l] * (n2y[k] * n2x[j]);
+        const double k2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+        double fac = 0.0, wst = 0.0;
+        if (k2 <= ip.cut2 && k2 > ip.test_fac) {
+            const double fp = sqrt(k2) * ip.rcf;
+            fac = bb1 * exp(-(fp * fp)) / (sqrpi * (fp * fp));
+            wst = fac * (-2.0 * (1.0 / k2 + ip.rcf * ip.rcf));
+        }
+        typename C2T<T>::type v;
+        v.x = (T)fac;
+        v.y = (T)wst;
+        th[((size_t)l * ip.K[1] + k) * K1hp + j] = v;
+    }
+}
+
+int theta_update(Ctx *c, const InflParams &ip, hipStream_t s)
+{
+    double key[13] = {(double)ip.K[0], (double)ip.K[1], (double)ip.K[2]};
+    for (int k = 0; k < 9; k++) key[3 + k] = ip.rcell[k];
+    key[12] = ip.alpha;
+    if (c->theta_valid && std::equal(key, key + 13, c->theta_key)) return 0;
+    const long total = (long)ip.K1h * ip.K[1] * ip.K[2];
+    const int nb = (int)std::min<long>(ceil_div(total, 256), 8192);
+    if (c->prm.precision == DLPX_MIXED)
+        hipLaunchKernelGGL(k_theta<float>, dim3(nb), dim3(256), 0, s, (float2 *)c->theta, ip, c->bsp_n2, c->K1hp);
+    else
+        hipLaunchKernelGGL(k_theta<double>, dim3(nb), dim3(256), 0, s, (double2 *)c->theta, ip, c->bsp_n2,
+                           c->K1hp);
+    DLPX_HIP(c, hipGetLastError());
+    std::copy(key, key + 13, c->theta_key);
+    c->theta_valid = true;
+    return 0;
 }
 
 // Single-system kernels (one 3D grid) and batched ones (the grids of many
@@ -684,6 +554,7 @@ static ZConvArgs<T> z_args(const Ctx *c, const InflParams &ip, int lpb)
     a.data = (T2 *)c->H;
     a.ip = ip;
     a.n2 = c->bsp_n2;
+    a.theta = (const typename C2T<T>::type *)c->theta;
     a.red = c->red;
     return a;
 }
@@ -726,9 +597,16 @@ static C2RArgs<T> c2r_args(const Ctx *c, void *grid)
 
 static int x_blocks(const Ctx *c, int lpb) { return ceil_div((c->K[1] * c->K[2] + 1) / 2, lpb); }
 
+static bool use_lines(const Ctx *c, int n) { return c->fft_lines && lines_supported(n); }
+
 template <typename T>
 static int run_y(Ctx *c, int sign)
 {
+    if (use_lines(c, c->K[1])) {
+        const int lpb = lines_lpb<T>();
+        const C2CArgs<T> a = y_args<T>(c, sign, lpb);
+        return lines_c2c<T>(c, c->sp, a, ceil_div(a.lm.nl, lpb));
+    }
     const int lpb = xy_lpb<T>(c->K[1]);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
     const C2CArgs<T> a = y_args<T>(c, sign, lpb);
@@ -748,6 +626,15 @@ static int run_z_conv(Ctx *c, const InflParams &ip)
 {
     const int lpb = z_lpb<T>(c);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+    int r;
+    if (ip.mode == 0 && (r = theta_update(c, ip, c->sp))) return r;
+    if (ip.mode == 0 && use_lines(c, c->K[2])) {
+        const int ll = lines_lpb<T>();
+        const ZConvArgs<T> a = z_args<T>(c, ip, ll);
+        const int nb = ceil_div(a.lm.nl, ll);
+        for (int k = R_RS11; k <= R_RS33; k++) c->red_rows[k] = nb;
+        return lines_zconv<T>(c, c->sp, a, nb);
+    }
     const ZConvArgs<T> a = z_args<T>(c, ip, lpb);
     const size_t lds = fft_lds<T>(lpb, c->K[2], kConvRedBytes);
     const int nb = ceil_div(a.lm.nl, lpb);
@@ -766,6 +653,15 @@ template <typename T>
 static int run_x(Ctx *c, bool forward, void *grid, bool clear = false)
 {
     const int n = c->K[0];
+    if (use_lines(c, n)) {
+        const int ll = lines_lpb<T>(), nb = x_blocks(c, ll);
+        if (forward) {
+            const R2CArgs<T> a = r2c_args<T>(c, grid, clear);
+            return lines_x<T>(c, c->sp, &a, nullptr, nb);
+        }
+        const C2RArgs<T> a = c2r_args<T>(c, grid);
+        return lines_x<T>(c, c->sp, nullptr, &a, nb);
+    }
     const int lpb = xy_lpb<T>(n);
     if (!lpb) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
     const size_t lds = fft_lds<T>(lpb, n, 0);
@@ -808,62 +704,108 @@ int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_i
 }
 
 // Batched convolution Q -> Phi (influence function mode 0, the forward pass
-// clearing Q) over the systems with electrostatics: one launch per pass
-// over every system's lines, with one lines-per-workgroup value per pass
-// (the smallest any system needs, so every system's lines fit its LDS).
+// clearing Q) over the systems with electrostatics: each pass is one launch
+// per group of systems, a group for each line length that has compiled line
+// kernels and one for the rest (generic kernels, with one lines-per-workgroup
+// value per pass: the smallest any of its systems needs, so every system's
+// lines fit its LDS).
+template <typename T, typename A, typename Build>
+static int add_group(const std::vector<Ctx *> &cs, FftBatch &fb, int kind, int key, int lpb,
+                     const std::vector<int> &keys, Build build)
+{
+    const size_t ns = cs.size();
+    std::vector<A> args(ns, A{});
+    std::vector<int> nb(ns, 0);
+    size_t lds = 0;
+    for (size_t k = 0; k < ns; k++) {
+        if (keys[k] != key) continue;
+        size_t l = 0;
+        args[k] = build(cs[k], k, lpb, nb[k], l);
+        lds = std::max(lds, l);
+        if (kind == 2) fb.conv_rows[k] = nb[k];
+    }
+    FftGroup g;
+    g.kind = kind;
+    g.n = key > 0 ? key : 0;
+    g.lpb = lpb;
+    int r;
+    if ((r = batch_upload(cs[0], g.bl, args.data(), sizeof(A), nb, false))) return r;
+    g.bl.lds = lds;
+    fb.groups.push_back(g);
+    return 0;
+}
+
 template <typename T>
 static int fft_prepare_t(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips, FftBatch &fb)
 {
     const int cap = fb.lpb_cap > 0 ? fb.lpb_cap : 16;
-    int lx = cap, ly = cap, lz = cap;
-    for (Ctx *c : cs) {
-        if (!c->prm.electrostatics) continue;
-        const int a = xy_lpb<T>(c->K[0]), b = xy_lpb<T>(c->K[1]), z = z_lpb<T>(c);
-        if (!a || !b || !z) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
-        lx = std::min(lx, a);
-        ly = std::min(ly, b);
-        lz = std::min(lz, z);
-    }
-    std::vector<R2CArgs<T>> r2c;
-    std::vector<C2CArgs<T>> yf, yb;
-    std::vector<ZConvArgs<T>> zc;
-    std::vector<C2RArgs<T>> c2r;
-    std::vector<int> nbx, nby, nbz;
-    size_t ldx = 0, ldy = 0, ldz = 0;
-    for (size_t k = 0; k < cs.size(); k++) {
+    for (auto &g : fb.groups) batch_free(g.bl);
+    fb.groups.clear();
+    const size_t ns = cs.size();
+    fb.conv_rows.assign(ns, 0);
+    int lg[3] = {cap, cap, cap};  // generic lines per workgroup per axis
+    for (size_t k = 0; k < ns; k++) {
         Ctx *c = cs[k];
-        const bool on = c->prm.electrostatics != 0;
-        // a system without electrostatics has no mesh: zero workgroups and
-        // an empty argument struct
-        r2c.push_back(on ? r2c_args<T>(c, c->Q, true) : R2CArgs<T>{});
-        c2r.push_back(on ? c2r_args<T>(c, c->Phi) : C2RArgs<T>{});
-        yf.push_back(on ? y_args<T>(c, +1, ly) : C2CArgs<T>{});
-        yb.push_back(on ? y_args<T>(c, -1, ly) : C2CArgs<T>{});
-        zc.push_back(on ? z_args<T>(c, ips[k], lz) : ZConvArgs<T>{});
-        nbx.push_back(on ? x_blocks(c, lx) : 0);
-        nby.push_back(on ? ceil_div(yf.back().lm.nl, ly) : 0);
-        nbz.push_back(on ? ceil_div(zc.back().lm.nl, lz) : 0);
-        if (on) {
-            ldx = std::max(ldx, fft_lds<T>(lx, c->K[0], 0));
-            ldy = std::max(ldy, fft_lds<T>(ly, c->K[1], 0));
-            ldz = std::max(ldz, fft_lds<T>(lz, c->K[2], kConvRedBytes));
+        if (!c->prm.electrostatics) continue;
+        int r;
+        if ((r = theta_update(c, ips[k], c->stream))) return r;
+        DLPX_HIP(c, hipStreamSynchronize(c->stream));
+        for (int d = 0; d < 3; d++) {
+            if (use_lines(c, c->K[d])) continue;
+            const int l = d == 2 ? z_lpb<T>(c) : xy_lpb<T>(c->K[d]);
+            if (!l) return fail(c, DLPX_ERR_UNSUPPORTED, "FFT length too large for LDS");
+            lg[d] = std::min(lg[d], l);
         }
     }
-    Ctx *c0 = cs[0];
-    int r;
-    if ((r = batch_upload(c0, fb.r2c, r2c.data(), sizeof(R2CArgs<T>), nbx, false)) ||
-        (r = batch_upload(c0, fb.yf, yf.data(), sizeof(C2CArgs<T>), nby, false)) ||
-        (r = batch_upload(c0, fb.zc, zc.data(), sizeof(ZConvArgs<T>), nbz, false)) ||
-        (r = batch_upload(c0, fb.yb, yb.data(), sizeof(C2CArgs<T>), nby, false)) ||
-        (r = batch_upload(c0, fb.c2r, c2r.data(), sizeof(C2RArgs<T>), nbx, false)))
-        return r;
-    fb.r2c.lds = fb.c2r.lds = ldx;
-    fb.yf.lds = fb.yb.lds = ldy;
-    fb.zc.lds = ldz;
-    fb.lpb_x = lx;
-    fb.lpb_y = ly;
-    fb.lpb_z = lz;
-    fb.conv_rows.assign(nbz.begin(), nbz.end());
+    const int axis_of[5] = {0, 1, 2, 1, 0};
+    for (int kind = 0; kind < 5; kind++) {
+        const int d = axis_of[kind];
+        // group key per system: its line length with line kernels, 0 for the
+        // generic group, -1 without a mesh
+        std::vector<int> keys(ns, -1), distinct;
+        for (size_t k = 0; k < ns; k++) {
+            Ctx *c = cs[k];
+            if (!c->prm.electrostatics) continue;
+            keys[k] = use_lines(c, c->K[d]) ? c->K[d] : 0;
+            if (std::find(distinct.begin(), distinct.end(), keys[k]) == distinct.end()) distinct.push_back(keys[k]);
+        }
+        for (int key : distinct) {
+            const int lpb = key > 0 ? lines_lpb<T>() : lg[d];
+            int r = 0;
+            if (kind == 0)
+                r = add_group<T, R2CArgs<T>>(cs, fb, kind, key, lpb, keys,
+                                             [&](Ctx *c, size_t, int l, int &nb, size_t &lds) {
+                                                 nb = x_blocks(c, l);
+                                                 lds = key > 0 ? lines_lds<T>(2, key) : fft_lds<T>(l, c->K[0], 0);
+                                                 return r2c_args<T>(c, c->Q, true);
+                                             });
+            else if (kind == 4)
+                r = add_group<T, C2RArgs<T>>(cs, fb, kind, key, lpb, keys,
+                                             [&](Ctx *c, size_t, int l, int &nb, size_t &lds) {
+                                                 nb = x_blocks(c, l);
+                                                 lds = key > 0 ? lines_lds<T>(3, key) : fft_lds<T>(l, c->K[0], 0);
+                                                 return c2r_args<T>(c, c->Phi);
+                                             });
+            else if (kind == 2)
+                r = add_group<T, ZConvArgs<T>>(cs, fb, kind, key, lpb, keys,
+                                               [&](Ctx *c, size_t k, int l, int &nb, size_t &lds) {
+                                                   const ZConvArgs<T> a = z_args<T>(c, ips[k], l);
+                                                   nb = ceil_div(a.lm.nl, l);
+                                                   lds = key > 0 ? lines_lds<T>(1, key)
+                                                                 : fft_lds<T>(l, c->K[2], kConvRedBytes);
+                                                   return a;
+                                               });
+            else
+                r = add_group<T, C2CArgs<T>>(cs, fb, kind, key, lpb, keys,
+                                             [&](Ctx *c, size_t, int l, int &nb, size_t &lds) {
+                                                 const C2CArgs<T> a = y_args<T>(c, kind == 1 ? +1 : -1, l);
+                                                 nb = ceil_div(a.lm.nl, l);
+                                                 lds = key > 0 ? lines_lds<T>(0, key) : fft_lds<T>(l, c->K[1], 0);
+                                                 return a;
+                                             });
+            if (r) return r;
+        }
+    }
     return 0;
 }
 
@@ -875,27 +817,34 @@ int fft_prepare(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips
 template <typename T>
 static int fft_launch_t(Ctx *c, hipStream_t s, const FftBatch &fb)
 {
-    if (fb.r2c.total == 0) return 0;
-#define DLPX_FB(KERNEL, ARGS, BL, L)                                                                         \
-    DLPX_LDS_MAX(c, (KERNEL<T, L>));                                                                         \
-    hipLaunchKernelGGL((KERNEL<T, L>), dim3((BL).total), dim3(kFftThreads), (BL).lds, s, (const ARGS<T> *)(BL).args, \
-                       (BL).boff, (BL).nsys)
-#define DLPX_FB_R2C(L) DLPX_FB(k_fft_x_r2c_batch, R2CArgs, fb.r2c, L)
-#define DLPX_FB_YF(L) DLPX_FB(k_fft_c2c_batch, C2CArgs, fb.yf, L)
-#define DLPX_FB_Z(L) DLPX_FB(k_fft_z_conv_batch, ZConvArgs, fb.zc, L)
-#define DLPX_FB_YB(L) DLPX_FB(k_fft_c2c_batch, C2CArgs, fb.yb, L)
-#define DLPX_FB_C2R(L) DLPX_FB(k_fft_x_c2r_batch, C2RArgs, fb.c2r, L)
-    DLPX_LPB_SWITCH(fb.lpb_x, DLPX_FB_R2C)
-    DLPX_LPB_SWITCH(fb.lpb_y, DLPX_FB_YF)
-    DLPX_LPB_SWITCH(fb.lpb_z, DLPX_FB_Z)
-    DLPX_LPB_SWITCH(fb.lpb_y, DLPX_FB_YB)
-    DLPX_LPB_SWITCH(fb.lpb_x, DLPX_FB_C2R)
+    for (const FftGroup &g : fb.groups) {
+        if (g.bl.total == 0) continue;
+        if (g.n > 0) {
+            int r;
+            if ((r = lines_batch<T>(c, s, g.kind, g.n, g.bl))) return r;
+            continue;
+        }
+        const BatchLaunch &bl = g.bl;
+#define DLPX_FB(KERNEL, ARGS, L)                                                                                 \
+    DLPX_LDS_MAX(c, (KERNEL<T, L>));                                                                             \
+    hipLaunchKernelGGL((KERNEL<T, L>), dim3(bl.total), dim3(kFftThreads), bl.lds, s, (const ARGS<T> *)bl.args,   \
+                       bl.boff, bl.nsys)
+#define DLPX_FB_R2C(L) DLPX_FB(k_fft_x_r2c_batch, R2CArgs, L)
+#define DLPX_FB_Y(L) DLPX_FB(k_fft_c2c_batch, C2CArgs, L)
+#define DLPX_FB_Z(L) DLPX_FB(k_fft_z_conv_batch, ZConvArgs, L)
+#define DLPX_FB_C2R(L) DLPX_FB(k_fft_x_c2r_batch, C2RArgs, L)
+        switch (g.kind) {
+            case 0: DLPX_LPB_SWITCH(g.lpb, DLPX_FB_R2C) break;
+            case 2: DLPX_LPB_SWITCH(g.lpb, DLPX_FB_Z) break;
+            case 4: DLPX_LPB_SWITCH(g.lpb, DLPX_FB_C2R) break;
+            default: DLPX_LPB_SWITCH(g.lpb, DLPX_FB_Y) break;
+        }
 #undef DLPX_FB_R2C
-#undef DLPX_FB_YF
+#undef DLPX_FB_Y
 #undef DLPX_FB_Z
-#undef DLPX_FB_YB
 #undef DLPX_FB_C2R
 #undef DLPX_FB
+    }
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
@@ -907,6 +856,10 @@ int fft_launch(Ctx *c, hipStream_t s, const FftBatch &fb)
 
 int conv_blocks(const Ctx *c)
 {
+    if (use_lines(c, c->K[2])) {
+        const int ll = c->prm.precision == DLPX_MIXED ? lines_lpb<float>() : lines_lpb<double>();
+        return ceil_div((long)z_lines(c, ll).nl, ll);
+    }
     const int lpb = c->prm.precision == DLPX_MIXED ? z_lpb<float>(c) : z_lpb<double>(c);
     return lpb ? ceil_div((long)z_lines(c, lpb).nl, lpb) : 0;
 }

@@ -684,6 +684,9 @@ int spme_setup(Ctx *c)
     DLPX_HIP(c, dev_zero(c->stream, c->Q, rs * c->nreal));  // kept zero from here on by the forward x pass
     DLPX_HIP(c, hipMalloc(&c->Phi, rs * c->nreal));
     DLPX_HIP(c, hipMalloc(&c->H, 2 * rs * c->nhalf));
+    DLPX_HIP(c, hipMalloc(&c->theta, 2 * rs * c->nhalf));
+    DLPX_HIP(c, dev_zero(c->stream, c->theta, 2 * rs * c->nhalf));  // the row padding stays zero
+    c->theta_valid = false;
     const int ksum = c->K[0] + c->K[1] + c->K[2];
     std::vector<double> n2(ksum);
     bspline_norm2(c->K[0], S, n2.data());

@@ -84,6 +84,32 @@ def test_fp64_matches_oracle_and_reference(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["nacl_1000_k45_27_25_s6", "nacl_1000_k30_36_40_s6", "mix_4000_k36"])
+def test_generic_and_line_ffts_agree(name, monkeypatch):
+    """The FFT lengths with compiled line kernels (the default for them) and
+    the generic Stockham kernels (DLPX_FFT_LINES=0, every length) agree to
+    FP64 rounding in forces, energies and the reciprocal stress, and each
+    matches the oracle, in both precisions; the meshes cover odd, even and
+    mixed-radix edges (45, 27, 25; 30, 36, 40; 36)."""
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    o = oracle.two_body(s)
+    for precision, tol in ((lib.FP64, 1e-10), (lib.MIXED, 1e-5)):
+        res = {}
+        for lines in ("1", "0"):
+            monkeypatch.setenv("DLPX_FFT_LINES", lines)
+            ctx, res[lines] = gpu_two_body(s, precision)
+            ctx.close()
+            assert rms_rel(res[lines]["frc"], o["frc"]) <= tol
+            check_energies(res[lines], o, 1e-11 if precision == lib.FP64 else 1e-6,
+                           energy_scales(o, mixed=precision == lib.MIXED))
+        a, b = res["1"], res["0"]
+        agree = 1e-12 if precision == lib.FP64 else 1e-6
+        assert rms_rel(a["frc"], b["frc"]) <= agree
+        assert np.abs(a["stress"] - b["stress"]).max() <= agree * np.abs(b["stress"]).max()
+        assert abs(a["engcpe"] - b["engcpe"]) <= agree * abs(b["engcpe"])
+
+
 @pytest.mark.parametrize("name", [n for n in STATIC if "tab_pot" in fixtures.load(n)[1]])
 def test_reference_vdw_tables_through_set_vdw_tables(name):
     """The host-table route: DL_POLY's own vdws%tab_potential / tab_force
