@@ -55,7 +55,7 @@ static int set_cell(Ctx *c, const double *cell)
 
 int upload_tables(Ctx *c)
 {
-    if (!c->tables_dirty) return 0;
+    if (!c->tables_dirty) return c->comb_dirty && !c->noalloc_now ? upload_comb(c) : 0;
     const int nt = c->prm.ntypes;
     if (c->nkeys != nt * (nt + 1) / 2) return fail(c, DLPX_ERR_STATE, "vdW tables not set for every type pair");
     if (c->prm.electrostatics && c->nsamples <= 0) return fail(c, DLPX_ERR_STATE, "erfc tables not set");
@@ -156,7 +156,8 @@ int upload_tables(Ctx *c)
         }
     }
     c->tables_dirty = false;
-    return 0;
+    c->comb_dirty = true;
+    return c->noalloc_now ? 0 : upload_comb(c);
 }
 
 // host arrays (strided) -> pinned staging -> device staging -> cell order
@@ -186,6 +187,25 @@ static int upload(Ctx *c, const dlpx_atoms *a, bool identity, const double *vx, 
         c->h_t[i] = t;
     }
     c->natms = n;
+    {
+        // charge per atom type, for the pair kernel's combined-table path
+        std::vector<double> tq(c->prm.ntypes, 0.0);
+        std::vector<char> seen(c->prm.ntypes, 0);
+        bool per_type = true;
+        for (int i = 0; i < n && per_type; i++) {
+            const int t = c->h_t[i] - 1;
+            const double q = c->h_st[(size_t)8 * i + 3];
+            if (!seen[t]) {
+                seen[t] = 1;
+                tq[t] = q;
+            } else if (q != tq[t]) {
+                per_type = false;
+            }
+        }
+        if (per_type != c->q_per_type || (per_type && tq != c->type_q)) c->comb_dirty = true;
+        c->q_per_type = per_type;
+        c->type_q = tq;
+    }
     DLPX_HIP(c, hipMemcpyAsync(c->stage, c->h_st, sizeof(double) * 8 * n, hipMemcpyHostToDevice, c->stream));
     DLPX_HIP(c, hipMemcpyAsync(c->stage_i, c->h_t, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
     return load_atoms(c, c->stage, c->stage_i, identity, vx != nullptr);
@@ -488,6 +508,10 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     Ctx *c = new (std::nothrow) Ctx();
     if (!c) return DLPX_ERR_BAD_ARG;
     c->prm = *p;
+    if (p->natms_max >= kMaxAtoms) {
+        delete c;
+        return DLPX_ERR_UNSUPPORTED;
+    }
     c->cap = p->natms_max;
     int r = 0;
     auto bail = [&](int code) {
@@ -509,6 +533,9 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     if (const char *e = std::getenv("DLPX_LIST_SUBCELLS")) c->list_subcells = std::atoi(e) != 0;
     // DLPX_FFT_LINES=0: the generic Stockham FFT kernels for every length
     if (const char *e = std::getenv("DLPX_FFT_LINES")) c->fft_lines = std::atoi(e) != 0;
+    // DLPX_REL_PAIRS=0: the mixed pair kernel forms separations from FP64 positions
+    if (const char *e = std::getenv("DLPX_REL_PAIRS")) c->rel_pairs = std::atoi(e) != 0;
+    if (const char *e = std::getenv("DLPX_COMB_THREADS")) c->comb_threads = std::atoi(e) == 512 ? 512 : 1024;
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;
@@ -522,7 +549,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
         (r = dalloc(c, &c->gid, n)) || (r = dalloc(c, &c->gid_alt, n)) || (r = dalloc(c, &c->slot, n)) ||
         (r = dalloc(c, &c->stage, 9 * n)) || (r = dalloc(c, &c->stage_i, n)) || (r = dalloc(c, &c->cell_of, n)) ||
         (r = dalloc(c, &c->perm, n)) || (r = dalloc(c, &c->cnt, n * kLanesPerAtom)) || (r = dalloc(c, &c->flags, F_WORDS)) ||
-        (r = dalloc(c, &c->red_out, kReduceSlots)) || (r = dalloc(c, &c->acc, 4 * kAccSlots)))
+        (r = dalloc(c, &c->red_out, kReduceSlots)) || (r = dalloc(c, &c->acc, 4 * kAccSlots)) ||
+        (r = dalloc(c, &c->sub_of, n)) || (r = dalloc(c, &c->prel, n)))
         return bail(r);
     if (dev_zero(c->stream, c->acc, sizeof(double) * 4 * kAccSlots) != hipSuccess) return bail(DLPX_ERR_HIP);
     if (dev_zero(c->stream, c->vel, sizeof(double4) * n) != hipSuccess) return bail(DLPX_ERR_HIP);
@@ -554,7 +582,7 @@ int dlpx_destroy(dlpx_ctx *ctx)
     void *dev[] = {c->pos, c->pos_alt, c->vel, c->vel_alt, c->frc, c->frc_recip, c->xref, c->typ, c->typ_alt,
                    c->gid, c->gid_alt, c->slot, c->stage, c->stage_i, c->cell_of, c->perm, c->cell_count,
                    c->cell_start, c->cell_fill, c->key_start, c->nbr, c->cnt, c->flags, c->d_vdw, c->d_erfc, c->d_keytab,
-                   c->d_vdw_direct, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
+                   c->d_vdw_direct, c->d_comb, c->d_comb_row, c->sub_of, c->prel, c->excl_off, c->excl_ids, c->frz, c->pp, c->Q2,
                    c->Q, c->Phi, c->H, c->theta, c->tw, c->bsp_n2, c->red, c->red_out, c->acc, c->energies_dev,
                    c->snap_d, c->snap_i};
     for (void *p : dev)
@@ -845,8 +873,8 @@ int dlpx_link_cell_pairs(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
     // tables already given go to the device with the list, so that the force
     // calls after it need no allocation of their own
     const int nt = c->prm.ntypes;
-    if (c->tables_dirty && c->nkeys == nt * (nt + 1) / 2 && (!c->prm.electrostatics || c->nsamples > 0) &&
-        (r = upload_tables(c)))
+    if ((c->tables_dirty || c->comb_dirty) && c->nkeys == nt * (nt + 1) / 2 &&
+        (!c->prm.electrostatics || c->nsamples > 0) && (r = upload_tables(c)))
         return r;
     if (npairs) *npairs = (int64_t)c->h_flags[3] / 2;
     return 0;
@@ -868,7 +896,7 @@ int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs)
     for (int i = 0; i < n; i++)
         for (int g = 0; g < G; g++)
             for (int k = 0; k < std::min(cnt[(size_t)i * G + g], c->cap_lane); k++) {
-                const int j = nbr[list_slot(i, k, g, c->cap_lane)];
+                const int j = list_j(nbr[list_slot(i, k, g, c->cap_lane)]);
                 const int hi = gid[i], hj = gid[j];
                 if (hi >= hj) continue;
                 if (np < cap && pairs) {
@@ -977,7 +1005,9 @@ int dlpx_two_body_forces_noalloc(dlpx_ctx *ctx, const double cell[9], const dlpx
     c->red = scr;
     c->red_cap = kScratchRows;
     c->red_out = scr + (size_t)kScratchRows * kReduceSlots;
+    c->noalloc_now = true;  // no table upload: stale combined rows fall back to the two-table pair path
     r = evaluate(c);
+    c->noalloc_now = false;
     if (!r) r = store_atoms(c, c->stage, false, false);
     if (!r) {
         const hipError_t e = hipMemcpyAsync(c->h_st, c->stage, sizeof(double) * 9 * c->natms,

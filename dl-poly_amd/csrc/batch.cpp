@@ -80,6 +80,7 @@ struct Ensemble {
         for (auto &g : spme.fft.groups) batch_free(g.bl);
         spme.fft.groups.clear();
         for (auto &g : pair.groups) batch_free(g.bl);
+        batch_free(pair.prel);
         pair.groups.clear();
         list_batch_free(list);
         if (graph) (void)hipGraphExecDestroy(graph);

@@ -99,6 +99,15 @@ __host__ __device__ inline size_t list_slot(int i, int c, int lane, int cap_lane
 {
     return ((size_t)i * cap_lane + c) * kLanesPerAtom + lane;
 }
+// A list entry is j | code << 25: j the neighbour's sorted index (below
+// 2^25, the largest natms_max), code the subcell offset of j from i in the
+// subcell build, (dz + 2) 25 + (dy + 2) 5 + (dx + 2) for offsets -2..2 (0 from
+// the link-cell build).
+constexpr int kListCodeShift = 25;
+constexpr uint32_t kListJMask = (1u << kListCodeShift) - 1u;
+constexpr int kMaxAtoms = 1 << kListCodeShift;
+__host__ __device__ inline int list_j(int32_t e) { return (int)((uint32_t)e & kListJMask); }
+__host__ __device__ inline int list_code(int32_t e) { return (int)((uint32_t)e >> kListCodeShift); }
 
 // Columns of the per-evaluation partial-sum buffer (FP64, one row per block;
 // each column is written by exactly one kernel, which records its row count
@@ -138,6 +147,8 @@ struct Ctx {
     bool vdw_direct = false;         // vdws%l_direct: analytic forms per pair
     bool list_subcells = true;       // list build over half-edge subcells (needs >= 3 link cells per axis)
     bool fft_lines = true;           // compile-time-length line kernels for the FFT lengths that have them
+    bool rel_pairs = true;           // mixed combined-table pairs from subcell-relative FP32 positions
+    int comb_threads = 1024;         // workgroup size of the combined-table pair kernel (512 or 1024)
     int device = 0;
 
     // box (orthorhombic)
@@ -201,6 +212,9 @@ struct Ctx {
 
     // Verlet list: lane-interleaved chunks; element (c, i, g) at (c*cap + i)*G + g
     int cap_lane = 0;
+    bool list_coded = false;         // entries carry subcell offsets (the subcell build)
+    int32_t *sub_of = nullptr;       // per sorted atom: its subcell at the build, sx | sy << 10 | sz << 20
+    float4 *prel = nullptr;          // per sorted atom: position relative to that subcell (FP32), type bits
     int32_t *nbr = nullptr;
     int32_t *cnt = nullptr;          // cap*G per-lane counts
     int32_t *flags = nullptr;        // [0] overflow lane max, [1] ibig, [3] pairs, [4..5] displacement
@@ -213,6 +227,16 @@ struct Ctx {
     void *d_vdw = nullptr;           // nkeys x (max_grid+1) x {frc, pot}, FP64 or FP32
     void *d_erfc = nullptr;          // nsamples+1 x {deriv, erfc} (index 0 virtual)
     int32_t *d_keytab = nullptr;     // ntypes*ntypes -> key or -1
+    // mixed precision, charges fixed per atom type and one sample grid for the
+    // vdW and erfc tables: one combined row per type pair (pair.hip)
+    std::vector<double> type_q;      // charge per type of the last atom upload
+    bool q_per_type = false;         // every atom of a type has that type's charge
+    bool comb_dirty = true;          // the combined rows do not match tables and charges
+    bool comb_valid = false;         // d_comb holds the combined rows
+    bool noalloc_now = false;        // the evaluation under way may not allocate
+    void *d_comb = nullptr;          // ntypes*(ntypes+1)/2 rows x (max_grid+1) float4 samples
+    int32_t *d_comb_row = nullptr;   // ntypes*ntypes -> combined row
+    size_t comb_bytes = 0;
     void *d_vdw_direct = nullptr;    // vdw_method direct: kDirectStride per row, FP64 or FP32
 
     // SPME
@@ -309,9 +333,11 @@ struct PairBatch {
     struct Group {
         BatchLaunch bl;
         int gp = 8;
-        bool fs = false, direct = false, pp = false, gt = false;
+        bool fs = false, direct = false, pp = false, gt = false, comb = false, rel = false;
+        int nt = 512;
     };
     std::vector<Group> groups;
+    BatchLaunch prel;  // the relative positions of the systems whose pair kernel reads them
 };
 struct RedRowsHost {
     int rows[kReduceSlots];
@@ -381,6 +407,7 @@ void vdw_lrc(int ntypes, const double *num_type, const int *kinds, const double 
 
 // device stages (launch wrappers, one per .hip file)
 int upload_tables(Ctx *c);
+int upload_comb(Ctx *c);                     // the combined rows when the conditions hold (pair.hip)
 int build_list(Ctx *c);                      // from c->pos (current order), sorts & permutes
 int vnl_check_dev(Ctx *c, int *update);
 int pair_forces(Ctx *c);                     // writes c->frc and reduction slots

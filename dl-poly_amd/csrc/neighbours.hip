@@ -333,6 +333,7 @@ struct BuildArgs {
     int cap_lane;
     int32_t *nbr, *cnt, *flags;
     const int32_t *upd;
+    int32_t *sub_of;  // subcell build: each atom's subcell (see Ctx::sub_of)
 };
 template <int G>
 __device__ __forceinline__ void build_list_body(const BuildArgs &a, int c)
@@ -525,6 +526,7 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
     const int cap_lane = a.cap_lane;
     int32_t *__restrict__ nbr = a.nbr;
     int32_t *__restrict__ cnt = a.cnt;
+    int32_t *__restrict__ sub_of = a.sub_of;
     if (b >= g.nc[0] * g.nc[1] * g.nc[2]) return;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     constexpr int NW = kListThreads / 64;
@@ -619,8 +621,10 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
                         if (k < m) {
                             const int q = qk[u];
                             const int qx = q % kSubSide, qy = (q / kSubSide) % kSubSide, qz = q / (kSubSide * kSubSide);
+                            // w: j and the candidate's staged x subcell (for the entry's code)
                             cand[k] = make_float4((float)(pk[u].x + s_sh[0][qx]), (float)(pk[u].y + s_sh[1][qy]),
-                                                  (float)(pk[u].z + s_sh[2][qz]), __int_as_float(jk[u]));
+                                                  (float)(pk[u].z + s_sh[2][qz]),
+                                                  __int_as_float(jk[u] | (qx << kListCodeShift)));
                         }
                     }
                 }
@@ -638,6 +642,8 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
 #pragma unroll
                 for (int k = 1; k < 8; k++) l += s_lo[k] <= i;
                 const int lx = l & 1, ly = (l >> 1) & 1, lz = l >> 2;
+                if (r0 == 0 && lane == 0)
+                    sub_of[i] = (2 * bc[0] + lx) | ((2 * bc[1] + ly) << 10) | ((2 * bc[2] + lz) << 20);
                 const float pfx = (float)pi.x, pfy = (float)pi.y, pfz = (float)pi.z;
                 // lane r < 25 holds the staged candidate range of row r (its 5
                 // subcells in x at dz = r / 5, dy = r % 5), relative to the round
@@ -652,8 +658,12 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
                 // one chunk of up to 64 candidates; accepted ones are packed into the
                 // next list entries (an overflowing list is rebuilt, so entries past
                 // the capacity all land on its last slot)
-                auto test = [&](const float4 e, const bool v) {
-                    const int j = __float_as_int(e.w);
+                // an accepted candidate of row `row` is stored as j | code << 25, its
+                // subcell offset from i's: dz + 2 = row / 5, dy + 2 = row % 5,
+                // dx + 2 = (staged x subcell) - lx
+                auto test = [&](const float4 e, const bool v, const int row) {
+                    const int w = __float_as_int(e.w);
+                    const int j = list_j(w);
                     const float ex = pfx - e.x, ey = pfy - e.y, ez = pfz - e.z;
                     const float r2f = ex * ex + ey * ey + ez * ez;
                     bool in = v && r2f < rc2_lo && j != i;
@@ -674,7 +684,7 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
                     const unsigned long long acc = __ballot(in);
                     const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(acc >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((unsigned)acc, 0u));
-                    if (in) row_out[min(count + pre, cap_tot - 1)] = j;
+                    if (in) row_out[min(count + pre, cap_tot - 1)] = j | ((row * 5 + list_code(w) - lx) << kListCodeShift);
                     count += __popcll(acc);
                 };
                 // five rows at a time: their first chunks' LDS reads are issued together
@@ -691,12 +701,12 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
                         e[u] = cand[v[u] ? lo[u] + lane : 0];
                     }
 #pragma unroll
-                    for (int u = 0; u < 5; u++) test(e[u], v[u]);
+                    for (int u = 0; u < 5; u++) test(e[u], v[u], g5 + u);
 #pragma unroll
                     for (int u = 0; u < 5; u++)
                         for (int k0 = lo[u] + 64; k0 < hi[u]; k0 += 64) {
                             const bool vv = k0 + lane < hi[u];
-                            test(cand[vv ? k0 + lane : 0], vv);
+                            test(cand[vv ? k0 + lane : 0], vv, g5 + u);
                         }
                 }
                 if (!last) {
@@ -763,11 +773,11 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
             const int m = min(a.cnt[t], a.cap_lane);
             int w = 0;
             for (int k = 0; k < m; k++) {
-                const int j = a.nbr[list_slot(i, k, g, a.cap_lane)];
-                const int hj = a.gid[j];
+                const int32_t en = a.nbr[list_slot(i, k, g, a.cap_lane)];
+                const int hj = a.gid[list_j(en)];
                 bool ex = fi && a.frz[hj] != 0;
                 for (int e = e0; e < e1; e++) ex = ex || a.excl_ids[e] == hj;
-                if (!ex) a.nbr[list_slot(i, w++, g, a.cap_lane)] = j;
+                if (!ex) a.nbr[list_slot(i, w++, g, a.cap_lane)] = en;
             }
             dropped = m - w;
             if (a.cnt[t] <= a.cap_lane) a.cnt[t] = w;  // an overflowed lane is rebuilt anyway
@@ -909,7 +919,8 @@ static int list_chain(Ctx *c, const int32_t *upd, ListChain &lc)
                        c->pos_alt, c->pos, c->typ_alt, c->typ, c->gid_alt, c->gid,
                        res ? c->vel_alt : nullptr, res ? c->vel : nullptr, c->slot, c->xref};
     lc.build = BuildArgs{c->pos, g, sub ? c->key_start : c->cell_start, rc_ext * rc_ext, c->cap_lane,
-                         c->nbr, c->cnt, c->flags, upd};
+                         c->nbr, c->cnt, c->flags, upd, c->sub_of};
+    c->list_coded = sub;
     lc.excl_on = c->excl_total > 0 || c->nfrozen > 1;
     lc.excl = ExclListArgs{n,      c->cap_lane, c->nbr, c->cnt, c->gid,
                            c->excl_total > 0 ? c->excl_off : nullptr, c->excl_ids,

@@ -287,12 +287,16 @@ struct StepArgs {
     int32_t *flags;
 };
 __device__ __forceinline__ bool atomic_column(int col) { return col == R_KIN || (col >= R_ERECIP && col <= R_FNETZ); }
+// 256 threads: one atom per thread in workgroups small enough to spread a
+// 64k-atom system over every CU (the partial-sum columns go to the first
+// workgroups, one column each)
+constexpr int kStepThreads = 256;
 template <bool S2, bool S1>
 __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
 {
-    __shared__ double lsum[kVv1Threads];
-    __shared__ double wmax[kVv1Threads / 64];
-    __shared__ double lred[kVv1Threads / 64];
+    __shared__ double lsum[kStepThreads];
+    __shared__ double wmax[kStepThreads / 64];
+    __shared__ double lred[kStepThreads / 64];
     __shared__ double recip[4];  // reciprocal energy and net force (unscaled)
     __shared__ bool last;
     const int t = threadIdx.x;
@@ -315,7 +319,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
         // the columns the step's kernels filled with partial rows
         for (int col = bid; col < kReduceSlots; col += nb) {
             if (atomic_column(col)) continue;
-            const double v = column_sum<kVv1Threads>(a.red, a.rows[col], kReduceSlots, col, lsum);
+            const double v = column_sum<kStepThreads>(a.red, a.rows[col], kReduceSlots, col, lsum);
             if (t == 0) {
                 a.red_out[col] = v;
                 if (record) record[col] = v;
@@ -425,7 +429,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a, int bid, int nb)
     __hip_atomic_store((unsigned int *)(a.flags + F_DONE), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <bool S2, bool S1>
-__global__ void __launch_bounds__(kVv1Threads) k_step_batch(const StepArgs *__restrict__ av,
+__global__ void __launch_bounds__(kStepThreads) k_step_batch(const StepArgs *__restrict__ av,
                                                             const int32_t *__restrict__ boff, int nsys)
 {
     const int lb = blockIdx.x;  // atoms in order: no XCD renumbering
@@ -434,7 +438,7 @@ __global__ void __launch_bounds__(kVv1Threads) k_step_batch(const StepArgs *__re
     step_body<S2, S1>(av[s], lb - boff[s], boff[s + 1] - boff[s]);
 }
 
-int step_blocks(const Ctx *c) { return ceil_div(c->natms, kVv1Threads); }
+int step_blocks(const Ctx *c) { return ceil_div(c->natms, kStepThreads); }
 
 int step_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &rows, bool energies,
                  BatchLaunch &bl)
@@ -475,7 +479,7 @@ int step_prepare(const std::vector<Ctx *> &cs, const std::vector<RedRowsHost> &r
 
 int step_launch(Ctx *c, hipStream_t s, const BatchLaunch &bl, bool stage2, bool stage1)
 {
-    const dim3 g(bl.total), b(kVv1Threads);
+    const dim3 g(bl.total), b(kStepThreads);
     const StepArgs *av = (const StepArgs *)bl.args;
     if (stage2 && stage1) hipLaunchKernelGGL((k_step_batch<true, true>), g, b, 0, s, av, bl.boff, bl.nsys);
     else if (stage2) hipLaunchKernelGGL((k_step_batch<true, false>), g, b, 0, s, av, bl.boff, bl.nsys);

@@ -319,6 +319,9 @@ __device__ __forceinline__ void vdw_form(int kind, const R *p, R r, R &e, R &g)
 // wavefronts per CU behind one copy to hide the latency of the neighbour
 // position gathers.
 constexpr int kPairThreads = 512;
+// the combined-table kernel: 1024 threads, so the two workgroups its LDS rows
+// leave room for on a CU hold 8 wavefronts per SIMD
+constexpr int kCombThreads = 1024;
 
 // GP lanes serve one atom (a multiple of the list's kLanesPerAtom lanes):
 // pair lane p walks list lane p % kLanesPerAtom, entries c = p / kLanesPerAtom,
@@ -349,6 +352,10 @@ struct PairArgs {
     double *red;
     const int32_t *gid;
     double *ppo;
+    const float4 *comb_g;        // combined rows (mixed precision, see k_pair_comb)
+    const int32_t *comb_row_g;   // ntypes*ntypes -> combined row
+    const float4 *prel;          // relative positions (k_pair_comb<REL>)
+    float hs[3], hs_lo[3];       // subcell edge: its FP32 rounding and the remainder
 };
 template <typename R, int GP, int NT, bool FS, bool DIRECT, bool PP = false, bool GT = false>
 __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
@@ -413,22 +420,26 @@ __device__ __forceinline__ void pair_body(const PairArgs<R> &a, int bid)
         // software pipeline: the neighbour index is loaded two iterations
         // ahead and its position and type one iteration ahead, so neither
         // load's latency sits in front of the pair arithmetic
-        int jn = c0 < m ? lst[(size_t)c0 * G] : i;
-        double4 pn = pos[jn];
-        int tn = typ[jn];
-        int gn = PP ? gid[jn] : 0;
+        // The prefetches are unconditional, from the lane's slot clamped to its
+        // last entry (always a real neighbour): a conditional load makes the
+        // compiler wait for it where the branches join, right after issuing
+        // it.  The entry is kept raw and masked where it is used, for the same
+        // reason.  A lane without entries skips the loop and its prologue.
+        const int mm = m > 0 ? m - 1 : 0;
+        int32_t jn = m > 0 ? lst[(size_t)min(c0, mm) * G] : i;
+        double4 pn = pos[list_j(jn)];
+        int tn = typ[list_j(jn)];
+        int gn = PP ? gid[list_j(jn)] : 0;
         const int gi = PP ? gid[i] : 0;
-        jn = c0 + CS < m ? lst[(size_t)(c0 + CS) * G] : i;
+        jn = m > 0 ? lst[(size_t)min(c0 + CS, mm) * G] : i;
         for (int c = c0; c < m; c += CS) {
             const double4 pj = pn;
             const int tj = tn;
             const int gj = gn;
-            if (c + CS < m) {
-                pn = pos[jn];
-                tn = typ[jn];
-                if (PP) gn = gid[jn];
-                if (c + 2 * CS < m) jn = lst[(size_t)(c + 2 * CS) * G];
-            }
+            pn = pos[list_j(jn)];
+            tn = typ[list_j(jn)];
+            if (PP) gn = gid[list_j(jn)];
+            jn = lst[(size_t)min(c + 2 * CS, mm) * G];
             const double dxd = min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]);
             const double dyd = min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]);
             const double dzd = min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]);
@@ -528,15 +539,326 @@ __global__ void __launch_bounds__(NT) k_pair_batch(const PairArgs<R> *__restrict
     pair_body<R, GP, NT, FS, DIRECT, PP, GT>(a, bp.b);
 }
 
+// The mixed-precision pair kernel for the common case of charges fixed per
+// atom type, tabulated vdW without force-shifting, and vdW and erfc tables on
+// one sample grid (rc = rvdw): one row per type pair holds, per sample r_l,
+//   x: H = Fv + q_a q_b r_l^2 Fe, the pair's force factor times r^2
+//      (f = H d / r^2, as the vdW table's Fv = r dU/dr),
+//   y: Fv (the vdW virial),
+//   z: Ev, the vdW energy,  w: Ec = q_a q_b erfc(alpha r)/r (scaled),
+// so a pair costs one table position and three 16-byte LDS reads instead of
+// two lookups.  The 3-point Newton interpolation of numerics.F90:250-290 is
+// written as a + p (b - a) + p (p - 1)/2 (a - 2b + c) (the same polynomial).
+// The vdW part of H interpolates exactly as the vdW table does; the Coulomb
+// part interpolates the smooth r^2 Fe instead of Fe, which moves a force by
+// the erfc table's interpolation error (~1e-8), inside the mixed bar.
+//
+// REL: positions come from prel, each atom's position relative to its
+// subcell at the list build (FP32) with its type in w, and a list entry's
+// code gives the subcell offset d of j from i, so x_i - x_j = (xr_i - xr_j)
+// - d h: one 16-byte gather per pair and no FP64 minimum image.  d h is
+// applied as d h_hi + d h_lo (h_hi the FP32 rounding of the subcell edge,
+// h_lo its remainder): a rounded d h alone would move every pair of one code
+// by the same amount, which sums into the virial instead of averaging out.
+// What remains is the unbiased FP32 rounding of the relative positions and
+// of the sums (below 5e-7 A each for subcells of a few A).
+template <int GP, int NT, bool REL>
+__device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid)
+{
+    const double4 *__restrict__ pos = a.pos;
+    const int32_t *__restrict__ typ = a.typ;
+    const int n = a.n;
+    const int32_t *__restrict__ nbr = a.nbr;
+    const int32_t *__restrict__ cnt = a.cnt;
+    const PairParams &pp = a.pp;
+    double4 *__restrict__ frc = a.frc;
+    extern __shared__ __align__(16) unsigned char smem[];
+    // LDS: (REL) the 125 subcell offsets -d h_hi and -d h_lo, each padded to
+    // 128; the combined rows; the row table; reduction scratch
+    float4 *offt = (float4 *)smem;
+    float4 *tc = offt + (REL ? 256 : 0);
+    int32_t *kt = (int32_t *)(tc + (size_t)pp.nkeys * pp.mg1);
+    double *lred = (double *)(kt + ((pp.ntypes * pp.ntypes + 1) & ~1));
+    for (int k = threadIdx.x; k < pp.nkeys * pp.mg1; k += blockDim.x) tc[k] = a.comb_g[k];
+    for (int k = threadIdx.x; k < pp.ntypes * pp.ntypes; k += blockDim.x) kt[k] = a.comb_row_g[k];
+    if (REL && threadIdx.x < 125) {
+        const int k = threadIdx.x;
+        const float d[3] = {(float)(k % 5 - 2), (float)((k / 5) % 5 - 2), (float)(k / 25 - 2)};
+        offt[k] = make_float4(-d[0] * a.hs[0], -d[1] * a.hs[1], -d[2] * a.hs[2], 0.0f);  // exact: |d| <= 2
+        offt[128 + k] = make_float4(-d[0] * a.hs_lo[0], -d[1] * a.hs_lo[1], -d[2] * a.hs_lo[2], 0.0f);
+    }
+    __syncthreads();
+
+    constexpr int G = kLanesPerAtom, CS = GP / G;
+    static_assert(GP % G == 0 && GP <= 64, "GP must be a multiple of the list lanes");
+    const int tid = bid * blockDim.x + threadIdx.x;
+    const int i = tid / GP, p = tid % GP, lane = p % G, c0 = p / G;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    // engvdw, virvdw, engrl, (unused), s11..s33 as in pair_body
+    float pacc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const int m = min(cnt[(size_t)i * G + lane], pp.cap_lane);
+        const float rdr = (float)pp.rdr_v, rdr_lo = (float)(pp.rdr_v - (double)rdr);
+        const float rmax2 = (float)pp.rmax2;
+        const int32_t *lst = nbr + list_slot(i, 0, lane, pp.cap_lane);
+        // software pipeline: pair_body's (entry two iterations ahead, position
+        // and type one ahead); REL, whose iterations are shorter, one deeper
+        // (entry three ahead, relative position two ahead)
+        // (prefetches unconditional from clamped slots, entries masked where
+        // used: see pair_body)
+        const int mm = m > 0 ? m - 1 : 0;
+        double4 pi, pn;
+        float4 qi, qa, qb;
+        int tn = 0, ka = 0, kb = 0;
+        int32_t en;
+        if constexpr (REL) {
+            qi = a.prel[i];
+            const int32_t e0 = m > 0 ? lst[(size_t)min(c0, mm) * G] : i;
+            const int32_t e1 = m > 0 ? lst[(size_t)min(c0 + CS, mm) * G] : i;
+            en = m > 0 ? lst[(size_t)min(c0 + 2 * CS, mm) * G] : i;
+            qa = a.prel[list_j(e0)];
+            ka = list_code(e0);
+            qb = a.prel[list_j(e1)];
+            kb = list_code(e1);
+        } else {
+            pi = pos[i];
+            const int32_t e0 = m > 0 ? lst[(size_t)min(c0, mm) * G] : i;
+            pn = pos[list_j(e0)];
+            tn = typ[list_j(e0)];
+            en = m > 0 ? lst[(size_t)min(c0 + CS, mm) * G] : i;
+        }
+        const int32_t *ktrow = kt + (REL ? __float_as_int(qi.w) : typ[i]) * pp.ntypes;
+        // one pair from its separation and j's type
+        auto term = [&](const float dx, const float dy, const float dz, const int tj) {
+            const float rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq >= rmax2) return;
+            float rrr, rinv2;
+            r_rinv2(rsq, rrr, rinv2);
+            const float4 *row = tc + ktrow[tj] * pp.mg1;
+            const float s = table_coord(rrr, rdr, rdr_lo);
+            const int l = (int)s;
+            const float q1 = s - (float)l;
+            float4 A = row[l];
+            const float4 B = row[l + 1], C = row[l + 2];
+            if (l == 0) {
+                // numerics.F90:262: the first sample scaled by r
+                A.x *= rrr;
+                A.y *= rrr;
+                A.z *= rrr;
+                A.w *= rrr;
+            }
+            const float q2 = 0.5f * q1 * (q1 - 1.0f);
+            const float gam = rinv2 * __builtin_fmaf(q2, __builtin_fmaf(-2.0f, B.x, A.x + C.x),
+                                                     __builtin_fmaf(q1, B.x - A.x, A.x));
+            const float fv = __builtin_fmaf(q2, __builtin_fmaf(-2.0f, B.y, A.y + C.y), __builtin_fmaf(q1, B.y - A.y, A.y));
+            const float ev = __builtin_fmaf(q2, __builtin_fmaf(-2.0f, B.z, A.z + C.z), __builtin_fmaf(q1, B.z - A.z, A.z));
+            const float ec = __builtin_fmaf(q2, __builtin_fmaf(-2.0f, B.w, A.w + C.w), __builtin_fmaf(q1, B.w - A.w, A.w));
+            pacc[0] += ev;
+            pacc[1] -= fv;
+            pacc[2] += ec;
+            const float fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
+            fx += (double)fxp;
+            fy += (double)fyp;
+            fz += (double)fzp;
+            pacc[4] += dx * fxp;
+            pacc[5] += dx * fyp;
+            pacc[6] += dx * fzp;
+            pacc[7] += dy * fyp;
+            pacc[8] += dy * fzp;
+            pacc[9] += dz * fzp;
+        };
+        if constexpr (REL) {
+            // two entries per iteration, c and c + CS, each with its relative
+            // position loaded two entries ahead (qa, qb) and its list entry
+            // four ahead (ea, eb), all unconditional from clamped slots, so a
+            // gather has two pairs' work to hide behind
+            int32_t ea = en, eb = m > 0 ? lst[(size_t)min(c0 + 3 * CS, mm) * G] : i;
+            auto rel_term = [&](const float4 qj, const int code) {
+                const float4 of = offt[code], ol = offt[128 + code];
+                term(((qi.x - qj.x) + of.x) + ol.x, ((qi.y - qj.y) + of.y) + ol.y, ((qi.z - qj.z) + of.z) + ol.z,
+                     __float_as_int(qj.w));
+            };
+            for (int c = c0; c < m; c += 2 * CS) {
+                const float4 q0 = qa, q1 = qb;
+                const int k0 = ka, k1 = kb;
+                qa = a.prel[list_j(ea)];
+                ka = list_code(ea);
+                qb = a.prel[list_j(eb)];
+                kb = list_code(eb);
+                ea = lst[(size_t)min(c + 4 * CS, mm) * G];
+                eb = lst[(size_t)min(c + 5 * CS, mm) * G];
+                rel_term(q0, k0);
+                if (c + CS < m) rel_term(q1, k1);
+            }
+        } else {
+            for (int c = c0; c < m; c += CS) {
+                const double4 pj = pn;
+                const int tj = tn;
+                pn = pos[list_j(en)];
+                tn = typ[list_j(en)];
+                en = lst[(size_t)min(c + 2 * CS, mm) * G];
+                term((float)min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]),
+                     (float)min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]),
+                     (float)min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]), tj);
+            }
+        }
+    }
+    fx = group_sum<GP>(fx);
+    fy = group_sum<GP>(fy);
+    fz = group_sum<GP>(fz);
+    if (i < n && p == 0) frc[i] = make_double4(fx, fy, fz, 0.0);
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.5 * (double)pacc[k];  // every pair is met twice
+    __syncthreads();
+    block_sum<10>(acc, lred, a.red + (size_t)bid * kReduceSlots + R_ENGVDW);
+}
+template <int GP, int NT, bool REL>
+__global__ void __launch_bounds__(NT) k_pair_comb(PairArgs<float> a)
+{
+    pair_comb_body<GP, NT, REL>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <int GP, int NT, bool REL>
+__global__ void __launch_bounds__(NT) k_pair_comb_batch(const PairArgs<float> *__restrict__ av,
+                                                         const int32_t *__restrict__ boff, int nsys)
+{
+    BatchPos bp;
+    if (!batch_pos(boff, nsys, bp)) return;
+    pair_comb_body<GP, NT, REL>(av[bp.s], bp.b);
+}
+
+// Positions relative to each atom's subcell at the list build (FP32), with
+// the type in w, for k_pair_comb<REL>: u = x - o, o = -L/2 + s h the subcell
+// origin (the binning of cell_coord), folded to the image nearest the
+// subcell (positions need not be folded into the cell between builds).
+struct PrelArgs {
+    const double4 *pos;
+    const int32_t *typ, *sub_of;
+    float4 *prel;
+    int n;
+    double L[3], invL[3], h[3];
+};
+__device__ __forceinline__ void prel_body(const PrelArgs &a, int bid)
+{
+    const int i = bid * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const double4 p = a.pos[i];
+    const int s = a.sub_of[i];
+    const int sc[3] = {s & 1023, (s >> 10) & 1023, s >> 20};
+    const double x[3] = {p.x, p.y, p.z};
+    float u[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double v = x[d] - (sc[d] * a.h[d] - 0.5 * a.L[d]);
+        v -= a.L[d] * rint((v - 0.5 * a.h[d]) * a.invL[d]);
+        u[d] = (float)v;
+    }
+    a.prel[i] = make_float4(u[0], u[1], u[2], __int_as_float(a.typ[i]));
+}
+__global__ void __launch_bounds__(256) k_prel(PrelArgs a) { prel_body(a, blockIdx.x); }
+__global__ void __launch_bounds__(256) k_prel_batch(const PrelArgs *__restrict__ av, const int32_t *__restrict__ boff,
+                                                    int nsys)
+{
+    const int lb = blockIdx.x;
+    if (lb >= boff[nsys]) return;
+    const int s = batch_find(boff, nsys, lb);
+    prel_body(av[s], lb - boff[s]);
+}
+
+static PrelArgs prel_args(const Ctx *c)
+{
+    PrelArgs a;
+    a.pos = c->pos;
+    a.typ = c->typ;
+    a.sub_of = c->sub_of;
+    a.prel = c->prel;
+    a.n = c->natms;
+    for (int d = 0; d < 3; d++) {
+        a.L[d] = c->L[d];
+        a.invL[d] = c->invL[d];
+        a.h[d] = c->L[d] / (2.0 * c->nc[d]);  // subcell edge
+    }
+    return a;
+}
+
+// The combined rows (see pair_comb_body) when the context qualifies: mixed
+// precision, electrostatics, charges fixed per type, tabulated vdW and erfc
+// tables on one grid (equal cutoffs and sample counts).  Built at set-up (the
+// list build, resident-run start) from the host tables in FP64, then rounded.
+int upload_comb(Ctx *c)
+{
+    c->comb_dirty = false;
+    c->comb_valid = false;
+    const int nt = c->prm.ntypes, mg1 = c->max_grid + 1;
+    if (c->prm.precision != DLPX_MIXED || !c->prm.electrostatics || !c->q_per_type || c->max_grid != c->nsamples ||
+        c->prm.cutoff != c->prm.vdw_cutoff || c->nkeys != nt * (nt + 1) / 2 || (int)c->type_q.size() != nt)
+        return 0;
+    const int nrows = nt * (nt + 1) / 2;
+    const size_t bytes = sizeof(float4) * (size_t)nrows * mg1;
+    // the rows must fit the LDS beside the row table and the reduction scratch
+    if (bytes + sizeof(float4) * 256 + sizeof(int32_t) * (nt * nt + 1) + sizeof(double) * 10 * (kCombThreads / 64) >
+        160 * 1024)
+        return 0;
+    const double scaling = kR4pie0 / c->prm.epsilon, dr = c->prm.cutoff / (double)(c->max_grid - 4);
+    std::vector<float4> rows((size_t)nrows * mg1);
+    std::vector<int32_t> rt((size_t)nt * nt);
+    for (int a = 1; a <= nt; a++)
+        for (int b = 1; b <= a; b++) {
+            const int key = a * (a - 1) / 2 + b - 1;  // the reference's pair key of types (a, b)
+            rt[(size_t)(a - 1) * nt + (b - 1)] = rt[(size_t)(b - 1) * nt + (a - 1)] = key;
+            const double qq = c->type_q[a - 1] * c->type_q[b - 1] * scaling;
+            const bool vdw = c->ltp[key] >= 0;  // vdw.F90:1883: an undefined pair has no vdW part
+            for (int l = 0; l < mg1; l++) {
+                // sample l at r = l dr; erfc samples 1.. are the reference's erfc(1:) (index 0 virtual)
+                const double fv = vdw ? c->vdw_frc[(size_t)key * mg1 + l] : 0.0;
+                const double ev = vdw ? c->vdw_pot[(size_t)key * mg1 + l] : 0.0;
+                const double fe = l > 0 ? c->erfc_d[l - 1] : 0.0, ee = l > 0 ? c->erfc_t[l - 1] : 0.0;
+                const double r = l * dr;
+                const double g = fv + qq * (r * r) * fe;
+                auto fin = [](double x) { return std::isfinite(x) ? (float)std::max(-3.0e38, std::min(3.0e38, x)) : 3.0e38f; };
+                float4 v;
+                v.x = fin(g);
+                v.y = fin(fv);
+                v.z = fin(ev);
+                v.w = fin(qq * ee);
+                rows[(size_t)key * mg1 + l] = v;
+            }
+        }
+    if (c->comb_bytes != bytes) {
+        if (c->d_comb) (void)hipFree(c->d_comb);
+        c->d_comb = nullptr;
+        c->comb_bytes = 0;
+        DLPX_HIP(c, hipMalloc(&c->d_comb, bytes));
+        c->comb_bytes = bytes;
+    }
+    if (!c->d_comb_row) DLPX_HIP(c, hipMalloc((void **)&c->d_comb_row, sizeof(int32_t) * (size_t)nt * nt));
+    DLPX_HIP(c, dev_upload(c->stream, c->d_comb, rows.data(), bytes));
+    DLPX_HIP(c, dev_upload(c->stream, c->d_comb_row, rt.data(), sizeof(int32_t) * rt.size()));
+    c->comb_valid = true;
+    return 0;
+}
+
+// The combined-table kernel runs for this context's evaluations.
+static int comb_threads(const Ctx *c) { return c->comb_threads == 512 ? 512 : kCombThreads; }
+static bool comb_ok(const Ctx *c)
+{
+    return c->prm.precision == DLPX_MIXED && c->comb_valid && !c->comb_dirty && !c->vdw_direct &&
+           !c->vdw_force_shift && !c->pp_now;
+}
+
 // The pair kernel's arguments, dynamic LDS and template variant for one
 // context.  Variant: lanes per atom (4, 8, 16), force-shifting, direct
 // evaluation, per-particle collection, vdW tables from global memory.
 struct PairVariant {
     int gp;
     bool fs, direct, pp, gt;
+    bool comb = false;  // k_pair_comb
+    bool rel = false;   // k_pair_comb<REL>
+    int nt = kPairThreads;  // workgroup size
     bool operator==(const PairVariant &o) const
     {
-        return gp == o.gp && fs == o.fs && direct == o.direct && pp == o.pp && gt == o.gt;
+        return gp == o.gp && fs == o.fs && direct == o.direct && pp == o.pp && gt == o.gt && comb == o.comb &&
+               rel == o.rel && nt == o.nt;
     }
 };
 
@@ -576,7 +898,25 @@ static int pair_setup(Ctx *c, PairArgs<R> &a, size_t &lds, PairVariant &v, int l
     a.red = c->red;
     a.gid = c->gid;
     a.ppo = c->pp;
+    a.comb_g = (const float4 *)c->d_comb;
+    a.comb_row_g = c->d_comb_row;
+    a.prel = c->prel;
+    for (int d = 0; d < 3; d++) {
+        const double h = c->L[d] / (2.0 * c->nc[d]);
+        a.hs[d] = (float)h;
+        a.hs_lo[d] = (float)(h - (double)a.hs[d]);
+    }
     const bool direct = c->vdw_direct;
+    if (sizeof(R) == 4 && comb_ok(c)) {
+        const int nt = pp.ntypes;
+        const bool rel = c->list_coded && c->rel_pairs;
+        pp.nkeys = nt * (nt + 1) / 2;  // combined rows
+        lds = (rel ? sizeof(float4) * 256 : 0) + sizeof(float4) * (size_t)pp.nkeys * pp.mg1 +
+              sizeof(int32_t) * ((nt * nt + 1) & ~1) + sizeof(double) * 10 * (kCombThreads / 64);
+        v = PairVariant{lanes, false, false, false, false, true, rel};
+        v.nt = comb_threads(c);
+        return 0;
+    }
     const size_t lds_rest = sizeof(R2) * (pp.electro ? pp.ns1 : 0) + sizeof(int32_t) * ((pp.ntypes * pp.ntypes + 1) & ~1) +
                             (direct ? sizeof(R) * kDirectStride * pp.nkeys : 0) + sizeof(double) * 10 * (kPairThreads / 64);
     // the vdW tables go to LDS when they fit in its 160 KiB with the rest,
@@ -625,6 +965,33 @@ static int launch_pair(Ctx *c)
     PairVariant v;
     int r;
     if ((r = pair_setup<R>(c, a, lds, v, c->pair_lanes))) return r;
+    if constexpr (sizeof(R) == 4) {
+        if (v.comb) {
+            if (v.rel)
+                hipLaunchKernelGGL(k_prel, dim3(ceil_div(c->natms, 256)), dim3(256), 0, c->stream, prel_args(c));
+#define DLPX_PAIRC(GP, REL, NT)                                                                               \
+    do {                                                                                                      \
+        DLPX_LDS_MAX(c, (k_pair_comb<GP, NT, REL>));                                                          \
+        hipLaunchKernelGGL((k_pair_comb<GP, NT, REL>), dim3(pair_blocks(c)), dim3(NT), lds, c->stream, a);    \
+    } while (0)
+            if (v.nt == 512) {
+                if (v.rel) DLPX_PAIRC(8, true, 512);
+                else DLPX_PAIRC(8, false, 512);
+            } else {
+                switch (v.gp * 2 + (v.rel ? 1 : 0)) {
+                    case 33: DLPX_PAIRC(16, true, kCombThreads); break;
+                    case 32: DLPX_PAIRC(16, false, kCombThreads); break;
+                    case 17: DLPX_PAIRC(8, true, kCombThreads); break;
+                    case 16: DLPX_PAIRC(8, false, kCombThreads); break;
+                    case 9: DLPX_PAIRC(4, true, kCombThreads); break;
+                    default: DLPX_PAIRC(4, false, kCombThreads); break;
+                }
+            }
+#undef DLPX_PAIRC
+            DLPX_HIP(c, hipGetLastError());
+            return 0;
+        }
+    }
 #define DLPX_PAIR1(GP, FS, DIR, PPK, GTK)                                                                \
     do {                                                                                                 \
         DLPX_LDS_MAX(c, (k_pair<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                                \
@@ -667,7 +1034,7 @@ static int pair_prepare_t(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes
             lds_max.push_back(0);
         }
         args[g].push_back(a);
-        nbs[g].push_back(ceil_div((long)c->natms * v.gp, kPairThreads));
+        nbs[g].push_back(ceil_div((long)c->natms * v.gp, v.nt));
         rows.push_back(nbs[g].back());
         lds_max[g] = std::max(lds_max[g], lds);
     }
@@ -679,10 +1046,25 @@ static int pair_prepare_t(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes
         gr.direct = vs[g].direct;
         gr.pp = vs[g].pp;
         gr.gt = vs[g].gt;
+        gr.comb = vs[g].comb;
+        gr.rel = vs[g].rel;
+        gr.nt = vs[g].nt;
         if ((r = batch_upload(cs[0], gr.bl, args[g].data(), sizeof(PairArgs<R>), nbs[g], true))) return r;
         gr.bl.lds = lds_max[g];
     }
-    return 0;
+    // relative positions for the systems of REL groups
+    std::vector<PrelArgs> pa;
+    std::vector<int> pnb;
+    for (size_t k = 0; k < cs.size(); k++) {
+        Ctx *c = cs[k];
+        PairArgs<R> a;
+        size_t lds;
+        PairVariant v;
+        if ((r = pair_setup<R>(c, a, lds, v, lanes))) return r;
+        pa.push_back(prel_args(c));
+        pnb.push_back(v.rel ? ceil_div(c->natms, 256) : 0);
+    }
+    return batch_upload(cs[0], pb.prel, pa.data(), sizeof(PrelArgs), pnb, false);
 }
 
 int pair_prepare(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::vector<int> &rows)
@@ -695,9 +1077,38 @@ int pair_prepare(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::ve
 template <typename R>
 static int pair_launch_t(Ctx *c, hipStream_t s, const PairBatch &pb)
 {
+    if (pb.prel.total > 0)
+        hipLaunchKernelGGL(k_prel_batch, dim3(pb.prel.total), dim3(256), 0, s, (const PrelArgs *)pb.prel.args,
+                           pb.prel.boff, pb.prel.nsys);
     for (const PairBatch::Group &gr : pb.groups) {
-        const PairVariant v{gr.gp, gr.fs, gr.direct, gr.pp, gr.gt};
+        PairVariant v{gr.gp, gr.fs, gr.direct, gr.pp, gr.gt, gr.comb, gr.rel};
+        v.nt = gr.nt;
         const size_t lds = gr.bl.lds;
+        if constexpr (sizeof(R) == 4) {
+            if (v.comb) {
+#define DLPX_PAIRCB(GP, REL, NT)                                                                                  \
+    do {                                                                                                          \
+        DLPX_LDS_MAX(c, (k_pair_comb_batch<GP, NT, REL>));                                                         \
+        hipLaunchKernelGGL((k_pair_comb_batch<GP, NT, REL>), dim3(gr.bl.total), dim3(NT), lds, s,                  \
+                           (const PairArgs<float> *)gr.bl.args, gr.bl.boff, gr.bl.nsys);                          \
+    } while (0)
+                if (v.nt == 512) {
+                    if (v.rel) DLPX_PAIRCB(8, true, 512);
+                    else DLPX_PAIRCB(8, false, 512);
+                } else {
+                    switch (v.gp * 2 + (v.rel ? 1 : 0)) {
+                        case 33: DLPX_PAIRCB(16, true, kCombThreads); break;
+                        case 32: DLPX_PAIRCB(16, false, kCombThreads); break;
+                        case 17: DLPX_PAIRCB(8, true, kCombThreads); break;
+                        case 16: DLPX_PAIRCB(8, false, kCombThreads); break;
+                        case 9: DLPX_PAIRCB(4, true, kCombThreads); break;
+                        default: DLPX_PAIRCB(4, false, kCombThreads); break;
+                    }
+                }
+#undef DLPX_PAIRCB
+                continue;
+            }
+        }
 #define DLPX_PAIRB(GP, FS, DIR, PPK, GTK)                                                                   \
     do {                                                                                                    \
         DLPX_LDS_MAX(c, (k_pair_batch<R, GP, kPairThreads, FS, DIR, PPK, GTK>));                             \
@@ -718,7 +1129,10 @@ int pair_launch(Ctx *c, hipStream_t s, const PairBatch &pb)
     return pair_launch_t<double>(c, s, pb);
 }
 
-int pair_blocks_exact(const Ctx *c) { return ceil_div((long)c->natms * c->pair_lanes, kPairThreads); }
+int pair_blocks_exact(const Ctx *c)
+{
+    return ceil_div((long)c->natms * c->pair_lanes, comb_ok(c) ? comb_threads(c) : kPairThreads);
+}
 int pair_blocks(const Ctx *c) { return xcd_grid(pair_blocks_exact(c)); }
 
 // Ewald correction of the excluded pairs (ewald_spole.F90:479-679, called

@@ -110,6 +110,26 @@ def test_generic_and_line_ffts_agree(name, monkeypatch):
         assert abs(a["engcpe"] - b["engcpe"]) <= agree * abs(b["engcpe"])
 
 
+@pytest.mark.parametrize("name", ["mix_4000_k36", "mix4_4000_k36", "mol3_2592_k32"])
+def test_mixed_relative_and_fp64_separations_agree(name, monkeypatch):
+    """The mixed combined-table pair kernel with separations from
+    subcell-relative FP32 positions and list codes (the default where the
+    list is built over subcells) and with FP64 minimum-image separations
+    (DLPX_REL_PAIRS=0): both within the mixed bar of the oracle, and within
+    it of each other."""
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    o = oracle.two_body(s)
+    res = {}
+    for rel in ("1", "0"):
+        monkeypatch.setenv("DLPX_REL_PAIRS", rel)
+        ctx, res[rel] = gpu_two_body(s, lib.MIXED)
+        ctx.close()
+        assert rms_rel(res[rel]["frc"], o["frc"]) <= 1e-5
+        check_energies(res[rel], o, 1e-6, energy_scales(o, mixed=True))
+    assert rms_rel(res["1"]["frc"], res["0"]["frc"]) <= 1e-5
+
+
 @pytest.mark.parametrize("name", [n for n in STATIC if "tab_pot" in fixtures.load(n)[1]])
 def test_reference_vdw_tables_through_set_vdw_tables(name):
     """The host-table route: DL_POLY's own vdws%tab_potential / tab_force
