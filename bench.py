@@ -36,12 +36,17 @@ HBM_PEAK = 8.0e12            # B/s
 VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP32
 
 
-# kernels of each stage (mixed precision, spline order 8)
-KERNELS = {"pair": "k_pair<float, 8, 512, false, false, false, false>", "spread": "k_spread_cells<float, 8>",
-           "fft": "k_fft_x_r2c/k_fft_c2c/k_fft_z_conv/k_fft_x_c2r<float, 16>", "gather": "k_gather<float, 8>"}
+# kernels of each stage (mixed precision, spline order 8, 128^3 mesh), as the
+# stage-timed pass launches them (single-system kernels; resident runs launch
+# the _batch forms of the same bodies)
+KERNELS = {"pair": "k_pair_comb<8, 1024, true>", "spread": "k_spread_cells<float, 8>",
+           "fft": "k_lines_r2c/k_lines_c2c/k_lines_zconv/k_lines_c2r<float, 16, 8, 16>",
+           "gather": "k_gather<float, 8>"}
 # the FFT stage's kernels and their launches per stage
-FFT_KERNELS = [("k_fft_x_r2c<float, 16>", 1), ("k_fft_c2c<float, 16>", 2), ("k_fft_z_conv<float, 16>", 1),
-               ("k_fft_x_c2r<float, 16>", 1)]
+FFT_KERNELS = [("k_lines_r2c<float, 16, 8, 16>", 1), ("k_lines_c2c<float, 16, 8, 16, 1>", 1),
+               ("k_lines_zconv<float, 16, 8, 16>", 1), ("k_lines_c2c<float, 16, 8, 16, -1>", 1),
+               ("k_lines_c2r<float, 16, 8, 16>", 1)]
+BATCH_PAIR_KERNEL = "k_pair_comb_batch<8, 1024, true>"
 
 
 def traffic_from_profile(workload, kernel):
@@ -97,6 +102,26 @@ def workload(name):
     return s, desc
 
 
+def ensemble_cpu_baseline(syss, steps=6):
+    """The compiled reference, serial on 1 core, on a sample of the ensemble
+    (the smallest and largest argon and NaCl systems): atom-steps/s over the
+    sample's steps 2..steps."""
+    argon = sorted([s for s in syss if not s.electrostatics], key=lambda s: s.natms)
+    nacl = sorted([s for s in syss if s.electrostatics], key=lambda s: s.natms)
+    sample = [x for x in (argon[:1] + argon[-1:] + nacl[:1] + nacl[-1:])]
+    tot_as, tot_t, names = 0.0, 0.0, []
+    for s in sample:
+        r = cpu_baseline(s, steps=steps, compare=False)
+        if r is None:
+            return None
+        tot_as += s.natms * (steps - 1)
+        tot_t += s.natms * (steps - 1) / r["value"]
+        names.append("%s %d" % ("NaCl" if s.electrostatics else "argon", s.natms))
+    return dict(value=tot_as / tot_t, unit="atom-steps/s", cores=1, kind="reference",
+                sample="DL_POLY 5.1 serial (oracle/_ref/DLPOLY.Z) on 4 of the ensemble's systems (%s), steps 2..%d "
+                       "of a %d-step NVE run each (cpu[s] column of OUTPUT)" % (", ".join(names), steps, steps))
+
+
 def run_ensemble(args):
     """Mixed-precision ensemble (dlpx_md_run_ensemble over all systems) against
     the per-system FP64 loop (dlpx_md_run one system after another)."""
@@ -124,9 +149,28 @@ def run_ensemble(args):
     t0 = time.perf_counter()
     nreb = lib.md_run_ensemble(ctxs, args.steps)
     tmx = time.perf_counter() - t0
+    # a second, separately timed pass of eager steps with per-stage HIP events
+    # on the library stream; the pair kernel's roofline over every system
+    ctxs[0].set_timing(True)
+    lib.md_run_ensemble(ctxs, 21)
+    ms = ctxs[0].get_timing()
+    ctxs[0].set_timing(False)
+    nev = max(ms[7], 1.0)
+    flops = 0.0
+    for c, s in zip(ctxs, syss):
+        rc = max(s.cutoff, s.vdw_cutoff) if s.electrostatics else s.vdw_cutoff
+        flops += (60.0 if s.electrostatics else 40.0) * c.count_pairs(rc)
+    t_pair = ms[0] / nev * 1e-3
+    roof = dict(bound="valu", achieved=flops / t_pair / 1e12, peak=VALU_PEAK[lib.MIXED] / 1e12, unit="TFLOP/s")
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["kernel"] = BATCH_PAIR_KERNEL + " (one launch over all systems)"
+    roof["traffic"] = traffic_from_profile("ensemble256", BATCH_PAIR_KERNEL)
+    stages = {nm: ms[k] / nev for k, nm in enumerate(["pair", "spread", "fft", "gather", "step"])}
+    stages["list"] = ms[5] / nev
     for c in ctxs:
         c.close()
     rate_mx, rate_64 = natoms * args.steps / tmx, natoms * args.steps / t64
+    cpu = None if args.no_cpu_baseline else ensemble_cpu_baseline(syss)
     return {
         "metric": METRIC + " (batched ensemble)",
         "value": rate_mx,
@@ -146,18 +190,10 @@ def run_ensemble(args):
         "fp64_per_system_loop": {"value": rate_64, "ms_per_step": t64 / args.steps * 1e3},
         "mixed_ensemble_over_fp64_loop": rate_mx / rate_64,
         "list_rebuilds": int(nreb.sum()),
-        "roofline": None,
-        "cpu_baseline": None,
+        "stages_ms_per_step": stages,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
-
-
-def pairs_within(ctx, s, r, pos):
-    """Unordered pairs with separation < r (from the GPU list, measured on the host)."""
-    p = ctx.get_pairs()
-    L = np.diag(s.cell.reshape(3, 3))
-    x = pos - L * np.floor(pos / L + 0.5)
-    d = x[p[:, 0]] - (x[p[:, 1]] + p[:, 2:5] * L)
-    return int(((d * d).sum(1) < r * r).sum())
 
 
 def stage_work(s, npairs_rc, npairs_vdw_only, precision):
@@ -226,7 +262,7 @@ def force_error(s, md_steps=50):
         res[prec] = ctx.two_body_forces(pos)
         if prec == lib.FP64:
             rc = max(s.cutoff, s.vdw_cutoff)
-            np_rc = pairs_within(ctx, s, rc, pos)
+            np_rc = ctx.count_pairs(rc)
         ctx.close()
     f64, fmx = res[lib.FP64]["frc"], res[lib.MIXED]["frc"]
     err = float(np.sqrt(((fmx - f64) ** 2).sum() / (f64 ** 2).sum()))
@@ -254,23 +290,8 @@ def vs_reference(s, steps, revcon):
     return out
 
 
-def cpu_baseline(s, steps=8):
-    """The compiled reference (oracle/_ref/DLPOLY.Z, serial, 1 core) on the same
-    configuration: cumulative 'cpu [s]' of OUTPUT between step 1 and the last
-    step (initialisation and the step-0 force call excluded)."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
-    if not os.path.exists(exe):
-        return None
-    with tempfile.TemporaryDirectory() as d:
-        fileio.write_inputs(d, s, steps=steps, print_every=1)
-        t0 = time.time()
-        r = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=900)
-        wall = time.time() - t0
-        out = open(os.path.join(d, "OUTPUT")).read() if os.path.exists(os.path.join(d, "OUTPUT")) else ""
-        revcon = fileio.read_revcon(os.path.join(d, "REVCON")) if os.path.exists(os.path.join(d, "REVCON")) else None
-    if r.returncode != 0:
-        log("reference run failed:", r.stdout[-500:])
-        return None
+def output_cpu(out):
+    """Cumulative 'cpu [s]' per step from a DL_POLY OUTPUT text."""
     lines = out.splitlines()
     cpu = {}
     for i, ln in enumerate(lines):
@@ -280,15 +301,72 @@ def cpu_baseline(s, steps=8):
                 cpu[int(m.group(1))] = float(lines[i + 2].split()[0])
             except (ValueError, IndexError):
                 pass
+    return cpu
+
+
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "DLPOLY.Z")
+
+
+def cpu_baseline(s, steps=8, compare=True):
+    """The compiled reference (oracle/_ref/DLPOLY.Z, serial, 1 core) on the same
+    configuration: cumulative 'cpu [s]' of OUTPUT between step 1 and the last
+    step (initialisation and the step-0 force call excluded)."""
+    if not os.path.exists(REF_EXE):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        fileio.write_inputs(d, s, steps=steps, print_every=1)
+        t0 = time.time()
+        r = subprocess.run([REF_EXE], cwd=d, capture_output=True, text=True, timeout=900)
+        wall = time.time() - t0
+        out = open(os.path.join(d, "OUTPUT")).read() if os.path.exists(os.path.join(d, "OUTPUT")) else ""
+        revcon = fileio.read_revcon(os.path.join(d, "REVCON")) if os.path.exists(os.path.join(d, "REVCON")) else None
+    if r.returncode != 0:
+        log("reference run failed:", r.stdout[-500:])
+        return None
+    cpu = output_cpu(out)
     if 1 not in cpu or steps not in cpu:
         return None
     dt = cpu[steps] - cpu[1]
-    vs_ref = vs_reference(s, steps, revcon) if revcon is not None else None
+    vs_ref = vs_reference(s, steps, revcon) if (compare and revcon is not None) else None
     return dict(value=s.natms * (steps - 1) / dt, unit="atom-steps/s", cores=1, kind="reference",
                 forces_vs_reference=vs_ref,
                 sample="DL_POLY 5.1 serial (oracle/_ref/DLPOLY.Z), same %d-ion configuration, steps 2..%d of a "
                        "%d-step NVE run (cpu[s] column of OUTPUT, %.1f s wall incl. set-up)"
                        % (s.natms, steps, steps, wall))
+
+
+def cpu_baseline_box(s, steps=6, nproc=None):
+    """The host's throughput bound without an MPI build: nproc concurrent serial
+    runs of the compiled reference on the same configuration, one per core
+    (the GPU box's CPU share, at most 16), each rated as in cpu_baseline; the
+    figure is their summed atom-steps/s."""
+    if not os.path.exists(REF_EXE):
+        return None
+    nproc = nproc or max(1, min(16, os.cpu_count() or 1))
+    dirs = [tempfile.mkdtemp() for _ in range(nproc)]
+    try:
+        for d in dirs:
+            fileio.write_inputs(d, s, steps=steps, print_every=1)
+        t0 = time.time()
+        procs = [subprocess.Popen([REF_EXE], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL) for d in dirs]
+        rcs = [p.wait(timeout=900) for p in procs]
+        wall = time.time() - t0
+        rates = []
+        for d, rc in zip(dirs, rcs):
+            f = os.path.join(d, "OUTPUT")
+            cpu = output_cpu(open(f).read()) if rc == 0 and os.path.exists(f) else {}
+            if 1 not in cpu or steps not in cpu:
+                return None
+            rates.append(s.natms * (steps - 1) / (cpu[steps] - cpu[1]))
+    finally:
+        import shutil
+        for d in dirs:
+            shutil.rmtree(d, ignore_errors=True)
+    return dict(value=float(sum(rates)), unit="atom-steps/s", cores=nproc, kind="reference",
+                sample="%d concurrent serial DL_POLY 5.1 runs (oracle/_ref/DLPOLY.Z; no MPI library on the box), one per "
+                       "host core, each on the same %d-ion configuration, steps 2..%d of a %d-step NVE run (cpu[s] "
+                       "column of OUTPUT); per-run rates %.3g-%.3g atom-steps/s, %.1f s wall"
+                       % (nproc, s.natms, steps, steps, min(rates), max(rates), wall))
 
 
 def main():
@@ -298,6 +376,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="nacl64k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="concurrent reference runs of the per-box CPU bound (0: "
+                    "one per host core, at most 16)")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="NVE steps of the energy-drift runs (0 skips them)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -372,7 +452,16 @@ def main():
     if args.drift_steps > 0:
         drift = {"mixed": nve_drift(s, lib.MIXED, args.drift_steps), "fp64": nve_drift(s, lib.FP64, args.drift_steps)}
         log("nve drift:", drift)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(s)
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the box's host cores (concurrent serial reference runs), with the
+        # single-core run (and its force comparison with the reference) inside
+        single = cpu_baseline(s)
+        cpu = cpu_baseline_box(s, nproc=args.cpu_procs or None)
+        if cpu is None:
+            cpu = single
+        elif single is not None:
+            cpu["single_core"] = single
     out = {
         "metric": METRIC,
         "value": rate[lib.MIXED],

@@ -912,6 +912,15 @@ int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs)
     return 0;
 }
 
+int dlpx_count_pairs(dlpx_ctx *ctx, double r, int64_t *npairs)
+{
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !npairs || !(r > 0.0)) return DLPX_ERR_BAD_ARG;
+    if (!c->have_list) return fail(c, DLPX_ERR_STATE, "no neighbour list");
+    DLPX_HIP(c, hipStreamSynchronize(c->stream));
+    return count_pairs_dev(c, r, npairs);
+}
+
 int dlpx_vnl_check(dlpx_ctx *ctx, const dlpx_atoms *atoms, int32_t *update)
 {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
@@ -1149,7 +1158,9 @@ int dlpx_md_run_ensemble_energies(dlpx_ctx *const *ctxs, int32_t nctx, int32_t n
     for (int k = 0; k < nctx; k++) {
         cs[k]->run_dt = tsteps[k];
         if (energies && (r = energies_room(cs[k], nsteps))) return r;
-        batched = batched && cs[k]->use_graphs && !cs[k]->timing && cs[k]->prm.precision == cs[0]->prm.precision;
+        // stage timing on the first context times the batched steps eagerly
+        batched = batched && cs[k]->use_graphs && (k == 0 || !cs[k]->timing) &&
+                  cs[k]->prm.precision == cs[0]->prm.precision;
     }
     if (batched) {
         // every stage one launch over all systems, steps replaying one graph

@@ -174,12 +174,35 @@ static int ens_enqueue(Ensemble &E, bool with_vv1)
     Ctx *c0 = E.cs[0];
     hipStream_t s = c0->stream, s2 = c0->recip_stream;
     int r;
-    if ((r = list_launch(c0, s, E.list))) return r;
+    if (c0->timing) {
+        // stage timing (eager): every stage on one stream between the first
+        // context's events, [6]-[7] list, [0]-[1] pair, [1]-[2] spreading,
+        // [2]-[3] convolution, [3]-[4] gather, [4]-[5] step kernel
+        DLPX_HIP(c0, hipEventRecord(c0->ev[6], s));
+        if ((r = list_launch(c0, s, E.list))) return r;
+        DLPX_HIP(c0, hipEventRecord(c0->ev[7], s));
+        DLPX_HIP(c0, hipEventRecord(c0->ev[0], s));
+        if ((r = pair_launch(c0, s, E.pair)) || (r = excl_launch(c0, s, E.excl))) return r;
+        DLPX_HIP(c0, hipEventRecord(c0->ev[1], s));
+        if (E.electro) {
+            if ((r = spme_launch(c0, s, E.spme, c0->ev + 2))) return r;
+        } else {
+            for (int k = 2; k <= 4; k++) DLPX_HIP(c0, hipEventRecord(c0->ev[k], s));
+        }
+        if ((r = step_launch(c0, s, E.step, true, with_vv1))) return r;
+        DLPX_HIP(c0, hipEventRecord(c0->ev[5], s));
+        return 0;
+    }
+    // the binning and sort of a due rebuild; the reciprocal-space chain needs
+    // no more (positions and link cells), so it forks here and runs beside
+    // the list build as well as the pair kernel
+    if ((r = list_launch(c0, s, E.list, 0))) return r;
     if (E.electro) {
         DLPX_HIP(c0, hipEventRecord(E.fork, s));
         DLPX_HIP(c0, hipStreamWaitEvent(s2, E.fork, 0));
         if ((r = spme_launch(c0, s2, E.spme))) return r;
     }
+    if ((r = list_launch(c0, s, E.list, 1))) return r;
     if ((r = pair_launch(c0, s, E.pair))) return r;
     if ((r = excl_launch(c0, s, E.excl))) return r;
     if (E.electro) {
@@ -334,7 +357,7 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         // a batch of several systems is tuned on its own first steps; a single
         // system keeps the default shape (8 pair lanes per atom, 16 FFT lines
         // per workgroup), the best one measured at 4k-1.7M atoms
-        else tune = tune_enabled() && !energies && nsteps >= 40 && cs.size() > 1;
+        else tune = tune_enabled() && !energies && !c0->timing && nsteps >= 40 && cs.size() > 1;
     }
     for (Ctx *c : cs)
         if ((r = upload_tables(c))) return r;
@@ -388,7 +411,22 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         }
         if ((r = ready(sh))) return r;
     }
-    if (s < nsteps - 1) {
+    if (c0->timing) {
+        // eager timed steps: each one's stage times into the first context's
+        // stage_ms (as an evaluation's; [5] the list chain, per step)
+        for (; s < nsteps - 1; s++) {
+            if ((r = ens_enqueue(*E, true))) return r;
+            DLPX_HIP(c0, hipEventSynchronize(c0->ev[5]));
+            float ms;
+            for (int k = 0; k < 5; k++) {
+                DLPX_HIP(c0, hipEventElapsedTime(&ms, c0->ev[k], c0->ev[k + 1]));
+                c0->stage_ms[k] += ms;
+            }
+            DLPX_HIP(c0, hipEventElapsedTime(&ms, c0->ev[6], c0->ev[7]));
+            c0->stage_ms[5] += ms;
+            c0->timed_evals++;
+        }
+    } else if (s < nsteps - 1) {
         // one eager step (every kernel's attributes set), then the graph
         if ((r = ens_enqueue(*E, true))) return r;
         s++;

@@ -306,6 +306,12 @@ struct BatchLaunch {
 struct ListBatch {
     BatchLaunch wrap, scat, sort, bsub, blink, excl;
     int32_t *gate = nullptr;  // one system: its F_UPD word, else null (per-system gates only)
+    // several systems: the compacted chain's plan (k_list_plan), its inputs
+    // and the fixed grid of each kernel
+    int32_t *plan = nullptr, *plan_nb = nullptr;
+    int32_t **plan_gates = nullptr;
+    int plan_grid[6] = {0, 0, 0, 0, 0, 0};
+    int nsys = 0;
 };
 // the batched passes of one SPME convolution (fft.hip) and of the
 // reciprocal-space stage (spme.hip).  Each pass is one launch per group of
@@ -410,6 +416,7 @@ int upload_tables(Ctx *c);
 int upload_comb(Ctx *c);                     // the combined rows when the conditions hold (pair.hip)
 int build_list(Ctx *c);                      // from c->pos (current order), sorts & permutes
 int vnl_check_dev(Ctx *c, int *update);
+int count_pairs_dev(Ctx *c, double r, int64_t *npairs);
 int pair_forces(Ctx *c);                     // writes c->frc and reduction slots
 int excl_forces(Ctx *c);                     // adds the exclusion correction into c->frc
 int excl_blocks(const Ctx *c);
@@ -448,7 +455,7 @@ int conv_blocks(const Ctx *c);  // workgroups of the z pass (reduction rows)
 
 // batched stages (one launch per stage over the systems of an ensemble)
 int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb);
-int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb);
+int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb, int part = -1);
 void list_batch_free(ListBatch &lb);
 int pair_prepare(const std::vector<Ctx *> &cs, PairBatch &pb, int lanes, std::vector<int> &rows);
 int pair_launch(Ctx *c, hipStream_t s, const PairBatch &pb);
@@ -459,7 +466,9 @@ InflParams infl_params(const Ctx *c);
 int fft_prepare(const std::vector<Ctx *> &cs, const std::vector<InflParams> &ips, FftBatch &fb);
 int fft_launch(Ctx *c, hipStream_t s, const FftBatch &fb);
 int spme_prepare(const std::vector<Ctx *> &cs, SpmeBatch &sb);
-int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb);
+// marks (timed runs): events recorded after the spreading, the convolution
+// and the gather
+int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb, hipEvent_t *marks = nullptr);
 int vv2_blocks(const Ctx *c);
 int excl_blocks(const Ctx *c);
 // one kernel per step of a batched run: stage 2 of the step, its reductions

@@ -727,6 +727,32 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
     if (t == 0) build_flags(a.flags, s_ovf, s_big, s_tot);
 }
 
+// Listed pairs closer than r (each pair is met from both ends: the count is
+// halved by the caller), FP64 minimum image; thread t = i*G + g walks lane g
+// of sorted atom i.
+__global__ void __launch_bounds__(256) k_count_pairs(const double4 *__restrict__ pos, const int32_t *__restrict__ nbr,
+                                                     const int32_t *__restrict__ cnt, int n, int cap_lane, double r2,
+                                                     CellGrid g, unsigned long long *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / kLanesPerAtom, lane = t % kLanesPerAtom;
+    unsigned long long k = 0;
+    if (i < n) {
+        const double4 pi = pos[i];
+        const int m = min(cnt[t], cap_lane);
+        for (int e = 0; e < m; e++) {
+            const double4 pj = pos[list_j(nbr[list_slot(i, e, lane, cap_lane)])];
+            const double dx = min_image_1(pi.x, pj.x, g.L[0], g.invL[0]);
+            const double dy = min_image_1(pi.y, pj.y, g.L[1], g.invL[1]);
+            const double dz = min_image_1(pi.z, pj.z, g.L[2], g.invL[2]);
+            k += (dx * dx + dy * dy + dz * dz < r2) ? 1ull : 0ull;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+    if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, k);
+}
+
 // vnl_check (neighbours.F90:150-178): largest minimum-image displacement since
 // the list was built, as the bit pattern of a non-negative double.
 __global__ void k_max_disp(const double4 *pos, const double4 *xref, int n, CellGrid g,
@@ -793,7 +819,11 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
 // bodies above; every one first honours the rebuild gate.  A batched launch
 // may also take one gate for the whole batch as a kernel argument (a batch
 // of one system: its F_UPD), which a skipped launch reads without first
-// fetching its argument struct.
+// fetching its argument struct.  A batch of several systems runs the
+// compacted form (_cbatch): a fixed grid of workgroups loops over the blocks
+// of only the systems whose rebuild is due, from the plan k_list_plan wrote
+// (their indices act[], count at [nsys], and per kernel `row` the prefix sums
+// of their block counts), so the systems without a rebuild cost no dispatch.
 #define DLPX_LIST_KERNELS(NAME, ARGS, NT, CALL_SINGLE, CALL_BATCH, XCD)                                      \
     __global__ void __launch_bounds__(NT) NAME(ARGS a)                                                    \
     {                                                                                                     \
@@ -817,6 +847,27 @@ __device__ __forceinline__ void list_exclude_body(const ExclListArgs &a, int bid
         const ARGS &a = av[bp.s];                                                                          \
         if (skip_build(a.upd)) return;                                                                    \
         CALL_BATCH;                                                                                       \
+    }                                                                                                     \
+    __global__ void __launch_bounds__(NT) NAME##_cbatch(const ARGS *__restrict__ av,                      \
+                                                        const int32_t *__restrict__ plan, int nsys, int row) \
+    {                                                                                                     \
+        const int32_t *act = plan, *coff = plan + nsys + 1 + row * (nsys + 1);                            \
+        const int nact = plan[nsys], tot = coff[nact];                                                    \
+        for (int lb = blockIdx.x; lb < tot; lb += gridDim.x) {                                            \
+            int lo = 0, hi = nact; /* coff[lo] <= lb < coff[hi] */                                       \
+            while (hi - lo > 1) {                                                                         \
+                const int mid = (lo + hi) >> 1;                                                           \
+                if (coff[mid] <= lb) lo = mid;                                                            \
+                else hi = mid;                                                                            \
+            }                                                                                             \
+            BatchPos bp;                                                                                  \
+            bp.s = act[lo];                                                                               \
+            bp.b = lb - coff[lo];                                                                         \
+            bp.nb = coff[lo + 1] - coff[lo];                                                              \
+            const ARGS &a = av[bp.s];                                                                      \
+            CALL_BATCH;                                                                                   \
+            __syncthreads(); /* the next block's use of LDS */                                           \
+        }                                                                                                 \
     }
 DLPX_LIST_KERNELS(k_wrap_scan, WrapArgs, kWrapThreads, wrap_scan_body(a, blockIdx.x, gridDim.x),
                   wrap_scan_body(a, bp.b, bp.nb), false)
@@ -831,6 +882,33 @@ DLPX_LIST_KERNELS(k_build_list_sub, BuildArgs, kListThreads,
                   build_list_sub_body<kLanesPerAtom>(a, xcd_block(blockIdx.x, gridDim.x)),
                   build_list_sub_body<kLanesPerAtom>(a, bp.b), true)
 #undef DLPX_LIST_KERNELS
+
+// The plan of a compacted list chain: the systems whose F_UPD asks for a
+// rebuild (act[0..nact), nact at [nsys]) and, per kernel row q, the prefix
+// sums coff_q[k] of their block counts nb[q][act[k]].  One workgroup: the
+// flags are read in parallel, the scan is serial over the systems.
+constexpr int kListKernels = 6;  // wrap, scatter, sort, subcell build, link-cell build, exclusions
+__global__ void __launch_bounds__(256) k_list_plan(int nsys, int32_t *const *__restrict__ gates,
+                                                   const int32_t *__restrict__ nb, int32_t *__restrict__ plan)
+{
+    extern __shared__ int32_t sflag[];
+    for (int k = threadIdx.x; k < nsys; k += blockDim.x) sflag[k] = *gates[k] != 0;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int32_t *act = plan, *coff = plan + nsys + 1;
+    int run[kListKernels] = {0, 0, 0, 0, 0, 0}, k = 0;
+    for (int s = 0; s < nsys; s++) {
+        if (!sflag[s]) continue;
+        act[k] = s;
+        for (int q = 0; q < kListKernels; q++) {
+            coff[q * (nsys + 1) + k] = run[q];
+            run[q] += nb[q * nsys + s];
+        }
+        k++;
+    }
+    for (int q = 0; q < kListKernels; q++) coff[q * (nsys + 1) + k] = run[q];
+    act[nsys] = k;
+}
 
 static CellGrid make_grid(const Ctx *c)
 {
@@ -1044,23 +1122,77 @@ int list_prepare(const std::vector<Ctx *> &cs, ListBatch &lb)
         (r = batch_upload(c0, lb.excl, excl.data(), sizeof(ExclListArgs), nb_excl, false)))
         return r;
     // a batch of one system gates every kernel on its own decision word, read
-    // first thing by each workgroup
+    // first thing by each workgroup; several systems run the compacted chain
     lb.gate = cs.size() == 1 ? c0->flags + F_UPD : nullptr;
+    if (cs.size() > 1) {
+        const int nsys = (int)cs.size();
+        const std::vector<int> *nbs[kListKernels] = {&nb_wrap, &nb_atoms, &nb_keys, &nb_sub, &nb_link, &nb_excl};
+        std::vector<int32_t> nb((size_t)kListKernels * nsys);
+        for (int q = 0; q < kListKernels; q++)
+            for (int k = 0; k < nsys; k++) nb[(size_t)q * nsys + k] = (*nbs[q])[k];
+        std::vector<int32_t *> gates(nsys);
+        for (int k = 0; k < nsys; k++) gates[k] = cs[k]->flags + F_UPD;
+        const size_t plan_words = (size_t)nsys + 1 + (size_t)kListKernels * (nsys + 1);
+        DLPX_HIP(c0, hipMalloc((void **)&lb.plan, sizeof(int32_t) * plan_words));
+        DLPX_HIP(c0, hipMalloc((void **)&lb.plan_nb, sizeof(int32_t) * nb.size()));
+        DLPX_HIP(c0, hipMalloc((void **)&lb.plan_gates, sizeof(int32_t *) * nsys));
+        DLPX_HIP(c0, dev_upload(c0->stream, lb.plan_nb, nb.data(), sizeof(int32_t) * nb.size()));
+        DLPX_HIP(c0, dev_upload(c0->stream, lb.plan_gates, gates.data(), sizeof(int32_t *) * nsys));
+        // fixed grids of the compacted kernels: enough workgroups to fill the
+        // GPU a few times over (the subcell and link-cell builds hold 52 KB of
+        // LDS each, three to a CU)
+        const int cap[kListKernels] = {2048, 2048, 8192, 1024, 1024, 2048};
+        for (int q = 0; q < kListKernels; q++) {
+            long tot = 0;
+            for (int k = 0; k < nsys; k++) tot += (*nbs[q])[k];
+            lb.plan_grid[q] = (int)std::min<long>(tot, cap[q]);
+        }
+        lb.nsys = nsys;
+    }
     return 0;
 }
 
-int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb)
+int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb, int part)
 {
+    // part 0: binning and sort (everything the reciprocal-space stages read);
+    // part 1: the list build and its exclusion filter; -1: both
+    const bool p0 = part != 1, p1 = part != 0;
+    if (lb.plan) {
+        if (p0)
+            hipLaunchKernelGGL(k_list_plan, dim3(1), dim3(256), sizeof(int32_t) * lb.nsys, s, lb.nsys,
+                               (int32_t *const *)lb.plan_gates, (const int32_t *)lb.plan_nb, lb.plan);
+#define DLPX_LC(K, BL, ARGS, NT, ROW)                                                                     \
+    if (lb.plan_grid[ROW] > 0)                                                                           \
+        hipLaunchKernelGGL(K, dim3(lb.plan_grid[ROW]), dim3(NT), 0, s, (const ARGS *)(BL).args,          \
+                           (const int32_t *)lb.plan, lb.nsys, ROW)
+        if (p0) {
+            DLPX_LC(k_wrap_scan_cbatch, lb.wrap, WrapArgs, kWrapThreads, 0);
+            DLPX_LC(k_scatter_copy_cbatch, lb.scat, ScatterArgs, 256, 1);
+            DLPX_LC(k_sort_permute_cbatch, lb.sort, SortArgs, 256, 2);
+        }
+        if (p1) {
+            DLPX_LC(k_build_list_sub_cbatch, lb.bsub, BuildArgs, kListThreads, 3);
+            DLPX_LC(k_build_list_cbatch, lb.blink, BuildArgs, kListThreads, 4);
+            DLPX_LC(k_list_exclude_cbatch, lb.excl, ExclListArgs, 256, 5);
+        }
+#undef DLPX_LC
+        DLPX_HIP(c, hipGetLastError());
+        return 0;
+    }
 #define DLPX_LB(K, BL, ARGS, NT)                                                                          \
     if ((BL).total > 0)                                                                                  \
         hipLaunchKernelGGL(K, dim3((BL).total), dim3(NT), 0, s, (const ARGS *)(BL).args, (BL).boff, (BL).nsys, \
                            (const int32_t *)lb.gate)
-    DLPX_LB(k_wrap_scan_batch, lb.wrap, WrapArgs, kWrapThreads);
-    DLPX_LB(k_scatter_copy_batch, lb.scat, ScatterArgs, 256);
-    DLPX_LB(k_sort_permute_batch, lb.sort, SortArgs, 256);
-    DLPX_LB(k_build_list_sub_batch, lb.bsub, BuildArgs, kListThreads);
-    DLPX_LB(k_build_list_batch, lb.blink, BuildArgs, kListThreads);
-    DLPX_LB(k_list_exclude_batch, lb.excl, ExclListArgs, 256);
+    if (p0) {
+        DLPX_LB(k_wrap_scan_batch, lb.wrap, WrapArgs, kWrapThreads);
+        DLPX_LB(k_scatter_copy_batch, lb.scat, ScatterArgs, 256);
+        DLPX_LB(k_sort_permute_batch, lb.sort, SortArgs, 256);
+    }
+    if (p1) {
+        DLPX_LB(k_build_list_sub_batch, lb.bsub, BuildArgs, kListThreads);
+        DLPX_LB(k_build_list_batch, lb.blink, BuildArgs, kListThreads);
+        DLPX_LB(k_list_exclude_batch, lb.excl, ExclListArgs, 256);
+    }
 #undef DLPX_LB
     DLPX_HIP(c, hipGetLastError());
     return 0;
@@ -1069,6 +1201,10 @@ int list_launch(Ctx *c, hipStream_t s, const ListBatch &lb)
 void list_batch_free(ListBatch &lb)
 {
     for (BatchLaunch *b : {&lb.wrap, &lb.scat, &lb.sort, &lb.bsub, &lb.blink, &lb.excl}) batch_free(*b);
+    for (void *p : {(void *)lb.plan, (void *)lb.plan_nb, (void *)lb.plan_gates})
+        if (p) (void)hipFree(p);
+    lb.plan = lb.plan_nb = nullptr;
+    lb.plan_gates = nullptr;
     lb.gate = nullptr;
 }
 
@@ -1087,6 +1223,25 @@ int vnl_check_dev(Ctx *c, int *update)
     memcpy(&r2, &bits, sizeof(r2));
     const double tol = sqrt(r2);
     *update = (tol >= kHalfMinus * c->prm.padding) ? 1 : 0;
+    return 0;
+}
+
+int count_pairs_dev(Ctx *c, double r, int64_t *npairs)
+{
+    unsigned long long *d = nullptr, h = 0;
+    DLPX_HIP(c, hipMalloc((void **)&d, sizeof(*d)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(*d), c->stream);
+    if (e == hipSuccess) {
+        const CellGrid g = make_grid(c);
+        hipLaunchKernelGGL(k_count_pairs, dim3(ceil_div((long)c->natms * kLanesPerAtom, 256)), dim3(256), 0, c->stream,
+                           c->pos, c->nbr, c->cnt, c->natms, c->cap_lane, r * r, g, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(c, DLPX_ERR_HIP + (int)e, std::string("count_pairs: ") + hipGetErrorString(e));
+    *npairs = (int64_t)(h / 2);
     return 0;
 }
 

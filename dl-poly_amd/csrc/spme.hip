@@ -848,7 +848,7 @@ int spme_prepare(const std::vector<Ctx *> &cs, SpmeBatch &sb)
 }
 
 template <typename R>
-static int spme_launch_t(Ctx *c, hipStream_t s, const SpmeBatch &sb)
+static int spme_launch_t(Ctx *c, hipStream_t s, const SpmeBatch &sb, hipEvent_t *marks)
 {
     if (sb.S == 0) return 0;
     const BatchLaunch &bs = sb.spread, &bg = sb.gather;
@@ -857,21 +857,24 @@ static int spme_launch_t(Ctx *c, hipStream_t s, const SpmeBatch &sb)
                        (const SpreadArgs<R> *)bs.args, bs.boff, bs.nsys)
     DLPX_S_SWITCH(sb.S, DLPX_SPREADB)
 #undef DLPX_SPREADB
+    if (marks) DLPX_HIP(c, hipEventRecord(marks[0], s));
     int r;
     if ((r = fft_launch(c, s, sb.fft))) return r;
+    if (marks) DLPX_HIP(c, hipEventRecord(marks[1], s));
 #define DLPX_GATHERB(SS)                                                                                         \
     hipLaunchKernelGGL((k_gather_batch<R, SS>), dim3(bg.total), dim3(256), 0, s, (const GatherArgs<R> *)bg.args, \
                        bg.boff, bg.nsys)
     DLPX_S_SWITCH(sb.S, DLPX_GATHERB)
 #undef DLPX_GATHERB
+    if (marks) DLPX_HIP(c, hipEventRecord(marks[2], s));
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
 
-int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb)
+int spme_launch(Ctx *c, hipStream_t s, const SpmeBatch &sb, hipEvent_t *marks)
 {
-    if (c->prm.precision == DLPX_MIXED) return spme_launch_t<float>(c, s, sb);
-    return spme_launch_t<double>(c, s, sb);
+    if (c->prm.precision == DLPX_MIXED) return spme_launch_t<float>(c, s, sb, marks);
+    return spme_launch_t<double>(c, s, sb, marks);
 }
 
 int spme_forces(Ctx *c)

@@ -26,7 +26,7 @@ EXPORTS = ["dlpx_create", "dlpx_destroy", "dlpx_last_error", "dlpx_version", "dl
            "dlpx_two_body_forces", "dlpx_md_load", "dlpx_md_run", "dlpx_md_run_ensemble", "dlpx_md_run_ensemble_energies", "dlpx_md_store",
            "dlpx_md_last_output", "dlpx_set_vdw_force_shift", "dlpx_set_vdw_direct", "dlpx_set_exclusions",
            "dlpx_set_per_particle", "dlpx_add_per_particle", "dlpx_set_frozen", "dlpx_compute_lrc_frozen",
-           "dlpx_set_timing", "dlpx_get_timing", "dlpx_two_body_forces_noalloc"]
+           "dlpx_set_timing", "dlpx_get_timing", "dlpx_two_body_forces_noalloc", "dlpx_count_pairs"]
 
 
 class DlpxError(RuntimeError):
@@ -221,6 +221,12 @@ class Context:
         out = np.zeros((max(n.value, 1), 5), np.int32)
         self._chk(self.lib.dlpx_get_pairs(self.h, _ptr(out), C.c_int64(n.value), C.byref(n)))
         return out[:n.value]
+
+    def count_pairs(self, r):
+        """Unordered listed pairs closer than r (device count)."""
+        n = C.c_int64()
+        self._chk(self.lib.dlpx_count_pairs(self.h, C.c_double(r), C.byref(n)))
+        return n.value
 
     def vnl_check(self, pos):
         a = self._atoms(pos)

@@ -189,6 +189,11 @@ int dlpx_link_cell_pairs(dlpx_ctx *ctx, const double cell[9], const dlpx_atoms *
  * image shift s such that r_ij = x_i - (x_j + s.L); 5 int32 per pair.
  * *npairs returns the count; at most cap pairs are written. */
 int dlpx_get_pairs(dlpx_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *npairs);
+/* Inspection, no DL_POLY counterpart: the number of unordered pairs of the
+ * current list closer than r at the device's current positions (FP64 minimum
+ * image), counted on the device; for a throughput figure in pairs within the
+ * cutoff (the list also holds the padding shell). */
+int dlpx_count_pairs(dlpx_ctx *ctx, double r, int64_t *npairs);
 
 /* vnl_check (neighbours.F90:123-296): *update = 1 when some atom moved by
  * at least padding/2 since the list was built. */

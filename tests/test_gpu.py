@@ -182,6 +182,24 @@ def test_pair_set_bit_exact(name):
     ctx.close()
 
 
+@pytest.mark.parametrize("name", ["nacl_4096_k48", "mix_4000_k36", "argon_4000_jit"])
+def test_count_pairs_matches_host_count(name):
+    """dlpx_count_pairs (device count of listed pairs closer than r) equals the
+    count of dlpx_get_pairs' pairs closer than r measured on the host, at the
+    cutoff and inside it."""
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    ctx = lib.Context(s, precision=lib.FP64)
+    ctx.link_cell_pairs(s.pos)
+    p = ctx.get_pairs()
+    L = np.diag(s.cell.reshape(3, 3))
+    x = s.pos - L * np.floor(s.pos / L + 0.5)
+    d2 = ((x[p[:, 0]] - (x[p[:, 1]] + p[:, 2:5] * L)) ** 2).sum(1)
+    for r in (max(s.cutoff, s.vdw_cutoff), 0.8 * s.cutoff):
+        assert ctx.count_pairs(r) == int((d2 < r * r).sum())
+    ctx.close()
+
+
 @pytest.mark.parametrize("name", ["argon_4000_jit", "nacl_4096_k48"])
 def test_link_cell_and_subcell_builds_give_the_same_set(name, monkeypatch):
     # boxes with >= 3 link cells per axis use the subcell build by default;
