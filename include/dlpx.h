@@ -46,7 +46,7 @@ extern "C" {
 
 typedef struct dlpx_params {
     int32_t precision;      /* DLPX_FP64 or DLPX_MIXED */
-    int32_t natms_max;      /* capacity in atoms (config%mxatms) */
+    int32_t natms_max;      /* capacity in atoms (config%mxatms), below 2^25 (else DLPX_ERR_UNSUPPORTED) */
     int32_t ntypes;         /* number of atom types (sites%ntype_atom) */
     int32_t max_list;       /* neighbours per atom (neigh%max_list); 0 = grow as needed */
     double cutoff;          /* neigh%cutoff (real-space Coulomb and list) */
