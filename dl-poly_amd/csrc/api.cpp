@@ -536,6 +536,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     // DLPX_REL_PAIRS=0: the mixed pair kernel forms separations from FP64 positions
     if (const char *e = std::getenv("DLPX_REL_PAIRS")) c->rel_pairs = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_COMB_THREADS")) c->comb_threads = std::atoi(e) == 512 ? 512 : 1024;
+    if (const char *e = std::getenv("DLPX_GRAPH_STEPS")) c->graph_steps = std::max(1, std::min(64, std::atoi(e)));
+    if (const char *e = std::getenv("DLPX_OVERLAP")) c->overlap = std::max(0, std::min(2, std::atoi(e)));
     if (const char *e = std::getenv("DLPX_PAIR_LANES")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16) c->pair_lanes = v;

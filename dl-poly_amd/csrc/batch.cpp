@@ -70,8 +70,9 @@ struct Ensemble {
     ListBatch list;
     PairBatch pair;
     SpmeBatch spme;
-    hipGraphExec_t graph = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipGraphExec_t graph = nullptr;   // one step
+    hipGraphExec_t graph_k = nullptr; // kGraphSteps steps
+    hipEvent_t fork = nullptr, join = nullptr, spread = nullptr;
 
     ~Ensemble() { release(); }
     void release()
@@ -84,10 +85,11 @@ struct Ensemble {
         pair.groups.clear();
         list_batch_free(list);
         if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        fork = join = nullptr;
+        if (graph_k) (void)hipGraphExecDestroy(graph_k);
+        graph = graph_k = nullptr;
+        for (hipEvent_t e : {fork, join, spread})
+            if (e) (void)hipEventDestroy(e);
+        fork = join = spread = nullptr;
     }
 };
 
@@ -162,6 +164,7 @@ static int ens_prepare(Ensemble &E, bool energies, const Shape &sh)
     if ((r = step_prepare(cs, rows, energies, E.step))) return r;
     DLPX_HIP(c0, hipEventCreateWithFlags(&E.fork, hipEventDisableTiming));
     DLPX_HIP(c0, hipEventCreateWithFlags(&E.join, hipEventDisableTiming));
+    DLPX_HIP(c0, hipEventCreateWithFlags(&E.spread, hipEventDisableTiming));
     E.key = ens_key(cs, energies, sh);
     return 0;
 }
@@ -194,31 +197,51 @@ static int ens_enqueue(Ensemble &E, bool with_vv1)
         return 0;
     }
     // the binning and sort of a due rebuild; the reciprocal-space chain needs
-    // no more (positions and link cells), so it forks here and runs beside
-    // the list build as well as the pair kernel
+    // no more (positions and link cells), so the branches fork here.
+    // Ctx::overlap: 2 the chain on the second stream beside the list build
+    // and the pair kernel; 1 the chain on the first stream and the pair
+    // kernel held until the spreading is done (both lean on LDS); 0 the chain
+    // on the first stream, the pair kernel free.  Default: 2 for one system,
+    // 0 for several (measured: 64k ions 0.229 ms/step with 2 against 0.236
+    // and 0.241 with 1 and 0; 256 systems 4.92 ms/step with 0 against 5.19
+    // with 2).  The graph executor maps branches onto its own queues, so the
+    // layout changes which kernels it starts together.
     if ((r = list_launch(c0, s, E.list, 0))) return r;
-    if (E.electro) {
-        DLPX_HIP(c0, hipEventRecord(E.fork, s));
-        DLPX_HIP(c0, hipStreamWaitEvent(s2, E.fork, 0));
-        if ((r = spme_launch(c0, s2, E.spme))) return r;
+    if (!E.electro) {
+        if ((r = list_launch(c0, s, E.list, 1)) || (r = pair_launch(c0, s, E.pair)) || (r = excl_launch(c0, s, E.excl)))
+            return r;
+        return step_launch(c0, s, E.step, true, with_vv1);
     }
-    if ((r = list_launch(c0, s, E.list, 1))) return r;
-    if ((r = pair_launch(c0, s, E.pair))) return r;
-    if ((r = excl_launch(c0, s, E.excl))) return r;
-    if (E.electro) {
-        DLPX_HIP(c0, hipEventRecord(E.join, s2));
-        DLPX_HIP(c0, hipStreamWaitEvent(s, E.join, 0));
+    DLPX_HIP(c0, hipEventRecord(E.fork, s));
+    DLPX_HIP(c0, hipStreamWaitEvent(s2, E.fork, 0));
+    const int ov = c0->overlap >= 0 ? c0->overlap : (E.cs.size() == 1 ? 2 : 0);
+    hipStream_t sr = ov == 2 ? s2 : s, sl = ov == 2 ? s : s2;  // reciprocal chain, list + pair
+    if (ov == 2 && (r = spme_launch(c0, sr, E.spme))) return r;
+    if ((r = list_launch(c0, sl, E.list, 1))) return r;
+    if (ov == 1) {
+        hipEvent_t marks[3] = {E.spread, nullptr, nullptr};
+        if ((r = spme_launch(c0, sr, E.spme, marks))) return r;
+        DLPX_HIP(c0, hipStreamWaitEvent(sl, E.spread, 0));
     }
+    if ((r = pair_launch(c0, sl, E.pair)) || (r = excl_launch(c0, sl, E.excl))) return r;
+    if (ov == 0 && (r = spme_launch(c0, sr, E.spme))) return r;
+    // the second stream joins the first
+    DLPX_HIP(c0, hipEventRecord(E.join, s2));
+    DLPX_HIP(c0, hipStreamWaitEvent(s, E.join, 0));
     return step_launch(c0, s, E.step, true, with_vv1);
 }
 
-static int ens_capture(Ensemble &E)
+
+// Capture nsteps consecutive steps (each with the next step's stage 1) into
+// one graph.
+static int ens_capture(Ensemble &E, hipGraphExec_t &out, int nsteps)
 {
     Ctx *c0 = E.cs[0];
-    if (E.graph) (void)hipGraphExecDestroy(E.graph);
-    E.graph = nullptr;
+    if (out) (void)hipGraphExecDestroy(out);
+    out = nullptr;
     DLPX_HIP(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
-    int r = ens_enqueue(E, true);
+    int r = 0;
+    for (int k = 0; k < nsteps && !r; k++) r = ens_enqueue(E, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c0->stream, &g);
     if (r) {
@@ -226,10 +249,10 @@ static int ens_capture(Ensemble &E)
         return r;
     }
     if (e != hipSuccess) return fail(c0, DLPX_ERR_HIP + (int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    const hipError_t ei = hipGraphInstantiate(&E.graph, g, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
-        E.graph = nullptr;
+        out = nullptr;
         return fail(c0, DLPX_ERR_HIP + (int)ei, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
     return 0;
@@ -385,7 +408,7 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
             if ((r = ready(cand))) return r;
             if ((r = ens_enqueue(*E, true))) return r;
             s++;
-            if ((r = ens_capture(*E))) return r;
+            if ((r = ens_capture(*E, E->graph, 1))) return r;
             std::vector<float> t;
             for (int k = 0; k < 5; k++, s++) {
                 DLPX_HIP(c0, hipEventRecord(e0, c0->stream));
@@ -430,8 +453,15 @@ int ensemble_run(const std::vector<Ctx *> &cs, Ensemble *&E, int nsteps, bool en
         // one eager step (every kernel's attributes set), then the graph
         if ((r = ens_enqueue(*E, true))) return r;
         s++;
+        // several steps per graph (Ctx::graph_steps) save the launch gap
+        // between replays
+        const int gk = c0->graph_steps;
+        if (gk > 1 && s + gk <= nsteps - 1) {
+            if (!E->graph_k && (r = ens_capture(*E, E->graph_k, gk))) return r;
+            for (; s + gk <= nsteps - 1; s += gk) DLPX_HIP(c0, hipGraphLaunch(E->graph_k, c0->stream));
+        }
         if (s < nsteps - 1) {
-            if (!E->graph && (r = ens_capture(*E))) return r;
+            if (!E->graph && (r = ens_capture(*E, E->graph, 1))) return r;
             for (; s < nsteps - 1; s++) DLPX_HIP(c0, hipGraphLaunch(E->graph, c0->stream));
         }
     }

@@ -149,6 +149,8 @@ struct Ctx {
     bool fft_lines = true;           // compile-time-length line kernels for the FFT lengths that have them
     bool rel_pairs = true;           // mixed combined-table pairs from subcell-relative FP32 positions
     int comb_threads = 1024;         // workgroup size of the combined-table pair kernel (512 or 1024)
+    int graph_steps = 8;             // resident steps per replayed graph (1..64)
+    int overlap = -1;                // resident steps' branch layout 0..2 (batch.cpp, ens_enqueue); -1 by batch size
     int device = 0;
 
     // box (orthorhombic)

@@ -857,16 +857,16 @@ static int spme_launch_t(Ctx *c, hipStream_t s, const SpmeBatch &sb, hipEvent_t 
                        (const SpreadArgs<R> *)bs.args, bs.boff, bs.nsys)
     DLPX_S_SWITCH(sb.S, DLPX_SPREADB)
 #undef DLPX_SPREADB
-    if (marks) DLPX_HIP(c, hipEventRecord(marks[0], s));
+    if (marks && marks[0]) DLPX_HIP(c, hipEventRecord(marks[0], s));
     int r;
     if ((r = fft_launch(c, s, sb.fft))) return r;
-    if (marks) DLPX_HIP(c, hipEventRecord(marks[1], s));
+    if (marks && marks[1]) DLPX_HIP(c, hipEventRecord(marks[1], s));
 #define DLPX_GATHERB(SS)                                                                                         \
     hipLaunchKernelGGL((k_gather_batch<R, SS>), dim3(bg.total), dim3(256), 0, s, (const GatherArgs<R> *)bg.args, \
                        bg.boff, bg.nsys)
     DLPX_S_SWITCH(sb.S, DLPX_GATHERB)
 #undef DLPX_GATHERB
-    if (marks) DLPX_HIP(c, hipEventRecord(marks[2], s));
+    if (marks && marks[2]) DLPX_HIP(c, hipEventRecord(marks[2], s));
     DLPX_HIP(c, hipGetLastError());
     return 0;
 }
