@@ -534,6 +534,8 @@ int dlpx_create(const dlpx_params *p, dlpx_ctx **out)
     // DLPX_FFT_LINES=0: the generic Stockham FFT kernels for every length
     if (const char *e = std::getenv("DLPX_FFT_LINES")) c->fft_lines = std::atoi(e) != 0;
     // DLPX_REL_PAIRS=0: the mixed pair kernel forms separations from FP64 positions
+    // DLPX_FFT_PLANES=0: separate x and y line kernels where the plane kernels would run
+    if (const char *e = std::getenv("DLPX_FFT_PLANES")) c->fft_planes = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_REL_PAIRS")) c->rel_pairs = std::atoi(e) != 0;
     if (const char *e = std::getenv("DLPX_COMB_THREADS")) c->comb_threads = std::atoi(e) == 512 ? 512 : 1024;
     if (const char *e = std::getenv("DLPX_GRAPH_STEPS")) c->graph_steps = std::max(1, std::min(64, std::atoi(e)));

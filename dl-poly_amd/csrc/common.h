@@ -149,6 +149,7 @@ struct Ctx {
     bool fft_lines = true;           // compile-time-length line kernels for the FFT lengths that have them
     bool rel_pairs = true;           // mixed combined-table pairs from subcell-relative FP32 positions
     int comb_threads = 1024;         // workgroup size of the combined-table pair kernel (512 or 1024)
+    bool fft_planes = true;          // x and y passes fused per z-plane where the mesh allows
     int graph_steps = 8;             // resident steps per replayed graph (1..64)
     int overlap = -1;                // resident steps' branch layout 0..2 (batch.cpp, ens_enqueue); -1 by batch size
     int device = 0;
@@ -321,7 +322,8 @@ struct ListBatch {
 // (fft_lines.hip), one group for the other systems (generic kernels).
 struct FftGroup {
     BatchLaunch bl;
-    int kind = 0;  // 0 forward x, 1 forward y, 2 z with the influence function, 3 inverse y, 4 inverse x
+    int kind = 0;  // 0 forward x, 1 forward y, 2 z with the influence function, 3 inverse y, 4 inverse x,
+                   // 5 forward x and y per plane, 6 inverse y and x per plane
     int n = 0;     // line length of a line-kernel group; 0: generic kernels
     int lpb = 0;   // lines (x-line pairs) per workgroup
 };

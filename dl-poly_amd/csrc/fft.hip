@@ -599,6 +599,32 @@ static int x_blocks(const Ctx *c, int lpb) { return ceil_div((c->K[1] * c->K[2] 
 
 static bool use_lines(const Ctx *c, int n) { return c->fft_lines && lines_supported(n); }
 
+// the x and y passes fused per z-plane (fft_planes.hip): K1 = K2 of a
+// line-kernel length whose plane takes at most 72 KB of LDS (FP32 up to 128,
+// FP64 up to 80).  A larger plane leaves a CU no room for a workgroup of the
+// pair kernel beside it, and in a resident step the plane kernel then waits
+// for the pair kernel's workgroups to drain (measured at 128^3 in FP64: the
+// step 0.311 -> 0.319 ms with planes of 132 KB).
+template <typename T>
+static bool use_plane(const Ctx *c)
+{
+    return c->fft_planes && use_lines(c, c->K[0]) && c->K[0] == c->K[1] && plane_lds<T>(c->K[0]) <= 72 * 1024;
+}
+
+template <typename T>
+static PlaneArgs<T> plane_args(const Ctx *c, void *grid, bool clear)
+{
+    using T2 = typename C2T<T>::type;
+    PlaneArgs<T> a;
+    a.Q = (T *)grid;
+    a.H = (T2 *)c->H;
+    a.nplanes = c->K[2];
+    a.hs = c->K1hp;
+    a.tw = (const T2 *)c->tw;
+    a.clear = clear ? 1 : 0;
+    return a;
+}
+
 template <typename T>
 static int run_y(Ctx *c, int sign)
 {
@@ -689,6 +715,11 @@ template <typename T>
 static int convolve_t(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_in)
 {
     int r;
+    if (use_plane<T>(c)) {
+        if ((r = plane_run<T>(c, c->sp, false, c->K[0], plane_args<T>(c, in, clear_in)))) return r;
+        if ((r = run_z_conv<T>(c, ip))) return r;
+        return plane_run<T>(c, c->sp, true, c->K[0], plane_args<T>(c, out, false));
+    }
     if ((r = run_x<T>(c, true, in, clear_in))) return r;
     if ((r = run_y<T>(c, +1))) return r;
     if ((r = run_z_conv<T>(c, ip))) return r;
@@ -703,6 +734,9 @@ int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_i
 ;
 }
 
+constexpr int kPlaneKey = 1 << 20;                 // group keys of the plane kernels: kPlaneKey + K1
+constexpr int kGroupPlaneFwd = 5, kGroupPlaneInv = 6;  // FftGroup::kind of the plane kernels
+
 // Batched convolution Q -> Phi (influence function mode 0, the forward pass
 // clearing Q) over the systems with electrostatics: each pass is one launch
 // per group of systems, a group for each line length that has compiled line
@@ -711,7 +745,7 @@ int fft_convolve(Ctx *c, const InflParams &ip, void *in, void *out, bool clear_i
 // lines fit its LDS).
 template <typename T, typename A, typename Build>
 static int add_group(const std::vector<Ctx *> &cs, FftBatch &fb, int kind, int key, int lpb,
-                     const std::vector<int> &keys, Build build)
+                     const std::vector<int> &keys, Build build, int gkind = -1, int gn = -1)
 {
     const size_t ns = cs.size();
     std::vector<A> args(ns, A{});
@@ -725,8 +759,8 @@ static int add_group(const std::vector<Ctx *> &cs, FftBatch &fb, int kind, int k
         if (kind == 2) fb.conv_rows[k] = nb[k];
     }
     FftGroup g;
-    g.kind = kind;
-    g.n = key > 0 ? key : 0;
+    g.kind = gkind >= 0 ? gkind : kind;
+    g.n = gn >= 0 ? gn : (key > 0 ? key : 0);
     g.lpb = lpb;
     int r;
     if ((r = batch_upload(cs[0], g.bl, args.data(), sizeof(A), nb, false))) return r;
@@ -762,17 +796,37 @@ static int fft_prepare_t(const std::vector<Ctx *> &cs, const std::vector<InflPar
         const int d = axis_of[kind];
         // group key per system: its line length with line kernels, 0 for the
         // generic group, -1 without a mesh
+        // (kPlaneKey + K1 for the fused x-y plane kernels, which also take the
+        // y passes)
         std::vector<int> keys(ns, -1), distinct;
         for (size_t k = 0; k < ns; k++) {
             Ctx *c = cs[k];
             if (!c->prm.electrostatics) continue;
+            if (use_plane<T>(c)) {
+                if (kind == 1 || kind == 3) continue;
+                if (kind != 2) {
+                    keys[k] = kPlaneKey + c->K[0];
+                    if (std::find(distinct.begin(), distinct.end(), keys[k]) == distinct.end()) distinct.push_back(keys[k]);
+                    continue;
+                }
+            }
             keys[k] = use_lines(c, c->K[d]) ? c->K[d] : 0;
             if (std::find(distinct.begin(), distinct.end(), keys[k]) == distinct.end()) distinct.push_back(keys[k]);
         }
         for (int key : distinct) {
             const int lpb = key > 0 ? lines_lpb<T>() : lg[d];
             int r = 0;
-            if (kind == 0)
+            if (key > kPlaneKey) {
+                const int n = key - kPlaneKey;
+                r = add_group<T, PlaneArgs<T>>(
+                    cs, fb, kind, key, 1, keys,
+                    [&](Ctx *c, size_t, int, int &nb, size_t &lds) {
+                        nb = c->K[2];
+                        lds = plane_lds<T>(n);
+                        return kind == 0 ? plane_args<T>(c, c->Q, true) : plane_args<T>(c, c->Phi, false);
+                    },
+                    kind == 0 ? kGroupPlaneFwd : kGroupPlaneInv, n);
+            } else if (kind == 0)
                 r = add_group<T, R2CArgs<T>>(cs, fb, kind, key, lpb, keys,
                                              [&](Ctx *c, size_t, int l, int &nb, size_t &lds) {
                                                  nb = x_blocks(c, l);
@@ -819,6 +873,11 @@ static int fft_launch_t(Ctx *c, hipStream_t s, const FftBatch &fb)
 {
     for (const FftGroup &g : fb.groups) {
         if (g.bl.total == 0) continue;
+        if (g.kind == kGroupPlaneFwd || g.kind == kGroupPlaneInv) {
+            int r;
+            if ((r = plane_batch<T>(c, s, g.kind == kGroupPlaneInv, g.n, g.bl))) return r;
+            continue;
+        }
         if (g.n > 0) {
             int r;
             if ((r = lines_batch<T>(c, s, g.kind, g.n, g.bl))) return r;

@@ -199,6 +199,19 @@ struct C2RArgs {
     FastDiv div_n;
 };
 
+// The x and y passes of one z-plane in one workgroup (fft_planes.hip), for
+// meshes with K1 = K2 of a line-kernel length: forward (real rows to the
+// half spectrum, then the y transforms; the read rows cleared when clear is
+// set) and inverse (y transforms, then the half spectrum back to real rows).
+template <typename T>
+struct PlaneArgs {
+    T *Q;                                // real grid, x fastest
+    typename C2T<T>::type *H;            // half spectrum, hs elements per row
+    int nplanes, hs;
+    const typename C2T<T>::type *tw;     // exp(2 pi i k / K1), k < K1
+    int clear;
+};
+
 constexpr int kConvRedBytes = 6 * (512 / 64) * sizeof(double);
 
 // Line kernels of compile-time length (fft_lines.hip) for the lengths
@@ -206,6 +219,9 @@ constexpr int kConvRedBytes = 6 * (512 / 64) * sizeof(double);
 // workgroup, lines_lds() their dynamic LDS (kind 0 y pass, 1 z pass, 2 forward
 // x, 3 inverse x).
 bool lines_supported(int n);
+template <typename T> size_t plane_lds(int n);
+template <typename T> int plane_run(Ctx *c, hipStream_t s, bool inverse, int n, const PlaneArgs<T> &a);
+template <typename T> int plane_batch(Ctx *c, hipStream_t s, bool inverse, int n, const BatchLaunch &bl);
 template <typename T> int lines_lpb();
 template <typename T> size_t lines_lds(int kind, int n);
 template <typename T> int lines_c2c(Ctx *c, hipStream_t s, const C2CArgs<T> &a, int nb);

@@ -22,7 +22,7 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-I", INCLUDE, 
          "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
          # no packed FP32 instructions (v_pk_*_f32) in generated code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["tables.cpp", "neighbours.hip", "pair.hip", "spme.hip", "fft.hip", "fft_lines.hip", "nve.hip", "api.cpp", "batch.cpp"]
+SOURCES = ["tables.cpp", "neighbours.hip", "pair.hip", "spme.hip", "fft.hip", "fft_lines.hip", "fft_planes.hip", "nve.hip", "api.cpp", "batch.cpp"]
 
 
 def _compile(src):

@@ -110,6 +110,32 @@ def test_generic_and_line_ffts_agree(name, monkeypatch):
         assert abs(a["engcpe"] - b["engcpe"]) <= agree * abs(b["engcpe"])
 
 
+@pytest.mark.parametrize("name", ["nacl_1000_k32", "mix_4000_k36", "nacl_4096_k48", "mol3_2592_k32"])
+def test_plane_and_line_ffts_agree(name, monkeypatch):
+    """The x and y passes fused per z-plane (the default for meshes with
+    K1 = K2 of a line-kernel length) and the separate x and y line kernels
+    (DLPX_FFT_PLANES=0) agree to FP64 rounding in forces, energies and the
+    reciprocal stress, and each matches the oracle, in both precisions
+    (meshes 32 = 8x4, 36 = 6x6, 48 = 12x4)."""
+    s, _ = fixtures.load(name)
+    print("case", name, flush=True)
+    o = oracle.two_body(s)
+    for precision, tol in ((lib.FP64, 1e-10), (lib.MIXED, 1e-5)):
+        res = {}
+        for planes in ("1", "0"):
+            monkeypatch.setenv("DLPX_FFT_PLANES", planes)
+            ctx, res[planes] = gpu_two_body(s, precision)
+            ctx.close()
+            assert rms_rel(res[planes]["frc"], o["frc"]) <= tol
+            check_energies(res[planes], o, 1e-11 if precision == lib.FP64 else 1e-6,
+                           energy_scales(o, mixed=precision == lib.MIXED))
+        a, b = res["1"], res["0"]
+        agree = 1e-12 if precision == lib.FP64 else 1e-6
+        assert rms_rel(a["frc"], b["frc"]) <= agree
+        assert np.abs(a["stress"] - b["stress"]).max() <= agree * np.abs(b["stress"]).max()
+        assert abs(a["engcpe"] - b["engcpe"]) <= agree * abs(b["engcpe"])
+
+
 @pytest.mark.parametrize("name", ["mix_4000_k36", "mix4_4000_k36", "mol3_2592_k32"])
 def test_mixed_relative_and_fp64_separations_agree(name, monkeypatch):
     """The mixed combined-table pair kernel with separations from
