@@ -668,27 +668,36 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
             pacc[9] += dz * fzp;
         };
         if constexpr (REL) {
-            // two entries per iteration, c and c + CS, each with its relative
-            // position loaded two entries ahead (qa, qb) and its list entry
-            // four ahead (ea, eb), all unconditional from clamped slots, so a
-            // gather has two pairs' work to hide behind
+            // four entries per iteration in two phases of two: phase A works
+            // on entries c, c + CS (qa, qb) while it loads the relative
+            // positions of c + 2CS, c + 3CS (qc, qd) and the list entries
+            // four ahead; phase B works on qc, qd while it loads the next
+            // qa, qb.  Every load lands in the register its consumer reads,
+            // so the loop carries no copies of values in flight (a copy at the
+            // loop's end would wait for the load to arrive there).  All loads
+            // are unconditional from clamped slots.
             int32_t ea = en, eb = m > 0 ? lst[(size_t)min(c0 + 3 * CS, mm) * G] : i;
             auto rel_term = [&](const float4 qj, const int code) {
                 const float4 of = offt[code], ol = offt[128 + code];
                 term(((qi.x - qj.x) + of.x) + ol.x, ((qi.y - qj.y) + of.y) + ol.y, ((qi.z - qj.z) + of.z) + ol.z,
                      __float_as_int(qj.w));
             };
-            for (int c = c0; c < m; c += 2 * CS) {
-                const float4 q0 = qa, q1 = qb;
-                const int k0 = ka, k1 = kb;
-                qa = a.prel[list_j(ea)];
-                ka = list_code(ea);
-                qb = a.prel[list_j(eb)];
-                kb = list_code(eb);
-                ea = lst[(size_t)min(c + 4 * CS, mm) * G];
-                eb = lst[(size_t)min(c + 5 * CS, mm) * G];
-                rel_term(q0, k0);
-                if (c + CS < m) rel_term(q1, k1);
+            for (int c = c0; c < m; c += 4 * CS) {
+                // list entries first, then the gathers they feed: a wait for
+                // an entry (in issue order) then never waits for a gather
+                const int32_t ec = lst[(size_t)min(c + 4 * CS, mm) * G], ed = lst[(size_t)min(c + 5 * CS, mm) * G];
+                const float4 qc = a.prel[list_j(ea)], qd = a.prel[list_j(eb)];
+                const int kc = list_code(ea), kd = list_code(eb);
+                rel_term(qa, ka);
+                if (c + CS < m) rel_term(qb, kb);
+                ea = lst[(size_t)min(c + 6 * CS, mm) * G];
+                eb = lst[(size_t)min(c + 7 * CS, mm) * G];
+                qa = a.prel[list_j(ec)];
+                qb = a.prel[list_j(ed)];
+                ka = list_code(ec);
+                kb = list_code(ed);
+                if (c + 2 * CS < m) rel_term(qc, kc);
+                if (c + 3 * CS < m) rel_term(qd, kd);
             }
         } else {
             for (int c = c0; c < m; c += CS) {
