@@ -40,12 +40,12 @@ VALU_PEAK = {lib.FP64: 78.6e12, lib.MIXED: 157.3e12}  # FLOP/s, vector FP64 / FP
 # stage-timed pass launches them (single-system kernels; resident runs launch
 # the _batch forms of the same bodies)
 KERNELS = {"pair": "k_pair_comb<8, 1024, true>", "spread": "k_spread_cells<float, 8>",
-           "fft": "k_lines_r2c/k_lines_c2c/k_lines_zconv/k_lines_c2r<float, 16, 8, 16>",
+           "fft": "k_plane/k_lines_zconv/k_plane<float, 16, 8>",
            "gather": "k_gather<float, 8>"}
-# the FFT stage's kernels and their launches per stage
-FFT_KERNELS = [("k_lines_r2c<float, 16, 8, 16>", 1), ("k_lines_c2c<float, 16, 8, 16, 1>", 1),
-               ("k_lines_zconv<float, 16, 8, 16>", 1), ("k_lines_c2c<float, 16, 8, 16, -1>", 1),
-               ("k_lines_c2r<float, 16, 8, 16>", 1)]
+# the FFT stage's kernels and their launches per stage: x and y passes fused
+# per z-plane (forward, inverse) around the z pass with the influence function
+FFT_KERNELS = [("k_plane<float, 16, 8, false>", 1), ("k_lines_zconv<float, 16, 8, 16>", 1),
+               ("k_plane<float, 16, 8, true>", 1)]
 BATCH_PAIR_KERNEL = "k_pair_comb_batch<8, 1024, true>"
 
 
@@ -423,7 +423,7 @@ def main():
                           achieved_tflops=flops / t / 1e12, achieved_gbs=byts / t / 1e9, tmin_frac=tmin / t)
         tmin_sum += tmin
         tmeas_sum += t
-    # the dominant single kernel (the FFT stage is five launches of four kernels)
+    # the dominant single kernel (the FFT stage is three launches of two kernels)
     dom = max((k for k in stages if k != "fft"), key=lambda k: stages[k]["ms"])
     d = stages[dom]
     flops, byts = work[dom]
