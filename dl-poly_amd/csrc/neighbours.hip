@@ -684,7 +684,11 @@ __device__ __forceinline__ void build_list_sub_body(const BuildArgs &a, int b)
                     const unsigned long long acc = __ballot(in);
                     const int pre = __builtin_amdgcn_mbcnt_hi((unsigned)(acc >> 32),
                                                               __builtin_amdgcn_mbcnt_lo((unsigned)acc, 0u));
-                    if (in) row_out[min(count + pre, cap_tot - 1)] = j | ((row * 5 + list_code(w) - lx) << kListCodeShift);
+                    // w = j | qx << 25, so the entry j | (row 5 + qx - lx) << 25 is w
+                    // plus a row constant (modulo 2^32; the code stays below 128)
+                    if (in)
+                        row_out[min(count + pre, cap_tot - 1)] =
+                            (int32_t)((uint32_t)w + ((uint32_t)(row * 5 - lx) << kListCodeShift));
                     count += __popcll(acc);
                 };
                 // five rows at a time: their first chunks' LDS reads are issued together
