@@ -185,14 +185,18 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
     // B-splines, base points and charges staged in the same pass (one read of
     // the positions)
     const bool once = n <= NA;
+    // per-thread bounds, reduced across the wavefront before one LDS atomic
+    // per wavefront: atomics of every atom on the same six LDS words
+    // serialise (measured: 9 us of the kernel at 64k ions)
+    int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-0x7fffffff, -0x7fffffff, -0x7fffffff};
     for (int k = t; k < n; k += NT) {
         const double4 p = pos[s0 + k];
         for (int d = 0; d < 3; d++) {
             const double u = grid_u(sp, d, p.x, p.y, p.z);
             const double fl = floor(u);
             const int b = (int)fl - S + 1;
-            atomicMin(&wlo[d], b);
-            atomicMax(&whi[d], b);
+            bmin[d] = min(bmin[d], b);
+            bmax[d] = max(bmax[d], b);
             if (once) {
                 R M[S];
                 bspline<R, false, S>((R)(u - fl), M, nullptr);
@@ -202,6 +206,18 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
             }
         }
         if (once) sQ[k] = (R)p.w;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            bmin[d] = min(bmin[d], __shfl_xor(bmin[d], o, 64));
+            bmax[d] = max(bmax[d], __shfl_xor(bmax[d], o, 64));
+        }
+        if (lane == 0 && bmin[d] <= bmax[d]) {
+            atomicMin(&wlo[d], bmin[d]);
+            atomicMax(&whi[d], bmax[d]);
+        }
     }
     __syncthreads();
     const int lo0 = wlo[0], lo1 = wlo[1], lo2 = wlo[2];
