@@ -110,6 +110,17 @@ __device__ __forceinline__ double grid_u(const SpmeParams &sp, int d, double x, 
     return sp.kr[d] * (sp.rcell[d] * x + sp.rcell[d + 3] * y + sp.rcell[d + 6] * z + 0.5);
 }
 
+// floor(u) for the spreading window, held within +-2^20: a coordinate far
+// outside the cell (a diverging run) then still gives window bounds whose
+// integer arithmetic cannot overflow (an overflowed bound passed the window's
+// one-period test and indexed the grid out of range); such a window exceeds
+// the mesh and takes the direct path, which wraps every index.  NaN maps to
+// the lower bound (fmax returns the non-NaN operand).
+__device__ __forceinline__ double spread_floor(double u)
+{
+    return fmin(fmax(floor(u), -1048576.0), 1048576.0);
+}
+
 __device__ __forceinline__ int wrapk(int v, int K)
 {
     v %= K;
@@ -193,7 +204,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
         const double4 p = pos[s0 + k];
         for (int d = 0; d < 3; d++) {
             const double u = grid_u(sp, d, p.x, p.y, p.z);
-            const double fl = floor(u);
+            const double fl = spread_floor(u);
             const int b = (int)fl - S + 1;
             bmin[d] = min(bmin[d], b);
             bmax[d] = max(bmax[d], b);
@@ -234,7 +245,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
             if (a0 + a < n) {
                 const double4 p = pos[s0 + a0 + a];
                 const double u = grid_u(sp, d, p.x, p.y, p.z);
-                const double fl = floor(u);
+                const double fl = spread_floor(u);
                 R M[S];
                 bspline<R, false, S>((R)(u - fl), M, nullptr);
 #pragma unroll

@@ -136,6 +136,23 @@ def test_plane_and_line_ffts_agree(name, monkeypatch):
         assert abs(a["engcpe"] - b["engcpe"]) <= agree * abs(b["engcpe"])
 
 
+def test_diverged_coordinates_stay_in_range():
+    """Coordinates scaled by 1e30, as a diverging run leaves them: the fold
+    at the list build cannot bring them back (its rounding exceeds the cell
+    by far), so every link cell's atoms reach the spreading with grid
+    coordinates beyond the int range.  The window bounds are held to
+    +-2^20 (unbounded, they saturated, overflowed in the window's
+    one-period test and indexed the grid out of range), so the call returns
+    normally in both precisions (the forces are meaningless)."""
+    s, _ = fixtures.load("nacl_1000_k32")
+    pos = np.array(s.pos, dtype=np.float64, copy=True) * 1.0e30
+    for precision in (lib.FP64, lib.MIXED):
+        print("case diverged precision", precision, flush=True)
+        ctx, r = gpu_two_body(s, precision, pos=pos)
+        ctx.close()
+        assert r["frc"].shape == pos.shape
+
+
 @pytest.mark.parametrize("name", ["mix_4000_k36", "mix4_4000_k36", "mol3_2592_k32"])
 def test_mixed_relative_and_fp64_separations_agree(name, monkeypatch):
     """The mixed combined-table pair kernel with separations from
