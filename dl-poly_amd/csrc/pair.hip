@@ -628,7 +628,10 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
             en = m > 0 ? lst[(size_t)min(c0 + CS, mm) * G] : i;
         }
         const int32_t *ktrow = kt + (REL ? __float_as_int(qi.w) : typ[i]) * pp.ntypes;
-        // one pair from its separation and j's type
+        // one pair from its separation and j's type; its force goes to the
+        // FP32 partial (gx, gy, gz), which the caller adds to the FP64 sums
+        // every few pairs
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         auto term = [&](const float dx, const float dy, const float dz, const int tj) {
             const float rsq = dx * dx + dy * dy + dz * dz;
             if (rsq >= rmax2) return;
@@ -640,12 +643,16 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
             const float q1 = s - (float)l;
             float4 A = row[l];
             const float4 B = row[l + 1], C = row[l + 2];
-            if (l == 0) {
-                // numerics.F90:262: the first sample scaled by r
-                A.x *= rrr;
-                A.y *= rrr;
-                A.z *= rrr;
-                A.w *= rrr;
+            if (__builtin_expect(__any(l == 0), 0)) {
+                // numerics.F90:262: the first sample scaled by r (a pair
+                // closer than one table step; the wavefront-uniform test
+                // keeps the common case free of the select)
+                if (l == 0) {
+                    A.x *= rrr;
+                    A.y *= rrr;
+                    A.z *= rrr;
+                    A.w *= rrr;
+                }
             }
             const float q2 = 0.5f * q1 * (q1 - 1.0f);
             const float gam = rinv2 * __builtin_fmaf(q2, __builtin_fmaf(-2.0f, B.x, A.x + C.x),
@@ -657,9 +664,9 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
             pacc[1] -= fv;
             pacc[2] += ec;
             const float fxp = gam * dx, fyp = gam * dy, fzp = gam * dz;
-            fx += (double)fxp;
-            fy += (double)fyp;
-            fz += (double)fzp;
+            gx += fxp;
+            gy += fyp;
+            gz += fzp;
             pacc[4] += dx * fxp;
             pacc[5] += dx * fyp;
             pacc[6] += dx * fzp;
@@ -698,6 +705,11 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
                 kb = list_code(ed);
                 if (c + 2 * CS < m) rel_term(qc, kc);
                 if (c + 3 * CS < m) rel_term(qd, kd);
+                // the four pairs' forces into the FP64 sums
+                fx += (double)gx;
+                fy += (double)gy;
+                fz += (double)gz;
+                gx = gy = gz = 0.0f;
             }
         } else {
             for (int c = c0; c < m; c += CS) {
@@ -709,6 +721,10 @@ __device__ __forceinline__ void pair_comb_body(const PairArgs<float> &a, int bid
                 term((float)min_image_1(pi.x, pj.x, pp.L[0], pp.invL[0]),
                      (float)min_image_1(pi.y, pj.y, pp.L[1], pp.invL[1]),
                      (float)min_image_1(pi.z, pj.z, pp.L[2], pp.invL[2]), tj);
+                fx += (double)gx;
+                fy += (double)gy;
+                fz += (double)gz;
+                gx = gy = gz = 0.0f;
             }
         }
     }

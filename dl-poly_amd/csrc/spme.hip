@@ -446,36 +446,35 @@ __device__ __forceinline__ void gather_body(const GatherArgs<R> &ga, int bid)
             zoff[tz] = (size_t)gz * plane + gx;
         }
         double E = 0.0, Fy = 0.0, Fz = 0.0;
-        // two y rows per iteration, their 2S grid values loaded into registers
-        // before any is used, so a lane has 2S loads in flight at once
-        for (int ty = 0; ty < S; ty += 2) {
-            int gy0 = by + ty, gy1 = by + ty + 1;
-            if (gy0 >= sp.K[1]) gy0 -= sp.K[1];
-            if (gy1 >= sp.K[1]) gy1 -= sp.K[1];
-            const size_t yo0 = (size_t)gy0 * sp.K[0], yo1 = (size_t)gy1 * sp.K[0];
-            R ph0[S], ph1[S];
+        // y rows per iteration (a divisor of S, which is even), their RY*S
+        // grid values in flight together
+        constexpr int RY = S % 4 == 0 ? 4 : 2;
+        for (int ty = 0; ty < S; ty += RY) {
+            size_t yo[RY];
 #pragma unroll
-            for (int tz = 0; tz < S; tz++) {
-                ph0[tz] = Phi[zoff[tz] + yo0];
-                ph1[tz] = Phi[zoff[tz] + yo1];
+            for (int r = 0; r < RY; r++) {
+                int gy = by + ty + r;
+                if (gy >= sp.K[1]) gy -= sp.K[1];
+                yo[r] = (size_t)gy * sp.K[0];
             }
-            R se0 = (R)0, sz0 = (R)0, se1 = (R)0, sz1 = (R)0;
+            R ph[RY][S];
 #pragma unroll
-            for (int tz = 0; tz < S; tz++) {
-                const R mz = sM[2][a][tz], dz = sD[2][a][tz];
-                se0 += mz * ph0[tz];
-                sz0 += dz * ph0[tz];
-                se1 += mz * ph1[tz];
-                sz1 += dz * ph1[tz];
+            for (int tz = 0; tz < S; tz++)
+#pragma unroll
+                for (int r = 0; r < RY; r++) ph[r][tz] = Phi[zoff[tz] + yo[r]];
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                R se = (R)0, sz = (R)0;
+#pragma unroll
+                for (int tz = 0; tz < S; tz++) {
+                    se += sM[2][a][tz] * ph[r][tz];
+                    sz += sD[2][a][tz] * ph[r][tz];
+                }
+                const double my = (double)sM[1][a][ty + r], dy = (double)sD[1][a][ty + r];
+                E += my * (double)se;
+                Fy += dy * (double)se;
+                Fz += my * (double)sz;
             }
-            const double my0 = (double)sM[1][a][ty], dy0 = (double)sD[1][a][ty];
-            const double my1 = (double)sM[1][a][ty + 1], dy1 = (double)sD[1][a][ty + 1];
-            E += my0 * (double)se0;
-            Fy += dy0 * (double)se0;
-            Fz += my0 * (double)sz0;
-            E += my1 * (double)se1;
-            Fy += dy1 * (double)se1;
-            Fz += my1 * (double)sz1;
         }
         const double mx = (double)sM[0][a][tx], dx = (double)sD[0][a][tx];
         e = mx * E;
