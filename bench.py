@@ -47,6 +47,7 @@ KERNELS = {"pair": "k_pair_comb<8, 1024, true>", "spread": "k_spread_cells<float
 FFT_KERNELS = [("k_plane<float, 16, 8, false>", 1), ("k_lines_zconv<float, 16, 8, 16>", 1),
                ("k_plane<float, 16, 8, true>", 1)]
 BATCH_PAIR_KERNEL = "k_pair_comb_batch<8, 1024, true>"
+PAIR_KERNEL_VDW = "k_pair<float, 8, 512, false, false, false, false>"  # mixed pair kernel of a vdW-only system
 
 
 def traffic_from_profile(workload, kernel):
@@ -433,7 +434,8 @@ def main():
     else:
         roof = dict(bound="valu", achieved=flops / t / 1e12, peak=VALU_PEAK[lib.MIXED] / 1e12, unit="TFLOP/s")
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["kernel"] = KERNELS[dom]
+    # (without electrostatics the combined-table kernel does not apply)
+    roof["kernel"] = KERNELS[dom] if (s.electrostatics or dom != "pair") else PAIR_KERNEL_VDW
     roof["traffic"] = traffic_from_profile(args.workload, roof["kernel"])
     roof["stage_fraction_sum_tmin_over_tmeasured"] = tmin_sum / tmeas_sum if tmeas_sum else None
     if "fft" in stages:
