@@ -121,6 +121,18 @@ __device__ __forceinline__ double spread_floor(double u)
     return fmin(fmax(floor(u), -1048576.0), 1048576.0);
 }
 
+// A value every lane of the wavefront holds, moved to scalar registers.
+__device__ __forceinline__ float uniform_value(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_value(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ int wrapk(int v, int K)
 {
     v %= K;
@@ -268,7 +280,9 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
             }
             const int na = min(NA, n - a0);
             for (int a = wid; a < na; a += NT / 64) {
-                const R q = sQ[a];
+                // the atom's charge as a wavefront-uniform value: the skip below
+                // is then a scalar branch
+                const R q = uniform_value(sQ[a]);
                 if (fabs((double)q) <= kZeroPlus) continue;
                 // one atom per wavefront: its base point and z-range are scalars,
                 // so the per-z tests below are scalar branches, not exec masks
@@ -286,11 +300,13 @@ __device__ __forceinline__ void spread_body(const SpreadArgs<R> &sa, int c)
                     const int tx = pp % S, ty = pp / S;
                     const R mxy = sM[0][a][tx] * sM[1][a][ty];
                     if (tile) {
-                        // slab index of spline point tz = 0 (points outside the slab are skipped)
+                        // slab index of spline point tz = 0 (points outside the slab are skipped);
+                        // the slab's spline points as a scalar bit mask, one bit test per point
                         const int l0 = (bz - lo2 - z0) * plane + (by + ty - lo1) * Wx + (bx + tx - lo0);
+                        const unsigned zm = ((1u << tz1) - 1u) & ~((1u << tz0) - 1u);
 #pragma unroll
                         for (int tz = 0; tz < S; tz++)
-                            if (tz >= tz0 && tz < tz1) atomicAdd(&slab[l0 + tz * plane], (double)(mz[tz] * mxy));
+                            if (zm & (1u << tz)) atomicAdd(&slab[l0 + tz * plane], (double)(mz[tz] * mxy));
                     } else {
                         const int gx = wrapk(bx + tx, sp.K[0]), gy = wrapk(by + ty, sp.K[1]);
 #pragma unroll
