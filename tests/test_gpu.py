@@ -418,6 +418,10 @@ def test_unfolded_positions_give_the_same_forces():
 
 
 def test_forces_added_to_existing_and_aos_stride():
+    """Forces are added to what the caller's arrays hold, and a 64-byte
+    array-of-structures record (DL_POLY's corePart: x, y, z, charge, fx,
+    fy, fz, one spare double per atom) through dlpx_atoms' byte stride gives
+    the same pairs, forces and energies as packed arrays."""
     s, _ = fixtures.load("nacl_1000_k32")
     ctx = lib.Context(s)
     ctx.link_cell_pairs(s.pos)
@@ -426,7 +430,24 @@ def test_forces_added_to_existing_and_aos_stride():
     r2 = ctx.two_body_forces(s.pos, frc=base.copy())
     assert np.array_equal(r2["frc"] - 7.25, r1["frc"] + 7.25 - 7.25) or \
         np.abs(r2["frc"] - 7.25 - r1["frc"]).max() < 1e-12 * np.abs(r1["frc"]).max()
+    # the same call on a 64-byte record per atom, forces starting from 1.5
+    rec = np.zeros((s.natms, 8))
+    rec[:, 0:3] = s.pos
+    rec[:, 3] = ctx.q
+    rec[:, 4:7] = 1.5
+    b = rec.ctypes.data
+    a = lib.Atoms(natms=s.natms, stride=64, x=b, y=b + 8, z=b + 16, chge=b + 24, fx=b + 32, fy=b + 40, fz=b + 48,
+                  ltype=ctx.ltype.ctypes.data)
+    npairs, ibig = lib.C.c_int64(), lib.C.c_int32()
+    ctx._chk(ctx.lib.dlpx_link_cell_pairs(ctx.h, lib._ptr(ctx.cell), lib.C.byref(a), lib.C.byref(npairs),
+                                          lib.C.byref(ibig)))
+    o = lib.TwoBodyOut()
+    ctx._chk(ctx.lib.dlpx_two_body_forces(ctx.h, lib._ptr(ctx.cell), lib.C.byref(a), lib.C.byref(o)))
     ctx.close()
+    assert np.abs(rec[:, 4:7] - 1.5 - r1["frc"]).max() <= 1e-12 * np.abs(r1["frc"]).max()
+    assert np.array_equal(rec[:, 0:3], s.pos) and np.all(rec[:, 7] == 0.0)  # inputs and the spare word untouched
+    assert abs(o.engsrp - r1["engsrp"]) <= 1e-12 * abs(r1["engsrp"])
+    assert abs(o.engcpe - r1["engcpe"]) <= 1e-12 * abs(r1["engcpe"])
 
 
 @pytest.mark.parametrize("name", TRAJ)
