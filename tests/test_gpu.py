@@ -153,6 +153,31 @@ def test_diverged_coordinates_stay_in_range():
         assert r["frc"].shape == pos.shape
 
 
+@pytest.mark.parametrize("k", [25, 27, 45])
+def test_plane_kernels_on_odd_meshes(k, monkeypatch):
+    """The plane kernels with an odd edge (25 = 5x5, 27 = 9x3, 45 = 9x5):
+    the last row pair of a plane holds one real row, and the half spectrum
+    has no Nyquist column.  NaCl 1000 on a k^3 mesh (S = 6) against the
+    oracle and against the separate line kernels, both precisions."""
+    import dataclasses
+    s0, _ = fixtures.load("nacl_1000_k32")
+    s = dataclasses.replace(s0, kvec=(k, k, k), nsplines=6)
+    print("case odd mesh", k, flush=True)
+    o = oracle.two_body(s)
+    for precision, tol in ((lib.FP64, 1e-10), (lib.MIXED, 1e-5)):
+        res = {}
+        for planes in ("1", "0"):
+            monkeypatch.setenv("DLPX_FFT_PLANES", planes)
+            ctx, res[planes] = gpu_two_body(s, precision)
+            ctx.close()
+            assert rms_rel(res[planes]["frc"], o["frc"]) <= tol
+            check_energies(res[planes], o, 1e-11 if precision == lib.FP64 else 1e-6,
+                           energy_scales(o, mixed=precision == lib.MIXED))
+        agree = 1e-12 if precision == lib.FP64 else 1e-6
+        assert rms_rel(res["1"]["frc"], res["0"]["frc"]) <= agree
+        assert np.abs(res["1"]["stress"] - res["0"]["stress"]).max() <= agree * np.abs(res["0"]["stress"]).max()
+
+
 @pytest.mark.parametrize("name", ["mix_4000_k36", "mix4_4000_k36", "mol3_2592_k32"])
 def test_mixed_relative_and_fp64_separations_agree(name, monkeypatch):
     """The mixed combined-table pair kernel with separations from
