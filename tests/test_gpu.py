@@ -688,6 +688,27 @@ def test_graph_replayed_steps_match_eager_steps(precision, tol):
         assert rms_rel(vg, ve) <= tol and rms_rel(fg, fe) <= 10 * tol
 
 
+@pytest.mark.parametrize("overlap", ["0", "1", "2"])
+def test_step_branch_layouts_agree(overlap, monkeypatch):
+    """Every layout of a resident step's branches (DLPX_OVERLAP: the
+    reciprocal chain on either stream, the pair kernel free or held until the
+    spreading is done) and one-step graphs (DLPX_GRAPH_STEPS=1) give the
+    eager steps' trajectory, across list rebuilds."""
+    s, _ = fixtures.load("nacl_4096_k48")
+    print("case layout", overlap, flush=True)
+    eager = _chunked_md(s, lib.MIXED, graphs=False)
+    monkeypatch.setenv("DLPX_OVERLAP", overlap)
+    monkeypatch.setenv("DLPX_GRAPH_STEPS", "1" if overlap == "1" else "8")
+    graph = _chunked_md(s, lib.MIXED, graphs=True)
+    L = np.diag(s.cell.reshape(3, 3))
+    for (xe, ve, fe, ne), (xg, vg, fg, ng) in zip(eager, graph):
+        assert ne == ng
+        dx = xg - xe
+        dx -= L * np.round(dx / L)
+        assert np.abs(dx).max() <= 1e-5 * L.max()
+        assert rms_rel(vg, ve) <= 1e-5 and rms_rel(fg, fe) <= 1e-4
+
+
 def test_ensemble_matches_separate_runs():
     """dlpx_md_run_ensemble over systems of different kinds and sizes gives
     each system's own dlpx_md_run (up to the order of the spreading atomics)."""
