@@ -741,6 +741,49 @@ def test_ensemble_matches_separate_runs():
         ctx.close()
 
 
+def test_ensemble_tuned_run_matches_separate_runs(tmp_path, monkeypatch):
+    """A batch run long enough to tune its launch shapes (48 steps: six
+    candidate shapes of six steps each, then graph replays) gives each
+    system's own dlpx_md_run, writes the chosen shape to DLPX_TUNE_CACHE, and
+    a second run of the same size class (which takes the cached shape) does
+    too."""
+    syss = [systems.argon(ncells=6, seed=15), systems.nacl(ncells=5, kvec=(32, 32, 32), seed=16),
+            systems.nacl(ncells=6, kvec=(40, 40, 40), seed=17)]
+    print("case tuned ensemble of", [s.natms for s in syss], flush=True)
+    steps = 48
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("DLPX_AUTOTUNE", "1")
+    monkeypatch.setenv("DLPX_TUNE_CACHE", str(cache))
+    ref = []
+    for s in syss:
+        ctx = lib.Context(s)
+        ctx.md_load(s.pos, s.vel)
+        _, nreb = ctx.md_run(steps, s.timestep, energies=False)
+        ref.append(ctx.md_store() + (nreb,))
+        ctx.close()
+    for run in range(2):
+        ctxs = []
+        for s in syss:
+            ctx = lib.Context(s)
+            ctx.md_load(s.pos, s.vel)
+            ctxs.append(ctx)
+        nreb = lib.md_run_ensemble(ctxs, steps)
+        for k, (s, ctx) in enumerate(zip(syss, ctxs)):
+            x, v, f = ctx.md_store()
+            xr, vr, fr, nr = ref[k]
+            L = np.diag(s.cell.reshape(3, 3))
+            dx = x - xr
+            dx -= L * np.round(dx / L)
+            assert nreb[k] == nr
+            assert np.abs(dx).max() <= 1e-8 * L.max()
+            assert rms_rel(v, vr) <= 1e-8 and rms_rel(f, fr) <= 1e-7
+            ctx.close()
+        lines = cache.read_text().split("\n")
+        tuned = [ln.split() for ln in lines if ln.strip()]
+        assert tuned and all(len(w) == 3 for w in tuned)
+        assert any(int(w[1]) in (4, 8, 16) and int(w[2]) in (8, 16) for w in tuned)
+
+
 def _ensemble_sample(syss, n=32):
     """Indices of n systems of the ensemble: the smallest and largest of each
     kind, one per NaCl mesh edge, the rest spread evenly over each kind."""
